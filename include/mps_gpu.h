@@ -1,0 +1,187 @@
+/*
+ * mps_gpu.h -- C ABI of the MI355X (gfx950) MPS time-step library.
+ *
+ * Plain pointers and sizes only.  Every call returns 0 on success or a
+ * negative MPS_E* code; mps_last_error() gives the message.  A context owns
+ * all device buffers and one HIP stream; it is single-thread affine.
+ *
+ * Particle data crossing the boundary is in the reference's order and
+ * layout: AoS double[3N] for vectors, double[N] / int[N] for scalars, ids as
+ * in the JSON file (fluid 0, wall 2, dummy 3, ghost -1; BC inner 0,
+ * surface 1, other -1).  Internally particles live in bucket-sorted SoA
+ * order; that order is never visible to the caller.
+ *
+ * Each operator entry point replaces one method of the reference; the
+ * citation names the method it stands for.  Operators run in stream order
+ * and leave their results on the device.
+ */
+#ifndef MPS_GPU_H_
+#define MPS_GPU_H_
+
+#include <stddef.h>
+#include "mps_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPS_OK 0
+#define MPS_E_ARG (-1)      /* invalid argument                 */
+#define MPS_E_HIP (-2)      /* HIP runtime error                */
+#define MPS_E_UNSUPPORTED (-3) /* configuration not supported   */
+#define MPS_E_IO (-4)       /* file could not be read / written */
+#define MPS_E_CAPACITY (-5) /* a fixed-size device table overflowed */
+#define MPS_E_SOLVER (-6)   /* the PPE solve did not converge   */
+
+typedef struct mps_ctx mps_ctx;
+
+/* ------------------------------------------------------------ parameters */
+/* sizeof(mps_params), for bindings that mirror the struct */
+size_t mps_params_size(void);
+/* JSON defaults of MpsInputOutput::readInputFile (src/MpsInputOutput.cpp:136-500) */
+int mps_params_defaults(mps_params* p);
+/* MpsInputOutput::readInputFile (src/MpsInputOutput.cpp:58-641): parse a
+ * JSON file (comments allowed, missing keys take the defaults).  grid_path
+ * and mesh_path receive pathNames.particle_grid_file / mesh_rigid_file. */
+int mps_params_from_json(const char* json_path, mps_params* p, char* grid_path, size_t grid_cap,
+                         char* mesh_path, size_t mesh_cap);
+/* MpsParticle::setParameters (src/MpsParticle.cpp:136-227) plus
+ * MpsBucket::allocateMemory / calcDomainLimits (src/MpsBucket.cpp:20-295).
+ * pos/type are needed only for particle walls (domain from particles). */
+int mps_params_derive(mps_params* p, int n, const double* pos, const int* type);
+
+/* --------------------------------------------------------------- context */
+/* Create a context for n particles on HIP device `device`.  p must be
+ * derived.  Replaces the allocation in MpsInputOutput::readMpsParticleFile
+ * (src/MpsInputOutput.cpp:667-750) and MpsBucket::allocateMemory. */
+int mps_create(mps_ctx** out, const mps_params* p, int n, int device);
+int mps_destroy(mps_ctx* ctx);
+const char* mps_last_error(const mps_ctx* ctx);
+/* Copy of the context's parameter block */
+int mps_get_params(const mps_ctx* ctx, mps_params* out);
+
+/* Upload the .grid state (src/MpsInputOutput.cpp:757-776): type[N],
+ * pos[3N], vel[3N], press[N], press_avg[N]; the remaining fields take the
+ * initial values of readMpsParticleFile (:812-838). */
+int mps_upload_state(mps_ctx* ctx, const int* type, const double* pos, const double* vel,
+                     const double* press, const double* press_avg);
+/* Download one field in reference order.  Double fields: "pos" "vel" "acc"
+ * "acc_star" "npcd" "mirror" "wall_pos" "polygon_normal" "force_wall"
+ * "cm1" "cm2" "cm3" "grad_conc" (3 per particle); "press" "press_avg" "pndi"
+ * "pndki" "pndski" "pnd_small" "npcd2" "conc" "div_u" "diffusive" "pnd_wall"
+ * "riw2" "ppe_x" "ppe_b" "dev_dot_n" (1 per particle).  Int fields: "type"
+ * "bc" "num_neigh" "num_neigh_wall" "near_wall" "near_mesh_type".  `count`
+ * is the element capacity of out. */
+int mps_download(mps_ctx* ctx, const char* field, void* out, size_t count);
+/* Overwrite one field (reference order), same names as mps_download. */
+int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t count);
+/* Scalars: "vel_max" "cfl" "time" "solver_iter" "solver_err" "num_particles"
+ * "iterations" "ppe_ms" "true_residual" "fallback" */
+int mps_get_scalar(mps_ctx* ctx, const char* name, double* out);
+int mps_set_scalar(mps_ctx* ctx, const char* name, double value);
+int mps_sync(mps_ctx* ctx);
+
+/* ------------------------------------------------------------- mesh wall */
+/* PolygonMesh::readPolygonMeshFile (src/PolygonMesh.cpp:42-108): binary or
+ * ASCII STL -> duplicate vertices merged (tolerance 2e-15) -> unit face
+ * normals.  Writes counts; arrays may be NULL to query sizes. */
+int mps_mesh_read_stl(const char* path, int* nv, double* V, int* nf, int* F, double* N);
+/* Install the rigid wall mesh (V[3nv], F[3nf], unit normals N[3nf]) and
+ * build the wall tables of PolygonMesh::initWijnNeigh (src/PolygonMesh.cpp:377-527). */
+int mps_set_mesh(mps_ctx* ctx, int nv, const double* V, int nf, const int* F, const double* N);
+/* Wall tables xPND, xNeigh, Zij, nNeighWall (50 entries each). */
+int mps_get_wall_tables(mps_ctx* ctx, double* xpnd, double* xneigh, double* zij, int* nneigh);
+
+/* ------------------------------------------------------------- operators */
+/* MpsBucket::updateParticlesID (src/MpsBucket.cpp:299-353): counting sort of
+ * particles into buckets, particles reordered on the device. */
+int mps_bucket_update(mps_ctx* ctx);
+/* MpsViscosity::calcViscosity (src/MpsViscosity.cpp:20-102) */
+int mps_viscosity(mps_ctx* ctx);
+/* MpsParticle::addGravity (src/MpsParticle.cpp:111-130) */
+int mps_add_gravity(mps_ctx* ctx);
+/* MpsParticleVelPos::predictionVelocityPosition (src/MpsParticleVelPos.cpp:20-50) */
+int mps_prediction(mps_ctx* ctx);
+/* MpsParticle::checkParticleOutDomain (src/MpsParticle.cpp:234-402) */
+int mps_check_out_domain(mps_ctx* ctx);
+/* Build the per-step neighbour list at the current positions (no reference
+ * counterpart: the reference re-walks the buckets in every operator). */
+int mps_build_neighbor_list(mps_ctx* ctx);
+/* MpsParticleCollision::checkParticleCollisions / checkDynamicParticleCollisions
+ * (src/MpsParticleCollision.cpp:22-119, 124-292) */
+int mps_collision(mps_ctx* ctx);
+/* PolygonMesh::closestPointPNDBoundaryAABB (src/PolygonMesh.cpp:585-715) */
+int mps_mesh_closest_point(mps_ctx* ctx);
+/* PolygonMesh::updateParticlesNearPolygonMesh (src/PolygonMesh.cpp:718-795) */
+int mps_mesh_update_near(mps_ctx* ctx);
+/* MpsPndNeigh::calcWallNPCD (src/MpsPndNeigh.cpp:105-195) */
+int mps_wall_npcd(mps_ctx* ctx);
+/* MpsVectorMatrix::correctionMatrix (src/MpsVectorMatrix.cpp:20-93) */
+int mps_correction_matrix(mps_ctx* ctx);
+/* MpsPndNeigh::setInitialPndNumberOfNeighNPCD (src/MpsPndNeigh.cpp:21-101) */
+int mps_initial_pnd(mps_ctx* ctx);
+/* MpsPndNeigh::calcPndnNeighNPCD (src/MpsPndNeigh.cpp:199-331) */
+int mps_pnd_neigh_npcd(mps_ctx* ctx);
+/* MpsPndNeigh::calcPndDiffusiveTerm + calcWall{Slip,NoSlip}PndDiffusiveTerm
+ * + updatePnd + meanPndParticlesWallDummySurface (src/MpsPndNeigh.cpp:334-852) */
+int mps_pnd_diffusive(mps_ctx* ctx);
+/* MpsPndNeigh::meanWallPnd (:923-994) then meanPnd (:855-920) */
+int mps_mean_pnd(mps_ctx* ctx);
+/* MpsPndNeigh::meanNeighFluidPnd (:997-1050) */
+int mps_mean_neigh_fluid_pnd(mps_ctx* ctx);
+/* MpsBoundaryCondition::updateParticleBC (src/MpsBoundaryCondition.cpp:66-400) */
+int mps_update_bc(mps_ctx* ctx);
+/* MpsPressure::calcPress (src/MpsPressure.cpp:32-58): EOS pressure, or PPE
+ * assembly (solvePressurePoissonPnd[DivU] :169-421) + Jacobi-PCG
+ * (replacing solveConjugateGradient :424-445) + setZeroOnNegativePressure. */
+int mps_calc_press(mps_ctx* ctx);
+/* PPE assembly only (the first half of mps_calc_press) */
+int mps_ppe_assemble(mps_ctx* ctx);
+/* PPE solve only, from the current ppe_x guess (second half of mps_calc_press) */
+int mps_ppe_solve(mps_ctx* ctx);
+/* Assembled iteration matrix in reference order: row_ptr[N+1], cols/vals[nnz]
+ * (column = reference particle index).  Arrays may be NULL to query nnz. */
+int mps_ppe_matrix(mps_ctx* ctx, int* row_ptr, int* cols, double* vals, int* nnz);
+/* MpsPressure::extrapolatePressParticlesWallDummy (src/MpsPressure.cpp:792-846) */
+int mps_extrapolate_wall_dummy(mps_ctx* ctx);
+/* MpsPressure::calcPressGradient (src/MpsPressure.cpp:1217-1338) */
+int mps_press_gradient(mps_ctx* ctx);
+/* MpsPressure::calcWallPressGradient (src/MpsPressure.cpp:1341-1550) */
+int mps_wall_press_gradient(mps_ctx* ctx);
+/* MpsViscosity::calcWall{Slip,NoSlip}Viscosity (src/MpsViscosity.cpp:1527-1910) */
+int mps_wall_viscosity(mps_ctx* ctx);
+/* MpsParticleVelPos::correctionVelocityPosition (src/MpsParticleVelPos.cpp:53-97) */
+int mps_correction(mps_ctx* ctx);
+/* MpsShifting::calcShifting or calcConcAndConcGradient +
+ * calcWallConcAndConcGradient (src/MpsShifting.cpp:23-104, 432-692), with
+ * read-old / write-new semantics */
+int mps_shifting(mps_ctx* ctx);
+/* MpsParticleVelPos::updateVelocityParticlesWallDummy (src/MpsParticleVelPos.cpp:100-158) */
+int mps_wall_dummy_velocity(mps_ctx* ctx);
+
+/* ---------------------------------------------------------------- driver */
+/* The initial-step pipeline of main() (src/main.cpp:119-180, initMesh
+ * :471-556): buckets, out-of-domain check, wall distance, initial PND, mean
+ * PND, BC, first pressure. */
+int mps_init(mps_ctx* ctx);
+/* One time step of mainLoopOfSimulation (src/main.cpp:252-467). */
+int mps_step(mps_ctx* ctx);
+/* k steps without host synchronisation in between. */
+int mps_steps(mps_ctx* ctx, int k);
+
+/* ------------------------------------------------------------------ I/O */
+/* .grid reader (src/MpsInputOutput.cpp:645-857); arrays may be NULL to query n */
+int mps_read_grid(const char* path, int* n, int* type, double* pos, double* vel, double* press,
+                  double* press_avg);
+/* .grid writer in the same format (checkpoint / restart) */
+int mps_write_grid(const char* path, int n, const int* type, const double* pos, const double* vel,
+                   const double* press, const double* press_avg);
+/* MpsInputOutput::writeVtuAscii (src/MpsInputOutput.cpp:1313-1579) for the
+ * context's current state; flags bit0 pnd, bit1 neigh, bit2 deviation,
+ * bit3 concentration, bit4 auxiliar. */
+int mps_write_vtu(mps_ctx* ctx, const char* path, int flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPS_GPU_H_ */

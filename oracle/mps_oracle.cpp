@@ -1,0 +1,2092 @@
+/*
+ * mps_oracle.cpp -- CPU ORACLE (test infrastructure, not product code).
+ *
+ * A C++ restatement of the reference's per-step MPS algorithm, written to
+ * follow the reference's arithmetic order expression by expression so that it
+ * can serve as the parity checker for the HIP path.  Only tests/, the smoke
+ * check of __graft_entry__ and the cpu_baseline leg of bench.py may load it.
+ *
+ * Data model follows the reference: AoS double pos[3N], vel[3N], ...; the
+ * bucket list is the reference's first/last/next linked list built in
+ * ascending particle order (src/MpsBucket.cpp:299-353).  Every neighbour loop
+ * visits the 3x3(x3) stencil jz -> jy -> jx ascending and walks each bucket in
+ * ascending particle index (e.g. src/MpsPndNeigh.cpp:209-258).
+ *
+ * Deliberate, documented differences to the reference:
+ *  - stencil cells outside [0,n) on any axis are skipped; the reference
+ *    indexes them linearly (wrapping to a far row, or reading before the
+ *    array).  Wrapped cells lie more than reL away, so results agree whenever
+ *    the reference stays inside its arrays.
+ *  - periodic domains are not supported (periodic length is 0 everywhere).
+ *  - Eigen's CG is restated from its published algorithm (Jacobi
+ *    preconditioner, relative residual test on the recursive residual,
+ *    default tol = DBL_EPSILON, default max iterations 2N) with sequential
+ *    dot products; Eigen's vectorised reductions are not reproduced, so the
+ *    iterate sequence itself is not pinned.
+ *  - libigl's AABB closest point is replaced by brute force over all faces
+ *    using Ericson's point-triangle closest point (RTCD 5.1.5); ties keep the
+ *    lowest face index.
+ */
+#include "../include/mps_params.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+struct Mesh {
+    int nv = 0, nf = 0;
+    std::vector<double> V;    // [nv*3]
+    std::vector<int> F;       // [nf*3]
+    std::vector<double> Nrm;  // [nf*3]
+    std::vector<double> xPND, xNeigh, Zij;
+    std::vector<int> nNeighWall;
+};
+
+struct Sys {
+    mps_params p;
+    int N = 0, N0 = 0;
+    // scalars per particle
+    std::vector<int> type, bc, numNeigh, ptype, nearMeshType, elementID, numNeighWall;
+    std::vector<char> nearWall;
+    std::vector<double> press, pressAvg, pndi, pndki, pndski, pndSmall, npcd2, conc,
+        divU, diffTerm, aux1, aux2, pndWall, devDotN, nNeighSurf, riw2, RHO, MEU;
+    // vectors (AoS, 3 per particle)
+    std::vector<double> pos, vel, acc, accStar, npcd, gradC, cm1, cm2, cm3, normal, dvel,
+        wallPos, mirror, wpf1, wpf2, polyN, forceWall;
+    // PPE
+    std::vector<double> ppeX, ppeB;
+    std::vector<int> rowPtr, colIdx;
+    std::vector<double> vals;
+    int solverIter = 0;
+    double solverErr = 0.0;
+    // buckets
+    std::vector<int> first, last, next;
+    Mesh mesh;
+    bool hasMesh = false;
+    double velMax = 0.0, cflCurrent = 0.0, timeCurrent = 0.0;
+    int numIter = 0;
+    int nThreads = 0;
+    std::string err;
+};
+
+const double kEps = 10.0 * DBL_EPSILON;
+
+// ---------------------------------------------------------------- kernels
+// src/MpsParticle.cpp:57-72 (weight), :75-90 (weightGradient, identical forms)
+inline double weight(double dst, double re, int t) {
+    switch (t) {
+        case 0: return re / dst - 1.0;
+        case 1: return re / dst + dst / re - 2.0;
+        case 2: return re / dst - dst / re;
+        case 3: return (1.0 - dst / re) * (1.0 - dst / re) * (1.0 - dst / re);
+        case 4: return (1.0 - dst / re) * (1.0 - dst / re);
+        default: return re / dst - 1.0;
+    }
+}
+// src/MpsParticle.cpp:93-108
+inline double delWeight(double dst, double re, int t) {
+    switch (t) {
+        case 0: return -re / (dst * dst);
+        case 1: return -re / (dst * dst) + 1.0 / re;
+        case 2: return -re / (dst * dst) - 1.0 / re;
+        case 3: return -3.0 / re * (1.0 - dst / re) * (1.0 - dst / re);
+        case 4: return -2.0 / re * (1.0 - dst / re);
+        default: return -re / (dst * dst);
+    }
+}
+
+// src/MpsBucket.cpp:420-425 (stencil centre, +eps on every axis)
+inline void bucketCoords(const Sys& s, double x, double y, double z, int& bx, int& by, int& bz) {
+    bx = (int)((x - s.p.domain_min[0]) * s.p.inv_bucket_side + kEps);
+    by = (int)((y - s.p.domain_min[1]) * s.p.inv_bucket_side + kEps);
+    bz = (int)((z - s.p.domain_min[2]) * s.p.inv_bucket_side + kEps);
+}
+
+template <class Fn>
+inline void walk(const Sys& s, double xi, double yi, double zi, Fn&& fn) {
+    int ix, iy, iz;
+    bucketCoords(s, xi, yi, zi, ix, iy, iz);
+    const int d3 = (int)(s.p.dim - 2.0);
+    const int minZ = (iz - 1) * d3, maxZ = (iz + 1) * d3;
+    const int nx = s.p.num_buckets_x, ny = s.p.num_buckets_y, nz = s.p.num_buckets_z;
+    for (int jz = minZ; jz <= maxZ; jz++) {
+        if (jz < 0 || jz >= nz) continue;
+        for (int jy = iy - 1; jy <= iy + 1; jy++) {
+            if (jy < 0 || jy >= ny) continue;
+            for (int jx = ix - 1; jx <= ix + 1; jx++) {
+                if (jx < 0 || jx >= nx) continue;
+                const int jb = jz * s.p.num_buckets_xy + jy * nx + jx;
+                int j = s.first[jb];
+                while (j != -1) {
+                    fn(j);
+                    j = s.next[j];
+                }
+            }
+        }
+    }
+}
+
+inline bool partWall(const Sys& s) { return s.p.wall_type == MPS_WALL_PARTICLE; }
+
+// Rref = I - 2 n n^T with n = normalised 0.5 (x - xm); shared by the wall
+// operators (e.g. src/MpsPndNeigh.cpp:116-136).  Returns |n| before
+// normalisation in *nsqrt.
+inline void reflection(const double* xi, const double* xm, double* n, double* R, double* nsqrt) {
+    n[0] = 0.5 * (xi[0] - xm[0]);
+    n[1] = 0.5 * (xi[1] - xm[1]);
+    n[2] = 0.5 * (xi[2] - xm[2]);
+    double ns = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (ns > kEps) {
+        n[0] = n[0] / ns; n[1] = n[1] / ns; n[2] = n[2] / ns;
+    } else {
+        n[0] = 0; n[1] = 0; n[2] = 0;
+    }
+    R[0] = 1.0 - 2.0 * n[0] * n[0]; R[1] = 0.0 - 2.0 * n[0] * n[1]; R[2] = 0.0 - 2.0 * n[0] * n[2];
+    R[3] = 0.0 - 2.0 * n[1] * n[0]; R[4] = 1.0 - 2.0 * n[1] * n[1]; R[5] = 0.0 - 2.0 * n[1] * n[2];
+    R[6] = 0.0 - 2.0 * n[2] * n[0]; R[7] = 0.0 - 2.0 * n[2] * n[1]; R[8] = 1.0 - 2.0 * n[2] * n[2];
+    if (nsqrt) *nsqrt = ns;
+}
+// variant using |n|^2 > eps as the viscosity operators do (src/MpsViscosity.cpp:1549-1562)
+inline void reflectionMod2(const double* xi, const double* xm, double* n, double* R) {
+    n[0] = 0.5 * (xi[0] - xm[0]);
+    n[1] = 0.5 * (xi[1] - xm[1]);
+    n[2] = 0.5 * (xi[2] - xm[2]);
+    double m2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    if (m2 > kEps) {
+        double m = sqrt(m2);
+        n[0] = n[0] / m; n[1] = n[1] / m; n[2] = n[2] / m;
+    } else {
+        n[0] = 0; n[1] = 0; n[2] = 0;
+    }
+    R[0] = 1.0 - 2.0 * n[0] * n[0]; R[1] = 0.0 - 2.0 * n[0] * n[1]; R[2] = 0.0 - 2.0 * n[0] * n[2];
+    R[3] = 0.0 - 2.0 * n[1] * n[0]; R[4] = 1.0 - 2.0 * n[1] * n[1]; R[5] = 0.0 - 2.0 * n[1] * n[2];
+    R[6] = 0.0 - 2.0 * n[2] * n[0]; R[7] = 0.0 - 2.0 * n[2] * n[1]; R[8] = 1.0 - 2.0 * n[2] * n[2];
+}
+
+// ------------------------------------------------------------ parameters
+// MpsParticle::setParameters (src/MpsParticle.cpp:136-227) followed by
+// MpsBucket::allocateMemory / calcDomainLimits (src/MpsBucket.cpp:20-295).
+int deriveParams(Sys& s) {
+    mps_params& P = s.p;
+    if (P.derived) return 0;
+    const double l0 = P.part_dist;
+    P.eps_zero = kEps;
+    for (int k = 0; k < 3; k++) {
+        P.domain_min[k] -= l0 * 3.0;
+        P.domain_max[k] += l0 * 3.0;
+    }
+    if ((int)P.dim == 2) { P.domain_min[2] = 0.0; P.domain_max[2] = 0.0; }
+    P.num_of_meshs = (P.wall_type == 1 ? 1 : 0) + (P.fem_on ? 1 : 0) + (P.forced_on ? 1 : 0);
+    P.re_s = l0 * P.re_s;
+    P.re_l = l0 * P.re_l;
+    P.re_repulsive_force = l0 * P.re_repulsive_force;
+    P.re_s2 = P.re_s * P.re_s;
+    P.re_l2 = P.re_l * P.re_l;
+    P.eps_re = P.eps_re * P.re_s2 / 4.0;
+    double n0S = 0, n0L = 0, n0G = 0, lam = 0, nn0 = 0;
+    int lmin = (int)ceil(P.re_l / l0) + 1;
+    int lmax = (int)ceil(P.re_l / l0) + 2;
+    int f2 = 0, f3 = 1;
+    if ((int)P.dim == 2) { f2 = 1; f3 = 0; }
+    for (int ix = -lmin; ix < lmax; ix++)
+        for (int iy = -lmin; iy < lmax; iy++)
+            for (int iz = -lmin * f3; iz < lmax * f3 + f2; iz++) {
+                double x = l0 * (double)ix, y = l0 * (double)iy, z = l0 * (double)iz;
+                double d2 = x * x + y * y + z * z;
+                if (d2 <= P.re_l2) {
+                    if (d2 <= kEps) continue;
+                    double d = sqrt(d2);
+                    n0L += weight(d, P.re_l, P.weight_type);
+                    lam += d2 * weight(d, P.re_l, P.weight_type);
+                    nn0 += 1;
+                    if (d2 <= P.re_s2) {
+                        n0S += weight(d, P.re_s, P.weight_type);
+                        n0G += weight(d, P.re_s, P.weight_type);
+                    }
+                }
+            }
+    P.pnd_small_zero = n0S; P.pnd_large_zero = n0L; P.pnd_gradient_zero = n0G;
+    P.num_neigh_zero = nn0;
+    P.lambda_zero = lam / n0L;
+    P.coeff_viscosity = 2.0 * P.knm_vs1 * P.dim / (n0L * P.lambda_zero);
+    P.coeff_visc_multiphase = 2.0 * P.dim / (n0L * P.lambda_zero);
+    P.coeff_press_emps = P.sound_speed * P.sound_speed / n0S;
+    P.coeff_press_grad = -P.dim / n0G;
+    P.coeff_press_wcmps = P.sound_speed * P.sound_speed;
+    P.coeff_shifting1 = P.dri * l0 / n0S;
+    P.coeff_shifting2 = P.coef_a * l0 * l0 * P.cfl_number * P.mach_number;
+    P.coeff_ppe = 2.0 * P.dim / (n0L * P.lambda_zero);
+    P.coeff_ppe_source = P.relax_pnd / (P.time_step * P.time_step * n0S);
+    P.dns_fluid = P.density_fluid; P.dns_wall = P.density_wall;
+    P.inv_dns_fluid = 1.0 / P.density_fluid; P.inv_dns_wall = 1.0 / P.density_wall;
+    P.inv_part_dist = 1.0 / l0;
+    P.dist_collision_limit = l0 * P.dist_limit_ratio;
+    P.dist_collision_limit2 = P.dist_collision_limit * P.dist_collision_limit;
+    P.restitution_collision = 1.0 + P.collision_ratio;
+    P.beta_pnd = P.pnd_threshold * n0S;
+    P.beta_neigh = P.neigh_threshold * nn0;
+    P.delta2 = P.npcd_threshold * P.npcd_threshold * l0 * l0;
+    P.theta_arc = P.theta_threshold / 180.0 * 3.14159265;
+    P.h_threshold2 = 1.33 * 1.33 * l0 * l0;
+    P.dst_threshold2 = 2.0 * P.h_threshold2;
+    P.norm_threshold2 = P.norm_threshold * P.norm_threshold;
+
+    // buckets
+    P.bucket_side = P.re_l * (1.0 + P.cfl_number);
+    P.inv_bucket_side = 1.0 / P.bucket_side;
+    P.num_buckets_x = (int)((P.domain_max[0] - P.domain_min[0]) * P.inv_bucket_side) + 3;
+    P.num_buckets_y = (int)((P.domain_max[1] - P.domain_min[1]) * P.inv_bucket_side) + 3;
+    P.num_buckets_z = (int)((P.domain_max[2] - P.domain_min[2]) * P.inv_bucket_side) + 3;
+    if ((int)P.dim == 2) P.num_buckets_z = 1;
+    if (P.domain_type_bc != 0) { s.err = "periodic domains are not supported"; return -1; }
+    if (P.wall_type == MPS_WALL_PARTICLE) {
+        // calcDomainLimits, limitTypeBC and non-periodic branch
+        double lim[3][2];
+        if (P.limit_type_bc == 0) {
+            for (int k = 0; k < 3; k++) lim[k][0] = lim[k][1] = s.pos[k];
+            for (int i = 0; i < s.N; i++)
+                for (int k = 0; k < 3; k++) {
+                    lim[k][0] = std::min(lim[k][0], s.pos[i * 3 + k]);
+                    lim[k][1] = std::max(lim[k][1], s.pos[i * 3 + k]);
+                }
+            int td = 0;
+            for (int k = 0; k < 3; k++) if (lim[k][0] != lim[k][1]) td++;
+            if (td != (int)P.dim) { s.err = "dimension mismatch between json and particles"; return -1; }
+        } else {
+            for (int k = 0; k < 3; k++) { lim[k][0] = P.domain_min[k]; lim[k][1] = P.domain_max[k]; }
+        }
+        lim[0][0] -= 0.5 * l0; lim[0][1] += 0.5 * l0;
+        lim[1][0] -= 0.5 * l0; lim[1][1] += 0.5 * l0;
+        if (P.dim == 3) { lim[2][0] -= 0.5 * l0; lim[2][1] += 0.5 * l0; }
+        for (int k = 0; k < (int)P.dim; k++) {
+            lim[k][0] -= P.bucket_side;
+            lim[k][1] += P.bucket_side;
+            long nb = (long)((lim[k][1] - lim[k][0]) / P.bucket_side + 1);
+            if (k == 0) P.num_buckets_x = (int)nb;
+            if (k == 1) P.num_buckets_y = (int)nb;
+            if (k == 2) P.num_buckets_z = (int)nb;
+            lim[k][1] = lim[k][0] + nb * P.bucket_side;
+        }
+        for (int k = 0; k < 3; k++) { P.domain_min[k] = lim[k][0]; P.domain_max[k] = lim[k][1]; }
+        P.inv_bucket_side = 1.0 / P.bucket_side;
+    }
+    P.num_buckets_xy = P.num_buckets_x * P.num_buckets_y;
+    P.num_buckets_xyz = P.num_buckets_xy * P.num_buckets_z;
+    P.derived = 1;
+    return 0;
+}
+
+// ---------------------------------------------------------- MpsBucket
+// MpsBucket::updateParticlesID (src/MpsBucket.cpp:299-353)
+void updateParticlesID(Sys& s) {
+    const mps_params& P = s.p;
+    s.first.assign(P.num_buckets_xyz, -1);
+    s.last.assign(P.num_buckets_xyz, -1);
+    s.next.assign(s.N0, -1);
+    for (int i = 0; i < s.N; i++) {
+        if (s.type[i] == P.ghost) continue;
+        int ib;
+        if ((int)P.dim == 2) {
+            int ix = (int)((s.pos[i * 3] - P.domain_min[0]) * P.inv_bucket_side + kEps);
+            int iy = (int)((s.pos[i * 3 + 1] - P.domain_min[1]) * P.inv_bucket_side + kEps);
+            ib = iy * P.num_buckets_x + ix;
+        } else {
+            int ix = (int)((s.pos[i * 3] - P.domain_min[0]) * P.inv_bucket_side);
+            int iy = (int)((s.pos[i * 3 + 1] - P.domain_min[1]) * P.inv_bucket_side);
+            int iz = (int)((s.pos[i * 3 + 2] - P.domain_min[2]) * P.inv_bucket_side);
+            ib = iz * P.num_buckets_xy + iy * P.num_buckets_x + ix;
+        }
+        if (ib < 0 || ib >= P.num_buckets_xyz) continue;  // outside the grid (UB in the reference)
+        int j = s.last[ib];
+        s.last[ib] = i;
+        if (j == -1) s.first[ib] = i; else s.next[j] = i;
+    }
+}
+
+// --------------------------------------------------------- MpsViscosity
+// MpsViscosity::calcViscosity (src/MpsViscosity.cpp:20-102)
+void calcViscosity(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        double meu_i = s.MEU[i];
+        double ax = 0, ay = 0, az = 0;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double ux = s.vel[i * 3], uy = s.vel[i * 3 + 1], uz = s.vel[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_l2 && (d2 < dm2 || pw)) {
+                if (j != i) {
+                    double dst = sqrt(d2);
+                    double wL = weight(dst, P.re_l, P.weight_type);
+                    double nu;
+                    if ((meu_i + s.MEU[j]) > kEps) nu = 2.0 * meu_i * s.MEU[j] / (meu_i + s.MEU[j]);
+                    else nu = 0.0;
+                    if (s.type[j] == P.wall) nu = 2.0 * meu_i;
+                    nu = nu / s.RHO[i];
+                    ax += (s.vel[j * 3] - ux) * wL * nu;
+                    ay += (s.vel[j * 3 + 1] - uy) * wL * nu;
+                    az += (s.vel[j * 3 + 2] - uz) * wL * nu;
+                }
+            }
+        });
+        s.acc[i * 3] = P.coeff_visc_multiphase * ax;
+        s.acc[i * 3 + 1] = P.coeff_visc_multiphase * ay;
+        s.acc[i * 3 + 2] = P.coeff_visc_multiphase * az;
+    }
+}
+
+// MpsViscosity::calcWallSlipViscosity / calcWallNoSlipViscosity
+// (src/MpsViscosity.cpp:1527-1686, 1687-1910)
+void calcWallViscosity(Sys& s) {
+    const mps_params& P = s.p;
+    const double vol = pow(P.part_dist, P.dim);
+    const bool slip = P.slip_condition == MPS_FREE_SLIP;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        if (!(s.type[i] == P.fluid && s.nearWall[i])) continue;
+        double meu_i = s.MEU[i];
+        double ax = 0, ay = 0, az = 0;
+        const double* xi = &s.pos[i * 3];
+        const double* xm = &s.mirror[i * 3];
+        const double ux = s.vel[i * 3], uy = s.vel[i * 3 + 1], uz = s.vel[i * 3 + 2];
+        double n[3], R[9];
+        reflectionMod2(xi, xm, n, R);
+        double umx, umy, umz;
+        double Rm[9];
+        if (slip) {
+            umx = R[0] * ux + R[1] * uy + R[2] * uz;
+            umy = R[3] * ux + R[4] * uy + R[5] * uz;
+            umz = R[6] * ux + R[7] * uy + R[8] * uz;
+            for (int k = 0; k < 9; k++) Rm[k] = R[k];
+        } else {
+            double Ri[9] = {-1.0, 0.0, 0.0, 0.0, -1.0, 0.0, 0.0, 0.0, -1.0};
+            double vw[3] = {0.0, 0.0, 0.0};
+            double dotnv = n[0] * vw[0] + n[1] * vw[1] + n[2] * vw[2];
+            double vt[3];
+            vt[0] = ux - 2.0 * (vw[0] - dotnv * n[0]);
+            vt[1] = uy - 2.0 * (vw[1] - dotnv * n[1]);
+            vt[2] = uz - 2.0 * (vw[2] - dotnv * n[2]);
+            umx = Ri[0] * vt[0] + Ri[1] * vt[1] + Ri[2] * vt[2];
+            umy = Ri[3] * vt[0] + Ri[4] * vt[1] + Ri[5] * vt[2];
+            umz = Ri[6] * vt[0] + Ri[7] * vt[1] + Ri[8] * vt[2];
+            for (int k = 0; k < 9; k++) Rm[k] = Ri[k];
+        }
+        const double dns_i = (s.ptype[i] == 1) ? P.dns_fl1 : P.dns_fl2;
+        walk(s, xi[0], xi[1], xi[2], [&](int j) {
+            double v0 = s.pos[j * 3] - xi[0], v1 = s.pos[j * 3 + 1] - xi[1], v2 = s.pos[j * 3 + 2] - xi[2];
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - xm[0], w1 = s.pos[j * 3 + 1] - xm[1], w2 = s.pos[j * 3 + 2] - xm[2];
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_l2 && dm2 < P.re_l2 && d2 < dm2) {
+                if (j != i) {
+                    double dst = sqrt(dm2);
+                    double wL = weight(dst, P.re_l, P.weight_type);
+                    double nu;
+                    if ((meu_i + s.MEU[j]) > kEps) nu = 2.0 * meu_i * s.MEU[j] / (meu_i + s.MEU[j]);
+                    else nu = 0.0;
+                    nu = nu / dns_i;
+                    ax += (s.vel[j * 3] - umx) * wL * nu;
+                    ay += (s.vel[j * 3 + 1] - umy) * wL * nu;
+                    az += (s.vel[j * 3 + 2] - umz) * wL * nu;
+                }
+            }
+        });
+        {
+            double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (dm2 < P.re_l2) {
+                double dst = sqrt(dm2);
+                double wL = weight(dst, P.re_l, P.weight_type);
+                double nu;
+                if (meu_i > kEps) nu = 2.0 * meu_i * meu_i / (meu_i + meu_i);
+                else nu = 0.0;
+                nu = nu / dns_i;
+                ax += (ux - umx) * wL * nu;
+                ay += (uy - umy) * wL * nu;
+                az += (uz - umz) * wL * nu;
+            }
+        }
+        double rx = (Rm[0] * ax + Rm[1] * ay + Rm[2] * az) * P.coeff_visc_multiphase;
+        double ry = (Rm[3] * ax + Rm[4] * ay + Rm[5] * az) * P.coeff_visc_multiphase;
+        double rz = (Rm[6] * ax + Rm[7] * ay + Rm[8] * az) * P.coeff_visc_multiphase;
+        s.acc[i * 3] += rx;
+        s.acc[i * 3 + 1] += ry;
+        s.acc[i * 3 + 2] += rz;
+        s.forceWall[i * 3] += -(Rm[0] * ax + Rm[1] * ay + Rm[2] * az) * P.coeff_visc_multiphase * vol * s.RHO[i];
+        s.forceWall[i * 3 + 1] += -(Rm[3] * ax + Rm[4] * ay + Rm[5] * az) * P.coeff_visc_multiphase * vol * s.RHO[i];
+        s.forceWall[i * 3 + 2] += -(Rm[6] * ax + Rm[7] * ay + Rm[8] * az) * P.coeff_visc_multiphase * vol * s.RHO[i];
+    }
+}
+
+// ------------------------------------------------ MpsParticle / VelPos
+// MpsParticle::addGravity (src/MpsParticle.cpp:111-130)
+void addGravity(Sys& s) {
+    const mps_params& P = s.p;
+#pragma omp parallel for
+    for (int i = 0; i < s.N; i++) {
+        s.acc[i * 3] += P.gravity[0];
+        s.acc[i * 3 + 1] += P.gravity[1];
+        s.acc[i * 3 + 2] += P.gravity[2];
+        s.accStar[i * 3] = s.acc[i * 3];
+        s.accStar[i * 3 + 1] = s.acc[i * 3 + 1];
+        s.accStar[i * 3 + 2] = s.acc[i * 3 + 2];
+    }
+}
+
+// MpsParticleVelPos::predictionVelocityPosition (src/MpsParticleVelPos.cpp:20-50)
+void prediction(Sys& s) {
+    const mps_params& P = s.p;
+    const double far = 10e8 * P.part_dist;
+#pragma omp parallel for
+    for (int i = 0; i < s.N; i++) {
+        if (s.type[i] == P.fluid) {
+            for (int k = 0; k < 3; k++) s.vel[i * 3 + k] += s.acc[i * 3 + k] * P.time_step;
+            for (int k = 0; k < 3; k++) s.pos[i * 3 + k] += s.vel[i * 3 + k] * P.time_step;
+        }
+        for (int k = 0; k < 3; k++) {
+            s.acc[i * 3 + k] = 0.0; s.dvel[i * 3 + k] = 0.0;
+            s.wpf1[i * 3 + k] = 0.0; s.wpf2[i * 3 + k] = 0.0; s.npcd[i * 3 + k] = 0.0;
+        }
+        s.numNeighWall[i] = 0;
+        s.nearWall[i] = 0;
+        s.riw2[i] = far;
+        s.nNeighSurf[i] = 0.0;
+        if (P.wall_type == MPS_WALL_POLYGON)
+            for (int k = 0; k < 3; k++) s.mirror[i * 3 + k] = far;
+    }
+}
+
+// MpsParticleVelPos::correctionVelocityPosition (src/MpsParticleVelPos.cpp:53-97)
+void correction(Sys& s) {
+    const mps_params& P = s.p;
+    double vmax = 0.0;
+    for (int i = 0; i < s.N; i++) {
+        if (s.type[i] != P.fluid) continue;
+        for (int k = 0; k < 3; k++) s.vel[i * 3 + k] += s.acc[i * 3 + k] * P.time_step;
+        for (int k = 0; k < 3; k++) s.pos[i * 3 + k] += s.acc[i * 3 + k] * P.time_step * P.time_step;
+        for (int k = 0; k < 3; k++) s.acc[i * 3 + k] = 0.0;
+        double v2 = s.vel[i * 3] * s.vel[i * 3] + s.vel[i * 3 + 1] * s.vel[i * 3 + 1] + s.vel[i * 3 + 2] * s.vel[i * 3 + 2];
+        if (v2 > vmax * vmax) vmax = sqrt(v2);
+    }
+    s.velMax = vmax;
+    s.cflCurrent = P.time_step * s.velMax / P.part_dist;
+}
+
+// MpsParticleVelPos::updateVelocityParticlesWallDummy (src/MpsParticleVelPos.cpp:100-158)
+void wallDummyVelocity(Sys& s) {
+    const mps_params& P = s.p;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        if (s.type[i] != P.dummy_wall) continue;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        double dx = 0, dy = 0, dz = 0, ni = 0;
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            if (d2 < P.re_l2 && j != i && s.type[j] == P.fluid) {
+                double wL = weight(sqrt(d2), P.re_l, P.weight_type);
+                ni += wL;
+                dx += s.vel[j * 3] * wL;
+                dy += s.vel[j * 3 + 1] * wL;
+                dz += s.vel[j * 3 + 2] * wL;
+            }
+        });
+        if (ni > kEps) {
+            s.vel[i * 3] = 2.0 * 0.0 - dx / ni;
+            s.vel[i * 3 + 1] = 2.0 * 0.0 - dy / ni;
+            s.vel[i * 3 + 2] = 2.0 * 0.0 - dz / ni;
+        } else {
+            s.vel[i * 3] = 2.0 * 0.0 - dx;
+            s.vel[i * 3 + 1] = 2.0 * 0.0 - dy;
+            s.vel[i * 3 + 2] = 2.0 * 0.0 - dz;
+        }
+    }
+}
+
+// MpsParticle::checkParticleOutDomain (src/MpsParticle.cpp:234-402), non-periodic part
+void checkOutDomain(Sys& s) {
+    const mps_params& P = s.p;
+    int newN = s.N;
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+        lo[k] = P.domain_min[k] + P.bucket_side;
+        hi[k] = P.domain_max[k] - P.bucket_side;
+    }
+    for (int i = 0; i < s.N; i++) {
+        const double* x = &s.pos[i * 3];
+        bool out = x[0] > hi[0] || x[0] < lo[0] || x[1] > hi[1] || x[1] < lo[1] ||
+                   (P.dim == 3 && (x[2] > hi[2] || x[2] < lo[2]));
+        if (s.type[i] == P.fluid && i < newN && out) {
+            int L = newN - 1;
+            s.type[i] = s.type[L]; s.bc[i] = s.bc[L]; s.numNeigh[i] = s.numNeigh[L];
+            s.press[i] = s.press[L]; s.pressAvg[i] = s.pressAvg[L];
+            s.pndi[i] = s.pndi[L]; s.pndki[i] = s.pndki[L]; s.pndski[i] = s.pndski[L];
+            s.pndSmall[i] = s.pndSmall[L]; s.npcd2[i] = s.npcd2[L]; s.conc[i] = s.conc[L];
+            s.divU[i] = s.divU[L]; s.diffTerm[i] = s.diffTerm[L]; s.nearMeshType[i] = s.nearMeshType[L];
+            s.nearWall[i] = s.nearWall[L]; s.numNeighWall[i] = s.numNeighWall[L];
+            s.pndWall[i] = s.pndWall[L]; s.devDotN[i] = s.devDotN[L]; s.nNeighSurf[i] = s.nNeighSurf[L];
+            s.riw2[i] = s.riw2[L]; s.ptype[i] = s.ptype[L]; s.RHO[i] = s.RHO[L]; s.MEU[i] = s.MEU[L];
+            s.elementID[i] = s.elementID[L];
+            if (P.mps_type == MPS_IMPLICIT_PND || P.mps_type == MPS_IMPLICIT_PND_DIVU) {
+                s.ppeX[i] = s.ppeX[L];
+                s.ppeB[i] = s.ppeB[L];
+            }
+            for (int k = 0; k < 3; k++) {
+                s.acc[i * 3 + k] = s.acc[L * 3 + k]; s.accStar[i * 3 + k] = s.accStar[L * 3 + k];
+                s.pos[i * 3 + k] = s.pos[L * 3 + k]; s.vel[i * 3 + k] = s.vel[L * 3 + k];
+                s.npcd[i * 3 + k] = s.npcd[L * 3 + k]; s.gradC[i * 3 + k] = s.gradC[L * 3 + k];
+                s.cm1[i * 3 + k] = s.cm1[L * 3 + k]; s.cm2[i * 3 + k] = s.cm2[L * 3 + k];
+                s.cm3[i * 3 + k] = s.cm3[L * 3 + k]; s.normal[i * 3 + k] = s.normal[L * 3 + k];
+                s.dvel[i * 3 + k] = s.dvel[L * 3 + k]; s.wallPos[i * 3 + k] = s.wallPos[L * 3 + k];
+                s.mirror[i * 3 + k] = s.mirror[L * 3 + k]; s.wpf1[i * 3 + k] = s.wpf1[L * 3 + k];
+                s.wpf2[i * 3 + k] = s.wpf2[L * 3 + k]; s.polyN[i * 3 + k] = s.polyN[L * 3 + k];
+                s.forceWall[i * 3 + k] = s.forceWall[L * 3 + k];
+            }
+            s.type[L] = P.ghost; s.bc[L] = P.other; s.nearWall[L] = 0; s.nearMeshType[L] = 0;
+            s.riw2[L] = 10e8 * P.part_dist;
+            for (int k = 0; k < 3; k++) s.vel[L * 3 + k] = 0.0;
+            s.press[L] = 0.0;
+            s.pos[L * 3] = P.domain_max[0] - P.part_dist;
+            s.pos[L * 3 + 1] = P.domain_max[1] - P.part_dist;
+            if (P.dim == 2) s.pos[L * 3 + 2] = 0.0;
+            else s.pos[L * 3 + 2] = P.domain_max[2] - P.part_dist;
+            newN--;
+        }
+        s.N = newN;
+    }
+}
+
+// ------------------------------------------------- MpsParticleCollision
+// checkParticleCollisions (src/MpsParticleCollision.cpp:22-119)
+void collisionPC(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        if (s.type[i] != P.fluid) continue;
+        double mi = (s.ptype[i] == 1) ? P.dns_fl1 : P.dns_fl2;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double ux = s.vel[i * 3], uy = s.vel[i * 3 + 1], uz = s.vel[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        double dux = 0, duy = 0, duz = 0;
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.dist_collision_limit2 && (d2 < dm2 || pw) && j != i) {
+                double fDT = (ux - s.vel[j * 3]) * v0 + (uy - s.vel[j * 3 + 1]) * v1 + (uz - s.vel[j * 3 + 2]) * v2;
+                if (fDT > 0.0) {
+                    double mj;
+                    if (s.type[j] == P.fluid) mj = (s.ptype[j] == 1) ? P.dns_fl1 : P.dns_fl2;
+                    else mj = P.dns_wall;
+                    fDT *= P.restitution_collision * mj / (mi + mj) / d2;
+                    if (s.type[j] == P.fluid) {
+                        dux -= v0 * fDT; duy -= v1 * fDT; duz -= v2 * fDT;
+                    } else {
+                        dux -= 2.0 * v0 * fDT; duy -= 2.0 * v1 * fDT; duz -= 2.0 * v2 * fDT;
+                    }
+                }
+            }
+        });
+        s.dvel[i * 3] = dux; s.dvel[i * 3 + 1] = duy; s.dvel[i * 3 + 2] = duz;
+    }
+#pragma omp parallel for
+    for (int i = 0; i < s.N; i++) {
+        if (s.type[i] == P.fluid)
+            for (int k = 0; k < 3; k++) s.vel[i * 3 + k] += s.dvel[i * 3 + k];
+        for (int k = 0; k < 3; k++) s.dvel[i * 3 + k] = 0.0;
+    }
+}
+
+// checkDynamicParticleCollisions (src/MpsParticleCollision.cpp:124-292)
+void collisionDPC(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+    const double Wij5 = 0.5 * 0.5 * 0.5 * 0.5 * 3.0;
+    const double gmod = sqrt(P.gravity[0] * P.gravity[0] + P.gravity[1] * P.gravity[1] + P.gravity[2] * P.gravity[2]);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        if (s.type[i] != P.fluid) continue;
+        double mi = (s.ptype[i] == 1) ? P.dns_fl1 : P.dns_fl2;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double ux = s.vel[i * 3], uy = s.vel[i * 3 + 1], uz = s.vel[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        double dux = 0, duy = 0, duz = 0;
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.part_dist * P.part_dist && (d2 < dm2 || pw) && j != i) {
+                double mj;
+                if (s.type[j] == P.fluid) mj = (s.ptype[j] == 1) ? P.dns_fl1 : P.dns_fl2;
+                else mj = P.dns_wall;
+                double dst = sqrt(d2);
+                double Wij = 0.0;
+                if (dst > kEps) {
+                    double w1 = 1.0 - dst * P.inv_part_dist;
+                    double w2 = 4.0 * dst * P.inv_part_dist + 1.0;
+                    Wij = w1 * w1 * w1 * w1 * w2;
+                }
+                double chi = sqrt(Wij / Wij5);
+                double kappa = (dst < 0.5 * P.part_dist) ? 1.0 : chi;
+                double fDT = (ux - s.vel[j * 3]) * v0 + (uy - s.vel[j * 3 + 1]) * v1 + (uz - s.vel[j * 3 + 2]) * v2;
+                if (fDT > 0.0) {
+                    fDT *= kappa * 2.0 * mj / (mi + mj) / d2;
+                    if (s.type[j] == P.fluid) {
+                        dux -= v0 * fDT; duy -= v1 * fDT; duz -= v2 * fDT;
+                    } else {
+                        dux -= 2.0 * v0 * fDT; duy -= 2.0 * v1 * fDT; duz -= 2.0 * v2 * fDT;
+                    }
+                } else {
+                    double pmax = 2.0 / 3.0 * s.RHO[i] * gmod * 0.2;
+                    double pmin = s.RHO[i] * gmod * P.part_dist;
+                    double ptil = std::max(std::min(P.lambda_collision * fabs(s.press[i] + s.press[j]), P.lambda_collision * pmax), pmin);
+                    double pb = ptil * chi;
+                    double rep = P.time_step / mi * chi * pb / d2;
+                    if (s.type[j] == P.fluid) {
+                        dux -= v0 * rep; duy -= v1 * rep; duz -= v2 * rep;
+                    } else {
+                        dux -= 2.0 * v0 * rep; duy -= 2.0 * v1 * rep; duz -= 2.0 * v2 * rep;
+                    }
+                }
+            }
+        });
+        s.dvel[i * 3] = dux; s.dvel[i * 3 + 1] = duy; s.dvel[i * 3 + 2] = duz;
+    }
+#pragma omp parallel for
+    for (int i = 0; i < s.N; i++) {
+        if (s.type[i] == P.fluid) {
+            for (int k = 0; k < 3; k++) s.vel[i * 3 + k] += s.dvel[i * 3 + k];
+            for (int k = 0; k < 3; k++) s.pos[i * 3 + k] += s.dvel[i * 3 + k] * P.time_step;
+        }
+        for (int k = 0; k < 3; k++) s.dvel[i * 3 + k] = 0.0;
+    }
+}
+
+// --------------------------------------------------------- PolygonMesh
+// PolygonMesh::initWijnNeigh (src/PolygonMesh.cpp:377-527)
+void initWijnNeigh(Sys& s) {
+    Mesh& m = s.mesh;
+    const int dim = (int)s.p.dim, wt = s.p.weight_type;
+    const double lo = s.p.part_dist, reL = s.p.re_l, reS = s.p.re_s;
+    const double reS2 = reS * reS, reL2 = reL * reL;
+    const int n = 50;
+    m.xPND.clear(); m.xNeigh.clear(); m.Zij.clear(); m.nNeighWall.clear();
+    int limS = (int)ceil(reS / lo), limL = (int)ceil(reL / lo);
+    for (int i = 1; i <= n; i++) {
+        double xi = 0.0, yi = i * reS / n, zi = 0.0;
+        double riw = (dim == 2) ? sqrt((0.0 - xi) * (0.0 - xi) + (0.0 - yi) * (0.0 - yi))
+                                : sqrt((0.0 - xi) * (0.0 - xi) + (0.0 - yi) * (0.0 - yi) + (0.0 - zi) * (0.0 - zi));
+        m.xPND.push_back(riw / reS);
+        m.xNeigh.push_back(riw / reL);
+        double n0 = 0.0;
+        int zlo = dim == 2 ? 0 : -limS, zhi = dim == 2 ? 0 : limS;
+        for (int ix = -limS; ix <= limS; ix++)
+            for (int iy = -limS; iy <= 0; iy++)
+                for (int iz = zlo; iz <= zhi; iz++) {
+                    double xj = lo * (double)ix, yj = lo * (double)(iy - 0.5), zj = lo * (double)iz;
+                    double r2 = (dim == 2) ? (xj - xi) * (xj - xi) + (yj - yi) * (yj - yi)
+                                           : (xj - xi) * (xj - xi) + (yj - yi) * (yj - yi) + (zj - zi) * (zj - zi);
+                    if (r2 < reS2) {
+                        if (r2 == 0.0) continue;
+                        double r = sqrt(r2);
+                        if (wt == 0) n0 += reS / r - 1.0;
+                        else if (wt == 1) n0 += reS / r + r / reS - 2.0;
+                        else if (wt == 2) n0 += reS / r - r / reS;
+                        else if (wt == 3) n0 += pow(1.0 - r / reS, 3.0);
+                        else if (wt == 4) n0 += pow(1.0 - r / reS, 2.0);
+                        else n0 += reS / r - 1.0;
+                    }
+                }
+        m.Zij.push_back(n0);
+        int nn = 0;
+        zlo = dim == 2 ? 0 : -limL; zhi = dim == 2 ? 0 : limL;
+        for (int ix = -limL; ix <= limL; ix++)
+            for (int iy = -limL; iy <= 0; iy++)
+                for (int iz = zlo; iz <= zhi; iz++) {
+                    double xj = lo * (double)ix, yj = lo * (double)(iy - 0.5), zj = lo * (double)iz;
+                    double r2 = (dim == 2) ? (xj - xi) * (xj - xi) + (yj - yi) * (yj - yi)
+                                           : (xj - xi) * (xj - xi) + (yj - yi) * (yj - yi) + (zj - zi) * (zj - zi);
+                    if (r2 < reL2) {
+                        if (r2 == 0.0) continue;
+                        nn += 1;
+                    }
+                }
+        m.nNeighWall.push_back(nn);
+    }
+}
+
+// src/PolygonMesh.cpp:533-554
+double interpolateWij(const Mesh& m, double re, double x) {
+    int sz = (int)m.xPND.size();
+    double r = x / re;
+    int i = 0;
+    if (r >= m.xPND[sz - 2]) i = sz - 2;
+    else while (r > m.xPND[i + 1]) i++;
+    double xL = m.xPND[i], yL = m.Zij[i], xR = m.xPND[i + 1], yR = m.Zij[i + 1];
+    double dydx = (yR - yL) / (xR - xL);
+    return yL + dydx * (r - xL);
+}
+// src/PolygonMesh.cpp:560-581
+int interpolateNumNeighWall(const Mesh& m, double re, double x) {
+    int sz = (int)m.xNeigh.size();
+    double r = x / re;
+    int i = 0;
+    if (r >= m.xNeigh[sz - 2]) i = sz - 2;
+    else while (r > m.xNeigh[i + 1]) i++;
+    double xL = m.xNeigh[i], yL = (double)m.nNeighWall[i], xR = m.xNeigh[i + 1], yR = (double)m.nNeighWall[i + 1];
+    double dydx = (yR - yL) / (xR - xL);
+    return (int)(yL + dydx * (r - xL));
+}
+
+// Closest point on triangle (a,b,c) to p: Ericson, Real-Time Collision
+// Detection, section 5.1.5 (Voronoi-region case analysis).
+inline void closestPtTriangle(const double* p, const double* a, const double* b, const double* c, double* q) {
+    double ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    double ap[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+    double d1 = ab[0] * ap[0] + ab[1] * ap[1] + ab[2] * ap[2];
+    double d2 = ac[0] * ap[0] + ac[1] * ap[1] + ac[2] * ap[2];
+    if (d1 <= 0.0 && d2 <= 0.0) { q[0] = a[0]; q[1] = a[1]; q[2] = a[2]; return; }
+    double bp[3] = {p[0] - b[0], p[1] - b[1], p[2] - b[2]};
+    double d3 = ab[0] * bp[0] + ab[1] * bp[1] + ab[2] * bp[2];
+    double d4 = ac[0] * bp[0] + ac[1] * bp[1] + ac[2] * bp[2];
+    if (d3 >= 0.0 && d4 <= d3) { q[0] = b[0]; q[1] = b[1]; q[2] = b[2]; return; }
+    double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        double v = d1 / (d1 - d3);
+        q[0] = a[0] + v * ab[0]; q[1] = a[1] + v * ab[1]; q[2] = a[2] + v * ab[2];
+        return;
+    }
+    double cp[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
+    double d5 = ab[0] * cp[0] + ab[1] * cp[1] + ab[2] * cp[2];
+    double d6 = ac[0] * cp[0] + ac[1] * cp[1] + ac[2] * cp[2];
+    if (d6 >= 0.0 && d5 <= d6) { q[0] = c[0]; q[1] = c[1]; q[2] = c[2]; return; }
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        double w = d2 / (d2 - d6);
+        q[0] = a[0] + w * ac[0]; q[1] = a[1] + w * ac[1]; q[2] = a[2] + w * ac[2];
+        return;
+    }
+    double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        q[0] = b[0] + w * (c[0] - b[0]); q[1] = b[1] + w * (c[1] - b[1]); q[2] = b[2] + w * (c[2] - b[2]);
+        return;
+    }
+    double denom = 1.0 / (va + vb + vc);
+    double v = vb * denom, w = vc * denom;
+    q[0] = a[0] + ab[0] * v + ac[0] * w;
+    q[1] = a[1] + ab[1] * v + ac[1] * w;
+    q[2] = a[2] + ab[2] * v + ac[2] * w;
+}
+
+// PolygonMesh::closestPointPNDBoundaryAABB (src/PolygonMesh.cpp:585-715), one
+// static mesh, brute force over faces.
+void closestPoint(Sys& s) {
+    const Mesh& m = s.mesh;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < s.N; i++) {
+        const double* p = &s.pos[i * 3];
+        double best = DBL_MAX, q[3] = {0, 0, 0}, bq[3] = {0, 0, 0};
+        int bf = -1;
+        for (int f = 0; f < m.nf; f++) {
+            const int* t = &m.F[f * 3];
+            closestPtTriangle(p, &m.V[t[0] * 3], &m.V[t[1] * 3], &m.V[t[2] * 3], q);
+            double e0 = p[0] - q[0], e1 = p[1] - q[1], e2 = p[2] - q[2];
+            double d2 = e0 * e0 + e1 * e1 + e2 * e2;
+            if (d2 < best) { best = d2; bf = f; bq[0] = q[0]; bq[1] = q[1]; bq[2] = q[2]; }
+        }
+        if (bf < 0) continue;
+        if (best < s.riw2[i]) {
+            s.riw2[i] = best;
+            for (int k = 0; k < 3; k++) {
+                s.wallPos[i * 3 + k] = bq[k];
+                s.mirror[i * 3 + k] = p[k] + 2.0 * (bq[k] - p[k]);
+                s.polyN[i * 3 + k] = m.Nrm[bf * 3 + k];
+            }
+            s.nearMeshType[i] = 0;
+        }
+    }
+}
+
+// PolygonMesh::updateParticlesNearPolygonMesh (src/PolygonMesh.cpp:718-795)
+void updateNearMesh(Sys& s) {
+    const mps_params& P = s.p;
+#pragma omp parallel for
+    for (int i = 0; i < s.N; i++) {
+        s.pndWall[i] = 0.0;
+        s.numNeighWall[i] = 0;
+        double x2 = s.riw2[i];
+        if (x2 < P.re_l2) {
+            s.nearWall[i] = 1;
+            double x = sqrt(x2);
+            s.pndWall[i] = interpolateWij(s.mesh, sqrt(P.re_s2), x);
+            s.numNeighWall[i] = interpolateNumNeighWall(s.mesh, sqrt(P.re_l2), x);
+        }
+    }
+}
+
+// ---------------------------------------------------------- MpsPndNeigh
+// setInitialPndNumberOfNeighNPCD (src/MpsPndNeigh.cpp:21-101)
+void initialPnd(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        double wSum = 0.0;
+        s.numNeigh[i] = 0;
+        s.npcd[i * 3] = s.npcd[i * 3 + 1] = s.npcd[i * 3 + 2] = 0.0;
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_l2 && (d2 < dm2 || pw) && j != i) {
+                s.numNeigh[i] += 1;
+                if (d2 < P.re_s2) {
+                    double wS = weight(sqrt(d2), P.re_s, P.weight_type);
+                    s.pndi[i] += wS;
+                    s.npcd[i * 3] += v0 * wS;
+                    s.npcd[i * 3 + 1] += v1 * wS;
+                    s.npcd[i * 3 + 2] += v2 * wS;
+                    wSum += wS;
+                }
+            }
+        });
+        s.pndi[i] += s.pndWall[i];
+        if (s.type[i] == P.wall) s.pndi[i] = P.pnd_small_zero;
+        s.pndSmall[i] = s.pndi[i];
+        s.pndki[i] = s.pndi[i];
+        s.numNeigh[i] += s.numNeighWall[i];
+        if (wSum > kEps) {
+            s.npcd[i * 3] /= s.pndSmall[i];
+            s.npcd[i * 3 + 1] /= s.pndSmall[i];
+            s.npcd[i * 3 + 2] /= s.pndSmall[i];
+        }
+        s.npcd2[i] = s.npcd[i * 3] * s.npcd[i * 3] + s.npcd[i * 3 + 1] * s.npcd[i * 3 + 1] + s.npcd[i * 3 + 2] * s.npcd[i * 3 + 2];
+        if (s.npcd[i * 3] * s.polyN[i * 3] + s.npcd[i * 3 + 1] * s.polyN[i * 3 + 1] + s.npcd[i * 3 + 2] * s.polyN[i * 3 + 2] < 0.0)
+            s.devDotN[i] = 1;
+        else
+            s.devDotN[i] = -1;
+    }
+}
+
+// calcWallNPCD (src/MpsPndNeigh.cpp:105-195)
+void wallNPCD(Sys& s) {
+    const mps_params& P = s.p;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        if (!(s.type[i] == P.fluid && s.nearWall[i])) continue;
+        double nx = 0, ny = 0, nz = 0;
+        const double* xi = &s.pos[i * 3];
+        const double* xm = &s.mirror[i * 3];
+        double n[3], R[9];
+        reflection(xi, xm, n, R, nullptr);
+        walk(s, xi[0], xi[1], xi[2], [&](int j) {
+            double v0 = s.pos[j * 3] - xi[0], v1 = s.pos[j * 3 + 1] - xi[1], v2 = s.pos[j * 3 + 2] - xi[2];
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - xm[0], w1 = s.pos[j * 3 + 1] - xm[1], w2 = s.pos[j * 3 + 2] - xm[2];
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2 && j != i) {
+                double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
+                nx += w0 * wS; ny += w1 * wS; nz += w2 * wS;
+            }
+        });
+        double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
+        double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (dm2 < P.re_s2) {
+            double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
+            nx += w0 * wS; ny += w1 * wS; nz += w2 * wS;
+        }
+        s.npcd[i * 3] += R[0] * nx + R[1] * ny + R[2] * nz;
+        s.npcd[i * 3 + 1] += R[3] * nx + R[4] * ny + R[5] * nz;
+        s.npcd[i * 3 + 2] += R[6] * nx + R[7] * ny + R[8] * nz;
+    }
+}
+
+// calcPndnNeighNPCD (src/MpsPndNeigh.cpp:199-331)
+void pndNeighNPCD(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        double ni = 0.0, wSum = 0.0;
+        int nn = s.numNeighWall[i];
+        double c0 = s.npcd[i * 3], c1 = s.npcd[i * 3 + 1], c2 = s.npcd[i * 3 + 2];
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_l2 && (d2 < dm2 || pw) && j != i) {
+                nn += 1;
+                if (d2 < P.re_s2) {
+                    double wS = weight(sqrt(d2), P.re_s, P.weight_type);
+                    ni += wS;
+                    c0 += v0 * wS; c1 += v1 * wS; c2 += v2 * wS;
+                    wSum += wS;
+                }
+            }
+        });
+        s.numNeigh[i] = nn;
+        if (P.pnd_type == MPS_PND_SUM_WIJ || P.pnd_type == MPS_PND_MEAN_SUM_WIJ) {
+            s.pndki[i] = s.pndi[i];
+            s.pndi[i] = ni + s.pndWall[i];
+        }
+        s.pndSmall[i] = ni + s.pndWall[i];
+        if (wSum > kEps) {
+            c0 /= s.pndSmall[i]; c1 /= s.pndSmall[i]; c2 /= s.pndSmall[i];
+        }
+        s.npcd[i * 3] = c0; s.npcd[i * 3 + 1] = c1; s.npcd[i * 3 + 2] = c2;
+        s.npcd2[i] = c0 * c0 + c1 * c1 + c2 * c2;
+        if (c0 * s.polyN[i * 3] + c1 * s.polyN[i * 3 + 1] + c2 * s.polyN[i * 3 + 2] < 0.0) s.devDotN[i] = 1;
+        else s.devDotN[i] = -1;
+    }
+}
+
+// calcPndDiffusiveTerm (src/MpsPndNeigh.cpp:334-489)
+void pndDiffusive(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        double Di = 0.0, DivV = 0.0, flagDi = 1.0;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double ux = s.vel[i * 3], uy = s.vel[i * 3 + 1], uz = s.vel[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        const double* MC0 = &s.cm1[i * 3]; const double* MC1 = &s.cm2[i * 3]; const double* MC2 = &s.cm3[i * 3];
+        double ni = s.pndi[i];
+        if (ni < kEps) continue;
+        double Pi = s.press[i];
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_l2 && (d2 < dm2 || pw) && j != i) {
+                double dst = sqrt(d2);
+                double wL = weight(dst, P.re_l, P.weight_type);
+                if (s.type[i] == P.fluid && s.type[j] == P.fluid) {
+                    double pgh = -2.0 * (s.RHO[i] * s.RHO[j] / (s.RHO[i] + s.RHO[j])) *
+                                 (P.gravity[0] * v0 + P.gravity[1] * v1 + P.gravity[2] * v2);
+                    double Pj = s.press[j];
+                    Di += P.time_step / s.RHO[i] * P.coeff_visc_multiphase * (Pj - Pi - pgh) * wL;
+                }
+                if (d2 < P.re_s2) {
+                    double vx = s.vel[j * 3] - ux, vy = s.vel[j * 3 + 1] - uy, vz = s.vel[j * 3 + 2] - uz;
+                    double wS = weight(dst, P.re_s, P.weight_type);
+                    if (!P.gradient_correction) {
+                        if (ni > kEps)
+                            DivV += (P.dim / P.pnd_small_zero) * (s.pndi[j] / ni) * (vx * v0 + vy * v1 + vz * v2) * wS / d2;
+                    } else {
+                        double c0 = (v0 * MC0[0] + v1 * MC0[1] + v2 * MC0[2]);
+                        double c1 = (v0 * MC1[0] + v1 * MC1[1] + v2 * MC1[2]);
+                        double c2 = (v0 * MC2[0] + v1 * MC2[1] + v2 * MC2[2]);
+                        DivV += (P.dim / P.pnd_small_zero) * (vx * c0 + vy * c1 + vz * c2) * wS / d2;
+                    }
+                }
+            }
+        });
+        s.aux1[i] = s.pndi[i] * (1.0 + P.time_step * (-DivV + Di * flagDi));
+        s.divU[i] = DivV;
+        s.diffTerm[i] = Di;
+    }
+}
+
+// calcWallSlipPndDiffusiveTerm / calcWallNoSlipPndDiffusiveTerm (src/MpsPndNeigh.cpp:492-744)
+void wallPndDiffusive(Sys& s) {
+    const mps_params& P = s.p;
+    const bool slip = P.slip_condition == MPS_FREE_SLIP;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        double ni = s.pndi[i];
+        if (!(s.type[i] == P.fluid && s.nearWall[i] && ni > kEps)) continue;
+        double DivV = 0.0;
+        const double* xi = &s.pos[i * 3];
+        const double* xm = &s.mirror[i * 3];
+        const double ux = s.vel[i * 3], uy = s.vel[i * 3 + 1], uz = s.vel[i * 3 + 2];
+        double n[3], R[9];
+        reflectionMod2(xi, xm, n, R);
+        double umx, umy, umz;
+        if (slip) {
+            umx = R[0] * ux + R[1] * uy + R[2] * uz;
+            umy = R[3] * ux + R[4] * uy + R[5] * uz;
+            umz = R[6] * ux + R[7] * uy + R[8] * uz;
+        } else {
+            double vw[3] = {0.0, 0.0, 0.0};
+            double dotnv = n[0] * vw[0] + n[1] * vw[1] + n[2] * vw[2];
+            double vt0 = ux - 2.0 * (vw[0] - dotnv * n[0]);
+            double vt1 = uy - 2.0 * (vw[1] - dotnv * n[1]);
+            double vt2 = uz - 2.0 * (vw[2] - dotnv * n[2]);
+            umx = -1.0 * vt0 + 0.0 * vt1 + 0.0 * vt2;
+            umy = 0.0 * vt0 + -1.0 * vt1 + 0.0 * vt2;
+            umz = 0.0 * vt0 + 0.0 * vt1 + -1.0 * vt2;
+        }
+        walk(s, xi[0], xi[1], xi[2], [&](int j) {
+            double v0 = s.pos[j * 3] - xi[0], v1 = s.pos[j * 3 + 1] - xi[1], v2 = s.pos[j * 3 + 2] - xi[2];
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - xm[0], w1 = s.pos[j * 3 + 1] - xm[1], w2 = s.pos[j * 3 + 2] - xm[2];
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2 && j != i) {
+                double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
+                if (slip) {
+                    double vx = s.vel[j * 3] - umx, vy = s.vel[j * 3 + 1] - umy, vz = s.vel[j * 3 + 2] - umz;
+                    DivV += (P.dim / P.pnd_small_zero) * (s.pndi[j] / ni) * (vx * w0 + vy * w1 + vz * w2) * wS / dm2;
+                } else {
+                    double vx = -(s.vel[j * 3] - umx), vy = -(s.vel[j * 3 + 1] - umy), vz = -(s.vel[j * 3 + 2] - umz);
+                    double m0 = (R[0] * w0 + R[1] * w1 + R[2] * w2);
+                    double m1 = (R[3] * w0 + R[4] * w1 + R[5] * w2);
+                    double m2 = (R[6] * w0 + R[7] * w1 + R[8] * w2);
+                    DivV += (P.dim / P.pnd_small_zero) * (s.pndi[j] / ni) * (vx * m0 + vy * m1 + vz * m2) * wS / dm2;
+                }
+            }
+        });
+        double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
+        double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (dm2 < P.re_s2) {
+            double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
+            if (slip) {
+                double vx = ux - umx, vy = uy - umy, vz = uz - umz;
+                DivV += (P.dim / P.pnd_small_zero) * (s.pndi[i] / ni) * (vx * w0 + vy * w1 + vz * w2) * wS / dm2;
+            } else {
+                double vx = -(ux - umx), vy = -(uy - umy), vz = -(uz - umz);
+                double m0 = (R[0] * w0 + R[1] * w1 + R[2] * w2);
+                double m1 = (R[3] * w0 + R[4] * w1 + R[5] * w2);
+                double m2 = (R[6] * w0 + R[7] * w1 + R[8] * w2);
+                DivV += (P.dim / P.pnd_small_zero) * (s.pndi[i] / ni) * (vx * m0 + vy * m1 + vz * m2) * wS / dm2;
+            }
+        }
+        s.aux1[i] += -s.pndi[i] * P.time_step * DivV;
+    }
+}
+
+// updatePnd (src/MpsPndNeigh.cpp:747-774)
+void updatePnd(Sys& s) {
+    for (int i = 0; i < s.N; i++) {
+        s.pndi[i] = s.aux1[i];
+        s.aux1[i] = 0.0;
+    }
+}
+
+// meanPndParticlesWallDummySurface (src/MpsPndNeigh.cpp:777-852)
+void meanPndWallDummySurface(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        s.aux1[i] = 0.0; s.aux2[i] = 0.0;
+        if (!(s.type[i] == P.wall || s.bc[i] == P.surface)) continue;
+        double up = 0, dn = 0;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) {
+                double wS = weight(sqrt(d2), P.re_s, P.weight_type);
+                up += s.pndi[j] * wS;
+                dn += wS;
+            }
+        });
+        s.aux1[i] = up; s.aux2[i] = dn;
+    }
+    for (int i = 0; i < s.N; i++) {
+        if (s.type[i] == P.wall || s.bc[i] == P.surface) {
+            if (s.aux2[i] < kEps) s.pndi[i] = s.aux1[i];
+            else s.pndi[i] = s.aux1[i] / (s.aux2[i]);
+        }
+        s.aux1[i] = 0.0; s.aux2[i] = 0.0;
+    }
+}
+
+// meanPnd (src/MpsPndNeigh.cpp:855-920)
+void meanPnd(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        double up = 0, dn = 0;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) {
+                double wS = weight(sqrt(d2), P.re_s, P.weight_type);
+                up += s.pndi[j] * wS;
+                dn += wS;
+            }
+        });
+        s.aux1[i] += up;
+        s.aux2[i] += dn;
+    }
+    for (int i = 0; i < s.N; i++) {
+        if (s.numNeigh[i] < 1) s.pndi[i] = s.aux1[i];
+        else s.pndi[i] = s.aux1[i] / s.aux2[i];
+        s.aux1[i] = 0.0; s.aux2[i] = 0.0;
+    }
+}
+
+// meanWallPnd (src/MpsPndNeigh.cpp:923-994)
+void meanWallPnd(Sys& s) {
+    const mps_params& P = s.p;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        s.aux1[i] = 0.0; s.aux2[i] = 0.0;
+        if (!s.nearWall[i]) continue;
+        double up = 0, dn = 0;
+        const double* xi = &s.pos[i * 3];
+        const double* xm = &s.mirror[i * 3];
+        walk(s, xi[0], xi[1], xi[2], [&](int j) {
+            double v0 = s.pos[j * 3] - xi[0], v1 = s.pos[j * 3 + 1] - xi[1], v2 = s.pos[j * 3 + 2] - xi[2];
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - xm[0], w1 = s.pos[j * 3 + 1] - xm[1], w2 = s.pos[j * 3 + 2] - xm[2];
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2 && j != i) {
+                double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
+                up += s.pndi[j] * wS;
+                dn += wS;
+            }
+        });
+        double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
+        double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (dm2 < P.re_s2) {
+            double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
+            up += s.pndi[i] * wS;
+            dn += wS;
+        }
+        s.aux1[i] = up; s.aux2[i] = dn;
+    }
+}
+
+// meanNeighFluidPnd (src/MpsPndNeigh.cpp:997-1050)
+void meanNeighFluidPnd(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        double up = 0, dn = 0;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i && s.type[j] == P.fluid) {
+                double wS = weight(sqrt(d2), P.re_s, P.weight_type);
+                up += s.pndki[j] * wS;
+                dn += wS;
+            }
+        });
+        if (dn > kEps) s.pndski[i] = up / dn;
+        else s.pndski[i] = up;
+    }
+}
+
+// ----------------------------------------------- MpsBoundaryCondition
+// updateParticleBC (src/MpsBoundaryCondition.cpp:66-400), PND / NEIGH / NPCD rules
+int updateBC(Sys& s) {
+    const mps_params& P = s.p;
+    if (P.free_surf_type == MPS_FS_PND_ARC) { s.err = "ARC free-surface rule is not supported"; return -1; }
+    for (int i = 0; i < s.N; i++) {
+        if (s.type[i] == P.dummy_wall) { s.bc[i] = P.other; continue; }
+        if (s.pndSmall[i] < P.beta_pnd) s.bc[i] = P.surface;
+        else s.bc[i] = P.inner;
+        if (P.free_surf_type == MPS_FS_PND_NEIGH) {
+            if (s.pndSmall[i] < P.beta_pnd && s.numNeigh[i] < P.beta_neigh) s.bc[i] = P.surface;
+            else s.bc[i] = P.inner;
+        } else if (P.free_surf_type == MPS_FS_PND_NPCD) {
+            if (s.bc[i] == P.surface)
+                if (s.numNeigh[i] > 4 && s.npcd2[i] < P.delta2) s.bc[i] = P.inner;
+        }
+    }
+    return 0;
+}
+
+// ------------------------------------------------------ MpsVectorMatrix
+// detMatrix / inverseMatrix (src/MpsVectorMatrix.cpp:96-165)
+int inverse3(double* M, int dim) {
+    double A[3][3], B[3][3];
+    for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) A[r][c] = M[r * 3 + c];
+    if (dim == 2) { A[0][2] = A[1][2] = A[2][0] = A[2][1] = 0.0; A[2][2] = 1.0; }
+    for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) B[r][c] = (r == c) ? 1.0 : 0.0;
+    double det = (A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0] + A[0][2] * A[1][0] * A[2][1]) -
+                 (A[0][2] * A[1][1] * A[2][0] + A[0][1] * A[1][0] * A[2][2] + A[0][0] * A[1][2] * A[2][1]);
+    if (det <= kEps) {
+        for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) M[r * 3 + c] = (r == c) ? 1.0 : 0.0;
+        return 0;
+    }
+    for (int k = 0; k < dim; k++)
+        for (int r = 0; r < dim; r++) {
+            if (r == k) continue;
+            double m = A[r][k] / A[k][k];
+            for (int c = 0; c < dim; c++) {
+                A[r][c] -= m * A[k][c];
+                B[r][c] -= m * B[k][c];
+            }
+        }
+    for (int r = 0; r < dim; r++) for (int c = 0; c < dim; c++) B[r][c] /= A[r][r];
+    for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) M[r * 3 + c] = B[r][c];
+    return 1;
+}
+
+// correctionMatrix (src/MpsVectorMatrix.cpp:20-93)
+void correctionMatrix(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) {
+                double wS = weight(sqrt(d2), P.re_s, P.weight_type);
+                double inv = 1.0 / d2;
+                M[0] += wS * v0 * v0 * inv; M[1] += wS * v0 * v1 * inv; M[2] += wS * v0 * v2 * inv;
+                M[3] += wS * v1 * v0 * inv; M[4] += wS * v1 * v1 * inv; M[5] += wS * v1 * v2 * inv;
+                M[6] += wS * v2 * v0 * inv; M[7] += wS * v2 * v1 * inv; M[8] += wS * v2 * v2 * inv;
+            }
+        });
+        for (int k = 0; k < 9; k++) M[k] *= -P.coeff_press_grad;
+        inverse3(M, (int)P.dim);
+        if (s.numNeigh[i] < 4) {
+            for (int k = 0; k < 9; k++) M[k] = (k % 4 == 0) ? 1.0 : 0.0;
+        }
+        for (int k = 0; k < 3; k++) {
+            s.cm1[i * 3 + k] = M[k]; s.cm2[i * 3 + k] = M[3 + k]; s.cm3[i * 3 + k] = M[6 + k];
+        }
+    }
+}
+
+// ----------------------------------------------------------- MpsPressure
+// calcPressEMPS / calcPressWCMPS (src/MpsPressure.cpp:62-163)
+void pressExplicit(Sys& s) {
+    const mps_params& P = s.p;
+#pragma omp parallel for
+    for (int i = 0; i < s.N; i++) {
+        double mi = (s.ptype[i] == 1) ? P.dns_fl1 : P.dns_fl2;
+        double pr = 0.0;
+        if (s.bc[i] == P.inner) {
+            if (P.mps_type == MPS_EXPLICIT) pr = (s.pndi[i] - P.pnd_small_zero) * P.coeff_press_emps * mi;
+            else pr = (mi * P.coeff_press_wcmps / P.gamma) * (pow(s.pndi[i] / P.pnd_small_zero, P.gamma) - 1);
+        }
+        if (pr < 0.0) pr = 0.0;
+        s.press[i] = pr;
+    }
+}
+
+// calcVelDivergence (src/MpsPressure.cpp:479-551)
+void velDivergence(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        s.divU[i] = 0.0;
+        if (s.type[i] != P.fluid) continue;
+        double DivV = 0.0, ni = s.pndi[i];
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double ux = s.vel[i * 3], uy = s.vel[i * 3 + 1], uz = s.vel[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        const double* MC0 = &s.cm1[i * 3]; const double* MC1 = &s.cm2[i * 3]; const double* MC2 = &s.cm3[i * 3];
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) {
+                double vx = s.vel[j * 3] - ux, vy = s.vel[j * 3 + 1] - uy, vz = s.vel[j * 3 + 2] - uz;
+                double wS = weight(sqrt(d2), P.re_s, P.weight_type);
+                if (!P.gradient_correction) {
+                    if (ni > kEps)
+                        DivV += (P.dim / P.pnd_small_zero) * (s.pndi[j] / ni) * (vx * v0 + vy * v1 + vz * v2) * wS / d2;
+                } else {
+                    double c0 = (v0 * MC0[0] + v1 * MC0[1] + v2 * MC0[2]);
+                    double c1 = (v0 * MC1[0] + v1 * MC1[1] + v2 * MC1[2]);
+                    double c2 = (v0 * MC2[0] + v1 * MC2[1] + v2 * MC2[2]);
+                    DivV += (P.dim / P.pnd_small_zero) * (vx * c0 + vy * c1 + vz * c2) * wS / d2;
+                }
+            }
+        });
+        s.divU[i] = DivV;
+    }
+}
+
+// PPE assembly: solvePressurePoissonPnd / ...PndDivU (src/MpsPressure.cpp:169-421).
+// Rows are stored in triplet order (neighbours in walk order, then the
+// diagonal); identity rows for surface / other particles.
+void assemblePPE(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+    const bool divu = P.mps_type == MPS_IMPLICIT_PND_DIVU;
+    s.ppeB.assign(s.N, 0.0);
+    s.rowPtr.assign(s.N + 1, 0);
+    s.colIdx.clear();
+    s.vals.clear();
+    for (int i = 0; i < s.N; i++) {
+        s.rowPtr[i] = (int)s.colIdx.size();
+        if (s.bc[i] == P.other || s.bc[i] == P.surface) {
+            s.colIdx.push_back(i); s.vals.push_back(1.0);
+            continue;
+        }
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        double ni = s.pndSmall[i];
+        double sum = 0.0;
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_l2 && (d2 < dm2 || pw) && j != i) {
+                double wL = weight(sqrt(d2), P.re_l, P.weight_type);
+                double mij = wL * P.coeff_ppe;
+                if (s.type[j] == P.dummy_wall) {
+                    double pgh = s.RHO[j] * (v0 * P.gravity[0] + v1 * P.gravity[1] + v2 * P.gravity[2]) * wL;
+                    s.ppeB[i] -= P.coeff_ppe * pgh;
+                } else {
+                    sum -= mij;
+                    if (s.bc[j] == P.inner) { s.colIdx.push_back(j); s.vals.push_back(mij); }
+                }
+            }
+        });
+        double density = (s.ptype[i] == 1) ? P.dns_fl1 : P.dns_fl2;
+        sum -= P.alpha_compressibility * density / (P.time_step * P.time_step);
+        s.colIdx.push_back(i); s.vals.push_back(sum);
+        if (!divu)
+            s.ppeB[i] += -P.coeff_ppe_source * density * (ni - P.pnd_small_zero);
+        else
+            s.ppeB[i] += -P.coeff_ppe_source * density * (s.pndki[i] - P.pnd_small_zero) +
+                         (1.0 - P.relax_pnd) * density * s.divU[i] / P.time_step;
+    }
+    s.rowPtr[s.N] = (int)s.colIdx.size();
+}
+
+// Restated Eigen ConjugateGradient with DiagonalPreconditioner and a guess.
+// The matrix used for iteration is A^T when ppe_transpose (Eigen applies the
+// row-major transpose wrapper for UpLo = Lower|Upper on a column-major
+// matrix), else A.  Products accumulate in ascending column index of the
+// iterated matrix, as a compressed sparse traversal does.
+void solveCG(Sys& s) {
+    const mps_params& P = s.p;
+    const int n = s.N;
+    // build the iterated matrix in row form with ascending columns
+    std::vector<int> rp(n + 1, 0), ci;
+    std::vector<double> va;
+    std::vector<double> diag(n, 0.0);
+    {
+        std::vector<std::vector<std::pair<int, double>>> rows(n);
+        for (int r = 0; r < n; r++)
+            for (int k = s.rowPtr[r]; k < s.rowPtr[r + 1]; k++) {
+                int c = s.colIdx[k];
+                double v = s.vals[k];
+                if (c == r) diag[r] = v;
+                if (P.ppe_transpose) rows[c].push_back({r, v});
+                else rows[r].push_back({c, v});
+            }
+        for (int r = 0; r < n; r++) {
+            std::stable_sort(rows[r].begin(), rows[r].end(),
+                             [](const std::pair<int, double>& a, const std::pair<int, double>& b) { return a.first < b.first; });
+            rp[r + 1] = rp[r] + (int)rows[r].size();
+            for (auto& e : rows[r]) { ci.push_back(e.first); va.push_back(e.second); }
+        }
+    }
+    auto spmv = [&](const std::vector<double>& x, std::vector<double>& y) {
+#pragma omp parallel for
+        for (int r = 0; r < n; r++) {
+            double acc = 0.0;
+            for (int k = rp[r]; k < rp[r + 1]; k++) acc += va[k] * x[ci[k]];
+            y[r] = acc;
+        }
+    };
+    auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {
+        double acc = 0.0;
+        for (int r = 0; r < n; r++) acc += a[r] * b[r];
+        return acc;
+    };
+    std::vector<double> invd(n);
+    for (int r = 0; r < n; r++) invd[r] = (diag[r] != 0.0) ? 1.0 / diag[r] : 1.0;
+    std::vector<double>& x = s.ppeX;
+    if ((int)x.size() < n) x.resize(n, 0.0);
+    const std::vector<double>& b = s.ppeB;
+    double tol = P.solver_tol > 0 ? P.solver_tol : DBL_EPSILON;
+    int maxIt = P.solver_max_iter > 0 ? P.solver_max_iter : 2 * n;
+    std::vector<double> r(n), p(n), z(n), tmp(n);
+    spmv(x, tmp);
+    for (int k = 0; k < n; k++) r[k] = b[k] - tmp[k];
+    double rhsN2 = dot(b, b);
+    if (rhsN2 == 0) {
+        for (int k = 0; k < n; k++) x[k] = 0.0;
+        s.solverIter = 0; s.solverErr = 0.0;
+        return;
+    }
+    double thr = std::max(tol * tol * rhsN2, DBL_MIN);
+    double rN2 = dot(r, r);
+    if (rN2 < thr) {
+        s.solverIter = 0; s.solverErr = sqrt(rN2 / rhsN2);
+        return;
+    }
+    for (int k = 0; k < n; k++) p[k] = invd[k] * r[k];
+    double absNew = dot(r, p);
+    int it = 0;
+    while (it < maxIt) {
+        spmv(p, tmp);
+        double alpha = absNew / dot(p, tmp);
+        for (int k = 0; k < n; k++) x[k] += alpha * p[k];
+        for (int k = 0; k < n; k++) r[k] -= alpha * tmp[k];
+        rN2 = dot(r, r);
+        if (rN2 < thr) break;
+        for (int k = 0; k < n; k++) z[k] = invd[k] * r[k];
+        double absOld = absNew;
+        absNew = dot(r, z);
+        double beta = absNew / absOld;
+        for (int k = 0; k < n; k++) p[k] = z[k] + beta * p[k];
+        it++;
+    }
+    s.solverErr = sqrt(rN2 / rhsN2);
+    s.solverIter = it;
+}
+
+int calcPress(Sys& s) {
+    const mps_params& P = s.p;
+    if (P.mps_type == MPS_EXPLICIT || P.mps_type == MPS_WEAKLY) {
+        pressExplicit(s);
+        return 0;
+    }
+    if (P.mps_type == MPS_IMPLICIT_PND_DIVU) {
+        velDivergence(s);
+        if (P.wall_type == MPS_WALL_POLYGON) { s.err = "PND+DivU with polygon walls is not supported"; return -1; }
+    }
+    if (P.solver_type != MPS_SOLVER_CG) { s.err = "only the CG solver is supported"; return -1; }
+    assemblePPE(s);
+    solveCG(s);
+    for (int i = 0; i < s.N; i++) s.press[i] = s.ppeX[i];
+    for (int i = 0; i < s.N; i++) if (s.press[i] < 0) s.press[i] = 0.0;
+    return 0;
+}
+
+// extrapolatePressParticlesWallDummy (src/MpsPressure.cpp:792-846)
+void extrapolateWallDummy(Sys& s) {
+    const mps_params& P = s.p;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        if (!(s.type[i] == P.dummy_wall || (P.mps_type == MPS_WEAKLY && s.type[i] == P.wall))) continue;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        double ni = 0.0, pr = 0.0;
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            if (d2 < P.re_s2 && j != i && s.type[j] == P.fluid) {
+                double wS = weight(sqrt(d2), P.re_s, P.weight_type);
+                ni += wS;
+                pr += (s.press[j] - s.RHO[j] * (P.gravity[0] * v0 + P.gravity[1] * v1 + P.gravity[2] * v2)) * wS;
+            }
+        });
+        if (pr < 0.0) continue;
+        if (ni > 0) s.press[i] = pr / ni;
+        else s.press[i] = pr;
+    }
+}
+
+// calcPressGradient (src/MpsPressure.cpp:1217-1338)
+void pressGradient(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+    const int gt = P.gradient_type;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        double ax = 0, ay = 0, az = 0;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        double pmin = s.press[i], Pi = s.press[i], ni = s.pndi[i];
+        const double* MC0 = &s.cm1[i * 3]; const double* MC1 = &s.cm2[i * 3]; const double* MC2 = &s.cm3[i * 3];
+        auto ok = [&](int j, double& v0, double& v1, double& v2, double& d2) {
+            v0 = s.pos[j * 3] - xi; v1 = s.pos[j * 3 + 1] - yi; v2 = s.pos[j * 3 + 2] - zi;
+            d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            return d2 < P.re_s2 && (d2 < dm2 || pw) && j != i;
+        };
+        if (gt == 0 || gt == 2)
+            walk(s, xi, yi, zi, [&](int j) {
+                double v0, v1, v2, d2;
+                if (ok(j, v0, v1, v2, d2))
+                    if (pmin > s.press[j]) pmin = s.press[j];
+            });
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0, v1, v2, d2;
+            if (!ok(j, v0, v1, v2, d2)) return;
+            double wS = weight(sqrt(d2), P.re_s, P.weight_type);
+            if (gt == 0) wS *= (s.press[j] - pmin) / d2;
+            else if (gt == 1) wS *= (s.press[j] + Pi) / d2;
+            else if (gt == 2) wS *= (s.press[j] + Pi - 2.0 * pmin) / d2;
+            else if (gt == 3) {
+                double nj = s.pndi[j];
+                if (ni > kEps && nj > kEps) wS *= (ni * s.press[j] / nj + nj * Pi / ni) / d2;
+            }
+            if (!P.gradient_correction) {
+                ax += v0 * wS; ay += v1 * wS; az += v2 * wS;
+            } else {
+                ax += (v0 * MC0[0] + v1 * MC0[1] + v2 * MC0[2]) * wS;
+                ay += (v0 * MC1[0] + v1 * MC1[1] + v2 * MC1[2]) * wS;
+                az += (v0 * MC2[0] + v1 * MC2[1] + v2 * MC2[2]) * wS;
+            }
+        });
+        s.acc[i * 3] = P.relax_press * ax * P.coeff_press_grad / s.RHO[i];
+        s.acc[i * 3 + 1] = P.relax_press * ay * P.coeff_press_grad / s.RHO[i];
+        s.acc[i * 3 + 2] = P.relax_press * az * P.coeff_press_grad / s.RHO[i];
+    }
+}
+
+// repulsive forces (src/MpsPressure.cpp:1554-1611)
+inline void repulsive(const Sys& s, int i, const double* n, double ns, double vmax2, double* f) {
+    const mps_params& P = s.p;
+    double w;
+    if (P.repulsive_force_type == MPS_REP_HARADA) {
+        w = s.RHO[i] / (P.time_step * P.time_step) * (P.re_repulsive_force - ns);
+    } else if (P.repulsive_force_type == MPS_REP_MITSUME) {
+        w = P.rep_coef_mitsume * weight(ns, P.re_repulsive_force, P.weight_type);
+    } else if (P.repulsive_force_type == MPS_REP_LENNARD_JONES) {
+        double R1 = (P.re_repulsive_force / ns) * (P.re_repulsive_force / ns);
+        double R2 = R1 * R1;
+        w = (P.rep_coef_lennard_jones * vmax2 / ns) * (R2 - R1) * s.RHO[i];
+    } else {
+        double W1 = (1.0 + 3.0 * 0.5 * ns / (P.re_repulsive_force));
+        double W2 = (1.0 - ns / (P.re_repulsive_force)) * (1.0 - ns / (P.re_repulsive_force)) * (1.0 - ns / (P.re_repulsive_force));
+        w = (P.rep_coef_monaghan_kajtar * vmax2 / (ns - 0.0 * P.part_dist)) * (1.0 / 8.0) * (W1) * (W2) * s.RHO[i];
+    }
+    f[0] = -w * n[0]; f[1] = -w * n[1]; f[2] = -w * n[2];
+}
+
+// calcWallPressGradient (src/MpsPressure.cpp:1341-1550)
+void wallPressGradient(Sys& s) {
+    const mps_params& P = s.p;
+    const double vol = pow(P.part_dist, P.dim);
+    double vm = std::min(s.velMax, P.expect_max_velocity);
+    double vmax2 = vm * vm;
+    const int gt = P.gradient_type;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        if (!(s.type[i] == P.fluid && s.nearWall[i])) continue;
+        double ax = 0, ay = 0, az = 0;
+        const double* xi = &s.pos[i * 3];
+        const double* xm = &s.mirror[i * 3];
+        double pmin = s.press[i], Pi = s.press[i], ni = s.pndi[i];
+        double n[3], R[9], ns;
+        reflection(xi, xm, n, R, &ns);
+        if (gt == 0 || gt == 2)
+            walk(s, xi[0], xi[1], xi[2], [&](int j) {
+                double v0 = s.pos[j * 3] - xi[0], v1 = s.pos[j * 3 + 1] - xi[1], v2 = s.pos[j * 3 + 2] - xi[2];
+                double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+                double w0 = s.pos[j * 3] - xm[0], w1 = s.pos[j * 3 + 1] - xm[1], w2 = s.pos[j * 3 + 2] - xm[2];
+                double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+                if (d2 < P.re_s2 && d2 < dm2 && j != i)
+                    if (pmin > s.press[j]) pmin = s.press[j];
+            });
+        auto term = [&](double Pj, double nj, double w0, double w1, double w2, double dm2) {
+            double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
+            if (gt == 0) wS *= (Pj - pmin) / dm2;
+            else if (gt == 1) wS *= (Pj + Pi) / dm2;
+            else if (gt == 2) wS *= (Pj + Pi - 2.0 * pmin) / dm2;
+            else if (gt == 3) {
+                if (ni > kEps && nj > kEps) wS *= (ni * Pj / nj + nj * Pi / ni) / dm2;
+            }
+            ax += w0 * wS; ay += w1 * wS; az += w2 * wS;
+        };
+        walk(s, xi[0], xi[1], xi[2], [&](int j) {
+            double v0 = s.pos[j * 3] - xi[0], v1 = s.pos[j * 3 + 1] - xi[1], v2 = s.pos[j * 3 + 2] - xi[2];
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - xm[0], w1 = s.pos[j * 3 + 1] - xm[1], w2 = s.pos[j * 3 + 2] - xm[2];
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2 && j != i) term(s.press[j], s.pndi[j], w0, w1, w2, dm2);
+        });
+        {
+            double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (dm2 < P.re_s2) term(Pi, s.pndi[i], w0, w1, w2, dm2);
+        }
+        double f[3] = {0.0, 0.0, 0.0};
+        if (ns < P.re_repulsive_force && ns > kEps) repulsive(s, i, n, ns, vmax2, f);
+        double g0 = P.relax_press * (R[0] * ax + R[1] * ay + R[2] * az) * P.coeff_press_grad - f[0];
+        double g1 = P.relax_press * (R[3] * ax + R[4] * ay + R[5] * az) * P.coeff_press_grad - f[1];
+        double g2 = P.relax_press * (R[6] * ax + R[7] * ay + R[8] * az) * P.coeff_press_grad - f[2];
+        s.acc[i * 3] += g0 / s.RHO[i];
+        s.acc[i * 3 + 1] += g1 / s.RHO[i];
+        s.acc[i * 3 + 2] += g2 / s.RHO[i];
+        s.forceWall[i * 3] += -g0 * vol;
+        s.forceWall[i * 3 + 1] += -g1 * vol;
+        s.forceWall[i * 3 + 2] += -g2 * vol;
+    }
+}
+
+// ----------------------------------------------------------- MpsShifting
+// calcShifting (src/MpsShifting.cpp:23-104).  sequential = in-place update in
+// ascending i (the reference at one thread); Jacobi = all reads see old values.
+void shifting1(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+    const bool seq = P.shift_semantics == MPS_SHIFT_SEQUENTIAL;
+    std::vector<double> out(s.vel);
+    std::vector<double>& velR = s.vel;
+    auto body = [&](int i, std::vector<double>& dst) {
+        if (s.type[i] != P.fluid) return;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double ux = velR[i * 3], uy = velR[i * 3 + 1], uz = velR[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        double dx = 0, dy = 0, dz = 0;
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) {
+                double dw = delWeight(sqrt(d2), P.re_s, P.weight_type);
+                dx += dw * (velR[j * 3] - ux);
+                dy += dw * (velR[j * 3 + 1] - uy);
+                dz += dw * (velR[j * 3 + 2] - uz);
+            }
+        });
+        if (s.bc[i] == P.inner) {
+            dst[i * 3] -= P.coeff_shifting1 * dx;
+            dst[i * 3 + 1] -= P.coeff_shifting1 * dy;
+            dst[i * 3 + 2] -= P.coeff_shifting1 * dz;
+        }
+    };
+    if (seq) {
+        for (int i = 0; i < s.N; i++) body(i, s.vel);
+    } else {
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < s.N; i++) body(i, out);
+        s.vel.swap(out);
+    }
+}
+
+// calcConcAndConcGradient + calcWallConcAndConcGradient (src/MpsShifting.cpp:432-692)
+void shifting2(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+    const bool seq = P.shift_semantics == MPS_SHIFT_SEQUENTIAL;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        double c = 0.0;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) c += weight(sqrt(d2), P.re_s, P.weight_type);
+        });
+        c += s.pndWall[i];
+        s.conc[i] = c / P.pnd_small_zero;
+    }
+    std::vector<double> newPos(s.pos);
+    auto body = [&](int i, std::vector<double>& dst) {
+        if (s.type[i] != P.fluid) return;
+        double g0 = 0, g1 = 0, g2 = 0;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        double ci = s.conc[i];
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) {
+                double wS = weight(sqrt(d2), P.re_s, P.weight_type);
+                g0 += (ci + s.conc[j]) * v0 * wS / d2;
+                g1 += (ci + s.conc[j]) * v1 * wS / d2;
+                g2 += (ci + s.conc[j]) * v2 * wS / d2;
+            }
+        });
+        g0 *= -P.coeff_press_grad; g1 *= -P.coeff_press_grad; g2 *= -P.coeff_press_grad;
+        s.gradC[i * 3] = g0; s.gradC[i * 3 + 1] = g1; s.gradC[i * 3 + 2] = g2;
+        if (s.bc[i] == P.inner) {
+            dst[i * 3] -= P.coeff_shifting2 * g0;
+            dst[i * 3 + 1] -= P.coeff_shifting2 * g1;
+            dst[i * 3 + 2] -= P.coeff_shifting2 * g2;
+        }
+    };
+    if (seq) {
+        for (int i = 0; i < s.N; i++) body(i, s.pos);
+    } else {
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < s.N; i++) body(i, newPos);
+        s.pos.swap(newPos);
+    }
+    if (P.wall_type != MPS_WALL_POLYGON) return;
+    newPos = s.pos;
+    auto wbody = [&](int i, std::vector<double>& dst) {
+        if (!(s.type[i] == P.fluid && s.nearWall[i])) return;
+        const double* xi = &s.pos[i * 3];
+        const double* xm = &s.mirror[i * 3];
+        double d0 = 0, d1 = 0, dd2 = 0, ci = s.conc[i];
+        double n[3], R[9];
+        reflection(xi, xm, n, R, nullptr);
+        double x0 = xi[0], x1 = xi[1], x2 = xi[2];
+        walk(s, x0, x1, x2, [&](int j) {
+            double v0 = s.pos[j * 3] - x0, v1 = s.pos[j * 3 + 1] - x1, v2 = s.pos[j * 3 + 2] - x2;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = s.pos[j * 3] - xm[0], w1 = s.pos[j * 3 + 1] - xm[1], w2 = s.pos[j * 3 + 2] - xm[2];
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2 && j != i) {
+                double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
+                d0 += (ci + s.conc[j]) * w0 * wS / dm2;
+                d1 += (ci + s.conc[j]) * w1 * wS / dm2;
+                dd2 += (ci + s.conc[j]) * w2 * wS / dm2;
+            }
+        });
+        double w0 = x0 - xm[0], w1 = x1 - xm[1], w2 = x2 - xm[2];
+        double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (dm2 < P.re_s2) {
+            double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
+            d0 += (ci + ci) * w0 * wS / dm2;
+            d1 += (ci + ci) * w1 * wS / dm2;
+            dd2 += (ci + ci) * w2 * wS / dm2;
+        }
+        double gx = -(R[0] * d0 + R[1] * d1 + R[2] * dd2) * P.coeff_press_grad;
+        double gy = -(R[3] * d0 + R[4] * d1 + R[5] * dd2) * P.coeff_press_grad;
+        double gz = -(R[6] * d0 + R[7] * d1 + R[8] * dd2) * P.coeff_press_grad;
+        s.gradC[i * 3] += gx; s.gradC[i * 3 + 1] += gy; s.gradC[i * 3 + 2] += gz;
+        if (s.bc[i] == P.inner) {
+            dst[i * 3] -= P.coeff_shifting2 * gx;
+            dst[i * 3 + 1] -= P.coeff_shifting2 * gy;
+            dst[i * 3 + 2] -= P.coeff_shifting2 * gz;
+        }
+    };
+    if (seq) {
+        for (int i = 0; i < s.N; i++) wbody(i, s.pos);
+    } else {
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < s.N; i++) wbody(i, newPos);
+        s.pos.swap(newPos);
+    }
+}
+
+// ------------------------------------------------------------ driver
+void wallContributions(Sys& s) {
+    closestPoint(s);
+    updateNearMesh(s);
+    wallNPCD(s);
+}
+
+// main() initial pipeline (src/main.cpp:105-180, initMesh :471-556)
+int initSystem(Sys& s) {
+    const mps_params& P = s.p;
+    updateParticlesID(s);
+    checkOutDomain(s);
+    if (P.wall_type == MPS_WALL_POLYGON) {
+        if (!s.hasMesh) { s.err = "polygon walls need a mesh"; return -1; }
+        initWijnNeigh(s);
+        wallContributions(s);
+    }
+    initialPnd(s);
+    if (P.wall_type == MPS_WALL_POLYGON) meanWallPnd(s);
+    meanPnd(s);
+    meanNeighFluidPnd(s);
+    if (updateBC(s)) return -1;
+    if (calcPress(s)) return -1;
+    return 0;
+}
+
+// mainLoopOfSimulation body (src/main.cpp:252-467), Newtonian fluid
+int stepSystem(Sys& s) {
+    const mps_params& P = s.p;
+    if (P.relax_press < 1.0) { s.err = "relax_fact < 1 (prediction gradient) is not supported"; return -1; }
+    updateParticlesID(s);
+    calcViscosity(s);
+    addGravity(s);
+    prediction(s);
+    checkOutDomain(s);
+    if (P.collision_type == MPS_COLLISION_PC) collisionPC(s);
+    else collisionDPC(s);
+    if (P.wall_type == MPS_WALL_POLYGON) wallContributions(s);
+    if (P.gradient_correction) correctionMatrix(s);
+    pndNeighNPCD(s);
+    if (P.pnd_type == MPS_PND_DIFFUSIVE) {
+        pndDiffusive(s);
+        if (P.wall_type == MPS_WALL_POLYGON) wallPndDiffusive(s);
+        updatePnd(s);
+        meanPndWallDummySurface(s);
+    } else if (P.pnd_type == MPS_PND_MEAN_SUM_WIJ) {
+        if (P.wall_type == MPS_WALL_POLYGON) meanWallPnd(s);
+        meanPnd(s);
+    }
+    if (updateBC(s)) return -1;
+    if (calcPress(s)) return -1;
+    if (P.wall_type == MPS_WALL_PARTICLE) extrapolateWallDummy(s);
+    if (P.gradient_correction) correctionMatrix(s);
+    pressGradient(s);
+    if (P.wall_type == MPS_WALL_POLYGON) wallPressGradient(s);
+    if (P.wall_type == MPS_WALL_POLYGON) calcWallViscosity(s);
+    correction(s);
+    checkOutDomain(s);
+    if (P.shifting_type == 1) shifting1(s);
+    else if (P.shifting_type == 2) shifting2(s);
+    if (P.wall_type == MPS_WALL_PARTICLE) wallDummyVelocity(s);
+    checkOutDomain(s);
+    s.numIter++;
+    s.timeCurrent += P.time_step;
+    return 0;
+}
+
+struct Field {
+    const char* name;
+    int kind;   // 0 double, 1 int, 2 char
+    int comps;
+};
+
+void* fieldPtr(Sys& s, const std::string& f, int& kind, int& comps) {
+    comps = 1; kind = 0;
+#define D1(nm, v) if (f == nm) { kind = 0; comps = 1; return v.data(); }
+#define D3(nm, v) if (f == nm) { kind = 0; comps = 3; return v.data(); }
+#define I1(nm, v) if (f == nm) { kind = 1; comps = 1; return v.data(); }
+    D3("pos", s.pos) D3("vel", s.vel) D3("acc", s.acc) D3("acc_star", s.accStar)
+    D3("npcd", s.npcd) D3("grad_conc", s.gradC) D3("cm1", s.cm1) D3("cm2", s.cm2) D3("cm3", s.cm3)
+    D3("wall_pos", s.wallPos) D3("mirror", s.mirror) D3("polygon_normal", s.polyN) D3("force_wall", s.forceWall)
+    D3("dvel", s.dvel)
+    D1("press", s.press) D1("press_avg", s.pressAvg) D1("pndi", s.pndi) D1("pndki", s.pndki) D1("pndski", s.pndski)
+    D1("pnd_small", s.pndSmall) D1("npcd2", s.npcd2) D1("conc", s.conc) D1("div_u", s.divU)
+    D1("diffusive", s.diffTerm) D1("pnd_wall", s.pndWall) D1("riw2", s.riw2) D1("rho", s.RHO) D1("meu", s.MEU)
+    D1("ppe_x", s.ppeX) D1("ppe_b", s.ppeB) D1("dev_dot_n", s.devDotN)
+    I1("type", s.type) I1("bc", s.bc) I1("num_neigh", s.numNeigh) I1("num_neigh_wall", s.numNeighWall)
+    I1("ptype", s.ptype) I1("near_mesh_type", s.nearMeshType)
+    if (f == "near_wall") { kind = 2; comps = 1; return s.nearWall.data(); }
+#undef D1
+#undef D3
+#undef I1
+    return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* orc_create(const mps_params* p, int n, const int* type, const double* pos, const double* vel,
+                 const double* press, const double* press_avg) {
+    Sys* s = new Sys();
+    s->p = *p;
+    s->N = s->N0 = n;
+    auto z = [&](std::vector<double>& v, int k, double val = 0.0) { v.assign((size_t)n * k, val); };
+    s->type.assign(type, type + n);
+    s->pos.assign(pos, pos + 3 * n);
+    s->vel.assign(vel, vel + 3 * n);
+    s->press.assign(press, press + n);
+    s->pressAvg.assign(press_avg, press_avg + n);
+    s->bc.assign(n, 0); s->numNeigh.assign(n, 0); s->numNeighWall.assign(n, 0); s->elementID.assign(n, 0);
+    s->nearWall.assign(n, 0); s->nearMeshType.assign(n, 0);
+    z(s->pndi, 1); z(s->pndki, 1); z(s->pndski, 1); z(s->pndSmall, 1); z(s->npcd2, 1); z(s->conc, 1);
+    z(s->divU, 1); z(s->diffTerm, 1); z(s->pndWall, 1); z(s->devDotN, 1); z(s->nNeighSurf, 1);
+    z(s->aux1, 1); z(s->aux2, 1);
+    z(s->riw2, 1, 10e8 * p->part_dist);
+    z(s->acc, 3); z(s->accStar, 3); z(s->npcd, 3); z(s->gradC, 3); z(s->cm1, 3); z(s->cm2, 3); z(s->cm3, 3);
+    z(s->normal, 3); z(s->dvel, 3); z(s->wallPos, 3); z(s->mirror, 3); z(s->wpf1, 3); z(s->wpf2, 3);
+    z(s->polyN, 3); z(s->forceWall, 3);
+    s->ptype.assign(n, 1);
+    s->RHO.assign(n, p->dns_fl1);
+    s->MEU.assign(n, p->knm_vs1 * p->dns_fl1);
+    s->ppeX.assign(n, 0.0);
+    s->ppeB.assign(n, 0.0);
+    if (deriveParams(*s) != 0) {
+        fprintf(stderr, "oracle: %s\n", s->err.c_str());
+        delete s;
+        return nullptr;
+    }
+    // RHO / MEU use DNS_FL1 of the derived parameter block (readMpsParticleFile :833-837)
+    s->RHO.assign(n, s->p.dns_fl1);
+    s->MEU.assign(n, s->p.knm_vs1 * s->p.dns_fl1);
+    s->first.assign(s->p.num_buckets_xyz, -1);
+    s->last.assign(s->p.num_buckets_xyz, -1);
+    s->next.assign(n, -1);
+    return s;
+}
+
+void orc_destroy(void* h) { delete (Sys*)h; }
+
+void orc_get_params(void* h, mps_params* out) { *out = ((Sys*)h)->p; }
+
+int orc_num_particles(void* h) { return ((Sys*)h)->N; }
+
+const char* orc_last_error(void* h) { return ((Sys*)h)->err.c_str(); }
+
+void orc_set_threads(void* h, int t) {
+    ((Sys*)h)->nThreads = t;
+#ifdef _OPENMP
+    if (t > 0) omp_set_num_threads(t);
+#endif
+}
+
+int orc_set_mesh(void* h, int nv, const double* V, int nf, const int* F, const double* Nrm) {
+    Sys* s = (Sys*)h;
+    s->mesh.nv = nv; s->mesh.nf = nf;
+    s->mesh.V.assign(V, V + 3 * nv);
+    s->mesh.F.assign(F, F + 3 * nf);
+    s->mesh.Nrm.assign(Nrm, Nrm + 3 * nf);
+    s->hasMesh = true;
+    initWijnNeigh(*s);
+    return 0;
+}
+
+int orc_get_wall_tables(void* h, double* xpnd, double* xneigh, double* zij, int* nneigh) {
+    Sys* s = (Sys*)h;
+    if (s->mesh.xPND.empty()) initWijnNeigh(*s);
+    for (size_t k = 0; k < s->mesh.xPND.size(); k++) {
+        xpnd[k] = s->mesh.xPND[k]; xneigh[k] = s->mesh.xNeigh[k];
+        zij[k] = s->mesh.Zij[k]; nneigh[k] = s->mesh.nNeighWall[k];
+    }
+    return (int)s->mesh.xPND.size();
+}
+
+int orc_field(void* h, const char* name, void* buf, int write) {
+    Sys* s = (Sys*)h;
+    int kind, comps;
+    void* p = fieldPtr(*s, name, kind, comps);
+    if (!p) { s->err = std::string("unknown field ") + name; return -1; }
+    size_t n = (size_t)s->N0 * comps;
+    size_t bytes = n * (kind == 0 ? 8 : kind == 1 ? 4 : 1);
+    if (write) memcpy(p, buf, bytes); else memcpy(buf, p, bytes);
+    return 0;
+}
+
+int orc_scalar(void* h, const char* name, double* out) {
+    Sys* s = (Sys*)h;
+    std::string f(name);
+    if (f == "vel_max") *out = s->velMax;
+    else if (f == "cfl") *out = s->cflCurrent;
+    else if (f == "time") *out = s->timeCurrent;
+    else if (f == "solver_iter") *out = s->solverIter;
+    else if (f == "solver_err") *out = s->solverErr;
+    else if (f == "num_particles") *out = s->N;
+    else return -1;
+    return 0;
+}
+
+int orc_set_scalar(void* h, const char* name, double v) {
+    Sys* s = (Sys*)h;
+    std::string f(name);
+    if (f == "vel_max") s->velMax = v;
+    else return -1;
+    return 0;
+}
+
+// PPE rows in triplet order: returns nnz; arrays may be null to query size.
+int orc_ppe_matrix(void* h, int* row_ptr, int* cols, double* vals) {
+    Sys* s = (Sys*)h;
+    int nnz = (int)s->colIdx.size();
+    if (row_ptr) for (int k = 0; k <= s->N; k++) row_ptr[k] = s->rowPtr[k];
+    if (cols) for (int k = 0; k < nnz; k++) cols[k] = s->colIdx[k];
+    if (vals) for (int k = 0; k < nnz; k++) vals[k] = s->vals[k];
+    return nnz;
+}
+
+// Bucket list as (first, next) arrays.
+int orc_buckets(void* h, int* first, int* next) {
+    Sys* s = (Sys*)h;
+    if (first) memcpy(first, s->first.data(), sizeof(int) * s->first.size());
+    if (next) memcpy(next, s->next.data(), sizeof(int) * s->next.size());
+    return (int)s->first.size();
+}
+
+// Neighbour list of particle i in walk order for radius^2 r2 with the
+// mirror test (mode 0: d2<r2 && (d2<dm2 || particle walls); mode 1: none).
+int orc_neighbors(void* h, int i, double r2, int mode, int* out, int cap) {
+    Sys* s = (Sys*)h;
+    const bool pw = partWall(*s);
+    int cnt = 0;
+    const double xi = s->pos[i * 3], yi = s->pos[i * 3 + 1], zi = s->pos[i * 3 + 2];
+    const double mx = s->mirror[i * 3], my = s->mirror[i * 3 + 1], mz = s->mirror[i * 3 + 2];
+    walk(*s, xi, yi, zi, [&](int j) {
+        if (j == i) return;
+        double v0 = s->pos[j * 3] - xi, v1 = s->pos[j * 3 + 1] - yi, v2 = s->pos[j * 3 + 2] - zi;
+        double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        double w0 = s->pos[j * 3] - mx, w1 = s->pos[j * 3 + 1] - my, w2 = s->pos[j * 3 + 2] - mz;
+        double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        bool ok = d2 < r2 && (mode == 1 || d2 < dm2 || pw);
+        if (ok) { if (cnt < cap) out[cnt] = j; cnt++; }
+    });
+    return cnt;
+}
+
+int orc_call(void* h, const char* op) {
+    Sys* s = (Sys*)h;
+    std::string o(op);
+    if (o == "update_buckets") updateParticlesID(*s);
+    else if (o == "viscosity") calcViscosity(*s);
+    else if (o == "add_gravity") addGravity(*s);
+    else if (o == "prediction") prediction(*s);
+    else if (o == "check_out_domain") checkOutDomain(*s);
+    else if (o == "collision") { if (s->p.collision_type == MPS_COLLISION_PC) collisionPC(*s); else collisionDPC(*s); }
+    else if (o == "closest_point") closestPoint(*s);
+    else if (o == "update_near_mesh") updateNearMesh(*s);
+    else if (o == "wall_npcd") wallNPCD(*s);
+    else if (o == "wall_contributions") wallContributions(*s);
+    else if (o == "correction_matrix") correctionMatrix(*s);
+    else if (o == "initial_pnd") initialPnd(*s);
+    else if (o == "pnd_neigh_npcd") pndNeighNPCD(*s);
+    else if (o == "pnd_diffusive") pndDiffusive(*s);
+    else if (o == "wall_pnd_diffusive") wallPndDiffusive(*s);
+    else if (o == "update_pnd") updatePnd(*s);
+    else if (o == "mean_pnd_wall_dummy_surface") meanPndWallDummySurface(*s);
+    else if (o == "mean_pnd") meanPnd(*s);
+    else if (o == "mean_wall_pnd") meanWallPnd(*s);
+    else if (o == "mean_neigh_fluid_pnd") meanNeighFluidPnd(*s);
+    else if (o == "update_bc") return updateBC(*s);
+    else if (o == "vel_divergence") velDivergence(*s);
+    else if (o == "assemble_ppe") assemblePPE(*s);
+    else if (o == "solve_ppe") { solveCG(*s); for (int i = 0; i < s->N; i++) s->press[i] = s->ppeX[i] < 0 ? 0.0 : s->ppeX[i]; }
+    else if (o == "calc_press") return calcPress(*s);
+    else if (o == "extrapolate_wall_dummy") extrapolateWallDummy(*s);
+    else if (o == "press_gradient") pressGradient(*s);
+    else if (o == "wall_press_gradient") wallPressGradient(*s);
+    else if (o == "wall_viscosity") calcWallViscosity(*s);
+    else if (o == "correction") correction(*s);
+    else if (o == "shifting") { if (s->p.shifting_type == 1) shifting1(*s); else if (s->p.shifting_type == 2) shifting2(*s); }
+    else if (o == "wall_dummy_velocity") wallDummyVelocity(*s);
+    else if (o == "init") return initSystem(*s);
+    else if (o == "step") return stepSystem(*s);
+    else { s->err = "unknown op " + o; return -1; }
+    return 0;
+}
+
+int orc_steps(void* h, int k) {
+    Sys* s = (Sys*)h;
+    for (int t = 0; t < k; t++)
+        if (stepSystem(*s)) return -1;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure only).
+
+Only tests/, the smoke check and bench.py's CPU-baseline leg use this module.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+from polymps_amd.params import MpsParams
+
+_PATH = Path(__file__).resolve().parent / "liboracle_mps.so"
+_lib = None
+
+VEC = {"pos", "vel", "acc", "acc_star", "npcd", "grad_conc", "cm1", "cm2", "cm3", "wall_pos", "mirror",
+       "polygon_normal", "force_wall", "dvel"}
+INT = {"type", "bc", "num_neigh", "num_neigh_wall", "ptype", "near_mesh_type"}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not _PATH.exists():
+            raise RuntimeError(f"{_PATH} missing: build the oracle (make -C oracle)")
+        L = ctypes.CDLL(str(_PATH))
+        vp, ip, cp = ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p
+        P = ctypes.POINTER(MpsParams)
+        L.orc_create.argtypes = [P, ip, vp, vp, vp, vp, vp]
+        L.orc_create.restype = vp
+        L.orc_destroy.argtypes = [vp]
+        L.orc_get_params.argtypes = [vp, P]
+        L.orc_num_particles.argtypes = [vp]
+        L.orc_last_error.argtypes = [vp]
+        L.orc_last_error.restype = cp
+        L.orc_set_threads.argtypes = [vp, ip]
+        L.orc_set_mesh.argtypes = [vp, ip, vp, ip, vp, vp]
+        L.orc_get_wall_tables.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_field.argtypes = [vp, cp, vp, ip]
+        L.orc_scalar.argtypes = [vp, cp, ctypes.POINTER(ctypes.c_double)]
+        L.orc_set_scalar.argtypes = [vp, cp, ctypes.c_double]
+        L.orc_ppe_matrix.argtypes = [vp, vp, vp, vp]
+        L.orc_buckets.argtypes = [vp, vp, vp]
+        L.orc_neighbors.argtypes = [vp, ip, ctypes.c_double, ip, vp, ip]
+        L.orc_call.argtypes = [vp, cp]
+        L.orc_steps.argtypes = [vp, ip]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Oracle:
+    def __init__(self, params: MpsParams, types, pos, vel=None, press=None, press_avg=None, mesh=None, threads=0):
+        L = lib()
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        n = len(types)
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        vel = np.zeros(3 * n) if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(-1)
+        press = np.zeros(n) if press is None else np.ascontiguousarray(press, dtype=np.float64)
+        pavg = np.zeros(n) if press_avg is None else np.ascontiguousarray(press_avg, dtype=np.float64)
+        p = params.copy()
+        p.derived = 0
+        self.n = n
+        self.h = L.orc_create(ctypes.byref(p), n, _p(types), _p(pos), _p(vel), _p(press), _p(pavg))
+        if not self.h:
+            raise RuntimeError("oracle create failed")
+        if threads:
+            L.orc_set_threads(self.h, threads)
+        if mesh is not None:
+            V = np.ascontiguousarray(mesh[0], dtype=np.float64)
+            F = np.ascontiguousarray(mesh[1], dtype=np.int32)
+            N = np.ascontiguousarray(mesh[2], dtype=np.float64)
+            L.orc_set_mesh(self.h, len(V) // 3, _p(V), len(F) // 3, _p(F), _p(N))
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self) -> MpsParams:
+        p = MpsParams()
+        lib().orc_get_params(self.h, ctypes.byref(p))
+        return p
+
+    def call(self, op: str):
+        rc = lib().orc_call(self.h, op.encode())
+        if rc:
+            raise RuntimeError(f"oracle {op}: {lib().orc_last_error(self.h).decode()}")
+
+    def steps(self, k: int):
+        if lib().orc_steps(self.h, k):
+            raise RuntimeError(lib().orc_last_error(self.h).decode())
+
+    def get(self, field: str) -> np.ndarray:
+        if field in INT:
+            out = np.zeros(self.n, dtype=np.int32)
+        elif field == "near_wall":
+            out = np.zeros(self.n, dtype=np.int8)
+        elif field in VEC:
+            out = np.zeros(3 * self.n)
+        else:
+            out = np.zeros(self.n)
+        if lib().orc_field(self.h, field.encode(), _p(out), 0):
+            raise KeyError(field)
+        return out.astype(np.int32) if field == "near_wall" else out
+
+    def set(self, field: str, values):
+        dt = np.int32 if field in INT else (np.int8 if field == "near_wall" else np.float64)
+        arr = np.ascontiguousarray(values, dtype=dt).reshape(-1)
+        if lib().orc_field(self.h, field.encode(), _p(arr), 1):
+            raise KeyError(field)
+
+    def scalar(self, name: str) -> float:
+        v = ctypes.c_double()
+        if lib().orc_scalar(self.h, name.encode(), ctypes.byref(v)):
+            raise KeyError(name)
+        return v.value
+
+    def set_scalar(self, name: str, value: float):
+        lib().orc_set_scalar(self.h, name.encode(), float(value))
+
+    def ppe_matrix(self):
+        nnz = lib().orc_ppe_matrix(self.h, None, None, None)
+        n = lib().orc_num_particles(self.h)
+        rp = np.zeros(n + 1, dtype=np.int32)
+        cols = np.zeros(nnz, dtype=np.int32)
+        vals = np.zeros(nnz)
+        lib().orc_ppe_matrix(self.h, _p(rp), _p(cols), _p(vals))
+        return rp, cols, vals
+
+    def buckets(self):
+        nb = self.params.num_buckets_xyz
+        first = np.zeros(nb, dtype=np.int32)
+        nxt = np.zeros(self.n, dtype=np.int32)
+        lib().orc_buckets(self.h, _p(first), _p(nxt))
+        return first, nxt
+
+    def neighbors(self, i: int, r2: float, mode: int = 0, cap: int = 4096):
+        out = np.zeros(cap, dtype=np.int32)
+        c = lib().orc_neighbors(self.h, i, r2, mode, _p(out), cap)
+        return out[: min(c, cap)].copy()
+
+    def wall_tables(self):
+        x1, x2, z = np.zeros(50), np.zeros(50), np.zeros(50)
+        nn = np.zeros(50, dtype=np.int32)
+        lib().orc_get_wall_tables(self.h, _p(x1), _p(x2), _p(z), _p(nn))
+        return x1, x2, z, nn

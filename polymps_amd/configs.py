@@ -1,0 +1,246 @@
+"""Deterministic generators of the dam-break workloads.
+
+The reference ships no particle files (its .grid inputs are not in the
+tree), so every workload is generated: lattice particles at cell centres,
+seeded uniform jitter from a counter-based splitmix64 stream (identical on
+every platform), polygon walls as triangle meshes.  Parameters follow the
+reference's own examples: input/InputDamINC.json for the incompressible
+polygon-wall cases and input/dam1610_h300_lo0p0050_WC.json for the 2-D
+weakly-compressible particle-wall case.
+
+Workloads (particle counts at the named spacing):
+  dam2d_wc   2-D WC dam break, particle walls, l0 = 0.005: 5,000 fluid + 1,338 wall/dummy
+  dam3d_inc  3-D INC dam break in a closed 1.61 x 0.15 x 0.6 m box (12 triangles),
+             fluid block 0.6 x 0.15 x 0.3 m: 216,000 at l0 = 0.005, 1,728,000 at l0 = 0.0025
+"""
+from __future__ import annotations
+
+import struct
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+from . import lib as _lib
+from .params import MpsParams
+
+FLUID, WALL, DUMMY = 0, 2, 3
+
+
+def splitmix64_uniform(seed: int, n: int) -> np.ndarray:
+    """n uniform doubles in [0, 1) from a splitmix64 counter stream."""
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) + np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+@dataclass
+class Workload:
+    name: str
+    params: MpsParams
+    types: np.ndarray
+    pos: np.ndarray
+    vel: np.ndarray
+    mesh: tuple | None = None
+    steps: int = 1
+    info: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return len(self.types)
+
+
+# ------------------------------------------------------------------ meshes
+def box_mesh(lo, hi):
+    """Closed axis-aligned box as 12 triangles (outward normals), vertices
+    rounded to float32 as an STL file stores them."""
+    x0, y0, z0 = (float(np.float32(v)) for v in lo)
+    x1, y1, z1 = (float(np.float32(v)) for v in hi)
+    V = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], dtype=np.float64)
+    F = np.array([[0, 2, 1], [0, 3, 2],   # z0 (normal -z)
+                  [4, 5, 6], [4, 6, 7],   # z1 (+z)
+                  [0, 1, 5], [0, 5, 4],   # y0 (-y)
+                  [3, 7, 6], [3, 6, 2],   # y1 (+y)
+                  [0, 4, 7], [0, 7, 3],   # x0 (-x)
+                  [1, 2, 6], [1, 6, 5]],  # x1 (+x)
+                 dtype=np.int32)
+    return V, F
+
+
+def face_normals(V, F):
+    V3 = V.reshape(-1, 3)
+    F3 = F.reshape(-1, 3)
+    a, b, c = V3[F3[:, 0]], V3[F3[:, 1]], V3[F3[:, 2]]
+    e1, e2 = b - a, c - a
+    n = np.stack([e1[:, 1] * e2[:, 2] - e1[:, 2] * e2[:, 1],
+                  e1[:, 2] * e2[:, 0] - e1[:, 0] * e2[:, 2],
+                  e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]], axis=1)
+    r = np.sqrt(n[:, 0] * n[:, 0] + n[:, 1] * n[:, 1] + n[:, 2] * n[:, 2])
+    out = np.zeros_like(n)
+    nz = r != 0
+    out[nz] = n[nz] / r[nz, None]
+    return out.reshape(-1)
+
+
+def write_stl(path, V, F):
+    V3 = np.asarray(V).reshape(-1, 3)
+    F3 = np.asarray(F).reshape(-1, 3)
+    N = face_normals(V3.reshape(-1), F3.reshape(-1)).reshape(-1, 3)
+    with open(path, "wb") as f:
+        f.write(b"mps-mi355x generated STL".ljust(80, b" "))
+        f.write(struct.pack("<I", len(F3)))
+        for t in range(len(F3)):
+            f.write(struct.pack("<3f", *N[t]))
+            for k in range(3):
+                f.write(struct.pack("<3f", *V3[F3[t, k]]))
+            f.write(b"\0\0")
+
+
+def mesh_arrays(V, F):
+    V = np.ascontiguousarray(np.asarray(V, dtype=np.float64).reshape(-1))
+    F = np.ascontiguousarray(np.asarray(F, dtype=np.int32).reshape(-1))
+    return V, F, face_normals(V, F)
+
+
+# -------------------------------------------------------------- workloads
+def dam3d_inc(l0: float = 0.005, tank=(1.61, 0.15, 0.6), block=(0.6, 0.15, 0.3), seed: int = 2,
+              jitter: float = 0.01, margin: float = 0.05, steps: int = 500, dt: float = 2e-4,
+              tol: float = 1e-8) -> Workload:
+    """3-D incompressible dam break with a closed polygon box (12 triangles)."""
+    nx, ny, nz = (int(round(block[k] / l0)) for k in range(3))
+    if any(abs(c * l0 - b) > 1e-9 for c, b in zip((nx, ny, nz), block)):
+        raise ValueError(f"spacing {l0} does not tile the fluid block {block}")
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    # reference order: x slowest ... z fastest is arbitrary; use z, y, x nesting like a raster
+    pos = np.stack([(ix.ravel(order="F") + 0.5) * l0, (iy.ravel(order="F") + 0.5) * l0,
+                    (iz.ravel(order="F") + 0.5) * l0], axis=1)
+    n = len(pos)
+    u = splitmix64_uniform(seed, 3 * n).reshape(n, 3)
+    pos = pos + (2.0 * u - 1.0) * jitter * l0
+    types = np.full(n, FLUID, dtype=np.int32)
+    p = _lib.defaults()
+    p.wall_type = 1
+    p.dim = 3.0
+    p.part_dist = l0
+    p.time_step = dt
+    p.time_simulation = steps * dt
+    p.iter_output = 100
+    p.cfl_number = 0.2
+    p.weight_type = 0
+    p.slip_condition = 0
+    p.re_s = 2.1
+    p.re_l = 3.1
+    p.gradient_type = 0
+    p.gradient_correction = 1
+    p.relax_press = 1.0
+    p.mps_type = 2
+    p.solver_type = 0
+    p.alpha_compressibility = 1e-6
+    p.relax_pnd = 0.01
+    p.shifting_type = 0
+    p.pnd_type = 0
+    p.repulsive_force_type = 2
+    p.re_repulsive_force = 0.5
+    p.expect_max_velocity = 4.0
+    p.free_surf_type = 1
+    p.pnd_threshold = 0.97
+    p.neigh_threshold = 0.85
+    p.npcd_threshold = 0.2
+    p.collision_type = 0
+    p.collision_ratio = 0.2
+    p.dist_limit_ratio = 0.85
+    p.solver_tol = tol
+    for k in range(3):
+        p.domain_min[k] = -margin
+        p.domain_max[k] = tank[k] + margin
+    p.gravity[0], p.gravity[1], p.gravity[2] = 0.0, 0.0, -9.81
+    V, F = box_mesh((0.0, 0.0, 0.0), tank)
+    return Workload(f"dam3d_inc_l0_{l0:g}", p, types, pos.reshape(-1), np.zeros(3 * n),
+                    mesh=mesh_arrays(V, F), steps=steps,
+                    info={"lattice": (nx, ny, nz), "seed": seed, "jitter": jitter})
+
+
+def dam2d_wc(l0: float = 0.005, tank=(1.0, 0.6), column=(0.25, 0.5), seed: int = 1, jitter: float = 1e-4,
+             steps: int = 2000, dt: float = 1e-4) -> Workload:
+    """2-D weakly-compressible dam break with 1 wall + 2 dummy particle layers."""
+    cx, cy = int(round(column[0] / l0)), int(round(column[1] / l0))
+    ix, iy = np.meshgrid(np.arange(cx), np.arange(cy), indexing="ij")
+    fluid = np.stack([(ix.ravel(order="F") + 0.5) * l0, (iy.ravel(order="F") + 0.5) * l0], axis=1)
+    nf = len(fluid)
+    u = splitmix64_uniform(seed, 2 * nf).reshape(nf, 2)
+    fluid = fluid + (2.0 * u - 1.0) * jitter * l0
+    walls, wtypes = [], []
+    ntx, nty = int(round(tank[0] / l0)), int(round(tank[1] / l0))
+    for layer in range(3):
+        t = WALL if layer == 0 else DUMMY
+        off = (layer + 0.5) * l0
+        # the bottom row of a layer also closes the corners of the side rows
+        for k in range(-(layer + 1), ntx + layer + 1):
+            walls.append(((k + 0.5) * l0, -off)); wtypes.append(t)
+        for k in range(nty):
+            walls.append((-off, (k + 0.5) * l0)); wtypes.append(t)
+            walls.append((tank[0] + off, (k + 0.5) * l0)); wtypes.append(t)
+    walls = np.array(walls)
+    pos2 = np.concatenate([fluid, walls])
+    types = np.concatenate([np.full(nf, FLUID, dtype=np.int32), np.array(wtypes, dtype=np.int32)])
+    n = len(types)
+    pos = np.zeros((n, 3))
+    pos[:, :2] = pos2
+    p = _lib.defaults()
+    p.wall_type = 0
+    p.dim = 2.0
+    p.part_dist = l0
+    p.time_step = dt
+    p.time_simulation = steps * dt
+    p.iter_output = 200
+    p.cfl_number = 0.2
+    p.weight_type = 0
+    p.slip_condition = 0
+    p.re_s = 2.1
+    p.re_l = 2.1
+    p.gradient_type = 0
+    p.gradient_correction = 0
+    p.relax_press = 1.0
+    p.mps_type = 1
+    p.sound_speed = 20.0
+    p.gamma = 7.0
+    p.shifting_type = 2
+    p.pnd_type = 2
+    p.diffusive_coef = 0.35
+    p.free_surf_type = 0
+    p.pnd_threshold = 0.97
+    p.neigh_threshold = 0.85
+    p.collision_type = 0
+    p.density_fluid = 1000.0
+    p.density_wall = 1000.0
+    p.knm_vs1 = 1e-6
+    for k in range(3):
+        p.domain_min[k] = 0.0
+        p.domain_max[k] = 0.0
+    p.domain_max[0], p.domain_max[1] = tank[0], tank[1]
+    p.gravity[0], p.gravity[1], p.gravity[2] = 0.0, -9.81, 0.0
+    return Workload(f"dam2d_wc_l0_{l0:g}", p, types, pos.reshape(-1), np.zeros(3 * n), mesh=None, steps=steps,
+                    info={"fluid": nf, "walls": n - nf, "seed": seed})
+
+
+def small_dam3d(l0: float = 0.025, **kw) -> Workload:
+    """The 3-D dam break at a coarse spacing that tiles the tank
+    (0.025 m: 24 x 6 x 12 = 1,728 particles; 0.0125 m: 13,824)."""
+    kw.setdefault("steps", 20)
+    return dam3d_inc(l0=l0, **kw)
+
+
+def small_dam2d(l0: float = 0.02, **kw) -> Workload:
+    kw.setdefault("steps", 50)
+    return dam2d_wc(l0=l0, **kw)
+
+
+WORKLOADS = {
+    "dam2d_wc": lambda: dam2d_wc(),
+    "dam3d_inc_216k": lambda: dam3d_inc(l0=0.005, steps=500),
+    "dam3d_inc_1728k": lambda: dam3d_inc(l0=0.0025, steps=300),
+}

@@ -1,0 +1,698 @@
+// Context management and the C ABI of the MI355X MPS library.
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "kern_common.h"
+
+namespace mpsgpu {
+
+static std::mutex gErrMu;
+static std::string gErr;
+void setGlobalError(const std::string& e) {
+    std::lock_guard<std::mutex> lk(gErrMu);
+    gErr = e;
+}
+std::string globalError() {
+    std::lock_guard<std::mutex> lk(gErrMu);
+    return gErr;
+}
+
+KArgs Ctx::args() const {
+    KArgs a;
+    a.P = P;
+    a.N = N;
+    a.nB = nB;
+    for (int f = 0; f < ND; f++) a.D[f] = D[f];
+    for (int f = 0; f < NI; f++) a.I[f] = I[f];
+    a.cellStart = cellStart;
+    a.nbr = nbr;
+    a.nbrCnt = nbrCnt;
+    a.K = K;
+    a.mV = mV;
+    a.mF = mF;
+    a.mN = mN;
+    a.nf = nf;
+    a.wtab = wtab;
+    a.ds = ds;
+    a.is = is;
+    a.red = red;
+    return a;
+}
+
+// ------------------------------------------------------------ field map
+struct FieldDesc {
+    const char* name;
+    int kind;      // 0: double (1 comp), 1: double vec (3 comps), 2: int
+    int f0;        // first field index
+};
+static const FieldDesc kFields[] = {
+    {"pos", 1, DX}, {"vel", 1, DU}, {"acc", 1, DAX}, {"acc_star", 1, DASX}, {"npcd", 1, DNPX},
+    {"mirror", 1, DMX}, {"wall_pos", 1, DWPX}, {"polygon_normal", 1, DPNX}, {"force_wall", 1, DFWX},
+    {"cm1", 1, DCM0}, {"cm2", 1, DCM3}, {"cm3", 1, DCM6}, {"grad_conc", 1, DGCX},
+    {"press", 0, DPRESS}, {"press_avg", 0, DPAVG}, {"pndi", 0, DPNDI}, {"pndki", 0, DPNDKI},
+    {"pndski", 0, DPNDSKI}, {"pnd_small", 0, DPNDSMALL}, {"npcd2", 0, DNPCD2}, {"conc", 0, DCONC},
+    {"div_u", 0, DDIVU}, {"diffusive", 0, DDIFF}, {"pnd_wall", 0, DPNDWALL}, {"riw2", 0, DRIW2},
+    {"ppe_x", 0, DPPEX}, {"ppe_b", 0, DPPEB}, {"dev_dot_n", 0, DDEVN},
+    {"type", 2, ITYPE}, {"bc", 2, IBC}, {"num_neigh", 2, INNEIGH}, {"num_neigh_wall", 2, INNWALL},
+    {"near_wall", 2, INEARWALL}, {"near_mesh_type", 2, INEARMESH},
+};
+static const FieldDesc* findField(const char* name) {
+    for (const auto& f : kFields)
+        if (strcmp(f.name, name) == 0) return &f;
+    return nullptr;
+}
+
+// slot -> reference order (AoS for vectors)
+__global__ void kToCanonD(int n, const int* canon, const double* s0, const double* s1, const double* s2, double* dst,
+                          int comps) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int c = canon[k];
+    if (c < 0 || c >= n) return;
+    if (comps == 1) {
+        dst[c] = s0[k];
+    } else {
+        dst[c * 3] = s0[k]; dst[c * 3 + 1] = s1[k]; dst[c * 3 + 2] = s2[k];
+    }
+}
+__global__ void kToCanonI(int n, const int* canon, const int* s, int* dst) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int c = canon[k];
+    if (c < 0 || c >= n) return;
+    dst[c] = s[k];
+}
+__global__ void kFromCanonD(int n, const int* canon, const double* src, double* d0, double* d1, double* d2, int comps) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int c = canon[k];
+    if (c < 0 || c >= n) return;
+    if (comps == 1) {
+        d0[k] = src[c];
+    } else {
+        d0[k] = src[c * 3]; d1[k] = src[c * 3 + 1]; d2[k] = src[c * 3 + 2];
+    }
+}
+__global__ void kFromCanonI(int n, const int* canon, const int* src, int* d) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int c = canon[k];
+    if (c < 0 || c >= n) return;
+    d[k] = src[c];
+}
+
+__global__ void kInitState(KArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.N) return;
+    for (int f = DAX; f < ND; f++) a.D[f][k] = 0.0;
+    a.D[DMX][k] = 0.0; a.D[DMY][k] = 0.0; a.D[DMZ][k] = 0.0;
+    a.D[DPNDI][k] = 0.0; a.D[DPPEX][k] = 0.0; a.D[DFWX][k] = 0.0; a.D[DFWY][k] = 0.0; a.D[DFWZ][k] = 0.0;
+    a.D[DPNDSKI][k] = 0.0;
+    a.D[DRIW2][k] = 10e8 * a.P.part_dist;
+    a.I[ICANON][k] = k;
+    a.I[IBC][k] = 0;
+    a.I[INNEIGH][k] = 0;
+    a.I[INNWALL][k] = 0;
+    a.I[INEARWALL][k] = 0;
+    a.I[INEARMESH][k] = 0;
+}
+
+// wall tables of PolygonMesh::initWijnNeigh (src/PolygonMesh.cpp:377-527)
+static void buildWallTables(Ctx* ctx) {
+    const mps_params& P = ctx->P;
+    const int dim = (int)P.dim, wt = P.weight_type;
+    const double lo = P.part_dist, reL = P.re_l, reS = P.re_s;
+    const double reS2 = reS * reS, reL2 = reL * reL;
+    const int n = kWallTable;
+    ctx->hxPND.clear(); ctx->hxNeigh.clear(); ctx->hZij.clear(); ctx->hnNeigh.clear();
+    const int limS = (int)ceil(reS / lo), limL = (int)ceil(reL / lo);
+    for (int i = 1; i <= n; i++) {
+        const double xi = 0.0, yi = i * reS / n, zi = 0.0;
+        const double riw = (dim == 2) ? sqrt((0.0 - xi) * (0.0 - xi) + (0.0 - yi) * (0.0 - yi))
+                                      : sqrt((0.0 - xi) * (0.0 - xi) + (0.0 - yi) * (0.0 - yi) + (0.0 - zi) * (0.0 - zi));
+        ctx->hxPND.push_back(riw / reS);
+        ctx->hxNeigh.push_back(riw / reL);
+        double n0 = 0.0;
+        for (int ix = -limS; ix <= limS; ix++)
+            for (int iy = -limS; iy <= 0; iy++)
+                for (int iz = (dim == 2 ? 0 : -limS); iz <= (dim == 2 ? 0 : limS); iz++) {
+                    const double xj = lo * (double)ix, yj = lo * (double)(iy - 0.5), zj = lo * (double)iz;
+                    const double r2 = (dim == 2) ? (xj - xi) * (xj - xi) + (yj - yi) * (yj - yi)
+                                                 : (xj - xi) * (xj - xi) + (yj - yi) * (yj - yi) + (zj - zi) * (zj - zi);
+                    if (!(r2 < reS2) || r2 == 0.0) continue;
+                    const double r = sqrt(r2);
+                    if (wt == 1) n0 += reS / r + r / reS - 2.0;
+                    else if (wt == 2) n0 += reS / r - r / reS;
+                    else if (wt == 3) n0 += pow(1.0 - r / reS, 3.0);
+                    else if (wt == 4) n0 += pow(1.0 - r / reS, 2.0);
+                    else n0 += reS / r - 1.0;
+                }
+        ctx->hZij.push_back(n0);
+        int nn = 0;
+        for (int ix = -limL; ix <= limL; ix++)
+            for (int iy = -limL; iy <= 0; iy++)
+                for (int iz = (dim == 2 ? 0 : -limL); iz <= (dim == 2 ? 0 : limL); iz++) {
+                    const double xj = lo * (double)ix, yj = lo * (double)(iy - 0.5), zj = lo * (double)iz;
+                    const double r2 = (dim == 2) ? (xj - xi) * (xj - xi) + (yj - yi) * (yj - yi)
+                                                 : (xj - xi) * (xj - xi) + (yj - yi) * (yj - yi) + (zj - zi) * (zj - zi);
+                    if (r2 < reL2 && r2 != 0.0) nn += 1;
+                }
+        ctx->hnNeigh.push_back(nn);
+    }
+}
+
+static int syncStatus(Ctx* ctx) {
+    MPS_HIP(hipMemcpyAsync(ctx->hostInts, ctx->is, sizeof(int) * NIS, hipMemcpyDeviceToHost, ctx->stream));
+    MPS_HIP(hipMemcpyAsync(ctx->hostScalars, ctx->ds, sizeof(double) * NDS, hipMemcpyDeviceToHost, ctx->stream));
+    MPS_HIP(hipStreamSynchronize(ctx->stream));
+    MPS_HIP(hipGetLastError());
+    ctx->nLiveHost = ctx->hostInts[I_NLIVE];
+    ctx->velMaxHost = ctx->hostScalars[S_VELMAX];
+    if (ctx->hostInts[I_OVERFLOW] > 0) {
+        char buf[160];
+        snprintf(buf, sizeof(buf), "neighbour list overflow: a particle has %d candidates, capacity %d",
+                 ctx->hostInts[I_OVERFLOW], ctx->K);
+        ctx->err = buf;
+        return MPS_E_CAPACITY;
+    }
+    if (ctx->hostInts[I_COMPACT_OVF]) {
+        ctx->err = "more out-of-domain particles in one call than the compaction list holds";
+        return MPS_E_CAPACITY;
+    }
+    return MPS_OK;
+}
+
+static int runInit(Ctx* ctx) {
+    const mps_params& P = ctx->P;
+    int rc;
+    if ((rc = launchBucketSort(ctx))) return rc;
+    if ((rc = launchCompaction(ctx))) return rc;
+    if (P.wall_type == MPS_WALL_POLYGON) {
+        if (!ctx->hasMesh) { ctx->err = "polygon walls need a mesh (mps_set_mesh)"; return MPS_E_ARG; }
+        if ((rc = launchOp(ctx, "closest_point"))) return rc;
+        if ((rc = launchOp(ctx, "update_near_mesh"))) return rc;
+    }
+    if ((rc = launchBuildList(ctx))) return rc;
+    if (P.wall_type == MPS_WALL_POLYGON)
+        if ((rc = launchOp(ctx, "wall_npcd"))) return rc;
+    if ((rc = launchOp(ctx, "initial_pnd"))) return rc;
+    if ((rc = launchOp(ctx, "mean_pnd"))) return rc;
+    if ((rc = launchOp(ctx, "mean_neigh_fluid_pnd"))) return rc;
+    if ((rc = launchOp(ctx, "update_bc"))) return rc;
+    if ((rc = syncStatus(ctx))) return rc;
+    if (P.mps_type == MPS_EXPLICIT || P.mps_type == MPS_WEAKLY) {
+        if ((rc = launchOp(ctx, "press_explicit"))) return rc;
+    } else {
+        if (P.mps_type == MPS_IMPLICIT_PND_DIVU && (rc = launchOp(ctx, "vel_divergence"))) return rc;
+        if ((rc = ppeAssemble(ctx))) return rc;
+        if ((rc = ppeSolve(ctx))) return rc;
+    }
+    return syncStatus(ctx);
+}
+
+static int calcPress(Ctx* ctx) {
+    const mps_params& P = ctx->P;
+    int rc;
+    if (P.mps_type == MPS_EXPLICIT || P.mps_type == MPS_WEAKLY) return launchOp(ctx, "press_explicit");
+    if (P.mps_type == MPS_IMPLICIT_PND_DIVU) {
+        if (P.wall_type == MPS_WALL_POLYGON) { ctx->err = "PND+DivU with polygon walls is not supported"; return MPS_E_UNSUPPORTED; }
+        if ((rc = launchOp(ctx, "vel_divergence"))) return rc;
+    }
+    if (P.solver_type != MPS_SOLVER_CG) { ctx->err = "only the CG solver is supported"; return MPS_E_UNSUPPORTED; }
+    if ((rc = ppeAssemble(ctx))) return rc;
+    return ppeSolve(ctx);
+}
+
+static int runStep(Ctx* ctx) {
+    const mps_params& P = ctx->P;
+    int rc;
+#define OP(name) if ((rc = launchOp(ctx, name))) return rc
+    if (P.relax_press < 1.0) { ctx->err = "relax_fact < 1 (prediction pressure gradient) is not supported"; return MPS_E_UNSUPPORTED; }
+    if ((rc = launchBucketSort(ctx))) return rc;
+    OP("viscosity");
+    OP("add_gravity");
+    OP("prediction");
+    if ((rc = launchCompaction(ctx))) return rc;
+    if (P.collision_type == MPS_COLLISION_PC) {
+        if ((rc = launchBuildList(ctx))) return rc;
+        OP("collision");
+    } else {
+        OP("collision");
+        if ((rc = launchBuildList(ctx))) return rc;
+    }
+    if (P.wall_type == MPS_WALL_POLYGON) {
+        OP("closest_point");
+        OP("update_near_mesh");
+        OP("wall_npcd");
+    }
+    if (P.gradient_correction) OP("correction_matrix");
+    OP("pnd_neigh_npcd");
+    if (P.pnd_type == MPS_PND_DIFFUSIVE) OP("pnd_diffusive");
+    else if (P.pnd_type == MPS_PND_MEAN_SUM_WIJ) OP("mean_pnd");
+    OP("update_bc");
+    if ((rc = calcPress(ctx))) return rc;
+    if (P.wall_type == MPS_WALL_PARTICLE) OP("extrapolate_wall_dummy");
+    if (P.gradient_correction) OP("correction_matrix");
+    OP("press_gradient");
+    if (P.wall_type == MPS_WALL_POLYGON) {
+        OP("wall_press_gradient");
+        OP("wall_viscosity");
+    }
+    OP("correction");
+    if ((rc = launchCompaction(ctx))) return rc;
+    OP("shifting");
+    if (P.wall_type == MPS_WALL_PARTICLE) OP("wall_dummy_velocity");
+    if ((rc = launchCompaction(ctx))) return rc;
+#undef OP
+    ctx->numIter++;
+    ctx->timeCurrent += P.time_step;
+    return syncStatus(ctx);
+}
+
+}  // namespace mpsgpu
+
+using namespace mpsgpu;
+
+struct mps_ctx : public Ctx {};
+
+#define CHECK_CTX(c) \
+    if (!(c)) return MPS_E_ARG
+
+extern "C" {
+
+int mps_create(mps_ctx** out, const mps_params* p, int n, int device) {
+    if (!out || !p || n <= 0) return MPS_E_ARG;
+    if (!p->derived) { setGlobalError("parameters are not derived (call mps_params_derive)"); return MPS_E_ARG; }
+    if ((int)p->dim != 2 && (int)p->dim != 3) { setGlobalError("dimension must be 2 or 3"); return MPS_E_ARG; }
+    mps_ctx* ctx = new mps_ctx();
+    *out = ctx;
+    ctx->P = *p;
+    ctx->N = n;
+    ctx->nLiveHost = n;
+    ctx->nB = p->num_buckets_xyz;
+    ctx->dev = device;
+    MPS_HIP(hipSetDevice(device));
+    MPS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    MPS_HIP(hipEventCreate(&ctx->ev0));
+    MPS_HIP(hipEventCreate(&ctx->ev1));
+    const size_t nd = sizeof(double) * (size_t)n, ni = sizeof(int) * (size_t)n;
+    for (int f = 0; f < ND; f++) MPS_HIP(hipMalloc(&ctx->D[f], nd));
+    for (int f = 0; f < NPD; f++) MPS_HIP(hipMalloc(&ctx->Dalt[f], nd));
+    for (int f = 0; f < NI; f++) MPS_HIP(hipMalloc(&ctx->I[f], ni));
+    for (int f = 0; f < NPI; f++) MPS_HIP(hipMalloc(&ctx->Ialt[f], ni));
+    MPS_HIP(hipMalloc(&ctx->cellStart, sizeof(int) * (ctx->nB + 2)));
+    MPS_HIP(hipMalloc(&ctx->cellCount, sizeof(int) * (ctx->nB + 1)));
+    MPS_HIP(hipMalloc(&ctx->key, ni));
+    MPS_HIP(hipMalloc(&ctx->rank, ni));
+    MPS_HIP(hipMalloc(&ctx->order, ni));
+    MPS_HIP(hipMalloc(&ctx->order2, ni));
+    MPS_HIP(hipMalloc(&ctx->slotOf, ni));
+    MPS_HIP(hipMalloc(&ctx->scanTmp, sizeof(int) * 8192));
+    MPS_HIP(hipMalloc(&ctx->outList, sizeof(int) * 4096));
+    // neighbour-list capacity: the reL ball holds ~122 particles at rest in
+    // 3D (~28 in 2D at reL = 3.1); the capacity leaves room for compression
+    const double rr = std::max(p->re_l, p->re_s) / p->part_dist;
+    const double ball = ((int)p->dim == 3) ? 4.18879 * rr * rr * rr : 3.14159 * rr * rr;
+    ctx->K = std::max(32, ((int)(ball * 2.0) + 31) / 32 * 32);
+    MPS_HIP(hipMalloc(&ctx->nbr, sizeof(int) * (size_t)ctx->K * n));
+    MPS_HIP(hipMalloc(&ctx->nbrCnt, ni));
+    MPS_HIP(hipMalloc(&ctx->aVal, sizeof(double) * (size_t)ctx->K * n));
+    MPS_HIP(hipMalloc(&ctx->aCol, sizeof(int) * (size_t)ctx->K * n));
+    MPS_HIP(hipMalloc(&ctx->aCnt, ni));
+    MPS_HIP(hipMalloc(&ctx->aDiag, nd));
+    MPS_HIP(hipMalloc(&ctx->cgR, nd));
+    MPS_HIP(hipMalloc(&ctx->cgP, nd));
+    MPS_HIP(hipMalloc(&ctx->cgZ, nd));
+    MPS_HIP(hipMalloc(&ctx->cgQ, nd));
+    MPS_HIP(hipMalloc(&ctx->cgInvD, nd));
+    MPS_HIP(hipMalloc(&ctx->ds, sizeof(double) * NDS));
+    MPS_HIP(hipMalloc(&ctx->is, sizeof(int) * NIS));
+    MPS_HIP(hipMalloc(&ctx->red, sizeof(double) * (4 * kRedBlocks + 8)));
+    MPS_HIP(hipMalloc(&ctx->wtab, sizeof(double) * 4 * kWallTable));
+    MPS_HIP(hipHostMalloc(&ctx->hostScalars, sizeof(double) * NDS));
+    MPS_HIP(hipHostMalloc(&ctx->hostInts, sizeof(int) * NIS));
+    MPS_HIP(hipMemsetAsync(ctx->ds, 0, sizeof(double) * NDS, ctx->stream));
+    MPS_HIP(hipMemsetAsync(ctx->is, 0, sizeof(int) * NIS, ctx->stream));
+    MPS_HIP(hipMemsetAsync(ctx->nbrCnt, 0, ni, ctx->stream));
+    buildWallTables(ctx);
+    std::vector<double> wt(4 * kWallTable);
+    for (int q = 0; q < kWallTable; q++) {
+        wt[q] = ctx->hxPND[q];
+        wt[kWallTable + q] = ctx->hxNeigh[q];
+        wt[2 * kWallTable + q] = ctx->hZij[q];
+        wt[3 * kWallTable + q] = (double)ctx->hnNeigh[q];
+    }
+    MPS_HIP(hipMemcpyAsync(ctx->wtab, wt.data(), sizeof(double) * wt.size(), hipMemcpyHostToDevice, ctx->stream));
+    MPS_HIP(hipStreamSynchronize(ctx->stream));
+    return MPS_OK;
+}
+
+int mps_destroy(mps_ctx* ctx) {
+    if (!ctx) return MPS_OK;
+    hipSetDevice(ctx->dev);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (int f = 0; f < ND; f++) hipFree(ctx->D[f]);
+    for (int f = 0; f < NPD; f++) hipFree(ctx->Dalt[f]);
+    for (int f = 0; f < NI; f++) hipFree(ctx->I[f]);
+    for (int f = 0; f < NPI; f++) hipFree(ctx->Ialt[f]);
+    void* bufs[] = {ctx->cellStart, ctx->cellCount, ctx->key, ctx->rank, ctx->order, ctx->order2, ctx->slotOf,
+                    ctx->scanTmp, ctx->outList, ctx->nbr, ctx->nbrCnt, ctx->mV, ctx->mF, ctx->mN, ctx->wtab,
+                    ctx->aVal, ctx->aCol, ctx->aCnt, ctx->aDiag, ctx->aVal32, ctx->aDiagInv32, ctx->cgR, ctx->cgP,
+                    ctx->cgZ, ctx->cgQ, ctx->cgInvD, ctx->cgX, ctx->fR, ctx->fP, ctx->fZ, ctx->fQ, ctx->fX,
+                    ctx->ds, ctx->is, ctx->red};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    if (ctx->hostScalars) hipHostFree(ctx->hostScalars);
+    if (ctx->hostInts) hipHostFree(ctx->hostInts);
+    if (ctx->ev0) hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MPS_OK;
+}
+
+const char* mps_last_error(const mps_ctx* ctx) {
+    static thread_local std::string tl;
+    if (ctx && !ctx->err.empty()) return ctx->err.c_str();
+    tl = globalError();
+    return tl.c_str();
+}
+
+int mps_get_params(const mps_ctx* ctx, mps_params* out) {
+    CHECK_CTX(ctx);
+    if (!out) return MPS_E_ARG;
+    *out = ctx->P;
+    return MPS_OK;
+}
+
+int mps_upload_state(mps_ctx* ctx, const int* type, const double* pos, const double* vel, const double* press,
+                     const double* press_avg) {
+    CHECK_CTX(ctx);
+    if (!type || !pos || !vel || !press || !press_avg) return MPS_E_ARG;
+    MPS_HIP(hipSetDevice(ctx->dev));
+    const int n = ctx->N;
+    KArgs a = ctx->args();
+    hipLaunchKernelGGL(kInitState, dim3(gridFor(n, 256)), dim3(256), 0, ctx->stream, a);
+    MPS_HIP(hipMemcpyAsync(ctx->I[ITYPE], type, sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<double> tmp(n);
+    for (int c = 0; c < 3; c++) {
+        for (int i = 0; i < n; i++) tmp[i] = pos[i * 3 + c];
+        MPS_HIP(hipMemcpyAsync(ctx->D[DX + c], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+        MPS_HIP(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < n; i++) tmp[i] = vel[i * 3 + c];
+        MPS_HIP(hipMemcpyAsync(ctx->D[DU + c], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+        MPS_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    MPS_HIP(hipMemcpyAsync(ctx->D[DPRESS], press, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    MPS_HIP(hipMemcpyAsync(ctx->D[DPAVG], press_avg, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    MPS_HIP(hipMemsetAsync(ctx->is, 0, sizeof(int) * NIS, ctx->stream));
+    MPS_HIP(hipMemsetAsync(ctx->ds, 0, sizeof(double) * NDS, ctx->stream));
+    MPS_HIP(hipMemcpyAsync(ctx->is + I_NLIVE, &n, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    MPS_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->nLiveHost = n;
+    ctx->listValid = false;
+    ctx->numIter = 0;
+    ctx->timeCurrent = 0.0;
+    return MPS_OK;
+}
+
+int mps_download(mps_ctx* ctx, const char* field, void* out, size_t count) {
+    CHECK_CTX(ctx);
+    if (!field || !out) return MPS_E_ARG;
+    const FieldDesc* fd = findField(field);
+    if (!fd) { ctx->err = std::string("unknown field ") + field; return MPS_E_ARG; }
+    MPS_HIP(hipSetDevice(ctx->dev));
+    const int n = ctx->N;
+    const int comps = fd->kind == 1 ? 3 : 1;
+    if (count < (size_t)n * comps) { ctx->err = "output buffer too small"; return MPS_E_ARG; }
+    const int g = gridFor(n, 256);
+    if (fd->kind == 2) {
+        int* tmp = nullptr;
+        MPS_HIP(hipMalloc(&tmp, sizeof(int) * n));
+        hipLaunchKernelGGL(kToCanonI, dim3(g), dim3(256), 0, ctx->stream, n, ctx->I[ICANON], ctx->I[fd->f0], tmp);
+        hipError_t e = hipMemcpyAsync(out, tmp, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream);
+        hipStreamSynchronize(ctx->stream);
+        hipFree(tmp);
+        if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return MPS_E_HIP; }
+    } else {
+        double* tmp = nullptr;
+        MPS_HIP(hipMalloc(&tmp, sizeof(double) * n * comps));
+        const double* s0 = ctx->D[fd->f0];
+        const double* s1 = comps == 3 ? ctx->D[fd->f0 + 1] : s0;
+        const double* s2 = comps == 3 ? ctx->D[fd->f0 + 2] : s0;
+        hipLaunchKernelGGL(kToCanonD, dim3(g), dim3(256), 0, ctx->stream, n, ctx->I[ICANON], s0, s1, s2, tmp, comps);
+        hipError_t e = hipMemcpyAsync(out, tmp, sizeof(double) * n * comps, hipMemcpyDeviceToHost, ctx->stream);
+        hipStreamSynchronize(ctx->stream);
+        hipFree(tmp);
+        if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return MPS_E_HIP; }
+    }
+    MPS_HIP(hipGetLastError());
+    return MPS_OK;
+}
+
+int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t count) {
+    CHECK_CTX(ctx);
+    if (!field || !in) return MPS_E_ARG;
+    const FieldDesc* fd = findField(field);
+    if (!fd) { ctx->err = std::string("unknown field ") + field; return MPS_E_ARG; }
+    MPS_HIP(hipSetDevice(ctx->dev));
+    const int n = ctx->N;
+    const int comps = fd->kind == 1 ? 3 : 1;
+    if (count < (size_t)n * comps) { ctx->err = "input buffer too small"; return MPS_E_ARG; }
+    const int g = gridFor(n, 256);
+    if (fd->kind == 2) {
+        int* tmp = nullptr;
+        MPS_HIP(hipMalloc(&tmp, sizeof(int) * n));
+        MPS_HIP(hipMemcpyAsync(tmp, in, sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(kFromCanonI, dim3(g), dim3(256), 0, ctx->stream, n, ctx->I[ICANON], tmp, ctx->I[fd->f0]);
+        hipStreamSynchronize(ctx->stream);
+        hipFree(tmp);
+    } else {
+        double* tmp = nullptr;
+        MPS_HIP(hipMalloc(&tmp, sizeof(double) * n * comps));
+        MPS_HIP(hipMemcpyAsync(tmp, in, sizeof(double) * n * comps, hipMemcpyHostToDevice, ctx->stream));
+        double* d0 = ctx->D[fd->f0];
+        double* d1 = comps == 3 ? ctx->D[fd->f0 + 1] : d0;
+        double* d2 = comps == 3 ? ctx->D[fd->f0 + 2] : d0;
+        hipLaunchKernelGGL(kFromCanonD, dim3(g), dim3(256), 0, ctx->stream, n, ctx->I[ICANON], tmp, d0, d1, d2, comps);
+        hipStreamSynchronize(ctx->stream);
+        hipFree(tmp);
+    }
+    ctx->listValid = false;
+    MPS_HIP(hipGetLastError());
+    return MPS_OK;
+}
+
+int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
+    CHECK_CTX(ctx);
+    if (!name || !out) return MPS_E_ARG;
+    int rc = syncStatus(ctx);
+    if (rc) return rc;
+    const std::string s(name);
+    if (s == "vel_max") *out = ctx->velMaxHost;
+    else if (s == "cfl") *out = ctx->P.time_step * ctx->velMaxHost / ctx->P.part_dist;
+    else if (s == "time") *out = ctx->timeCurrent;
+    else if (s == "solver_iter") *out = ctx->solverIter;
+    else if (s == "solver_err") *out = ctx->solverErr;
+    else if (s == "num_particles") *out = ctx->nLiveHost;
+    else if (s == "iterations") *out = ctx->numIter;
+    else if (s == "ppe_ms") *out = ctx->ppeMs;
+    else if (s == "true_residual") *out = ctx->trueResidual;
+    else if (s == "fallback") *out = ctx->fallbackUsed;
+    else if (s == "list_max") *out = ctx->hostInts[I_LIST_MAX];
+    else if (s == "list_capacity") *out = ctx->K;
+    else if (s == "capacity") *out = ctx->N;
+    else { ctx->err = "unknown scalar " + s; return MPS_E_ARG; }
+    return MPS_OK;
+}
+
+int mps_set_scalar(mps_ctx* ctx, const char* name, double value) {
+    CHECK_CTX(ctx);
+    const std::string s(name ? name : "");
+    if (s == "vel_max") {
+        MPS_HIP(hipMemcpyAsync(ctx->ds + S_VELMAX, &value, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        MPS_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->velMaxHost = value;
+    } else if (s == "time") {
+        ctx->timeCurrent = value;
+    } else {
+        ctx->err = "unknown scalar " + s;
+        return MPS_E_ARG;
+    }
+    return MPS_OK;
+}
+
+int mps_sync(mps_ctx* ctx) {
+    CHECK_CTX(ctx);
+    return syncStatus(ctx);
+}
+
+int mps_set_mesh(mps_ctx* ctx, int nv, const double* V, int nf, const int* F, const double* N) {
+    CHECK_CTX(ctx);
+    if (nv <= 0 || nf <= 0 || !V || !F || !N) return MPS_E_ARG;
+    for (int q = 0; q < 3 * nf; q++)
+        if (F[q] < 0 || F[q] >= nv) { ctx->err = "face index out of range"; return MPS_E_ARG; }
+    MPS_HIP(hipSetDevice(ctx->dev));
+    if (ctx->mV) hipFree(ctx->mV);
+    if (ctx->mF) hipFree(ctx->mF);
+    if (ctx->mN) hipFree(ctx->mN);
+    MPS_HIP(hipMalloc(&ctx->mV, sizeof(double) * 3 * nv));
+    MPS_HIP(hipMalloc(&ctx->mF, sizeof(int) * 3 * nf));
+    MPS_HIP(hipMalloc(&ctx->mN, sizeof(double) * 3 * nf));
+    MPS_HIP(hipMemcpy(ctx->mV, V, sizeof(double) * 3 * nv, hipMemcpyHostToDevice));
+    MPS_HIP(hipMemcpy(ctx->mF, F, sizeof(int) * 3 * nf, hipMemcpyHostToDevice));
+    MPS_HIP(hipMemcpy(ctx->mN, N, sizeof(double) * 3 * nf, hipMemcpyHostToDevice));
+    ctx->nv = nv;
+    ctx->nf = nf;
+    ctx->hasMesh = true;
+    return MPS_OK;
+}
+
+int mps_get_wall_tables(mps_ctx* ctx, double* xpnd, double* xneigh, double* zij, int* nneigh) {
+    CHECK_CTX(ctx);
+    for (int q = 0; q < kWallTable; q++) {
+        if (xpnd) xpnd[q] = ctx->hxPND[q];
+        if (xneigh) xneigh[q] = ctx->hxNeigh[q];
+        if (zij) zij[q] = ctx->hZij[q];
+        if (nneigh) nneigh[q] = ctx->hnNeigh[q];
+    }
+    return MPS_OK;
+}
+
+#define OPFN(fn, name)                          \
+    int fn(mps_ctx* ctx) {                      \
+        CHECK_CTX(ctx);                         \
+        hipSetDevice(ctx->dev);                 \
+        int rc = launchOp(ctx, name);           \
+        if (rc) return rc;                      \
+        return syncStatus(ctx);                 \
+    }
+
+int mps_bucket_update(mps_ctx* ctx) {
+    CHECK_CTX(ctx);
+    hipSetDevice(ctx->dev);
+    int rc = launchBucketSort(ctx);
+    if (rc) return rc;
+    return syncStatus(ctx);
+}
+OPFN(mps_viscosity, "viscosity")
+OPFN(mps_add_gravity, "add_gravity")
+OPFN(mps_prediction, "prediction")
+int mps_check_out_domain(mps_ctx* ctx) {
+    CHECK_CTX(ctx);
+    hipSetDevice(ctx->dev);
+    int rc = launchCompaction(ctx);
+    if (rc) return rc;
+    return syncStatus(ctx);
+}
+int mps_build_neighbor_list(mps_ctx* ctx) {
+    CHECK_CTX(ctx);
+    hipSetDevice(ctx->dev);
+    int rc = launchBuildList(ctx);
+    if (rc) return rc;
+    return syncStatus(ctx);
+}
+OPFN(mps_collision, "collision")
+OPFN(mps_mesh_closest_point, "closest_point")
+OPFN(mps_mesh_update_near, "update_near_mesh")
+OPFN(mps_wall_npcd, "wall_npcd")
+OPFN(mps_correction_matrix, "correction_matrix")
+OPFN(mps_initial_pnd, "initial_pnd")
+OPFN(mps_pnd_neigh_npcd, "pnd_neigh_npcd")
+OPFN(mps_pnd_diffusive, "pnd_diffusive")
+OPFN(mps_mean_pnd, "mean_pnd")
+OPFN(mps_mean_neigh_fluid_pnd, "mean_neigh_fluid_pnd")
+OPFN(mps_update_bc, "update_bc")
+OPFN(mps_extrapolate_wall_dummy, "extrapolate_wall_dummy")
+OPFN(mps_press_gradient, "press_gradient")
+OPFN(mps_wall_press_gradient, "wall_press_gradient")
+OPFN(mps_wall_viscosity, "wall_viscosity")
+OPFN(mps_correction, "correction")
+OPFN(mps_shifting, "shifting")
+OPFN(mps_wall_dummy_velocity, "wall_dummy_velocity")
+
+int mps_calc_press(mps_ctx* ctx) {
+    CHECK_CTX(ctx);
+    hipSetDevice(ctx->dev);
+    int rc = calcPress(ctx);
+    if (rc) return rc;
+    return syncStatus(ctx);
+}
+int mps_ppe_assemble(mps_ctx* ctx) {
+    CHECK_CTX(ctx);
+    hipSetDevice(ctx->dev);
+    int rc = ppeAssemble(ctx);
+    if (rc) return rc;
+    return syncStatus(ctx);
+}
+int mps_ppe_solve(mps_ctx* ctx) {
+    CHECK_CTX(ctx);
+    hipSetDevice(ctx->dev);
+    int rc = ppeSolve(ctx);
+    if (rc) return rc;
+    return syncStatus(ctx);
+}
+
+int mps_ppe_matrix(mps_ctx* ctx, int* row_ptr, int* cols, double* vals, int* nnz_out) {
+    CHECK_CTX(ctx);
+    hipSetDevice(ctx->dev);
+    const int n = ctx->N, K = ctx->K;
+    std::vector<int> cnt(n), canon(n), col((size_t)K * n);
+    std::vector<double> val((size_t)K * n), diag(n);
+    MPS_HIP(hipMemcpy(cnt.data(), ctx->aCnt, sizeof(int) * n, hipMemcpyDeviceToHost));
+    MPS_HIP(hipMemcpy(canon.data(), ctx->I[ICANON], sizeof(int) * n, hipMemcpyDeviceToHost));
+    MPS_HIP(hipMemcpy(col.data(), ctx->aCol, sizeof(int) * (size_t)K * n, hipMemcpyDeviceToHost));
+    MPS_HIP(hipMemcpy(val.data(), ctx->aVal, sizeof(double) * (size_t)K * n, hipMemcpyDeviceToHost));
+    MPS_HIP(hipMemcpy(diag.data(), ctx->aDiag, sizeof(double) * n, hipMemcpyDeviceToHost));
+    const int nl = ctx->nLiveHost;
+    std::vector<int> slot(n, -1);
+    for (int k = 0; k < n; k++)
+        if (canon[k] >= 0 && canon[k] < n) slot[canon[k]] = k;
+    int nnz = 0;
+    for (int r = 0; r < nl; r++) nnz += cnt[slot[r]] + 1;
+    if (nnz_out) *nnz_out = nnz;
+    if (!row_ptr && !cols && !vals) return MPS_OK;
+    int p = 0;
+    for (int r = 0; r < nl; r++) {
+        const int k = slot[r];
+        if (row_ptr) row_ptr[r] = p;
+        for (int c = 0; c < cnt[k]; c++) {
+            if (cols) cols[p] = canon[col[(size_t)c * n + k]];
+            if (vals) vals[p] = val[(size_t)c * n + k];
+            p++;
+        }
+        if (cols) cols[p] = r;
+        if (vals) vals[p] = diag[k];
+        p++;
+    }
+    if (row_ptr) row_ptr[nl] = p;
+    return MPS_OK;
+}
+
+int mps_init(mps_ctx* ctx) {
+    CHECK_CTX(ctx);
+    hipSetDevice(ctx->dev);
+    return runInit(ctx);
+}
+
+int mps_step(mps_ctx* ctx) {
+    CHECK_CTX(ctx);
+    hipSetDevice(ctx->dev);
+    return runStep(ctx);
+}
+
+int mps_steps(mps_ctx* ctx, int k) {
+    CHECK_CTX(ctx);
+    hipSetDevice(ctx->dev);
+    for (int t = 0; t < k; t++) {
+        int rc = runStep(ctx);
+        if (rc) return rc;
+    }
+    return MPS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,272 @@
+// Host-side file formats: binary / ASCII STL (PolygonMesh::readPolygonMeshFile,
+// src/PolygonMesh.cpp:42-108), the .grid particle file
+// (MpsInputOutput::readMpsParticleFile, src/MpsInputOutput.cpp:645-857) and the
+// ASCII VTU particle output (MpsInputOutput::writeVtuAscii,
+// src/MpsInputOutput.cpp:1313-1579).
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "mps_gpu.h"
+#include "mps_internal.h"
+
+namespace {
+
+struct Tri {
+    float v[3][3];
+};
+
+bool readStlBinary(FILE* f, std::vector<Tri>& tris) {
+    if (fseek(f, 80, SEEK_SET) != 0) return false;
+    uint32_t n = 0;
+    if (fread(&n, 4, 1, f) != 1) return false;
+    fseek(f, 0, SEEK_END);
+    long size = ftell(f);
+    if (size != 84 + 50L * (long)n) return false;
+    fseek(f, 84, SEEK_SET);
+    tris.resize(n);
+    for (uint32_t t = 0; t < n; t++) {
+        float rec[12];
+        uint16_t attr;
+        if (fread(rec, 4, 12, f) != 12 || fread(&attr, 2, 1, f) != 1) return false;
+        for (int v = 0; v < 3; v++)
+            for (int c = 0; c < 3; c++) tris[t].v[v][c] = rec[3 + v * 3 + c];
+    }
+    return true;
+}
+
+bool readStlAscii(FILE* f, std::vector<Tri>& tris) {
+    fseek(f, 0, SEEK_SET);
+    char word[256];
+    tris.clear();
+    Tri cur;
+    int nv = 0;
+    while (fscanf(f, "%255s", word) == 1) {
+        if (strcmp(word, "vertex") == 0) {
+            float x, y, z;
+            if (fscanf(f, "%f %f %f", &x, &y, &z) != 3) return false;
+            if (nv < 3) { cur.v[nv][0] = x; cur.v[nv][1] = y; cur.v[nv][2] = z; }
+            nv++;
+        } else if (strcmp(word, "endfacet") == 0) {
+            if (nv != 3) return false;
+            tris.push_back(cur);
+            nv = 0;
+        }
+    }
+    return !tris.empty();
+}
+
+}  // namespace
+
+extern "C" {
+
+int mps_mesh_read_stl(const char* path, int* nv_out, double* V, int* nf_out, int* F, double* N) {
+    if (!path || !nv_out || !nf_out) return MPS_E_ARG;
+    FILE* f = fopen(path, "rb");
+    if (!f) { mpsgpu::setGlobalError(std::string("cannot open ") + path); return MPS_E_IO; }
+    std::vector<Tri> tris;
+    bool ok = readStlBinary(f, tris);
+    if (!ok) ok = readStlAscii(f, tris);
+    fclose(f);
+    if (!ok) { mpsgpu::setGlobalError(std::string("cannot parse STL ") + path); return MPS_E_IO; }
+    // merge duplicate vertices: STL coordinates are float32, so the 2e-15
+    // tolerance of the reference merges exactly the identical ones
+    std::map<std::tuple<float, float, float>, int> idx;
+    std::vector<double> verts;
+    std::vector<int> faces;
+    for (const Tri& t : tris)
+        for (int v = 0; v < 3; v++) {
+            auto key = std::make_tuple(t.v[v][0], t.v[v][1], t.v[v][2]);
+            auto it = idx.find(key);
+            int id;
+            if (it == idx.end()) {
+                id = (int)(verts.size() / 3);
+                idx[key] = id;
+                for (int c = 0; c < 3; c++) verts.push_back((double)t.v[v][c]);
+            } else {
+                id = it->second;
+            }
+            faces.push_back(id);
+        }
+    const int nv = (int)(verts.size() / 3), nf = (int)tris.size();
+    if (V && *nv_out >= nv) memcpy(V, verts.data(), sizeof(double) * verts.size());
+    if (F && *nf_out >= nf) memcpy(F, faces.data(), sizeof(int) * faces.size());
+    if (N && *nf_out >= nf) {
+        // unit face normals (v1 - v0) x (v2 - v0); zero for degenerate faces
+        for (int t = 0; t < nf; t++) {
+            const double* a = &verts[faces[t * 3] * 3];
+            const double* b = &verts[faces[t * 3 + 1] * 3];
+            const double* c = &verts[faces[t * 3 + 2] * 3];
+            const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+            const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+            double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+            const double r = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            for (int k = 0; k < 3; k++) N[t * 3 + k] = (r == 0) ? 0.0 : n[k] / r;
+        }
+    }
+    *nv_out = nv;
+    *nf_out = nf;
+    return MPS_OK;
+}
+
+int mps_read_grid(const char* path, int* n_out, int* type, double* pos, double* vel, double* press,
+                  double* press_avg) {
+    if (!path || !n_out) return MPS_E_ARG;
+    FILE* f = fopen(path, "r");
+    if (!f) { mpsgpu::setGlobalError(std::string("cannot open ") + path); return MPS_E_IO; }
+    int zero = 0, n = 0;
+    if (fscanf(f, "%d", &zero) != 1 || fscanf(f, "%d", &n) != 1 || n < 0) {
+        fclose(f);
+        mpsgpu::setGlobalError("bad grid header");
+        return MPS_E_IO;
+    }
+    if (!type) { fclose(f); *n_out = n; return MPS_OK; }
+    if (*n_out < n) { fclose(f); mpsgpu::setGlobalError("grid arrays too small"); return MPS_E_ARG; }
+    for (int i = 0; i < n; i++) {
+        int t;
+        double b[8];
+        if (fscanf(f, "%d %lf %lf %lf %lf %lf %lf %lf %lf", &t, &b[0], &b[1], &b[2], &b[3], &b[4], &b[5], &b[6], &b[7]) != 9) {
+            fclose(f);
+            mpsgpu::setGlobalError("truncated grid file");
+            return MPS_E_IO;
+        }
+        type[i] = t;
+        for (int c = 0; c < 3; c++) { pos[i * 3 + c] = b[c]; vel[i * 3 + c] = b[3 + c]; }
+        press[i] = b[6];
+        press_avg[i] = b[7];
+    }
+    fclose(f);
+    *n_out = n;
+    return MPS_OK;
+}
+
+int mps_write_grid(const char* path, int n, const int* type, const double* pos, const double* vel,
+                   const double* press, const double* press_avg) {
+    if (!path || n < 0 || !type || !pos || !vel || !press || !press_avg) return MPS_E_ARG;
+    FILE* f = fopen(path, "w");
+    if (!f) { mpsgpu::setGlobalError(std::string("cannot write ") + path); return MPS_E_IO; }
+    fprintf(f, "0\n%d\n", n);
+    for (int i = 0; i < n; i++)
+        fprintf(f, "%d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", type[i], pos[i * 3], pos[i * 3 + 1],
+                pos[i * 3 + 2], vel[i * 3], vel[i * 3 + 1], vel[i * 3 + 2], press[i], press_avg[i]);
+    fclose(f);
+    return MPS_OK;
+}
+
+int mps_write_vtu(mps_ctx* ctx, const char* path, int flags) {
+    if (!ctx || !path) return MPS_E_ARG;
+    double np = 0, capd = 0;
+    int rc = mps_get_scalar(ctx, "num_particles", &np);
+    if (rc) return rc;
+    if ((rc = mps_get_scalar(ctx, "capacity", &capd))) return rc;
+    mps_params P;
+    mps_get_params(ctx, &P);
+    const int n = (int)np;     // live particles come first in reference order
+    const int N = (int)capd;   // download buffers hold every slot
+    std::vector<double> pos, vel, press, pndi, small, npcd, pnorm, conc, gradc;
+    std::vector<int> bc, nn, type, nmt;
+    pos.resize(3 * N); vel.resize(3 * N); press.resize(N); bc.resize(N);
+    if ((rc = mps_download(ctx, "pos", pos.data(), pos.size()))) return rc;
+    if ((rc = mps_download(ctx, "vel", vel.data(), vel.size()))) return rc;
+    if ((rc = mps_download(ctx, "press", press.data(), press.size()))) return rc;
+    if ((rc = mps_download(ctx, "bc", bc.data(), bc.size()))) return rc;
+    FILE* fp = fopen(path, "w");
+    if (!fp) return MPS_E_IO;
+    fprintf(fp, "<?xml version='1.0' encoding='UTF-8'?>\n");
+    fprintf(fp, "<VTKFile xmlns='VTK' byte_order='LittleEndian' version='0.1' type='UnstructuredGrid'>\n");
+    fprintf(fp, "  <UnstructuredGrid>\n");
+    fprintf(fp, "    <Piece NumberOfCells='%d' NumberOfPoints='%d'>\n", n, n);
+    fprintf(fp, "      <Points>\n");
+    fprintf(fp, "        <DataArray type='Float32' Name='Position' NumberOfComponents='3' format='ascii'>\n          ");
+    for (int i = 0; i < n; i++) fprintf(fp, "%lf %lf %lf ", pos[i * 3], pos[i * 3 + 1], pos[i * 3 + 2]);
+    fprintf(fp, "\n        </DataArray>\n");
+    fprintf(fp, "      </Points>\n");
+    fprintf(fp, "      <PointData>\n");
+    fprintf(fp, "        <DataArray type='Float32' Name='Velocity' NumberOfComponents='3' format='ascii'>\n");
+    for (int i = 0; i < n; i++) fprintf(fp, "%f %f %f ", (float)vel[i * 3], (float)vel[i * 3 + 1], (float)vel[i * 3 + 2]);
+    fprintf(fp, "\n        </DataArray>\n");
+    fprintf(fp, "        <DataArray type='Float32' Name='pressure' format='ascii'>\n");
+    for (int i = 0; i < n; i++) fprintf(fp, "%f ", (float)press[i]);
+    fprintf(fp, "\n        </DataArray>\n");
+    fprintf(fp, "        <DataArray type='Int32' Name='BC' format='ascii'>\n");
+    for (int i = 0; i < n; i++) fprintf(fp, "%d ", bc[i]);
+    fprintf(fp, "\n        </DataArray>\n");
+    if (flags & 1) {
+        pndi.resize(N); small.resize(N);
+        mps_download(ctx, "pndi", pndi.data(), N);
+        mps_download(ctx, "pnd_small", small.data(), N);
+        fprintf(fp, "        <DataArray type='Float32' Name='pnd' format='ascii'>\n");
+        for (int i = 0; i < n; i++) fprintf(fp, "%f ", (float)pndi[i]);
+        fprintf(fp, "\n        </DataArray>\n");
+        fprintf(fp, "        <DataArray type='Float32' Name='pndSmall' format='ascii'>\n");
+        for (int i = 0; i < n; i++) fprintf(fp, "%f ", (float)small[i]);
+        fprintf(fp, "\n        </DataArray>\n");
+    }
+    if (flags & 2) {
+        nn.resize(N);
+        mps_download(ctx, "num_neigh", nn.data(), N);
+        fprintf(fp, "        <DataArray type='Int32' Name='nNeigh' format='ascii'>\n");
+        for (int i = 0; i < n; i++) fprintf(fp, "%d ", nn[i]);
+        fprintf(fp, "\n        </DataArray>\n");
+    }
+    if (flags & 4) {
+        npcd.resize(3 * N); pnorm.resize(3 * N);
+        mps_download(ctx, "npcd", npcd.data(), 3 * N);
+        mps_download(ctx, "polygon_normal", pnorm.data(), 3 * N);
+        fprintf(fp, "        <DataArray type='Float32' Name='npcdDeviation' NumberOfComponents='3' format='ascii'>\n");
+        for (int i = 0; i < n; i++) fprintf(fp, "%f %f %f ", (float)npcd[i * 3], (float)npcd[i * 3 + 1], (float)npcd[i * 3 + 2]);
+        fprintf(fp, "\n        </DataArray>\n");
+        fprintf(fp, "        <DataArray type='Float32' Name='polygonNormal' NumberOfComponents='3' format='ascii'>\n");
+        for (int i = 0; i < n; i++) fprintf(fp, "%f %f %f ", (float)pnorm[i * 3], (float)pnorm[i * 3 + 1], (float)pnorm[i * 3 + 2]);
+        fprintf(fp, "\n        </DataArray>\n");
+    }
+    if (flags & 8) {
+        conc.resize(N); gradc.resize(3 * N);
+        mps_download(ctx, "conc", conc.data(), N);
+        mps_download(ctx, "grad_conc", gradc.data(), 3 * N);
+        fprintf(fp, "        <DataArray type='Float32' Name='concentration' format='ascii'>\n");
+        for (int i = 0; i < n; i++) fprintf(fp, "%f ", (float)conc[i]);
+        fprintf(fp, "\n        </DataArray>\n");
+        fprintf(fp, "        <DataArray type='Float32' Name='GradCi' NumberOfComponents='3' format='ascii'>\n");
+        for (int i = 0; i < n; i++) fprintf(fp, "%f %f %f ", (float)gradc[i * 3], (float)gradc[i * 3 + 1], (float)gradc[i * 3 + 2]);
+        fprintf(fp, "\n        </DataArray>\n");
+    }
+    if (flags & 16) {
+        type.resize(N);
+        mps_download(ctx, "type", type.data(), N);
+        fprintf(fp, "        <DataArray type='Int32' Name='ParticleType' format='ascii'>\n");
+        for (int i = 0; i < n; i++) fprintf(fp, "%d ", type[i]);
+        fprintf(fp, "\n        </DataArray>\n");
+        if (P.wall_type == MPS_WALL_POLYGON) {
+            nmt.resize(N);
+            mps_download(ctx, "near_mesh_type", nmt.data(), N);
+            fprintf(fp, "        <DataArray type='Int32' Name='nearMeshType' format='ascii'>\n");
+            for (int i = 0; i < n; i++) fprintf(fp, "%d ", nmt[i]);
+            fprintf(fp, "\n        </DataArray>\n");
+        }
+    }
+    fprintf(fp, "      </PointData>\n");
+    fprintf(fp, "      <Cells>\n");
+    fprintf(fp, "        <DataArray type='Int32' Name='connectivity' format='ascii'>\n");
+    for (int i = 0; i < n; i++) fprintf(fp, "%d ", i);
+    fprintf(fp, "\n        </DataArray>\n");
+    fprintf(fp, "        <DataArray type='Int32' Name='offsets' format='ascii'>\n");
+    for (int i = 0; i < n; i++) fprintf(fp, "%d ", i + 1);
+    fprintf(fp, "\n        </DataArray>\n");
+    fprintf(fp, "        <DataArray type='UInt8' Name='types' format='ascii'>\n");
+    for (int i = 0; i < n; i++) fprintf(fp, "1 ");
+    fprintf(fp, "\n        </DataArray>\n");
+    fprintf(fp, "      </Cells>\n");
+    fprintf(fp, "    </Piece>\n");
+    fprintf(fp, "  </UnstructuredGrid>\n");
+    fprintf(fp, "</VTKFile>\n");
+    fclose(fp);
+    return MPS_OK;
+}
+
+}  // extern "C"

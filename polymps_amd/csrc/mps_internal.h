@@ -1,0 +1,159 @@
+// Internal declarations of the MI355X MPS library (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "mps_gpu.h"
+
+namespace mpsgpu {
+
+void setGlobalError(const std::string& e);
+void paramsDefaults(mps_params* p);
+int paramsFromJson(const char* path, mps_params* p, std::string& grid, std::string& mesh, std::string& err);
+int paramsDerive(mps_params* p, int n, const double* pos, std::string& err);
+
+// ---------------------------------------------------------------- fields
+// Per-particle double fields.  The first NPD are persistent across steps and
+// are gathered into bucket order by every sort; the rest are recomputed
+// inside a step before they are read.
+enum DField {
+    // persistent
+    DX, DY, DZ, DU, DV, DW, DMX, DMY, DMZ, DPRESS, DPNDI, DPPEX, DPAVG, DFWX, DFWY, DFWZ, DPNDSKI,
+    NPD,
+    // per step
+    DAX = NPD, DAY, DAZ, DASX, DASY, DASZ, DNPX, DNPY, DNPZ, DNPCD2, DPNDKI, DPNDSMALL, DCONC, DDIVU,
+    DDIFF, DAUX1, DAUX2, DPNDWALL, DRIW2, DDEVN, DWPX, DWPY, DWPZ, DPNX, DPNY, DPNZ, DGCX, DGCY, DGCZ,
+    DCM0, DCM1, DCM2, DCM3, DCM4, DCM5, DCM6, DCM7, DCM8, DDVX, DDVY, DDVZ, DPPEB,
+    ND
+};
+// Per-particle int fields; the first NPI are persistent.
+enum IField { ITYPE, ICANON, IBC, INNEIGH, NPI, INNWALL = NPI, INEARWALL, INEARMESH, NI };
+
+// device scalar slots (double)
+enum DScalar { S_VELMAX, S_ABSNEW, S_ABSOLD, S_RHSN2, S_THR, S_RN2, S_ALPHA, S_BETA, S_PAP, S_BN2, S_TRUE_RN2,
+               S_RELTOL, NDS };
+// device scalar slots (int)
+enum IScalar { I_NLIVE, I_OUTCNT, I_OVERFLOW, I_CG_ITER, I_CG_DONE, I_COMPACT_OVF, I_LIST_MAX, NIS };
+
+constexpr int kWallTable = 50;
+constexpr int kRedBlocks = 1024;   // partial-sum slots of a reduction
+
+// Everything a kernel may touch, passed by value.
+struct KArgs {
+    mps_params P;
+    int N;                 // slots (capacity)
+    int nB;                // buckets
+    double* D[ND];
+    int* I[NI];
+    const int* cellStart;  // nB + 2 (last entry = N)
+    const int* nbr;        // column-major [K][N]
+    const int* nbrCnt;
+    int K;
+    const double* mV;      // mesh
+    const int* mF;
+    const double* mN;
+    int nf;
+    const double* wtab;    // xPND[50] xNeigh[50] Zij[50] nNeigh[50] (as double)
+    double* ds;            // device scalars
+    int* is;
+    double* red;           // reduction partials [kRedBlocks * 4]
+};
+
+struct Ctx {
+    mps_params P;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    int N = 0;            // slots = initial particle count
+    int nLiveHost = 0;
+    int nB = 0;
+    double* D[ND] = {};
+    double* Dalt[NPD] = {};
+    int* I[NI] = {};
+    int* Ialt[NPI] = {};
+    // buckets
+    int* cellStart = nullptr;   // nB + 2
+    int* cellCount = nullptr;   // nB + 1
+    int* key = nullptr;         // N
+    int* rank = nullptr;        // N
+    int* order = nullptr;       // N
+    int* order2 = nullptr;      // N
+    int* slotOf = nullptr;      // N (canonical -> slot)
+    int* scanTmp = nullptr;
+    int* outList = nullptr;     // compaction candidates
+    // neighbour list
+    int* nbr = nullptr;
+    int* nbrCnt = nullptr;
+    int K = 0;
+    bool listValid = false;
+    // mesh
+    double* mV = nullptr;
+    int* mF = nullptr;
+    double* mN = nullptr;
+    int nv = 0, nf = 0;
+    double* wtab = nullptr;
+    std::vector<double> hxPND, hxNeigh, hZij;
+    std::vector<int> hnNeigh;
+    bool hasMesh = false;
+    // PPE
+    double* aVal = nullptr;     // [K][N]
+    int* aCol = nullptr;        // [K][N]
+    int* aCnt = nullptr;        // N
+    double* aDiag = nullptr;    // N
+    float* aVal32 = nullptr;    // [K][N] (mixed precision)
+    float* aDiagInv32 = nullptr;
+    double* cgR = nullptr;
+    double* cgP = nullptr;
+    double* cgZ = nullptr;
+    double* cgQ = nullptr;
+    double* cgInvD = nullptr;
+    double* cgX = nullptr;      // refinement correction
+    float* fR = nullptr;
+    float* fP = nullptr;
+    float* fZ = nullptr;
+    float* fQ = nullptr;
+    float* fX = nullptr;
+    // scalars
+    double* ds = nullptr;
+    int* is = nullptr;
+    double* red = nullptr;
+    double* hostScalars = nullptr;   // pinned
+    int* hostInts = nullptr;         // pinned
+    // host mirrors / stats
+    double velMaxHost = 0.0;
+    double timeCurrent = 0.0;
+    int numIter = 0;
+    int solverIter = 0;
+    double solverErr = 0.0;
+    double trueResidual = 0.0;
+    double ppeMs = 0.0;
+    int fallbackUsed = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+
+    KArgs args() const;
+};
+
+#define MPS_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t _e = (call);                                                         \
+        if (_e != hipSuccess) {                                                         \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(_e);               \
+            return MPS_E_HIP;                                                           \
+        }                                                                               \
+    } while (0)
+
+inline int gridFor(int n, int block) { return (n + block - 1) / block; }
+
+// launchers (defined in the .hip translation units)
+int launchBucketSort(Ctx* ctx);
+int launchBuildList(Ctx* ctx);
+int launchOp(Ctx* ctx, const char* op);
+int ppeAssemble(Ctx* ctx);
+int ppeSolve(Ctx* ctx);
+int launchCompaction(Ctx* ctx);
+
+}  // namespace mpsgpu

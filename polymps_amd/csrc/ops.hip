@@ -1,0 +1,1453 @@
+// Particle operators of the MPS time step on gfx950.
+//
+// One thread per particle slot; slots are bucket-sorted, so the threads of a
+// wavefront work on neighbouring particles and share the buckets they read.
+// Neighbours come either from a bucket walk (walkCells) or from the per-step
+// neighbour list (listNbrs); both yield the reference's visiting order, so
+// FP64 sums are accumulated in the reference's order.
+#include "kern_common.h"
+
+namespace mpsgpu {
+
+constexpr int kBlock = 256;
+
+template <bool LIST, class F>
+__device__ __forceinline__ void forNbrs(const KArgs& a, int i, double x, double y, double z, F&& f) {
+    if (LIST) {
+        listNbrs(a, i, f);
+    } else {
+        walkCells(a, x, y, z, [&](int j) {
+            if (j != i) f(j);
+        });
+    }
+}
+
+#define PX(k) a.D[DX][k]
+#define PY(k) a.D[DY][k]
+#define PZ(k) a.D[DZ][k]
+
+// --------------------------------------------------- neighbour list build
+// Candidates within the largest operator radius, j != i, walk order.
+__global__ void kBuildList(KArgs a, int* nbr, int* cnt, double r2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    if (!live(a, i)) { cnt[i] = 0; return; }
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    int c = 0;
+    walkCells(a, xi, yi, zi, [&](int j) {
+        if (j == i) return;
+        const double v0 = PX(j) - xi, v1 = PY(j) - yi, v2 = PZ(j) - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        if (d2 < r2) {
+            if (c < a.K) nbr[(size_t)c * a.N + i] = j;
+            c++;
+        }
+    });
+    if (c > a.K) {
+        atomicMax(&a.is[I_OVERFLOW], c);
+        c = a.K;
+    }
+    atomicMax(&a.is[I_LIST_MAX], c);
+    cnt[i] = c;
+}
+
+// ------------------------------------------------------------- viscosity
+// MpsViscosity::calcViscosity (src/MpsViscosity.cpp:20-102)
+template <bool LIST>
+__global__ void kViscosity(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    const double meu = P.knm_vs1 * P.dns_fl1, rho = P.dns_fl1;
+    double nuF;
+    if ((meu + meu) > KEPS) nuF = 2.0 * meu * meu / (meu + meu);
+    else nuF = 0.0;
+    nuF = nuF / rho;
+    const double nuW = 2.0 * meu / rho;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
+    const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+    double ax = 0, ay = 0, az = 0;
+    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_l2 && (d2 < dm2 || pw)) {
+            const double wL = wgt(sqrt(d2), P.re_l, P.weight_type);
+            const double nu = (a.I[ITYPE][j] == P.wall) ? nuW : nuF;
+            ax += (a.D[DU][j] - ux) * wL * nu;
+            ay += (a.D[DV][j] - uy) * wL * nu;
+            az += (a.D[DW][j] - uz) * wL * nu;
+        }
+    });
+    a.D[DAX][i] = P.coeff_visc_multiphase * ax;
+    a.D[DAY][i] = P.coeff_visc_multiphase * ay;
+    a.D[DAZ][i] = P.coeff_visc_multiphase * az;
+}
+
+// MpsParticle::addGravity (src/MpsParticle.cpp:111-130)
+__global__ void kAddGravity(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const double ax = a.D[DAX][i] + a.P.gravity[0];
+    const double ay = a.D[DAY][i] + a.P.gravity[1];
+    const double az = a.D[DAZ][i] + a.P.gravity[2];
+    a.D[DAX][i] = ax; a.D[DAY][i] = ay; a.D[DAZ][i] = az;
+    a.D[DASX][i] = ax; a.D[DASY][i] = ay; a.D[DASZ][i] = az;
+}
+
+// MpsParticleVelPos::predictionVelocityPosition (src/MpsParticleVelPos.cpp:20-50)
+__global__ void kPrediction(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const double far = 10e8 * P.part_dist;
+    if (a.I[ITYPE][i] == P.fluid) {
+        const double u = a.D[DU][i] + a.D[DAX][i] * P.time_step;
+        const double v = a.D[DV][i] + a.D[DAY][i] * P.time_step;
+        const double w = a.D[DW][i] + a.D[DAZ][i] * P.time_step;
+        a.D[DU][i] = u; a.D[DV][i] = v; a.D[DW][i] = w;
+        a.D[DX][i] = a.D[DX][i] + u * P.time_step;
+        a.D[DY][i] = a.D[DY][i] + v * P.time_step;
+        a.D[DZ][i] = a.D[DZ][i] + w * P.time_step;
+    }
+    a.D[DAX][i] = 0.0; a.D[DAY][i] = 0.0; a.D[DAZ][i] = 0.0;
+    a.D[DDVX][i] = 0.0; a.D[DDVY][i] = 0.0; a.D[DDVZ][i] = 0.0;
+    a.D[DNPX][i] = 0.0; a.D[DNPY][i] = 0.0; a.D[DNPZ][i] = 0.0;
+    a.I[INNWALL][i] = 0;
+    a.I[INEARWALL][i] = 0;
+    a.D[DRIW2][i] = far;
+    if (P.wall_type == MPS_WALL_POLYGON) {
+        a.D[DMX][i] = far; a.D[DMY][i] = far; a.D[DMZ][i] = far;
+    }
+}
+
+// MpsParticleVelPos::correctionVelocityPosition (src/MpsParticleVelPos.cpp:53-97);
+// velMax = sqrt(max |u|^2) over fluid particles, reduced with an integer max
+// on the (non-negative) bit pattern so the result does not depend on order.
+__global__ void kCorrection(KArgs a, unsigned long long* vmax2bits) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v2 = 0.0;
+    if (live(a, i) && a.I[ITYPE][i] == a.P.fluid) {
+        const mps_params& P = a.P;
+        const double ax = a.D[DAX][i], ay = a.D[DAY][i], az = a.D[DAZ][i];
+        const double u = a.D[DU][i] + ax * P.time_step;
+        const double v = a.D[DV][i] + ay * P.time_step;
+        const double w = a.D[DW][i] + az * P.time_step;
+        a.D[DU][i] = u; a.D[DV][i] = v; a.D[DW][i] = w;
+        a.D[DX][i] = a.D[DX][i] + ax * P.time_step * P.time_step;
+        a.D[DY][i] = a.D[DY][i] + ay * P.time_step * P.time_step;
+        a.D[DZ][i] = a.D[DZ][i] + az * P.time_step * P.time_step;
+        a.D[DAX][i] = 0.0; a.D[DAY][i] = 0.0; a.D[DAZ][i] = 0.0;
+        v2 = u * u + v * v + w * w;
+    }
+    v2 = waveMax(v2);
+    if ((threadIdx.x & 63) == 0 && v2 > 0.0) atomicMax(vmax2bits, (unsigned long long)__double_as_longlong(v2));
+}
+
+__global__ void kVelMaxFinish(KArgs a, const unsigned long long* vmax2bits) {
+    a.ds[S_VELMAX] = sqrt(__longlong_as_double((long long)*vmax2bits));
+}
+
+// --------------------------------------------------------------- domain
+// MpsParticle::checkParticleOutDomain (src/MpsParticle.cpp:234-402).  Flag
+// pass collects the reference indices of fluid particles outside the domain
+// less one bucket; one thread then replays the reference's compaction in
+// ascending reference order (a slot receives the data of the last live
+// particle, the last particle becomes a ghost).
+__global__ void kOutFlag(KArgs a, int* outList, int cap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    if (a.I[ITYPE][i] != P.fluid) return;
+    const double x = PX(i), y = PY(i), z = PZ(i);
+    const bool out = x > P.domain_max[0] - P.bucket_side || x < P.domain_min[0] + P.bucket_side ||
+                     y > P.domain_max[1] - P.bucket_side || y < P.domain_min[1] + P.bucket_side ||
+                     (P.dim == 3 && (z > P.domain_max[2] - P.bucket_side || z < P.domain_min[2] + P.bucket_side));
+    if (!out) return;
+    const int s = atomicAdd(&a.is[I_OUTCNT], 1);
+    if (s < cap) outList[s] = a.I[ICANON][i];
+    else a.is[I_COMPACT_OVF] = 1;
+}
+
+__global__ void kCompact(KArgs a, int* outList, int cap, const int* slotOf) {
+    int n = a.is[I_OUTCNT];
+    if (n == 0) return;
+    if (n > cap) n = cap;
+    for (int p = 1; p < n; p++) {   // insertion sort (few entries)
+        const int v = outList[p];
+        int q = p - 1;
+        while (q >= 0 && outList[q] > v) { outList[q + 1] = outList[q]; q--; }
+        outList[q + 1] = v;
+    }
+    const mps_params& P = a.P;
+    int newN = a.is[I_NLIVE];
+    for (int p = 0; p < n; p++) {
+        const int c = outList[p];
+        if (c >= newN) break;
+        const int L = newN - 1;
+        const int kc = slotOf[c], kl = slotOf[L];
+        if (kc != kl) {
+            for (int f = 0; f < ND; f++) a.D[f][kc] = a.D[f][kl];
+            for (int f = 0; f < NI; f++)
+                if (f != ICANON) a.I[f][kc] = a.I[f][kl];
+        }
+        a.I[ITYPE][kl] = P.ghost;
+        a.I[IBC][kl] = P.other;
+        a.I[INEARWALL][kl] = 0;
+        a.I[INEARMESH][kl] = 0;
+        a.D[DRIW2][kl] = 10e8 * P.part_dist;
+        a.D[DU][kl] = 0.0; a.D[DV][kl] = 0.0; a.D[DW][kl] = 0.0;
+        a.D[DPRESS][kl] = 0.0;
+        a.D[DX][kl] = P.domain_max[0] - P.part_dist;
+        a.D[DY][kl] = P.domain_max[1] - P.part_dist;
+        a.D[DZ][kl] = (P.dim == 2) ? 0.0 : P.domain_max[2] - P.part_dist;
+        newN--;
+    }
+    a.is[I_NLIVE] = newN;
+    a.is[I_OUTCNT] = 0;
+}
+
+// ------------------------------------------------------------ collisions
+// MpsParticleCollision::checkParticleCollisions (src/MpsParticleCollision.cpp:22-119)
+template <bool LIST>
+__global__ void kCollisionPC(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    if (a.I[ITYPE][i] != P.fluid) return;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    const double mi = P.dns_fl1;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
+    const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+    double dux = 0, duy = 0, duz = 0;
+    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.dist_collision_limit2 && (d2 < dm2 || pw)) {
+            double fDT = (ux - a.D[DU][j]) * v0 + (uy - a.D[DV][j]) * v1 + (uz - a.D[DW][j]) * v2;
+            if (fDT > 0.0) {
+                const bool fj = a.I[ITYPE][j] == P.fluid;
+                const double mj = fj ? P.dns_fl1 : P.dns_wall;
+                fDT *= P.restitution_collision * mj / (mi + mj) / d2;
+                if (fj) {
+                    dux -= v0 * fDT; duy -= v1 * fDT; duz -= v2 * fDT;
+                } else {
+                    dux -= 2.0 * v0 * fDT; duy -= 2.0 * v1 * fDT; duz -= 2.0 * v2 * fDT;
+                }
+            }
+        }
+    });
+    a.D[DDVX][i] = dux; a.D[DDVY][i] = duy; a.D[DDVZ][i] = duz;
+}
+
+// checkDynamicParticleCollisions (src/MpsParticleCollision.cpp:124-292)
+template <bool LIST>
+__global__ void kCollisionDPC(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    if (a.I[ITYPE][i] != P.fluid) return;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    const double Wij5 = 0.5 * 0.5 * 0.5 * 0.5 * 3.0;
+    const double gmod = sqrt(P.gravity[0] * P.gravity[0] + P.gravity[1] * P.gravity[1] + P.gravity[2] * P.gravity[2]);
+    const double mi = P.dns_fl1, rhoi = P.dns_fl1;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
+    const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+    const double pi = a.D[DPRESS][i];
+    double dux = 0, duy = 0, duz = 0;
+    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.part_dist * P.part_dist && (d2 < dm2 || pw)) {
+            const bool fj = a.I[ITYPE][j] == P.fluid;
+            const double mj = fj ? P.dns_fl1 : P.dns_wall;
+            const double dst = sqrt(d2);
+            double Wij = 0.0;
+            if (dst > KEPS) {
+                const double q1 = 1.0 - dst * P.inv_part_dist;
+                const double q2 = 4.0 * dst * P.inv_part_dist + 1.0;
+                Wij = q1 * q1 * q1 * q1 * q2;
+            }
+            const double chi = sqrt(Wij / Wij5);
+            const double kappa = (dst < 0.5 * P.part_dist) ? 1.0 : chi;
+            double fDT = (ux - a.D[DU][j]) * v0 + (uy - a.D[DV][j]) * v1 + (uz - a.D[DW][j]) * v2;
+            if (fDT > 0.0) {
+                fDT *= kappa * 2.0 * mj / (mi + mj) / d2;
+                if (fj) {
+                    dux -= v0 * fDT; duy -= v1 * fDT; duz -= v2 * fDT;
+                } else {
+                    dux -= 2.0 * v0 * fDT; duy -= 2.0 * v1 * fDT; duz -= 2.0 * v2 * fDT;
+                }
+            } else {
+                const double pmax = 2.0 / 3.0 * rhoi * gmod * 0.2;
+                const double pmin = rhoi * gmod * P.part_dist;
+                const double ptil = fmax(fmin(P.lambda_collision * fabs(pi + a.D[DPRESS][j]), P.lambda_collision * pmax), pmin);
+                const double pb = ptil * chi;
+                const double rep = P.time_step / mi * chi * pb / d2;
+                if (fj) {
+                    dux -= v0 * rep; duy -= v1 * rep; duz -= v2 * rep;
+                } else {
+                    dux -= 2.0 * v0 * rep; duy -= 2.0 * v1 * rep; duz -= 2.0 * v2 * rep;
+                }
+            }
+        }
+    });
+    a.D[DDVX][i] = dux; a.D[DDVY][i] = duy; a.D[DDVZ][i] = duz;
+}
+
+__global__ void kApplyDvel(KArgs a, int movePos) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    if (a.I[ITYPE][i] == a.P.fluid) {
+        const double dx = a.D[DDVX][i], dy = a.D[DDVY][i], dz = a.D[DDVZ][i];
+        a.D[DU][i] = a.D[DU][i] + dx;
+        a.D[DV][i] = a.D[DV][i] + dy;
+        a.D[DW][i] = a.D[DW][i] + dz;
+        if (movePos) {
+            a.D[DX][i] = a.D[DX][i] + dx * a.P.time_step;
+            a.D[DY][i] = a.D[DY][i] + dy * a.P.time_step;
+            a.D[DZ][i] = a.D[DZ][i] + dz * a.P.time_step;
+        }
+    }
+    a.D[DDVX][i] = 0.0; a.D[DDVY][i] = 0.0; a.D[DDVZ][i] = 0.0;
+}
+
+// ------------------------------------------------------------ mesh walls
+// Closest point on triangle abc to p (Ericson, Real-Time Collision Detection 5.1.5)
+__device__ __forceinline__ void closestPtTri(const double* p, const double* A, const double* B, const double* C, double* q) {
+    const double ab[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+    const double ac[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+    const double ap[3] = {p[0] - A[0], p[1] - A[1], p[2] - A[2]};
+    const double d1 = ab[0] * ap[0] + ab[1] * ap[1] + ab[2] * ap[2];
+    const double d2 = ac[0] * ap[0] + ac[1] * ap[1] + ac[2] * ap[2];
+    if (d1 <= 0.0 && d2 <= 0.0) { q[0] = A[0]; q[1] = A[1]; q[2] = A[2]; return; }
+    const double bp[3] = {p[0] - B[0], p[1] - B[1], p[2] - B[2]};
+    const double d3 = ab[0] * bp[0] + ab[1] * bp[1] + ab[2] * bp[2];
+    const double d4 = ac[0] * bp[0] + ac[1] * bp[1] + ac[2] * bp[2];
+    if (d3 >= 0.0 && d4 <= d3) { q[0] = B[0]; q[1] = B[1]; q[2] = B[2]; return; }
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        const double v = d1 / (d1 - d3);
+        q[0] = A[0] + v * ab[0]; q[1] = A[1] + v * ab[1]; q[2] = A[2] + v * ab[2];
+        return;
+    }
+    const double cp[3] = {p[0] - C[0], p[1] - C[1], p[2] - C[2]};
+    const double d5 = ab[0] * cp[0] + ab[1] * cp[1] + ab[2] * cp[2];
+    const double d6 = ac[0] * cp[0] + ac[1] * cp[1] + ac[2] * cp[2];
+    if (d6 >= 0.0 && d5 <= d6) { q[0] = C[0]; q[1] = C[1]; q[2] = C[2]; return; }
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        const double w = d2 / (d2 - d6);
+        q[0] = A[0] + w * ac[0]; q[1] = A[1] + w * ac[1]; q[2] = A[2] + w * ac[2];
+        return;
+    }
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        q[0] = B[0] + w * (C[0] - B[0]); q[1] = B[1] + w * (C[1] - B[1]); q[2] = B[2] + w * (C[2] - B[2]);
+        return;
+    }
+    const double denom = 1.0 / (va + vb + vc);
+    const double v = vb * denom, w = vc * denom;
+    q[0] = A[0] + ab[0] * v + ac[0] * w;
+    q[1] = A[1] + ab[1] * v + ac[1] * w;
+    q[2] = A[2] + ab[2] * v + ac[2] * w;
+}
+
+// PolygonMesh::closestPointPNDBoundaryAABB (src/PolygonMesh.cpp:585-715),
+// brute force over the faces; ties keep the lowest face index.
+__global__ void kClosestPoint(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const double p[3] = {PX(i), PY(i), PZ(i)};
+    double best = 1.79769313486231570e308, q[3], bq[3] = {0, 0, 0};
+    int bf = -1;
+    for (int f = 0; f < a.nf; f++) {
+        const int t0 = a.mF[f * 3], t1 = a.mF[f * 3 + 1], t2 = a.mF[f * 3 + 2];
+        closestPtTri(p, a.mV + t0 * 3, a.mV + t1 * 3, a.mV + t2 * 3, q);
+        const double e0 = p[0] - q[0], e1 = p[1] - q[1], e2 = p[2] - q[2];
+        const double d2 = e0 * e0 + e1 * e1 + e2 * e2;
+        if (d2 < best) { best = d2; bf = f; bq[0] = q[0]; bq[1] = q[1]; bq[2] = q[2]; }
+    }
+    if (bf < 0) return;
+    if (best < a.D[DRIW2][i]) {
+        a.D[DRIW2][i] = best;
+        a.D[DWPX][i] = bq[0]; a.D[DWPY][i] = bq[1]; a.D[DWPZ][i] = bq[2];
+        a.D[DMX][i] = p[0] + 2.0 * (bq[0] - p[0]);
+        a.D[DMY][i] = p[1] + 2.0 * (bq[1] - p[1]);
+        a.D[DMZ][i] = p[2] + 2.0 * (bq[2] - p[2]);
+        a.D[DPNX][i] = a.mN[bf * 3]; a.D[DPNY][i] = a.mN[bf * 3 + 1]; a.D[DPNZ][i] = a.mN[bf * 3 + 2];
+        a.I[INEARMESH][i] = 0;
+    }
+}
+
+// PolygonMesh::interpolateWij / interpolateNumNeighWall (src/PolygonMesh.cpp:533-581)
+__device__ __forceinline__ double interpZ(const double* xd, const double* yd, double re, double x) {
+    const double r = x / re;
+    int i = 0;
+    if (r >= xd[kWallTable - 2]) i = kWallTable - 2;
+    else while (r > xd[i + 1]) i++;
+    const double xL = xd[i], yL = yd[i], xR = xd[i + 1], yR = yd[i + 1];
+    const double dydx = (yR - yL) / (xR - xL);
+    return yL + dydx * (r - xL);
+}
+
+// PolygonMesh::updateParticlesNearPolygonMesh (src/PolygonMesh.cpp:718-795)
+__global__ void kUpdateNear(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    double niw = 0.0;
+    int nnw = 0;
+    const double x2 = a.D[DRIW2][i];
+    if (x2 < P.re_l2) {
+        a.I[INEARWALL][i] = 1;
+        const double x = sqrt(x2);
+        niw = interpZ(a.wtab, a.wtab + 2 * kWallTable, sqrt(P.re_s2), x);
+        nnw = (int)interpZ(a.wtab + kWallTable, a.wtab + 3 * kWallTable, sqrt(P.re_l2), x);
+    }
+    a.D[DPNDWALL][i] = niw;
+    a.I[INNWALL][i] = nnw;
+}
+
+// MpsPndNeigh::calcWallNPCD (src/MpsPndNeigh.cpp:105-195)
+template <bool LIST>
+__global__ void kWallNPCD(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    if (!(a.I[ITYPE][i] == P.fluid && a.I[INEARWALL][i])) return;
+    const double xi[3] = {PX(i), PY(i), PZ(i)};
+    const double xm[3] = {a.D[DMX][i], a.D[DMY][i], a.D[DMZ][i]};
+    double n[3], R[9], ns;
+    reflectSqrt(xi, xm, n, R, &ns);
+    double nx = 0, ny = 0, nz = 0;
+    forNbrs<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2) {
+            const double wS = wgt(sqrt(dm2), P.re_s, P.weight_type);
+            nx += w0 * wS; ny += w1 * wS; nz += w2 * wS;
+        }
+    });
+    const double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
+    const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+    if (dm2 < P.re_s2) {
+        const double wS = wgt(sqrt(dm2), P.re_s, P.weight_type);
+        nx += w0 * wS; ny += w1 * wS; nz += w2 * wS;
+    }
+    a.D[DNPX][i] = a.D[DNPX][i] + (R[0] * nx + R[1] * ny + R[2] * nz);
+    a.D[DNPY][i] = a.D[DNPY][i] + (R[3] * nx + R[4] * ny + R[5] * nz);
+    a.D[DNPZ][i] = a.D[DNPZ][i] + (R[6] * nx + R[7] * ny + R[8] * nz);
+}
+
+// ------------------------------------------------------ correction matrix
+// MpsVectorMatrix::correctionMatrix + inverseMatrix (src/MpsVectorMatrix.cpp:20-165)
+__device__ __forceinline__ void inverse3(double* M, int dim) {
+    double A[9], B[9];
+    for (int q = 0; q < 9; q++) A[q] = M[q];
+    if (dim == 2) { A[2] = A[5] = A[6] = A[7] = 0.0; A[8] = 1.0; }
+    for (int q = 0; q < 9; q++) B[q] = (q % 4 == 0) ? 1.0 : 0.0;
+    const double det = (A[0] * A[4] * A[8] + A[1] * A[5] * A[6] + A[2] * A[3] * A[7]) -
+                       (A[2] * A[4] * A[6] + A[1] * A[3] * A[8] + A[0] * A[5] * A[7]);
+    if (det <= KEPS) {
+        for (int q = 0; q < 9; q++) M[q] = (q % 4 == 0) ? 1.0 : 0.0;
+        return;
+    }
+    for (int k = 0; k < dim; k++)
+        for (int r = 0; r < dim; r++) {
+            if (r == k) continue;
+            const double m = A[r * 3 + k] / A[k * 3 + k];
+            for (int c = 0; c < dim; c++) {
+                A[r * 3 + c] -= m * A[k * 3 + c];
+                B[r * 3 + c] -= m * B[k * 3 + c];
+            }
+        }
+    for (int r = 0; r < dim; r++)
+        for (int c = 0; c < dim; c++) B[r * 3 + c] /= A[r * 3 + r];
+    for (int q = 0; q < 9; q++) M[q] = B[q];
+}
+
+template <bool LIST>
+__global__ void kCorrectionMatrix(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+    double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_s2 && (d2 < dm2 || pw)) {
+            const double wS = wgt(sqrt(d2), P.re_s, P.weight_type);
+            const double inv = 1.0 / d2;
+            M[0] += wS * v0 * v0 * inv; M[1] += wS * v0 * v1 * inv; M[2] += wS * v0 * v2 * inv;
+            M[3] += wS * v1 * v0 * inv; M[4] += wS * v1 * v1 * inv; M[5] += wS * v1 * v2 * inv;
+            M[6] += wS * v2 * v0 * inv; M[7] += wS * v2 * v1 * inv; M[8] += wS * v2 * v2 * inv;
+        }
+    });
+    for (int q = 0; q < 9; q++) M[q] *= -P.coeff_press_grad;
+    inverse3(M, (int)P.dim);
+    if (a.I[INNEIGH][i] < 4)
+        for (int q = 0; q < 9; q++) M[q] = (q % 4 == 0) ? 1.0 : 0.0;
+    for (int q = 0; q < 9; q++) a.D[DCM0 + q][i] = M[q];
+}
+
+// ------------------------------------------------------------------ PND
+// MpsPndNeigh::setInitialPndNumberOfNeighNPCD (src/MpsPndNeigh.cpp:21-101)
+template <bool LIST>
+__global__ void kInitialPnd(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+    double pnd = a.D[DPNDI][i], wSum = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
+    int nn = 0;
+    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_l2 && (d2 < dm2 || pw)) {
+            nn += 1;
+            if (d2 < P.re_s2) {
+                const double wS = wgt(sqrt(d2), P.re_s, P.weight_type);
+                pnd += wS;
+                c0 += v0 * wS; c1 += v1 * wS; c2 += v2 * wS;
+                wSum += wS;
+            }
+        }
+    });
+    pnd += a.D[DPNDWALL][i];
+    if (a.I[ITYPE][i] == P.wall) pnd = P.pnd_small_zero;
+    a.D[DPNDI][i] = pnd;
+    a.D[DPNDSMALL][i] = pnd;
+    a.D[DPNDKI][i] = pnd;
+    a.I[INNEIGH][i] = nn + a.I[INNWALL][i];
+    if (wSum > KEPS) { c0 /= pnd; c1 /= pnd; c2 /= pnd; }
+    a.D[DNPX][i] = c0; a.D[DNPY][i] = c1; a.D[DNPZ][i] = c2;
+    a.D[DNPCD2][i] = c0 * c0 + c1 * c1 + c2 * c2;
+    a.D[DDEVN][i] = (c0 * a.D[DPNX][i] + c1 * a.D[DPNY][i] + c2 * a.D[DPNZ][i] < 0.0) ? 1.0 : -1.0;
+}
+
+// MpsPndNeigh::calcPndnNeighNPCD (src/MpsPndNeigh.cpp:199-331)
+template <bool LIST>
+__global__ void kPndNeighNPCD(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+    double ni = 0.0, wSum = 0.0;
+    double c0 = a.D[DNPX][i], c1 = a.D[DNPY][i], c2 = a.D[DNPZ][i];
+    int nn = a.I[INNWALL][i];
+    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_l2 && (d2 < dm2 || pw)) {
+            nn += 1;
+            if (d2 < P.re_s2) {
+                const double wS = wgt(sqrt(d2), P.re_s, P.weight_type);
+                ni += wS;
+                c0 += v0 * wS; c1 += v1 * wS; c2 += v2 * wS;
+                wSum += wS;
+            }
+        }
+    });
+    a.I[INNEIGH][i] = nn;
+    const double small = ni + a.D[DPNDWALL][i];
+    if (P.pnd_type == MPS_PND_SUM_WIJ || P.pnd_type == MPS_PND_MEAN_SUM_WIJ) {
+        a.D[DPNDKI][i] = a.D[DPNDI][i];
+        a.D[DPNDI][i] = small;
+    }
+    a.D[DPNDSMALL][i] = small;
+    if (wSum > KEPS) { c0 /= small; c1 /= small; c2 /= small; }
+    a.D[DNPX][i] = c0; a.D[DNPY][i] = c1; a.D[DNPZ][i] = c2;
+    a.D[DNPCD2][i] = c0 * c0 + c1 * c1 + c2 * c2;
+    a.D[DDEVN][i] = (c0 * a.D[DPNX][i] + c1 * a.D[DPNY][i] + c2 * a.D[DPNZ][i] < 0.0) ? 1.0 : -1.0;
+}
+
+// calcPndDiffusiveTerm (src/MpsPndNeigh.cpp:334-489)
+template <bool LIST>
+__global__ void kPndDiffusive(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    const double ni = a.D[DPNDI][i];
+    if (ni < KEPS) return;
+    const double rho = P.dns_fl1;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
+    const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+    const double Pi = a.D[DPRESS][i];
+    const bool fi = a.I[ITYPE][i] == P.fluid;
+    double MC[9];
+    for (int q = 0; q < 9; q++) MC[q] = a.D[DCM0 + q][i];
+    double Di = 0.0, DivV = 0.0;
+    const double flagDi = 1.0;
+    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_l2 && (d2 < dm2 || pw)) {
+            const double dst = sqrt(d2);
+            const double wL = wgt(dst, P.re_l, P.weight_type);
+            if (fi && a.I[ITYPE][j] == P.fluid) {
+                const double pgh = -2.0 * (rho * rho / (rho + rho)) *
+                                   (P.gravity[0] * v0 + P.gravity[1] * v1 + P.gravity[2] * v2);
+                const double Pj = a.D[DPRESS][j];
+                Di += P.time_step / rho * P.coeff_visc_multiphase * (Pj - Pi - pgh) * wL;
+            }
+            if (d2 < P.re_s2) {
+                const double vx = a.D[DU][j] - ux, vy = a.D[DV][j] - uy, vz = a.D[DW][j] - uz;
+                const double wS = wgt(dst, P.re_s, P.weight_type);
+                if (!P.gradient_correction) {
+                    if (ni > KEPS)
+                        DivV += (P.dim / P.pnd_small_zero) * (a.D[DPNDI][j] / ni) * (vx * v0 + vy * v1 + vz * v2) * wS / d2;
+                } else {
+                    const double q0 = (v0 * MC[0] + v1 * MC[1] + v2 * MC[2]);
+                    const double q1 = (v0 * MC[3] + v1 * MC[4] + v2 * MC[5]);
+                    const double q2 = (v0 * MC[6] + v1 * MC[7] + v2 * MC[8]);
+                    DivV += (P.dim / P.pnd_small_zero) * (vx * q0 + vy * q1 + vz * q2) * wS / d2;
+                }
+            }
+        }
+    });
+    a.D[DAUX1][i] = ni * (1.0 + P.time_step * (-DivV + Di * flagDi));
+    a.D[DDIVU][i] = DivV;
+    a.D[DDIFF][i] = Di;
+}
+
+// calcWall{Slip,NoSlip}PndDiffusiveTerm (src/MpsPndNeigh.cpp:492-744)
+template <bool LIST>
+__global__ void kWallPndDiffusive(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const double ni = a.D[DPNDI][i];
+    if (!(a.I[ITYPE][i] == P.fluid && a.I[INEARWALL][i] && ni > KEPS)) return;
+    const bool slip = P.slip_condition == MPS_FREE_SLIP;
+    const double xi[3] = {PX(i), PY(i), PZ(i)};
+    const double xm[3] = {a.D[DMX][i], a.D[DMY][i], a.D[DMZ][i]};
+    const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
+    double n[3], R[9];
+    reflectMod2(xi, xm, n, R);
+    double umx, umy, umz;
+    if (slip) {
+        umx = R[0] * ux + R[1] * uy + R[2] * uz;
+        umy = R[3] * ux + R[4] * uy + R[5] * uz;
+        umz = R[6] * ux + R[7] * uy + R[8] * uz;
+    } else {
+        const double dotnv = n[0] * 0.0 + n[1] * 0.0 + n[2] * 0.0;
+        const double t0 = ux - 2.0 * (0.0 - dotnv * n[0]);
+        const double t1 = uy - 2.0 * (0.0 - dotnv * n[1]);
+        const double t2 = uz - 2.0 * (0.0 - dotnv * n[2]);
+        umx = -1.0 * t0 + 0.0 * t1 + 0.0 * t2;
+        umy = 0.0 * t0 + -1.0 * t1 + 0.0 * t2;
+        umz = 0.0 * t0 + 0.0 * t1 + -1.0 * t2;
+    }
+    double DivV = 0.0;
+    auto term = [&](double vjx, double vjy, double vjz, double nj, double w0, double w1, double w2, double dm2) {
+        const double wS = wgt(sqrt(dm2), P.re_s, P.weight_type);
+        if (slip) {
+            const double vx = vjx - umx, vy = vjy - umy, vz = vjz - umz;
+            DivV += (P.dim / P.pnd_small_zero) * (nj / ni) * (vx * w0 + vy * w1 + vz * w2) * wS / dm2;
+        } else {
+            const double vx = -(vjx - umx), vy = -(vjy - umy), vz = -(vjz - umz);
+            const double m0 = (R[0] * w0 + R[1] * w1 + R[2] * w2);
+            const double m1 = (R[3] * w0 + R[4] * w1 + R[5] * w2);
+            const double m2 = (R[6] * w0 + R[7] * w1 + R[8] * w2);
+            DivV += (P.dim / P.pnd_small_zero) * (nj / ni) * (vx * m0 + vy * m1 + vz * m2) * wS / dm2;
+        }
+    };
+    forNbrs<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2)
+            term(a.D[DU][j], a.D[DV][j], a.D[DW][j], a.D[DPNDI][j], w0, w1, w2, dm2);
+    });
+    const double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
+    const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+    if (dm2 < P.re_s2) term(ux, uy, uz, ni, w0, w1, w2, dm2);
+    a.D[DAUX1][i] = a.D[DAUX1][i] + -ni * P.time_step * DivV;
+}
+
+// updatePnd (src/MpsPndNeigh.cpp:747-774)
+__global__ void kUpdatePnd(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    a.D[DPNDI][i] = a.D[DAUX1][i];
+    a.D[DAUX1][i] = 0.0;
+}
+
+// meanPndParticlesWallDummySurface (src/MpsPndNeigh.cpp:777-852), first pass
+template <bool LIST>
+__global__ void kMeanPndWDS(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    a.D[DAUX1][i] = 0.0;
+    a.D[DAUX2][i] = 0.0;
+    if (!(a.I[ITYPE][i] == P.wall || a.I[IBC][i] == P.surface)) return;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+    double up = 0, dn = 0;
+    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_s2 && (d2 < dm2 || pw)) {
+            const double wS = wgt(sqrt(d2), P.re_s, P.weight_type);
+            up += a.D[DPNDI][j] * wS;
+            dn += wS;
+        }
+    });
+    a.D[DAUX1][i] = up;
+    a.D[DAUX2][i] = dn;
+}
+__global__ void kMeanPndWDSFinish(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    if (a.I[ITYPE][i] == P.wall || a.I[IBC][i] == P.surface) {
+        if (a.D[DAUX2][i] < KEPS) a.D[DPNDI][i] = a.D[DAUX1][i];
+        else a.D[DPNDI][i] = a.D[DAUX1][i] / (a.D[DAUX2][i]);
+    }
+    a.D[DAUX1][i] = 0.0;
+    a.D[DAUX2][i] = 0.0;
+}
+
+// meanWallPnd (src/MpsPndNeigh.cpp:923-994)
+template <bool LIST>
+__global__ void kMeanWallPnd(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    a.D[DAUX1][i] = 0.0;
+    a.D[DAUX2][i] = 0.0;
+    if (!a.I[INEARWALL][i]) return;
+    const double xi[3] = {PX(i), PY(i), PZ(i)};
+    const double xm[3] = {a.D[DMX][i], a.D[DMY][i], a.D[DMZ][i]};
+    double up = 0, dn = 0;
+    forNbrs<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2) {
+            const double wS = wgt(sqrt(dm2), P.re_s, P.weight_type);
+            up += a.D[DPNDI][j] * wS;
+            dn += wS;
+        }
+    });
+    const double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
+    const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+    if (dm2 < P.re_s2) {
+        const double wS = wgt(sqrt(dm2), P.re_s, P.weight_type);
+        up += a.D[DPNDI][i] * wS;
+        dn += wS;
+    }
+    a.D[DAUX1][i] = up;
+    a.D[DAUX2][i] = dn;
+}
+
+// meanPnd (src/MpsPndNeigh.cpp:855-920)
+template <bool LIST>
+__global__ void kMeanPnd(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+    double up = 0, dn = 0;
+    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_s2 && (d2 < dm2 || pw)) {
+            const double wS = wgt(sqrt(d2), P.re_s, P.weight_type);
+            up += a.D[DPNDI][j] * wS;
+            dn += wS;
+        }
+    });
+    a.D[DAUX1][i] = a.D[DAUX1][i] + up;
+    a.D[DAUX2][i] = a.D[DAUX2][i] + dn;
+}
+__global__ void kMeanPndFinish(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    if (a.I[INNEIGH][i] < 1) a.D[DPNDI][i] = a.D[DAUX1][i];
+    else a.D[DPNDI][i] = a.D[DAUX1][i] / a.D[DAUX2][i];
+    a.D[DAUX1][i] = 0.0;
+    a.D[DAUX2][i] = 0.0;
+}
+
+// meanNeighFluidPnd (src/MpsPndNeigh.cpp:997-1050)
+template <bool LIST>
+__global__ void kMeanNeighFluidPnd(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+    double up = 0, dn = 0;
+    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_s2 && (d2 < dm2 || pw) && a.I[ITYPE][j] == P.fluid) {
+            const double wS = wgt(sqrt(d2), P.re_s, P.weight_type);
+            up += a.D[DPNDKI][j] * wS;
+            dn += wS;
+        }
+    });
+    a.D[DPNDSKI][i] = (dn > KEPS) ? up / dn : up;
+}
+
+// MpsBoundaryCondition::updateParticleBC (src/MpsBoundaryCondition.cpp:66-400)
+__global__ void kUpdateBC(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    if (a.I[ITYPE][i] == P.dummy_wall) { a.I[IBC][i] = P.other; return; }
+    const double small = a.D[DPNDSMALL][i];
+    const int nn = a.I[INNEIGH][i];
+    int bc = (small < P.beta_pnd) ? P.surface : P.inner;
+    if (P.free_surf_type == MPS_FS_PND_NEIGH) {
+        bc = (small < P.beta_pnd && nn < P.beta_neigh) ? P.surface : P.inner;
+    } else if (P.free_surf_type == MPS_FS_PND_NPCD) {
+        if (bc == P.surface && nn > 4 && a.D[DNPCD2][i] < P.delta2) bc = P.inner;
+    }
+    a.I[IBC][i] = bc;
+}
+
+// --------------------------------------------------------------- pressure
+// calcPressEMPS / calcPressWCMPS (src/MpsPressure.cpp:62-163)
+__global__ void kPressExplicit(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const double mi = P.dns_fl1;
+    double pr = 0.0;
+    if (a.I[IBC][i] == P.inner) {
+        if (P.mps_type == MPS_EXPLICIT) pr = (a.D[DPNDI][i] - P.pnd_small_zero) * P.coeff_press_emps * mi;
+        else pr = (mi * P.coeff_press_wcmps / P.gamma) * (pow(a.D[DPNDI][i] / P.pnd_small_zero, P.gamma) - 1);
+    }
+    if (pr < 0.0) pr = 0.0;
+    a.D[DPRESS][i] = pr;
+}
+
+// calcVelDivergence (src/MpsPressure.cpp:479-551)
+template <bool LIST>
+__global__ void kVelDivergence(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    a.D[DDIVU][i] = 0.0;
+    if (a.I[ITYPE][i] != P.fluid) return;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    const double ni = a.D[DPNDI][i];
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
+    const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+    double MC[9];
+    for (int q = 0; q < 9; q++) MC[q] = a.D[DCM0 + q][i];
+    double DivV = 0.0;
+    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_s2 && (d2 < dm2 || pw)) {
+            const double vx = a.D[DU][j] - ux, vy = a.D[DV][j] - uy, vz = a.D[DW][j] - uz;
+            const double wS = wgt(sqrt(d2), P.re_s, P.weight_type);
+            if (!P.gradient_correction) {
+                if (ni > KEPS)
+                    DivV += (P.dim / P.pnd_small_zero) * (a.D[DPNDI][j] / ni) * (vx * v0 + vy * v1 + vz * v2) * wS / d2;
+            } else {
+                const double q0 = (v0 * MC[0] + v1 * MC[1] + v2 * MC[2]);
+                const double q1 = (v0 * MC[3] + v1 * MC[4] + v2 * MC[5]);
+                const double q2 = (v0 * MC[6] + v1 * MC[7] + v2 * MC[8]);
+                DivV += (P.dim / P.pnd_small_zero) * (vx * q0 + vy * q1 + vz * q2) * wS / d2;
+            }
+        }
+    });
+    a.D[DDIVU][i] = DivV;
+}
+
+// calcPressGradient (src/MpsPressure.cpp:1217-1338)
+template <bool LIST>
+__global__ void kPressGradient(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    const int gt = P.gradient_type;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+    const double Pi = a.D[DPRESS][i], ni = a.D[DPNDI][i];
+    double pmin = Pi;
+    double MC[9];
+    for (int q = 0; q < 9; q++) MC[q] = a.D[DCM0 + q][i];
+    if (gt == 0 || gt == 2)
+        forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+            const double xj = PX(j), yj = PY(j), zj = PZ(j);
+            const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+            const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+            const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && (d2 < dm2 || pw)) {
+                const double pj = a.D[DPRESS][j];
+                if (pmin > pj) pmin = pj;
+            }
+        });
+    double ax = 0, ay = 0, az = 0;
+    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_s2 && (d2 < dm2 || pw)) {
+            double wS = wgt(sqrt(d2), P.re_s, P.weight_type);
+            const double pj = a.D[DPRESS][j];
+            if (gt == 0) wS *= (pj - pmin) / d2;
+            else if (gt == 1) wS *= (pj + Pi) / d2;
+            else if (gt == 2) wS *= (pj + Pi - 2.0 * pmin) / d2;
+            else if (gt == 3) {
+                const double nj = a.D[DPNDI][j];
+                if (ni > KEPS && nj > KEPS) wS *= (ni * pj / nj + nj * Pi / ni) / d2;
+            }
+            if (!P.gradient_correction) {
+                ax += v0 * wS; ay += v1 * wS; az += v2 * wS;
+            } else {
+                ax += (v0 * MC[0] + v1 * MC[1] + v2 * MC[2]) * wS;
+                ay += (v0 * MC[3] + v1 * MC[4] + v2 * MC[5]) * wS;
+                az += (v0 * MC[6] + v1 * MC[7] + v2 * MC[8]) * wS;
+            }
+        }
+    });
+    const double rho = P.dns_fl1;
+    a.D[DAX][i] = P.relax_press * ax * P.coeff_press_grad / rho;
+    a.D[DAY][i] = P.relax_press * ay * P.coeff_press_grad / rho;
+    a.D[DAZ][i] = P.relax_press * az * P.coeff_press_grad / rho;
+}
+
+// calcWallPressGradient + repulsive forces (src/MpsPressure.cpp:1341-1611)
+template <bool LIST>
+__global__ void kWallPressGradient(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    if (!(a.I[ITYPE][i] == P.fluid && a.I[INEARWALL][i])) return;
+    const int gt = P.gradient_type;
+    const double vol = pow(P.part_dist, P.dim);
+    const double vm = fmin(a.ds[S_VELMAX], P.expect_max_velocity);
+    const double vmax2 = vm * vm;
+    const double rho = P.dns_fl1;
+    const double xi[3] = {PX(i), PY(i), PZ(i)};
+    const double xm[3] = {a.D[DMX][i], a.D[DMY][i], a.D[DMZ][i]};
+    const double Pi = a.D[DPRESS][i], ni = a.D[DPNDI][i];
+    double pmin = Pi;
+    double n[3], R[9], ns;
+    reflectSqrt(xi, xm, n, R, &ns);
+    if (gt == 0 || gt == 2)
+        forNbrs<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j) {
+            const double xj = PX(j), yj = PY(j), zj = PZ(j);
+            const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
+            const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
+            const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && d2 < dm2) {
+                const double pj = a.D[DPRESS][j];
+                if (pmin > pj) pmin = pj;
+            }
+        });
+    double ax = 0, ay = 0, az = 0;
+    auto term = [&](double Pj, double nj, double w0, double w1, double w2, double dm2) {
+        double wS = wgt(sqrt(dm2), P.re_s, P.weight_type);
+        if (gt == 0) wS *= (Pj - pmin) / dm2;
+        else if (gt == 1) wS *= (Pj + Pi) / dm2;
+        else if (gt == 2) wS *= (Pj + Pi - 2.0 * pmin) / dm2;
+        else if (gt == 3) {
+            if (ni > KEPS && nj > KEPS) wS *= (ni * Pj / nj + nj * Pi / ni) / dm2;
+        }
+        ax += w0 * wS; ay += w1 * wS; az += w2 * wS;
+    };
+    forNbrs<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2) term(a.D[DPRESS][j], a.D[DPNDI][j], w0, w1, w2, dm2);
+    });
+    {
+        const double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (dm2 < P.re_s2) term(Pi, ni, w0, w1, w2, dm2);
+    }
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    if (ns < P.re_repulsive_force && ns > KEPS) {
+        double w;
+        if (P.repulsive_force_type == MPS_REP_HARADA) {
+            w = rho / (P.time_step * P.time_step) * (P.re_repulsive_force - ns);
+        } else if (P.repulsive_force_type == MPS_REP_MITSUME) {
+            w = P.rep_coef_mitsume * wgt(ns, P.re_repulsive_force, P.weight_type);
+        } else if (P.repulsive_force_type == MPS_REP_LENNARD_JONES) {
+            const double R1 = (P.re_repulsive_force / ns) * (P.re_repulsive_force / ns);
+            const double R2 = R1 * R1;
+            w = (P.rep_coef_lennard_jones * vmax2 / ns) * (R2 - R1) * rho;
+        } else {
+            const double W1 = (1.0 + 3.0 * 0.5 * ns / (P.re_repulsive_force));
+            const double W2 = (1.0 - ns / (P.re_repulsive_force)) * (1.0 - ns / (P.re_repulsive_force)) *
+                              (1.0 - ns / (P.re_repulsive_force));
+            w = (P.rep_coef_monaghan_kajtar * vmax2 / (ns - 0.0 * P.part_dist)) * (1.0 / 8.0) * (W1) * (W2) * rho;
+        }
+        f0 = -w * n[0]; f1 = -w * n[1]; f2 = -w * n[2];
+    }
+    const double g0 = P.relax_press * (R[0] * ax + R[1] * ay + R[2] * az) * P.coeff_press_grad - f0;
+    const double g1 = P.relax_press * (R[3] * ax + R[4] * ay + R[5] * az) * P.coeff_press_grad - f1;
+    const double g2 = P.relax_press * (R[6] * ax + R[7] * ay + R[8] * az) * P.coeff_press_grad - f2;
+    a.D[DAX][i] = a.D[DAX][i] + g0 / rho;
+    a.D[DAY][i] = a.D[DAY][i] + g1 / rho;
+    a.D[DAZ][i] = a.D[DAZ][i] + g2 / rho;
+    a.D[DFWX][i] = a.D[DFWX][i] + -g0 * vol;
+    a.D[DFWY][i] = a.D[DFWY][i] + -g1 * vol;
+    a.D[DFWZ][i] = a.D[DFWZ][i] + -g2 * vol;
+}
+
+// calcWall{Slip,NoSlip}Viscosity (src/MpsViscosity.cpp:1527-1910)
+template <bool LIST>
+__global__ void kWallViscosity(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    if (!(a.I[ITYPE][i] == P.fluid && a.I[INEARWALL][i])) return;
+    const bool slip = P.slip_condition == MPS_FREE_SLIP;
+    const double vol = pow(P.part_dist, P.dim);
+    const double meu = P.knm_vs1 * P.dns_fl1, rho = P.dns_fl1, dnsI = P.dns_fl1;
+    const double xi[3] = {PX(i), PY(i), PZ(i)};
+    const double xm[3] = {a.D[DMX][i], a.D[DMY][i], a.D[DMZ][i]};
+    const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
+    double n[3], R[9];
+    reflectMod2(xi, xm, n, R);
+    double umx, umy, umz, Rm[9];
+    if (slip) {
+        umx = R[0] * ux + R[1] * uy + R[2] * uz;
+        umy = R[3] * ux + R[4] * uy + R[5] * uz;
+        umz = R[6] * ux + R[7] * uy + R[8] * uz;
+        for (int q = 0; q < 9; q++) Rm[q] = R[q];
+    } else {
+        const double dotnv = n[0] * 0.0 + n[1] * 0.0 + n[2] * 0.0;
+        const double t0 = ux - 2.0 * (0.0 - dotnv * n[0]);
+        const double t1 = uy - 2.0 * (0.0 - dotnv * n[1]);
+        const double t2 = uz - 2.0 * (0.0 - dotnv * n[2]);
+        umx = -1.0 * t0 + 0.0 * t1 + 0.0 * t2;
+        umy = 0.0 * t0 + -1.0 * t1 + 0.0 * t2;
+        umz = 0.0 * t0 + 0.0 * t1 + -1.0 * t2;
+        for (int q = 0; q < 9; q++) Rm[q] = (q % 4 == 0) ? -1.0 : 0.0;
+    }
+    double nuJ;
+    if ((meu + meu) > KEPS) nuJ = 2.0 * meu * meu / (meu + meu);
+    else nuJ = 0.0;
+    nuJ = nuJ / dnsI;
+    double nuI;
+    if (meu > KEPS) nuI = 2.0 * meu * meu / (meu + meu);
+    else nuI = 0.0;
+    nuI = nuI / dnsI;
+    double ax = 0, ay = 0, az = 0;
+    forNbrs<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_l2 && dm2 < P.re_l2 && d2 < dm2) {
+            const double wL = wgt(sqrt(dm2), P.re_l, P.weight_type);
+            ax += (a.D[DU][j] - umx) * wL * nuJ;
+            ay += (a.D[DV][j] - umy) * wL * nuJ;
+            az += (a.D[DW][j] - umz) * wL * nuJ;
+        }
+    });
+    {
+        const double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (dm2 < P.re_l2) {
+            const double wL = wgt(sqrt(dm2), P.re_l, P.weight_type);
+            ax += (ux - umx) * wL * nuI;
+            ay += (uy - umy) * wL * nuI;
+            az += (uz - umz) * wL * nuI;
+        }
+    }
+    const double r0 = (Rm[0] * ax + Rm[1] * ay + Rm[2] * az) * P.coeff_visc_multiphase;
+    const double r1 = (Rm[3] * ax + Rm[4] * ay + Rm[5] * az) * P.coeff_visc_multiphase;
+    const double r2 = (Rm[6] * ax + Rm[7] * ay + Rm[8] * az) * P.coeff_visc_multiphase;
+    a.D[DAX][i] = a.D[DAX][i] + r0;
+    a.D[DAY][i] = a.D[DAY][i] + r1;
+    a.D[DAZ][i] = a.D[DAZ][i] + r2;
+    a.D[DFWX][i] = a.D[DFWX][i] + -(Rm[0] * ax + Rm[1] * ay + Rm[2] * az) * P.coeff_visc_multiphase * vol * rho;
+    a.D[DFWY][i] = a.D[DFWY][i] + -(Rm[3] * ax + Rm[4] * ay + Rm[5] * az) * P.coeff_visc_multiphase * vol * rho;
+    a.D[DFWZ][i] = a.D[DFWZ][i] + -(Rm[6] * ax + Rm[7] * ay + Rm[8] * az) * P.coeff_visc_multiphase * vol * rho;
+}
+
+// ------------------------------------------------------------- shifting
+// calcShifting (src/MpsShifting.cpp:23-104), read-old / write-new: the new
+// velocity is staged in the dvel slots, then copied.
+template <bool LIST>
+__global__ void kShift1(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    double nu = a.D[DU][i], nv = a.D[DV][i], nw = a.D[DW][i];
+    if (a.I[ITYPE][i] == P.fluid) {
+        const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+        const double xi = PX(i), yi = PY(i), zi = PZ(i);
+        const double ux = nu, uy = nv, uz = nw;
+        const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+        double dx = 0, dy = 0, dz = 0;
+        forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+            const double xj = PX(j), yj = PY(j), zj = PZ(j);
+            const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+            const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+            const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && (d2 < dm2 || pw)) {
+                const double dw = delWgt(sqrt(d2), P.re_s, P.weight_type);
+                dx += dw * (a.D[DU][j] - ux);
+                dy += dw * (a.D[DV][j] - uy);
+                dz += dw * (a.D[DW][j] - uz);
+            }
+        });
+        if (a.I[IBC][i] == P.inner) {
+            nu = nu - P.coeff_shifting1 * dx;
+            nv = nv - P.coeff_shifting1 * dy;
+            nw = nw - P.coeff_shifting1 * dz;
+        }
+    }
+    a.D[DDVX][i] = nu; a.D[DDVY][i] = nv; a.D[DDVZ][i] = nw;
+}
+// copy staged values into (dst x,y,z) and clear the staging slots
+__global__ void kCommitStaged(KArgs a, int dst0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    a.D[dst0][i] = a.D[DDVX][i];
+    a.D[dst0 + 1][i] = a.D[DDVY][i];
+    a.D[dst0 + 2][i] = a.D[DDVZ][i];
+    a.D[DDVX][i] = 0.0; a.D[DDVY][i] = 0.0; a.D[DDVZ][i] = 0.0;
+}
+
+// calcConcAndConcGradient (src/MpsShifting.cpp:432-576), concentration pass
+template <bool LIST>
+__global__ void kShift2Conc(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+    double c = 0.0;
+    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_s2 && (d2 < dm2 || pw)) c += wgt(sqrt(d2), P.re_s, P.weight_type);
+    });
+    c += a.D[DPNDWALL][i];
+    a.D[DCONC][i] = c / P.pnd_small_zero;
+}
+// gradient pass: new positions staged in dvel slots
+template <bool LIST>
+__global__ void kShift2Grad(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    double nx = PX(i), ny = PY(i), nz = PZ(i);
+    if (a.I[ITYPE][i] == P.fluid) {
+        const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+        const double xi = nx, yi = ny, zi = nz;
+        const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+        const double ci = a.D[DCONC][i];
+        double g0 = 0, g1 = 0, g2 = 0;
+        forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+            const double xj = PX(j), yj = PY(j), zj = PZ(j);
+            const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+            const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+            const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && (d2 < dm2 || pw)) {
+                const double wS = wgt(sqrt(d2), P.re_s, P.weight_type);
+                const double cj = a.D[DCONC][j];
+                g0 += (ci + cj) * v0 * wS / d2;
+                g1 += (ci + cj) * v1 * wS / d2;
+                g2 += (ci + cj) * v2 * wS / d2;
+            }
+        });
+        g0 *= -P.coeff_press_grad; g1 *= -P.coeff_press_grad; g2 *= -P.coeff_press_grad;
+        a.D[DGCX][i] = g0; a.D[DGCY][i] = g1; a.D[DGCZ][i] = g2;
+        if (a.I[IBC][i] == P.inner) {
+            nx = nx - P.coeff_shifting2 * g0;
+            ny = ny - P.coeff_shifting2 * g1;
+            nz = nz - P.coeff_shifting2 * g2;
+        }
+    }
+    a.D[DDVX][i] = nx; a.D[DDVY][i] = ny; a.D[DDVZ][i] = nz;
+}
+// calcWallConcAndConcGradient (src/MpsShifting.cpp:578-692)
+template <bool LIST>
+__global__ void kShift2Wall(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    double nx = PX(i), ny = PY(i), nz = PZ(i);
+    if (a.I[ITYPE][i] == P.fluid && a.I[INEARWALL][i]) {
+        const double xi[3] = {nx, ny, nz};
+        const double xm[3] = {a.D[DMX][i], a.D[DMY][i], a.D[DMZ][i]};
+        const double ci = a.D[DCONC][i];
+        double n[3], R[9], ns;
+        reflectSqrt(xi, xm, n, R, &ns);
+        double d0 = 0, d1 = 0, dd2 = 0;
+        forNbrs<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j) {
+            const double xj = PX(j), yj = PY(j), zj = PZ(j);
+            const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
+            const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
+            const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2) {
+                const double wS = wgt(sqrt(dm2), P.re_s, P.weight_type);
+                const double cj = a.D[DCONC][j];
+                d0 += (ci + cj) * w0 * wS / dm2;
+                d1 += (ci + cj) * w1 * wS / dm2;
+                dd2 += (ci + cj) * w2 * wS / dm2;
+            }
+        });
+        const double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (dm2 < P.re_s2) {
+            const double wS = wgt(sqrt(dm2), P.re_s, P.weight_type);
+            d0 += (ci + ci) * w0 * wS / dm2;
+            d1 += (ci + ci) * w1 * wS / dm2;
+            dd2 += (ci + ci) * w2 * wS / dm2;
+        }
+        const double gx = -(R[0] * d0 + R[1] * d1 + R[2] * dd2) * P.coeff_press_grad;
+        const double gy = -(R[3] * d0 + R[4] * d1 + R[5] * dd2) * P.coeff_press_grad;
+        const double gz = -(R[6] * d0 + R[7] * d1 + R[8] * dd2) * P.coeff_press_grad;
+        a.D[DGCX][i] = a.D[DGCX][i] + gx;
+        a.D[DGCY][i] = a.D[DGCY][i] + gy;
+        a.D[DGCZ][i] = a.D[DGCZ][i] + gz;
+        if (a.I[IBC][i] == P.inner) {
+            nx = nx - P.coeff_shifting2 * gx;
+            ny = ny - P.coeff_shifting2 * gy;
+            nz = nz - P.coeff_shifting2 * gz;
+        }
+    }
+    a.D[DDVX][i] = nx; a.D[DDVY][i] = ny; a.D[DDVZ][i] = nz;
+}
+
+// updateVelocityParticlesWallDummy (src/MpsParticleVelPos.cpp:100-158)
+template <bool LIST>
+__global__ void kWallDummyVelocity(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    if (a.I[ITYPE][i] != P.dummy_wall) return;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    double dx = 0, dy = 0, dz = 0, ni = 0;
+    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+        const double v0 = PX(j) - xi, v1 = PY(j) - yi, v2 = PZ(j) - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        if (d2 < P.re_l2 && a.I[ITYPE][j] == P.fluid) {
+            const double wL = wgt(sqrt(d2), P.re_l, P.weight_type);
+            ni += wL;
+            dx += a.D[DU][j] * wL;
+            dy += a.D[DV][j] * wL;
+            dz += a.D[DW][j] * wL;
+        }
+    });
+    if (ni > KEPS) {
+        a.D[DU][i] = 2.0 * 0.0 - dx / ni;
+        a.D[DV][i] = 2.0 * 0.0 - dy / ni;
+        a.D[DW][i] = 2.0 * 0.0 - dz / ni;
+    } else {
+        a.D[DU][i] = 2.0 * 0.0 - dx;
+        a.D[DV][i] = 2.0 * 0.0 - dy;
+        a.D[DW][i] = 2.0 * 0.0 - dz;
+    }
+}
+
+// ---------------------------------------------------------------- launch
+#define L1(K) hipLaunchKernelGGL(K, dim3(g), dim3(kBlock), 0, ctx->stream, a)
+#define LT(K)                                                                       \
+    do {                                                                            \
+        if (ctx->listValid) hipLaunchKernelGGL(K<true>, dim3(g), dim3(kBlock), 0, ctx->stream, a); \
+        else hipLaunchKernelGGL(K<false>, dim3(g), dim3(kBlock), 0, ctx->stream, a); \
+    } while (0)
+#define LW(K) hipLaunchKernelGGL(K<false>, dim3(g), dim3(kBlock), 0, ctx->stream, a)
+
+int launchBuildList(Ctx* ctx) {
+    KArgs a = ctx->args();
+    const int g = gridFor(ctx->N, kBlock);
+    const mps_params& P = ctx->P;
+    double r2 = std::max(P.re_l2, P.re_s2);
+    r2 = std::max(r2, P.dist_collision_limit2);
+    r2 = std::max(r2, P.part_dist * P.part_dist);
+    hipLaunchKernelGGL(kBuildList, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbr, ctx->nbrCnt, r2);
+    MPS_HIP(hipGetLastError());
+    ctx->listValid = true;
+    return MPS_OK;
+}
+
+int launchCompaction(Ctx* ctx) {
+    KArgs a = ctx->args();
+    const int g = gridFor(ctx->N, kBlock);
+    const int cap = 4096;
+    hipLaunchKernelGGL(kOutFlag, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->outList, cap);
+    hipLaunchKernelGGL(kCompact, dim3(1), dim3(1), 0, ctx->stream, a, ctx->outList, cap, ctx->slotOf);
+    MPS_HIP(hipGetLastError());
+    return MPS_OK;
+}
+
+// extrapolatePressParticlesWallDummy (src/MpsPressure.cpp:792-846); in place
+// is safe: only wall / dummy slots are written and only fluid slots are read.
+template <bool LIST>
+__global__ void kExtrapolateWallDummy2(KArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const int t = a.I[ITYPE][i];
+    if (!(t == P.dummy_wall || (P.mps_type == MPS_WEAKLY && t == P.wall))) return;
+    const double rho = P.dns_fl1;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    double ni = 0.0, pr = 0.0;
+    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
+        const double v0 = PX(j) - xi, v1 = PY(j) - yi, v2 = PZ(j) - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        if (d2 < P.re_s2 && a.I[ITYPE][j] == P.fluid) {
+            const double wS = wgt(sqrt(d2), P.re_s, P.weight_type);
+            ni += wS;
+            pr += (a.D[DPRESS][j] - rho * (P.gravity[0] * v0 + P.gravity[1] * v1 + P.gravity[2] * v2)) * wS;
+        }
+    });
+    if (pr < 0.0) return;
+    a.D[DPRESS][i] = (ni > 0) ? pr / ni : pr;
+}
+
+int launchOp(Ctx* ctx, const char* op) {
+    KArgs a = ctx->args();
+    const int g = gridFor(ctx->N, kBlock);
+    const std::string o(op);
+    const mps_params& P = ctx->P;
+    if (o == "viscosity") LT(kViscosity);
+    else if (o == "add_gravity") L1(kAddGravity);
+    else if (o == "prediction") L1(kPrediction);
+    else if (o == "collision") {
+        if (P.collision_type == MPS_COLLISION_PC) LT(kCollisionPC);
+        else LT(kCollisionDPC);
+        hipLaunchKernelGGL(kApplyDvel, dim3(g), dim3(kBlock), 0, ctx->stream, a, P.collision_type == MPS_COLLISION_DPC ? 1 : 0);
+        if (P.collision_type == MPS_COLLISION_DPC) ctx->listValid = false;
+    } else if (o == "closest_point") {
+        if (!ctx->hasMesh) { ctx->err = "no wall mesh installed"; return MPS_E_ARG; }
+        L1(kClosestPoint);
+    } else if (o == "update_near_mesh") L1(kUpdateNear);
+    else if (o == "wall_npcd") LT(kWallNPCD);
+    else if (o == "correction_matrix") LT(kCorrectionMatrix);
+    else if (o == "initial_pnd") LT(kInitialPnd);
+    else if (o == "pnd_neigh_npcd") LT(kPndNeighNPCD);
+    else if (o == "pnd_diffusive") {
+        LT(kPndDiffusive);
+        if (P.wall_type == MPS_WALL_POLYGON) LT(kWallPndDiffusive);
+        L1(kUpdatePnd);
+        LT(kMeanPndWDS);
+        L1(kMeanPndWDSFinish);
+    } else if (o == "mean_pnd") {
+        if (P.wall_type == MPS_WALL_POLYGON) LT(kMeanWallPnd);
+        LT(kMeanPnd);
+        L1(kMeanPndFinish);
+    } else if (o == "mean_pnd_only") {
+        LT(kMeanPnd);
+        L1(kMeanPndFinish);
+    } else if (o == "mean_neigh_fluid_pnd") LT(kMeanNeighFluidPnd);
+    else if (o == "update_bc") {
+        if (P.free_surf_type == MPS_FS_PND_ARC) { ctx->err = "ARC free-surface rule is not supported"; return MPS_E_UNSUPPORTED; }
+        L1(kUpdateBC);
+    } else if (o == "press_explicit") L1(kPressExplicit);
+    else if (o == "vel_divergence") LT(kVelDivergence);
+    else if (o == "extrapolate_wall_dummy") LT(kExtrapolateWallDummy2); else if (o == "press_gradient") LT(kPressGradient);
+    else if (o == "wall_press_gradient") LT(kWallPressGradient);
+    else if (o == "wall_viscosity") LT(kWallViscosity);
+    else if (o == "correction") {
+        unsigned long long* bits = (unsigned long long*)(ctx->red + 4 * kRedBlocks);
+        MPS_HIP(hipMemsetAsync(bits, 0, sizeof(unsigned long long), ctx->stream));
+        hipLaunchKernelGGL(kCorrection, dim3(g), dim3(kBlock), 0, ctx->stream, a, bits);
+        hipLaunchKernelGGL(kVelMaxFinish, dim3(1), dim3(1), 0, ctx->stream, a, bits);
+        ctx->listValid = false;
+    } else if (o == "shifting") {
+        if (P.shifting_type == 1) {
+            LW(kShift1);
+            hipLaunchKernelGGL(kCommitStaged, dim3(g), dim3(kBlock), 0, ctx->stream, a, (int)DU);
+        } else if (P.shifting_type == 2) {
+            LW(kShift2Conc);
+            LW(kShift2Grad);
+            hipLaunchKernelGGL(kCommitStaged, dim3(g), dim3(kBlock), 0, ctx->stream, a, (int)DX);
+            if (P.wall_type == MPS_WALL_POLYGON) {
+                LW(kShift2Wall);
+                hipLaunchKernelGGL(kCommitStaged, dim3(g), dim3(kBlock), 0, ctx->stream, a, (int)DX);
+            }
+            ctx->listValid = false;
+        }
+    } else if (o == "wall_dummy_velocity") LW(kWallDummyVelocity);
+    else {
+        ctx->err = "unknown operator " + o;
+        return MPS_E_ARG;
+    }
+    MPS_HIP(hipGetLastError());
+    return MPS_OK;
+}
+
+}  // namespace mpsgpu

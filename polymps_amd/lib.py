@@ -1,0 +1,220 @@
+"""Python host binding of the C ABI (include/mps_gpu.h).
+
+``Simulation`` mirrors the reference driver's objects (MpsParticleSystem /
+MpsParticle plus the operator classes): one method per operator, each a thin
+call into libmps_gpu.so.  There is no fallback: if the library is missing the
+import of this module raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+from .params import MpsParams, check_layout
+
+_LIB_PATH = Path(__file__).resolve().parent / "libmps_gpu.so"
+
+_OPS = [
+    "bucket_update", "viscosity", "add_gravity", "prediction", "check_out_domain", "build_neighbor_list",
+    "collision", "mesh_closest_point", "mesh_update_near", "wall_npcd", "correction_matrix", "initial_pnd",
+    "pnd_neigh_npcd", "pnd_diffusive", "mean_pnd", "mean_neigh_fluid_pnd", "update_bc", "calc_press",
+    "ppe_assemble", "ppe_solve", "extrapolate_wall_dummy", "press_gradient", "wall_press_gradient",
+    "wall_viscosity", "correction", "shifting", "wall_dummy_velocity", "init", "step",
+]
+
+VEC_FIELDS = {"pos", "vel", "acc", "acc_star", "npcd", "mirror", "wall_pos", "polygon_normal", "force_wall",
+              "cm1", "cm2", "cm3", "grad_conc"}
+INT_FIELDS = {"type", "bc", "num_neigh", "num_neigh_wall", "near_wall", "near_mesh_type"}
+
+
+class MpsError(RuntimeError):
+    pass
+
+
+def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
+    p = Path(path) if path else _LIB_PATH
+    if not p.exists():
+        raise MpsError(f"{p} is not built; run polymps_amd.build.build() (no CPU fallback exists)")
+    lib = ctypes.CDLL(str(p))
+    vp, ip, dp, cp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_char_p
+    P = ctypes.POINTER(MpsParams)
+    lib.mps_params_size.restype = ctypes.c_size_t
+    lib.mps_params_defaults.argtypes = [P]
+    lib.mps_params_from_json.argtypes = [cp, P, cp, ctypes.c_size_t, cp, ctypes.c_size_t]
+    lib.mps_params_derive.argtypes = [P, ip, vp, vp]
+    lib.mps_create.argtypes = [ctypes.POINTER(vp), P, ip, ip]
+    lib.mps_destroy.argtypes = [vp]
+    lib.mps_last_error.argtypes = [vp]
+    lib.mps_last_error.restype = cp
+    lib.mps_get_params.argtypes = [vp, P]
+    lib.mps_upload_state.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.mps_download.argtypes = [vp, cp, vp, ctypes.c_size_t]
+    lib.mps_upload_field.argtypes = [vp, cp, vp, ctypes.c_size_t]
+    lib.mps_get_scalar.argtypes = [vp, cp, dp]
+    lib.mps_set_scalar.argtypes = [vp, cp, ctypes.c_double]
+    lib.mps_sync.argtypes = [vp]
+    lib.mps_mesh_read_stl.argtypes = [cp, ctypes.POINTER(ip), vp, ctypes.POINTER(ip), vp, vp]
+    lib.mps_set_mesh.argtypes = [vp, ip, vp, ip, vp, vp]
+    lib.mps_get_wall_tables.argtypes = [vp, vp, vp, vp, vp]
+    lib.mps_ppe_matrix.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ip)]
+    lib.mps_steps.argtypes = [vp, ip]
+    lib.mps_read_grid.argtypes = [cp, ctypes.POINTER(ip), vp, vp, vp, vp, vp]
+    lib.mps_write_grid.argtypes = [cp, ip, vp, vp, vp, vp, vp]
+    lib.mps_write_vtu.argtypes = [vp, cp, ip]
+    for op in _OPS:
+        getattr(lib, "mps_" + ("mesh_closest_point" if op == "mesh_closest_point" else op)).argtypes = [vp]
+    check_layout(lib)
+    return lib
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = load_library()
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def defaults() -> MpsParams:
+    p = MpsParams()
+    lib().mps_params_defaults(ctypes.byref(p))
+    return p
+
+
+def params_from_json(path: str | os.PathLike):
+    p = MpsParams()
+    grid = ctypes.create_string_buffer(4096)
+    mesh = ctypes.create_string_buffer(4096)
+    rc = lib().mps_params_from_json(str(path).encode(), ctypes.byref(p), grid, 4096, mesh, 4096)
+    if rc:
+        raise MpsError(lib().mps_last_error(None).decode())
+    return p, grid.value.decode(), mesh.value.decode()
+
+
+def derive(p: MpsParams, pos: np.ndarray | None = None, types: np.ndarray | None = None) -> MpsParams:
+    q = p.copy()
+    n = 0 if pos is None else len(pos) // 3
+    pp = None if pos is None else _ptr(np.ascontiguousarray(pos, dtype=np.float64))
+    tp = None if types is None else _ptr(np.ascontiguousarray(types, dtype=np.int32))
+    rc = lib().mps_params_derive(ctypes.byref(q), n, pp, tp)
+    if rc:
+        raise MpsError(lib().mps_last_error(None).decode())
+    return q
+
+
+def read_stl(path: str | os.PathLike):
+    L = lib()
+    nv, nf = ctypes.c_int(0), ctypes.c_int(0)
+    rc = L.mps_mesh_read_stl(str(path).encode(), ctypes.byref(nv), None, ctypes.byref(nf), None, None)
+    if rc:
+        raise MpsError(L.mps_last_error(None).decode())
+    V = np.zeros(3 * nv.value)
+    F = np.zeros(3 * nf.value, dtype=np.int32)
+    N = np.zeros(3 * nf.value)
+    rc = L.mps_mesh_read_stl(str(path).encode(), ctypes.byref(nv), _ptr(V), ctypes.byref(nf), _ptr(F), _ptr(N))
+    if rc:
+        raise MpsError(L.mps_last_error(None).decode())
+    return V, F, N
+
+
+class Simulation:
+    """A device-resident particle system driven through the C ABI."""
+
+    def __init__(self, params: MpsParams, types, pos, vel=None, press=None, press_avg=None, mesh=None, device=0):
+        self._lib = lib()
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        n = len(types)
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        vel = np.zeros(3 * n) if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(-1)
+        press = np.zeros(n) if press is None else np.ascontiguousarray(press, dtype=np.float64)
+        press_avg = np.zeros(n) if press_avg is None else np.ascontiguousarray(press_avg, dtype=np.float64)
+        self.params = params if params.derived else derive(params, pos, types)
+        self.n = n
+        h = ctypes.c_void_p()
+        rc = self._lib.mps_create(ctypes.byref(h), ctypes.byref(self.params), n, device)
+        self._h = h
+        if rc:
+            msg = self._lib.mps_last_error(h).decode()
+            self._lib.mps_destroy(h)
+            self._h = None
+            raise MpsError(f"mps_create failed ({rc}): {msg}")
+        self._check(self._lib.mps_upload_state(h, _ptr(types), _ptr(pos), _ptr(vel), _ptr(press), _ptr(press_avg)))
+        if mesh is not None:
+            V, F, N = (np.ascontiguousarray(mesh[0], dtype=np.float64), np.ascontiguousarray(mesh[1], dtype=np.int32),
+                       np.ascontiguousarray(mesh[2], dtype=np.float64))
+            self._check(self._lib.mps_set_mesh(h, len(V) // 3, _ptr(V), len(F) // 3, _ptr(F), _ptr(N)))
+
+    def _check(self, rc):
+        if rc:
+            raise MpsError(f"MPS call failed ({rc}): {self._lib.mps_last_error(self._h).decode()}")
+
+    def close(self):
+        if self._h:
+            self._lib.mps_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def call(self, op: str):
+        self._check(getattr(self._lib, "mps_" + op)(self._h))
+
+    def steps(self, k: int):
+        self._check(self._lib.mps_steps(self._h, k))
+
+    def get(self, field: str) -> np.ndarray:
+        if field in INT_FIELDS:
+            out = np.zeros(self.n, dtype=np.int32)
+        elif field in VEC_FIELDS:
+            out = np.zeros(3 * self.n)
+        else:
+            out = np.zeros(self.n)
+        self._check(self._lib.mps_download(self._h, field.encode(), _ptr(out), out.size))
+        return out
+
+    def set(self, field: str, values) -> None:
+        dt = np.int32 if field in INT_FIELDS else np.float64
+        arr = np.ascontiguousarray(values, dtype=dt).reshape(-1)
+        self._check(self._lib.mps_upload_field(self._h, field.encode(), _ptr(arr), arr.size))
+
+    def scalar(self, name: str) -> float:
+        v = ctypes.c_double()
+        self._check(self._lib.mps_get_scalar(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    def set_scalar(self, name: str, value: float) -> None:
+        self._check(self._lib.mps_set_scalar(self._h, name.encode(), float(value)))
+
+    def ppe_matrix(self):
+        nnz = ctypes.c_int()
+        self._check(self._lib.mps_ppe_matrix(self._h, None, None, None, ctypes.byref(nnz)))
+        nl = int(self.scalar("num_particles"))
+        rp = np.zeros(nl + 1, dtype=np.int32)
+        cols = np.zeros(nnz.value, dtype=np.int32)
+        vals = np.zeros(nnz.value)
+        self._check(self._lib.mps_ppe_matrix(self._h, _ptr(rp), _ptr(cols), _ptr(vals), ctypes.byref(nnz)))
+        return rp, cols, vals
+
+    def write_vtu(self, path, flags: int = 1):
+        self._check(self._lib.mps_write_vtu(self._h, str(path).encode(), flags))
+
+    def sync(self):
+        self._check(self._lib.mps_sync(self._h))
