@@ -40,6 +40,7 @@ KArgs Ctx::args() const {
     a.ds = ds;
     a.is = is;
     a.red = red;
+    a.volForce = pow(P.part_dist, P.dim);
     return a;
 }
 
@@ -111,7 +112,7 @@ __global__ void kInitState(KArgs a) {
     for (int f = DAX; f < ND; f++) a.D[f][k] = 0.0;
     a.D[DMX][k] = 0.0; a.D[DMY][k] = 0.0; a.D[DMZ][k] = 0.0;
     a.D[DPNDI][k] = 0.0; a.D[DPPEX][k] = 0.0; a.D[DFWX][k] = 0.0; a.D[DFWY][k] = 0.0; a.D[DFWZ][k] = 0.0;
-    a.D[DPNDSKI][k] = 0.0;
+    for (int f = DPNDSKI; f < NPD; f++) a.D[f][k] = 0.0;
     a.D[DRIW2][k] = 10e8 * a.P.part_dist;
     a.I[ICANON][k] = k;
     a.I[IBC][k] = 0;
@@ -370,6 +371,7 @@ int mps_destroy(mps_ctx* ctx) {
     if (ctx->hostInts) hipHostFree(ctx->hostInts);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    for (auto e : ctx->evPool) hipEventDestroy(e);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return MPS_OK;
@@ -482,7 +484,7 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
         hipStreamSynchronize(ctx->stream);
         hipFree(tmp);
     }
-    ctx->listValid = false;
+    if (fd->f0 == DX) ctx->listValid = false;   // positions moved: the neighbour list is stale
     MPS_HIP(hipGetLastError());
     return MPS_OK;
 }
@@ -506,6 +508,10 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "list_max") *out = ctx->hostInts[I_LIST_MAX];
     else if (s == "list_capacity") *out = ctx->K;
     else if (s == "capacity") *out = ctx->N;
+    else if (s == "nnz") *out = ctx->hostInts[I_NNZ];
+    else if (s == "spmv_ms_total") *out = ctx->spmvMsTotal;
+    else if (s == "spmv_launches") *out = (double)ctx->spmvLaunches;
+    else if (s == "spmv_bytes_total") *out = ctx->spmvBytesTotal;
     else { ctx->err = "unknown scalar " + s; return MPS_E_ARG; }
     return MPS_OK;
 }
@@ -519,6 +525,11 @@ int mps_set_scalar(mps_ctx* ctx, const char* name, double value) {
         ctx->velMaxHost = value;
     } else if (s == "time") {
         ctx->timeCurrent = value;
+    } else if (s == "profile") {
+        ctx->profile = value != 0.0;
+        ctx->spmvMsTotal = 0.0;
+        ctx->spmvLaunches = 0;
+        ctx->spmvBytesTotal = 0.0;
     } else {
         ctx->err = "unknown scalar " + s;
         return MPS_E_ARG;

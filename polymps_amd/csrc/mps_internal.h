@@ -22,11 +22,14 @@ int paramsDerive(mps_params* p, int n, const double* pos, std::string& err);
 // inside a step before they are read.
 enum DField {
     // persistent
+    // (pndki, div_u, diffusive and grad_conc are written for a subset of the
+    // particles in some modes, so they are carried with their particle)
     DX, DY, DZ, DU, DV, DW, DMX, DMY, DMZ, DPRESS, DPNDI, DPPEX, DPAVG, DFWX, DFWY, DFWZ, DPNDSKI,
+    DPNDKI, DDIVU, DDIFF, DGCX, DGCY, DGCZ,
     NPD,
     // per step
-    DAX = NPD, DAY, DAZ, DASX, DASY, DASZ, DNPX, DNPY, DNPZ, DNPCD2, DPNDKI, DPNDSMALL, DCONC, DDIVU,
-    DDIFF, DAUX1, DAUX2, DPNDWALL, DRIW2, DDEVN, DWPX, DWPY, DWPZ, DPNX, DPNY, DPNZ, DGCX, DGCY, DGCZ,
+    DAX = NPD, DAY, DAZ, DASX, DASY, DASZ, DNPX, DNPY, DNPZ, DNPCD2, DPNDSMALL, DCONC,
+    DAUX1, DAUX2, DPNDWALL, DRIW2, DDEVN, DWPX, DWPY, DWPZ, DPNX, DPNY, DPNZ,
     DCM0, DCM1, DCM2, DCM3, DCM4, DCM5, DCM6, DCM7, DCM8, DDVX, DDVY, DDVZ, DPPEB,
     ND
 };
@@ -37,7 +40,7 @@ enum IField { ITYPE, ICANON, IBC, INNEIGH, NPI, INNWALL = NPI, INEARWALL, INEARM
 enum DScalar { S_VELMAX, S_ABSNEW, S_ABSOLD, S_RHSN2, S_THR, S_RN2, S_ALPHA, S_BETA, S_PAP, S_BN2, S_TRUE_RN2,
                S_RELTOL, NDS };
 // device scalar slots (int)
-enum IScalar { I_NLIVE, I_OUTCNT, I_OVERFLOW, I_CG_ITER, I_CG_DONE, I_COMPACT_OVF, I_LIST_MAX, NIS };
+enum IScalar { I_NLIVE, I_OUTCNT, I_OVERFLOW, I_CG_ITER, I_CG_DONE, I_COMPACT_OVF, I_LIST_MAX, I_NNZ, NIS };
 
 constexpr int kWallTable = 50;
 constexpr int kRedBlocks = 1024;   // partial-sum slots of a reduction
@@ -61,6 +64,7 @@ struct KArgs {
     double* ds;            // device scalars
     int* is;
     double* red;           // reduction partials [kRedBlocks * 4]
+    double volForce;       // pow(l0, dim) evaluated on the host, as the reference's CPU build does
 };
 
 struct Ctx {
@@ -132,6 +136,13 @@ struct Ctx {
     double ppeMs = 0.0;
     int fallbackUsed = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // optional per-launch timing of the SpMV kernel (HIP events on the
+    // context stream); enabled with mps_set_scalar(ctx, "profile", 1)
+    bool profile = false;
+    std::vector<hipEvent_t> evPool;
+    double spmvMsTotal = 0.0;
+    long spmvLaunches = 0;
+    double spmvBytesTotal = 0.0;
     std::string err;
 
     KArgs args() const;

@@ -985,7 +985,7 @@ __global__ void kWallPressGradient(KArgs a) {
     const mps_params& P = a.P;
     if (!(a.I[ITYPE][i] == P.fluid && a.I[INEARWALL][i])) return;
     const int gt = P.gradient_type;
-    const double vol = pow(P.part_dist, P.dim);
+    const double vol = a.volForce;
     const double vm = fmin(a.ds[S_VELMAX], P.expect_max_velocity);
     const double vmax2 = vm * vm;
     const double rho = P.dns_fl1;
@@ -1069,7 +1069,7 @@ __global__ void kWallViscosity(KArgs a) {
     const mps_params& P = a.P;
     if (!(a.I[ITYPE][i] == P.fluid && a.I[INEARWALL][i])) return;
     const bool slip = P.slip_condition == MPS_FREE_SLIP;
-    const double vol = pow(P.part_dist, P.dim);
+    const double vol = a.volForce;
     const double meu = P.knm_vs1 * P.dns_fl1, rho = P.dns_fl1, dnsI = P.dns_fl1;
     const double xi[3] = {PX(i), PY(i), PZ(i)};
     const double xm[3] = {a.D[DMX][i], a.D[DMY][i], a.D[DMZ][i]};

@@ -84,6 +84,7 @@ __global__ void kPpeAssemble(KArgs a, double* val, int* col, int* cnt, double* d
     cnt[i] = c;
     diag[i] = sum;
     b[i] = rhs;
+    atomicAdd(&a.is[I_NNZ], c);
 }
 
 // ------------------------------------------------------------ reductions
@@ -300,6 +301,7 @@ int ppeAssemble(Ctx* ctx) {
     KArgs a = ctx->args();
     const int g = gridFor(ctx->N, kB);
     const bool T = ctx->P.ppe_transpose != 0;
+    MPS_HIP(hipMemsetAsync(ctx->is + I_NNZ, 0, sizeof(int), ctx->stream));
 #define ASM(L, TT) hipLaunchKernelGGL((kPpeAssemble<L, TT>), dim3(g), dim3(kB), 0, ctx->stream, a, ctx->aVal, ctx->aCol, ctx->aCnt, ctx->aDiag, ctx->D[DPPEB])
     if (ctx->listValid) {
         if (T) ASM(true, true); else ASM(true, false);
@@ -327,9 +329,16 @@ int ppeSolve(Ctx* ctx) {
     hipLaunchKernelGGL(kCgInitPFinish, dim3(1), dim3(1024), 0, s, a, nb);
     const int batch = 8;
     int done = 0, iters = 0;
+    if (ctx->profile && ctx->evPool.size() < 2 * batch) {
+        ctx->evPool.resize(2 * batch);
+        for (auto& e : ctx->evPool) MPS_HIP(hipEventCreate(&e));
+    }
+    std::vector<float> spmvTimes;
     for (int launched = 0; !done; launched += batch) {
         for (int q = 0; q < batch; q++) {
+            if (ctx->profile) MPS_HIP(hipEventRecord(ctx->evPool[2 * q], s));
             hipLaunchKernelGGL(kCgSpmv, dim3(nb), dim3(kB), 0, s, a, c);
+            if (ctx->profile) MPS_HIP(hipEventRecord(ctx->evPool[2 * q + 1], s));
             hipLaunchKernelGGL(kCgAlpha, dim3(1), dim3(1024), 0, s, a, nb);
             hipLaunchKernelGGL(kCgUpdate, dim3(nb), dim3(kB), 0, s, a, c);
             hipLaunchKernelGGL(kCgBeta, dim3(1), dim3(1024), 0, s, a, nb, maxIt);
@@ -339,6 +348,12 @@ int ppeSolve(Ctx* ctx) {
         MPS_HIP(hipStreamSynchronize(s));
         done = ctx->hostInts[I_CG_DONE];
         iters = ctx->hostInts[I_CG_ITER];
+        if (ctx->profile)
+            for (int q = 0; q < batch; q++) {
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, ctx->evPool[2 * q], ctx->evPool[2 * q + 1]);
+                spmvTimes.push_back(ms);
+            }
         if (launched > 4 * maxIt + 64) break;   // safety net: the device flag ends every batch
     }
     hipLaunchKernelGGL(kTrueResidual, dim3(nb), dim3(kB), 0, s, a, c);
@@ -353,6 +368,17 @@ int ppeSolve(Ctx* ctx) {
     ctx->ppeMs = ms;
     const double* hs = ctx->hostScalars;
     ctx->solverIter = iters;
+    if (ctx->profile) {
+        // launches that did work: one per iteration plus the one that converged
+        const int real = std::min((int)spmvTimes.size(), ctx->hostInts[I_CG_DONE] == 1 || ctx->hostInts[I_CG_DONE] == 3 ? iters + 1 : 0);
+        const double nnz = (double)ctx->hostInts[I_NNZ];
+        const double bytes = nnz * 12.0 + 28.0 * (double)ctx->nLiveHost;
+        for (int q = 0; q < real; q++) {
+            ctx->spmvMsTotal += spmvTimes[q];
+            ctx->spmvBytesTotal += bytes;
+        }
+        ctx->spmvLaunches += real;
+    }
     ctx->solverErr = hs[S_RHSN2] > 0 ? sqrt(hs[S_RN2] / hs[S_RHSN2]) : 0.0;
     ctx->trueResidual = hs[S_BN2] > 0 ? sqrt(hs[S_TRUE_RN2] / hs[S_BN2]) : 0.0;
     return MPS_OK;

@@ -1,0 +1,162 @@
+"""Lockstep comparison of the HIP path with the CPU oracle.
+
+Both sides run the reference's per-step operator sequence
+(src/main.cpp:252-467).  After every operator the fields it writes are
+compared bit for bit.  The one place where the two may legitimately differ
+is the PPE solve (its dot products reduce in a different order): the solution
+is compared within a tolerance and then the oracle's solution is copied into
+the device state so that the following operators again start from identical
+inputs.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from polymps_amd.lib import Simulation
+from oracle.oracle import Oracle
+
+PERSISTENT = ["pos", "vel", "press", "ppe_x", "pndi", "bc", "num_neigh", "mirror", "force_wall", "press_avg",
+              "pndki", "pndski", "div_u", "diffusive", "grad_conc"]
+
+
+class Mismatch(AssertionError):
+    pass
+
+
+def compare_exact(name, g, o, where=""):
+    if g.dtype.kind == "f":
+        same = (g == o) | (np.isnan(g) & np.isnan(o))
+    else:
+        same = g == o
+    if not np.all(same):
+        bad = np.flatnonzero(~same)
+        k = bad[0]
+        raise Mismatch(f"{where}: field {name} differs at {len(bad)} entries; first index {k}: "
+                       f"gpu {g[k]!r} oracle {o[k]!r}")
+
+
+def compare_close(name, g, o, rtol, atol, where=""):
+    if not np.allclose(g, o, rtol=rtol, atol=atol):
+        d = np.abs(g - o)
+        k = int(np.argmax(d))
+        raise Mismatch(f"{where}: field {name} max |diff| {d[k]:.3e} at {k}: gpu {g[k]!r} oracle {o[k]!r}")
+
+
+def sync_state(g: Simulation, o: Oracle, fields=PERSISTENT):
+    for f in fields:
+        g.set(f, o.get(f))
+    g.set_scalar("vel_max", o.scalar("vel_max"))
+
+
+def new_pair(w, threads=0):
+    g = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh)
+    o = Oracle(w.params, w.types, w.pos, w.vel, mesh=w.mesh, threads=threads)
+    return g, o
+
+
+def check(g, o, fields, where):
+    for f in fields:
+        compare_exact(f, g.get(f), o.get(f), where)
+
+
+def ppe_check(g, o, where, press_rtol):
+    """Compare the assembled iteration matrix and right-hand side exactly,
+    then the solution within a tolerance relative to the largest pressure."""
+    p = g.params
+    rpg, cg, vg = g.ppe_matrix()
+    rpo, co, vo = o.ppe_matrix()
+    n = len(rpo) - 1
+    # oracle rows are A; the device iterates A^T when ppe_transpose is set
+    ent_o = {}
+    for r in range(n):
+        for k in range(rpo[r], rpo[r + 1]):
+            key = (int(co[k]), r) if p.ppe_transpose else (r, int(co[k]))
+            ent_o[key] = vo[k]
+    ent_g = {}
+    for r in range(len(rpg) - 1):
+        for k in range(rpg[r], rpg[r + 1]):
+            ent_g[(r, int(cg[k]))] = vg[k]
+    if ent_g.keys() != ent_o.keys():
+        only_g = sorted(set(ent_g) - set(ent_o))[:5]
+        only_o = sorted(set(ent_o) - set(ent_g))[:5]
+        raise Mismatch(f"{where}: PPE sparsity differs; gpu-only {only_g} oracle-only {only_o}")
+    for key, v in ent_o.items():
+        if ent_g[key] != v:
+            raise Mismatch(f"{where}: PPE entry {key} gpu {ent_g[key]!r} oracle {v!r}")
+    if not p.ppe_transpose:
+        # without the transpose the rows must also come in the same order
+        compare_exact("ppe_cols", cg, co, where)
+    compare_exact("ppe_b", g.get("ppe_b"), o.get("ppe_b"), where)
+
+
+def lockstep_step(g: Simulation, o: Oracle, step: int, press_rtol=1e-6, log=None):
+    p = g.params
+    W = f"step {step}"
+    poly = p.wall_type == 1
+
+    def both(gop, oops, fields):
+        g.call(gop)
+        for op in oops:
+            o.call(op)
+        check(g, o, fields, f"{W} {gop}")
+        if log is not None:
+            log.append(gop)
+
+    both("bucket_update", ["update_buckets"], [])
+    both("viscosity", ["viscosity"], ["acc"])
+    both("add_gravity", ["add_gravity"], ["acc", "acc_star"])
+    both("prediction", ["prediction"], ["pos", "vel", "acc", "riw2", "npcd"])
+    both("check_out_domain", ["check_out_domain"], ["type", "pos"])
+    if p.collision_type == 0:
+        g.call("build_neighbor_list")
+        both("collision", ["collision"], ["vel"])
+    else:
+        both("collision", ["collision"], ["vel", "pos"])
+        g.call("build_neighbor_list")
+    if poly:
+        both("mesh_closest_point", ["closest_point"], ["riw2", "mirror", "wall_pos", "polygon_normal"])
+        both("mesh_update_near", ["update_near_mesh"], ["pnd_wall", "num_neigh_wall", "near_wall"])
+        both("wall_npcd", ["wall_npcd"], ["npcd"])
+    if p.gradient_correction:
+        both("correction_matrix", ["correction_matrix"], ["cm1", "cm2", "cm3"])
+    both("pnd_neigh_npcd", ["pnd_neigh_npcd"], ["num_neigh", "pnd_small", "pndi", "pndki", "npcd", "npcd2"])
+    if p.pnd_type == 2:
+        ops = ["pnd_diffusive"] + (["wall_pnd_diffusive"] if poly else []) + ["update_pnd", "mean_pnd_wall_dummy_surface"]
+        both("pnd_diffusive", ops, ["pndi", "div_u", "diffusive"])
+    elif p.pnd_type == 1:
+        both("mean_pnd", (["mean_wall_pnd"] if poly else []) + ["mean_pnd"], ["pndi"])
+    both("update_bc", ["update_bc"], ["bc"])
+    if p.mps_type in (0, 1):
+        g.call("calc_press")
+        o.call("calc_press")
+        compare_close("press", g.get("press"), o.get("press"), 1e-13, 1e-9, f"{W} calc_press")
+        g.set("press", o.get("press"))
+    else:
+        g.call("ppe_assemble")
+        o.call("assemble_ppe")
+        ppe_check(g, o, f"{W} ppe_assemble", press_rtol)
+        g.call("ppe_solve")
+        o.call("solve_ppe")
+        pg, po = g.get("press"), o.get("press")
+        scale = max(np.abs(po).max(), 1e-30)
+        compare_close("press", pg, po, 0.0, press_rtol * scale, f"{W} ppe_solve")
+        g.set("ppe_x", o.get("ppe_x"))
+        g.set("press", po)
+    if not poly:
+        both("extrapolate_wall_dummy", ["extrapolate_wall_dummy"], ["press"])
+    if p.gradient_correction:
+        both("correction_matrix", ["correction_matrix"], ["cm1", "cm2", "cm3"])
+    both("press_gradient", ["press_gradient"], ["acc"])
+    if poly:
+        both("wall_press_gradient", ["wall_press_gradient"], ["acc", "force_wall"])
+        both("wall_viscosity", ["wall_viscosity"], ["acc", "force_wall"])
+    both("correction", ["correction"], ["pos", "vel", "acc"])
+    vg, vo = g.scalar("vel_max"), o.scalar("vel_max")
+    if vg != vo:
+        raise Mismatch(f"{W}: vel_max gpu {vg!r} oracle {vo!r}")
+    both("check_out_domain", ["check_out_domain"], ["type", "pos"])
+    if p.shifting_type:
+        both("shifting", ["shifting"], ["pos", "vel"])
+    if not poly:
+        both("wall_dummy_velocity", ["wall_dummy_velocity"], ["vel"])
+    both("check_out_domain", ["check_out_domain"], ["type", "pos"])
