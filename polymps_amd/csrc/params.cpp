@@ -441,7 +441,9 @@ int paramsDerive(mps_params* p, int n, const double* pos, std::string& err) {
             lim[k][0] -= p->bucket_side;
             lim[k][1] += p->bucket_side;
             long nb = (long)((lim[k][1] - lim[k][0]) / p->bucket_side + 1);
-            (&p->num_buckets_x)[k] = (int)nb;
+            if (k == 0) p->num_buckets_x = (int)nb;
+            else if (k == 1) p->num_buckets_y = (int)nb;
+            else p->num_buckets_z = (int)nb;
             lim[k][1] = lim[k][0] + nb * p->bucket_side;
         }
         for (int k = 0; k < 3; k++) { p->domain_min[k] = lim[k][0]; p->domain_max[k] = lim[k][1]; }
