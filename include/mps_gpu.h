@@ -76,7 +76,11 @@ int mps_download(mps_ctx* ctx, const char* field, void* out, size_t count);
 /* Overwrite one field (reference order), same names as mps_download. */
 int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t count);
 /* Scalars: "vel_max" "cfl" "time" "solver_iter" "solver_err" "num_particles"
- * "iterations" "ppe_ms" "true_residual" "fallback" */
+ * "iterations" "ppe_ms" "true_residual" "fallback" "outer_iterations"
+ * "list_max" "list_capacity" "capacity" "nnz" "spmv_ms_total"
+ * "spmv_launches" "spmv_bytes_total".  Settable: "vel_max" "time"
+ * "profile" (nonzero: time every SpMV launch with HIP events, counters reset)
+ * "precision_mode" (MPS_PREC_*, used from the next PPE assembly). */
 int mps_get_scalar(mps_ctx* ctx, const char* name, double* out);
 int mps_set_scalar(mps_ctx* ctx, const char* name, double value);
 int mps_sync(mps_ctx* ctx);

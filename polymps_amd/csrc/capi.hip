@@ -332,7 +332,8 @@ int mps_create(mps_ctx** out, const mps_params* p, int n, int device) {
     MPS_HIP(hipMalloc(&ctx->cgInvD, nd));
     MPS_HIP(hipMalloc(&ctx->ds, sizeof(double) * NDS));
     MPS_HIP(hipMalloc(&ctx->is, sizeof(int) * NIS));
-    MPS_HIP(hipMalloc(&ctx->red, sizeof(double) * (4 * kRedBlocks + 8)));
+    ctx->redCap = std::max(kRedBlocks, gridFor(n, 256));
+    MPS_HIP(hipMalloc(&ctx->red, sizeof(double) * (2 * ctx->redCap + 16)));
     MPS_HIP(hipMalloc(&ctx->wtab, sizeof(double) * 4 * kWallTable));
     MPS_HIP(hipHostMalloc(&ctx->hostScalars, sizeof(double) * NDS));
     MPS_HIP(hipHostMalloc(&ctx->hostInts, sizeof(int) * NIS));
@@ -500,6 +501,7 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "time") *out = ctx->timeCurrent;
     else if (s == "solver_iter") *out = ctx->solverIter;
     else if (s == "solver_err") *out = ctx->solverErr;
+    else if (s == "outer_iterations") *out = ctx->outerIter;
     else if (s == "num_particles") *out = ctx->nLiveHost;
     else if (s == "iterations") *out = ctx->numIter;
     else if (s == "ppe_ms") *out = ctx->ppeMs;
@@ -530,6 +532,13 @@ int mps_set_scalar(mps_ctx* ctx, const char* name, double value) {
         ctx->spmvMsTotal = 0.0;
         ctx->spmvLaunches = 0;
         ctx->spmvBytesTotal = 0.0;
+    } else if (s == "precision_mode") {
+        // takes effect at the next assembly; a solve needs an assembly in the same mode
+        if (value != MPS_PREC_FP64 && value != MPS_PREC_MIXED) {
+            ctx->err = "precision_mode must be 0 (fp64) or 1 (mixed)";
+            return MPS_E_ARG;
+        }
+        ctx->P.precision_mode = (int)value;
     } else {
         ctx->err = "unknown scalar " + s;
         return MPS_E_ARG;

@@ -63,7 +63,7 @@ struct KArgs {
     const double* wtab;    // xPND[50] xNeigh[50] Zij[50] nNeigh[50] (as double)
     double* ds;            // device scalars
     int* is;
-    double* red;           // reduction partials [kRedBlocks * 4]
+    double* red;           // reduction partials: one pair per block, then the vel-max slot
     double volForce;       // pow(l0, dim) evaluated on the host, as the reference's CPU build does
 };
 
@@ -131,6 +131,8 @@ struct Ctx {
     double timeCurrent = 0.0;
     int numIter = 0;
     int solverIter = 0;
+    int outerIter = 0;
+    int redCap = 0;              // partial-sum slots in red (pairs)
     double solverErr = 0.0;
     double trueResidual = 0.0;
     double ppeMs = 0.0;

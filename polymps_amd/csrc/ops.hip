@@ -1422,7 +1422,7 @@ int launchOp(Ctx* ctx, const char* op) {
     else if (o == "wall_press_gradient") LT(kWallPressGradient);
     else if (o == "wall_viscosity") LT(kWallViscosity);
     else if (o == "correction") {
-        unsigned long long* bits = (unsigned long long*)(ctx->red + 4 * kRedBlocks);
+        unsigned long long* bits = (unsigned long long*)(ctx->red + 2 * ctx->redCap + 8);
         MPS_HIP(hipMemsetAsync(bits, 0, sizeof(unsigned long long), ctx->stream));
         hipLaunchKernelGGL(kCorrection, dim3(g), dim3(kBlock), 0, ctx->stream, a, bits);
         hipLaunchKernelGGL(kVelMaxFinish, dim3(1), dim3(1), 0, ctx->stream, a, bits);

@@ -23,14 +23,20 @@ namespace mpsgpu {
 constexpr int kB = 256;
 
 template <bool LIST, bool TRANS>
-__global__ void kPpeAssemble(KArgs a, double* val, int* col, int* cnt, double* diag, double* b) {
+__global__ void kPpeAssemble(KArgs a, double* val, int* col, int* cnt, double* diag, double* b, float* val32,
+                             float* invd32) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.N) return;
-    if (!live(a, i)) { cnt[i] = 0; diag[i] = 1.0; b[i] = 0.0; return; }
+    if (!live(a, i)) {
+        cnt[i] = 0; diag[i] = 1.0; b[i] = 0.0;
+        if (invd32) invd32[i] = 1.0f;
+        return;
+    }
     const mps_params& P = a.P;
     const int bci = a.I[IBC][i];
     if (bci == P.other || bci == P.surface) {
         cnt[i] = 0; diag[i] = 1.0; b[i] = 0.0;
+        if (invd32) invd32[i] = 1.0f;
         return;
     }
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
@@ -57,6 +63,7 @@ __global__ void kPpeAssemble(KArgs a, double* val, int* col, int* cnt, double* d
                 sum -= mij;
                 if (!TRANS && a.I[IBC][j] == P.inner) {
                     val[(size_t)c * a.N + i] = mij;
+                    if (val32) val32[(size_t)c * a.N + i] = (float)mij;
                     col[(size_t)c * a.N + i] = j;
                     c++;
                 }
@@ -68,6 +75,7 @@ __global__ void kPpeAssemble(KArgs a, double* val, int* col, int* cnt, double* d
             const double dmj = u0 * u0 + u1 * u1 + u2 * u2;
             if (d2 < dmj || pw) {
                 val[(size_t)c * a.N + i] = mij;
+                if (val32) val32[(size_t)c * a.N + i] = (float)mij;
                 col[(size_t)c * a.N + i] = j;
                 c++;
             }
@@ -84,10 +92,14 @@ __global__ void kPpeAssemble(KArgs a, double* val, int* col, int* cnt, double* d
     cnt[i] = c;
     diag[i] = sum;
     b[i] = rhs;
+    if (invd32) invd32[i] = (sum != 0.0) ? (float)(1.0 / sum) : 1.0f;
     atomicAdd(&a.is[I_NNZ], c);
 }
 
 // ------------------------------------------------------------ reductions
+// Every vector kernel runs one thread per row; each block leaves a pair of
+// FP64 partial sums in red[2 * block], and a one-block kernel adds the
+// partials in block order, so every reduction is deterministic.
 __device__ __forceinline__ void blockReduce2(double& s0, double& s1) {
     __shared__ double sh[2][16];
     s0 = waveSum(s0);
@@ -101,8 +113,10 @@ __device__ __forceinline__ void blockReduce2(double& s0, double& s1) {
         s0 = t0; s1 = t1;
     }
 }
-
-// Sum the per-block partial pairs in a fixed order (one block of 1024 threads).
+__device__ __forceinline__ void writePartials(const KArgs& a, double s0, double s1) {
+    blockReduce2(s0, s1);
+    if (threadIdx.x == 0) { a.red[blockIdx.x * 2] = s0; a.red[blockIdx.x * 2 + 1] = s1; }
+}
 __device__ __forceinline__ void finishPartials(const double* red, int nb, double& s0, double& s1) {
     __shared__ double sh[2][16];
     double t0 = 0, t1 = 0;
@@ -116,13 +130,25 @@ __device__ __forceinline__ void finishPartials(const double* red, int nb, double
     for (int q = 0; q < (int)(blockDim.x >> 6); q++) { s0 += sh[0][q]; s1 += sh[1][q]; }
 }
 
-__device__ __forceinline__ double spmvRow(const KArgs& a, const double* val, const int* col, const int* cnt,
-                                          const double* diag, const double* x, int i) {
-    const int n = cnt[i];
-    double acc = diag[i] * x[i];
-    for (int c = 0; c < n; c++) {
-        const size_t o = (size_t)c * a.N + i;
-        acc += val[o] * x[col[o]];
+// y_i = diag_i x_i + sum_c val[c][i] x[col[c][i]]; ELL columns are read four
+// at a time so that four gathers are in flight per thread.
+template <typename T>
+__device__ __forceinline__ double ellRow(const T* __restrict__ val, const int* __restrict__ col, int n, size_t N,
+                                         const T* __restrict__ x, int i, double acc) {
+    int c = 0;
+    for (; c + 4 <= n; c += 4) {
+        const size_t o = (size_t)c * N + i;
+        const T v0 = val[o], v1 = val[o + N], v2 = val[o + 2 * N], v3 = val[o + 3 * N];
+        const int c0 = col[o], c1 = col[o + N], c2 = col[o + 2 * N], c3 = col[o + 3 * N];
+        const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        acc += (double)(v0 * x0);
+        acc += (double)(v1 * x1);
+        acc += (double)(v2 * x2);
+        acc += (double)(v3 * x3);
+    }
+    for (; c < n; c++) {
+        const size_t o = (size_t)c * N + i;
+        acc += (double)(val[o] * x[col[o]]);
     }
     return acc;
 }
@@ -139,22 +165,35 @@ struct CgPtrs {
     double* z;
     double* q;
     double* invd;
+    // mixed precision
+    const float* val32;
+    const float* invd32;
+    float* fr;
+    float* fp;
+    float* fz;
+    float* fq;
+    float* fd;
 };
 
+__device__ __forceinline__ double spmv64(const KArgs& a, const CgPtrs& c, const double* x, int i) {
+    return ellRow<double>(c.val, c.col, c.cnt[i], a.N, x, i, c.diag[i] * x[i]);
+}
+
+// ---------------------------------------------------------- FP64 PCG
 // r = b - M x; partials |b|^2, |r|^2; invd = 1/diag (1 where diag == 0)
 __global__ void kCgStart(KArgs a, CgPtrs c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double s0 = 0, s1 = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.N; i += gridDim.x * blockDim.x) {
+    if (i < a.N) {
         if (!live(a, i)) c.x[i] = 0.0;   // slots beyond the live count do not take part
         const double d = c.diag[i];
         c.invd[i] = (d != 0.0) ? 1.0 / d : 1.0;
-        const double r = c.b[i] - spmvRow(a, c.val, c.col, c.cnt, c.diag, c.x, i);
+        const double r = c.b[i] - spmv64(a, c, c.x, i);
         c.r[i] = r;
-        s0 += c.b[i] * c.b[i];
-        s1 += r * r;
+        s0 = c.b[i] * c.b[i];
+        s1 = r * r;
     }
-    blockReduce2(s0, s1);
-    if (threadIdx.x == 0) { a.red[blockIdx.x * 2] = s0; a.red[blockIdx.x * 2 + 1] = s1; }
+    writePartials(a, s0, s1);
 }
 
 __global__ void kCgStartFinish(KArgs a, int nb, double tol) {
@@ -175,21 +214,22 @@ __global__ void kCgStartFinish(KArgs a, int nb, double tol) {
 }
 
 __global__ void kCgZeroX(KArgs a, double* x) {
-    if (a.is[I_CG_DONE] != 2) return;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.N; i += gridDim.x * blockDim.x) x[i] = 0.0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.is[I_CG_DONE] != 2 || i >= a.N) return;
+    x[i] = 0.0;
 }
 
 // p = invd r; partial r.p
 __global__ void kCgInitP(KArgs a, CgPtrs c) {
     if (a.is[I_CG_DONE]) return;
-    double s0 = 0, s1 = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.N; i += gridDim.x * blockDim.x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s0 = 0;
+    if (i < a.N) {
         const double p = c.invd[i] * c.r[i];
         c.p[i] = p;
-        s0 += c.r[i] * p;
+        s0 = c.r[i] * p;
     }
-    blockReduce2(s0, s1);
-    if (threadIdx.x == 0) { a.red[blockIdx.x * 2] = s0; a.red[blockIdx.x * 2 + 1] = s1; }
+    writePartials(a, s0, 0.0);
 }
 __global__ void kCgInitPFinish(KArgs a, int nb) {
     if (a.is[I_CG_DONE]) return;
@@ -199,16 +239,16 @@ __global__ void kCgInitPFinish(KArgs a, int nb) {
 }
 
 // q = M p; partial p.q
-__global__ void kCgSpmv(KArgs a, CgPtrs c) {
+__global__ __launch_bounds__(256) void kCgSpmv(KArgs a, CgPtrs c) {
     if (a.is[I_CG_DONE]) return;
-    double s0 = 0, s1 = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.N; i += gridDim.x * blockDim.x) {
-        const double q = spmvRow(a, c.val, c.col, c.cnt, c.diag, c.p, i);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s0 = 0;
+    if (i < a.N) {
+        const double q = spmv64(a, c, c.p, i);
         c.q[i] = q;
-        s0 += c.p[i] * q;
+        s0 = c.p[i] * q;
     }
-    blockReduce2(s0, s1);
-    if (threadIdx.x == 0) { a.red[blockIdx.x * 2] = s0; a.red[blockIdx.x * 2 + 1] = s1; }
+    writePartials(a, s0, 0.0);
 }
 __global__ void kCgAlpha(KArgs a, int nb) {
     if (a.is[I_CG_DONE]) return;
@@ -222,19 +262,19 @@ __global__ void kCgAlpha(KArgs a, int nb) {
 // x += alpha p; r -= alpha q; z = invd r; partials |r|^2, r.z
 __global__ void kCgUpdate(KArgs a, CgPtrs c) {
     if (a.is[I_CG_DONE]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const double alpha = a.ds[S_ALPHA];
     double s0 = 0, s1 = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.N; i += gridDim.x * blockDim.x) {
+    if (i < a.N) {
         c.x[i] += alpha * c.p[i];
         const double r = c.r[i] - alpha * c.q[i];
         c.r[i] = r;
         const double z = c.invd[i] * r;
         c.z[i] = z;
-        s0 += r * r;
-        s1 += r * z;
+        s0 = r * r;
+        s1 = r * z;
     }
-    blockReduce2(s0, s1);
-    if (threadIdx.x == 0) { a.red[blockIdx.x * 2] = s0; a.red[blockIdx.x * 2 + 1] = s1; }
+    writePartials(a, s0, s1);
 }
 __global__ void kCgBeta(KArgs a, int nb, int maxIt) {
     if (a.is[I_CG_DONE]) return;
@@ -257,21 +297,22 @@ __global__ void kCgBeta(KArgs a, int nb, int maxIt) {
 // p = z + beta p
 __global__ void kCgP(KArgs a, CgPtrs c) {
     if (a.is[I_CG_DONE]) return;
-    const double beta = a.ds[S_BETA];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.N; i += gridDim.x * blockDim.x)
-        c.p[i] = c.z[i] + beta * c.p[i];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.N) c.p[i] = c.z[i] + a.ds[S_BETA] * c.p[i];
 }
 
-// true residual |b - M x|^2 and |b|^2 partials
-__global__ void kTrueResidual(KArgs a, CgPtrs c) {
+// true residual r = b - M x in FP64 (kept in c.r); partials |r|^2, |b|^2
+__global__ __launch_bounds__(256) void kTrueResidual(KArgs a, CgPtrs c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double s0 = 0, s1 = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.N; i += gridDim.x * blockDim.x) {
-        const double r = c.b[i] - spmvRow(a, c.val, c.col, c.cnt, c.diag, c.x, i);
-        s0 += r * r;
-        s1 += c.b[i] * c.b[i];
+    if (i < a.N) {
+        if (!live(a, i)) c.x[i] = 0.0;
+        const double r = c.b[i] - spmv64(a, c, c.x, i);
+        c.r[i] = r;
+        s0 = r * r;
+        s1 = c.b[i] * c.b[i];
     }
-    blockReduce2(s0, s1);
-    if (threadIdx.x == 0) { a.red[blockIdx.x * 2] = s0; a.red[blockIdx.x * 2 + 1] = s1; }
+    writePartials(a, s0, s1);
 }
 __global__ void kTrueResidualFinish(KArgs a, int nb) {
     double s0, s1;
@@ -280,6 +321,77 @@ __global__ void kTrueResidualFinish(KArgs a, int nb) {
         a.ds[S_TRUE_RN2] = s0;
         a.ds[S_BN2] = s1;
     }
+}
+
+// --------------------------------------- mixed precision (inner FP32 PCG)
+// Solves M d = r with FP32 values, SpMV and Jacobi preconditioner; every dot
+// product accumulates in FP64.  Convergence: |r_inner|^2 < tol_in^2 |r_0|^2.
+__global__ void kMxInit(KArgs a, CgPtrs c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s0 = 0, s1 = 0;
+    if (i < a.N) {
+        const float r = (float)c.r[i];
+        const float z = c.invd32[i] * r;
+        c.fr[i] = r;
+        c.fd[i] = 0.0f;
+        c.fz[i] = z;
+        c.fp[i] = z;
+        s0 = (double)r * (double)z;
+        s1 = (double)r * (double)r;
+    }
+    writePartials(a, s0, s1);
+}
+__global__ void kMxInitFinish(KArgs a, int nb, double tolIn) {
+    double rz, rn2;
+    finishPartials(a.red, nb, rz, rn2);
+    if (threadIdx.x != 0) return;
+    a.ds[S_ABSNEW] = rz;
+    a.ds[S_RN2] = rn2;
+    a.ds[S_THR] = fmax(tolIn * tolIn * rn2, DBL_MIN);
+    a.is[I_CG_ITER] = 0;
+    a.is[I_CG_DONE] = (rn2 == 0.0) ? 2 : 0;
+}
+__global__ __launch_bounds__(256) void kMxSpmv(KArgs a, CgPtrs c) {
+    if (a.is[I_CG_DONE]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s0 = 0;
+    if (i < a.N) {
+        const float p = c.fp[i];
+        const float q = (float)ellRow<float>(c.val32, c.col, c.cnt[i], a.N, c.fp, i, (double)((float)c.diag[i] * p));
+        c.fq[i] = q;
+        s0 = (double)p * (double)q;
+    }
+    writePartials(a, s0, 0.0);
+}
+__global__ void kMxUpdate(KArgs a, CgPtrs c) {
+    if (a.is[I_CG_DONE]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const float alpha = (float)a.ds[S_ALPHA];
+    double s0 = 0, s1 = 0;
+    if (i < a.N) {
+        c.fd[i] += alpha * c.fp[i];
+        const float r = c.fr[i] - alpha * c.fq[i];
+        c.fr[i] = r;
+        const float z = c.invd32[i] * r;
+        c.fz[i] = z;
+        s0 = (double)r * (double)r;
+        s1 = (double)r * (double)z;
+    }
+    writePartials(a, s0, s1);
+}
+__global__ void kMxP(KArgs a, CgPtrs c) {
+    if (a.is[I_CG_DONE]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.N) c.fp[i] = c.fz[i] + (float)a.ds[S_BETA] * c.fp[i];
+}
+__global__ void kZero(KArgs a, double* x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.N) x[i] = 0.0;
+}
+// x += d (FP64 accumulation of the correction)
+__global__ void kMxApply(KArgs a, CgPtrs c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.N && live(a, i)) c.x[i] += (double)c.fd[i];
 }
 
 // press = max(x, 0) for live slots (setZeroOnNegativePressure :471-476)
@@ -294,15 +406,37 @@ static CgPtrs cgPtrs(Ctx* ctx) {
     CgPtrs c;
     c.val = ctx->aVal; c.col = ctx->aCol; c.cnt = ctx->aCnt; c.diag = ctx->aDiag; c.b = ctx->D[DPPEB];
     c.x = ctx->D[DPPEX]; c.r = ctx->cgR; c.p = ctx->cgP; c.z = ctx->cgZ; c.q = ctx->cgQ; c.invd = ctx->cgInvD;
+    c.val32 = ctx->aVal32; c.invd32 = ctx->aDiagInv32;
+    c.fr = ctx->fR; c.fp = ctx->fP; c.fz = ctx->fZ; c.fq = ctx->fQ; c.fd = ctx->fX;
     return c;
 }
 
+static int ensureMixed(Ctx* ctx) {
+    if (ctx->aVal32) return MPS_OK;
+    const size_t n = ctx->N;
+    MPS_HIP(hipMalloc(&ctx->aVal32, sizeof(float) * (size_t)ctx->K * n));
+    MPS_HIP(hipMalloc(&ctx->aDiagInv32, sizeof(float) * n));
+    MPS_HIP(hipMalloc(&ctx->fR, sizeof(float) * n));
+    MPS_HIP(hipMalloc(&ctx->fP, sizeof(float) * n));
+    MPS_HIP(hipMalloc(&ctx->fZ, sizeof(float) * n));
+    MPS_HIP(hipMalloc(&ctx->fQ, sizeof(float) * n));
+    MPS_HIP(hipMalloc(&ctx->fX, sizeof(float) * n));
+    return MPS_OK;
+}
+
 int ppeAssemble(Ctx* ctx) {
+    const bool mixed = ctx->P.precision_mode == MPS_PREC_MIXED;
+    if (mixed) {
+        int rc = ensureMixed(ctx);
+        if (rc) return rc;
+    }
     KArgs a = ctx->args();
     const int g = gridFor(ctx->N, kB);
     const bool T = ctx->P.ppe_transpose != 0;
+    float* v32 = mixed ? ctx->aVal32 : nullptr;
+    float* i32 = mixed ? ctx->aDiagInv32 : nullptr;
     MPS_HIP(hipMemsetAsync(ctx->is + I_NNZ, 0, sizeof(int), ctx->stream));
-#define ASM(L, TT) hipLaunchKernelGGL((kPpeAssemble<L, TT>), dim3(g), dim3(kB), 0, ctx->stream, a, ctx->aVal, ctx->aCol, ctx->aCnt, ctx->aDiag, ctx->D[DPPEB])
+#define ASM(L, TT) hipLaunchKernelGGL((kPpeAssemble<L, TT>), dim3(g), dim3(kB), 0, ctx->stream, a, ctx->aVal, ctx->aCol, ctx->aCnt, ctx->aDiag, ctx->D[DPPEB], v32, i32)
     if (ctx->listValid) {
         if (T) ASM(true, true); else ASM(true, false);
     } else {
@@ -313,73 +447,156 @@ int ppeAssemble(Ctx* ctx) {
     return MPS_OK;
 }
 
-int ppeSolve(Ctx* ctx) {
-    KArgs a = ctx->args();
-    CgPtrs c = cgPtrs(ctx);
-    const int N = ctx->N;
-    const int nb = std::min(kRedBlocks, gridFor(N, kB));
+// Launch one SpMV kernel, bracketed by HIP events when profiling.
+template <typename K>
+static int timedSpmv(Ctx* ctx, K kern, const KArgs& a, const CgPtrs& c, int nb, int slot) {
+    hipStream_t s = ctx->stream;
+    if (ctx->profile) MPS_HIP(hipEventRecord(ctx->evPool[2 * slot], s));
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(kB), 0, s, a, c);
+    if (ctx->profile) MPS_HIP(hipEventRecord(ctx->evPool[2 * slot + 1], s));
+    return MPS_OK;
+}
+
+static void addSpmvStats(Ctx* ctx, const std::vector<float>& t, int real, double bytes) {
+    if (!ctx->profile) return;
+    real = std::min(real, (int)t.size());
+    for (int q = 0; q < real; q++) {
+        ctx->spmvMsTotal += t[q];
+        ctx->spmvBytesTotal += bytes;
+    }
+    ctx->spmvLaunches += real;
+}
+
+static int readStatus(Ctx* ctx, std::vector<float>* times, int batch) {
+    hipStream_t s = ctx->stream;
+    MPS_HIP(hipMemcpyAsync(ctx->hostInts, ctx->is, sizeof(int) * NIS, hipMemcpyDeviceToHost, s));
+    MPS_HIP(hipMemcpyAsync(ctx->hostScalars, ctx->ds, sizeof(double) * NDS, hipMemcpyDeviceToHost, s));
+    MPS_HIP(hipStreamSynchronize(s));
+    if (ctx->profile && times)
+        for (int q = 0; q < batch; q++) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, ctx->evPool[2 * q], ctx->evPool[2 * q + 1]);
+            times->push_back(ms);
+        }
+    return MPS_OK;
+}
+
+// Eigen-semantics Jacobi-PCG in FP64 (solveConjugateGradient, src/MpsPressure.cpp:424-445)
+static int solveFp64(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int batch) {
+    hipStream_t s = ctx->stream;
     const double tol = ctx->P.solver_tol > 0 ? ctx->P.solver_tol : DBL_EPSILON;
     const int maxIt = ctx->P.solver_max_iter > 0 ? ctx->P.solver_max_iter : 2 * ctx->nLiveHost;
-    hipStream_t s = ctx->stream;
-    MPS_HIP(hipEventRecord(ctx->ev0, s));
     hipLaunchKernelGGL(kCgStart, dim3(nb), dim3(kB), 0, s, a, c);
     hipLaunchKernelGGL(kCgStartFinish, dim3(1), dim3(1024), 0, s, a, nb, tol);
     hipLaunchKernelGGL(kCgZeroX, dim3(nb), dim3(kB), 0, s, a, c.x);
     hipLaunchKernelGGL(kCgInitP, dim3(nb), dim3(kB), 0, s, a, c);
     hipLaunchKernelGGL(kCgInitPFinish, dim3(1), dim3(1024), 0, s, a, nb);
-    const int batch = 8;
+    std::vector<float> times;
     int done = 0, iters = 0;
-    if (ctx->profile && ctx->evPool.size() < 2 * batch) {
-        ctx->evPool.resize(2 * batch);
-        for (auto& e : ctx->evPool) MPS_HIP(hipEventCreate(&e));
-    }
-    std::vector<float> spmvTimes;
     for (int launched = 0; !done; launched += batch) {
         for (int q = 0; q < batch; q++) {
-            if (ctx->profile) MPS_HIP(hipEventRecord(ctx->evPool[2 * q], s));
-            hipLaunchKernelGGL(kCgSpmv, dim3(nb), dim3(kB), 0, s, a, c);
-            if (ctx->profile) MPS_HIP(hipEventRecord(ctx->evPool[2 * q + 1], s));
+            timedSpmv(ctx, kCgSpmv, a, c, nb, q);
             hipLaunchKernelGGL(kCgAlpha, dim3(1), dim3(1024), 0, s, a, nb);
             hipLaunchKernelGGL(kCgUpdate, dim3(nb), dim3(kB), 0, s, a, c);
             hipLaunchKernelGGL(kCgBeta, dim3(1), dim3(1024), 0, s, a, nb, maxIt);
             hipLaunchKernelGGL(kCgP, dim3(nb), dim3(kB), 0, s, a, c);
         }
-        MPS_HIP(hipMemcpyAsync(ctx->hostInts, ctx->is, sizeof(int) * NIS, hipMemcpyDeviceToHost, s));
-        MPS_HIP(hipStreamSynchronize(s));
+        int rc = readStatus(ctx, &times, batch);
+        if (rc) return rc;
         done = ctx->hostInts[I_CG_DONE];
         iters = ctx->hostInts[I_CG_ITER];
-        if (ctx->profile)
-            for (int q = 0; q < batch; q++) {
-                float ms = 0.f;
-                hipEventElapsedTime(&ms, ctx->evPool[2 * q], ctx->evPool[2 * q + 1]);
-                spmvTimes.push_back(ms);
-            }
-        if (launched > 4 * maxIt + 64) break;   // safety net: the device flag ends every batch
+        if (launched > 4 * maxIt + 64) break;   // the device flag ends every batch; this only bounds the host loop
     }
-    hipLaunchKernelGGL(kTrueResidual, dim3(nb), dim3(kB), 0, s, a, c);
-    hipLaunchKernelGGL(kTrueResidualFinish, dim3(1), dim3(1024), 0, s, a, nb);
-    hipLaunchKernelGGL(kPressFromX, dim3(gridFor(N, kB)), dim3(kB), 0, s, a, c.x);
+    const int D = ctx->hostInts[I_CG_DONE];
+    addSpmvStats(ctx, times, (D == 1 || D == 3) ? iters + 1 : 0,
+                 12.0 * ctx->hostInts[I_NNZ] + 28.0 * ctx->nLiveHost);
+    ctx->solverIter = iters;
+    ctx->outerIter = 1;
+    ctx->solverErr = ctx->hostScalars[S_RHSN2] > 0 ? sqrt(ctx->hostScalars[S_RN2] / ctx->hostScalars[S_RHSN2]) : 0.0;
+    return MPS_OK;
+}
+
+// Mixed precision: FP64 iterative refinement on the true residual around an
+// FP32 Jacobi-PCG (FP64 dot products).  Stops when |b - A x| < tol |b| in FP64.
+static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int batch) {
+    constexpr int kMaxOuter = 30;
+    hipStream_t s = ctx->stream;
+    const double tol = ctx->P.solver_tol > 0 ? ctx->P.solver_tol : 1e-8;
+    const double tolIn = ctx->P.mixed_inner_tol > 0 ? ctx->P.mixed_inner_tol : 1e-4;
+    const int maxIt = ctx->P.solver_max_iter > 0 ? ctx->P.solver_max_iter : 2 * ctx->nLiveHost;
+    std::vector<float> times;
+    int total = 0, outer = 0, rc;
+    const double bytes32 = 8.0 * ctx->hostInts[I_NNZ] + 20.0 * ctx->nLiveHost;
+    for (;;) {
+        hipLaunchKernelGGL(kTrueResidual, dim3(nb), dim3(kB), 0, s, a, c);
+        hipLaunchKernelGGL(kTrueResidualFinish, dim3(1), dim3(1024), 0, s, a, nb);
+        if ((rc = readStatus(ctx, nullptr, 0))) return rc;
+        const double rn2 = ctx->hostScalars[S_TRUE_RN2], bn2 = ctx->hostScalars[S_BN2];
+        if (bn2 == 0.0) {   // zero right-hand side: x = 0, as the FP64 path
+            hipLaunchKernelGGL(kZero, dim3(nb), dim3(kB), 0, s, a, c.x);
+            break;
+        }
+        if (rn2 < tol * tol * bn2 || outer >= kMaxOuter || total >= maxIt) break;
+        outer++;
+        hipLaunchKernelGGL(kMxInit, dim3(nb), dim3(kB), 0, s, a, c);
+        hipLaunchKernelGGL(kMxInitFinish, dim3(1), dim3(1024), 0, s, a, nb, tolIn);
+        int done = 0, it = 0;
+        times.clear();
+        for (int launched = 0; !done; launched += batch) {
+            for (int q = 0; q < batch; q++) {
+                timedSpmv(ctx, kMxSpmv, a, c, nb, q);
+                hipLaunchKernelGGL(kCgAlpha, dim3(1), dim3(1024), 0, s, a, nb);
+                hipLaunchKernelGGL(kMxUpdate, dim3(nb), dim3(kB), 0, s, a, c);
+                hipLaunchKernelGGL(kCgBeta, dim3(1), dim3(1024), 0, s, a, nb, maxIt);
+                hipLaunchKernelGGL(kMxP, dim3(nb), dim3(kB), 0, s, a, c);
+            }
+            if ((rc = readStatus(ctx, &times, batch))) return rc;
+            done = ctx->hostInts[I_CG_DONE];
+            it = ctx->hostInts[I_CG_ITER];
+            if (launched > 4 * maxIt + 64) break;
+        }
+        const int D = ctx->hostInts[I_CG_DONE];
+        addSpmvStats(ctx, times, (D == 1 || D == 3) ? it + 1 : 0, bytes32);
+        total += it + 1;
+        hipLaunchKernelGGL(kMxApply, dim3(nb), dim3(kB), 0, s, a, c);
+    }
+    ctx->solverIter = total;
+    ctx->outerIter = outer;
+    ctx->solverErr = ctx->hostScalars[S_BN2] > 0 ? sqrt(ctx->hostScalars[S_TRUE_RN2] / ctx->hostScalars[S_BN2]) : 0.0;
+    return MPS_OK;
+}
+
+int ppeSolve(Ctx* ctx) {
+    const bool mixed = ctx->P.precision_mode == MPS_PREC_MIXED;
+    if (mixed && !ctx->aVal32) {
+        ctx->err = "mixed-precision solve needs a mixed-precision assembly first";
+        return MPS_E_ARG;
+    }
+    KArgs a = ctx->args();
+    CgPtrs c = cgPtrs(ctx);
+    const int N = ctx->N;
+    const int nb = gridFor(N, kB);
+    const int batch = 8;
+    hipStream_t s = ctx->stream;
+    if (ctx->profile && ctx->evPool.size() < 2 * batch) {
+        ctx->evPool.resize(2 * batch);
+        for (auto& e : ctx->evPool) MPS_HIP(hipEventCreate(&e));
+    }
+    MPS_HIP(hipEventRecord(ctx->ev0, s));
+    int rc = mixed ? solveMixed(ctx, a, c, nb, batch) : solveFp64(ctx, a, c, nb, batch);
+    if (rc) return rc;
+    if (!mixed) {   // the mixed path ends on a fresh true residual already
+        hipLaunchKernelGGL(kTrueResidual, dim3(nb), dim3(kB), 0, s, a, c);
+        hipLaunchKernelGGL(kTrueResidualFinish, dim3(1), dim3(1024), 0, s, a, nb);
+    }
+    hipLaunchKernelGGL(kPressFromX, dim3(nb), dim3(kB), 0, s, a, c.x);
     MPS_HIP(hipEventRecord(ctx->ev1, s));
-    MPS_HIP(hipMemcpyAsync(ctx->hostScalars, ctx->ds, sizeof(double) * NDS, hipMemcpyDeviceToHost, s));
-    MPS_HIP(hipStreamSynchronize(s));
+    if ((rc = readStatus(ctx, nullptr, 0))) return rc;
     MPS_HIP(hipGetLastError());
     float ms = 0.f;
     hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     ctx->ppeMs = ms;
     const double* hs = ctx->hostScalars;
-    ctx->solverIter = iters;
-    if (ctx->profile) {
-        // launches that did work: one per iteration plus the one that converged
-        const int real = std::min((int)spmvTimes.size(), ctx->hostInts[I_CG_DONE] == 1 || ctx->hostInts[I_CG_DONE] == 3 ? iters + 1 : 0);
-        const double nnz = (double)ctx->hostInts[I_NNZ];
-        const double bytes = nnz * 12.0 + 28.0 * (double)ctx->nLiveHost;
-        for (int q = 0; q < real; q++) {
-            ctx->spmvMsTotal += spmvTimes[q];
-            ctx->spmvBytesTotal += bytes;
-        }
-        ctx->spmvLaunches += real;
-    }
-    ctx->solverErr = hs[S_RHSN2] > 0 ? sqrt(hs[S_RN2] / hs[S_RHSN2]) : 0.0;
     ctx->trueResidual = hs[S_BN2] > 0 ? sqrt(hs[S_TRUE_RN2] / hs[S_BN2]) : 0.0;
     return MPS_OK;
 }
