@@ -180,9 +180,19 @@ int mps_read_grid(const char* path, int* n, int* type, double* pos, double* vel,
 /* .grid writer in the same format (checkpoint / restart) */
 int mps_write_grid(const char* path, int n, const int* type, const double* pos, const double* vel,
                    const double* press, const double* press_avg);
+/* Output options (flags.output_VTU of the JSON file) */
+#define MPS_VTU_PND 1
+#define MPS_VTU_NEIGH 2
+#define MPS_VTU_DEVIATION 4
+#define MPS_VTU_CONCENTRATION 8
+#define MPS_VTU_AUXILIAR 16
+#define MPS_VTU_ONLY_FREE_SURFACE 32  /* parsed; the writer always writes every particle */
+#define MPS_VTU_BINARY 64             /* parsed; the writer always writes ASCII */
+/* The output part of MpsInputOutput::readInputFile (src/MpsInputOutput.cpp:140-151,
+ * 173-179): pathNames.vtu_output_folder and the MPS_VTU_* flags. */
+int mps_output_from_json(const char* json_path, char* vtu_folder, size_t folder_cap, int* vtu_flags);
 /* MpsInputOutput::writeVtuAscii (src/MpsInputOutput.cpp:1313-1579) for the
- * context's current state; flags bit0 pnd, bit1 neigh, bit2 deviation,
- * bit3 concentration, bit4 auxiliar. */
+ * context's current state; flags: MPS_VTU_PND .. MPS_VTU_AUXILIAR. */
 int mps_write_vtu(mps_ctx* ctx, const char* path, int flags);
 
 #ifdef __cplusplus

@@ -72,6 +72,20 @@ def build(force: bool = False, jobs: int | None = None) -> Path:
     return LIB
 
 
+DRIVER = PKG / "polymps_gpu"
+
+
+def build_driver() -> Path:
+    """Link the polymps_gpu program (host/main.cpp over libmps_gpu.so)."""
+    host = PKG / "host"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", f"-I{host}", str(host / "main.cpp"),
+           f"-L{PKG}", "-lmps_gpu", "-Wl,-rpath,$ORIGIN", "-o", str(DRIVER)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    return DRIVER
+
+
 def build_oracle() -> Path:
     """Compile the CPU oracle (test infrastructure) with its own Makefile."""
     odir = ROOT / "oracle"
@@ -84,4 +98,5 @@ def build_oracle() -> Path:
 if __name__ == "__main__":
     force = "--force" in sys.argv
     print(build(force=force))
+    print(build_driver())
     print(build_oracle())

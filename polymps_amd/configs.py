@@ -227,6 +227,85 @@ def dam2d_wc(l0: float = 0.005, tank=(1.0, 0.6), column=(0.25, 0.5), seed: int =
                     info={"fluid": nf, "walls": n - nf, "seed": seed})
 
 
+# -------------------------------------------------------------- case files
+def case_json(p: MpsParams, grid: str, mesh: str, vtu_folder: str, vtu: dict | None = None) -> dict:
+    """The parameter block as an input JSON file of the reference's schema
+    (the keys read by src/MpsInputOutput.cpp:136-500, plus the solver
+    extension keys under numerical.semi_implicit_mps)."""
+    ax = ("x", "y", "z")
+    return {
+        "flags": {"wall_type": int(p.wall_type), "fem_MESH": bool(p.fem_on), "forced_MESH": bool(p.forced_on),
+                  "output_VTU": dict({"type": 0, "only_freeSurface": False, "pnd": False, "neigh": False,
+                                      "deviation": False, "concentration": False, "auxiliar": False},
+                                     **(vtu or {}))},
+        "pathNames": {"particle_grid_file": grid, "mesh_rigid_file": mesh, "vtu_output_folder": vtu_folder},
+        "domain": {"min": {a: float(p.domain_min[k]) for k, a in enumerate(ax)},
+                   "max": {a: float(p.domain_max[k]) for k, a in enumerate(ax)}},
+        "physical": {"fluid_density": p.density_fluid, "wall_density": p.density_wall,
+                     "kinematic_visc": p.knm_vs1, "fluid_type": int(p.fluid_type),
+                     "gravity": {a: float(p.gravity[k]) for k, a in enumerate(ax)},
+                     "rheological": {"kinematic_visc_phase_2": p.knm_vs2, "fluid_density_phase_1": p.dns_fl1,
+                                     "fluid_density_phase_2": p.dns_fl2}},
+        "numerical": {
+            "dimension": p.dim, "particle_dist": p.part_dist, "time_step": p.time_step,
+            "final_time": p.time_simulation, "iter_output": int(p.iter_output), "CFL_number": p.cfl_number,
+            "weight_type": int(p.weight_type), "slip_condition": int(p.slip_condition),
+            "effective_radius": {"small": p.re_s, "large": p.re_l},
+            "gradient": {"type": int(p.gradient_type), "correction": bool(p.gradient_correction),
+                         "relax_fact": p.relax_press},
+            "mps_type": int(p.mps_type),
+            "explicit_mps": {"equation_state": {"speed_sound": p.sound_speed, "gamma": p.gamma}},
+            "semi_implicit_mps": {"solver_type": int(p.solver_type),
+                                  "weak_compressibility": {"alpha": p.alpha_compressibility},
+                                  "source_term": {"relax_pnd": p.relax_pnd},
+                                  "tolerance": p.solver_tol, "max_iterations": int(p.solver_max_iter),
+                                  "precision_mode": int(p.precision_mode),
+                                  "eigen_transpose": int(p.ppe_transpose),
+                                  "mixed_inner_tolerance": p.mixed_inner_tol},
+            "particle_shifting": {"type": int(p.shifting_type), "DRI": p.dri, "coef_A": p.coef_a,
+                                  "mach_number": p.mach_number},
+            "pnd": {"type": int(p.pnd_type), "diffusive_coeff": p.diffusive_coef},
+            "wall_repulsive_force": {"type": int(p.repulsive_force_type), "re": p.re_repulsive_force,
+                                     "maxVel": p.expect_max_velocity,
+                                     "coefficient": {"Mitsume": p.rep_coef_mitsume,
+                                                     "Lennard-Jones": p.rep_coef_lennard_jones,
+                                                     "Monaghan-Kajtar": p.rep_coef_monaghan_kajtar},
+                                     "eps_re": p.eps_re},
+            "free_surface_threshold": {"type": int(p.free_surf_type), "pnd": p.pnd_threshold,
+                                       "neigh": p.neigh_threshold, "NPCD": p.npcd_threshold,
+                                       "ARC": p.theta_threshold, "normal": p.norm_threshold},
+            "particle_collision": {"type": int(p.collision_type), "ratio": p.collision_ratio,
+                                   "dist_limit_ratio": p.dist_limit_ratio, "lambda": p.lambda_collision},
+            "particle_type": {"ghost": int(p.ghost), "fluid": int(p.fluid), "wall": int(p.wall),
+                              "dummyWall": int(p.dummy_wall)},
+            "boundary_type": {"free_surface": int(p.surface), "inner": int(p.inner), "other": int(p.other)},
+        },
+    }
+
+
+def write_case(w: Workload, folder, name: str = "case", vtu: dict | None = None) -> Path:
+    """Write <folder>/<name>.json, <name>.grid and (polygon walls) <name>.stl
+    with paths relative to <folder>; returns the JSON path."""
+    import json
+    folder = Path(folder)
+    folder.mkdir(parents=True, exist_ok=True)
+    grid = f"{name}.grid"
+    lib = _lib.lib()
+    n = w.n
+    zeros = np.zeros(n)
+    rc = lib.mps_write_grid(str(folder / grid).encode(), n, _lib._ptr(w.types), _lib._ptr(w.pos),
+                            _lib._ptr(w.vel), _lib._ptr(zeros), _lib._ptr(zeros))
+    if rc != 0:
+        raise _lib.MpsError(f"mps_write_grid failed ({rc})")
+    mesh = "oops"
+    if w.mesh is not None:
+        mesh = f"{name}.stl"
+        write_stl(folder / mesh, w.mesh[0], w.mesh[1])
+    path = folder / f"{name}.json"
+    path.write_text(json.dumps(case_json(w.params, grid, mesh, f"{name}_out", vtu), indent=1))
+    return path
+
+
 def small_dam3d(l0: float = 0.025, **kw) -> Workload:
     """The 3-D dam break at a coarse spacing that tiles the tank
     (0.025 m: 24 x 6 x 12 = 1,728 particles; 0.0125 m: 13,824)."""

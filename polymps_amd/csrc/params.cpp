@@ -481,6 +481,34 @@ int mps_params_from_json(const char* json_path, mps_params* p, char* grid_path, 
     return MPS_OK;
 }
 
+int mps_output_from_json(const char* json_path, char* vtu_folder, size_t folder_cap, int* vtu_flags) {
+    if (!json_path) return MPS_E_ARG;
+    std::ifstream f(json_path);
+    if (!f) { mpsgpu::setGlobalError(std::string("cannot open ") + json_path); return MPS_E_IO; }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string text = ss.str();
+    mpsgpu::JVal root;
+    std::string err;
+    if (!mpsgpu::JParser(text).parse(root, err)) { mpsgpu::setGlobalError("JSON parse error: " + err); return MPS_E_IO; }
+    const mpsgpu::JVal* fl = root.get("flags");
+    const mpsgpu::JVal* vtu = fl ? fl->get("output_VTU") : nullptr;
+    const mpsgpu::JVal* pn = root.get("pathNames");
+    if (vtu_folder && folder_cap) snprintf(vtu_folder, folder_cap, "%s", mpsgpu::jstr(pn, "vtu_output_folder", "oops").c_str());
+    if (vtu_flags) {
+        int v = 0;
+        if (mpsgpu::jint(vtu, "pnd", 0)) v |= MPS_VTU_PND;
+        if (mpsgpu::jint(vtu, "neigh", 0)) v |= MPS_VTU_NEIGH;
+        if (mpsgpu::jint(vtu, "deviation", 0)) v |= MPS_VTU_DEVIATION;
+        if (mpsgpu::jint(vtu, "concentration", 0)) v |= MPS_VTU_CONCENTRATION;
+        if (mpsgpu::jint(vtu, "auxiliar", 0)) v |= MPS_VTU_AUXILIAR;
+        if (mpsgpu::jint(vtu, "only_freeSurface", 0)) v |= MPS_VTU_ONLY_FREE_SURFACE;
+        if (mpsgpu::jint(vtu, "type", 0)) v |= MPS_VTU_BINARY;
+        *vtu_flags = v;
+    }
+    return MPS_OK;
+}
+
 int mps_params_derive(mps_params* p, int n, const double* pos, const int* type) {
     (void)type;
     if (!p) return MPS_E_ARG;
