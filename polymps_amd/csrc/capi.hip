@@ -29,8 +29,11 @@ KArgs Ctx::args() const {
     for (int f = 0; f < ND; f++) a.D[f] = D[f];
     for (int f = 0; f < NI; f++) a.I[f] = I[f];
     a.cellStart = cellStart;
+    a.nbrMask = nbrMask;
+    a.maskW = maskW;
     a.nbr = nbr;
     a.nbrCnt = nbrCnt;
+    a.P4 = P4;
     a.K = K;
     a.mV = mV;
     a.mF = mF;
@@ -175,8 +178,10 @@ static int syncStatus(Ctx* ctx) {
     ctx->velMaxHost = ctx->hostScalars[S_VELMAX];
     if (ctx->hostInts[I_OVERFLOW] > 0) {
         char buf[160];
-        snprintf(buf, sizeof(buf), "neighbour list overflow: a particle has %d candidates, capacity %d",
-                 ctx->hostInts[I_OVERFLOW], ctx->K);
+        snprintf(buf, sizeof(buf),
+                 "neighbour table overflow: %d entries where %d (stencil row) / %d (PPE row) fit; "
+                 "particles are packed far denser than the particle distance",
+                 ctx->hostInts[I_OVERFLOW], 32 * ctx->maskW, ctx->K);
         ctx->err = buf;
         return MPS_E_CAPACITY;
     }
@@ -231,7 +236,10 @@ static int calcPress(Ctx* ctx) {
 static int runStep(Ctx* ctx) {
     const mps_params& P = ctx->P;
     int rc;
-#define OP(name) if ((rc = launchOp(ctx, name))) return rc
+#define OP(name)                                   \
+    do {                                           \
+        if ((rc = launchOp(ctx, name))) return rc; \
+    } while (0)
     if (P.relax_press < 1.0) { ctx->err = "relax_fact < 1 (prediction pressure gradient) is not supported"; return MPS_E_UNSUPPORTED; }
     if ((rc = launchBucketSort(ctx))) return rc;
     OP("viscosity");
@@ -250,7 +258,12 @@ static int runStep(Ctx* ctx) {
         OP("update_near_mesh");
         OP("wall_npcd");
     }
-    if (P.gradient_correction) OP("correction_matrix");
+    // The first correction matrix of the step (src/main.cpp:317-319) is read
+    // before the second call rewrites every entry (:375-377) only by the
+    // diffusive PND and the velocity divergence of the DivU source term;
+    // otherwise it is never read and is skipped.
+    const bool cmReadEarly = P.pnd_type == MPS_PND_DIFFUSIVE || P.mps_type == MPS_IMPLICIT_PND_DIVU;
+    if (P.gradient_correction && cmReadEarly) OP("correction_matrix");
     OP("pnd_neigh_npcd");
     if (P.pnd_type == MPS_PND_DIFFUSIVE) OP("pnd_diffusive");
     else if (P.pnd_type == MPS_PND_MEAN_SUM_WIJ) OP("mean_pnd");
@@ -319,8 +332,17 @@ int mps_create(mps_ctx** out, const mps_params* p, int n, int device) {
     const double rr = std::max(p->re_l, p->re_s) / p->part_dist;
     const double ball = ((int)p->dim == 3) ? 4.18879 * rr * rr * rr : 3.14159 * rr * rr;
     ctx->K = std::max(32, ((int)(ball * 2.0) + 31) / 32 * 32);
+    // neighbour bitmasks: a stencil row spans 3 buckets of side bucket_side,
+    // (bucket_side / l0)^dim particles each at rest; 1.6x that leaves room for
+    // compression (the build reports an overflow as MPS_E_CAPACITY)
+    const double perRow = 3.0 * std::pow(p->bucket_side / p->part_dist, p->dim);
+    ctx->maskW = std::max(1, (int)std::ceil(1.6 * perRow / 32.0));
+    const int rows = ((int)p->dim == 3) ? 9 : 3;
+    MPS_HIP(hipMalloc(&ctx->nbrMask, sizeof(unsigned) * (size_t)rows * ctx->maskW * n));
     MPS_HIP(hipMalloc(&ctx->nbr, sizeof(int) * (size_t)ctx->K * n));
     MPS_HIP(hipMalloc(&ctx->nbrCnt, ni));
+    MPS_HIP(hipMemsetAsync(ctx->nbrCnt, 0, ni, ctx->stream));
+    MPS_HIP(hipMalloc(&ctx->P4, sizeof(double4) * (size_t)n));
     MPS_HIP(hipMalloc(&ctx->aVal, sizeof(double) * (size_t)ctx->K * n));
     MPS_HIP(hipMalloc(&ctx->aCol, sizeof(int) * (size_t)ctx->K * n));
     MPS_HIP(hipMalloc(&ctx->aCnt, ni));
@@ -339,7 +361,6 @@ int mps_create(mps_ctx** out, const mps_params* p, int n, int device) {
     MPS_HIP(hipHostMalloc(&ctx->hostInts, sizeof(int) * NIS));
     MPS_HIP(hipMemsetAsync(ctx->ds, 0, sizeof(double) * NDS, ctx->stream));
     MPS_HIP(hipMemsetAsync(ctx->is, 0, sizeof(int) * NIS, ctx->stream));
-    MPS_HIP(hipMemsetAsync(ctx->nbrCnt, 0, ni, ctx->stream));
     buildWallTables(ctx);
     std::vector<double> wt(4 * kWallTable);
     for (int q = 0; q < kWallTable; q++) {
@@ -362,7 +383,7 @@ int mps_destroy(mps_ctx* ctx) {
     for (int f = 0; f < NI; f++) hipFree(ctx->I[f]);
     for (int f = 0; f < NPI; f++) hipFree(ctx->Ialt[f]);
     void* bufs[] = {ctx->cellStart, ctx->cellCount, ctx->key, ctx->rank, ctx->order, ctx->order2, ctx->slotOf,
-                    ctx->scanTmp, ctx->outList, ctx->nbr, ctx->nbrCnt, ctx->mV, ctx->mF, ctx->mN, ctx->wtab,
+                    ctx->scanTmp, ctx->outList, ctx->nbrMask, ctx->nbr, ctx->nbrCnt, ctx->P4, ctx->mV, ctx->mF, ctx->mN, ctx->wtab,
                     ctx->aVal, ctx->aCol, ctx->aCnt, ctx->aDiag, ctx->aVal32, ctx->aDiagInv32, ctx->cgR, ctx->cgP,
                     ctx->cgZ, ctx->cgQ, ctx->cgInvD, ctx->cgX, ctx->fR, ctx->fP, ctx->fZ, ctx->fQ, ctx->fX,
                     ctx->ds, ctx->is, ctx->red};
@@ -485,7 +506,8 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
         hipStreamSynchronize(ctx->stream);
         hipFree(tmp);
     }
-    if (fd->f0 == DX) ctx->listValid = false;   // positions moved: the neighbour list is stale
+    // positions or types changed: the neighbour list and its packed copy are stale
+    if ((fd->kind != 2 && fd->f0 == DX) || (fd->kind == 2 && fd->f0 == ITYPE)) ctx->listValid = false;
     MPS_HIP(hipGetLastError());
     return MPS_OK;
 }
@@ -508,7 +530,7 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "true_residual") *out = ctx->trueResidual;
     else if (s == "fallback") *out = ctx->fallbackUsed;
     else if (s == "list_max") *out = ctx->hostInts[I_LIST_MAX];
-    else if (s == "list_capacity") *out = ctx->K;
+    else if (s == "list_capacity") *out = 32 * ctx->maskW;
     else if (s == "capacity") *out = ctx->N;
     else if (s == "nnz") *out = ctx->hostInts[I_NNZ];
     else if (s == "spmv_ms_total") *out = ctx->spmvMsTotal;

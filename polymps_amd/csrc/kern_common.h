@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "mps_internal.h"
 
 namespace mpsgpu {
@@ -33,6 +35,11 @@ __device__ __forceinline__ double delWgt(double dst, double re, int t) {
         default: return -re / (dst * dst);
     }
 }
+
+// Slot handled by this thread in a one-thread-per-slot launch.  (A
+// renumbering that gives each of the 8 XCDs one contiguous slot range was
+// measured slower than this plain mapping on the 1.73M dam break.)
+__device__ __forceinline__ int slotId() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
 // Particle slot k takes part in per-particle loops when its reference index
 // is below the live particle count (the reference loops i < numParticles).
@@ -70,12 +77,89 @@ __device__ __forceinline__ void walkCells(const KArgs& a, double x, double y, do
     }
 }
 
-// Iterate the per-step neighbour list of slot i (walk order).
+// walkCells with the neighbour's position handed to f(j, xj, yj, zj); slot
+// i itself is skipped.  Positions of four consecutive candidates are loaded
+// before the first of them is tested, so the loads overlap.
 template <class F>
-__device__ __forceinline__ void listNbrs(const KArgs& a, int i, F&& f) {
+__device__ __forceinline__ void walkCellsPos(const KArgs& a, int i, double x, double y, double z, F&& f) {
+    const mps_params& P = a.P;
+    const int ix = (int)((x - P.domain_min[0]) * P.inv_bucket_side + KEPS);
+    const int iy = (int)((y - P.domain_min[1]) * P.inv_bucket_side + KEPS);
+    const int iz = (int)((z - P.domain_min[2]) * P.inv_bucket_side + KEPS);
+    const int d3 = (int)(P.dim - 2.0);
+    const int nx = P.num_buckets_x, ny = P.num_buckets_y, nz = P.num_buckets_z;
+    const double* __restrict__ X = a.D[DX];
+    const double* __restrict__ Y = a.D[DY];
+    const double* __restrict__ Z = a.D[DZ];
+    for (int jz = (iz - 1) * d3; jz <= (iz + 1) * d3; jz++) {
+        if (jz < 0 || jz >= nz) continue;
+        for (int jy = iy - 1; jy <= iy + 1; jy++) {
+            if (jy < 0 || jy >= ny) continue;
+            const int row = jz * P.num_buckets_xy + jy * nx;
+            int jx0 = ix - 1, jx1 = ix + 1;
+            if (jx0 < 0) jx0 = 0;
+            if (jx1 >= nx) jx1 = nx - 1;
+            if (jx0 > jx1) continue;
+            const int s = a.cellStart[row + jx0];
+            const int e = a.cellStart[row + jx1 + 1];
+            int j = s;
+            for (; j + 4 <= e; j += 4) {
+                const double x0 = X[j], x1 = X[j + 1], x2 = X[j + 2], x3 = X[j + 3];
+                const double y0 = Y[j], y1 = Y[j + 1], y2 = Y[j + 2], y3 = Y[j + 3];
+                const double z0 = Z[j], z1 = Z[j + 1], z2 = Z[j + 2], z3 = Z[j + 3];
+                if (j != i) f(j, x0, y0, z0);
+                if (j + 1 != i) f(j + 1, x1, y1, z1);
+                if (j + 2 != i) f(j + 2, x2, y2, z2);
+                if (j + 3 != i) f(j + 3, x3, y3, z3);
+            }
+            for (; j < e; j++)
+                if (j != i) f(j, X[j], Y[j], Z[j]);
+        }
+    }
+}
+
+// The per-step neighbour list of slot i: column-major [K][N] slot indices in
+// bucket-walk order, nbrCnt[i] entries.  Four entries and their packed
+// (x, y, z, type) copies are loaded per batch before g(j, q) runs for each.
+template <class G>
+__device__ __forceinline__ void listNbrsQ(const KArgs& a, int i, G&& g) {
     const int n = a.nbrCnt[i];
-    const int* col = a.nbr + i;
-    for (int c = 0; c < n; c++) f(col[(size_t)c * a.N]);
+    const int* __restrict__ col = a.nbr + i;
+    const double4* __restrict__ P4 = a.P4;
+    const size_t N = a.N;
+    int c = 0;
+    for (; c + 4 <= n; c += 4) {
+        const int j0 = col[c * N], j1 = col[(c + 1) * N], j2 = col[(c + 2) * N], j3 = col[(c + 3) * N];
+        const double4 q0 = P4[j0], q1 = P4[j1], q2 = P4[j2], q3 = P4[j3];
+        g(j0, q0);
+        g(j1, q1);
+        g(j2, q2);
+        g(j3, q3);
+    }
+    for (; c < n; c++) {
+        const int j = col[c * N];
+        const double4 q = P4[j];
+        g(j, q);
+    }
+}
+
+// Neighbours of slot i with their positions, f(j, xj, yj, zj): from the list
+// (LIST) or from a bucket walk.
+template <bool LIST, class F>
+__device__ __forceinline__ void forNbrsPos(const KArgs& a, int i, double x, double y, double z, F&& f) {
+    if (LIST) listNbrsQ(a, i, [&](int j, const double4& q) { f(j, q.x, q.y, q.z); });
+    else walkCellsPos(a, i, x, y, z, f);
+}
+
+// The same with the neighbour's particle type, f(j, xj, yj, zj, tj).
+template <bool LIST, class F>
+__device__ __forceinline__ void forNbrsPosType(const KArgs& a, int i, double x, double y, double z, F&& f) {
+    if (LIST) {
+        listNbrsQ(a, i, [&](int j, const double4& q) { f(j, q.x, q.y, q.z, (int)q.w); });
+    } else {
+        const int* __restrict__ T = a.I[ITYPE];
+        walkCellsPos(a, i, x, y, z, [&](int j, double xj, double yj, double zj) { f(j, xj, yj, zj, T[j]); });
+    }
 }
 
 // Rref = I - 2 n n^T, n = normalised 0.5 (x - xm), |n| > eps test on |n|
@@ -120,6 +204,81 @@ __device__ __forceinline__ double waveSum(double v) {
 __device__ __forceinline__ double waveMax(double v) {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     return v;
+}
+
+__device__ __forceinline__ int waveMinInt(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int waveMaxInt(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Wave-cooperative stencil walk for one-wave (64-thread) workgroups.
+//
+// The rows (jz, jy) of the 3x3(x3) stencil are taken in walk order.  For
+// each row the wave forms the union [us, ue) of its lanes' candidate ranges
+// [s, e) (the three jx buckets of the row), stages it into LDS 64 slots at a
+// time with one coalesced fetch per lane, and every lane then runs over the
+// staged slots in ascending order, visiting those in its own range.  Each
+// lane therefore visits exactly the slots of its own bucket walk, in walk
+// order, while the loads are shared and the LDS reads are broadcasts.  When
+// the union is longer than maxUnion (lanes of the wave sit in different
+// bucket rows) the row is walked per lane with direct loads instead.
+//
+//   fetch(j) -> T             the candidate data a visit needs
+//   begin(r, s, e)            row r starts; the lane's range (e may be lowered)
+//   visit(j, const T&)        every j of the lane's range, ascending (j == i included)
+//   end(r, s, e)              row r is done
+template <class T, class Fetch, class Begin, class Visit, class End>
+__device__ __forceinline__ void coopRows(const KArgs& a, bool act, double x, double y, double z, T* stage,
+                                         int maxUnion, Fetch&& fetch, Begin&& begin, Visit&& visit, End&& end) {
+    const mps_params& P = a.P;
+    const int ix = (int)((x - P.domain_min[0]) * P.inv_bucket_side + KEPS);
+    const int iy = (int)((y - P.domain_min[1]) * P.inv_bucket_side + KEPS);
+    const int iz = (int)((z - P.domain_min[2]) * P.inv_bucket_side + KEPS);
+    const int nx = P.num_buckets_x, ny = P.num_buckets_y, nz = P.num_buckets_z;
+    const int rows = ((int)P.dim == 3) ? 9 : 3;
+    const int t = threadIdx.x;
+    for (int r = 0; r < rows; r++) {
+        int s = 0, e = 0;
+        if (act) {
+            const int jz = ((int)P.dim == 3) ? iz - 1 + r / 3 : 0;
+            const int jy = iy - 1 + r % 3;
+            if (!(jz < 0 || jz >= nz || jy < 0 || jy >= ny)) {
+                const int row = jz * P.num_buckets_xy + jy * nx;
+                int jx0 = ix - 1, jx1 = ix + 1;
+                if (jx0 < 0) jx0 = 0;
+                if (jx1 >= nx) jx1 = nx - 1;
+                if (jx0 <= jx1) {
+                    s = a.cellStart[row + jx0];
+                    e = a.cellStart[row + jx1 + 1];
+                }
+            }
+            begin(r, s, e);
+        }
+        const bool has = act && e > s;
+        const int us = waveMinInt(has ? s : INT_MAX);
+        const int ue = waveMaxInt(has ? e : INT_MIN);
+        if (us < ue) {
+            if (ue - us <= maxUnion) {
+                for (int cb = us; cb < ue; cb += 64) {
+                    __syncthreads();
+                    if (cb + t < ue) stage[t] = fetch(cb + t);
+                    __syncthreads();
+                    const int m = min(64, ue - cb);
+                    for (int k = 0; k < m; k++) {
+                        const int j = cb + k;
+                        if (j >= s && j < e) visit(j, stage[k]);
+                    }
+                }
+            } else if (has) {
+                for (int j = s; j < e; j++) visit(j, fetch(j));
+            }
+        }
+        if (act) end(r, s, e);
+    }
 }
 
 }  // namespace mpsgpu

@@ -53,9 +53,12 @@ struct KArgs {
     double* D[ND];
     int* I[NI];
     const int* cellStart;  // nB + 2 (last entry = N)
-    const int* nbr;        // column-major [K][N]
+    const unsigned* nbrMask;  // candidate bitmasks [rows][maskW][N], see kBuildList
+    int maskW;                // 32-bit words per stencil row
+    const int* nbr;           // neighbour list, column-major [K][N] (decoded from the bitmasks)
     const int* nbrCnt;
-    int K;
+    double4* P4;              // (x, y, z, type) per slot, written by the list build; valid with the list
+    int K;                    // PPE row capacity (ELL width)
     const double* mV;      // mesh
     const int* mF;
     const double* mN;
@@ -89,8 +92,11 @@ struct Ctx {
     int* scanTmp = nullptr;
     int* outList = nullptr;     // compaction candidates
     // neighbour list
+    unsigned* nbrMask = nullptr;
+    int maskW = 0;
     int* nbr = nullptr;
     int* nbrCnt = nullptr;
+    double4* P4 = nullptr;
     int K = 0;
     bool listValid = false;
     // mesh

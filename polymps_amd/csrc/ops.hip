@@ -11,51 +11,158 @@ namespace mpsgpu {
 
 constexpr int kBlock = 256;
 
-template <bool LIST, class F>
-__device__ __forceinline__ void forNbrs(const KArgs& a, int i, double x, double y, double z, F&& f) {
-    if (LIST) {
-        listNbrs(a, i, f);
-    } else {
-        walkCells(a, x, y, z, [&](int j) {
-            if (j != i) f(j);
-        });
-    }
-}
-
 #define PX(k) a.D[DX][k]
 #define PY(k) a.D[DY][k]
 #define PZ(k) a.D[DZ][k]
 
 // --------------------------------------------------- neighbour list build
-// Candidates within the largest operator radius, j != i, walk order.
-__global__ void kBuildList(KArgs a, int* nbr, int* cnt, double r2) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.N) return;
-    if (!live(a, i)) { cnt[i] = 0; return; }
-    const double xi = PX(i), yi = PY(i), zi = PZ(i);
-    int c = 0;
-    walkCells(a, xi, yi, zi, [&](int j) {
-        if (j == i) return;
-        const double v0 = PX(j) - xi, v1 = PY(j) - yi, v2 = PZ(j) - zi;
-        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-        if (d2 < r2) {
-            if (c < a.K) nbr[(size_t)c * a.N + i] = j;
-            c++;
-        }
-    });
-    if (c > a.K) {
-        atomicMax(&a.is[I_OVERFLOW], c);
-        c = a.K;
+// Candidate bitmasks for the largest operator radius, j != i, and the packed
+// (x, y, z, type) copy that list consumers gather.  Word (r, w) of slot i is
+// at mask[(r * maskW + w) * N + i]; bit b marks candidate slot
+// rowStart(r) + 32 w + b of stencil row r as inside the radius.  Every lane
+// writes its words in row order, so the stores coalesce.  One-wave
+// workgroups (coopRows).
+constexpr int kWave = 64;
+
+__global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int* cnt, double r2) {
+    __shared__ double4 stage[kWave];
+    const int i = blockIdx.x * kWave + threadIdx.x;
+    const bool inRange = i < a.N;
+    const double xi = inRange ? PX(i) : 0.0, yi = inRange ? PY(i) : 0.0, zi = inRange ? PZ(i) : 0.0;
+    if (inRange) a.P4[i] = make_double4(xi, yi, zi, (double)a.I[ITYPE][i]);
+    const bool act = inRange && live(a, i);
+    if (inRange && !act) cnt[i] = 0;
+    const double* __restrict__ X = a.D[DX];
+    const double* __restrict__ Y = a.D[DY];
+    const double* __restrict__ Z = a.D[DZ];
+    const size_t N = a.N;
+    const int W = a.maskW;
+    unsigned* out = mask + (act ? i : 0);
+    int total = 0, over = 0;
+    unsigned cur = 0;
+    int w = 0, row = 0, s0 = 0;
+    coopRows(
+        a, act, xi, yi, zi, stage, 2 * 32 * W,
+        [&](int j) { return make_double4(X[j], Y[j], Z[j], 0.0); },
+        [&](int r, int& s, int& e) {
+            if (e - s > 32 * W) {
+                over = max(over, e - s);
+                e = s + 32 * W;
+            }
+            cur = 0;
+            w = 0;
+            row = r;
+            s0 = s;
+        },
+        [&](int j, const double4& q) {
+            const double v0 = q.x - xi, v1 = q.y - yi, v2 = q.z - zi;
+            const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            const int k = j - s0;
+            cur |= ((j != i && d2 < r2) ? 1u : 0u) << (k & 31);
+            if ((k & 31) == 31) {
+                out[(size_t)(row * W + w) * N] = cur;
+                total += __builtin_popcount(cur);
+                w++;
+                cur = 0;
+            }
+        },
+        [&](int r, int s, int e) {
+            if ((e - s) & 31) {
+                out[(size_t)(r * W + w) * N] = cur;
+                total += __builtin_popcount(cur);
+                w++;
+            }
+            for (; w < W; w++) out[(size_t)(r * W + w) * N] = 0u;
+        });
+    if (!act) return;
+    if (over > 0) atomicMax(&a.is[I_OVERFLOW], over);
+    if (total > a.K) {
+        atomicMax(&a.is[I_OVERFLOW], total);
+        total = a.K;
     }
-    atomicMax(&a.is[I_LIST_MAX], c);
-    cnt[i] = c;
+    atomicMax(&a.is[I_LIST_MAX], total);
+    cnt[i] = total;
+}
+
+// Start slot of stencil row r of a particle in bucket (ix, iy, iz), or -1
+// when the row lies outside the grid (the same rows and order as the walk).
+__device__ __forceinline__ int rowStart(const KArgs& a, int ix, int iy, int iz, int r) {
+    const mps_params& P = a.P;
+    const int jz = ((int)P.dim == 3) ? iz - 1 + r / 3 : 0;
+    const int jy = iy - 1 + r % 3;
+    if (jz < 0 || jz >= P.num_buckets_z || jy < 0 || jy >= P.num_buckets_y) return -1;
+    int jx0 = ix - 1;
+    if (jx0 < 0) jx0 = 0;
+    if (jx0 > ix + 1 || jx0 >= P.num_buckets_x) return -1;
+    return a.cellStart[jz * P.num_buckets_xy + jy * P.num_buckets_x + jx0];
+}
+
+// Decode the candidate bitmasks into the column-major index list.  Every
+// lane pulls its next neighbour in the same step and the k-th entries of all
+// lanes are stored by one instruction into one list column, so the stores
+// coalesce.  The WMAX words of a row are loaded together when the decoder
+// enters the row (WMAX >= maskW; unused words read as empty).
+template <int WMAX>
+__global__ void kMaskToList(KArgs a, const unsigned* mask, int* nbr) {
+    const int i = slotId();
+    if (i >= a.N || !live(a, i)) return;
+    const mps_params& P = a.P;
+    const int n = a.nbrCnt[i];
+    if (n == 0) return;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const int ix = (int)((xi - P.domain_min[0]) * P.inv_bucket_side + KEPS);
+    const int iy = (int)((yi - P.domain_min[1]) * P.inv_bucket_side + KEPS);
+    const int iz = (int)((zi - P.domain_min[2]) * P.inv_bucket_side + KEPS);
+    const int rows = ((int)P.dim == 3) ? 9 : 3;
+    const int W = a.maskW;
+    const size_t N = a.N;
+    const unsigned* __restrict__ M = mask + i;
+    unsigned wd[WMAX];
+    int r = -1, sr = 0, w = 0;   // current row, its start slot, index of wd[0] in the row
+    auto loadRow = [&]() {
+#pragma unroll
+        for (int k = 0; k < WMAX; k++) wd[k] = (k < W) ? M[(size_t)(r * W + k) * N] : 0u;
+        sr = rowStart(a, ix, iy, iz, r);
+        w = 0;
+    };
+    auto next = [&]() -> int {
+        for (;;) {
+            if (r >= 0 && wd[0] != 0u) break;
+            bool rowDone = true;
+            if (r >= 0) {
+#pragma unroll
+                for (int k = 0; k < WMAX - 1; k++) rowDone = rowDone && wd[k + 1] == 0u;
+            }
+            if (rowDone) {
+                if (++r == rows) return -1;   // not reached: n bits are set
+                loadRow();
+            } else {
+#pragma unroll
+                for (int k = 0; k < WMAX - 1; k++) wd[k] = wd[k + 1];
+                wd[WMAX - 1] = 0u;
+                w++;
+            }
+        }
+        const int b = __builtin_ctz(wd[0]);
+        wd[0] &= wd[0] - 1;
+        return sr + 32 * w + b;
+    };
+    constexpr int kChunk = 16;
+    for (int c0 = 0; c0 < n; c0 += kChunk) {
+        int e[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; k++) e[k] = (c0 + k < n) ? next() : 0;
+#pragma unroll
+        for (int k = 0; k < kChunk; k++)
+            if (c0 + k < n) nbr[(size_t)(c0 + k) * N + i] = e[k];
+    }
 }
 
 // ------------------------------------------------------------- viscosity
 // MpsViscosity::calcViscosity (src/MpsViscosity.cpp:20-102)
 template <bool LIST>
 __global__ void kViscosity(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
@@ -69,8 +176,7 @@ __global__ void kViscosity(KArgs a) {
     const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double ax = 0, ay = 0, az = 0;
-    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -88,9 +194,62 @@ __global__ void kViscosity(KArgs a) {
     a.D[DAZ][i] = P.coeff_visc_multiphase * az;
 }
 
+// The same from a cooperative bucket walk (one-wave workgroups): the
+// candidates' position, velocity and type are staged in LDS.
+struct ViscCand {
+    double x, y, z, u, v, w, wall;
+};
+__global__ __launch_bounds__(kWave) void kViscosityWalk(KArgs a) {
+    __shared__ ViscCand stage[kWave];
+    const int i = blockIdx.x * kWave + threadIdx.x;
+    const bool act = i < a.N && live(a, i);
+    const mps_params& P = a.P;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    const double meu = P.knm_vs1 * P.dns_fl1, rho = P.dns_fl1;
+    double nuF;
+    if ((meu + meu) > KEPS) nuF = 2.0 * meu * meu / (meu + meu);
+    else nuF = 0.0;
+    nuF = nuF / rho;
+    const double nuW = 2.0 * meu / rho;
+    const int k = act ? i : 0;
+    const double xi = PX(k), yi = PY(k), zi = PZ(k);
+    const double ux = a.D[DU][k], uy = a.D[DV][k], uz = a.D[DW][k];
+    const double mx = a.D[DMX][k], my = a.D[DMY][k], mz = a.D[DMZ][k];
+    const double* __restrict__ U = a.D[DU];
+    const double* __restrict__ V = a.D[DV];
+    const double* __restrict__ Wv = a.D[DW];
+    const int* __restrict__ T = a.I[ITYPE];
+    double ax = 0, ay = 0, az = 0;
+    coopRows(
+        a, act, xi, yi, zi, stage, 2 * 32 * a.maskW,
+        [&](int j) {
+            return ViscCand{PX(j), PY(j), PZ(j), U[j], V[j], Wv[j], (T[j] == P.wall) ? 1.0 : 0.0};
+        },
+        [&](int, int&, int&) {},
+        [&](int j, const ViscCand& q) {
+            if (j == i) return;
+            const double v0 = q.x - xi, v1 = q.y - yi, v2 = q.z - zi;
+            const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            const double w0 = q.x - mx, w1 = q.y - my, w2 = q.z - mz;
+            const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_l2 && (d2 < dm2 || pw)) {
+                const double wL = wgt(sqrt(d2), P.re_l, P.weight_type);
+                const double nu = (q.wall != 0.0) ? nuW : nuF;
+                ax += (q.u - ux) * wL * nu;
+                ay += (q.v - uy) * wL * nu;
+                az += (q.w - uz) * wL * nu;
+            }
+        },
+        [&](int, int, int) {});
+    if (!act) return;
+    a.D[DAX][i] = P.coeff_visc_multiphase * ax;
+    a.D[DAY][i] = P.coeff_visc_multiphase * ay;
+    a.D[DAZ][i] = P.coeff_visc_multiphase * az;
+}
+
 // MpsParticle::addGravity (src/MpsParticle.cpp:111-130)
 __global__ void kAddGravity(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const double ax = a.D[DAX][i] + a.P.gravity[0];
     const double ay = a.D[DAY][i] + a.P.gravity[1];
@@ -101,7 +260,7 @@ __global__ void kAddGravity(KArgs a) {
 
 // MpsParticleVelPos::predictionVelocityPosition (src/MpsParticleVelPos.cpp:20-50)
 __global__ void kPrediction(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     const double far = 10e8 * P.part_dist;
@@ -129,7 +288,7 @@ __global__ void kPrediction(KArgs a) {
 // velMax = sqrt(max |u|^2) over fluid particles, reduced with an integer max
 // on the (non-negative) bit pattern so the result does not depend on order.
 __global__ void kCorrection(KArgs a, unsigned long long* vmax2bits) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     double v2 = 0.0;
     if (live(a, i) && a.I[ITYPE][i] == a.P.fluid) {
         const mps_params& P = a.P;
@@ -159,7 +318,7 @@ __global__ void kVelMaxFinish(KArgs a, const unsigned long long* vmax2bits) {
 // ascending reference order (a slot receives the data of the last live
 // particle, the last particle becomes a ghost).
 __global__ void kOutFlag(KArgs a, int* outList, int cap) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     if (a.I[ITYPE][i] != P.fluid) return;
@@ -215,7 +374,7 @@ __global__ void kCompact(KArgs a, int* outList, int cap, const int* slotOf) {
 // MpsParticleCollision::checkParticleCollisions (src/MpsParticleCollision.cpp:22-119)
 template <bool LIST>
 __global__ void kCollisionPC(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     if (a.I[ITYPE][i] != P.fluid) return;
@@ -225,8 +384,7 @@ __global__ void kCollisionPC(KArgs a) {
     const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double dux = 0, duy = 0, duz = 0;
-    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -251,7 +409,7 @@ __global__ void kCollisionPC(KArgs a) {
 // checkDynamicParticleCollisions (src/MpsParticleCollision.cpp:124-292)
 template <bool LIST>
 __global__ void kCollisionDPC(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     if (a.I[ITYPE][i] != P.fluid) return;
@@ -264,8 +422,7 @@ __global__ void kCollisionDPC(KArgs a) {
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     const double pi = a.D[DPRESS][i];
     double dux = 0, duy = 0, duz = 0;
-    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -308,7 +465,7 @@ __global__ void kCollisionDPC(KArgs a) {
 }
 
 __global__ void kApplyDvel(KArgs a, int movePos) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     if (a.I[ITYPE][i] == a.P.fluid) {
         const double dx = a.D[DDVX][i], dy = a.D[DDVY][i], dz = a.D[DDVZ][i];
@@ -369,7 +526,7 @@ __device__ __forceinline__ void closestPtTri(const double* p, const double* A, c
 // PolygonMesh::closestPointPNDBoundaryAABB (src/PolygonMesh.cpp:585-715),
 // brute force over the faces; ties keep the lowest face index.
 __global__ void kClosestPoint(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const double p[3] = {PX(i), PY(i), PZ(i)};
     double best = 1.79769313486231570e308, q[3], bq[3] = {0, 0, 0};
@@ -406,7 +563,7 @@ __device__ __forceinline__ double interpZ(const double* xd, const double* yd, do
 
 // PolygonMesh::updateParticlesNearPolygonMesh (src/PolygonMesh.cpp:718-795)
 __global__ void kUpdateNear(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     double niw = 0.0;
@@ -425,7 +582,7 @@ __global__ void kUpdateNear(KArgs a) {
 // MpsPndNeigh::calcWallNPCD (src/MpsPndNeigh.cpp:105-195)
 template <bool LIST>
 __global__ void kWallNPCD(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     if (!(a.I[ITYPE][i] == P.fluid && a.I[INEARWALL][i])) return;
@@ -434,8 +591,7 @@ __global__ void kWallNPCD(KArgs a) {
     double n[3], R[9], ns;
     reflectSqrt(xi, xm, n, R, &ns);
     double nx = 0, ny = 0, nz = 0;
-    forNbrs<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
@@ -485,15 +641,14 @@ __device__ __forceinline__ void inverse3(double* M, int dim) {
 
 template <bool LIST>
 __global__ void kCorrectionMatrix(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -517,7 +672,7 @@ __global__ void kCorrectionMatrix(KArgs a) {
 // MpsPndNeigh::setInitialPndNumberOfNeighNPCD (src/MpsPndNeigh.cpp:21-101)
 template <bool LIST>
 __global__ void kInitialPnd(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
@@ -525,8 +680,7 @@ __global__ void kInitialPnd(KArgs a) {
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double pnd = a.D[DPNDI][i], wSum = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
     int nn = 0;
-    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -556,7 +710,7 @@ __global__ void kInitialPnd(KArgs a) {
 // MpsPndNeigh::calcPndnNeighNPCD (src/MpsPndNeigh.cpp:199-331)
 template <bool LIST>
 __global__ void kPndNeighNPCD(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
@@ -565,8 +719,7 @@ __global__ void kPndNeighNPCD(KArgs a) {
     double ni = 0.0, wSum = 0.0;
     double c0 = a.D[DNPX][i], c1 = a.D[DNPY][i], c2 = a.D[DNPZ][i];
     int nn = a.I[INNWALL][i];
-    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -597,7 +750,7 @@ __global__ void kPndNeighNPCD(KArgs a) {
 // calcPndDiffusiveTerm (src/MpsPndNeigh.cpp:334-489)
 template <bool LIST>
 __global__ void kPndDiffusive(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
@@ -613,8 +766,7 @@ __global__ void kPndDiffusive(KArgs a) {
     for (int q = 0; q < 9; q++) MC[q] = a.D[DCM0 + q][i];
     double Di = 0.0, DivV = 0.0;
     const double flagDi = 1.0;
-    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -651,7 +803,7 @@ __global__ void kPndDiffusive(KArgs a) {
 // calcWall{Slip,NoSlip}PndDiffusiveTerm (src/MpsPndNeigh.cpp:492-744)
 template <bool LIST>
 __global__ void kWallPndDiffusive(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     const double ni = a.D[DPNDI][i];
@@ -690,8 +842,7 @@ __global__ void kWallPndDiffusive(KArgs a) {
             DivV += (P.dim / P.pnd_small_zero) * (nj / ni) * (vx * m0 + vy * m1 + vz * m2) * wS / dm2;
         }
     };
-    forNbrs<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
@@ -707,7 +858,7 @@ __global__ void kWallPndDiffusive(KArgs a) {
 
 // updatePnd (src/MpsPndNeigh.cpp:747-774)
 __global__ void kUpdatePnd(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     a.D[DPNDI][i] = a.D[DAUX1][i];
     a.D[DAUX1][i] = 0.0;
@@ -716,7 +867,7 @@ __global__ void kUpdatePnd(KArgs a) {
 // meanPndParticlesWallDummySurface (src/MpsPndNeigh.cpp:777-852), first pass
 template <bool LIST>
 __global__ void kMeanPndWDS(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     a.D[DAUX1][i] = 0.0;
@@ -726,8 +877,7 @@ __global__ void kMeanPndWDS(KArgs a) {
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double up = 0, dn = 0;
-    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -742,7 +892,7 @@ __global__ void kMeanPndWDS(KArgs a) {
     a.D[DAUX2][i] = dn;
 }
 __global__ void kMeanPndWDSFinish(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     if (a.I[ITYPE][i] == P.wall || a.I[IBC][i] == P.surface) {
@@ -756,7 +906,7 @@ __global__ void kMeanPndWDSFinish(KArgs a) {
 // meanWallPnd (src/MpsPndNeigh.cpp:923-994)
 template <bool LIST>
 __global__ void kMeanWallPnd(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     a.D[DAUX1][i] = 0.0;
@@ -765,8 +915,7 @@ __global__ void kMeanWallPnd(KArgs a) {
     const double xi[3] = {PX(i), PY(i), PZ(i)};
     const double xm[3] = {a.D[DMX][i], a.D[DMY][i], a.D[DMZ][i]};
     double up = 0, dn = 0;
-    forNbrs<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
@@ -791,15 +940,14 @@ __global__ void kMeanWallPnd(KArgs a) {
 // meanPnd (src/MpsPndNeigh.cpp:855-920)
 template <bool LIST>
 __global__ void kMeanPnd(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double up = 0, dn = 0;
-    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -814,7 +962,7 @@ __global__ void kMeanPnd(KArgs a) {
     a.D[DAUX2][i] = a.D[DAUX2][i] + dn;
 }
 __global__ void kMeanPndFinish(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     if (a.I[INNEIGH][i] < 1) a.D[DPNDI][i] = a.D[DAUX1][i];
     else a.D[DPNDI][i] = a.D[DAUX1][i] / a.D[DAUX2][i];
@@ -825,15 +973,14 @@ __global__ void kMeanPndFinish(KArgs a) {
 // meanNeighFluidPnd (src/MpsPndNeigh.cpp:997-1050)
 template <bool LIST>
 __global__ void kMeanNeighFluidPnd(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double up = 0, dn = 0;
-    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -849,7 +996,7 @@ __global__ void kMeanNeighFluidPnd(KArgs a) {
 
 // MpsBoundaryCondition::updateParticleBC (src/MpsBoundaryCondition.cpp:66-400)
 __global__ void kUpdateBC(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     if (a.I[ITYPE][i] == P.dummy_wall) { a.I[IBC][i] = P.other; return; }
@@ -867,7 +1014,7 @@ __global__ void kUpdateBC(KArgs a) {
 // --------------------------------------------------------------- pressure
 // calcPressEMPS / calcPressWCMPS (src/MpsPressure.cpp:62-163)
 __global__ void kPressExplicit(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     const double mi = P.dns_fl1;
@@ -883,7 +1030,7 @@ __global__ void kPressExplicit(KArgs a) {
 // calcVelDivergence (src/MpsPressure.cpp:479-551)
 template <bool LIST>
 __global__ void kVelDivergence(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     a.D[DDIVU][i] = 0.0;
@@ -896,8 +1043,7 @@ __global__ void kVelDivergence(KArgs a) {
     double MC[9];
     for (int q = 0; q < 9; q++) MC[q] = a.D[DCM0 + q][i];
     double DivV = 0.0;
-    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -922,7 +1068,7 @@ __global__ void kVelDivergence(KArgs a) {
 // calcPressGradient (src/MpsPressure.cpp:1217-1338)
 template <bool LIST>
 __global__ void kPressGradient(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
@@ -934,8 +1080,7 @@ __global__ void kPressGradient(KArgs a) {
     double MC[9];
     for (int q = 0; q < 9; q++) MC[q] = a.D[DCM0 + q][i];
     if (gt == 0 || gt == 2)
-        forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-            const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
             const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
             const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
             const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -946,8 +1091,7 @@ __global__ void kPressGradient(KArgs a) {
             }
         });
     double ax = 0, ay = 0, az = 0;
-    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -980,7 +1124,7 @@ __global__ void kPressGradient(KArgs a) {
 // calcWallPressGradient + repulsive forces (src/MpsPressure.cpp:1341-1611)
 template <bool LIST>
 __global__ void kWallPressGradient(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     if (!(a.I[ITYPE][i] == P.fluid && a.I[INEARWALL][i])) return;
@@ -996,8 +1140,7 @@ __global__ void kWallPressGradient(KArgs a) {
     double n[3], R[9], ns;
     reflectSqrt(xi, xm, n, R, &ns);
     if (gt == 0 || gt == 2)
-        forNbrs<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j) {
-            const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        forNbrsPos<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
             const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
             const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
             const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
@@ -1018,8 +1161,7 @@ __global__ void kWallPressGradient(KArgs a) {
         }
         ax += w0 * wS; ay += w1 * wS; az += w2 * wS;
     };
-    forNbrs<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
@@ -1064,7 +1206,7 @@ __global__ void kWallPressGradient(KArgs a) {
 // calcWall{Slip,NoSlip}Viscosity (src/MpsViscosity.cpp:1527-1910)
 template <bool LIST>
 __global__ void kWallViscosity(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     if (!(a.I[ITYPE][i] == P.fluid && a.I[INEARWALL][i])) return;
@@ -1101,8 +1243,7 @@ __global__ void kWallViscosity(KArgs a) {
     else nuI = 0.0;
     nuI = nuI / dnsI;
     double ax = 0, ay = 0, az = 0;
-    forNbrs<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
@@ -1140,7 +1281,7 @@ __global__ void kWallViscosity(KArgs a) {
 // velocity is staged in the dvel slots, then copied.
 template <bool LIST>
 __global__ void kShift1(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     double nu = a.D[DU][i], nv = a.D[DV][i], nw = a.D[DW][i];
@@ -1150,8 +1291,7 @@ __global__ void kShift1(KArgs a) {
         const double ux = nu, uy = nv, uz = nw;
         const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
         double dx = 0, dy = 0, dz = 0;
-        forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-            const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
             const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
             const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
             const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -1173,7 +1313,7 @@ __global__ void kShift1(KArgs a) {
 }
 // copy staged values into (dst x,y,z) and clear the staging slots
 __global__ void kCommitStaged(KArgs a, int dst0) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     a.D[dst0][i] = a.D[DDVX][i];
     a.D[dst0 + 1][i] = a.D[DDVY][i];
@@ -1184,15 +1324,14 @@ __global__ void kCommitStaged(KArgs a, int dst0) {
 // calcConcAndConcGradient (src/MpsShifting.cpp:432-576), concentration pass
 template <bool LIST>
 __global__ void kShift2Conc(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double c = 0.0;
-    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-        const double xj = PX(j), yj = PY(j), zj = PZ(j);
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -1205,7 +1344,7 @@ __global__ void kShift2Conc(KArgs a) {
 // gradient pass: new positions staged in dvel slots
 template <bool LIST>
 __global__ void kShift2Grad(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     double nx = PX(i), ny = PY(i), nz = PZ(i);
@@ -1215,8 +1354,7 @@ __global__ void kShift2Grad(KArgs a) {
         const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
         const double ci = a.D[DCONC][i];
         double g0 = 0, g1 = 0, g2 = 0;
-        forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-            const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
             const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
             const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
             const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -1242,7 +1380,7 @@ __global__ void kShift2Grad(KArgs a) {
 // calcWallConcAndConcGradient (src/MpsShifting.cpp:578-692)
 template <bool LIST>
 __global__ void kShift2Wall(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     double nx = PX(i), ny = PY(i), nz = PZ(i);
@@ -1253,8 +1391,7 @@ __global__ void kShift2Wall(KArgs a) {
         double n[3], R[9], ns;
         reflectSqrt(xi, xm, n, R, &ns);
         double d0 = 0, d1 = 0, dd2 = 0;
-        forNbrs<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j) {
-            const double xj = PX(j), yj = PY(j), zj = PZ(j);
+        forNbrsPos<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
             const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
             const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
             const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
@@ -1293,14 +1430,14 @@ __global__ void kShift2Wall(KArgs a) {
 // updateVelocityParticlesWallDummy (src/MpsParticleVelPos.cpp:100-158)
 template <bool LIST>
 __global__ void kWallDummyVelocity(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     if (a.I[ITYPE][i] != P.dummy_wall) return;
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     double dx = 0, dy = 0, dz = 0, ni = 0;
-    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-        const double v0 = PX(j) - xi, v1 = PY(j) - yi, v2 = PZ(j) - zi;
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         if (d2 < P.re_l2 && a.I[ITYPE][j] == P.fluid) {
             const double wL = wgt(sqrt(d2), P.re_l, P.weight_type);
@@ -1337,7 +1474,18 @@ int launchBuildList(Ctx* ctx) {
     double r2 = std::max(P.re_l2, P.re_s2);
     r2 = std::max(r2, P.dist_collision_limit2);
     r2 = std::max(r2, P.part_dist * P.part_dist);
-    hipLaunchKernelGGL(kBuildList, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbr, ctx->nbrCnt, r2);
+    hipLaunchKernelGGL(kBuildList, dim3(gridFor(ctx->N, kWave)), dim3(kWave), 0, ctx->stream, a, ctx->nbrMask,
+                       ctx->nbrCnt, r2);
+    if (ctx->maskW <= 4)
+        hipLaunchKernelGGL(kMaskToList<4>, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr);
+    else if (ctx->maskW <= 8)
+        hipLaunchKernelGGL(kMaskToList<8>, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr);
+    else if (ctx->maskW <= 16)
+        hipLaunchKernelGGL(kMaskToList<16>, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr);
+    else {
+        ctx->err = "neighbour bitmask rows wider than 512 candidates are not supported";
+        return MPS_E_UNSUPPORTED;
+    }
     MPS_HIP(hipGetLastError());
     ctx->listValid = true;
     return MPS_OK;
@@ -1350,6 +1498,7 @@ int launchCompaction(Ctx* ctx) {
     hipLaunchKernelGGL(kOutFlag, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->outList, cap);
     hipLaunchKernelGGL(kCompact, dim3(1), dim3(1), 0, ctx->stream, a, ctx->outList, cap, ctx->slotOf);
     MPS_HIP(hipGetLastError());
+    ctx->listValid = false;   // a compaction may move particles between slots
     return MPS_OK;
 }
 
@@ -1357,7 +1506,7 @@ int launchCompaction(Ctx* ctx) {
 // is safe: only wall / dummy slots are written and only fluid slots are read.
 template <bool LIST>
 __global__ void kExtrapolateWallDummy2(KArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
     const int t = a.I[ITYPE][i];
@@ -1365,8 +1514,8 @@ __global__ void kExtrapolateWallDummy2(KArgs a) {
     const double rho = P.dns_fl1;
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     double ni = 0.0, pr = 0.0;
-    forNbrs<LIST>(a, i, xi, yi, zi, [&](int j) {
-        const double v0 = PX(j) - xi, v1 = PY(j) - yi, v2 = PZ(j) - zi;
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         if (d2 < P.re_s2 && a.I[ITYPE][j] == P.fluid) {
             const double wS = wgt(sqrt(d2), P.re_s, P.weight_type);
@@ -1383,7 +1532,10 @@ int launchOp(Ctx* ctx, const char* op) {
     const int g = gridFor(ctx->N, kBlock);
     const std::string o(op);
     const mps_params& P = ctx->P;
-    if (o == "viscosity") LT(kViscosity);
+    if (o == "viscosity") {
+        if (ctx->listValid) L1(kViscosity<true>);
+        else hipLaunchKernelGGL(kViscosityWalk, dim3(gridFor(ctx->N, kWave)), dim3(kWave), 0, ctx->stream, a);
+    }
     else if (o == "add_gravity") L1(kAddGravity);
     else if (o == "prediction") L1(kPrediction);
     else if (o == "collision") {

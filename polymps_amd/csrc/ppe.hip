@@ -25,7 +25,7 @@ constexpr int kB = 256;
 template <bool LIST, bool TRANS>
 __global__ void kPpeAssemble(KArgs a, double* val, int* col, int* cnt, double* diag, double* b, float* val32,
                              float* invd32) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (i >= a.N) return;
     if (!live(a, i)) {
         cnt[i] = 0; diag[i] = 1.0; b[i] = 0.0;
@@ -43,10 +43,15 @@ __global__ void kPpeAssemble(KArgs a, double* val, int* col, int* cnt, double* d
     const double xi = a.D[DX][i], yi = a.D[DY][i], zi = a.D[DZ][i];
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     const double rho = P.dns_fl1;
+    // A neighbour j whose wall distance is at least 2 reL has its mirror point
+    // at least 3 reL away from any particle of its reL ball, so the mirror test
+    // of row j passes for i without reading j's mirror point.
+    const double farWall2 = 4.0 * P.re_l2;
+    const int* __restrict__ BC = a.I[IBC];
+    const double* __restrict__ RIW2 = a.D[DRIW2];
     double sum = 0.0, rhs = 0.0;
     int c = 0;
-    auto body = [&](int j) {
-        const double xj = a.D[DX][j], yj = a.D[DY][j], zj = a.D[DZ][j];
+    forNbrsPosType<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj, int tj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         if (!(d2 < P.re_l2)) return;
@@ -55,34 +60,43 @@ __global__ void kPpeAssemble(KArgs a, double* val, int* col, int* cnt, double* d
         const double wL = wgt(sqrt(d2), P.re_l, P.weight_type);
         const double mij = wL * P.coeff_ppe;
         const bool okRow = d2 < dm2 || pw;   // j is a neighbour of row i
-        if (okRow) {
-            if (a.I[ITYPE][j] == P.dummy_wall) {
-                const double pgh = rho * (v0 * P.gravity[0] + v1 * P.gravity[1] + v2 * P.gravity[2]) * wL;
-                rhs -= P.coeff_ppe * pgh;
-            } else {
-                sum -= mij;
-                if (!TRANS && a.I[IBC][j] == P.inner) {
-                    val[(size_t)c * a.N + i] = mij;
-                    if (val32) val32[(size_t)c * a.N + i] = (float)mij;
-                    col[(size_t)c * a.N + i] = j;
-                    c++;
-                }
-            }
+        const bool dummy = tj == P.dummy_wall;
+        // both accumulators are updated every time (x - 0.0 == x) so that no
+        // accumulator is selected through memory
+        double subSum = 0.0, subRhs = 0.0;
+        if (okRow && dummy) {
+            const double pgh = rho * (v0 * P.gravity[0] + v1 * P.gravity[1] + v2 * P.gravity[2]) * wL;
+            subRhs = P.coeff_ppe * pgh;
         }
-        if (TRANS && a.I[IBC][j] == P.inner) {
+        if (okRow && !dummy) subSum = mij;
+        sum -= subSum;
+        rhs -= subRhs;
+        bool put = false;
+        if (!TRANS) {
+            put = okRow && !dummy && BC[j] == P.inner;
+        } else if (BC[j] == P.inner) {
             // entry (j, i) of A: i seen from row j, mirror test with j's mirror
-            const double u0 = xi - a.D[DMX][j], u1 = yi - a.D[DMY][j], u2 = zi - a.D[DMZ][j];
-            const double dmj = u0 * u0 + u1 * u1 + u2 * u2;
-            if (d2 < dmj || pw) {
-                val[(size_t)c * a.N + i] = mij;
-                if (val32) val32[(size_t)c * a.N + i] = (float)mij;
-                col[(size_t)c * a.N + i] = j;
-                c++;
+            put = pw || RIW2[j] >= farWall2;
+            if (!put) {
+                const double u0 = xi - a.D[DMX][j], u1 = yi - a.D[DMY][j], u2 = zi - a.D[DMZ][j];
+                const double dmj = u0 * u0 + u1 * u1 + u2 * u2;
+                put = d2 < dmj;
             }
         }
-    };
-    if (LIST) listNbrs(a, i, body);
-    else walkCells(a, xi, yi, zi, [&](int j) { if (j != i) body(j); });
+        if (put) {
+            if (c < a.K) {
+                const size_t o = (size_t)c * a.N + i;
+                val[o] = mij;
+                if (val32) val32[o] = (float)mij;
+                col[o] = j;
+            }
+            c++;
+        }
+    });
+    if (c > a.K) {   // reported as a capacity error after the step
+        atomicMax(&a.is[I_OVERFLOW], c);
+        c = a.K;
+    }
     sum -= P.alpha_compressibility * rho / (P.time_step * P.time_step);
     if (P.mps_type == MPS_IMPLICIT_PND_DIVU)
         rhs += -P.coeff_ppe_source * rho * (a.D[DPNDKI][i] - P.pnd_small_zero) +
@@ -182,7 +196,7 @@ __device__ __forceinline__ double spmv64(const KArgs& a, const CgPtrs& c, const 
 // ---------------------------------------------------------- FP64 PCG
 // r = b - M x; partials |b|^2, |r|^2; invd = 1/diag (1 where diag == 0)
 __global__ void kCgStart(KArgs a, CgPtrs c) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     double s0 = 0, s1 = 0;
     if (i < a.N) {
         if (!live(a, i)) c.x[i] = 0.0;   // slots beyond the live count do not take part
@@ -214,7 +228,7 @@ __global__ void kCgStartFinish(KArgs a, int nb, double tol) {
 }
 
 __global__ void kCgZeroX(KArgs a, double* x) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (a.is[I_CG_DONE] != 2 || i >= a.N) return;
     x[i] = 0.0;
 }
@@ -222,7 +236,7 @@ __global__ void kCgZeroX(KArgs a, double* x) {
 // p = invd r; partial r.p
 __global__ void kCgInitP(KArgs a, CgPtrs c) {
     if (a.is[I_CG_DONE]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     double s0 = 0;
     if (i < a.N) {
         const double p = c.invd[i] * c.r[i];
@@ -241,7 +255,7 @@ __global__ void kCgInitPFinish(KArgs a, int nb) {
 // q = M p; partial p.q
 __global__ __launch_bounds__(256) void kCgSpmv(KArgs a, CgPtrs c) {
     if (a.is[I_CG_DONE]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     double s0 = 0;
     if (i < a.N) {
         const double q = spmv64(a, c, c.p, i);
@@ -262,7 +276,7 @@ __global__ void kCgAlpha(KArgs a, int nb) {
 // x += alpha p; r -= alpha q; z = invd r; partials |r|^2, r.z
 __global__ void kCgUpdate(KArgs a, CgPtrs c) {
     if (a.is[I_CG_DONE]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     const double alpha = a.ds[S_ALPHA];
     double s0 = 0, s1 = 0;
     if (i < a.N) {
@@ -297,13 +311,13 @@ __global__ void kCgBeta(KArgs a, int nb, int maxIt) {
 // p = z + beta p
 __global__ void kCgP(KArgs a, CgPtrs c) {
     if (a.is[I_CG_DONE]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (i < a.N) c.p[i] = c.z[i] + a.ds[S_BETA] * c.p[i];
 }
 
 // true residual r = b - M x in FP64 (kept in c.r); partials |r|^2, |b|^2
 __global__ __launch_bounds__(256) void kTrueResidual(KArgs a, CgPtrs c) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     double s0 = 0, s1 = 0;
     if (i < a.N) {
         if (!live(a, i)) c.x[i] = 0.0;
@@ -327,7 +341,7 @@ __global__ void kTrueResidualFinish(KArgs a, int nb) {
 // Solves M d = r with FP32 values, SpMV and Jacobi preconditioner; every dot
 // product accumulates in FP64.  Convergence: |r_inner|^2 < tol_in^2 |r_0|^2.
 __global__ void kMxInit(KArgs a, CgPtrs c) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     double s0 = 0, s1 = 0;
     if (i < a.N) {
         const float r = (float)c.r[i];
@@ -353,7 +367,7 @@ __global__ void kMxInitFinish(KArgs a, int nb, double tolIn) {
 }
 __global__ __launch_bounds__(256) void kMxSpmv(KArgs a, CgPtrs c) {
     if (a.is[I_CG_DONE]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     double s0 = 0;
     if (i < a.N) {
         const float p = c.fp[i];
@@ -365,7 +379,7 @@ __global__ __launch_bounds__(256) void kMxSpmv(KArgs a, CgPtrs c) {
 }
 __global__ void kMxUpdate(KArgs a, CgPtrs c) {
     if (a.is[I_CG_DONE]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     const float alpha = (float)a.ds[S_ALPHA];
     double s0 = 0, s1 = 0;
     if (i < a.N) {
@@ -381,22 +395,22 @@ __global__ void kMxUpdate(KArgs a, CgPtrs c) {
 }
 __global__ void kMxP(KArgs a, CgPtrs c) {
     if (a.is[I_CG_DONE]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (i < a.N) c.fp[i] = c.fz[i] + (float)a.ds[S_BETA] * c.fp[i];
 }
 __global__ void kZero(KArgs a, double* x) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (i < a.N) x[i] = 0.0;
 }
 // x += d (FP64 accumulation of the correction)
 __global__ void kMxApply(KArgs a, CgPtrs c) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (i < a.N && live(a, i)) c.x[i] += (double)c.fd[i];
 }
 
 // press = max(x, 0) for live slots (setZeroOnNegativePressure :471-476)
 __global__ void kPressFromX(KArgs a, const double* x) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = slotId();
     if (!live(a, i)) return;
     const double v = x[i];
     a.D[DPRESS][i] = (v < 0) ? 0.0 : v;
