@@ -72,6 +72,8 @@ def main(argv=None):
     ap.add_argument("--l0", type=float, default=0.0025)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--inner-tol", type=float, default=None,
+                    help="relative tolerance of the inner FP32 solve of the mixed PPE")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration)")
@@ -88,6 +90,8 @@ def main(argv=None):
 
     w = configs.dam3d_inc(l0=args.l0, steps=args.steps)
     w.params.precision_mode = 1 if args.precision == "mixed" else 0
+    if args.inner_tol is not None:
+        w.params.mixed_inner_tol = args.inner_tol
     sim = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh, device=0)
     t_init = time.perf_counter()
     sim.call("init")
@@ -123,6 +127,7 @@ def main(argv=None):
                    "particles": n, "precision": args.precision, "solver_tol": w.params.solver_tol,
                    "dt": w.params.time_step},
         "ppe": {"last_solve_ms": sim.scalar("ppe_ms"), "last_iterations": sim.scalar("solver_iter"),
+                "last_outer_iterations": sim.scalar("outer_iterations"),
                 "true_residual": sim.scalar("true_residual"), "spmv_launches": launches},
         "roofline": {"kernel": "kCgSpmv" if args.precision == "fp64" else "kMxSpmv", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,

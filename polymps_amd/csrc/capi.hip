@@ -344,7 +344,8 @@ int mps_create(mps_ctx** out, const mps_params* p, int n, int device) {
     MPS_HIP(hipMemsetAsync(ctx->nbrCnt, 0, ni, ctx->stream));
     MPS_HIP(hipMalloc(&ctx->P4, sizeof(double4) * (size_t)n));
     MPS_HIP(hipMalloc(&ctx->aVal, sizeof(double) * (size_t)ctx->K * n));
-    MPS_HIP(hipMalloc(&ctx->aCol, sizeof(int) * (size_t)ctx->K * n));
+    MPS_HIP(hipMalloc(&ctx->aCol, sizeof(unsigned short) * (size_t)ctx->K * n));
+    MPS_HIP(hipMalloc(&ctx->aEsc, sizeof(int) * (size_t)kPpeEsc * n));
     MPS_HIP(hipMalloc(&ctx->aCnt, ni));
     MPS_HIP(hipMalloc(&ctx->aDiag, nd));
     MPS_HIP(hipMalloc(&ctx->cgR, nd));
@@ -384,7 +385,8 @@ int mps_destroy(mps_ctx* ctx) {
     for (int f = 0; f < NPI; f++) hipFree(ctx->Ialt[f]);
     void* bufs[] = {ctx->cellStart, ctx->cellCount, ctx->key, ctx->rank, ctx->order, ctx->order2, ctx->slotOf,
                     ctx->scanTmp, ctx->outList, ctx->nbrMask, ctx->nbr, ctx->nbrCnt, ctx->P4, ctx->mV, ctx->mF, ctx->mN, ctx->wtab,
-                    ctx->aVal, ctx->aCol, ctx->aCnt, ctx->aDiag, ctx->aVal32, ctx->aDiagInv32, ctx->cgR, ctx->cgP,
+                    ctx->aVal, ctx->aCol, ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->aVal32, ctx->aDiagInv32,
+                    ctx->aDiag32, ctx->cgR, ctx->cgP,
                     ctx->cgZ, ctx->cgQ, ctx->cgInvD, ctx->cgX, ctx->fR, ctx->fP, ctx->fZ, ctx->fQ, ctx->fX,
                     ctx->ds, ctx->is, ctx->red};
     for (void* b : bufs)
@@ -533,6 +535,7 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "list_capacity") *out = 32 * ctx->maskW;
     else if (s == "capacity") *out = ctx->N;
     else if (s == "nnz") *out = ctx->hostInts[I_NNZ];
+    else if (s == "nnz_escaped") *out = ctx->hostInts[I_NESC];
     else if (s == "spmv_ms_total") *out = ctx->spmvMsTotal;
     else if (s == "spmv_launches") *out = (double)ctx->spmvLaunches;
     else if (s == "spmv_bytes_total") *out = ctx->spmvBytesTotal;
@@ -683,11 +686,21 @@ int mps_ppe_matrix(mps_ctx* ctx, int* row_ptr, int* cols, double* vals, int* nnz
     CHECK_CTX(ctx);
     hipSetDevice(ctx->dev);
     const int n = ctx->N, K = ctx->K;
-    std::vector<int> cnt(n), canon(n), col((size_t)K * n);
+    std::vector<int> cnt(n), canon(n), esc((size_t)kPpeEsc * n), col((size_t)K * n);
+    std::vector<unsigned short> code((size_t)K * n);
     std::vector<double> val((size_t)K * n), diag(n);
     MPS_HIP(hipMemcpy(cnt.data(), ctx->aCnt, sizeof(int) * n, hipMemcpyDeviceToHost));
     MPS_HIP(hipMemcpy(canon.data(), ctx->I[ICANON], sizeof(int) * n, hipMemcpyDeviceToHost));
-    MPS_HIP(hipMemcpy(col.data(), ctx->aCol, sizeof(int) * (size_t)K * n, hipMemcpyDeviceToHost));
+    MPS_HIP(hipMemcpy(code.data(), ctx->aCol, sizeof(unsigned short) * (size_t)K * n, hipMemcpyDeviceToHost));
+    MPS_HIP(hipMemcpy(esc.data(), ctx->aEsc, sizeof(int) * (size_t)kPpeEsc * n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; k++) {   // decode the column codes
+        int j = 0, e = 0;
+        for (int c = 0; c < cnt[k]; c++) {
+            const unsigned d = code[(size_t)c * n + k];
+            j = d ? j + (int)d : esc[(size_t)(e++) * n + k];
+            col[(size_t)c * n + k] = j;
+        }
+    }
     MPS_HIP(hipMemcpy(val.data(), ctx->aVal, sizeof(double) * (size_t)K * n, hipMemcpyDeviceToHost));
     MPS_HIP(hipMemcpy(diag.data(), ctx->aDiag, sizeof(double) * n, hipMemcpyDeviceToHost));
     const int nl = ctx->nLiveHost;

@@ -40,9 +40,12 @@ enum IField { ITYPE, ICANON, IBC, INNEIGH, NPI, INNWALL = NPI, INEARWALL, INEARM
 enum DScalar { S_VELMAX, S_ABSNEW, S_ABSOLD, S_RHSN2, S_THR, S_RN2, S_ALPHA, S_BETA, S_PAP, S_BN2, S_TRUE_RN2,
                S_RELTOL, NDS };
 // device scalar slots (int)
-enum IScalar { I_NLIVE, I_OUTCNT, I_OVERFLOW, I_CG_ITER, I_CG_DONE, I_COMPACT_OVF, I_LIST_MAX, I_NNZ, NIS };
+enum IScalar { I_NLIVE, I_OUTCNT, I_OVERFLOW, I_CG_ITER, I_CG_DONE, I_COMPACT_OVF, I_LIST_MAX, I_NNZ, I_NESC, NIS };
 
 constexpr int kWallTable = 50;
+// escaped (non-delta) columns per PPE row: the first entry and stencil-row
+// jumps of 65536 slots or more, at most 1 + 8
+constexpr int kPpeEsc = 12;
 constexpr int kRedBlocks = 1024;   // partial-sum slots of a reduction
 
 // Everything a kernel may touch, passed by value.
@@ -110,11 +113,13 @@ struct Ctx {
     bool hasMesh = false;
     // PPE
     double* aVal = nullptr;     // [K][N]
-    int* aCol = nullptr;        // [K][N]
+    unsigned short* aCol = nullptr;   // [K][N] column codes: slot delta from the previous entry, 0 = escape
+    int* aEsc = nullptr;              // [kPpeEsc][N] absolute slots of the escaped entries
     int* aCnt = nullptr;        // N
     double* aDiag = nullptr;    // N
     float* aVal32 = nullptr;    // [K][N] (mixed precision)
     float* aDiagInv32 = nullptr;
+    float* aDiag32 = nullptr;
     double* cgR = nullptr;
     double* cgP = nullptr;
     double* cgZ = nullptr;

@@ -23,20 +23,20 @@ namespace mpsgpu {
 constexpr int kB = 256;
 
 template <bool LIST, bool TRANS>
-__global__ void kPpeAssemble(KArgs a, double* val, int* col, int* cnt, double* diag, double* b, float* val32,
-                             float* invd32) {
+__global__ void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc, int* cnt, double* diag, double* b,
+                             float* val32, float* diag32, float* invd32) {
     const int i = slotId();
     if (i >= a.N) return;
     if (!live(a, i)) {
         cnt[i] = 0; diag[i] = 1.0; b[i] = 0.0;
-        if (invd32) invd32[i] = 1.0f;
+        if (invd32) { invd32[i] = 1.0f; diag32[i] = 1.0f; }
         return;
     }
     const mps_params& P = a.P;
     const int bci = a.I[IBC][i];
     if (bci == P.other || bci == P.surface) {
         cnt[i] = 0; diag[i] = 1.0; b[i] = 0.0;
-        if (invd32) invd32[i] = 1.0f;
+        if (invd32) { invd32[i] = 1.0f; diag32[i] = 1.0f; }
         return;
     }
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
@@ -50,7 +50,8 @@ __global__ void kPpeAssemble(KArgs a, double* val, int* col, int* cnt, double* d
     const int* __restrict__ BC = a.I[IBC];
     const double* __restrict__ RIW2 = a.D[DRIW2];
     double sum = 0.0, rhs = 0.0;
-    int c = 0;
+    int c = 0, prev = -1, ne = 0;
+    bool full = false;
     forNbrsPosType<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj, int tj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
@@ -84,19 +85,32 @@ __global__ void kPpeAssemble(KArgs a, double* val, int* col, int* cnt, double* d
             }
         }
         if (put) {
-            if (c < a.K) {
+            // column code: delta from the previous entry (walk order is
+            // ascending slot order), 0 = the slot is in the escape table.  A
+            // row that does not fit keeps its first entries (consistent for
+            // the solver) and is reported as a capacity error.
+            const int dj = j - prev;
+            const bool delta = prev >= 0 && dj > 0 && dj < 65536;
+            if (c < a.K && (delta || ne < kPpeEsc)) {
                 const size_t o = (size_t)c * a.N + i;
                 val[o] = mij;
                 if (val32) val32[o] = (float)mij;
-                col[o] = j;
+                if (delta) {
+                    col[o] = (unsigned short)dj;
+                } else {
+                    col[o] = 0;
+                    esc[(size_t)ne * a.N + i] = j;
+                    ne++;
+                }
+                prev = j;
+                c++;
+            } else {
+                full = true;
             }
-            c++;
         }
     });
-    if (c > a.K) {   // reported as a capacity error after the step
-        atomicMax(&a.is[I_OVERFLOW], c);
-        c = a.K;
-    }
+    if (full) atomicMax(&a.is[I_OVERFLOW], c + 1);   // reported as a capacity error after the step
+    if (ne) atomicAdd(&a.is[I_NESC], ne);
     sum -= P.alpha_compressibility * rho / (P.time_step * P.time_step);
     if (P.mps_type == MPS_IMPLICIT_PND_DIVU)
         rhs += -P.coeff_ppe_source * rho * (a.D[DPNDKI][i] - P.pnd_small_zero) +
@@ -106,7 +120,10 @@ __global__ void kPpeAssemble(KArgs a, double* val, int* col, int* cnt, double* d
     cnt[i] = c;
     diag[i] = sum;
     b[i] = rhs;
-    if (invd32) invd32[i] = (sum != 0.0) ? (float)(1.0 / sum) : 1.0f;
+    if (invd32) {
+        invd32[i] = (sum != 0.0) ? (float)(1.0 / sum) : 1.0f;
+        diag32[i] = (float)sum;
+    }
     atomicAdd(&a.is[I_NNZ], c);
 }
 
@@ -144,32 +161,50 @@ __device__ __forceinline__ void finishPartials(const double* red, int nb, double
     for (int q = 0; q < (int)(blockDim.x >> 6); q++) { s0 += sh[0][q]; s1 += sh[1][q]; }
 }
 
-// y_i = diag_i x_i + sum_c val[c][i] x[col[c][i]]; ELL columns are read four
-// at a time so that four gathers are in flight per thread.
+// y_i = diag_i x_i + sum_c val[c][i] x[j_c]; the column of entry c is the
+// previous one plus code[c][i], or the next slot of the escape table when the
+// code is 0.  Four entries are read per step so that four gathers are in
+// flight per thread.
 template <typename T>
-__device__ __forceinline__ double ellRow(const T* __restrict__ val, const int* __restrict__ col, int n, size_t N,
-                                         const T* __restrict__ x, int i, double acc) {
+__device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsigned short* __restrict__ code,
+                                         const int* __restrict__ esc, int n, size_t N, const T* __restrict__ x, int i,
+                                         double acc) {
+    const unsigned short* cd = code + i;
+    const T* vl = val + i;
+    const int* es = esc + i;
+    int j = 0, e = 0;
+    auto next = [&](unsigned d) {
+        if (d) j += (int)d;
+        else j = es[(size_t)(e++) * N];
+        return j;
+    };
     int c = 0;
     for (; c + 4 <= n; c += 4) {
-        const size_t o = (size_t)c * N + i;
-        const T v0 = val[o], v1 = val[o + N], v2 = val[o + 2 * N], v3 = val[o + 3 * N];
-        const int c0 = col[o], c1 = col[o + N], c2 = col[o + 2 * N], c3 = col[o + 3 * N];
-        const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        const size_t o = (size_t)c * N;
+        const T v0 = vl[o], v1 = vl[o + N], v2 = vl[o + 2 * N], v3 = vl[o + 3 * N];
+        const unsigned d0 = cd[o], d1 = cd[o + N], d2 = cd[o + 2 * N], d3 = cd[o + 3 * N];
+        const int j0 = next(d0);
+        const int j1 = next(d1);
+        const int j2 = next(d2);
+        const int j3 = next(d3);
+        const T x0 = x[j0], x1 = x[j1], x2 = x[j2], x3 = x[j3];
         acc += (double)(v0 * x0);
         acc += (double)(v1 * x1);
         acc += (double)(v2 * x2);
         acc += (double)(v3 * x3);
     }
     for (; c < n; c++) {
-        const size_t o = (size_t)c * N + i;
-        acc += (double)(val[o] * x[col[o]]);
+        const size_t o = (size_t)c * N;
+        const int jj = next(cd[o]);
+        acc += (double)(vl[o] * x[jj]);
     }
     return acc;
 }
 
 struct CgPtrs {
     const double* val;
-    const int* col;
+    const unsigned short* col;
+    const int* esc;
     const int* cnt;
     const double* diag;
     const double* b;
@@ -181,6 +216,7 @@ struct CgPtrs {
     double* invd;
     // mixed precision
     const float* val32;
+    const float* diag32;
     const float* invd32;
     float* fr;
     float* fp;
@@ -190,7 +226,7 @@ struct CgPtrs {
 };
 
 __device__ __forceinline__ double spmv64(const KArgs& a, const CgPtrs& c, const double* x, int i) {
-    return ellRow<double>(c.val, c.col, c.cnt[i], a.N, x, i, c.diag[i] * x[i]);
+    return ellRow<double>(c.val, c.col, c.esc, c.cnt[i], a.N, x, i, c.diag[i] * x[i]);
 }
 
 // ---------------------------------------------------------- FP64 PCG
@@ -355,13 +391,16 @@ __global__ void kMxInit(KArgs a, CgPtrs c) {
     }
     writePartials(a, s0, s1);
 }
-__global__ void kMxInitFinish(KArgs a, int nb, double tolIn) {
+// The inner solve stops at tolIn relative to its start or as soon as its
+// residual is below half the outer target (tol |b|), whichever comes first.
+__global__ void kMxInitFinish(KArgs a, int nb, double tolIn, double tol) {
     double rz, rn2;
     finishPartials(a.red, nb, rz, rn2);
     if (threadIdx.x != 0) return;
     a.ds[S_ABSNEW] = rz;
     a.ds[S_RN2] = rn2;
-    a.ds[S_THR] = fmax(tolIn * tolIn * rn2, DBL_MIN);
+    const double target = 0.25 * tol * tol * a.ds[S_BN2];
+    a.ds[S_THR] = fmax(fmax(tolIn * tolIn * rn2, target), DBL_MIN);
     a.is[I_CG_ITER] = 0;
     a.is[I_CG_DONE] = (rn2 == 0.0) ? 2 : 0;
 }
@@ -371,7 +410,7 @@ __global__ __launch_bounds__(256) void kMxSpmv(KArgs a, CgPtrs c) {
     double s0 = 0;
     if (i < a.N) {
         const float p = c.fp[i];
-        const float q = (float)ellRow<float>(c.val32, c.col, c.cnt[i], a.N, c.fp, i, (double)((float)c.diag[i] * p));
+        const float q = (float)ellRow<float>(c.val32, c.col, c.esc, c.cnt[i], a.N, c.fp, i, (double)(c.diag32[i] * p));
         c.fq[i] = q;
         s0 = (double)p * (double)q;
     }
@@ -418,9 +457,9 @@ __global__ void kPressFromX(KArgs a, const double* x) {
 
 static CgPtrs cgPtrs(Ctx* ctx) {
     CgPtrs c;
-    c.val = ctx->aVal; c.col = ctx->aCol; c.cnt = ctx->aCnt; c.diag = ctx->aDiag; c.b = ctx->D[DPPEB];
+    c.val = ctx->aVal; c.col = ctx->aCol; c.esc = ctx->aEsc; c.cnt = ctx->aCnt; c.diag = ctx->aDiag; c.b = ctx->D[DPPEB];
     c.x = ctx->D[DPPEX]; c.r = ctx->cgR; c.p = ctx->cgP; c.z = ctx->cgZ; c.q = ctx->cgQ; c.invd = ctx->cgInvD;
-    c.val32 = ctx->aVal32; c.invd32 = ctx->aDiagInv32;
+    c.val32 = ctx->aVal32; c.diag32 = ctx->aDiag32; c.invd32 = ctx->aDiagInv32;
     c.fr = ctx->fR; c.fp = ctx->fP; c.fz = ctx->fZ; c.fq = ctx->fQ; c.fd = ctx->fX;
     return c;
 }
@@ -430,6 +469,7 @@ static int ensureMixed(Ctx* ctx) {
     const size_t n = ctx->N;
     MPS_HIP(hipMalloc(&ctx->aVal32, sizeof(float) * (size_t)ctx->K * n));
     MPS_HIP(hipMalloc(&ctx->aDiagInv32, sizeof(float) * n));
+    MPS_HIP(hipMalloc(&ctx->aDiag32, sizeof(float) * n));
     MPS_HIP(hipMalloc(&ctx->fR, sizeof(float) * n));
     MPS_HIP(hipMalloc(&ctx->fP, sizeof(float) * n));
     MPS_HIP(hipMalloc(&ctx->fZ, sizeof(float) * n));
@@ -448,9 +488,12 @@ int ppeAssemble(Ctx* ctx) {
     const int g = gridFor(ctx->N, kB);
     const bool T = ctx->P.ppe_transpose != 0;
     float* v32 = mixed ? ctx->aVal32 : nullptr;
+    float* d32 = mixed ? ctx->aDiag32 : nullptr;
     float* i32 = mixed ? ctx->aDiagInv32 : nullptr;
-    MPS_HIP(hipMemsetAsync(ctx->is + I_NNZ, 0, sizeof(int), ctx->stream));
-#define ASM(L, TT) hipLaunchKernelGGL((kPpeAssemble<L, TT>), dim3(g), dim3(kB), 0, ctx->stream, a, ctx->aVal, ctx->aCol, ctx->aCnt, ctx->aDiag, ctx->D[DPPEB], v32, i32)
+    MPS_HIP(hipMemsetAsync(ctx->is + I_NNZ, 0, 2 * sizeof(int), ctx->stream));   // I_NNZ, I_NESC
+#define ASM(L, TT)                                                                                         \
+    hipLaunchKernelGGL((kPpeAssemble<L, TT>), dim3(g), dim3(kB), 0, ctx->stream, a, ctx->aVal, ctx->aCol,   \
+                       ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->D[DPPEB], v32, d32, i32)
     if (ctx->listValid) {
         if (T) ASM(true, true); else ASM(true, false);
     } else {
@@ -523,7 +566,7 @@ static int solveFp64(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int batc
     }
     const int D = ctx->hostInts[I_CG_DONE];
     addSpmvStats(ctx, times, (D == 1 || D == 3) ? iters + 1 : 0,
-                 12.0 * ctx->hostInts[I_NNZ] + 28.0 * ctx->nLiveHost);
+                 10.0 * ctx->hostInts[I_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 28.0 * ctx->nLiveHost);
     ctx->solverIter = iters;
     ctx->outerIter = 1;
     ctx->solverErr = ctx->hostScalars[S_RHSN2] > 0 ? sqrt(ctx->hostScalars[S_RN2] / ctx->hostScalars[S_RHSN2]) : 0.0;
@@ -540,7 +583,7 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
     const int maxIt = ctx->P.solver_max_iter > 0 ? ctx->P.solver_max_iter : 2 * ctx->nLiveHost;
     std::vector<float> times;
     int total = 0, outer = 0, rc;
-    const double bytes32 = 8.0 * ctx->hostInts[I_NNZ] + 20.0 * ctx->nLiveHost;
+    const double bytes32 = 6.0 * ctx->hostInts[I_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 16.0 * ctx->nLiveHost;
     for (;;) {
         hipLaunchKernelGGL(kTrueResidual, dim3(nb), dim3(kB), 0, s, a, c);
         hipLaunchKernelGGL(kTrueResidualFinish, dim3(1), dim3(1024), 0, s, a, nb);
@@ -553,7 +596,7 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
         if (rn2 < tol * tol * bn2 || outer >= kMaxOuter || total >= maxIt) break;
         outer++;
         hipLaunchKernelGGL(kMxInit, dim3(nb), dim3(kB), 0, s, a, c);
-        hipLaunchKernelGGL(kMxInitFinish, dim3(1), dim3(1024), 0, s, a, nb, tolIn);
+        hipLaunchKernelGGL(kMxInitFinish, dim3(1), dim3(1024), 0, s, a, nb, tolIn, tol);
         int done = 0, it = 0;
         times.clear();
         for (int launched = 0; !done; launched += batch) {
