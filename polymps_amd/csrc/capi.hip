@@ -356,7 +356,7 @@ int mps_create(mps_ctx** out, const mps_params* p, int n, int device) {
     MPS_HIP(hipMalloc(&ctx->ds, sizeof(double) * NDS));
     MPS_HIP(hipMalloc(&ctx->is, sizeof(int) * NIS));
     ctx->redCap = std::max(kRedBlocks, gridFor(n, 256));
-    MPS_HIP(hipMalloc(&ctx->red, sizeof(double) * (2 * ctx->redCap + 16)));
+    MPS_HIP(hipMalloc(&ctx->red, sizeof(double) * (3 * ctx->redCap + 16)));
     MPS_HIP(hipMalloc(&ctx->wtab, sizeof(double) * 4 * kWallTable));
     MPS_HIP(hipHostMalloc(&ctx->hostScalars, sizeof(double) * NDS));
     MPS_HIP(hipHostMalloc(&ctx->hostInts, sizeof(int) * NIS));
@@ -396,6 +396,7 @@ int mps_destroy(mps_ctx* ctx) {
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     for (auto e : ctx->evPool) hipEventDestroy(e);
+    gnnFree(ctx);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return MPS_OK;
@@ -557,6 +558,13 @@ int mps_set_scalar(mps_ctx* ctx, const char* name, double value) {
         ctx->spmvMsTotal = 0.0;
         ctx->spmvLaunches = 0;
         ctx->spmvBytesTotal = 0.0;
+    } else if (s == "learned_mode") {
+        // 0: off, 1: learned warm start, 2: warm start + learned flexible-CG preconditioner
+        if (value != 0.0 && value != 1.0 && value != 2.0) {
+            ctx->err = "learned_mode must be 0, 1 or 2";
+            return MPS_E_ARG;
+        }
+        ctx->P.learned_mode = (int)value;
     } else if (s == "precision_mode") {
         // takes effect at the next assembly; a solve needs an assembly in the same mode
         if (value != MPS_PREC_FP64 && value != MPS_PREC_MIXED) {
@@ -726,6 +734,104 @@ int mps_ppe_matrix(mps_ctx* ctx, int* row_ptr, int* cols, double* vals, int* nnz
     }
     if (row_ptr) row_ptr[nl] = p;
     return MPS_OK;
+}
+
+// ---------------------------------------------------------- learned operator
+int mps_gnn_param_count(void) { return gnnParamCount(); }
+
+int mps_gnn_init(mps_ctx* ctx, unsigned long long seed) {
+    CHECK_CTX(ctx);
+    hipSetDevice(ctx->dev);
+    int rc = gnnEnsure(ctx);
+    if (rc) return rc;
+    return gnnSetParams(ctx, nullptr, seed);
+}
+
+int mps_gnn_set_params(mps_ctx* ctx, const double* theta, int n) {
+    CHECK_CTX(ctx);
+    if (!theta || n != gnnParamCount()) { ctx->err = "parameter count mismatch"; return MPS_E_ARG; }
+    hipSetDevice(ctx->dev);
+    int rc = gnnEnsure(ctx);
+    if (rc) return rc;
+    return gnnSetParams(ctx, theta, 0);
+}
+
+int mps_gnn_get_params(mps_ctx* ctx, double* theta, int n) {
+    CHECK_CTX(ctx);
+    if (!theta || n != gnnParamCount()) { ctx->err = "parameter count mismatch"; return MPS_E_ARG; }
+    hipSetDevice(ctx->dev);
+    int rc = gnnEnsure(ctx);
+    if (rc) return rc;
+    return gnnGetParams(ctx, theta);
+}
+
+int mps_gnn_forward(mps_ctx* ctx) {
+    CHECK_CTX(ctx);
+    hipSetDevice(ctx->dev);
+    return gnnForward(ctx);
+}
+
+int mps_gnn_graph(mps_ctx* ctx, int* indptr, int* cols, double* e, double* w, double* feat, double* diag, double* cs,
+                  double* dsi, int* nedges) {
+    CHECK_CTX(ctx);
+    if (!ctx->gnn) { ctx->err = "call mps_gnn_forward first"; return MPS_E_ARG; }
+    hipSetDevice(ctx->dev);
+    return gnnDownloadGraph(ctx, indptr, cols, e, w, feat, diag, cs, dsi, nedges);
+}
+
+int mps_gnn_backward(mps_ctx* ctx, const double* r, const double* gz, double* grad, int n) {
+    CHECK_CTX(ctx);
+    if (!ctx->gnn || !r || !gz || !grad || n != gnnParamCount()) { ctx->err = "bad arguments"; return MPS_E_ARG; }
+    hipSetDevice(ctx->dev);
+    int rc = gnnUploadVectors(ctx, r, gz);
+    if (rc) return rc;
+    if ((rc = gnnBackward(ctx))) return rc;
+    std::vector<float> g(n);
+    MPS_HIP(hipMemcpyAsync(g.data(), gnnGradPtr(ctx), sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    MPS_HIP(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < n; k++) grad[k] = g[k];
+    return MPS_OK;
+}
+
+int mps_gnn_train_step(mps_ctx* ctx, double lr, unsigned long long seed, double* loss) {
+    CHECK_CTX(ctx);
+    hipSetDevice(ctx->dev);
+    return gnnTrainStep(ctx, lr, seed, loss);
+}
+
+int mps_gnn_save(mps_ctx* ctx, const char* path) {
+    CHECK_CTX(ctx);
+    if (!path) return MPS_E_ARG;
+    std::vector<double> th(gnnParamCount());
+    int rc = mps_gnn_get_params(ctx, th.data(), (int)th.size());
+    if (rc) return rc;
+    FILE* f = fopen(path, "wb");
+    if (!f) { ctx->err = std::string("cannot write ") + path; return MPS_E_IO; }
+    const char magic[8] = {'M', 'P', 'S', 'G', 'N', 'N', '1', 0};
+    const int np = (int)th.size();
+    std::vector<float> tf(th.begin(), th.end());
+    const bool ok = fwrite(magic, 1, 8, f) == 8 && fwrite(&np, sizeof(int), 1, f) == 1 &&
+                    fwrite(tf.data(), sizeof(float), tf.size(), f) == tf.size();
+    fclose(f);
+    if (!ok) { ctx->err = std::string("short write to ") + path; return MPS_E_IO; }
+    return MPS_OK;
+}
+
+int mps_gnn_load(mps_ctx* ctx, const char* path) {
+    CHECK_CTX(ctx);
+    if (!path) return MPS_E_ARG;
+    FILE* f = fopen(path, "rb");
+    if (!f) { ctx->err = std::string("cannot open ") + path; return MPS_E_IO; }
+    char magic[8];
+    int np = 0;
+    bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "MPSGNN1", 8) == 0 && fread(&np, sizeof(int), 1, f) == 1 &&
+              np == gnnParamCount();
+    std::vector<float> tf(ok ? np : 0);
+    if (ok) ok = fread(tf.data(), sizeof(float), tf.size(), f) == tf.size();
+    fclose(f);
+    if (!ok) { ctx->err = std::string("not a parameter file of this network: ") + path; return MPS_E_IO; }
+    std::vector<double> th(tf.begin(), tf.end());
+    return mps_gnn_set_params(ctx, th.data(), np);
 }
 
 int mps_init(mps_ctx* ctx) {

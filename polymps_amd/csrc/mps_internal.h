@@ -69,9 +69,11 @@ struct KArgs {
     const double* wtab;    // xPND[50] xNeigh[50] Zij[50] nNeigh[50] (as double)
     double* ds;            // device scalars
     int* is;
-    double* red;           // reduction partials: one pair per block, then the vel-max slot
+    double* red;           // reduction partials: up to three per block, then the vel-max slot
     double volForce;       // pow(l0, dim) evaluated on the host, as the reference's CPU build does
 };
+
+struct Gnn;
 
 struct Ctx {
     mps_params P;
@@ -143,6 +145,7 @@ struct Ctx {
     int numIter = 0;
     int solverIter = 0;
     int outerIter = 0;
+    Gnn* gnn = nullptr;          // learned PPE operator (created on first use)
     int redCap = 0;              // partial-sum slots in red (pairs)
     double solverErr = 0.0;
     double trueResidual = 0.0;
@@ -179,5 +182,20 @@ int launchOp(Ctx* ctx, const char* op);
 int ppeAssemble(Ctx* ctx);
 int ppeSolve(Ctx* ctx);
 int launchCompaction(Ctx* ctx);
+// learned PPE operator (gnn.hip)
+int gnnEnsure(Ctx* ctx);
+int gnnSetParams(Ctx* ctx, const double* theta, unsigned long long seed);
+int gnnGetParams(Ctx* ctx, double* theta);
+int gnnParamCount();
+int gnnForward(Ctx* ctx);
+int gnnBackward(Ctx* ctx);
+int gnnTrainStep(Ctx* ctx, double lr, unsigned long long seed, double* loss);
+int gnnWarmStart(Ctx* ctx);
+int gnnApply32(Ctx* ctx, const float* r, float* z);
+int gnnDownloadGraph(Ctx* ctx, int* indptr, int* cols, double* e, double* w, double* feat, double* diag,
+                     double* cs, double* dsi, int* nedges);
+int gnnUploadVectors(Ctx* ctx, const double* r, const double* gz);
+float* gnnGradPtr(Ctx* ctx);
+void gnnFree(Ctx* ctx);
 
 }  // namespace mpsgpu

@@ -1574,7 +1574,7 @@ int launchOp(Ctx* ctx, const char* op) {
     else if (o == "wall_press_gradient") LT(kWallPressGradient);
     else if (o == "wall_viscosity") LT(kWallViscosity);
     else if (o == "correction") {
-        unsigned long long* bits = (unsigned long long*)(ctx->red + 2 * ctx->redCap + 8);
+        unsigned long long* bits = (unsigned long long*)(ctx->red + 3 * ctx->redCap + 8);
         MPS_HIP(hipMemsetAsync(bits, 0, sizeof(unsigned long long), ctx->stream));
         hipLaunchKernelGGL(kCorrection, dim3(g), dim3(kBlock), 0, ctx->stream, a, bits);
         hipLaunchKernelGGL(kVelMaxFinish, dim3(1), dim3(1), 0, ctx->stream, a, bits);

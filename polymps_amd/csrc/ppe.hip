@@ -148,6 +148,34 @@ __device__ __forceinline__ void writePartials(const KArgs& a, double s0, double 
     blockReduce2(s0, s1);
     if (threadIdx.x == 0) { a.red[blockIdx.x * 2] = s0; a.red[blockIdx.x * 2 + 1] = s1; }
 }
+// three sums, partials at red[3 * block]
+__device__ __forceinline__ void writePartials3(const KArgs& a, double s0, double s1, double s2) {
+    __shared__ double sh[3][16];
+    s0 = waveSum(s0);
+    s1 = waveSum(s1);
+    s2 = waveSum(s2);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) { sh[0][w] = s0; sh[1][w] = s1; sh[2][w] = s2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0, t1 = 0, t2 = 0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) { t0 += sh[0][q]; t1 += sh[1][q]; t2 += sh[2][q]; }
+        a.red[blockIdx.x * 3] = t0; a.red[blockIdx.x * 3 + 1] = t1; a.red[blockIdx.x * 3 + 2] = t2;
+    }
+}
+__device__ __forceinline__ void finishPartials3(const double* red, int nb, double& s0, double& s1, double& s2) {
+    __shared__ double sh[3][16];
+    double t0 = 0, t1 = 0, t2 = 0;
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) { t0 += red[q * 3]; t1 += red[q * 3 + 1]; t2 += red[q * 3 + 2]; }
+    t0 = waveSum(t0);
+    t1 = waveSum(t1);
+    t2 = waveSum(t2);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) { sh[0][w] = t0; sh[1][w] = t1; sh[2][w] = t2; }
+    __syncthreads();
+    s0 = 0; s1 = 0; s2 = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) { s0 += sh[0][q]; s1 += sh[1][q]; s2 += sh[2][q]; }
+}
 __device__ __forceinline__ void finishPartials(const double* red, int nb, double& s0, double& s1) {
     __shared__ double sh[2][16];
     double t0 = 0, t1 = 0;
@@ -447,6 +475,72 @@ __global__ void kMxApply(KArgs a, CgPtrs c) {
     if (i < a.N && live(a, i)) c.x[i] += (double)c.fd[i];
 }
 
+// ------------------------------------- flexible CG with the learned operator
+// Inner solve of the mixed path with z = M r (M from the learned operator):
+// p_{k+1} = z_{k+1} - (z_{k+1} . q_k) / (p_k . q_k) p_k, q_k = A p_k.
+__global__ void kFcgInit(KArgs a, CgPtrs c) {
+    const int i = slotId();
+    if (i < a.N) {
+        c.fr[i] = (float)c.r[i];
+        c.fd[i] = 0.0f;
+    }
+}
+// after z = M r: p = z; partials r.z, |r|^2 (same finish as kMxInit)
+__global__ void kFcgInitDots(KArgs a, CgPtrs c) {
+    const int i = slotId();
+    double s0 = 0, s1 = 0;
+    if (i < a.N) {
+        const float r = c.fr[i], z = c.fz[i];
+        c.fp[i] = z;
+        s0 = (double)r * (double)z;
+        s1 = (double)r * (double)r;
+    }
+    writePartials(a, s0, s1);
+}
+__global__ void kFcgUpdate(KArgs a, CgPtrs c) {
+    if (a.is[I_CG_DONE]) return;
+    const int i = slotId();
+    const float alpha = (float)a.ds[S_ALPHA];
+    if (i < a.N) {
+        c.fd[i] += alpha * c.fp[i];
+        c.fr[i] = c.fr[i] - alpha * c.fq[i];
+    }
+}
+// partials |r|^2, r.z, z.q
+__global__ void kFcgDots(KArgs a, CgPtrs c) {
+    if (a.is[I_CG_DONE]) return;
+    const int i = slotId();
+    double s0 = 0, s1 = 0, s2 = 0;
+    if (i < a.N) {
+        const double r = c.fr[i], z = c.fz[i], q = c.fq[i];
+        s0 = r * r;
+        s1 = r * z;
+        s2 = z * q;
+    }
+    writePartials3(a, s0, s1, s2);
+}
+// stop test, breakdown test (r.z <= 0: M not positive on r), beta
+__global__ void kFcgBeta(KArgs a, int nb, int maxIt) {
+    if (a.is[I_CG_DONE]) return;
+    double rn2, rz, zq;
+    finishPartials3(a.red, nb, rn2, rz, zq);
+    if (threadIdx.x != 0) return;
+    a.ds[S_RN2] = rn2;
+    if (rn2 < a.ds[S_THR]) {
+        a.is[I_CG_DONE] = 1;
+        return;
+    }
+    if (!(rz > 0.0)) {
+        a.is[I_CG_DONE] = 4;
+        return;
+    }
+    a.ds[S_ABSNEW] = rz;
+    a.ds[S_BETA] = -zq / a.ds[S_PAP];
+    const int it = a.is[I_CG_ITER] + 1;
+    a.is[I_CG_ITER] = it;
+    if (it >= maxIt) a.is[I_CG_DONE] = 3;
+}
+
 // press = max(x, 0) for live slots (setZeroOnNegativePressure :471-476)
 __global__ void kPressFromX(KArgs a, const double* x) {
     const int i = slotId();
@@ -583,6 +677,8 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
     const int maxIt = ctx->P.solver_max_iter > 0 ? ctx->P.solver_max_iter : 2 * ctx->nLiveHost;
     std::vector<float> times;
     int total = 0, outer = 0, rc;
+    bool useM = ctx->P.learned_mode == 2 && ctx->gnn != nullptr;
+    double prevRn2 = 0.0;
     const double bytes32 = 6.0 * ctx->hostInts[I_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 16.0 * ctx->nLiveHost;
     for (;;) {
         hipLaunchKernelGGL(kTrueResidual, dim3(nb), dim3(kB), 0, s, a, c);
@@ -594,8 +690,21 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
             break;
         }
         if (rn2 < tol * tol * bn2 || outer >= kMaxOuter || total >= maxIt) break;
+        // the learned operator stays in use while each refinement pass at
+        // least halves the true residual; otherwise Jacobi takes over
+        if (useM && prevRn2 > 0.0 && rn2 > 0.25 * prevRn2) {
+            useM = false;
+            ctx->fallbackUsed = 1;
+        }
+        prevRn2 = rn2;
         outer++;
-        hipLaunchKernelGGL(kMxInit, dim3(nb), dim3(kB), 0, s, a, c);
+        if (useM) {
+            hipLaunchKernelGGL(kFcgInit, dim3(nb), dim3(kB), 0, s, a, c);
+            gnnApply32(ctx, c.fr, c.fz);
+            hipLaunchKernelGGL(kFcgInitDots, dim3(nb), dim3(kB), 0, s, a, c);
+        } else {
+            hipLaunchKernelGGL(kMxInit, dim3(nb), dim3(kB), 0, s, a, c);
+        }
         hipLaunchKernelGGL(kMxInitFinish, dim3(1), dim3(1024), 0, s, a, nb, tolIn, tol);
         int done = 0, it = 0;
         times.clear();
@@ -603,8 +712,15 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
             for (int q = 0; q < batch; q++) {
                 timedSpmv(ctx, kMxSpmv, a, c, nb, q);
                 hipLaunchKernelGGL(kCgAlpha, dim3(1), dim3(1024), 0, s, a, nb);
-                hipLaunchKernelGGL(kMxUpdate, dim3(nb), dim3(kB), 0, s, a, c);
-                hipLaunchKernelGGL(kCgBeta, dim3(1), dim3(1024), 0, s, a, nb, maxIt);
+                if (useM) {
+                    hipLaunchKernelGGL(kFcgUpdate, dim3(nb), dim3(kB), 0, s, a, c);
+                    gnnApply32(ctx, c.fr, c.fz);
+                    hipLaunchKernelGGL(kFcgDots, dim3(nb), dim3(kB), 0, s, a, c);
+                    hipLaunchKernelGGL(kFcgBeta, dim3(1), dim3(1024), 0, s, a, nb, maxIt);
+                } else {
+                    hipLaunchKernelGGL(kMxUpdate, dim3(nb), dim3(kB), 0, s, a, c);
+                    hipLaunchKernelGGL(kCgBeta, dim3(1), dim3(1024), 0, s, a, nb, maxIt);
+                }
                 hipLaunchKernelGGL(kMxP, dim3(nb), dim3(kB), 0, s, a, c);
             }
             if ((rc = readStatus(ctx, &times, batch))) return rc;
@@ -613,8 +729,12 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
             if (launched > 4 * maxIt + 64) break;
         }
         const int D = ctx->hostInts[I_CG_DONE];
-        addSpmvStats(ctx, times, (D == 1 || D == 3) ? it + 1 : 0, bytes32);
+        addSpmvStats(ctx, times, (D == 1 || D == 3 || D == 4) ? it + 1 : 0, bytes32);
         total += it + 1;
+        if (D == 4) {   // breakdown of the flexible CG: keep its progress, continue with Jacobi
+            useM = false;
+            ctx->fallbackUsed = 1;
+        }
         hipLaunchKernelGGL(kMxApply, dim3(nb), dim3(kB), 0, s, a, c);
     }
     ctx->solverIter = total;
@@ -640,7 +760,14 @@ int ppeSolve(Ctx* ctx) {
         for (auto& e : ctx->evPool) MPS_HIP(hipEventCreate(&e));
     }
     MPS_HIP(hipEventRecord(ctx->ev0, s));
-    int rc = mixed ? solveMixed(ctx, a, c, nb, batch) : solveFp64(ctx, a, c, nb, batch);
+    ctx->fallbackUsed = 0;
+    int rc;
+    if (ctx->P.learned_mode >= 1) {
+        // learned operator for this matrix, then x += M (b - A x)
+        if ((rc = gnnForward(ctx))) return rc;
+        if ((rc = gnnWarmStart(ctx))) return rc;
+    }
+    rc = mixed ? solveMixed(ctx, a, c, nb, batch) : solveFp64(ctx, a, c, nb, batch);
     if (rc) return rc;
     if (!mixed) {   // the mixed path ends on a fresh true residual already
         hipLaunchKernelGGL(kTrueResidual, dim3(nb), dim3(kB), 0, s, a, c);

@@ -64,6 +64,17 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     lib.mps_read_grid.argtypes = [cp, ctypes.POINTER(ip), vp, vp, vp, vp, vp]
     lib.mps_write_grid.argtypes = [cp, ip, vp, vp, vp, vp, vp]
     lib.mps_write_vtu.argtypes = [vp, cp, ip]
+    ull = ctypes.c_ulonglong
+    lib.mps_gnn_param_count.argtypes = []
+    lib.mps_gnn_init.argtypes = [vp, ull]
+    lib.mps_gnn_set_params.argtypes = [vp, vp, ip]
+    lib.mps_gnn_get_params.argtypes = [vp, vp, ip]
+    lib.mps_gnn_forward.argtypes = [vp]
+    lib.mps_gnn_graph.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(ip)]
+    lib.mps_gnn_backward.argtypes = [vp, vp, vp, vp, ip]
+    lib.mps_gnn_train_step.argtypes = [vp, ctypes.c_double, ull, dp]
+    lib.mps_gnn_save.argtypes = [vp, cp]
+    lib.mps_gnn_load.argtypes = [vp, cp]
     for op in _OPS:
         getattr(lib, "mps_" + ("mesh_closest_point" if op == "mesh_closest_point" else op)).argtypes = [vp]
     check_layout(lib)
@@ -212,6 +223,58 @@ class Simulation:
         vals = np.zeros(nnz.value)
         self._check(self._lib.mps_ppe_matrix(self._h, _ptr(rp), _ptr(cols), _ptr(vals), ctypes.byref(nnz)))
         return rp, cols, vals
+
+    # ------------------------------------------------ learned PPE operator
+    def gnn_init(self, seed: int = 0):
+        self._check(self._lib.mps_gnn_init(self._h, seed))
+
+    def gnn_params(self) -> np.ndarray:
+        n = self._lib.mps_gnn_param_count()
+        th = np.zeros(n)
+        self._check(self._lib.mps_gnn_get_params(self._h, _ptr(th), n))
+        return th
+
+    def gnn_set_params(self, theta) -> None:
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        self._check(self._lib.mps_gnn_set_params(self._h, _ptr(th), th.size))
+
+    def gnn_forward(self) -> None:
+        self._check(self._lib.mps_gnn_forward(self._h))
+
+    def gnn_graph(self) -> dict:
+        """Graph, features, diagonal and operator of the last forward pass, in
+        reference order (CSR over the live particles)."""
+        ne = ctypes.c_int()
+        self._check(self._lib.mps_gnn_graph(self._h, None, None, None, None, None, None, None, None, ctypes.byref(ne)))
+        nl = int(self.scalar("num_particles"))
+        out = dict(indptr=np.zeros(nl + 1, dtype=np.int32), cols=np.zeros(ne.value, dtype=np.int32),
+                   e=np.zeros(4 * ne.value), w=np.zeros(ne.value), feat=np.zeros(4 * nl), diag=np.zeros(nl),
+                   cs=np.zeros(ne.value), dsi=np.zeros(nl))
+        self._check(self._lib.mps_gnn_graph(self._h, *(_ptr(out[k]) for k in ("indptr", "cols", "e", "w", "feat",
+                                                                              "diag", "cs", "dsi")),
+                                            ctypes.byref(ne)))
+        out["e"] = out["e"].reshape(-1, 4)
+        out["feat"] = out["feat"].reshape(-1, 4)
+        return out
+
+    def gnn_backward(self, r, gz) -> np.ndarray:
+        n = self._lib.mps_gnn_param_count()
+        g = np.zeros(n)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        gz = np.ascontiguousarray(gz, dtype=np.float64)
+        self._check(self._lib.mps_gnn_backward(self._h, _ptr(r), _ptr(gz), _ptr(g), n))
+        return g
+
+    def gnn_train_step(self, lr: float, seed: int) -> float:
+        loss = ctypes.c_double()
+        self._check(self._lib.mps_gnn_train_step(self._h, lr, seed, ctypes.byref(loss)))
+        return loss.value
+
+    def gnn_save(self, path) -> None:
+        self._check(self._lib.mps_gnn_save(self._h, str(path).encode()))
+
+    def gnn_load(self, path) -> None:
+        self._check(self._lib.mps_gnn_load(self._h, str(path).encode()))
 
     def write_vtu(self, path, flags: int = 1):
         self._check(self._lib.mps_write_vtu(self._h, str(path).encode(), flags))

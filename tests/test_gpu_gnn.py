@@ -1,0 +1,133 @@
+"""The learned PPE operator on the device (needs a gfx950 GPU).
+
+Forward and backward passes are compared with the CPU reference
+(oracle/gnn_oracle.py) on the graph, features and diagonal the device built,
+with the reference rounding the same values to bfloat16 as the device.
+Differences come from FP32 versus FP64 accumulation and the occasional
+activation landing on the other side of a bfloat16 rounding boundary, so
+the tolerances are a few bfloat16 ulps; a wrong lane layout or index gives
+errors of order one.
+"""
+import numpy as np
+import pytest
+
+from oracle import gnn_oracle as G
+from polymps_amd import configs
+from polymps_amd.lib import Simulation
+
+pytestmark = pytest.mark.gpu
+
+
+def developed(l0=0.0125, steps=20, precision=1):
+    w = configs.small_dam3d(l0=l0)
+    w.params.precision_mode = precision
+    g = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh, device=0)
+    g.call("init")
+    g.steps(steps)
+    return w, g
+
+
+def ref_graph(d):
+    return G.Graph(d["indptr"], d["cols"], d["e"], d["w"])
+
+
+def random_theta(seed):
+    rng = np.random.default_rng(seed)
+    th = G.init_params(seed)
+    o = G.offsets()
+    for k in ("v", "vd", "bd", "b0", "p") + tuple(f"c{l}" for l in range(3)) + tuple(f"d{l}" for l in range(3)):
+        a, s = o[k]
+        n = int(np.prod(s))
+        th[a:a + n] = rng.normal(scale=0.2, size=n)
+    return th
+
+
+def test_initial_operator_is_jacobi(built):
+    w, g = developed(steps=5)
+    print("gnn jacobi start", flush=True)
+    g.gnn_init(7)
+    g.gnn_forward()
+    d = g.gnn_graph()
+    assert d["indptr"][-1] > 10 * len(d["diag"])    # about 32 edges per particle
+    assert np.all(d["cs"] == 0.0)
+    assert np.allclose(d["dsi"], 1.0 / d["diag"], rtol=1e-6, atol=0)
+
+
+def test_forward_matches_reference(built):
+    w, g = developed(steps=5)
+    print("gnn forward", flush=True)
+    th = random_theta(3)
+    g.gnn_set_params(th)
+    g.gnn_forward()
+    d = g.gnn_graph()
+    gr = ref_graph(d)
+    cs, dsi, _ = G.forward(th, gr, d["feat"], d["diag"], rb=True)
+    scale = np.abs(cs).max()
+    assert scale > 0
+    err = np.abs(d["cs"] - cs) / scale
+    assert np.quantile(err, 0.999) < 2e-2 and err.mean() < 2e-3, (err.max(), err.mean())
+    rel = np.abs(d["dsi"] - dsi) / np.abs(dsi)
+    assert np.quantile(rel, 0.999) < 2e-2 and rel.mean() < 2e-3, (rel.max(), rel.mean())
+
+
+def test_backward_matches_reference(built):
+    w, g = developed(steps=5)
+    print("gnn backward", flush=True)
+    th = random_theta(4)
+    g.gnn_set_params(th)
+    g.gnn_forward()
+    d = g.gnn_graph()
+    gr = ref_graph(d)
+    rng = np.random.default_rng(11)
+    n = len(d["diag"])
+    r = rng.normal(size=n)
+    gz = rng.normal(size=n)
+    grad = g.gnn_backward(r, gz)
+    _, _, cache = G.forward(th, gr, d["feat"], d["diag"], rb=True)
+    # the device holds r and gz in FP32
+    ref = G.backward(th, gr, cache, r.astype(np.float32).astype(np.float64),
+                     gz.astype(np.float32).astype(np.float64), rb=True)
+    assert np.linalg.norm(grad - ref) < 3e-2 * np.linalg.norm(ref)
+    for k, (a, s) in G.offsets().items():
+        m = int(np.prod(s))
+        gk, rk = grad[a:a + m], ref[a:a + m]
+        nr = np.linalg.norm(rk)
+        if nr > 1e-6 * np.linalg.norm(ref):
+            assert np.linalg.norm(gk - rk) < 0.1 * nr, (k, np.linalg.norm(gk - rk) / nr)
+
+
+def test_training_reduces_loss(built):
+    w, g = developed(steps=5)
+    print("gnn training", flush=True)
+    g.gnn_init(1)
+    losses = []
+    for k in range(40):
+        g.call("step")
+        losses.append(g.gnn_train_step(2e-3, 1000 + k))
+    first, last = np.mean(losses[:5]), np.mean(losses[-5:])
+    assert np.all(np.isfinite(losses))
+    assert last < 0.9 * first, (first, last)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_learned_solve_reaches_tolerance(built, mode):
+    w, g = developed(steps=10)
+    print(f"gnn learned solve mode {mode}", flush=True)
+    g.gnn_init(2)
+    for k in range(20):
+        g.call("step")
+        g.gnn_train_step(2e-3, 2000 + k)
+    x0 = g.get("ppe_x")
+    g.set_scalar("learned_mode", mode)
+    g.call("ppe_assemble")
+    g.call("ppe_solve")
+    pl = g.get("press")
+    assert g.scalar("true_residual") < w.params.solver_tol
+    g.set("ppe_x", x0)
+    g.set_scalar("learned_mode", 0)
+    g.set_scalar("precision_mode", 0)
+    g.call("ppe_assemble")
+    g.call("ppe_solve")
+    pf = g.get("press")
+    scale = np.abs(pf).max()
+    assert np.max(np.abs(pl - pf)) <= 1e-6 * scale
