@@ -911,7 +911,7 @@ int gnnBackward(Ctx* ctx) {
 
 // One training step on the current assembled matrix: vectors v = the
 // residual b - A x of the current solution estimate and seeded noise; loss
-// sum_v |A M v - v|^2 / |v|^2; Adam update.
+// sum_v |A M v - v|^2 / |v|^2; Adam update.  lr = 0 evaluates the loss only.
 int gnnTrainStep(Ctx* ctx, double lr, unsigned long long seed, double* lossOut) {
     int rc = gnnForward(ctx);
     if (rc) return rc;
@@ -931,10 +931,16 @@ int gnnTrainStep(Ctx* ctx, double lr, unsigned long long seed, double* lossOut) 
         MPS_HIP(hipMemsetAsync(g->acc, 0, sizeof(double) * 4, s));
         hipLaunchKernelGGL(kGnnResid, dim3(gb), dim3(256), 0, s, a, ctx->aVal, ctx->aCol, ctx->aEsc, ctx->aCnt,
                            ctx->aDiag, g->zt, g->rv, g->dv1, g->acc);
+        double hacc[2];
+        if (lr == 0.0) {
+            MPS_HIP(hipMemcpyAsync(hacc, g->acc, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+            MPS_HIP(hipStreamSynchronize(s));
+            if (hacc[1] > 0) loss += hacc[0] / hacc[1];
+            continue;
+        }
         hipLaunchKernelGGL(kGnnLossGrad, dim3(gb), dim3(256), 0, s, a, ctx->aVal, ctx->aCol, ctx->aEsc, ctx->aCnt,
                            ctx->aDiag, g->dv1, g->acc, g->gz);
         if ((rc = gnnBackward(ctx))) break;
-        double hacc[2];
         MPS_HIP(hipMemcpyAsync(hacc, g->acc, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
         MPS_HIP(hipStreamSynchronize(s));
         if (hacc[1] > 0) loss += hacc[0] / hacc[1];
@@ -945,6 +951,8 @@ int gnnTrainStep(Ctx* ctx, double lr, unsigned long long seed, double* lossOut) 
         for (int k = 0; k < gnn::NPARAM; k++) gsum[k] += gh[k];
     }
     if (rc) return rc;
+    if (lossOut) *lossOut = loss;
+    if (lr == 0.0) return MPS_OK;
     MPS_HIP(hipMemcpyAsync(g->grad, gsum.data(), sizeof(float) * gnn::NPARAM, hipMemcpyHostToDevice, s));
     g->step++;
     const float b1 = 0.9f, b2 = 0.999f;

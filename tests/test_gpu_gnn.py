@@ -131,3 +131,33 @@ def test_learned_solve_reaches_tolerance(built, mode):
     pf = g.get("press")
     scale = np.abs(pf).max()
     assert np.max(np.abs(pl - pf)) <= 1e-6 * scale
+
+
+def test_trained_network_gain(built):
+    """The committed network (polymps_amd/data/gnn_dam3d.bin, trained by
+    polymps_amd/train_gnn.py on columns of seeds 0-5) against the Jacobi
+    operator it starts from, on a held-out column (seed 950): its objective
+    |A M v - v|^2 / |v|^2 is lower on the same states and vectors, and the
+    learned solves still reach the tolerance."""
+    from pathlib import Path
+    from polymps_amd.train_gnn import column_workload
+    weights = Path(__file__).resolve().parent.parent / "polymps_amd" / "data" / "gnn_dam3d.bin"
+    w = column_workload(950, 0.005, 40)
+    w.params.precision_mode = 1
+    g = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh, device=0)
+    print(f"gnn gain, {w.n} particles", flush=True)
+    g.call("init")
+    g.steps(30)
+    trained, jacobi = [], []
+    for k in range(3):
+        g.call("step")
+        g.gnn_load(weights)
+        trained.append(g.gnn_train_step(0.0, 77 + k))
+        g.gnn_init(0)
+        jacobi.append(g.gnn_train_step(0.0, 77 + k))
+    print(f"objective trained {np.mean(trained):.4g} jacobi {np.mean(jacobi):.4g}", flush=True)
+    assert np.mean(trained) < 0.5 * np.mean(jacobi)
+    g.gnn_load(weights)
+    g.set_scalar("learned_mode", 2)
+    g.call("step")
+    assert g.scalar("true_residual") < w.params.solver_tol
