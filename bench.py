@@ -3,10 +3,13 @@
 
 Workload (default): the 3-D incompressible dam break, closed 1.61 x 0.15 x
 0.6 m box of 12 triangles, fluid block 0.6 x 0.15 x 0.3 m at l0 = 0.0025 m
-(240 x 60 x 120 = 1,728,000 particles), Jacobi-PCG pressure solve to a
-relative residual of 1e-8, dt = 2e-4 s.  Synthetic input: lattice at cell
-centres with seeded +-1% l0 jitter; the initial state and the first pressure
-solve are built before timing.
+(240 x 60 x 120 = 1,728,000 particles), pressure solve to a relative
+residual of 1e-8, dt = 2e-4 s.  Default precision "mixed": FP32 matrix,
+SpMV and Jacobi preconditioner inside an FP64 iterative refinement on the
+true residual (FP64 dot products and positions); "fp64" is the all-FP64
+Jacobi-PCG.  Synthetic input: lattice at cell centres with seeded +-1% l0
+jitter; the initial state and the first pressure solve are built before
+timing.
 
 A step is one full MPS time step (buckets, viscosity, prediction, collision,
 wall distance, PND/NPCD, free surface, PPE assembly + solve, pressure
@@ -15,8 +18,10 @@ exactly --steps steps with all state resident in HBM.
 
 The JSON line also carries the HBM roofline of the dominant kernel (the PPE
 SpMV: algorithmic bytes per launch / mean launch time from HIP events on the
-library's stream) and the CPU oracle timed on this host on one step of the
-same workload.
+library's stream), the PPE comparison (ms/solve to 1e-8 of plain FP64
+Jacobi-PCG, the mixed solve and the learned+FP32 solve on the same in-step
+systems, with the pressure difference to FP64) and the CPU oracle timed on
+this host on one step of the same workload.
 """
 from __future__ import annotations
 
@@ -64,13 +69,79 @@ def copy_calibration(nbytes: int, reps: int = 3):
     return 0
 
 
+# operators of the time step before and after its PPE (the order of mps_step
+# for polygon walls, incompressible, no shifting)
+BEFORE_PPE = ["bucket_update", "viscosity", "add_gravity", "prediction", "check_out_domain", "build_neighbor_list",
+              "collision", "mesh_closest_point", "mesh_update_near", "wall_npcd", "pnd_neigh_npcd", "update_bc"]
+AFTER_PPE = ["correction_matrix", "press_gradient", "wall_press_gradient", "wall_viscosity", "correction",
+             "check_out_domain", "check_out_domain"]
+
+
+def ppe_compare(sim, solves: int, gnn_weights):
+    """PPE ms/solve to the tolerance on the same in-step systems: each step
+    runs up to its PPE, the system is solved from the previous pressure by
+    plain FP64 Jacobi-PCG, the mixed-precision solve and (with weights) the
+    learned warm start + learned flexible-CG preconditioner in the mixed
+    solve; the step then finishes with the FP64 pressure."""
+    import numpy as np
+    rec = {k: [] for k in ("fp64", "mixed", "learned")}
+    its = {k: [] for k in rec}
+    err = {"mixed": 0.0, "learned": 0.0}
+    sim.set_scalar("profile", 0)
+    if gnn_weights is not None:
+        sim.gnn_load(gnn_weights)
+    for _ in range(solves):
+        for op in BEFORE_PPE:
+            sim.call(op)
+        x0 = sim.get("ppe_x")
+        modes = [("fp64", 0, 0), ("mixed", 0, 1)] + ([("learned", 2, 1)] if gnn_weights is not None else [])
+        press = {}
+        for name, learned, prec in modes:
+            sim.set("ppe_x", x0)
+            sim.set_scalar("learned_mode", learned)
+            sim.set_scalar("precision_mode", prec)
+            sim.call("ppe_assemble")
+            sim.call("ppe_solve")
+            rec[name].append(sim.scalar("ppe_ms"))
+            its[name].append(sim.scalar("solver_iter"))
+            press[name] = sim.get("press")
+        scale = max(np.abs(press["fp64"]).max(), 1e-30)
+        for name in err:
+            if name in press:
+                err[name] = max(err[name], float(np.abs(press[name] - press["fp64"]).max() / scale))
+        # finish the step with the FP64 pressure
+        sim.set("ppe_x", x0)
+        sim.set_scalar("learned_mode", 0)
+        sim.set_scalar("precision_mode", 0)
+        sim.call("ppe_assemble")
+        sim.call("ppe_solve")
+        for op in AFTER_PPE:
+            sim.call(op)
+    out = {"solves": solves, "tolerance": "relative residual 1e-8"}
+    for name in rec:
+        if rec[name]:
+            out[f"{name}_ms"] = float(np.mean(rec[name]))
+            out[f"{name}_iterations"] = float(np.mean(its[name]))
+    out["mixed_vs_fp64_time"] = out["mixed_ms"] / out["fp64_ms"]
+    out["mixed_press_rel_err_vs_fp64"] = err["mixed"]
+    if rec["learned"]:
+        out["learned_vs_fp64_time"] = out["learned_ms"] / out["fp64_ms"]
+        out["learned_press_rel_err_vs_fp64"] = err["learned"]
+        out["learned_weights"] = str(gnn_weights.name)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--l0", type=float, default=0.0025)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--ppe-solves", type=int, default=3,
+                    help="in-step PPE systems solved by every method for the PPE comparison (0: skip)")
+    ap.add_argument("--gnn", type=Path, default=ROOT / "polymps_amd" / "data" / "gnn_dam3d.bin",
+                    help="learned-operator weights for the learned+FP32 PPE comparison")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--inner-tol", type=float, default=None,
                     help="relative tolerance of the inner FP32 solve of the mixed PPE")
@@ -111,7 +182,7 @@ def main(argv=None):
     spmv_bytes = sim.scalar("spmv_bytes_total") / max(launches, 1)
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
     result = {
-        "metric": "particle-steps/s at 1.73M 3D INC (PPE Jacobi-PCG to 1e-8)",
+        "metric": "particle-steps/s at 1.73M 3D INC; PPE ms to 1e-8, learned+FP32 vs plain FP64 CG",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -134,6 +205,8 @@ def main(argv=None):
                      "bytes_per_launch": spmv_bytes, "mean_launch_ms": spmv_ms},
         "init_s": t_init,
     }
+    if args.ppe_solves > 0:
+        result["ppe_compare"] = ppe_compare(sim, args.ppe_solves, args.gnn if args.gnn.exists() else None)
     sim.close()
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
