@@ -36,6 +36,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes / algorithmic bytes of the SpMV per launch on this workload, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs calibrated on a 1 GiB copy
+# (FETCH_SIZE x 2, WRITE_SIZE x 1): profiles/spmv_traffic_dam3d_inc_1728k.md
+SPMV_TRAFFIC_RATIO = {"fp64": 1.151, "mixed": 1.089}
 
 
 def cpu_baseline(w, steps: int, threads: int):
@@ -201,7 +205,9 @@ def main(argv=None):
                 "last_outer_iterations": sim.scalar("outer_iterations"),
                 "true_residual": sim.scalar("true_residual"), "spmv_launches": launches},
         "roofline": {"kernel": "kCgSpmv" if args.precision == "fp64" else "kMxSpmv", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": SPMV_TRAFFIC_RATIO[args.precision] * spmv_bytes if args.l0 == 0.0025 else None,
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE ratio measured at l0=0.0025",
                      "bytes_per_launch": spmv_bytes, "mean_launch_ms": spmv_ms},
         "init_s": t_init,
     }
