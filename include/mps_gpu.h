@@ -195,6 +195,32 @@ int mps_output_from_json(const char* json_path, char* vtu_folder, size_t folder_
  * context's current state; flags: MPS_VTU_PND .. MPS_VTU_AUXILIAR. */
 int mps_write_vtu(mps_ctx* ctx, const char* path, int flags);
 
+/* ------------------------------------------------- learned PPE operator */
+/* No counterpart in the reference.  A message-passing network over the reS
+ * neighbour graph that outputs the sparse operator
+ *   (M r)_i = s_i r_i / a_ii + sum_j c_ij r_j / sqrt(a_ii a_jj),
+ * used as warm start and flexible-CG preconditioner when
+ * params.learned_mode is 1 or 2 (Jacobi when its output weights are 0). */
+int mps_gnn_param_count(void);
+/* seeded initial parameters (the Jacobi operator); resets the optimiser */
+int mps_gnn_init(mps_ctx* ctx, unsigned long long seed);
+int mps_gnn_set_params(mps_ctx* ctx, const double* theta, int n);
+int mps_gnn_get_params(mps_ctx* ctx, double* theta, int n);
+int mps_gnn_load(mps_ctx* ctx, const char* path);
+int mps_gnn_save(mps_ctx* ctx, const char* path);
+/* graph, features and operator for the current state and assembled matrix */
+int mps_gnn_forward(mps_ctx* ctx);
+/* the last forward pass in reference order: CSR indptr[nlive+1], cols, edge
+ * features e[4 E], weights w[E], node features feat[4 nlive], diagonal,
+ * operator cs[E] (c_ij / sqrt(a_ii a_jj)), dsi[nlive] (s_i / a_ii); arrays
+ * may be NULL to query E */
+int mps_gnn_graph(mps_ctx* ctx, int* indptr, int* cols, double* e, double* w, double* feat, double* diag,
+                  double* cs, double* dsi, int* nedges);
+/* gradient of sum_i gz_i (M r)_i with respect to the parameters */
+int mps_gnn_backward(mps_ctx* ctx, const double* r, const double* gz, double* grad, int n);
+/* one Adam step on the current assembled matrix (lr = 0: loss only) */
+int mps_gnn_train_step(mps_ctx* ctx, double lr, unsigned long long seed, double* loss);
+
 #ifdef __cplusplus
 }
 #endif
