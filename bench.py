@@ -140,7 +140,10 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--l0", type=float, default=0.0025)
+    ap.add_argument("--workload", choices=["dam3d", "channel4m"], default="dam3d",
+                    help="dam3d: the 1.73M dam break (the metric's workload); channel4m: 4M particles over a "
+                         "32k-triangle heightmap bed with obstacles")
+    ap.add_argument("--l0", type=float, default=None, help="particle spacing (default 0.0025 dam3d, 0.005 channel4m)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--ppe-solves", type=int, default=3,
                     help="in-step PPE systems solved by every method for the PPE comparison (0: skip)")
@@ -163,7 +166,15 @@ def main(argv=None):
     from polymps_amd import configs
     from polymps_amd.lib import Simulation
 
-    w = configs.dam3d_inc(l0=args.l0, steps=args.steps)
+    if args.workload == "dam3d":
+        args.l0 = args.l0 or 0.0025
+        w = configs.dam3d_inc(l0=args.l0, steps=args.steps)
+        wname = f"dam3d_inc l0={args.l0:g} m, {{n}} particles, 12-triangle closed box"
+    else:
+        args.l0 = args.l0 or 0.005
+        w = configs.channel4m(l0=args.l0, steps=args.steps)
+        wname = (f"channel4m l0={args.l0:g} m, {{n}} particles, {w.info['triangles']}-triangle heightmap bed, "
+                 "channel and 3 obstacles")
     w.params.precision_mode = 1 if args.precision == "mixed" else 0
     if args.inner_tol is not None:
         w.params.mixed_inner_tol = args.inner_tol
@@ -197,8 +208,9 @@ def main(argv=None):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
-        "data": "synthetic (seeded jittered lattice, generated 12-triangle box)",
-        "config": {"workload": f"dam3d_inc l0={args.l0:g} m, {n} particles, 12-triangle closed box",
+        "data": ("synthetic (seeded jittered lattice, generated 12-triangle box)" if args.workload == "dam3d" else
+                 "synthetic (seeded jittered lattice, generated fractal heightmap bed, channel and obstacles)"),
+        "config": {"workload": wname.format(n=n),
                    "particles": n, "precision": args.precision, "solver_tol": w.params.solver_tol,
                    "dt": w.params.time_step},
         "ppe": {"last_solve_ms": sim.scalar("ppe_ms"), "last_iterations": sim.scalar("solver_iter"),
@@ -206,7 +218,8 @@ def main(argv=None):
                 "true_residual": sim.scalar("true_residual"), "spmv_launches": launches},
         "roofline": {"kernel": "kCgSpmv" if args.precision == "fp64" else "kMxSpmv", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": SPMV_TRAFFIC_RATIO[args.precision] * spmv_bytes if args.l0 == 0.0025 else None,
+                     "traffic": (SPMV_TRAFFIC_RATIO[args.precision] * spmv_bytes
+                                 if args.workload == "dam3d" and args.l0 == 0.0025 else None),
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE ratio measured at l0=0.0025",
                      "bytes_per_launch": spmv_bytes, "mean_launch_ms": spmv_ms},
         "init_s": t_init,
@@ -214,6 +227,11 @@ def main(argv=None):
     if args.ppe_solves > 0:
         result["ppe_compare"] = ppe_compare(sim, args.ppe_solves, args.gnn if args.gnn.exists() else None)
     sim.close()
+    if args.workload == "channel4m" and not args.no_cpu_baseline:
+        # the oracle's brute-force wall distance (32k faces per particle) makes
+        # even one CPU step of this workload take many minutes
+        result["cpu_baseline"] = None
+        args.no_cpu_baseline = True
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         v, dt = cpu_baseline(w, args.cpu_steps, threads)

@@ -227,6 +227,93 @@ def dam2d_wc(l0: float = 0.005, tank=(1.0, 0.6), column=(0.25, 0.5), seed: int =
                     info={"fluid": nf, "walls": n - nf, "seed": seed})
 
 
+def _lattice_values(seed: int, ix, iy, octave: int):
+    """Seeded uniform [0, 1) value per integer lattice point (splitmix64 of a hash)."""
+    key = (np.asarray(ix, dtype=np.int64) * 73856093) ^ (np.asarray(iy, dtype=np.int64) * 19349663) ^ (octave * 83492791)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + key.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def fractal_noise(x, y, seed: int, octaves: int = 5, base: float = 4.0):
+    """Sum of bilinear value-noise octaves (frequency x2, amplitude x0.5), in [0, 1]."""
+    tot = np.zeros_like(x)
+    amp, norm = 1.0, 0.0
+    for o in range(octaves):
+        f = base * 2.0 ** o
+        u, v = x * f, y * f
+        i0, j0 = np.floor(u).astype(np.int64), np.floor(v).astype(np.int64)
+        fu, fv = u - i0, v - j0
+        fu, fv = fu * fu * (3 - 2 * fu), fv * fv * (3 - 2 * fv)
+        v00 = _lattice_values(seed, i0, j0, o)
+        v10 = _lattice_values(seed, i0 + 1, j0, o)
+        v01 = _lattice_values(seed, i0, j0 + 1, o)
+        v11 = _lattice_values(seed, i0 + 1, j0 + 1, o)
+        tot += amp * ((v00 * (1 - fu) + v10 * fu) * (1 - fv) + (v01 * (1 - fu) + v11 * fu) * fv)
+        norm += amp
+        amp *= 0.5
+    return tot / norm
+
+
+def channel_mesh(seed: int = 5, length=4.0, width=1.0, height=1.0, nhx: int = 256, nhy: int = 64, amp: float = 0.15,
+                 flat_until: float = 1.05, ramp: float = 0.3, boxes: int = 3):
+    """The wall mesh of the 4 x 1 x 1 m channel: a heightmap bed of nhx x nhy
+    vertices (2 (nhx-1)(nhy-1) triangles, seeded fractal noise of amplitude
+    amp, flat under the reservoir x < flat_until and ramped in over `ramp`),
+    the channel's four sides and top (10 triangles) and `boxes` seeded
+    axis-aligned obstacles of 12 triangles standing on the bed.  Normals point
+    away from the fluid.  Vertices are rounded to float32 as in an STL file."""
+    xs = np.linspace(0.0, length, nhx)
+    ys = np.linspace(0.0, width, nhy)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    t = np.clip((X - flat_until) / ramp, 0.0, 1.0)
+    Z = amp * (t * t * (3 - 2 * t)) * fractal_noise(X, Y, seed)
+    V = [np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)]
+    F = []
+    vid = lambda i, j: i * nhy + j  # noqa: E731
+    for i in range(nhx - 1):
+        for j in range(nhy - 1):
+            a, b, c, d = vid(i, j), vid(i + 1, j), vid(i + 1, j + 1), vid(i, j + 1)
+            F += [[a, c, b], [a, d, c]]   # clockwise from above: normal -z (away from the fluid)
+    F = [np.array(F, dtype=np.int64)]
+    nv = nhx * nhy
+    # sides and top of the channel: the closed box without its bottom pair
+    BV, BF = box_mesh((0.0, 0.0, 0.0), (length, width, height))
+    V.append(BV)
+    F.append(BF[2:] + nv)
+    nv += len(BV)
+    u = splitmix64_uniform(7_000 + seed, 4 * boxes)
+    for k in range(boxes):
+        sx, sy = 0.10 + 0.10 * u[4 * k], 0.10 + 0.10 * u[4 * k + 1]
+        x0 = 1.6 + (1.8 - sx) * u[4 * k + 2] + 0.6 * k
+        y0 = 0.05 + (0.9 - sy) * u[4 * k + 3]
+        OV, OF = box_mesh((x0, y0, 0.0), (x0 + sx, y0 + sy, amp + 0.15))
+        V.append(OV)
+        F.append(OF[:, ::-1] + nv)   # reversed winding: normals into the obstacle, away from the fluid
+        nv += len(OV)
+    V = np.concatenate(V).astype(np.float32).astype(np.float64)
+    F = np.concatenate(F).astype(np.int32)
+    return V, F
+
+
+def channel4m(l0: float = 0.005, seed: int = 5, jitter: float = 0.01, steps: int = 200, dt: float = 2e-4,
+              tol: float = 1e-8, reservoir=(1.0, 1.0, 0.5), **mesh_kw) -> Workload:
+    """3-D incompressible flow over a heightmap bed with obstacles: channel
+    4 x 1 x 1 m, reservoir block 1.0 x 1.0 x 0.5 m at l0 = 0.005 m
+    (200 x 200 x 100 = 4,000,000 particles), mixed-precision PPE."""
+    w = dam3d_inc(l0=l0, tank=(4.0, 1.0, 1.0), block=reservoir, seed=seed, jitter=jitter, steps=steps, dt=dt,
+                  tol=tol)
+    V, F = channel_mesh(seed=seed, **mesh_kw)
+    w.mesh = mesh_arrays(V, F)
+    w.params.precision_mode = 1
+    w.name = f"channel4m_l0_{l0:g}"
+    w.info.update(triangles=len(F))
+    return w
+
+
 # -------------------------------------------------------------- case files
 def case_json(p: MpsParams, grid: str, mesh: str, vtu_folder: str, vtu: dict | None = None) -> dict:
     """The parameter block as an input JSON file of the reference's schema
@@ -322,4 +409,5 @@ WORKLOADS = {
     "dam2d_wc": lambda: dam2d_wc(),
     "dam3d_inc_216k": lambda: dam3d_inc(l0=0.005, steps=500),
     "dam3d_inc_1728k": lambda: dam3d_inc(l0=0.0025, steps=300),
+    "channel4m": lambda: channel4m(),
 }

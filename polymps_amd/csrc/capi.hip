@@ -1,4 +1,5 @@
 // Context management and the C ABI of the MI355X MPS library.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -39,6 +40,8 @@ KArgs Ctx::args() const {
     a.mF = mF;
     a.mN = mN;
     a.nf = nf;
+    a.bvh = (bvhMode == 1 || (bvhMode < 0 && nf > 64)) ? bvh : nullptr;
+    a.bvhTri = bvhTri;
     a.wtab = wtab;
     a.ds = ds;
     a.is = is;
@@ -384,7 +387,7 @@ int mps_destroy(mps_ctx* ctx) {
     for (int f = 0; f < NI; f++) hipFree(ctx->I[f]);
     for (int f = 0; f < NPI; f++) hipFree(ctx->Ialt[f]);
     void* bufs[] = {ctx->cellStart, ctx->cellCount, ctx->key, ctx->rank, ctx->order, ctx->order2, ctx->slotOf,
-                    ctx->scanTmp, ctx->outList, ctx->nbrMask, ctx->nbr, ctx->nbrCnt, ctx->P4, ctx->mV, ctx->mF, ctx->mN, ctx->wtab,
+                    ctx->bvh, ctx->bvhTri, ctx->scanTmp, ctx->outList, ctx->nbrMask, ctx->nbr, ctx->nbrCnt, ctx->P4, ctx->mV, ctx->mF, ctx->mN, ctx->wtab,
                     ctx->aVal, ctx->aCol, ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->aVal32, ctx->aDiagInv32,
                     ctx->aDiag32, ctx->cgR, ctx->cgP,
                     ctx->cgZ, ctx->cgQ, ctx->cgInvD, ctx->cgX, ctx->fR, ctx->fP, ctx->fZ, ctx->fQ, ctx->fX,
@@ -558,6 +561,13 @@ int mps_set_scalar(mps_ctx* ctx, const char* name, double value) {
         ctx->spmvMsTotal = 0.0;
         ctx->spmvLaunches = 0;
         ctx->spmvBytesTotal = 0.0;
+    } else if (s == "mesh_bvh") {
+        // closest-point search: -1 BVH above 64 faces (default), 0 brute force, 1 BVH
+        if (value != -1.0 && value != 0.0 && value != 1.0) {
+            ctx->err = "mesh_bvh must be -1, 0 or 1";
+            return MPS_E_ARG;
+        }
+        ctx->bvhMode = (int)value;
     } else if (s == "learned_mode") {
         // 0: off, 1: learned warm start, 2: warm start + learned flexible-CG preconditioner
         if (value != 0.0 && value != 1.0 && value != 2.0) {
@@ -584,6 +594,83 @@ int mps_sync(mps_ctx* ctx) {
     return syncStatus(ctx);
 }
 
+}  // extern "C"
+
+namespace mpsgpu {
+// Median-split BVH over the triangles (longest axis of the centroid bounds,
+// at most 4 triangles per leaf), built once on the host for a static mesh.
+static int buildBvh(Ctx* ctx, const double* V, const int* F, int nf) {
+    struct Box { double lo[3], hi[3]; };
+    std::vector<Box> tb(nf);
+    std::vector<double> cen(3 * (size_t)nf);
+    for (int f = 0; f < nf; f++) {
+        for (int k = 0; k < 3; k++) { tb[f].lo[k] = 1e300; tb[f].hi[k] = -1e300; }
+        for (int c = 0; c < 3; c++) {
+            const double* v = V + 3 * F[3 * f + c];
+            for (int k = 0; k < 3; k++) {
+                tb[f].lo[k] = std::min(tb[f].lo[k], v[k]);
+                tb[f].hi[k] = std::max(tb[f].hi[k], v[k]);
+            }
+        }
+        for (int k = 0; k < 3; k++) cen[3 * f + k] = 0.5 * (tb[f].lo[k] + tb[f].hi[k]);
+    }
+    std::vector<int> idx(nf);
+    for (int f = 0; f < nf; f++) idx[f] = f;
+    std::vector<BvhNode> nodes;
+    nodes.reserve(2 * (size_t)nf / 2 + 8);
+    struct Item { int node, begin, end; };
+    std::vector<Item> work;
+    nodes.push_back(BvhNode{});
+    work.push_back({0, 0, nf});
+    while (!work.empty()) {
+        const Item it = work.back();
+        work.pop_back();
+        BvhNode nd;
+        for (int k = 0; k < 3; k++) { nd.lo[k] = 1e300; nd.hi[k] = -1e300; }
+        double clo[3] = {1e300, 1e300, 1e300}, chi[3] = {-1e300, -1e300, -1e300};
+        for (int q = it.begin; q < it.end; q++) {
+            const int f = idx[q];
+            for (int k = 0; k < 3; k++) {
+                nd.lo[k] = std::min(nd.lo[k], tb[f].lo[k]);
+                nd.hi[k] = std::max(nd.hi[k], tb[f].hi[k]);
+                clo[k] = std::min(clo[k], cen[3 * f + k]);
+                chi[k] = std::max(chi[k], cen[3 * f + k]);
+            }
+        }
+        if (it.end - it.begin <= 4) {
+            nd.a = it.begin;
+            nd.b = -(it.end - it.begin);
+            nodes[it.node] = nd;
+            continue;
+        }
+        int ax = 0;
+        for (int k = 1; k < 3; k++)
+            if (chi[k] - clo[k] > chi[ax] - clo[ax]) ax = k;
+        const int mid = (it.begin + it.end) / 2;
+        std::nth_element(idx.begin() + it.begin, idx.begin() + mid, idx.begin() + it.end, [&](int x, int y) {
+            return cen[3 * x + ax] < cen[3 * y + ax] || (cen[3 * x + ax] == cen[3 * y + ax] && x < y);
+        });
+        nd.a = (int)nodes.size();
+        nodes.push_back(BvhNode{});
+        nd.b = (int)nodes.size();
+        nodes.push_back(BvhNode{});
+        nodes[it.node] = nd;
+        work.push_back({nd.a, it.begin, mid});
+        work.push_back({nd.b, mid, it.end});
+    }
+    if (ctx->bvh) (void)hipFree(ctx->bvh);
+    if (ctx->bvhTri) (void)hipFree(ctx->bvhTri);
+    MPS_HIP(hipMalloc(&ctx->bvh, sizeof(BvhNode) * nodes.size()));
+    MPS_HIP(hipMalloc(&ctx->bvhTri, sizeof(int) * (size_t)nf));
+    MPS_HIP(hipMemcpy(ctx->bvh, nodes.data(), sizeof(BvhNode) * nodes.size(), hipMemcpyHostToDevice));
+    MPS_HIP(hipMemcpy(ctx->bvhTri, idx.data(), sizeof(int) * (size_t)nf, hipMemcpyHostToDevice));
+    ctx->bvhNodes = (int)nodes.size();
+    return MPS_OK;
+}
+}  // namespace mpsgpu
+
+extern "C" {
+
 int mps_set_mesh(mps_ctx* ctx, int nv, const double* V, int nf, const int* F, const double* N) {
     CHECK_CTX(ctx);
     if (nv <= 0 || nf <= 0 || !V || !F || !N) return MPS_E_ARG;
@@ -602,7 +689,7 @@ int mps_set_mesh(mps_ctx* ctx, int nv, const double* V, int nf, const int* F, co
     ctx->nv = nv;
     ctx->nf = nf;
     ctx->hasMesh = true;
-    return MPS_OK;
+    return buildBvh(ctx, V, F, nf);
 }
 
 int mps_get_wall_tables(mps_ctx* ctx, double* xpnd, double* xneigh, double* zij, int* nneigh) {

@@ -48,6 +48,13 @@ constexpr int kWallTable = 50;
 constexpr int kPpeEsc = 12;
 constexpr int kRedBlocks = 1024;   // partial-sum slots of a reduction
 
+// Bounding-volume hierarchy over the wall triangles: a leaf holds the
+// triangles bvhTri[a .. a - b - 1] (b < 0); an inner node has children a, b.
+struct BvhNode {
+    double lo[3], hi[3];
+    int a, b;
+};
+
 // Everything a kernel may touch, passed by value.
 struct KArgs {
     mps_params P;
@@ -66,6 +73,8 @@ struct KArgs {
     const int* mF;
     const double* mN;
     int nf;
+    const BvhNode* bvh;    // nullptr: brute-force closest point
+    const int* bvhTri;
     const double* wtab;    // xPND[50] xNeigh[50] Zij[50] nNeigh[50] (as double)
     double* ds;            // device scalars
     int* is;
@@ -74,6 +83,7 @@ struct KArgs {
 };
 
 struct Gnn;
+
 
 struct Ctx {
     mps_params P;
@@ -109,6 +119,10 @@ struct Ctx {
     int* mF = nullptr;
     double* mN = nullptr;
     int nv = 0, nf = 0;
+    BvhNode* bvh = nullptr;
+    int* bvhTri = nullptr;
+    int bvhNodes = 0;
+    int bvhMode = -1;            // -1: BVH for meshes of more than 64 faces, 0: brute force, 1: BVH
     double* wtab = nullptr;
     std::vector<double> hxPND, hxNeigh, hZij;
     std::vector<int> hnNeigh;

@@ -550,6 +550,60 @@ __global__ void kClosestPoint(KArgs a) {
     }
 }
 
+// The same query through the triangle BVH: nearer child first, a subtree is
+// skipped when its box is farther than the best distance (with a relative
+// margin far above rounding, so that no subtree holding an equally close face
+// is skipped), and among equally close faces the lowest index wins -- the
+// result of the brute-force loop above.
+__device__ __forceinline__ double boxDist2(const BvhNode& n, const double* p) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double d = fmax(fmax(n.lo[k] - p[k], p[k] - n.hi[k]), 0.0);
+        s += d * d;
+    }
+    return s;
+}
+constexpr int kBvhStack = 64;
+__global__ void kClosestPointBvh(KArgs a) {
+    const int i = slotId();
+    if (!live(a, i)) return;
+    const double p[3] = {PX(i), PY(i), PZ(i)};
+    double best = 1.79769313486231570e308, q[3], bq[3] = {0, 0, 0};
+    int bf = -1;
+    int stack[kBvhStack];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const BvhNode n = a.bvh[stack[--sp]];
+        if (boxDist2(n, p) > best * (1.0 + 1e-9)) continue;
+        if (n.b < 0) {
+            for (int t = n.a; t < n.a - n.b; t++) {
+                const int f = a.bvhTri[t];
+                const int t0 = a.mF[f * 3], t1 = a.mF[f * 3 + 1], t2 = a.mF[f * 3 + 2];
+                closestPtTri(p, a.mV + t0 * 3, a.mV + t1 * 3, a.mV + t2 * 3, q);
+                const double e0 = p[0] - q[0], e1 = p[1] - q[1], e2 = p[2] - q[2];
+                const double d2 = e0 * e0 + e1 * e1 + e2 * e2;
+                if (d2 < best || (d2 == best && f < bf)) { best = d2; bf = f; bq[0] = q[0]; bq[1] = q[1]; bq[2] = q[2]; }
+            }
+        } else if (sp + 2 <= kBvhStack) {   // the depth of a median-split tree is far below the stack
+            const double da = boxDist2(a.bvh[n.a], p), db = boxDist2(a.bvh[n.b], p);
+            if (da <= db) { stack[sp++] = n.b; stack[sp++] = n.a; }
+            else { stack[sp++] = n.a; stack[sp++] = n.b; }
+        }
+    }
+    if (bf < 0) return;
+    if (best < a.D[DRIW2][i]) {
+        a.D[DRIW2][i] = best;
+        a.D[DWPX][i] = bq[0]; a.D[DWPY][i] = bq[1]; a.D[DWPZ][i] = bq[2];
+        a.D[DMX][i] = p[0] + 2.0 * (bq[0] - p[0]);
+        a.D[DMY][i] = p[1] + 2.0 * (bq[1] - p[1]);
+        a.D[DMZ][i] = p[2] + 2.0 * (bq[2] - p[2]);
+        a.D[DPNX][i] = a.mN[bf * 3]; a.D[DPNY][i] = a.mN[bf * 3 + 1]; a.D[DPNZ][i] = a.mN[bf * 3 + 2];
+        a.I[INEARMESH][i] = 0;
+    }
+}
+
 // PolygonMesh::interpolateWij / interpolateNumNeighWall (src/PolygonMesh.cpp:533-581)
 __device__ __forceinline__ double interpZ(const double* xd, const double* yd, double re, double x) {
     const double r = x / re;
@@ -1545,7 +1599,8 @@ int launchOp(Ctx* ctx, const char* op) {
         if (P.collision_type == MPS_COLLISION_DPC) ctx->listValid = false;
     } else if (o == "closest_point") {
         if (!ctx->hasMesh) { ctx->err = "no wall mesh installed"; return MPS_E_ARG; }
-        L1(kClosestPoint);
+        if (a.bvh) L1(kClosestPointBvh);
+        else L1(kClosestPoint);
     } else if (o == "update_near_mesh") L1(kUpdateNear);
     else if (o == "wall_npcd") LT(kWallNPCD);
     else if (o == "correction_matrix") LT(kCorrectionMatrix);

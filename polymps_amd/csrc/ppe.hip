@@ -191,8 +191,10 @@ __device__ __forceinline__ void finishPartials(const double* red, int nb, double
 
 // y_i = diag_i x_i + sum_c val[c][i] x[j_c]; the column of entry c is the
 // previous one plus code[c][i], or the next slot of the escape table when the
-// code is 0.  Four entries are read per step so that four gathers are in
-// flight per thread.
+// code is 0.  Eight entries are read per step so that eight value / code
+// loads and then eight gathers are in flight per thread; the value and code
+// streams are read once per launch and are loaded non-temporally so that they
+// do not displace the gathered x from the caches.
 template <typename T>
 __device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsigned short* __restrict__ code,
                                          const int* __restrict__ esc, int n, size_t N, const T* __restrict__ x, int i,
@@ -207,24 +209,28 @@ __device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsign
         return j;
     };
     int c = 0;
-    for (; c + 4 <= n; c += 4) {
+    for (; c + 8 <= n; c += 8) {
         const size_t o = (size_t)c * N;
-        const T v0 = vl[o], v1 = vl[o + N], v2 = vl[o + 2 * N], v3 = vl[o + 3 * N];
-        const unsigned d0 = cd[o], d1 = cd[o + N], d2 = cd[o + 2 * N], d3 = cd[o + 3 * N];
-        const int j0 = next(d0);
-        const int j1 = next(d1);
-        const int j2 = next(d2);
-        const int j3 = next(d3);
-        const T x0 = x[j0], x1 = x[j1], x2 = x[j2], x3 = x[j3];
-        acc += (double)(v0 * x0);
-        acc += (double)(v1 * x1);
-        acc += (double)(v2 * x2);
-        acc += (double)(v3 * x3);
+        T v[8];
+        unsigned d[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            v[k] = __builtin_nontemporal_load(vl + o + k * N);
+            d[k] = __builtin_nontemporal_load(cd + o + k * N);
+        }
+        int jj[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) jj[k] = next(d[k]);
+        T xv[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) xv[k] = x[jj[k]];
+#pragma unroll
+        for (int k = 0; k < 8; k++) acc += (double)(v[k] * xv[k]);
     }
     for (; c < n; c++) {
         const size_t o = (size_t)c * N;
-        const int jj = next(cd[o]);
-        acc += (double)(vl[o] * x[jj]);
+        const int jc = next(cd[o]);
+        acc += (double)(vl[o] * x[jc]);
     }
     return acc;
 }

@@ -110,3 +110,17 @@ def test_full_step_tracks_oracle(built):
     assert np.max(np.abs(pg - po)) < 1e-9 * w.params.part_dist * 1e3
     assert np.array_equal(g.get("bc"), o.get("bc"))
     assert g.scalar("true_residual") < 1e-8
+
+
+def test_lockstep_fp64_pressure_1e10(built):
+    """All-FP64 mode with both solvers run to a relative residual of 1e-13:
+    the pressure agrees with the oracle to 1e-10 of the largest pressure
+    (the two CGs differ only in the order of their dot-product sums)."""
+    w = configs.small_dam3d(l0=0.0125)
+    w.params.solver_tol = 1e-13
+    g, o = new_pair(w)
+    o.call("init")
+    o.steps(40)
+    sync_state(g, o)
+    for s in range(2):
+        lockstep_step(g, o, s, press_rtol=1e-10)
