@@ -80,7 +80,10 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
  * "list_max" "list_capacity" "capacity" "nnz" "spmv_ms_total"
  * "spmv_launches" "spmv_bytes_total".  Settable: "vel_max" "time"
  * "profile" (nonzero: time every SpMV launch with HIP events, counters reset)
- * "precision_mode" (MPS_PREC_*, used from the next PPE assembly). */
+ * "precision_mode" (MPS_PREC_*, used from the next PPE assembly)
+ * "learned_mode" (0 off, 1 learned warm start, 2 warm start + learned
+ * flexible-CG preconditioner) "mesh_bvh" (closest-point search: -1 BVH for
+ * meshes above 64 faces, 0 brute force, 1 BVH; both give the same result). */
 int mps_get_scalar(mps_ctx* ctx, const char* name, double* out);
 int mps_set_scalar(mps_ctx* ctx, const char* name, double value);
 int mps_sync(mps_ctx* ctx);
@@ -186,14 +189,30 @@ int mps_write_grid(const char* path, int n, const int* type, const double* pos, 
 #define MPS_VTU_DEVIATION 4
 #define MPS_VTU_CONCENTRATION 8
 #define MPS_VTU_AUXILIAR 16
-#define MPS_VTU_ONLY_FREE_SURFACE 32  /* parsed; the writer always writes every particle */
-#define MPS_VTU_BINARY 64             /* parsed; the writer always writes ASCII */
+#define MPS_VTU_ONLY_FREE_SURFACE 32  /* free-surface and wall particles only (writeVtuAsciiFreeSurface) */
+#define MPS_VTU_BINARY 64             /* binary VTU (appended raw data) instead of ASCII */
 /* The output part of MpsInputOutput::readInputFile (src/MpsInputOutput.cpp:140-151,
  * 173-179): pathNames.vtu_output_folder and the MPS_VTU_* flags. */
 int mps_output_from_json(const char* json_path, char* vtu_folder, size_t folder_cap, int* vtu_flags);
 /* MpsInputOutput::writeVtuAscii (src/MpsInputOutput.cpp:1313-1579) for the
- * context's current state; flags: MPS_VTU_PND .. MPS_VTU_AUXILIAR. */
+ * context's current state; flags: MPS_VTU_PND .. MPS_VTU_AUXILIAR;
+ * MPS_VTU_ONLY_FREE_SURFACE for writeVtuAsciiFreeSurface (:1582-1875: the
+ * particles with the free-surface BC or the wall type, no ParticleType array,
+ * polygonNormal of every particle as the reference prints it); and
+ * MPS_VTU_BINARY for a binary file with the same arrays (appended raw data,
+ * UInt64 block headers; the reference's binary writer, :921-1108, is marked
+ * as not working in include/MpsInputOutput.h:85). */
 int mps_write_vtu(mps_ctx* ctx, const char* path, int flags);
+/* The same file, written in the background: the state is snapshotted in
+ * stream order (later operator calls may run at once), copied to pinned host
+ * memory on a second stream and formatted by a writer thread.  Two snapshots
+ * can be in flight; a third call waits for the oldest.  Errors of background
+ * writes are returned by mps_output_wait (and by mps_write_vtu, which waits
+ * for every pending write).  mps_destroy waits as well.  No reference
+ * counterpart: the reference's writeOutputFiles (src/MpsInputOutput.cpp:861-886)
+ * blocks the step loop while it prints. */
+int mps_write_vtu_async(mps_ctx* ctx, const char* path, int flags);
+int mps_output_wait(mps_ctx* ctx);
 
 /* ------------------------------------------------- learned PPE operator */
 /* No counterpart in the reference.  A message-passing network over the reS

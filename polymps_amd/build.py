@@ -25,7 +25,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MPS_OFFLOAD_ARCH", "gfx950")
 
 COMMON = [
-    "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-O3", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off",
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]
 HIP_ONLY = [
@@ -64,7 +64,7 @@ def build(force: bool = False, jobs: int | None = None) -> Path:
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *map(str, objs), "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

@@ -382,6 +382,7 @@ int mps_destroy(mps_ctx* ctx) {
     if (!ctx) return MPS_OK;
     hipSetDevice(ctx->dev);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    vtuFree(ctx);
     for (int f = 0; f < ND; f++) hipFree(ctx->D[f]);
     for (int f = 0; f < NPD; f++) hipFree(ctx->Dalt[f]);
     for (int f = 0; f < NI; f++) hipFree(ctx->I[f]);
@@ -448,6 +449,24 @@ int mps_upload_state(mps_ctx* ctx, const int* type, const double* pos, const dou
     ctx->numIter = 0;
     ctx->timeCurrent = 0.0;
     return MPS_OK;
+}
+
+int mps_write_vtu(mps_ctx* ctx, const char* path, int flags) {
+    CHECK_CTX(ctx);
+    if (!path) return MPS_E_ARG;
+    const int rc = vtuSubmit(ctx, path, flags);
+    return rc != MPS_OK ? rc : vtuWait(ctx);
+}
+
+int mps_write_vtu_async(mps_ctx* ctx, const char* path, int flags) {
+    CHECK_CTX(ctx);
+    if (!path) return MPS_E_ARG;
+    return vtuSubmit(ctx, path, flags);
+}
+
+int mps_output_wait(mps_ctx* ctx) {
+    CHECK_CTX(ctx);
+    return vtuWait(ctx);
 }
 
 int mps_download(mps_ctx* ctx, const char* field, void* out, size_t count) {

@@ -1,8 +1,7 @@
-// Host-side file formats: binary / ASCII STL (PolygonMesh::readPolygonMeshFile,
-// src/PolygonMesh.cpp:42-108), the .grid particle file
-// (MpsInputOutput::readMpsParticleFile, src/MpsInputOutput.cpp:645-857) and the
-// ASCII VTU particle output (MpsInputOutput::writeVtuAscii,
-// src/MpsInputOutput.cpp:1313-1579).
+// Host-side input formats: binary / ASCII STL
+// (PolygonMesh::readPolygonMeshFile, src/PolygonMesh.cpp:42-108) and the .grid
+// particle file (MpsInputOutput::readMpsParticleFile,
+// src/MpsInputOutput.cpp:645-857).  VTU output is in output.hip.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -155,117 +154,6 @@ int mps_write_grid(const char* path, int n, const int* type, const double* pos, 
         fprintf(f, "%d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", type[i], pos[i * 3], pos[i * 3 + 1],
                 pos[i * 3 + 2], vel[i * 3], vel[i * 3 + 1], vel[i * 3 + 2], press[i], press_avg[i]);
     fclose(f);
-    return MPS_OK;
-}
-
-int mps_write_vtu(mps_ctx* ctx, const char* path, int flags) {
-    if (!ctx || !path) return MPS_E_ARG;
-    double np = 0, capd = 0;
-    int rc = mps_get_scalar(ctx, "num_particles", &np);
-    if (rc) return rc;
-    if ((rc = mps_get_scalar(ctx, "capacity", &capd))) return rc;
-    mps_params P;
-    mps_get_params(ctx, &P);
-    const int n = (int)np;     // live particles come first in reference order
-    const int N = (int)capd;   // download buffers hold every slot
-    std::vector<double> pos, vel, press, pndi, small, npcd, pnorm, conc, gradc;
-    std::vector<int> bc, nn, type, nmt;
-    pos.resize(3 * N); vel.resize(3 * N); press.resize(N); bc.resize(N);
-    if ((rc = mps_download(ctx, "pos", pos.data(), pos.size()))) return rc;
-    if ((rc = mps_download(ctx, "vel", vel.data(), vel.size()))) return rc;
-    if ((rc = mps_download(ctx, "press", press.data(), press.size()))) return rc;
-    if ((rc = mps_download(ctx, "bc", bc.data(), bc.size()))) return rc;
-    FILE* fp = fopen(path, "w");
-    if (!fp) return MPS_E_IO;
-    fprintf(fp, "<?xml version='1.0' encoding='UTF-8'?>\n");
-    fprintf(fp, "<VTKFile xmlns='VTK' byte_order='LittleEndian' version='0.1' type='UnstructuredGrid'>\n");
-    fprintf(fp, "  <UnstructuredGrid>\n");
-    fprintf(fp, "    <Piece NumberOfCells='%d' NumberOfPoints='%d'>\n", n, n);
-    fprintf(fp, "      <Points>\n");
-    fprintf(fp, "        <DataArray type='Float32' Name='Position' NumberOfComponents='3' format='ascii'>\n          ");
-    for (int i = 0; i < n; i++) fprintf(fp, "%lf %lf %lf ", pos[i * 3], pos[i * 3 + 1], pos[i * 3 + 2]);
-    fprintf(fp, "\n        </DataArray>\n");
-    fprintf(fp, "      </Points>\n");
-    fprintf(fp, "      <PointData>\n");
-    fprintf(fp, "        <DataArray type='Float32' Name='Velocity' NumberOfComponents='3' format='ascii'>\n");
-    for (int i = 0; i < n; i++) fprintf(fp, "%f %f %f ", (float)vel[i * 3], (float)vel[i * 3 + 1], (float)vel[i * 3 + 2]);
-    fprintf(fp, "\n        </DataArray>\n");
-    fprintf(fp, "        <DataArray type='Float32' Name='pressure' format='ascii'>\n");
-    for (int i = 0; i < n; i++) fprintf(fp, "%f ", (float)press[i]);
-    fprintf(fp, "\n        </DataArray>\n");
-    fprintf(fp, "        <DataArray type='Int32' Name='BC' format='ascii'>\n");
-    for (int i = 0; i < n; i++) fprintf(fp, "%d ", bc[i]);
-    fprintf(fp, "\n        </DataArray>\n");
-    if (flags & 1) {
-        pndi.resize(N); small.resize(N);
-        mps_download(ctx, "pndi", pndi.data(), N);
-        mps_download(ctx, "pnd_small", small.data(), N);
-        fprintf(fp, "        <DataArray type='Float32' Name='pnd' format='ascii'>\n");
-        for (int i = 0; i < n; i++) fprintf(fp, "%f ", (float)pndi[i]);
-        fprintf(fp, "\n        </DataArray>\n");
-        fprintf(fp, "        <DataArray type='Float32' Name='pndSmall' format='ascii'>\n");
-        for (int i = 0; i < n; i++) fprintf(fp, "%f ", (float)small[i]);
-        fprintf(fp, "\n        </DataArray>\n");
-    }
-    if (flags & 2) {
-        nn.resize(N);
-        mps_download(ctx, "num_neigh", nn.data(), N);
-        fprintf(fp, "        <DataArray type='Int32' Name='nNeigh' format='ascii'>\n");
-        for (int i = 0; i < n; i++) fprintf(fp, "%d ", nn[i]);
-        fprintf(fp, "\n        </DataArray>\n");
-    }
-    if (flags & 4) {
-        npcd.resize(3 * N); pnorm.resize(3 * N);
-        mps_download(ctx, "npcd", npcd.data(), 3 * N);
-        mps_download(ctx, "polygon_normal", pnorm.data(), 3 * N);
-        fprintf(fp, "        <DataArray type='Float32' Name='npcdDeviation' NumberOfComponents='3' format='ascii'>\n");
-        for (int i = 0; i < n; i++) fprintf(fp, "%f %f %f ", (float)npcd[i * 3], (float)npcd[i * 3 + 1], (float)npcd[i * 3 + 2]);
-        fprintf(fp, "\n        </DataArray>\n");
-        fprintf(fp, "        <DataArray type='Float32' Name='polygonNormal' NumberOfComponents='3' format='ascii'>\n");
-        for (int i = 0; i < n; i++) fprintf(fp, "%f %f %f ", (float)pnorm[i * 3], (float)pnorm[i * 3 + 1], (float)pnorm[i * 3 + 2]);
-        fprintf(fp, "\n        </DataArray>\n");
-    }
-    if (flags & 8) {
-        conc.resize(N); gradc.resize(3 * N);
-        mps_download(ctx, "conc", conc.data(), N);
-        mps_download(ctx, "grad_conc", gradc.data(), 3 * N);
-        fprintf(fp, "        <DataArray type='Float32' Name='concentration' format='ascii'>\n");
-        for (int i = 0; i < n; i++) fprintf(fp, "%f ", (float)conc[i]);
-        fprintf(fp, "\n        </DataArray>\n");
-        fprintf(fp, "        <DataArray type='Float32' Name='GradCi' NumberOfComponents='3' format='ascii'>\n");
-        for (int i = 0; i < n; i++) fprintf(fp, "%f %f %f ", (float)gradc[i * 3], (float)gradc[i * 3 + 1], (float)gradc[i * 3 + 2]);
-        fprintf(fp, "\n        </DataArray>\n");
-    }
-    if (flags & 16) {
-        type.resize(N);
-        mps_download(ctx, "type", type.data(), N);
-        fprintf(fp, "        <DataArray type='Int32' Name='ParticleType' format='ascii'>\n");
-        for (int i = 0; i < n; i++) fprintf(fp, "%d ", type[i]);
-        fprintf(fp, "\n        </DataArray>\n");
-        if (P.wall_type == MPS_WALL_POLYGON) {
-            nmt.resize(N);
-            mps_download(ctx, "near_mesh_type", nmt.data(), N);
-            fprintf(fp, "        <DataArray type='Int32' Name='nearMeshType' format='ascii'>\n");
-            for (int i = 0; i < n; i++) fprintf(fp, "%d ", nmt[i]);
-            fprintf(fp, "\n        </DataArray>\n");
-        }
-    }
-    fprintf(fp, "      </PointData>\n");
-    fprintf(fp, "      <Cells>\n");
-    fprintf(fp, "        <DataArray type='Int32' Name='connectivity' format='ascii'>\n");
-    for (int i = 0; i < n; i++) fprintf(fp, "%d ", i);
-    fprintf(fp, "\n        </DataArray>\n");
-    fprintf(fp, "        <DataArray type='Int32' Name='offsets' format='ascii'>\n");
-    for (int i = 0; i < n; i++) fprintf(fp, "%d ", i + 1);
-    fprintf(fp, "\n        </DataArray>\n");
-    fprintf(fp, "        <DataArray type='UInt8' Name='types' format='ascii'>\n");
-    for (int i = 0; i < n; i++) fprintf(fp, "1 ");
-    fprintf(fp, "\n        </DataArray>\n");
-    fprintf(fp, "      </Cells>\n");
-    fprintf(fp, "    </Piece>\n");
-    fprintf(fp, "  </UnstructuredGrid>\n");
-    fprintf(fp, "</VTKFile>\n");
-    fclose(fp);
     return MPS_OK;
 }
 

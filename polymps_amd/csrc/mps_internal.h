@@ -83,6 +83,7 @@ struct KArgs {
 };
 
 struct Gnn;
+struct VtuQueue;
 
 
 struct Ctx {
@@ -174,6 +175,8 @@ struct Ctx {
     long spmvLaunches = 0;
     double spmvBytesTotal = 0.0;
     std::string err;
+    // asynchronous VTU output (output.hip)
+    VtuQueue* vtu = nullptr;
 
     KArgs args() const;
 };
@@ -211,5 +214,10 @@ int gnnDownloadGraph(Ctx* ctx, int* indptr, int* cols, double* e, double* w, dou
 int gnnUploadVectors(Ctx* ctx, const double* r, const double* gz);
 float* gnnGradPtr(Ctx* ctx);
 void gnnFree(Ctx* ctx);
+// VTU output (output.hip): snapshot in stream order, file written by a
+// background thread; vtuWait joins every pending writer.
+int vtuSubmit(Ctx* ctx, const char* path, int flags);
+int vtuWait(Ctx* ctx);
+void vtuFree(Ctx* ctx);
 
 }  // namespace mpsgpu

@@ -224,8 +224,16 @@ __device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsign
         T xv[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) xv[k] = x[jj[k]];
+        if (sizeof(T) == 4) {
+            // FP32 (inner solve): the batch is summed in FP32, then added in FP64
+            T b8 = v[0] * xv[0];
 #pragma unroll
-        for (int k = 0; k < 8; k++) acc += (double)(v[k] * xv[k]);
+            for (int k = 1; k < 8; k++) b8 += v[k] * xv[k];
+            acc += (double)b8;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; k++) acc += (double)(v[k] * xv[k]);
+        }
     }
     for (; c < n; c++) {
         const size_t o = (size_t)c * N;

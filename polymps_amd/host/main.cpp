@@ -175,6 +175,7 @@ int main(int argc, char** argv) {
         mainLoop(&particleSystem, &particles, solidMesh.empty() ? nullptr : solidMesh.data(), &inputOutput, &buckets,
                  &particlePndNeigh, &particlePress, &particleShifting, &particleViscosity, &particleCollision,
                  &boundaryConditions, &vectorMatrix, &particleVelPos);
+        inputOutput.finishOutput(&particles);
     } catch (const std::exception& e) {
         fprintf(stderr, "polymps_gpu: %s\n", e.what());
         return 1;

@@ -514,12 +514,20 @@ public:
         printf(" RunTime: %02dh%02dm%02dsec\n", hours, minutes % 60, seconds % 60);
     }
     // MpsInputOutput::writeOutputFiles (src/MpsInputOutput.cpp:861-886):
-    // ASCII VTU of every particle (binary and free-surface-only variants are
-    // written as the ASCII full file).
+    // VTU, ASCII or (output_VTU.type = 1) binary, of every particle or
+    // (output_VTU.only_freeSurface) of the free-surface and wall particles
+    // (writeVtuAsciiFreeSurface, src/MpsInputOutput.cpp:1582-1875).  The file is
+    // written in the background from a snapshot of the current state, so the
+    // next steps run while it is formatted; finishOutput waits for it.
     void writeOutputFiles(MpsParticleSystem* PSystem, MpsParticle* Particles) {
         char name[4352];
         snprintf(name, sizeof name, "%s/output%05d.vtu", vtuOutputFoldername.c_str(), PSystem->fileNumber);
-        polymps_gpu::check(mps_write_vtu(Particles->ctx, name, vtuFlags & 31), Particles->ctx, "writeOutputFiles");
+        polymps_gpu::check(mps_write_vtu_async(Particles->ctx, name, vtuFlags),
+                           Particles->ctx, "writeOutputFiles");
+    }
+    // Waits for every VTU file still being written (reports write errors).
+    void finishOutput(MpsParticle* Particles) {
+        if (Particles->ctx) polymps_gpu::check(mps_output_wait(Particles->ctx), Particles->ctx, "writeOutputFiles");
     }
     // MpsInputOutput::writePvd (src/MpsInputOutput.cpp:1878-1935)
     void writePvd(MpsParticleSystem* PSystem, MpsParticle* Particles) {

@@ -64,6 +64,8 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     lib.mps_read_grid.argtypes = [cp, ctypes.POINTER(ip), vp, vp, vp, vp, vp]
     lib.mps_write_grid.argtypes = [cp, ip, vp, vp, vp, vp, vp]
     lib.mps_write_vtu.argtypes = [vp, cp, ip]
+    lib.mps_write_vtu_async.argtypes = [vp, cp, ip]
+    lib.mps_output_wait.argtypes = [vp]
     ull = ctypes.c_ulonglong
     lib.mps_gnn_param_count.argtypes = []
     lib.mps_gnn_init.argtypes = [vp, ull]
@@ -278,6 +280,13 @@ class Simulation:
 
     def write_vtu(self, path, flags: int = 1):
         self._check(self._lib.mps_write_vtu(self._h, str(path).encode(), flags))
+
+    def write_vtu_async(self, path, flags: int = 1):
+        """Snapshot now; the file is written in the background (output_wait)."""
+        self._check(self._lib.mps_write_vtu_async(self._h, str(path).encode(), flags))
+
+    def output_wait(self):
+        self._check(self._lib.mps_output_wait(self._h))
 
     def sync(self):
         self._check(self._lib.mps_sync(self._h))

@@ -10,9 +10,12 @@ import subprocess
 
 import pytest
 
+import numpy as np
+
 from polymps_amd import build as B
 from polymps_amd import configs
 from polymps_amd.lib import Simulation
+from polymps_amd.vtu import read_vtu
 
 pytestmark = pytest.mark.gpu
 
@@ -42,3 +45,145 @@ def test_driver_matches_library_steps(built, tmp_path, make, flags):
     g.write_vtu(tmp_path / "lib5.vtu", flags)
     assert (out / "output00000.vtu").read_bytes() == (tmp_path / "lib0.vtu").read_bytes()
     assert (out / "output00001.vtu").read_bytes() == (tmp_path / "lib5.vtu").read_bytes()
+
+
+def test_binary_vtu_matches_ascii(built, tmp_path):
+    """output_VTU.type = 1: the driver writes binary VTU files equal to the
+    library's binary writer after the same steps, and each array of the binary
+    file equals the ASCII file's (which prints six decimals)."""
+    w = configs.small_dam3d(l0=0.025, steps=5)
+    w.params.iter_output = 5
+    flags = 1 | 2 | 16
+    vtu = {"type": 1, "pnd": True, "neigh": True, "auxiliar": True}
+    path = configs.write_case(w, tmp_path, "case", vtu=vtu)
+    exe = B.DRIVER if B.DRIVER.exists() else B.build_driver()
+    print(f"driver binary case {w.name}", flush=True)
+    res = subprocess.run([str(exe), path.name], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr[-2000:]
+    g = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh, device=0)
+    g.call("init")
+    for _ in range(5):
+        g.call("step")
+    g.write_vtu(tmp_path / "lib5.vtu", flags | 64)
+    g.write_vtu(tmp_path / "lib5a.vtu", flags)
+    assert (tmp_path / "case_out" / "output00001.vtu").read_bytes() == (tmp_path / "lib5.vtu").read_bytes()
+    b = read_vtu(tmp_path / "lib5.vtu")
+    a = read_vtu(tmp_path / "lib5a.vtu")
+    assert set(a) == set(b)
+    n = int(g.scalar("num_particles"))
+    pos = g.get("pos").reshape(-1, 3)[:n]
+    np.testing.assert_array_equal(b["Position"], pos.astype(np.float32))
+    np.testing.assert_array_equal(b["pressure"], g.get("press")[:n].astype(np.float32))
+    for k in a:
+        assert a[k].shape == b[k].shape, k
+        if a[k].dtype.kind == "f":
+            np.testing.assert_allclose(b[k], a[k], rtol=1e-6, atol=6e-7, err_msg=k)
+        else:
+            np.testing.assert_array_equal(b[k], a[k], err_msg=k)
+    assert np.array_equal(b["connectivity"], np.arange(n)) and np.array_equal(b["offsets"], np.arange(1, n + 1))
+
+
+@pytest.mark.parametrize("flags", [1 | 2 | 4 | 16, 1 | 8 | 16 | 64])
+def test_async_vtu_equals_sync(built, tmp_path, flags):
+    """Background VTU writes, with several snapshots in flight while the
+    steps go on, give the same bytes as blocking writes of the same states."""
+    w = configs.small_dam3d(l0=0.025, steps=6)
+    print(f"async vtu flags {flags}", flush=True)
+    ga = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh, device=0)
+    gs = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh, device=0)
+    ga.call("init")
+    gs.call("init")
+    counts = []
+    for k in range(4):
+        ga.write_vtu_async(tmp_path / f"a{k}.vtu", flags)
+        gs.write_vtu(tmp_path / f"s{k}.vtu", flags)
+        counts.append(int(gs.scalar("num_particles")))
+        ga.call("step")
+        gs.call("step")
+    ga.output_wait()
+    for k in range(4):
+        assert (tmp_path / f"a{k}.vtu").read_bytes() == (tmp_path / f"s{k}.vtu").read_bytes(), k
+    last = read_vtu(tmp_path / "a3.vtu")
+    assert len(last["pressure"]) == counts[3]
+    ga.close()
+    gs.close()
+
+
+def _reference_ascii_vtu(g, flags, polygon, surface_id=1, wall_id=2):
+    """The text MpsInputOutput::writeVtuAscii (src/MpsInputOutput.cpp:1313-1579)
+    prints for the state of `g` or, with flag 32, writeVtuAsciiFreeSurface
+    (:1582-1875): free-surface BC or wall-type particles only, no
+    ParticleType, polygonNormal of every particle.  Built here from
+    downloaded fields."""
+    n = int(g.scalar("num_particles"))
+    fs = bool(flags & 32)
+    bc_all, type_all = g.get("bc")[:n], g.get("type")[:n]
+    sel = np.nonzero((bc_all == surface_id) | (type_all == wall_id))[0] if fs else np.arange(n)
+    m = len(sel)
+
+    def f32(field, comps, every=False):
+        v = g.get(field).reshape(-1, comps)[:n].astype(np.float32)
+        v = v if every else v[sel]
+        return "".join(("%f " * comps) % tuple(map(float, row)) for row in v)
+
+    def i32(field):
+        return "".join("%d " % v for v in g.get(field)[:n][sel])
+
+    def arr(typ, name, body, comps=1):
+        nc = " NumberOfComponents='3'" if comps == 3 else ""
+        return f"        <DataArray type='{typ}' Name='{name}'{nc} format='ascii'>\n{body}\n        </DataArray>\n"
+
+    pos = g.get("pos").reshape(-1, 3)[:n][sel]
+    out = ["<?xml version='1.0' encoding='UTF-8'?>\n",
+           "<VTKFile xmlns='VTK' byte_order='LittleEndian' version='0.1' type='UnstructuredGrid'>\n",
+           "  <UnstructuredGrid>\n", f"    <Piece NumberOfCells='{m}' NumberOfPoints='{m}'>\n", "      <Points>\n",
+           arr("Float32", "Position", "          " + "".join("%f %f %f " % tuple(r) for r in pos), 3),
+           "      </Points>\n", "      <PointData>\n",
+           arr("Float32", "Velocity", f32("vel", 3), 3), arr("Float32", "pressure", f32("press", 1)),
+           arr("Int32", "BC", i32("bc"))]
+    if flags & 1:
+        out += [arr("Float32", "pnd", f32("pndi", 1)), arr("Float32", "pndSmall", f32("pnd_small", 1))]
+    if flags & 2:
+        out += [arr("Int32", "nNeigh", i32("num_neigh"))]
+    if flags & 4:
+        out += [arr("Float32", "npcdDeviation", f32("npcd", 3), 3),
+                arr("Float32", "polygonNormal", f32("polygon_normal", 3, every=fs), 3)]
+    if flags & 8:
+        out += [arr("Float32", "concentration", f32("conc", 1)), arr("Float32", "GradCi", f32("grad_conc", 3), 3)]
+    if flags & 16:
+        if not fs:
+            out += [arr("Int32", "ParticleType", i32("type"))]
+        if polygon:
+            out += [arr("Int32", "nearMeshType", i32("near_mesh_type"))]
+    out += ["      </PointData>\n", "      <Cells>\n",
+            arr("Int32", "connectivity", "".join("%d " % i for i in range(m))),
+            arr("Int32", "offsets", "".join("%d " % (i + 1) for i in range(m))),
+            arr("UInt8", "types", "1 " * m),
+            "      </Cells>\n", "    </Piece>\n", "  </UnstructuredGrid>\n", "</VTKFile>\n"]
+    return "".join(out).encode()
+
+
+@pytest.mark.parametrize("make,flags,polygon", [
+    (lambda: configs.small_dam3d(l0=0.025, steps=3), 1 | 2 | 4 | 8 | 16, True),
+    (lambda: configs.small_dam2d(l0=0.02, steps=3), 1 | 2 | 4 | 8 | 16, False),
+    (lambda: configs.small_dam3d(l0=0.025, steps=3), 1 | 2 | 4 | 8 | 16 | 32, True),
+    (lambda: configs.small_dam2d(l0=0.02, steps=3), 1 | 4 | 16 | 32, False),
+])
+def test_ascii_vtu_matches_reference_format(built, tmp_path, make, flags, polygon):
+    w = make()
+    print(f"ascii vtu format {w.name}", flush=True)
+    g = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh, device=0)
+    g.call("init")
+    for _ in range(3):
+        g.call("step")
+    g.write_vtu(tmp_path / "o.vtu", flags)
+    expect = _reference_ascii_vtu(g, flags, polygon, w.params.surface, w.params.wall)
+    assert (tmp_path / "o.vtu").read_bytes() == expect
+    if flags & 32:
+        # the binary free-surface file holds the same particles
+        g.write_vtu(tmp_path / "b.vtu", flags | 64)
+        b, a = read_vtu(tmp_path / "b.vtu"), read_vtu(tmp_path / "o.vtu")
+        assert len(b["pressure"]) == len(a["pressure"]) > 0
+        np.testing.assert_array_equal(b["BC"], a["BC"])
+        np.testing.assert_allclose(b["Velocity"], a["Velocity"], rtol=1e-6, atol=6e-7)
+    g.close()
