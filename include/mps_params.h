@@ -41,7 +41,7 @@ typedef struct mps_params {
     /* ---- JSON "domain" ---- */
     double domain_min[3];       /* min.x/y/z (extended by 3 l0 in derive) */
     double domain_max[3];
-    int domain_type_bc;         /* boundary.type (0 only)             */
+    int domain_type_bc;         /* boundary.type (0 none, 1 periodic) */
     int limit_type_bc;          /* boundary.limit                     */
     int num_bc;                 /* 1 when "boundary" present          */
     int periodic_dir[3];
@@ -129,6 +129,8 @@ typedef struct mps_params {
     double beta_pnd, beta_neigh, delta2;
     double theta_arc, h_threshold2, dst_threshold2, norm_threshold2;
     double bucket_side, inv_bucket_side;
+    double periodic_length[3];  /* periodicLength (0 where not periodic) */
+    double phys_min[3], phys_max[3];   /* physDomMin/Max: the periodic wrap limits */
     int num_buckets_x, num_buckets_y, num_buckets_z;
     int num_buckets_xy, num_buckets_xyz;
     int num_of_meshs;

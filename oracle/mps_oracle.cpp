@@ -17,7 +17,8 @@
  *    indexes them linearly (wrapping to a far row, or reading before the
  *    array).  Wrapped cells lie more than reL away, so results agree whenever
  *    the reference stays inside its arrays.
- *  - periodic domains are not supported (periodic length is 0 everywhere).
+ *  - periodic domains: one periodic direction with particle walls, the
+ *    setting MpsBucket::calcDomainLimits handles (src/MpsBucket.cpp:141-287).
  *  - Eigen's CG is restated from its published algorithm (Jacobi
  *    preconditioner, relative residual test on the recursive residual,
  *    default tol = DBL_EPSILON, default max iterations 2N) with sequential
@@ -72,6 +73,7 @@ struct Sys {
     double solverErr = 0.0;
     // buckets
     std::vector<int> first, last, next;
+    std::vector<int> bucketPBC;   // bucketPeriodicBC: -1 / +1 on the ghost layers, 0 elsewhere
     Mesh mesh;
     bool hasMesh = false;
     double velMax = 0.0, cflCurrent = 0.0, timeCurrent = 0.0;
@@ -113,6 +115,12 @@ inline void bucketCoords(const Sys& s, double x, double y, double z, int& bx, in
     bz = (int)((z - s.p.domain_min[2]) * s.p.inv_bucket_side + kEps);
 }
 
+// Periodic shift of the bucket being walked (MpsParticle::getPeriodicLengths,
+// src/MpsParticle.cpp:48-54): every neighbour position inside a walk is
+// pos_j + pl, as sqrDistBetweenParticles (src/MpsParticle.cpp:35-45) forms it.
+thread_local double tlPl[3] = {0.0, 0.0, 0.0};
+inline double PJ(const Sys& s, int j, int k) { return s.pos[j * 3 + k] + tlPl[k]; }
+
 template <class Fn>
 inline void walk(const Sys& s, double xi, double yi, double zi, Fn&& fn) {
     int ix, iy, iz;
@@ -127,6 +135,8 @@ inline void walk(const Sys& s, double xi, double yi, double zi, Fn&& fn) {
             for (int jx = ix - 1; jx <= ix + 1; jx++) {
                 if (jx < 0 || jx >= nx) continue;
                 const int jb = jz * s.p.num_buckets_xy + jy * nx + jx;
+                const int bPBC = s.bucketPBC.empty() ? 0 : s.bucketPBC[jb];
+                for (int k = 0; k < 3; k++) tlPl[k] = bPBC * s.p.periodic_length[k];
                 int j = s.first[jb];
                 while (j != -1) {
                     fn(j);
@@ -135,6 +145,7 @@ inline void walk(const Sys& s, double xi, double yi, double zi, Fn&& fn) {
             }
         }
     }
+    tlPl[0] = tlPl[1] = tlPl[2] = 0.0;
 }
 
 inline bool partWall(const Sys& s) { return s.p.wall_type == MPS_WALL_PARTICLE; }
@@ -249,9 +260,20 @@ int deriveParams(Sys& s) {
     P.num_buckets_y = (int)((P.domain_max[1] - P.domain_min[1]) * P.inv_bucket_side) + 3;
     P.num_buckets_z = (int)((P.domain_max[2] - P.domain_min[2]) * P.inv_bucket_side) + 3;
     if ((int)P.dim == 2) P.num_buckets_z = 1;
-    if (P.domain_type_bc != 0) { s.err = "periodic domains are not supported"; return -1; }
+    const bool periodic = P.domain_type_bc == 1;
+    int pdir = -1;
+    for (int k = 0; k < 3; k++) P.periodic_length[k] = 0.0;
+    if (periodic) {
+        int nper = 0;
+        for (int k = 0; k < 3; k++) if (P.periodic_dir[k]) { nper++; pdir = k; }
+        if (nper != 1 || P.wall_type != MPS_WALL_PARTICLE) { s.err = "periodic: one direction, particle walls"; return -1; }
+    } else if (P.domain_type_bc != 0) {
+        s.err = "unknown domain boundary type";
+        return -1;
+    }
     if (P.wall_type == MPS_WALL_PARTICLE) {
-        // calcDomainLimits, limitTypeBC and non-periodic branch
+        // calcDomainLimits (src/MpsBucket.cpp:71-295): limitTypeBC, then the
+        // non-periodic or the periodic branch
         double lim[3][2];
         if (P.limit_type_bc == 0) {
             for (int k = 0; k < 3; k++) lim[k][0] = lim[k][1] = s.pos[k];
@@ -266,23 +288,64 @@ int deriveParams(Sys& s) {
         } else {
             for (int k = 0; k < 3; k++) { lim[k][0] = P.domain_min[k]; lim[k][1] = P.domain_max[k]; }
         }
-        lim[0][0] -= 0.5 * l0; lim[0][1] += 0.5 * l0;
-        lim[1][0] -= 0.5 * l0; lim[1][1] += 0.5 * l0;
-        if (P.dim == 3) { lim[2][0] -= 0.5 * l0; lim[2][1] += 0.5 * l0; }
-        for (int k = 0; k < (int)P.dim; k++) {
-            lim[k][0] -= P.bucket_side;
-            lim[k][1] += P.bucket_side;
-            long nb = (long)((lim[k][1] - lim[k][0]) / P.bucket_side + 1);
+        auto setNb = [&](int k, long nb) {
             if (k == 0) P.num_buckets_x = (int)nb;
             if (k == 1) P.num_buckets_y = (int)nb;
             if (k == 2) P.num_buckets_z = (int)nb;
-            lim[k][1] = lim[k][0] + nb * P.bucket_side;
+        };
+        if (!periodic) {
+            lim[0][0] -= 0.5 * l0; lim[0][1] += 0.5 * l0;
+            lim[1][0] -= 0.5 * l0; lim[1][1] += 0.5 * l0;
+            if (P.dim == 3) { lim[2][0] -= 0.5 * l0; lim[2][1] += 0.5 * l0; }
+            for (int k = 0; k < (int)P.dim; k++) {
+                lim[k][0] -= P.bucket_side;
+                lim[k][1] += P.bucket_side;
+                long nb = (long)((lim[k][1] - lim[k][0]) / P.bucket_side + 1);
+                setNb(k, nb);
+                lim[k][1] = lim[k][0] + nb * P.bucket_side;
+            }
+        } else {
+            // e.g. periodic in x, src/MpsBucket.cpp:148-190
+            const int d = pdir;
+            P.periodic_length[d] = lim[d][1] - lim[d][0] + P.part_dist;
+            lim[d][0] -= P.part_dist * 0.5;
+            lim[d][1] += P.part_dist * 0.5;
+            P.phys_min[d] = lim[d][0];
+            P.phys_max[d] = lim[d][1];
+            long nbd = (long)((lim[d][1] - lim[d][0]) / P.bucket_side);
+            P.bucket_side = P.periodic_length[d] / (long)(P.periodic_length[d] / P.bucket_side);
+            lim[d][0] -= P.bucket_side;
+            lim[d][1] += P.bucket_side;
+            nbd += 2;
+            setNb(d, nbd);
+            for (int k = 0; k < 3; k++) {
+                if (k == d || (k == 2 && P.dim != 3)) continue;
+                lim[k][0] -= P.bucket_side;
+                lim[k][1] += P.bucket_side;
+                P.phys_min[k] = lim[k][0];
+                P.phys_max[k] = lim[k][1];
+                long nb = (long)((lim[k][1] - lim[k][0]) / P.bucket_side + 1);
+                setNb(k, nb);
+                lim[k][1] = lim[k][0] + nb * P.bucket_side;
+            }
         }
         for (int k = 0; k < 3; k++) { P.domain_min[k] = lim[k][0]; P.domain_max[k] = lim[k][1]; }
         P.inv_bucket_side = 1.0 / P.bucket_side;
     }
     P.num_buckets_xy = P.num_buckets_x * P.num_buckets_y;
     P.num_buckets_xyz = P.num_buckets_xy * P.num_buckets_z;
+    // MpsBoundaryCondition::setBucketBC (src/MpsBoundaryCondition.cpp:20-63)
+    s.bucketPBC.assign(P.num_buckets_xyz, 0);
+    if (periodic)
+        for (int iz = 0; iz < P.num_buckets_z; iz++)
+            for (int iy = 0; iy < P.num_buckets_y; iy++)
+                for (int ix = 0; ix < P.num_buckets_x; ix++) {
+                    const int ib = iz * P.num_buckets_xy + iy * P.num_buckets_x + ix;
+                    const int c[3] = {ix, iy, iz};
+                    const int n[3] = {P.num_buckets_x, P.num_buckets_y, P.num_buckets_z};
+                    if (c[pdir] == 0) s.bucketPBC[ib] = -1;
+                    if (c[pdir] == n[pdir] - 1) s.bucketPBC[ib] = 1;
+                }
     P.derived = 1;
     return 0;
 }
@@ -312,6 +375,18 @@ void updateParticlesID(Sys& s) {
         s.last[ib] = i;
         if (j == -1) s.first[ib] = i; else s.next[j] = i;
     }
+    // MpsBucket::copyDataBetweenBuckets (src/MpsBucket.cpp:356-417): the ghost
+    // layer of the periodic direction lists the particles of the opposite
+    // interior layer
+    if (P.domain_type_bc == 1) {
+        const int off = P.periodic_dir[0] ? P.num_buckets_x - 2
+                      : P.periodic_dir[1] ? P.num_buckets_x * (P.num_buckets_y - 2)
+                                          : P.num_buckets_xy * (P.num_buckets_z - 2);
+        for (int ib = 0; ib < P.num_buckets_xyz; ib++) {
+            if (s.bucketPBC[ib] == -1) { s.first[ib] = s.first[ib + off]; s.last[ib] = s.last[ib + off]; }
+            if (s.bucketPBC[ib] == 1) { s.first[ib] = s.first[ib - off]; s.last[ib] = s.last[ib - off]; }
+        }
+    }
 }
 
 // --------------------------------------------------------- MpsViscosity
@@ -327,9 +402,9 @@ void calcViscosity(Sys& s) {
         const double ux = s.vel[i * 3], uy = s.vel[i * 3 + 1], uz = s.vel[i * 3 + 2];
         const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_l2 && (d2 < dm2 || pw)) {
                 if (j != i) {
@@ -390,9 +465,9 @@ void calcWallViscosity(Sys& s) {
         }
         const double dns_i = (s.ptype[i] == 1) ? P.dns_fl1 : P.dns_fl2;
         walk(s, xi[0], xi[1], xi[2], [&](int j) {
-            double v0 = s.pos[j * 3] - xi[0], v1 = s.pos[j * 3 + 1] - xi[1], v2 = s.pos[j * 3 + 2] - xi[2];
+            double v0 = PJ(s, j, 0) - xi[0], v1 = PJ(s, j, 1) - xi[1], v2 = PJ(s, j, 2) - xi[2];
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - xm[0], w1 = s.pos[j * 3 + 1] - xm[1], w2 = s.pos[j * 3 + 2] - xm[2];
+            double w0 = PJ(s, j, 0) - xm[0], w1 = PJ(s, j, 1) - xm[1], w2 = PJ(s, j, 2) - xm[2];
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_l2 && dm2 < P.re_l2 && d2 < dm2) {
                 if (j != i) {
@@ -498,7 +573,7 @@ void wallDummyVelocity(Sys& s) {
         const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
         double dx = 0, dy = 0, dz = 0, ni = 0;
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
             if (d2 < P.re_l2 && j != i && s.type[j] == P.fluid) {
                 double wL = weight(sqrt(d2), P.re_l, P.weight_type);
@@ -520,9 +595,17 @@ void wallDummyVelocity(Sys& s) {
     }
 }
 
-// MpsParticle::checkParticleOutDomain (src/MpsParticle.cpp:234-402), non-periodic part
+// MpsParticle::checkParticleOutDomain (src/MpsParticle.cpp:234-402): the
+// periodic wrap (:236-269), then the removal of fluid outside the domain
 void checkOutDomain(Sys& s) {
     const mps_params& P = s.p;
+    if (P.domain_type_bc == 1)
+        for (int i = 0; i < s.N; i++)
+            for (int k = 0; k < 3; k++) {
+                if (!P.periodic_dir[k]) continue;
+                if (s.pos[i * 3 + k] > P.phys_max[k]) s.pos[i * 3 + k] -= P.periodic_length[k];
+                if (s.pos[i * 3 + k] < P.phys_min[k]) s.pos[i * 3 + k] += P.periodic_length[k];
+            }
     int newN = s.N;
     double lo[3], hi[3];
     for (int k = 0; k < 3; k++) {
@@ -587,9 +670,9 @@ void collisionPC(Sys& s) {
         const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
         double dux = 0, duy = 0, duz = 0;
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.dist_collision_limit2 && (d2 < dm2 || pw) && j != i) {
                 double fDT = (ux - s.vel[j * 3]) * v0 + (uy - s.vel[j * 3 + 1]) * v1 + (uz - s.vel[j * 3 + 2]) * v2;
@@ -631,9 +714,9 @@ void collisionDPC(Sys& s) {
         const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
         double dux = 0, duy = 0, duz = 0;
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.part_dist * P.part_dist && (d2 < dm2 || pw) && j != i) {
                 double mj;
@@ -859,9 +942,9 @@ void initialPnd(Sys& s) {
         s.numNeigh[i] = 0;
         s.npcd[i * 3] = s.npcd[i * 3 + 1] = s.npcd[i * 3 + 2] = 0.0;
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_l2 && (d2 < dm2 || pw) && j != i) {
                 s.numNeigh[i] += 1;
@@ -905,9 +988,9 @@ void wallNPCD(Sys& s) {
         double n[3], R[9];
         reflection(xi, xm, n, R, nullptr);
         walk(s, xi[0], xi[1], xi[2], [&](int j) {
-            double v0 = s.pos[j * 3] - xi[0], v1 = s.pos[j * 3 + 1] - xi[1], v2 = s.pos[j * 3 + 2] - xi[2];
+            double v0 = PJ(s, j, 0) - xi[0], v1 = PJ(s, j, 1) - xi[1], v2 = PJ(s, j, 2) - xi[2];
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - xm[0], w1 = s.pos[j * 3 + 1] - xm[1], w2 = s.pos[j * 3 + 2] - xm[2];
+            double w0 = PJ(s, j, 0) - xm[0], w1 = PJ(s, j, 1) - xm[1], w2 = PJ(s, j, 2) - xm[2];
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2 && j != i) {
                 double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
@@ -938,9 +1021,9 @@ void pndNeighNPCD(Sys& s) {
         int nn = s.numNeighWall[i];
         double c0 = s.npcd[i * 3], c1 = s.npcd[i * 3 + 1], c2 = s.npcd[i * 3 + 2];
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_l2 && (d2 < dm2 || pw) && j != i) {
                 nn += 1;
@@ -983,9 +1066,9 @@ void pndDiffusive(Sys& s) {
         if (ni < kEps) continue;
         double Pi = s.press[i];
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_l2 && (d2 < dm2 || pw) && j != i) {
                 double dst = sqrt(d2);
@@ -1047,9 +1130,9 @@ void wallPndDiffusive(Sys& s) {
             umz = 0.0 * vt0 + 0.0 * vt1 + -1.0 * vt2;
         }
         walk(s, xi[0], xi[1], xi[2], [&](int j) {
-            double v0 = s.pos[j * 3] - xi[0], v1 = s.pos[j * 3 + 1] - xi[1], v2 = s.pos[j * 3 + 2] - xi[2];
+            double v0 = PJ(s, j, 0) - xi[0], v1 = PJ(s, j, 1) - xi[1], v2 = PJ(s, j, 2) - xi[2];
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - xm[0], w1 = s.pos[j * 3 + 1] - xm[1], w2 = s.pos[j * 3 + 2] - xm[2];
+            double w0 = PJ(s, j, 0) - xm[0], w1 = PJ(s, j, 1) - xm[1], w2 = PJ(s, j, 2) - xm[2];
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2 && j != i) {
                 double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
@@ -1104,9 +1187,9 @@ void meanPndWallDummySurface(Sys& s) {
         const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
         const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) {
                 double wS = weight(sqrt(d2), P.re_s, P.weight_type);
@@ -1135,9 +1218,9 @@ void meanPnd(Sys& s) {
         const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
         const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) {
                 double wS = weight(sqrt(d2), P.re_s, P.weight_type);
@@ -1166,9 +1249,9 @@ void meanWallPnd(Sys& s) {
         const double* xi = &s.pos[i * 3];
         const double* xm = &s.mirror[i * 3];
         walk(s, xi[0], xi[1], xi[2], [&](int j) {
-            double v0 = s.pos[j * 3] - xi[0], v1 = s.pos[j * 3 + 1] - xi[1], v2 = s.pos[j * 3 + 2] - xi[2];
+            double v0 = PJ(s, j, 0) - xi[0], v1 = PJ(s, j, 1) - xi[1], v2 = PJ(s, j, 2) - xi[2];
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - xm[0], w1 = s.pos[j * 3 + 1] - xm[1], w2 = s.pos[j * 3 + 2] - xm[2];
+            double w0 = PJ(s, j, 0) - xm[0], w1 = PJ(s, j, 1) - xm[1], w2 = PJ(s, j, 2) - xm[2];
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2 && j != i) {
                 double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
@@ -1197,9 +1280,9 @@ void meanNeighFluidPnd(Sys& s) {
         const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
         const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i && s.type[j] == P.fluid) {
                 double wS = weight(sqrt(d2), P.re_s, P.weight_type);
@@ -1269,9 +1352,9 @@ void correctionMatrix(Sys& s) {
         const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
         const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) {
                 double wS = weight(sqrt(d2), P.re_s, P.weight_type);
@@ -1323,9 +1406,9 @@ void velDivergence(Sys& s) {
         const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
         const double* MC0 = &s.cm1[i * 3]; const double* MC1 = &s.cm2[i * 3]; const double* MC2 = &s.cm3[i * 3];
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) {
                 double vx = s.vel[j * 3] - ux, vy = s.vel[j * 3 + 1] - uy, vz = s.vel[j * 3 + 2] - uz;
@@ -1367,9 +1450,9 @@ void assemblePPE(Sys& s) {
         double ni = s.pndSmall[i];
         double sum = 0.0;
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_l2 && (d2 < dm2 || pw) && j != i) {
                 double wL = weight(sqrt(d2), P.re_l, P.weight_type);
@@ -1507,7 +1590,7 @@ void extrapolateWallDummy(Sys& s) {
         const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
         double ni = 0.0, pr = 0.0;
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
             if (d2 < P.re_s2 && j != i && s.type[j] == P.fluid) {
                 double wS = weight(sqrt(d2), P.re_s, P.weight_type);
@@ -1534,9 +1617,9 @@ void pressGradient(Sys& s) {
         double pmin = s.press[i], Pi = s.press[i], ni = s.pndi[i];
         const double* MC0 = &s.cm1[i * 3]; const double* MC1 = &s.cm2[i * 3]; const double* MC2 = &s.cm3[i * 3];
         auto ok = [&](int j, double& v0, double& v1, double& v2, double& d2) {
-            v0 = s.pos[j * 3] - xi; v1 = s.pos[j * 3 + 1] - yi; v2 = s.pos[j * 3 + 2] - zi;
+            v0 = PJ(s, j, 0) - xi; v1 = PJ(s, j, 1) - yi; v2 = PJ(s, j, 2) - zi;
             d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             return d2 < P.re_s2 && (d2 < dm2 || pw) && j != i;
         };
@@ -1609,9 +1692,9 @@ void wallPressGradient(Sys& s) {
         reflection(xi, xm, n, R, &ns);
         if (gt == 0 || gt == 2)
             walk(s, xi[0], xi[1], xi[2], [&](int j) {
-                double v0 = s.pos[j * 3] - xi[0], v1 = s.pos[j * 3 + 1] - xi[1], v2 = s.pos[j * 3 + 2] - xi[2];
+                double v0 = PJ(s, j, 0) - xi[0], v1 = PJ(s, j, 1) - xi[1], v2 = PJ(s, j, 2) - xi[2];
                 double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-                double w0 = s.pos[j * 3] - xm[0], w1 = s.pos[j * 3 + 1] - xm[1], w2 = s.pos[j * 3 + 2] - xm[2];
+                double w0 = PJ(s, j, 0) - xm[0], w1 = PJ(s, j, 1) - xm[1], w2 = PJ(s, j, 2) - xm[2];
                 double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
                 if (d2 < P.re_s2 && d2 < dm2 && j != i)
                     if (pmin > s.press[j]) pmin = s.press[j];
@@ -1627,9 +1710,9 @@ void wallPressGradient(Sys& s) {
             ax += w0 * wS; ay += w1 * wS; az += w2 * wS;
         };
         walk(s, xi[0], xi[1], xi[2], [&](int j) {
-            double v0 = s.pos[j * 3] - xi[0], v1 = s.pos[j * 3 + 1] - xi[1], v2 = s.pos[j * 3 + 2] - xi[2];
+            double v0 = PJ(s, j, 0) - xi[0], v1 = PJ(s, j, 1) - xi[1], v2 = PJ(s, j, 2) - xi[2];
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - xm[0], w1 = s.pos[j * 3 + 1] - xm[1], w2 = s.pos[j * 3 + 2] - xm[2];
+            double w0 = PJ(s, j, 0) - xm[0], w1 = PJ(s, j, 1) - xm[1], w2 = PJ(s, j, 2) - xm[2];
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2 && j != i) term(s.press[j], s.pndi[j], w0, w1, w2, dm2);
         });
@@ -1668,9 +1751,9 @@ void shifting1(Sys& s) {
         const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
         double dx = 0, dy = 0, dz = 0;
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) {
                 double dw = delWeight(sqrt(d2), P.re_s, P.weight_type);
@@ -1705,9 +1788,9 @@ void shifting2(Sys& s) {
         const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
         const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) c += weight(sqrt(d2), P.re_s, P.weight_type);
         });
@@ -1722,9 +1805,9 @@ void shifting2(Sys& s) {
         const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
         double ci = s.conc[i];
         walk(s, xi, yi, zi, [&](int j) {
-            double v0 = s.pos[j * 3] - xi, v1 = s.pos[j * 3 + 1] - yi, v2 = s.pos[j * 3 + 2] - zi;
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - mx, w1 = s.pos[j * 3 + 1] - my, w2 = s.pos[j * 3 + 2] - mz;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) {
                 double wS = weight(sqrt(d2), P.re_s, P.weight_type);
@@ -1759,9 +1842,9 @@ void shifting2(Sys& s) {
         reflection(xi, xm, n, R, nullptr);
         double x0 = xi[0], x1 = xi[1], x2 = xi[2];
         walk(s, x0, x1, x2, [&](int j) {
-            double v0 = s.pos[j * 3] - x0, v1 = s.pos[j * 3 + 1] - x1, v2 = s.pos[j * 3 + 2] - x2;
+            double v0 = PJ(s, j, 0) - x0, v1 = PJ(s, j, 1) - x1, v2 = PJ(s, j, 2) - x2;
             double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            double w0 = s.pos[j * 3] - xm[0], w1 = s.pos[j * 3 + 1] - xm[1], w2 = s.pos[j * 3 + 2] - xm[2];
+            double w0 = PJ(s, j, 0) - xm[0], w1 = PJ(s, j, 1) - xm[1], w2 = PJ(s, j, 2) - xm[2];
             double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2 && j != i) {
                 double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
@@ -2031,9 +2114,9 @@ int orc_neighbors(void* h, int i, double r2, int mode, int* out, int cap) {
     const double mx = s->mirror[i * 3], my = s->mirror[i * 3 + 1], mz = s->mirror[i * 3 + 2];
     walk(*s, xi, yi, zi, [&](int j) {
         if (j == i) return;
-        double v0 = s->pos[j * 3] - xi, v1 = s->pos[j * 3 + 1] - yi, v2 = s->pos[j * 3 + 2] - zi;
+        double v0 = PJ(*s, j, 0) - xi, v1 = PJ(*s, j, 1) - yi, v2 = PJ(*s, j, 2) - zi;
         double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-        double w0 = s->pos[j * 3] - mx, w1 = s->pos[j * 3 + 1] - my, w2 = s->pos[j * 3 + 2] - mz;
+        double w0 = PJ(*s, j, 0) - mx, w1 = PJ(*s, j, 1) - my, w2 = PJ(*s, j, 2) - mz;
         double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
         bool ok = d2 < r2 && (mode == 1 || d2 < dm2 || pw);
         if (ok) { if (cnt < cap) out[cnt] = j; cnt++; }

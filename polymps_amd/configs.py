@@ -227,6 +227,96 @@ def dam2d_wc(l0: float = 0.005, tank=(1.0, 0.6), column=(0.25, 0.5), seed: int =
                     info={"fluid": nf, "walls": n - nf, "seed": seed})
 
 
+def periodic_channel2d(l0: float = 0.01, length: float = 0.4, depth: float = 0.06, u0: float = 2.0, seed: int = 3,
+                       jitter: float = 1e-4, steps: int = 200, dt: float = 1e-4, mps_type: int = 1) -> Workload:
+    """2-D open channel, periodic in x (the reference's domain.boundary.type 1
+    with direction x), a floor of 1 wall + 2 dummy particle layers, fluid of
+    the given depth moving at u0 along x so that particles cross the seam.
+    mps_type 1: weakly compressible; 2: incompressible PPE."""
+    nx, cy = int(round(length / l0)), int(round(depth / l0))
+    ix, iy = np.meshgrid(np.arange(nx), np.arange(cy), indexing="ij")
+    fluid = np.stack([(ix.ravel(order="F") + 0.5) * l0, (iy.ravel(order="F") + 0.5) * l0], axis=1)
+    nf = len(fluid)
+    u = splitmix64_uniform(seed, 2 * nf).reshape(nf, 2)
+    fluid = fluid + (2.0 * u - 1.0) * jitter * l0
+    walls, wtypes = [], []
+    for layer in range(3):
+        for k in range(nx):
+            walls.append(((k + 0.5) * l0, -(layer + 0.5) * l0))
+            wtypes.append(WALL if layer == 0 else DUMMY)
+    walls = np.array(walls)
+    n = nf + len(walls)
+    pos = np.zeros((n, 3))
+    pos[:nf, :2] = fluid
+    pos[nf:, :2] = walls
+    types = np.concatenate([np.full(nf, FLUID, dtype=np.int32), np.array(wtypes, dtype=np.int32)])
+    vel = np.zeros((n, 3))
+    vel[:nf, 0] = u0
+    w = dam2d_wc(l0=l0, steps=steps, dt=dt)
+    p = w.params
+    p.domain_type_bc = 1
+    p.limit_type_bc = 0
+    p.num_bc = 1
+    p.periodic_dir[0], p.periodic_dir[1], p.periodic_dir[2] = 1, 0, 0
+    p.domain_min[0], p.domain_min[1] = 0.0, -3 * l0
+    p.domain_max[0], p.domain_max[1] = length, 2 * depth
+    p.mps_type = mps_type
+    if mps_type != 1:
+        p.pnd_type = 0
+        p.shifting_type = 0
+        p.re_l = 3.1
+        p.gradient_type = 2
+        p.gradient_correction = 1
+        p.free_surf_type = 1
+    return Workload(f"periodic_channel2d_l0_{l0:g}_mps{mps_type}", p, types, pos.reshape(-1), vel.reshape(-1),
+                    mesh=None, steps=steps, info={"fluid": nf, "walls": n - nf, "seed": seed, "u0": u0})
+
+
+def periodic_channel3d(l0: float = 0.02, size=(0.3, 0.4), depth: float = 0.1, u0: float = 1.0, seed: int = 4,
+                       jitter: float = 1e-3, steps: int = 100, dt: float = 2e-4) -> Workload:
+    """3-D incompressible channel, periodic in y, with a floor and two side
+    walls of 1 wall + 2 dummy particle layers; the fluid moves at u0 along y."""
+    nx, ny, cz = int(round(size[0] / l0)), int(round(size[1] / l0)), int(round(depth / l0))
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(cz), indexing="ij")
+    fluid = np.stack([(ix.ravel(order="F") + 0.5) * l0, (iy.ravel(order="F") + 0.5) * l0,
+                      (iz.ravel(order="F") + 0.5) * l0], axis=1)
+    nf = len(fluid)
+    u = splitmix64_uniform(seed, 3 * nf).reshape(nf, 3)
+    fluid = fluid + (2.0 * u - 1.0) * jitter * l0
+    walls, wtypes = [], []
+    hz = int(round(1.6 * depth / l0))
+    for layer in range(3):
+        t = WALL if layer == 0 else DUMMY
+        off = (layer + 0.5) * l0
+        for k in range(ny):
+            y = (k + 0.5) * l0
+            # floor row of this layer, closing the corners under the side walls
+            for m in range(-(layer + 1), nx + layer + 1):
+                walls.append(((m + 0.5) * l0, y, -off)); wtypes.append(t)
+            for m in range(hz):
+                walls.append((-off, y, (m + 0.5) * l0)); wtypes.append(t)
+                walls.append((size[0] + off, y, (m + 0.5) * l0)); wtypes.append(t)
+    walls = np.array(walls)
+    pos = np.concatenate([fluid, walls])
+    n = len(pos)
+    types = np.concatenate([np.full(nf, FLUID, dtype=np.int32), np.array(wtypes, dtype=np.int32)])
+    vel = np.zeros((n, 3))
+    vel[:nf, 1] = u0
+    p = small_dam3d(l0=0.025, steps=steps).params
+    p.part_dist = l0
+    p.wall_type = 0
+    p.time_step = dt
+    p.time_simulation = steps * dt
+    p.domain_type_bc = 1
+    p.limit_type_bc = 0
+    p.num_bc = 1
+    p.periodic_dir[0], p.periodic_dir[1], p.periodic_dir[2] = 0, 1, 0
+    p.domain_min[0], p.domain_min[1], p.domain_min[2] = -3 * l0, 0.0, -3 * l0
+    p.domain_max[0], p.domain_max[1], p.domain_max[2] = size[0] + 3 * l0, size[1], 1.6 * depth
+    return Workload(f"periodic_channel3d_l0_{l0:g}", p, types, pos.reshape(-1), vel.reshape(-1), mesh=None,
+                    steps=steps, info={"fluid": nf, "walls": n - nf, "seed": seed, "u0": u0})
+
+
 def _lattice_values(seed: int, ix, iy, octave: int):
     """Seeded uniform [0, 1) value per integer lattice point (splitmix64 of a hash)."""
     key = (np.asarray(ix, dtype=np.int64) * 73856093) ^ (np.asarray(iy, dtype=np.int64) * 19349663) ^ (octave * 83492791)
@@ -327,7 +417,9 @@ def case_json(p: MpsParams, grid: str, mesh: str, vtu_folder: str, vtu: dict | N
                                      **(vtu or {}))},
         "pathNames": {"particle_grid_file": grid, "mesh_rigid_file": mesh, "vtu_output_folder": vtu_folder},
         "domain": {"min": {a: float(p.domain_min[k]) for k, a in enumerate(ax)},
-                   "max": {a: float(p.domain_max[k]) for k, a in enumerate(ax)}},
+                   "max": {a: float(p.domain_max[k]) for k, a in enumerate(ax)},
+                   "boundary": {"type": int(p.domain_type_bc), "limit": int(p.limit_type_bc),
+                                "direction": {a: bool(p.periodic_dir[k]) for k, a in enumerate(ax)}}},
         "physical": {"fluid_density": p.density_fluid, "wall_density": p.density_wall,
                      "kinematic_visc": p.knm_vs1, "fluid_type": int(p.fluid_type),
                      "gravity": {a: float(p.gravity[k]) for k, a in enumerate(ax)},

@@ -13,6 +13,10 @@ namespace mpsgpu {
 
 #define KEPS (10.0 * 2.220446049250313e-16)
 
+// slot bits of a neighbour-list entry (periodic domains keep the ghost shift
+// in the two bits above)
+constexpr unsigned kListSlotMask = 0x3fffffffu;
+
 // src/MpsParticle.cpp:57-72 (weight; weightGradient has the same forms)
 __device__ __forceinline__ double wgt(double dst, double re, int t) {
     switch (t) {
@@ -77,12 +81,61 @@ __device__ __forceinline__ void walkCells(const KArgs& a, double x, double y, do
     }
 }
 
+// Periodic domains (one periodic axis d; MpsBoundaryCondition::setBucketBC,
+// src/MpsBoundaryCondition.cpp:20-63): bucket layers 0 and n-1 of axis d are
+// ghosts that list the particles of layers n-2 and 1
+// (MpsBucket::copyDataBetweenBuckets, src/MpsBucket.cpp:356-417), seen
+// shifted by bPBC * periodicLength with bPBC = -1 and +1
+// (MpsParticle::getPeriodicLengths, src/MpsParticle.cpp:48-54).  The walk
+// visits f(j, xj + plx, yj + ply, zj + plz, bPBC) in the reference's order,
+// adding pl (0.0 outside the ghosts) as sqrDistBetweenParticles does
+// (src/MpsParticle.cpp:35-45); slot i itself is skipped.
+__device__ __forceinline__ bool periodicDomain(const mps_params& P) { return P.domain_type_bc == 1; }
+
+template <class F>
+__device__ void walkCellsPeriodic(const KArgs& a, int i, double x, double y, double z, F&& f) {
+    const mps_params& P = a.P;
+    const int c[3] = {(int)((x - P.domain_min[0]) * P.inv_bucket_side + KEPS),
+                      (int)((y - P.domain_min[1]) * P.inv_bucket_side + KEPS),
+                      (int)((z - P.domain_min[2]) * P.inv_bucket_side + KEPS)};
+    const int n[3] = {P.num_buckets_x, P.num_buckets_y, P.num_buckets_z};
+    const int d = P.periodic_dir[0] ? 0 : (P.periodic_dir[1] ? 1 : 2);
+    const int d3 = (int)(P.dim - 2.0);
+    const double* __restrict__ X = a.D[DX];
+    const double* __restrict__ Y = a.D[DY];
+    const double* __restrict__ Z = a.D[DZ];
+    for (int jz = (c[2] - 1) * d3; jz <= (c[2] + 1) * d3; jz++) {
+        if (jz < 0 || jz >= n[2]) continue;
+        for (int jy = c[1] - 1; jy <= c[1] + 1; jy++) {
+            if (jy < 0 || jy >= n[1]) continue;
+            for (int jx = c[0] - 1; jx <= c[0] + 1; jx++) {
+                if (jx < 0 || jx >= n[0]) continue;
+                int b[3] = {jx, jy, jz};
+                int pbc = 0;
+                if (b[d] == 0) { b[d] = n[d] - 2; pbc = -1; }
+                else if (b[d] == n[d] - 1) { b[d] = 1; pbc = 1; }
+                const int key = b[2] * P.num_buckets_xy + b[1] * n[0] + b[0];
+                const double plx = pbc * P.periodic_length[0];
+                const double ply = pbc * P.periodic_length[1];
+                const double plz = pbc * P.periodic_length[2];
+                const int e = a.cellStart[key + 1];
+                for (int j = a.cellStart[key]; j < e; j++)
+                    if (j != i) f(j, X[j] + plx, Y[j] + ply, Z[j] + plz, pbc);
+            }
+        }
+    }
+}
+
 // walkCells with the neighbour's position handed to f(j, xj, yj, zj); slot
 // i itself is skipped.  Positions of four consecutive candidates are loaded
 // before the first of them is tested, so the loads overlap.
 template <class F>
 __device__ __forceinline__ void walkCellsPos(const KArgs& a, int i, double x, double y, double z, F&& f) {
     const mps_params& P = a.P;
+    if (periodicDomain(P)) {
+        walkCellsPeriodic(a, i, x, y, z, [&](int j, double xj, double yj, double zj, int) { f(j, xj, yj, zj); });
+        return;
+    }
     const int ix = (int)((x - P.domain_min[0]) * P.inv_bucket_side + KEPS);
     const int iy = (int)((y - P.domain_min[1]) * P.inv_bucket_side + KEPS);
     const int iz = (int)((z - P.domain_min[2]) * P.inv_bucket_side + KEPS);
@@ -128,6 +181,22 @@ __device__ __forceinline__ void listNbrsQ(const KArgs& a, int i, G&& g) {
     const double4* __restrict__ P4 = a.P4;
     const size_t N = a.N;
     int c = 0;
+    if (periodicDomain(a.P)) {
+        // entries carry the ghost shift in bits 30-31 (1: bPBC -1, 2: +1)
+        const mps_params& P = a.P;
+        for (; c < n; c++) {
+            const unsigned e = (unsigned)col[c * N];
+            const int j = (int)(e & kListSlotMask);
+            const unsigned t = e >> 30;
+            const int pbc = t == 1u ? -1 : (t == 2u ? 1 : 0);
+            double4 q = P4[j];
+            q.x = q.x + pbc * P.periodic_length[0];
+            q.y = q.y + pbc * P.periodic_length[1];
+            q.z = q.z + pbc * P.periodic_length[2];
+            g(j, q);
+        }
+        return;
+    }
     for (; c + 4 <= n; c += 4) {
         const int j0 = col[c * N], j1 = col[(c + 1) * N], j2 = col[(c + 2) * N], j3 = col[(c + 3) * N];
         const double4 q0 = P4[j0], q1 = P4[j1], q2 = P4[j2], q3 = P4[j3];

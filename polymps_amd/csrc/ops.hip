@@ -84,6 +84,35 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
     cnt[i] = total;
 }
 
+// Periodic domains: the list is built by a per-particle walk through the
+// ghost layers (walkCellsPeriodic); each entry keeps its ghost shift in bits
+// 30-31 for listNbrsQ.
+__global__ __launch_bounds__(256) void kBuildListWalk(KArgs a, int* nbr, int* cnt, double r2) {
+    const int i = slotId();
+    if (i >= a.N) return;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    a.P4[i] = make_double4(xi, yi, zi, (double)a.I[ITYPE][i]);
+    if (!live(a, i)) {
+        cnt[i] = 0;
+        return;
+    }
+    const size_t N = a.N;
+    int c = 0;
+    walkCellsPeriodic(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj, int pbc) {
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        if (!(d2 < r2)) return;
+        if (c < a.K) nbr[(size_t)c * N + i] = (int)((unsigned)j | (pbc < 0 ? 1u << 30 : (pbc > 0 ? 2u << 30 : 0u)));
+        c++;
+    });
+    if (c > a.K) {
+        atomicMax(&a.is[I_OVERFLOW], c);
+        c = a.K;
+    }
+    atomicMax(&a.is[I_LIST_MAX], c);
+    cnt[i] = c;
+}
+
 // Start slot of stencil row r of a particle in bucket (ix, iy, iz), or -1
 // when the row lies outside the grid (the same rows and order as the walk).
 __device__ __forceinline__ int rowStart(const KArgs& a, int ix, int iy, int iz, int r) {
@@ -317,6 +346,23 @@ __global__ void kVelMaxFinish(KArgs a, const unsigned long long* vmax2bits) {
 // less one bucket; one thread then replays the reference's compaction in
 // ascending reference order (a slot receives the data of the last live
 // particle, the last particle becomes a ghost).
+// Periodic wrap of MpsParticle::checkParticleOutDomain
+// (src/MpsParticle.cpp:236-269): every particle past the physical domain of
+// the periodic axis moves by one periodic length.
+__global__ void kPeriodicWrap(KArgs a) {
+    const int i = slotId();
+    if (i >= a.N || !live(a, i)) return;
+    const mps_params& P = a.P;
+    double* const X[3] = {a.D[DX], a.D[DY], a.D[DZ]};
+    for (int k = 0; k < 3; k++) {
+        if (!P.periodic_dir[k]) continue;
+        double v = X[k][i];
+        if (v > P.phys_max[k]) v -= P.periodic_length[k];
+        if (v < P.phys_min[k]) v += P.periodic_length[k];
+        X[k][i] = v;
+    }
+}
+
 __global__ void kOutFlag(KArgs a, int* outList, int cap) {
     const int i = slotId();
     if (!live(a, i)) return;
@@ -1528,6 +1574,12 @@ int launchBuildList(Ctx* ctx) {
     double r2 = std::max(P.re_l2, P.re_s2);
     r2 = std::max(r2, P.dist_collision_limit2);
     r2 = std::max(r2, P.part_dist * P.part_dist);
+    if (P.domain_type_bc == 1) {
+        hipLaunchKernelGGL(kBuildListWalk, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbr, ctx->nbrCnt, r2);
+        MPS_HIP(hipGetLastError());
+        ctx->listValid = true;
+        return MPS_OK;
+    }
     hipLaunchKernelGGL(kBuildList, dim3(gridFor(ctx->N, kWave)), dim3(kWave), 0, ctx->stream, a, ctx->nbrMask,
                        ctx->nbrCnt, r2);
     if (ctx->maskW <= 4)
@@ -1549,6 +1601,7 @@ int launchCompaction(Ctx* ctx) {
     KArgs a = ctx->args();
     const int g = gridFor(ctx->N, kBlock);
     const int cap = 4096;
+    if (ctx->P.domain_type_bc == 1) hipLaunchKernelGGL(kPeriodicWrap, dim3(g), dim3(kBlock), 0, ctx->stream, a);
     hipLaunchKernelGGL(kOutFlag, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->outList, cap);
     hipLaunchKernelGGL(kCompact, dim3(1), dim3(1), 0, ctx->stream, a, ctx->outList, cap, ctx->slotOf);
     MPS_HIP(hipGetLastError());
@@ -1588,6 +1641,7 @@ int launchOp(Ctx* ctx, const char* op) {
     const mps_params& P = ctx->P;
     if (o == "viscosity") {
         if (ctx->listValid) L1(kViscosity<true>);
+        else if (ctx->P.domain_type_bc == 1) LW(kViscosity);   // walk through the periodic ghosts
         else hipLaunchKernelGGL(kViscosityWalk, dim3(gridFor(ctx->N, kWave)), dim3(kWave), 0, ctx->stream, a);
     }
     else if (o == "add_gravity") L1(kAddGravity);

@@ -349,7 +349,18 @@ int paramsDerive(mps_params* p, int n, const double* pos, std::string& err) {
     const double eps0 = 10.0 * DBL_EPSILON;
     const double l0 = p->part_dist;
     p->eps_zero = eps0;
-    if (p->domain_type_bc != 0) { err = "periodic domains are not supported"; return MPS_E_UNSUPPORTED; }
+    const bool periodic = p->domain_type_bc == 1;
+    int pdir = -1;
+    if (p->domain_type_bc != 0 && !periodic) { err = "unknown domain boundary type"; return MPS_E_UNSUPPORTED; }
+    if (periodic) {
+        // MpsBucket::calcDomainLimits (src/MpsBucket.cpp:141-287) handles one
+        // periodic direction, with particle walls
+        const int nper = (p->periodic_dir[0] != 0) + (p->periodic_dir[1] != 0) + (p->periodic_dir[2] != 0);
+        if (nper != 1) { err = "periodic domains need exactly one periodic direction"; return MPS_E_UNSUPPORTED; }
+        for (int k = 0; k < 3; k++) if (p->periodic_dir[k]) pdir = k;
+        if ((int)p->dim == 2 && pdir == 2) { err = "periodic z in a 2-D case"; return MPS_E_UNSUPPORTED; }
+        if (p->wall_type != MPS_WALL_PARTICLE) { err = "periodic domains are supported with particle walls"; return MPS_E_UNSUPPORTED; }
+    }
     if (p->fluid_type != 0) { err = "only Newtonian fluids are supported"; return MPS_E_UNSUPPORTED; }
     if (p->fem_on || p->forced_on) { err = "deformable / forced meshes are not supported"; return MPS_E_UNSUPPORTED; }
     // domain extension (setParameters :139-148)
@@ -418,8 +429,9 @@ int paramsDerive(mps_params* p, int n, const double* pos, std::string& err) {
     p->num_buckets_y = (int)((p->domain_max[1] - p->domain_min[1]) * p->inv_bucket_side) + 3;
     p->num_buckets_z = (int)((p->domain_max[2] - p->domain_min[2]) * p->inv_bucket_side) + 3;
     if (two) p->num_buckets_z = 1;
+    for (int k = 0; k < 3; k++) p->periodic_length[k] = 0.0;
     if (p->wall_type == MPS_WALL_PARTICLE) {
-        // calcDomainLimits (:71-295), non-periodic
+        // calcDomainLimits (:71-295)
         double lim[3][2];
         if (p->limit_type_bc == 0) {
             if (n <= 0 || !pos) { err = "particle walls need particle positions to size the domain"; return MPS_E_ARG; }
@@ -435,16 +447,44 @@ int paramsDerive(mps_params* p, int n, const double* pos, std::string& err) {
         } else {
             for (int k = 0; k < 3; k++) { lim[k][0] = p->domain_min[k]; lim[k][1] = p->domain_max[k]; }
         }
-        const int nd = two ? 2 : 3;
-        for (int k = 0; k < nd; k++) { lim[k][0] -= 0.5 * l0; lim[k][1] += 0.5 * l0; }
-        for (int k = 0; k < (int)p->dim; k++) {
-            lim[k][0] -= p->bucket_side;
-            lim[k][1] += p->bucket_side;
-            long nb = (long)((lim[k][1] - lim[k][0]) / p->bucket_side + 1);
-            if (k == 0) p->num_buckets_x = (int)nb;
-            else if (k == 1) p->num_buckets_y = (int)nb;
-            else p->num_buckets_z = (int)nb;
-            lim[k][1] = lim[k][0] + nb * p->bucket_side;
+        int* nbk[3] = {&p->num_buckets_x, &p->num_buckets_y, &p->num_buckets_z};
+        if (!periodic) {
+            const int nd = two ? 2 : 3;
+            for (int k = 0; k < nd; k++) { lim[k][0] -= 0.5 * l0; lim[k][1] += 0.5 * l0; }
+            for (int k = 0; k < (int)p->dim; k++) {
+                lim[k][0] -= p->bucket_side;
+                lim[k][1] += p->bucket_side;
+                long nb = (long)((lim[k][1] - lim[k][0]) / p->bucket_side + 1);
+                *nbk[k] = (int)nb;
+                lim[k][1] = lim[k][0] + nb * p->bucket_side;
+            }
+        } else {
+            // periodic direction d (:148-160 for x, :194-206 for y, :240-252 for z)
+            const int d = pdir;
+            p->periodic_length[d] = lim[d][1] - lim[d][0] + l0;
+            lim[d][0] -= l0 * 0.5;
+            lim[d][1] += l0 * 0.5;
+            p->phys_min[d] = lim[d][0];
+            p->phys_max[d] = lim[d][1];
+            long nbd = (long)((lim[d][1] - lim[d][0]) / p->bucket_side);
+            // the bucket side divides the periodic length exactly
+            p->bucket_side = p->periodic_length[d] / (long)(p->periodic_length[d] / p->bucket_side);
+            lim[d][0] -= p->bucket_side;
+            lim[d][1] += p->bucket_side;
+            nbd += 2;
+            *nbk[d] = (int)nbd;
+            // the other axes, one bucket beyond the wall particles
+            for (int k = 0; k < 3; k++) {
+                if (k == d || (k == 2 && two)) continue;
+                lim[k][0] -= p->bucket_side;
+                lim[k][1] += p->bucket_side;
+                p->phys_min[k] = lim[k][0];
+                p->phys_max[k] = lim[k][1];
+                long nb = (long)((lim[k][1] - lim[k][0]) / p->bucket_side + 1);
+                *nbk[k] = (int)nb;
+                lim[k][1] = lim[k][0] + nb * p->bucket_side;
+            }
+            if (*nbk[d] < 3) { err = "periodic direction shorter than one bucket"; return MPS_E_UNSUPPORTED; }
         }
         for (int k = 0; k < 3; k++) { p->domain_min[k] = lim[k][0]; p->domain_max[k] = lim[k][1]; }
         p->inv_bucket_side = 1.0 / p->bucket_side;

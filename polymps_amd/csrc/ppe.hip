@@ -52,9 +52,25 @@ __global__ void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc
     double sum = 0.0, rhs = 0.0;
     int c = 0, prev = -1, ne = 0;
     bool full = false;
+    const bool periodic = periodicDomain(P);
     forNbrsPosType<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj, int tj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        // Entry (j, i) of A is formed in row j of the reference, where i is
+        // seen through the opposite ghost layer: (x_i - pl) - x_j, whose
+        // rounding differs from -(x_j + pl - x_i) (periodic domains only)
+        double d2T = d2;
+        if (TRANS && periodic) {
+            const int pd = P.periodic_dir[0] ? 0 : (P.periodic_dir[1] ? 1 : 2);
+            const double raw = a.D[DX + pd][j], sh = (pd == 0) ? xj : ((pd == 1) ? yj : zj);
+            const int pbc = sh > raw ? 1 : (sh < raw ? -1 : 0);
+            if (pbc != 0) {
+                const double u0 = (xi + (-pbc) * P.periodic_length[0]) - a.D[DX][j];
+                const double u1 = (yi + (-pbc) * P.periodic_length[1]) - a.D[DY][j];
+                const double u2 = (zi + (-pbc) * P.periodic_length[2]) - a.D[DZ][j];
+                d2T = u0 * u0 + u1 * u1 + u2 * u2;
+            }
+        }
         if (!(d2 < P.re_l2)) return;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
         const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
@@ -93,8 +109,9 @@ __global__ void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc
             const bool delta = prev >= 0 && dj > 0 && dj < 65536;
             if (c < a.K && (delta || ne < kPpeEsc)) {
                 const size_t o = (size_t)c * a.N + i;
-                val[o] = mij;
-                if (val32) val32[o] = (float)mij;
+                const double mv = (TRANS && d2T != d2) ? wgt(sqrt(d2T), P.re_l, P.weight_type) * P.coeff_ppe : mij;
+                val[o] = mv;
+                if (val32) val32[o] = (float)mv;
                 if (delta) {
                     col[o] = (unsigned short)dj;
                 } else {

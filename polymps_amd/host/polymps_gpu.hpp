@@ -218,10 +218,11 @@ public:
 class MpsBoundaryCondition {
 public:
     // MpsBoundaryCondition::setBucketBC (src/MpsBoundaryCondition.cpp:20-63):
-    // periodic domains are not supported by the device library.
+    // the periodic ghost layers follow from the bucket grid that
+    // mps_params_derive computes, and the device walks resolve them; nothing
+    // is stored per bucket.
     void setBucketBC(MpsParticleSystem* PSystem, MpsParticle* Particles, MpsBucket* Buckets) {
-        (void)Particles; (void)Buckets;
-        if (PSystem->prm.domain_type_bc == PERIODIC) throw MpsGpuError("periodic domains are not supported");
+        (void)PSystem; (void)Particles; (void)Buckets;
     }
     // MpsBoundaryCondition::updateParticleBC (src/MpsBoundaryCondition.cpp:66-400)
     void updateParticleBC(MpsParticleSystem* PSystem, MpsParticle* Particles, MpsBucket* Buckets) {

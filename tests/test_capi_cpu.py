@@ -93,7 +93,8 @@ def test_json_incompressible_needs_pnd_type_0(built, tmp_path):
         L.params_from_json(f)
 
 
-@pytest.mark.parametrize("make", [lambda: configs.small_dam3d(l0=0.025), lambda: configs.small_dam2d(l0=0.02)])
+@pytest.mark.parametrize("make", [lambda: configs.small_dam3d(l0=0.025), lambda: configs.small_dam2d(l0=0.02),
+                                  lambda: configs.periodic_channel2d(), lambda: configs.periodic_channel3d()])
 def test_derive_matches_oracle(built, make):
     from oracle.oracle import Oracle
     w = make()

@@ -14,7 +14,8 @@ from polymps_amd import lib as L
 from polymps_amd.params import MpsParams
 
 
-@pytest.mark.parametrize("make", [lambda: configs.small_dam3d(l0=0.025), lambda: configs.small_dam2d(l0=0.02)])
+@pytest.mark.parametrize("make", [lambda: configs.small_dam3d(l0=0.025), lambda: configs.small_dam2d(l0=0.02),
+                                  lambda: configs.periodic_channel2d()])
 def test_case_json_round_trip(built, tmp_path, make):
     w = make()
     path = configs.write_case(w, tmp_path, "case")

@@ -23,6 +23,7 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("make,flags", [
     (lambda: configs.small_dam3d(l0=0.025, steps=10), 1 | 2 | 16),
     (lambda: configs.small_dam2d(l0=0.02, steps=10), 1 | 2 | 8 | 16),
+    (lambda: configs.periodic_channel2d(steps=10), 1 | 2 | 16),
 ])
 def test_driver_matches_library_steps(built, tmp_path, make, flags):
     w = make()

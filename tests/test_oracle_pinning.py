@@ -427,3 +427,59 @@ def test_weakly_compressible_eos(built):
     expect = 1000.0 * 400.0 / 7.0 * (1.01 ** 7 - 1)
     assert np.allclose(pr[bc == q.inner], expect, rtol=1e-12)
     assert np.all(pr[bc != q.inner] == 0.0)
+
+
+# ----------------------------------------------------- periodic domains
+def _min_image(v, axis, L):
+    v = v.copy()
+    v[:, axis] = np.where(v[:, axis] > 0.5 * L, v[:, axis] - L, v[:, axis])
+    v[:, axis] = np.where(v[:, axis] < -0.5 * L, v[:, axis] + L, v[:, axis])
+    return v
+
+
+@pytest.mark.parametrize("mk", ["2d", "3d"])
+def test_periodic_neighbour_sets_minimum_image(built, mk):
+    """Walks through the ghost bucket layers find exactly the minimum-image
+    neighbours of the periodic axis, also after particles crossed the seam."""
+    w = configs.periodic_channel2d(u0=5.0) if mk == "2d" else configs.periodic_channel3d(u0=5.0)
+    axis = 0 if mk == "2d" else 1
+    o = Oracle(w.params, w.types, w.pos, w.vel)
+    o.call("init")
+    o.steps(20)
+    q = o.params
+    L = q.periodic_length[axis]
+    P = o.get("pos").reshape(-1, 3)[: int(o.scalar("num_particles"))]
+    assert P[:, axis].min() >= q.phys_min[axis] and P[:, axis].max() <= q.phys_max[axis]
+    crossed = np.abs(P[:, axis] - w.pos.reshape(-1, 3)[: len(P), axis]) > 0.5 * L
+    assert crossed.any()   # some particles wrapped
+    rng = np.random.default_rng(5)
+    near = np.flatnonzero((P[:, axis] < q.phys_min[axis] + q.re_l) | (P[:, axis] > q.phys_max[axis] - q.re_l))
+    for i in np.concatenate([rng.choice(near, 20, replace=False), rng.choice(len(P), 10, replace=False)]):
+        v = _min_image(P - P[i], axis, L)
+        d2 = (v * v).sum(1)
+        ok = d2 < q.re_l2
+        ok[i] = False
+        got = o.neighbors(int(i), q.re_l2, 1)
+        assert sorted(got.tolist()) == np.flatnonzero(ok).tolist()
+
+
+def test_periodic_pnd_is_translation_invariant(built):
+    """On an unjittered periodic channel the initial particle number density
+    and neighbour count depend on the row only: the particles next to the seam
+    see the same lattice as those in the middle (wrong ghost shifts would drop
+    or double neighbours there)."""
+    w = configs.periodic_channel2d(jitter=0.0, mps_type=2)
+    o = Oracle(w.params, w.types, w.pos, w.vel)
+    o.call("init")
+    q = o.params
+    nf = w.info["fluid"]
+    P = w.pos.reshape(-1, 3)[:nf]
+    assert abs(q.periodic_length[0] - 40 * q.part_dist) < 1e-12
+    sides = q.periodic_length[0] / q.bucket_side
+    assert abs(sides - round(sides)) < 1e-9
+    pnd, nn = o.get("pnd_small")[:nf], o.get("num_neigh")[:nf]
+    rows = np.round(P[:, 1] / q.part_dist - 0.5).astype(int)
+    for r in np.unique(rows):
+        m = rows == r
+        assert np.ptp(pnd[m]) < 1e-12 * pnd[m].max(), r
+        assert np.ptp(nn[m]) == 0, r
