@@ -142,6 +142,11 @@ int mps_update_bc(mps_ctx* ctx);
  * assembly (solvePressurePoissonPnd[DivU] :169-421) + Jacobi-PCG
  * (replacing solveConjugateGradient :424-445) + setZeroOnNegativePressure. */
 int mps_calc_press(mps_ctx* ctx);
+/* MpsPressure::calcVelDivergence (src/MpsPressure.cpp:479-551) and, with
+ * polygon walls, calcWall{Slip,NoSlip}VelDivergence (:554-788): the velocity
+ * divergence of the PND + DivU source term (part of mps_calc_press when
+ * mps_type is 3) */
+int mps_vel_divergence(mps_ctx* ctx);
 /* PPE assembly only (the first half of mps_calc_press) */
 int mps_ppe_assemble(mps_ctx* ctx);
 /* PPE solve only, from the current ppe_x guess (second half of mps_calc_press) */

@@ -1428,6 +1428,68 @@ void velDivergence(Sys& s) {
     }
 }
 
+// calcWallSlipVelDivergence / calcWallNoSlipVelDivergence
+// (src/MpsPressure.cpp:554-657, 658-788): the mirror neighbourhood of a
+// near-wall fluid particle adds to its velocity divergence.  The mirror
+// velocity is R v (free slip) or -(v - 2 (v_wall - (n.v_wall) n)) with a
+// fixed wall (no slip, where the relative position is reflected by R too).
+void wallVelDivergence(Sys& s) {
+    const mps_params& P = s.p;
+    const bool slip = P.slip_condition == MPS_FREE_SLIP;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        const double ni = s.pndi[i];
+        if (!(s.type[i] == P.fluid && s.nearWall[i] && ni > kEps)) continue;
+        double DivV = 0.0;
+        const double* xi = &s.pos[i * 3];
+        const double* xm = &s.mirror[i * 3];
+        const double velXi = s.vel[i * 3], velYi = s.vel[i * 3 + 1], velZi = s.vel[i * 3 + 2];
+        double n[3], R[9];
+        reflectionMod2(xi, xm, n, R);
+        double vmx, vmy, vmz;
+        if (slip) {
+            vmx = (R[0] * velXi + R[1] * velYi + R[2] * velZi);
+            vmy = (R[3] * velXi + R[4] * velYi + R[5] * velZi);
+            vmz = (R[6] * velXi + R[7] * velYi + R[8] * velZi);
+        } else {
+            const double Rinv[9] = {-1.0, 0.0, 0.0, 0.0, -1.0, 0.0, 0.0, 0.0, -1.0};
+            const double vw[3] = {0.0, 0.0, 0.0};   // fixed wall (forced meshes are not supported)
+            const double dotnv = n[0] * vw[0] + n[1] * vw[1] + n[2] * vw[2];
+            const double vt[3] = {velXi - 2.0 * (vw[0] - dotnv * n[0]), velYi - 2.0 * (vw[1] - dotnv * n[1]),
+                                  velZi - 2.0 * (vw[2] - dotnv * n[2])};
+            vmx = (Rinv[0] * vt[0] + Rinv[1] * vt[1] + Rinv[2] * vt[2]);
+            vmy = (Rinv[3] * vt[0] + Rinv[4] * vt[1] + Rinv[5] * vt[2]);
+            vmz = (Rinv[6] * vt[0] + Rinv[7] * vt[1] + Rinv[8] * vt[2]);
+        }
+        auto term = [&](double pj, double ux, double uy, double uz, double w0, double w1, double w2, double dm2) {
+            const double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
+            if (slip) {
+                const double vx = ux - vmx, vy = uy - vmy, vz = uz - vmz;
+                DivV += (P.dim / P.pnd_small_zero) * (pj / ni) * (vx * w0 + vy * w1 + vz * w2) * wS / dm2;
+            } else {
+                const double vx = -(ux - vmx), vy = -(uy - vmy), vz = -(uz - vmz);
+                const double m0 = (R[0] * w0 + R[1] * w1 + R[2] * w2);
+                const double m1 = (R[3] * w0 + R[4] * w1 + R[5] * w2);
+                const double m2 = (R[6] * w0 + R[7] * w1 + R[8] * w2);
+                DivV += (P.dim / P.pnd_small_zero) * (pj / ni) * (vx * m0 + vy * m1 + vz * m2) * wS / dm2;
+            }
+        };
+        walk(s, xi[0], xi[1], xi[2], [&](int j) {
+            const double v0 = PJ(s, j, 0) - xi[0], v1 = PJ(s, j, 1) - xi[1], v2 = PJ(s, j, 2) - xi[2];
+            const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            const double w0 = PJ(s, j, 0) - xm[0], w1 = PJ(s, j, 1) - xm[1], w2 = PJ(s, j, 2) - xm[2];
+            const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2 && j != i)
+                term(s.pndi[j], s.vel[j * 3], s.vel[j * 3 + 1], s.vel[j * 3 + 2], w0, w1, w2, dm2);
+        });
+        // i is a neighbour of its own mirror
+        const double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (dm2 < P.re_s2) term(s.pndi[i], velXi, velYi, velZi, w0, w1, w2, dm2);
+        s.divU[i] += DivV;
+    }
+}
+
 // PPE assembly: solvePressurePoissonPnd / ...PndDivU (src/MpsPressure.cpp:169-421).
 // Rows are stored in triplet order (neighbours in walk order, then the
 // diagonal); identity rows for surface / other particles.
@@ -1571,7 +1633,7 @@ int calcPress(Sys& s) {
     }
     if (P.mps_type == MPS_IMPLICIT_PND_DIVU) {
         velDivergence(s);
-        if (P.wall_type == MPS_WALL_POLYGON) { s.err = "PND+DivU with polygon walls is not supported"; return -1; }
+        if (P.wall_type == MPS_WALL_POLYGON) wallVelDivergence(s);
     }
     if (P.solver_type != MPS_SOLVER_CG) { s.err = "only the CG solver is supported"; return -1; }
     assemblePPE(s);
@@ -2149,6 +2211,7 @@ int orc_call(void* h, const char* op) {
     else if (o == "mean_neigh_fluid_pnd") meanNeighFluidPnd(*s);
     else if (o == "update_bc") return updateBC(*s);
     else if (o == "vel_divergence") velDivergence(*s);
+    else if (o == "wall_vel_divergence") wallVelDivergence(*s);
     else if (o == "assemble_ppe") assemblePPE(*s);
     else if (o == "solve_ppe") { solveCG(*s); for (int i = 0; i < s->N; i++) s->press[i] = s->ppeX[i] < 0 ? 0.0 : s->ppeX[i]; }
     else if (o == "calc_press") return calcPress(*s);

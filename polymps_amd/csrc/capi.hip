@@ -195,6 +195,13 @@ static int syncStatus(Ctx* ctx) {
     return MPS_OK;
 }
 
+// calcVelDivergence, then the polygon-wall term (src/MpsPressure.cpp:46-53)
+static int velDivergence(Ctx* ctx) {
+    int rc = launchOp(ctx, "vel_divergence");
+    if (rc == MPS_OK && ctx->P.wall_type == MPS_WALL_POLYGON) rc = launchOp(ctx, "wall_vel_divergence");
+    return rc;
+}
+
 static int runInit(Ctx* ctx) {
     const mps_params& P = ctx->P;
     int rc;
@@ -216,7 +223,7 @@ static int runInit(Ctx* ctx) {
     if (P.mps_type == MPS_EXPLICIT || P.mps_type == MPS_WEAKLY) {
         if ((rc = launchOp(ctx, "press_explicit"))) return rc;
     } else {
-        if (P.mps_type == MPS_IMPLICIT_PND_DIVU && (rc = launchOp(ctx, "vel_divergence"))) return rc;
+        if (P.mps_type == MPS_IMPLICIT_PND_DIVU && (rc = velDivergence(ctx))) return rc;
         if ((rc = ppeAssemble(ctx))) return rc;
         if ((rc = ppeSolve(ctx))) return rc;
     }
@@ -227,10 +234,7 @@ static int calcPress(Ctx* ctx) {
     const mps_params& P = ctx->P;
     int rc;
     if (P.mps_type == MPS_EXPLICIT || P.mps_type == MPS_WEAKLY) return launchOp(ctx, "press_explicit");
-    if (P.mps_type == MPS_IMPLICIT_PND_DIVU) {
-        if (P.wall_type == MPS_WALL_POLYGON) { ctx->err = "PND+DivU with polygon walls is not supported"; return MPS_E_UNSUPPORTED; }
-        if ((rc = launchOp(ctx, "vel_divergence"))) return rc;
-    }
+    if (P.mps_type == MPS_IMPLICIT_PND_DIVU && (rc = velDivergence(ctx))) return rc;
     if (P.solver_type != MPS_SOLVER_CG) { ctx->err = "only the CG solver is supported"; return MPS_E_UNSUPPORTED; }
     if ((rc = ppeAssemble(ctx))) return rc;
     return ppeSolve(ctx);
@@ -778,6 +782,13 @@ int mps_calc_press(mps_ctx* ctx) {
     CHECK_CTX(ctx);
     hipSetDevice(ctx->dev);
     int rc = calcPress(ctx);
+    if (rc) return rc;
+    return syncStatus(ctx);
+}
+int mps_vel_divergence(mps_ctx* ctx) {
+    CHECK_CTX(ctx);
+    hipSetDevice(ctx->dev);
+    int rc = velDivergence(ctx);
     if (rc) return rc;
     return syncStatus(ctx);
 }

@@ -900,14 +900,13 @@ __global__ void kPndDiffusive(KArgs a) {
     a.D[DDIFF][i] = Di;
 }
 
-// calcWall{Slip,NoSlip}PndDiffusiveTerm (src/MpsPndNeigh.cpp:492-744)
+// Velocity divergence over the mirror neighbourhood of near-wall fluid
+// particle i (pndi = ni > eps), shared by calcWall{Slip,NoSlip}PndDiffusiveTerm
+// (src/MpsPndNeigh.cpp:492-746) and calcWall{Slip,NoSlip}VelDivergence
+// (src/MpsPressure.cpp:554-788), which form it identically.
 template <bool LIST>
-__global__ void kWallPndDiffusive(KArgs a) {
-    const int i = slotId();
-    if (!live(a, i)) return;
+__device__ double wallDivV(const KArgs& a, int i, double ni) {
     const mps_params& P = a.P;
-    const double ni = a.D[DPNDI][i];
-    if (!(a.I[ITYPE][i] == P.fluid && a.I[INEARWALL][i] && ni > KEPS)) return;
     const bool slip = P.slip_condition == MPS_FREE_SLIP;
     const double xi[3] = {PX(i), PY(i), PZ(i)};
     const double xm[3] = {a.D[DMX][i], a.D[DMY][i], a.D[DMZ][i]};
@@ -953,7 +952,31 @@ __global__ void kWallPndDiffusive(KArgs a) {
     const double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
     const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
     if (dm2 < P.re_s2) term(ux, uy, uz, ni, w0, w1, w2, dm2);
+    return DivV;
+}
+
+// calcWall{Slip,NoSlip}PndDiffusiveTerm: pndAuxVar1 += -pndi dt DivV
+template <bool LIST>
+__global__ void kWallPndDiffusive(KArgs a) {
+    const int i = slotId();
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const double ni = a.D[DPNDI][i];
+    if (!(a.I[ITYPE][i] == P.fluid && a.I[INEARWALL][i] && ni > KEPS)) return;
+    const double DivV = wallDivV<LIST>(a, i, ni);
     a.D[DAUX1][i] = a.D[DAUX1][i] + -ni * P.time_step * DivV;
+}
+
+// calcWall{Slip,NoSlip}VelDivergence (src/MpsPressure.cpp:554-788):
+// velDivergence += DivV
+template <bool LIST>
+__global__ void kWallVelDivergence(KArgs a) {
+    const int i = slotId();
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const double ni = a.D[DPNDI][i];
+    if (!(a.I[ITYPE][i] == P.fluid && a.I[INEARWALL][i] && ni > KEPS)) return;
+    a.D[DDIVU][i] = a.D[DDIVU][i] + wallDivV<LIST>(a, i, ni);
 }
 
 // updatePnd (src/MpsPndNeigh.cpp:747-774)
@@ -1679,6 +1702,7 @@ int launchOp(Ctx* ctx, const char* op) {
         L1(kUpdateBC);
     } else if (o == "press_explicit") L1(kPressExplicit);
     else if (o == "vel_divergence") LT(kVelDivergence);
+    else if (o == "wall_vel_divergence") LT(kWallVelDivergence);
     else if (o == "extrapolate_wall_dummy") LT(kExtrapolateWallDummy2); else if (o == "press_gradient") LT(kPressGradient);
     else if (o == "wall_press_gradient") LT(kWallPressGradient);
     else if (o == "wall_viscosity") LT(kWallViscosity);

@@ -21,7 +21,7 @@ _OPS = [
     "bucket_update", "viscosity", "add_gravity", "prediction", "check_out_domain", "build_neighbor_list",
     "collision", "mesh_closest_point", "mesh_update_near", "wall_npcd", "correction_matrix", "initial_pnd",
     "pnd_neigh_npcd", "pnd_diffusive", "mean_pnd", "mean_neigh_fluid_pnd", "update_bc", "calc_press",
-    "ppe_assemble", "ppe_solve", "extrapolate_wall_dummy", "press_gradient", "wall_press_gradient",
+    "vel_divergence", "ppe_assemble", "ppe_solve", "extrapolate_wall_dummy", "press_gradient", "wall_press_gradient",
     "wall_viscosity", "correction", "shifting", "wall_dummy_velocity", "init", "step",
 ]
 

@@ -132,6 +132,8 @@ def lockstep_step(g: Simulation, o: Oracle, step: int, press_rtol=1e-6, log=None
         compare_close("press", g.get("press"), o.get("press"), 1e-13, 1e-9, f"{W} calc_press")
         g.set("press", o.get("press"))
     else:
+        if p.mps_type == 3:
+            both("vel_divergence", ["vel_divergence"] + (["wall_vel_divergence"] if poly else []), ["div_u"])
         g.call("ppe_assemble")
         o.call("assemble_ppe")
         ppe_check(g, o, f"{W} ppe_assemble", press_rtol)

@@ -42,6 +42,11 @@ def lockstep_init(g, o):
     g.call("update_bc"); o.call("update_bc")
     check(g, o, ["bc"], "init update_bc")
     if p.mps_type >= 2:
+        if p.mps_type == 3:
+            g.call("vel_divergence"); o.call("vel_divergence")
+            if poly:
+                o.call("wall_vel_divergence")
+            check(g, o, ["div_u"], "init vel_divergence")
         g.call("ppe_assemble"); o.call("assemble_ppe")
         ppe_check(g, o, "init ppe", 1e-6)
         g.call("ppe_solve"); o.call("solve_ppe")
@@ -83,6 +88,23 @@ def test_lockstep_dam3d_matrix_orientation(built, transpose):
     o.steps(60)
     sync_state(g, o)
     lockstep_step(g, o, 0)
+
+
+@pytest.mark.parametrize("slip", [0, 1])
+def test_lockstep_dam3d_divu(built, slip):
+    """PND + DivU source term (mps_type 3) with polygon walls: the velocity
+    divergence including the mirror (wall) term, free slip and no slip."""
+    w = configs.small_dam3d(l0=0.025)
+    w.params.mps_type = 3
+    w.params.slip_condition = slip
+    g, o = new_pair(w)
+    lockstep_init(g, o)
+    for s in range(2):
+        lockstep_step(g, o, s)
+    o.steps(40)
+    sync_state(g, o)
+    for s in range(2):
+        lockstep_step(g, o, 10 + s)
 
 
 def test_lockstep_dam2d_wc(built):

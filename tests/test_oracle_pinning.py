@@ -483,3 +483,43 @@ def test_periodic_pnd_is_translation_invariant(built):
         m = rows == r
         assert np.ptp(pnd[m]) < 1e-12 * pnd[m].max(), r
         assert np.ptp(nn[m]) == 0, r
+
+
+@pytest.mark.parametrize("slip", [1, 0])
+def test_wall_velocity_divergence_two_particles(built, slip):
+    """calcWall{Slip,NoSlip}VelDivergence for two fluid particles at height h
+    over the floor of the box, dx apart, moving with the same velocity U
+    along x.  Free slip: the mirror velocity R v equals v, so the wall term
+    is 0.  No slip (fixed wall): the mirror velocity is -v, the pair
+    term is (d/n0) (n_j/n_i) (-2U dx) w(|r'|) / |r'|^2 with r' = x_j - x_i'
+    (the mirror x_i' = x_i - 2h e_z); the own-mirror term is normal to U and
+    vanishes."""
+    w = configs.small_dam3d(l0=0.025)
+    p = w.params
+    p.mps_type = 3
+    p.slip_condition = slip
+    l0, h, dx, U = 0.025, 0.6 * 0.025, 1.3 * 0.025, 0.7
+    pos = np.array([[0.8, 0.075, h], [0.8 + dx, 0.075, h]])
+    types = np.zeros(2, dtype=np.int32)
+    vel = np.array([[U, 0.0, 0.0], [U, 0.0, 0.0]])
+    o = Oracle(p, types, pos.reshape(-1), vel.reshape(-1), mesh=w.mesh)
+    for op in ["update_buckets", "closest_point", "update_near_mesh"]:
+        o.call(op)
+    q = o.params
+    ni, nj = 1.7, 2.3
+    o.set("pndi", np.array([ni, nj]))
+    o.set("div_u", np.zeros(2))
+    o.call("wall_vel_divergence")
+    got = o.get("div_u")[0]
+    if slip:
+        # zero up to the rounding of the computed mirror point (a reflected
+        # mirror velocity of the wrong sign gives a term of order U / l0)
+        assert abs(got) < 1e-9 * U / l0
+    else:
+        xm = o.get("mirror").reshape(-1, 3)[0]
+        assert np.allclose(xm, [0.8, 0.075, -h], atol=1e-15)
+        r2 = dx * dx + (2 * h) ** 2
+        r = math.sqrt(r2)
+        expect = (3.0 / q.pnd_small_zero) * (nj / ni) * (-2.0 * U * dx) * (q.re_s / r - 1.0) / r2
+        assert got == pytest.approx(expect, rel=1e-12)
+        assert got != 0.0
