@@ -156,6 +156,11 @@ int mps_ppe_solve(mps_ctx* ctx);
 int mps_ppe_matrix(mps_ctx* ctx, int* row_ptr, int* cols, double* vals, int* nnz);
 /* MpsPressure::extrapolatePressParticlesWallDummy (src/MpsPressure.cpp:792-846) */
 int mps_extrapolate_wall_dummy(mps_ctx* ctx);
+/* MpsPressure::predictionPressGradient (src/MpsPressure.cpp:932-1028) and
+ * predictionWallPressGradient (:1031-1214): acc += (1 - relax_fact) grad p / rho
+ * before the predictor when relax_fact < 1 */
+int mps_prediction_press_gradient(mps_ctx* ctx);
+int mps_prediction_wall_press_gradient(mps_ctx* ctx);
 /* MpsPressure::calcPressGradient (src/MpsPressure.cpp:1217-1338) */
 int mps_press_gradient(mps_ctx* ctx);
 /* MpsPressure::calcWallPressGradient (src/MpsPressure.cpp:1341-1550) */

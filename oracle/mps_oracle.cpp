@@ -1716,6 +1716,51 @@ void pressGradient(Sys& s) {
     }
 }
 
+// predictionPressGradient (src/MpsPressure.cpp:932-1028): the (1 - relax)
+// part of the pressure gradient before the predictor, without the gradient
+// correction, added to acc of every particle
+void predictionPressGradient(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+    const int gt = P.gradient_type;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        double accX = 0.0, accY = 0.0, accZ = 0.0;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        const double Pi = s.press[i], ni = s.pndi[i];
+        double pressMin = Pi;
+        if (gt == 0 || gt == 2)
+            walk(s, xi, yi, zi, [&](int j) {
+                const double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
+                const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+                const double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
+                const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+                if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i)
+                    if (pressMin > s.press[j]) pressMin = s.press[j];
+            });
+        walk(s, xi, yi, zi, [&](int j) {
+            const double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
+            const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            const double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
+            const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (!(d2 < P.re_s2 && (d2 < dm2 || pw) && j != i)) return;
+            double wS = weight(sqrt(d2), P.re_s, P.weight_type);
+            if (gt == 0) wS *= (s.press[j] - pressMin) / d2;
+            else if (gt == 1) wS *= (s.press[j] + Pi) / d2;
+            else if (gt == 2) wS *= (s.press[j] + Pi - 2.0 * pressMin) / d2;
+            else if (gt == 3) {
+                const double nj = s.pndi[j];
+                if (ni > kEps && nj > kEps) wS *= (ni * s.press[j] / nj + nj * Pi / ni) / d2;
+            }
+            accX += v0 * wS; accY += v1 * wS; accZ += v2 * wS;
+        });
+        s.acc[i * 3] += (1.0 - P.relax_press) * accX * P.coeff_press_grad / s.RHO[i];
+        s.acc[i * 3 + 1] += (1.0 - P.relax_press) * accY * P.coeff_press_grad / s.RHO[i];
+        s.acc[i * 3 + 2] += (1.0 - P.relax_press) * accZ * P.coeff_press_grad / s.RHO[i];
+    }
+}
+
 // repulsive forces (src/MpsPressure.cpp:1554-1611)
 inline void repulsive(const Sys& s, int i, const double* n, double ns, double vmax2, double* f) {
     const mps_params& P = s.p;
@@ -1794,6 +1839,63 @@ void wallPressGradient(Sys& s) {
         s.forceWall[i * 3] += -g0 * vol;
         s.forceWall[i * 3 + 1] += -g1 * vol;
         s.forceWall[i * 3 + 2] += -g2 * vol;
+    }
+}
+
+// predictionWallPressGradient (src/MpsPressure.cpp:1031-1214): the polygon
+// wall part of the (1 - relax) gradient with the repulsive force; the wall
+// force is not accumulated here
+void predictionWallPressGradient(Sys& s) {
+    const mps_params& P = s.p;
+    double vm = std::min(s.velMax, P.expect_max_velocity);
+    double vmax2 = vm * vm;
+    const int gt = P.gradient_type;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        if (!(s.type[i] == P.fluid && s.nearWall[i])) continue;
+        double accX = 0.0, accY = 0.0, accZ = 0.0;
+        const double* xi = &s.pos[i * 3];
+        const double* xm = &s.mirror[i * 3];
+        const double Pi = s.press[i], ni = s.pndi[i];
+        double pressMin = Pi;
+        double n[3], R[9], ns;
+        reflection(xi, xm, n, R, &ns);
+        if (gt == 0 || gt == 2)
+            walk(s, xi[0], xi[1], xi[2], [&](int j) {
+                const double v0 = PJ(s, j, 0) - xi[0], v1 = PJ(s, j, 1) - xi[1], v2 = PJ(s, j, 2) - xi[2];
+                const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+                const double w0 = PJ(s, j, 0) - xm[0], w1 = PJ(s, j, 1) - xm[1], w2 = PJ(s, j, 2) - xm[2];
+                const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+                if (d2 < P.re_s2 && d2 < dm2 && j != i)
+                    if (pressMin > s.press[j]) pressMin = s.press[j];
+            });
+        auto term = [&](double Pj, double nj, double w0, double w1, double w2, double dm2) {
+            double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
+            if (gt == 0) wS *= (Pj - pressMin) / dm2;
+            else if (gt == 1) wS *= (Pj + Pi) / dm2;
+            else if (gt == 2) wS *= (Pj + Pi - 2.0 * pressMin) / dm2;
+            else if (gt == 3) {
+                if (ni > kEps && nj > kEps) wS *= (ni * Pj / nj + nj * Pi / ni) / dm2;
+            }
+            accX += w0 * wS; accY += w1 * wS; accZ += w2 * wS;
+        };
+        walk(s, xi[0], xi[1], xi[2], [&](int j) {
+            const double v0 = PJ(s, j, 0) - xi[0], v1 = PJ(s, j, 1) - xi[1], v2 = PJ(s, j, 2) - xi[2];
+            const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            const double w0 = PJ(s, j, 0) - xm[0], w1 = PJ(s, j, 1) - xm[1], w2 = PJ(s, j, 2) - xm[2];
+            const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2 && j != i) term(s.press[j], s.pndi[j], w0, w1, w2, dm2);
+        });
+        {
+            const double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
+            const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (dm2 < P.re_s2) term(Pi, s.pndi[i], w0, w1, w2, dm2);
+        }
+        double f[3] = {0.0, 0.0, 0.0};
+        if (ns < P.re_repulsive_force && ns > kEps) repulsive(s, i, n, ns, vmax2, f);
+        s.acc[i * 3] += ((1.0 - P.relax_press) * (R[0] * accX + R[1] * accY + R[2] * accZ) * P.coeff_press_grad - f[0]) / s.RHO[i];
+        s.acc[i * 3 + 1] += ((1.0 - P.relax_press) * (R[3] * accX + R[4] * accY + R[5] * accZ) * P.coeff_press_grad - f[1]) / s.RHO[i];
+        s.acc[i * 3 + 2] += ((1.0 - P.relax_press) * (R[6] * accX + R[7] * accY + R[8] * accZ) * P.coeff_press_grad - f[2]) / s.RHO[i];
     }
 }
 
@@ -1971,10 +2073,13 @@ int initSystem(Sys& s) {
 // mainLoopOfSimulation body (src/main.cpp:252-467), Newtonian fluid
 int stepSystem(Sys& s) {
     const mps_params& P = s.p;
-    if (P.relax_press < 1.0) { s.err = "relax_fact < 1 (prediction gradient) is not supported"; return -1; }
     updateParticlesID(s);
     calcViscosity(s);
     addGravity(s);
+    if (P.relax_press < 1.0) {
+        predictionPressGradient(s);
+        if (P.wall_type == MPS_WALL_POLYGON) predictionWallPressGradient(s);
+    }
     prediction(s);
     checkOutDomain(s);
     if (P.collision_type == MPS_COLLISION_PC) collisionPC(s);
@@ -2212,6 +2317,8 @@ int orc_call(void* h, const char* op) {
     else if (o == "update_bc") return updateBC(*s);
     else if (o == "vel_divergence") velDivergence(*s);
     else if (o == "wall_vel_divergence") wallVelDivergence(*s);
+    else if (o == "prediction_press_gradient") predictionPressGradient(*s);
+    else if (o == "prediction_wall_press_gradient") predictionWallPressGradient(*s);
     else if (o == "assemble_ppe") assemblePPE(*s);
     else if (o == "solve_ppe") { solveCG(*s); for (int i = 0; i < s->N; i++) s->press[i] = s->ppeX[i] < 0 ? 0.0 : s->ppeX[i]; }
     else if (o == "calc_press") return calcPress(*s);

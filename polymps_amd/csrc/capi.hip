@@ -247,10 +247,13 @@ static int runStep(Ctx* ctx) {
     do {                                           \
         if ((rc = launchOp(ctx, name))) return rc; \
     } while (0)
-    if (P.relax_press < 1.0) { ctx->err = "relax_fact < 1 (prediction pressure gradient) is not supported"; return MPS_E_UNSUPPORTED; }
     if ((rc = launchBucketSort(ctx))) return rc;
     OP("viscosity");
     OP("add_gravity");
+    if (P.relax_press < 1.0) {
+        OP("prediction_press_gradient");
+        if (P.wall_type == MPS_WALL_POLYGON) OP("prediction_wall_press_gradient");
+    }
     OP("prediction");
     if ((rc = launchCompaction(ctx))) return rc;
     if (P.collision_type == MPS_COLLISION_PC) {
@@ -773,6 +776,8 @@ OPFN(mps_update_bc, "update_bc")
 OPFN(mps_extrapolate_wall_dummy, "extrapolate_wall_dummy")
 OPFN(mps_press_gradient, "press_gradient")
 OPFN(mps_wall_press_gradient, "wall_press_gradient")
+OPFN(mps_prediction_press_gradient, "prediction_press_gradient")
+OPFN(mps_prediction_wall_press_gradient, "prediction_wall_press_gradient")
 OPFN(mps_wall_viscosity, "wall_viscosity")
 OPFN(mps_correction, "correction")
 OPFN(mps_shifting, "shifting")

@@ -1189,8 +1189,11 @@ __global__ void kVelDivergence(KArgs a) {
 }
 
 // calcPressGradient (src/MpsPressure.cpp:1217-1338)
-template <bool LIST>
-__global__ void kPressGradient(KArgs a) {
+// calcPressGradient (src/MpsPressure.cpp:1217-1338): acc = relax grad p / rho,
+// and with PRED predictionPressGradient (:932-1028): acc += (1 - relax)
+// grad p / rho without the gradient correction
+template <bool LIST, bool PRED>
+__device__ __forceinline__ void pressGradientBody(const KArgs& a) {
     const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
@@ -1201,7 +1204,7 @@ __global__ void kPressGradient(KArgs a) {
     const double Pi = a.D[DPRESS][i], ni = a.D[DPNDI][i];
     double pmin = Pi;
     double MC[9];
-    for (int q = 0; q < 9; q++) MC[q] = a.D[DCM0 + q][i];
+    for (int q = 0; q < 9; q++) MC[q] = PRED ? 0.0 : a.D[DCM0 + q][i];
     if (gt == 0 || gt == 2)
         forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
             const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
@@ -1229,7 +1232,7 @@ __global__ void kPressGradient(KArgs a) {
                 const double nj = a.D[DPNDI][j];
                 if (ni > KEPS && nj > KEPS) wS *= (ni * pj / nj + nj * Pi / ni) / d2;
             }
-            if (!P.gradient_correction) {
+            if (PRED || !P.gradient_correction) {
                 ax += v0 * wS; ay += v1 * wS; az += v2 * wS;
             } else {
                 ax += (v0 * MC[0] + v1 * MC[1] + v2 * MC[2]) * wS;
@@ -1239,14 +1242,27 @@ __global__ void kPressGradient(KArgs a) {
         }
     });
     const double rho = P.dns_fl1;
-    a.D[DAX][i] = P.relax_press * ax * P.coeff_press_grad / rho;
-    a.D[DAY][i] = P.relax_press * ay * P.coeff_press_grad / rho;
-    a.D[DAZ][i] = P.relax_press * az * P.coeff_press_grad / rho;
+    if (PRED) {
+        const double f = 1.0 - P.relax_press;
+        a.D[DAX][i] = a.D[DAX][i] + f * ax * P.coeff_press_grad / rho;
+        a.D[DAY][i] = a.D[DAY][i] + f * ay * P.coeff_press_grad / rho;
+        a.D[DAZ][i] = a.D[DAZ][i] + f * az * P.coeff_press_grad / rho;
+    } else {
+        a.D[DAX][i] = P.relax_press * ax * P.coeff_press_grad / rho;
+        a.D[DAY][i] = P.relax_press * ay * P.coeff_press_grad / rho;
+        a.D[DAZ][i] = P.relax_press * az * P.coeff_press_grad / rho;
+    }
 }
-
-// calcWallPressGradient + repulsive forces (src/MpsPressure.cpp:1341-1611)
 template <bool LIST>
-__global__ void kWallPressGradient(KArgs a) {
+__global__ void kPressGradient(KArgs a) { pressGradientBody<LIST, false>(a); }
+template <bool LIST>
+__global__ void kPredPressGradient(KArgs a) { pressGradientBody<LIST, true>(a); }
+
+// calcWallPressGradient + repulsive forces (src/MpsPressure.cpp:1341-1611);
+// with PRED predictionWallPressGradient (:1031-1214): factor 1 - relax and
+// no wall force
+template <bool LIST, bool PRED>
+__device__ __forceinline__ void wallPressGradientBody(const KArgs& a) {
     const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
@@ -1315,16 +1331,23 @@ __global__ void kWallPressGradient(KArgs a) {
         }
         f0 = -w * n[0]; f1 = -w * n[1]; f2 = -w * n[2];
     }
-    const double g0 = P.relax_press * (R[0] * ax + R[1] * ay + R[2] * az) * P.coeff_press_grad - f0;
-    const double g1 = P.relax_press * (R[3] * ax + R[4] * ay + R[5] * az) * P.coeff_press_grad - f1;
-    const double g2 = P.relax_press * (R[6] * ax + R[7] * ay + R[8] * az) * P.coeff_press_grad - f2;
+    const double fr = PRED ? (1.0 - P.relax_press) : P.relax_press;
+    const double g0 = fr * (R[0] * ax + R[1] * ay + R[2] * az) * P.coeff_press_grad - f0;
+    const double g1 = fr * (R[3] * ax + R[4] * ay + R[5] * az) * P.coeff_press_grad - f1;
+    const double g2 = fr * (R[6] * ax + R[7] * ay + R[8] * az) * P.coeff_press_grad - f2;
     a.D[DAX][i] = a.D[DAX][i] + g0 / rho;
     a.D[DAY][i] = a.D[DAY][i] + g1 / rho;
     a.D[DAZ][i] = a.D[DAZ][i] + g2 / rho;
-    a.D[DFWX][i] = a.D[DFWX][i] + -g0 * vol;
-    a.D[DFWY][i] = a.D[DFWY][i] + -g1 * vol;
-    a.D[DFWZ][i] = a.D[DFWZ][i] + -g2 * vol;
+    if (!PRED) {
+        a.D[DFWX][i] = a.D[DFWX][i] + -g0 * vol;
+        a.D[DFWY][i] = a.D[DFWY][i] + -g1 * vol;
+        a.D[DFWZ][i] = a.D[DFWZ][i] + -g2 * vol;
+    }
 }
+template <bool LIST>
+__global__ void kWallPressGradient(KArgs a) { wallPressGradientBody<LIST, false>(a); }
+template <bool LIST>
+__global__ void kPredWallPressGradient(KArgs a) { wallPressGradientBody<LIST, true>(a); }
 
 // calcWall{Slip,NoSlip}Viscosity (src/MpsViscosity.cpp:1527-1910)
 template <bool LIST>
@@ -1705,6 +1728,8 @@ int launchOp(Ctx* ctx, const char* op) {
     else if (o == "wall_vel_divergence") LT(kWallVelDivergence);
     else if (o == "extrapolate_wall_dummy") LT(kExtrapolateWallDummy2); else if (o == "press_gradient") LT(kPressGradient);
     else if (o == "wall_press_gradient") LT(kWallPressGradient);
+    else if (o == "prediction_press_gradient") LT(kPredPressGradient);
+    else if (o == "prediction_wall_press_gradient") LT(kPredWallPressGradient);
     else if (o == "wall_viscosity") LT(kWallViscosity);
     else if (o == "correction") {
         unsigned long long* bits = (unsigned long long*)(ctx->red + 3 * ctx->redCap + 8);

@@ -278,11 +278,11 @@ public:
     void extrapolatePressParticlesWallDummy(MpsParticleSystem*, MpsParticle* P, MpsBucket*) {
         polymps_gpu::check(mps_extrapolate_wall_dummy(P->ctx), P->ctx, "extrapolatePressParticlesWallDummy");
     }
-    void predictionPressGradient(MpsParticleSystem*, MpsParticle*, MpsBucket*) {
-        throw MpsGpuError("relax_press < 1 (prediction pressure gradient) is not supported");
+    void predictionPressGradient(MpsParticleSystem*, MpsParticle* P, MpsBucket*) {
+        polymps_gpu::check(mps_prediction_press_gradient(P->ctx), P->ctx, "predictionPressGradient");
     }
-    void predictionWallPressGradient(MpsParticleSystem*, MpsParticle*, MpsBucket*) {
-        throw MpsGpuError("relax_press < 1 (prediction pressure gradient) is not supported");
+    void predictionWallPressGradient(MpsParticleSystem*, MpsParticle* P, MpsBucket*) {
+        polymps_gpu::check(mps_prediction_wall_press_gradient(P->ctx), P->ctx, "predictionWallPressGradient");
     }
     void calcPressGradient(MpsParticleSystem*, MpsParticle* P, MpsBucket*) {
         polymps_gpu::check(mps_press_gradient(P->ctx), P->ctx, "calcPressGradient");

@@ -105,6 +105,10 @@ def lockstep_step(g: Simulation, o: Oracle, step: int, press_rtol=1e-6, log=None
     both("bucket_update", ["update_buckets"], [])
     both("viscosity", ["viscosity"], ["acc"])
     both("add_gravity", ["add_gravity"], ["acc", "acc_star"])
+    if p.relax_press < 1.0:
+        both("prediction_press_gradient", ["prediction_press_gradient"], ["acc"])
+        if poly:
+            both("prediction_wall_press_gradient", ["prediction_wall_press_gradient"], ["acc"])
     both("prediction", ["prediction"], ["pos", "vel", "acc", "riw2", "npcd"])
     both("check_out_domain", ["check_out_domain"], ["type", "pos"])
     if p.collision_type == 0:

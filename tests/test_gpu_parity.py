@@ -107,6 +107,22 @@ def test_lockstep_dam3d_divu(built, slip):
         lockstep_step(g, o, 10 + s)
 
 
+@pytest.mark.parametrize("mk", ["dam3d", "dam2d"])
+def test_lockstep_relaxed_pressure_gradient(built, mk):
+    """relax_fact < 1: the prediction pressure gradient (and its polygon-wall
+    part) before the predictor, the relaxed gradient after the solve."""
+    w = configs.small_dam3d(l0=0.025) if mk == "dam3d" else configs.small_dam2d(l0=0.02)
+    w.params.relax_press = 0.8
+    g, o = new_pair(w)
+    lockstep_init(g, o)
+    for s in range(2):
+        lockstep_step(g, o, s)
+    o.steps(40)
+    sync_state(g, o)
+    for s in range(2):
+        lockstep_step(g, o, 10 + s)
+
+
 def test_lockstep_dam2d_wc(built):
     w = configs.small_dam2d(l0=0.02)
     g, o = new_pair(w)

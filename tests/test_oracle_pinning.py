@@ -523,3 +523,24 @@ def test_wall_velocity_divergence_two_particles(built, slip):
         expect = (3.0 / q.pnd_small_zero) * (nj / ni) * (-2.0 * U * dx) * (q.re_s / r - 1.0) / r2
         assert got == pytest.approx(expect, rel=1e-12)
         assert got != 0.0
+
+
+def test_prediction_gradient_of_linear_pressure_on_lattice(built):
+    """predictionPressGradient adds (1 - relax) of the uncorrected gradient:
+    on the symmetric lattice -(1 - relax) grad p / rho for a linear field, and
+    with the relax part of calcPressGradient the whole gradient."""
+    p, t, x = lattice_block(12, 12, 12)
+    p.relax_press = 0.7
+    P = x.reshape(-1, 3)
+    grad = np.array([300.0, -200.0, 1000.0])
+    o = Oracle(p, t, x)
+    o.set("press", P @ grad + 50.0)
+    o.call("update_buckets")
+    o.set("acc", np.zeros(3 * len(P)))
+    o.call("prediction_press_gradient")
+    pred = o.get("acc").reshape(-1, 3)
+    inn = interior(x, p.part_dist, 4)
+    assert np.allclose(pred[inn], -(1 - 0.7) * grad / p.density_fluid, rtol=1e-9, atol=1e-12)
+    o.call("press_gradient")
+    main = o.get("acc").reshape(-1, 3)
+    assert np.allclose(pred[inn] + main[inn], -grad / p.density_fluid, rtol=1e-9, atol=1e-12)
