@@ -156,6 +156,11 @@ int mps_ppe_solve(mps_ctx* ctx);
 int mps_ppe_matrix(mps_ctx* ctx, int* row_ptr, int* cols, double* vals, int* nnz);
 /* MpsPressure::extrapolatePressParticlesWallDummy (src/MpsPressure.cpp:792-846) */
 int mps_extrapolate_wall_dummy(mps_ctx* ctx);
+/* MpsShifting::calcNormalParticles (src/MpsShifting.cpp:106-175) and
+ * calcWallNormalParticles (:319-430): the free-surface normals of the ARC
+ * rule (free_surface_threshold.type 2) */
+int mps_normal_particles(mps_ctx* ctx);
+int mps_wall_normal_particles(mps_ctx* ctx);
 /* MpsPressure::predictionPressGradient (src/MpsPressure.cpp:932-1028) and
  * predictionWallPressGradient (:1031-1214): acc += (1 - relax_fact) grad p / rho
  * before the predictor when relax_fact < 1 */

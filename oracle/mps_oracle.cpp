@@ -1295,11 +1295,147 @@ void meanNeighFluidPnd(Sys& s) {
     }
 }
 
+// ----------------------------------------------------- ARC normals
+// calcNormalParticles (src/MpsShifting.cpp:106-175): n_i = -sum_j r_ij w / |r_ij| / n0S
+void normalParticles(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        double dx = 0.0, dy = 0.0, dz = 0.0;
+        walk(s, xi, yi, zi, [&](int j) {
+            const double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
+            const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            const double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
+            const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) {
+                const double dst = sqrt(d2);
+                const double wS = weight(dst, P.re_s, P.weight_type);
+                dx += v0 * wS / dst;
+                dy += v1 * wS / dst;
+                dz += v2 * wS / dst;
+            }
+        });
+        s.normal[i * 3] = -dx / P.pnd_small_zero;
+        s.normal[i * 3 + 1] = -dy / P.pnd_small_zero;
+        s.normal[i * 3 + 2] = -dz / P.pnd_small_zero;
+    }
+}
+
+// calcWallNormalParticles (src/MpsShifting.cpp:319-430): the mirror
+// neighbourhood's part, reflected back by R, for near-wall fluid particles
+void wallNormalParticles(Sys& s) {
+    const mps_params& P = s.p;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        if (!(s.type[i] == P.fluid && s.nearWall[i])) continue;
+        const double* xi = &s.pos[i * 3];
+        const double* xm = &s.mirror[i * 3];
+        double n[3], R[9], ns;
+        reflection(xi, xm, n, R, &ns);
+        double dx = 0.0, dy = 0.0, dz = 0.0;
+        walk(s, xi[0], xi[1], xi[2], [&](int j) {
+            const double v0 = PJ(s, j, 0) - xi[0], v1 = PJ(s, j, 1) - xi[1], v2 = PJ(s, j, 2) - xi[2];
+            const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            const double w0 = PJ(s, j, 0) - xm[0], w1 = PJ(s, j, 1) - xm[1], w2 = PJ(s, j, 2) - xm[2];
+            const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2 && j != i) {
+                const double dst = sqrt(dm2);
+                const double wS = weight(dst, P.re_s, P.weight_type);
+                dx += w0 * wS / dst;
+                dy += w1 * wS / dst;
+                dz += w2 * wS / dst;
+            }
+        });
+        const double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (dm2 < P.re_s2) {
+            const double dst = sqrt(dm2);
+            const double wS = weight(dst, P.re_s, P.weight_type);
+            dx += w0 * wS / dst;
+            dy += w1 * wS / dst;
+            dz += w2 * wS / dst;
+        }
+        const double rx = R[0] * dx + R[1] * dy + R[2] * dz;
+        const double ry = R[3] * dx + R[4] * dy + R[5] * dz;
+        const double rz = R[6] * dx + R[7] * dy + R[8] * dz;
+        s.normal[i * 3] += -rx / P.pnd_small_zero;
+        s.normal[i * 3 + 1] += -ry / P.pnd_small_zero;
+        s.normal[i * 3 + 2] += -rz / P.pnd_small_zero;
+    }
+}
+
 // ----------------------------------------------- MpsBoundaryCondition
-// updateParticleBC (src/MpsBoundaryCondition.cpp:66-400), PND / NEIGH / NPCD rules
+// The ARC rule of updateParticleBC (src/MpsBoundaryCondition.cpp:205-392): a
+// particle is inner if, in walk order, a neighbour j (or, with polygon walls,
+// j seen from the mirror point) either lies far (r >= sqrt(2) 1.33 l0) but
+// close to the point one l0 along the normal, or near with an angle to the
+// normal below thetaArc; the first such j ends the walk.  Otherwise the last
+// neighbour examined leaves the particle at the surface.
+int arcRule(const Sys& s, int i, int bc) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+    double nx = s.normal[i * 3], ny = s.normal[i * 3 + 1], nz = s.normal[i * 3 + 2];
+    const double norm2 = nx * nx + ny * ny + nz * nz;
+    if (s.pndi[i] >= P.beta_pnd && s.numNeigh[i] >= P.beta_neigh && norm2 <= P.norm_threshold2) return P.inner;
+    const double* xi = &s.pos[i * 3];
+    const double* xm = &s.mirror[i * 3];
+    if (norm2 > kEps) {
+        const double nn = sqrt(norm2);
+        nx /= nn; ny /= nn; nz /= nn;
+    } else {
+        nx = 0.0; ny = 0.0; nz = 0.0;
+    }
+    double n[3], R[9];
+    reflectionMod2(xi, xm, n, R);
+    double mx = R[0] * nx + R[1] * ny + R[2] * nz;
+    double my = R[3] * nx + R[4] * ny + R[5] * nz;
+    double mz = R[6] * nx + R[7] * ny + R[8] * nz;
+    const double mn2 = mx * mx + my * my + mz * mz;
+    if (mn2 > kEps) {
+        const double mn = sqrt(mn2);
+        mx /= mn; my /= mn; mz /= mn;
+    } else {
+        mx = 0.0; my = 0.0; mz = 0.0;
+    }
+    const double l0 = P.part_dist;
+    const double ax = xi[0] + l0 * nx, ay = xi[1] + l0 * ny, az = xi[2] + l0 * nz;
+    const double bx = xm[0] + l0 * mx, by = xm[1] + l0 * my, bz = xm[2] + l0 * mz;
+    bool done = false;
+    walk(s, xi[0], xi[1], xi[2], [&](int j) {
+        if (done) return;
+        const double v0 = PJ(s, j, 0) - xi[0], v1 = PJ(s, j, 1) - xi[1], v2 = PJ(s, j, 2) - xi[2];
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = PJ(s, j, 0) - xm[0], w1 = PJ(s, j, 1) - xm[1], w2 = PJ(s, j, 2) - xm[2];
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_s2 && (d2 < dm2 || pw) && j != i) {
+            const double u0 = PJ(s, j, 0) - ax, u1 = PJ(s, j, 1) - ay, u2 = PJ(s, j, 2) - az;
+            const double dn2 = u0 * u0 + u1 * u1 + u2 * u2;
+            const double rn = v0 * nx + v1 * ny + v2 * nz;
+            const double ang = acos(rn / sqrt(d2));
+            if (d2 >= P.dst_threshold2 && dn2 < P.h_threshold2) { bc = P.inner; done = true; return; }
+            else if (d2 < P.dst_threshold2 && ang < P.theta_arc) { bc = P.inner; done = true; return; }
+            else bc = P.surface;
+        }
+        if (!pw && d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2 && j != i) {
+            const double u0 = PJ(s, j, 0) - bx, u1 = PJ(s, j, 1) - by, u2 = PJ(s, j, 2) - bz;
+            const double dn2 = u0 * u0 + u1 * u1 + u2 * u2;
+            const double rn = w0 * mx + w1 * my + w2 * mz;
+            const double ang = acos(rn / sqrt(dm2));
+            if (dm2 >= P.dst_threshold2 && dn2 < P.h_threshold2) { bc = P.inner; done = true; return; }
+            else if (dm2 < P.dst_threshold2 && ang < P.theta_arc) { bc = P.inner; done = true; return; }
+            else bc = P.surface;
+        }
+    });
+    return bc;
+}
+
+// updateParticleBC (src/MpsBoundaryCondition.cpp:66-400): PND / NEIGH / NPCD / ARC rules
 int updateBC(Sys& s) {
     const mps_params& P = s.p;
-    if (P.free_surf_type == MPS_FS_PND_ARC) { s.err = "ARC free-surface rule is not supported"; return -1; }
+#pragma omp parallel for schedule(dynamic, 64)
     for (int i = 0; i < s.N; i++) {
         if (s.type[i] == P.dummy_wall) { s.bc[i] = P.other; continue; }
         if (s.pndSmall[i] < P.beta_pnd) s.bc[i] = P.surface;
@@ -1310,6 +1446,8 @@ int updateBC(Sys& s) {
         } else if (P.free_surf_type == MPS_FS_PND_NPCD) {
             if (s.bc[i] == P.surface)
                 if (s.numNeigh[i] > 4 && s.npcd2[i] < P.delta2) s.bc[i] = P.inner;
+        } else if (P.free_surf_type == MPS_FS_PND_ARC) {
+            s.bc[i] = arcRule(s, i, s.bc[i]);
         }
     }
     return 0;
@@ -2065,6 +2203,10 @@ int initSystem(Sys& s) {
     if (P.wall_type == MPS_WALL_POLYGON) meanWallPnd(s);
     meanPnd(s);
     meanNeighFluidPnd(s);
+    if (P.free_surf_type == MPS_FS_PND_ARC) {
+        normalParticles(s);
+        if (P.wall_type == MPS_WALL_POLYGON) wallNormalParticles(s);
+    }
     if (updateBC(s)) return -1;
     if (calcPress(s)) return -1;
     return 0;
@@ -2095,6 +2237,10 @@ int stepSystem(Sys& s) {
     } else if (P.pnd_type == MPS_PND_MEAN_SUM_WIJ) {
         if (P.wall_type == MPS_WALL_POLYGON) meanWallPnd(s);
         meanPnd(s);
+    }
+    if (P.free_surf_type == MPS_FS_PND_ARC) {
+        normalParticles(s);
+        if (P.wall_type == MPS_WALL_POLYGON) wallNormalParticles(s);
     }
     if (updateBC(s)) return -1;
     if (calcPress(s)) return -1;
@@ -2128,6 +2274,7 @@ void* fieldPtr(Sys& s, const std::string& f, int& kind, int& comps) {
     D3("pos", s.pos) D3("vel", s.vel) D3("acc", s.acc) D3("acc_star", s.accStar)
     D3("npcd", s.npcd) D3("grad_conc", s.gradC) D3("cm1", s.cm1) D3("cm2", s.cm2) D3("cm3", s.cm3)
     D3("wall_pos", s.wallPos) D3("mirror", s.mirror) D3("polygon_normal", s.polyN) D3("force_wall", s.forceWall)
+    D3("normal", s.normal)
     D3("dvel", s.dvel)
     D1("press", s.press) D1("press_avg", s.pressAvg) D1("pndi", s.pndi) D1("pndki", s.pndki) D1("pndski", s.pndski)
     D1("pnd_small", s.pndSmall) D1("npcd2", s.npcd2) D1("conc", s.conc) D1("div_u", s.divU)
@@ -2318,6 +2465,8 @@ int orc_call(void* h, const char* op) {
     else if (o == "vel_divergence") velDivergence(*s);
     else if (o == "wall_vel_divergence") wallVelDivergence(*s);
     else if (o == "prediction_press_gradient") predictionPressGradient(*s);
+    else if (o == "normal_particles") normalParticles(*s);
+    else if (o == "wall_normal_particles") wallNormalParticles(*s);
     else if (o == "prediction_wall_press_gradient") predictionWallPressGradient(*s);
     else if (o == "assemble_ppe") assemblePPE(*s);
     else if (o == "solve_ppe") { solveCG(*s); for (int i = 0; i < s->N; i++) s->press[i] = s->ppeX[i] < 0 ? 0.0 : s->ppeX[i]; }

@@ -15,7 +15,7 @@ _PATH = Path(__file__).resolve().parent / "liboracle_mps.so"
 _lib = None
 
 VEC = {"pos", "vel", "acc", "acc_star", "npcd", "grad_conc", "cm1", "cm2", "cm3", "wall_pos", "mirror",
-       "polygon_normal", "force_wall", "dvel"}
+       "polygon_normal", "force_wall", "dvel", "normal"}
 INT = {"type", "bc", "num_neigh", "num_neigh_wall", "ptype", "near_mesh_type"}
 
 

@@ -59,7 +59,7 @@ struct FieldDesc {
 static const FieldDesc kFields[] = {
     {"pos", 1, DX}, {"vel", 1, DU}, {"acc", 1, DAX}, {"acc_star", 1, DASX}, {"npcd", 1, DNPX},
     {"mirror", 1, DMX}, {"wall_pos", 1, DWPX}, {"polygon_normal", 1, DPNX}, {"force_wall", 1, DFWX},
-    {"cm1", 1, DCM0}, {"cm2", 1, DCM3}, {"cm3", 1, DCM6}, {"grad_conc", 1, DGCX},
+    {"cm1", 1, DCM0}, {"cm2", 1, DCM3}, {"cm3", 1, DCM6}, {"grad_conc", 1, DGCX}, {"normal", 1, DNRMX},
     {"press", 0, DPRESS}, {"press_avg", 0, DPAVG}, {"pndi", 0, DPNDI}, {"pndki", 0, DPNDKI},
     {"pndski", 0, DPNDSKI}, {"pnd_small", 0, DPNDSMALL}, {"npcd2", 0, DNPCD2}, {"conc", 0, DCONC},
     {"div_u", 0, DDIVU}, {"diffusive", 0, DDIFF}, {"pnd_wall", 0, DPNDWALL}, {"riw2", 0, DRIW2},
@@ -195,6 +195,14 @@ static int syncStatus(Ctx* ctx) {
     return MPS_OK;
 }
 
+// calcNormalParticles, then the polygon-wall part (src/main.cpp order before
+// updateParticleBC when the ARC rule is used)
+static int arcNormals(Ctx* ctx) {
+    int rc = launchOp(ctx, "normal_particles");
+    if (rc == MPS_OK && ctx->P.wall_type == MPS_WALL_POLYGON) rc = launchOp(ctx, "wall_normal_particles");
+    return rc;
+}
+
 // calcVelDivergence, then the polygon-wall term (src/MpsPressure.cpp:46-53)
 static int velDivergence(Ctx* ctx) {
     int rc = launchOp(ctx, "vel_divergence");
@@ -218,6 +226,7 @@ static int runInit(Ctx* ctx) {
     if ((rc = launchOp(ctx, "initial_pnd"))) return rc;
     if ((rc = launchOp(ctx, "mean_pnd"))) return rc;
     if ((rc = launchOp(ctx, "mean_neigh_fluid_pnd"))) return rc;
+    if (P.free_surf_type == MPS_FS_PND_ARC && (rc = arcNormals(ctx))) return rc;
     if ((rc = launchOp(ctx, "update_bc"))) return rc;
     if ((rc = syncStatus(ctx))) return rc;
     if (P.mps_type == MPS_EXPLICIT || P.mps_type == MPS_WEAKLY) {
@@ -277,6 +286,7 @@ static int runStep(Ctx* ctx) {
     OP("pnd_neigh_npcd");
     if (P.pnd_type == MPS_PND_DIFFUSIVE) OP("pnd_diffusive");
     else if (P.pnd_type == MPS_PND_MEAN_SUM_WIJ) OP("mean_pnd");
+    if (P.free_surf_type == MPS_FS_PND_ARC && (rc = arcNormals(ctx))) return rc;
     OP("update_bc");
     if ((rc = calcPress(ctx))) return rc;
     if (P.wall_type == MPS_WALL_PARTICLE) OP("extrapolate_wall_dummy");
@@ -777,6 +787,8 @@ OPFN(mps_extrapolate_wall_dummy, "extrapolate_wall_dummy")
 OPFN(mps_press_gradient, "press_gradient")
 OPFN(mps_wall_press_gradient, "wall_press_gradient")
 OPFN(mps_prediction_press_gradient, "prediction_press_gradient")
+OPFN(mps_normal_particles, "normal_particles")
+OPFN(mps_wall_normal_particles, "wall_normal_particles")
 OPFN(mps_prediction_wall_press_gradient, "prediction_wall_press_gradient")
 OPFN(mps_wall_viscosity, "wall_viscosity")
 OPFN(mps_correction, "correction")

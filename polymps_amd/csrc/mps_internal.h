@@ -30,7 +30,7 @@ enum DField {
     // per step
     DAX = NPD, DAY, DAZ, DASX, DASY, DASZ, DNPX, DNPY, DNPZ, DNPCD2, DPNDSMALL, DCONC,
     DAUX1, DAUX2, DPNDWALL, DRIW2, DDEVN, DWPX, DWPY, DWPZ, DPNX, DPNY, DPNZ,
-    DCM0, DCM1, DCM2, DCM3, DCM4, DCM5, DCM6, DCM7, DCM8, DDVX, DDVY, DDVZ, DPPEB,
+    DCM0, DCM1, DCM2, DCM3, DCM4, DCM5, DCM6, DCM7, DCM8, DDVX, DDVY, DDVZ, DPPEB, DNRMX, DNRMY, DNRMZ,
     ND
 };
 // Per-particle int fields; the first NPI are persistent.

@@ -1117,7 +1117,143 @@ __global__ void kMeanNeighFluidPnd(KArgs a) {
     a.D[DPNDSKI][i] = (dn > KEPS) ? up / dn : up;
 }
 
+// calcNormalParticles (src/MpsShifting.cpp:106-175): n_i = -sum_j r_ij w / |r_ij| / n0S
+template <bool LIST>
+__global__ void kNormalParticles(KArgs a) {
+    const int i = slotId();
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
+    double dx = 0.0, dy = 0.0, dz = 0.0;
+    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_s2 && (d2 < dm2 || pw)) {
+            const double dst = sqrt(d2);
+            const double wS = wgt(dst, P.re_s, P.weight_type);
+            dx += v0 * wS / dst;
+            dy += v1 * wS / dst;
+            dz += v2 * wS / dst;
+        }
+    });
+    a.D[DNRMX][i] = -dx / P.pnd_small_zero;
+    a.D[DNRMY][i] = -dy / P.pnd_small_zero;
+    a.D[DNRMZ][i] = -dz / P.pnd_small_zero;
+}
+
+// calcWallNormalParticles (src/MpsShifting.cpp:319-430): the mirror
+// neighbourhood's part, reflected back by R
+template <bool LIST>
+__global__ void kWallNormalParticles(KArgs a) {
+    const int i = slotId();
+    if (!live(a, i)) return;
+    const mps_params& P = a.P;
+    if (!(a.I[ITYPE][i] == P.fluid && a.I[INEARWALL][i])) return;
+    const double xi[3] = {PX(i), PY(i), PZ(i)};
+    const double xm[3] = {a.D[DMX][i], a.D[DMY][i], a.D[DMZ][i]};
+    double n[3], R[9], ns;
+    reflectSqrt(xi, xm, n, R, &ns);
+    double dx = 0.0, dy = 0.0, dz = 0.0;
+    forNbrsPos<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
+        const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2) {
+            const double dst = sqrt(dm2);
+            const double wS = wgt(dst, P.re_s, P.weight_type);
+            dx += w0 * wS / dst;
+            dy += w1 * wS / dst;
+            dz += w2 * wS / dst;
+        }
+    });
+    const double w0 = xi[0] - xm[0], w1 = xi[1] - xm[1], w2 = xi[2] - xm[2];
+    const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+    if (dm2 < P.re_s2) {
+        const double dst = sqrt(dm2);
+        const double wS = wgt(dst, P.re_s, P.weight_type);
+        dx += w0 * wS / dst;
+        dy += w1 * wS / dst;
+        dz += w2 * wS / dst;
+    }
+    const double rx = R[0] * dx + R[1] * dy + R[2] * dz;
+    const double ry = R[3] * dx + R[4] * dy + R[5] * dz;
+    const double rz = R[6] * dx + R[7] * dy + R[8] * dz;
+    a.D[DNRMX][i] = a.D[DNRMX][i] + -rx / P.pnd_small_zero;
+    a.D[DNRMY][i] = a.D[DNRMY][i] + -ry / P.pnd_small_zero;
+    a.D[DNRMZ][i] = a.D[DNRMZ][i] + -rz / P.pnd_small_zero;
+}
+
+// The ARC rule of updateParticleBC (src/MpsBoundaryCondition.cpp:205-392):
+// in walk order, the first neighbour j (or j seen from the mirror point with
+// polygon walls) that lies far but close to the point one l0 along the
+// normal, or near within thetaArc of the normal, makes the particle inner and
+// ends the walk; otherwise every neighbour examined leaves it at the surface.
+template <bool LIST>
+__device__ int arcRule(const KArgs& a, int i, int bc) {
+    const mps_params& P = a.P;
+    const bool pw = P.wall_type == MPS_WALL_PARTICLE;
+    double nx = a.D[DNRMX][i], ny = a.D[DNRMY][i], nz = a.D[DNRMZ][i];
+    const double norm2 = nx * nx + ny * ny + nz * nz;
+    if (a.D[DPNDI][i] >= P.beta_pnd && a.I[INNEIGH][i] >= P.beta_neigh && norm2 <= P.norm_threshold2) return P.inner;
+    const double xi[3] = {PX(i), PY(i), PZ(i)};
+    const double xm[3] = {a.D[DMX][i], a.D[DMY][i], a.D[DMZ][i]};
+    if (norm2 > KEPS) {
+        const double nn = sqrt(norm2);
+        nx /= nn; ny /= nn; nz /= nn;
+    } else {
+        nx = 0.0; ny = 0.0; nz = 0.0;
+    }
+    double n[3], R[9];
+    reflectMod2(xi, xm, n, R);
+    double mx = R[0] * nx + R[1] * ny + R[2] * nz;
+    double my = R[3] * nx + R[4] * ny + R[5] * nz;
+    double mz = R[6] * nx + R[7] * ny + R[8] * nz;
+    const double mn2 = mx * mx + my * my + mz * mz;
+    if (mn2 > KEPS) {
+        const double mn = sqrt(mn2);
+        mx /= mn; my /= mn; mz /= mn;
+    } else {
+        mx = 0.0; my = 0.0; mz = 0.0;
+    }
+    const double l0 = P.part_dist;
+    const double ax = xi[0] + l0 * nx, ay = xi[1] + l0 * ny, az = xi[2] + l0 * nz;
+    const double bx = xm[0] + l0 * mx, by = xm[1] + l0 * my, bz = xm[2] + l0 * mz;
+    bool done = false;
+    forNbrsPos<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
+        if (done) return;
+        const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
+        const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+        if (d2 < P.re_s2 && (d2 < dm2 || pw)) {
+            const double u0 = xj - ax, u1 = yj - ay, u2 = zj - az;
+            const double dn2 = u0 * u0 + u1 * u1 + u2 * u2;
+            const double rn = v0 * nx + v1 * ny + v2 * nz;
+            const double ang = acos(rn / sqrt(d2));
+            if (d2 >= P.dst_threshold2 && dn2 < P.h_threshold2) { bc = P.inner; done = true; return; }
+            else if (d2 < P.dst_threshold2 && ang < P.theta_arc) { bc = P.inner; done = true; return; }
+            else bc = P.surface;
+        }
+        if (!pw && d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2) {
+            const double u0 = xj - bx, u1 = yj - by, u2 = zj - bz;
+            const double dn2 = u0 * u0 + u1 * u1 + u2 * u2;
+            const double rn = w0 * mx + w1 * my + w2 * mz;
+            const double ang = acos(rn / sqrt(dm2));
+            if (dm2 >= P.dst_threshold2 && dn2 < P.h_threshold2) { bc = P.inner; done = true; return; }
+            else if (dm2 < P.dst_threshold2 && ang < P.theta_arc) { bc = P.inner; done = true; return; }
+            else bc = P.surface;
+        }
+    });
+    return bc;
+}
+
 // MpsBoundaryCondition::updateParticleBC (src/MpsBoundaryCondition.cpp:66-400)
+template <bool LIST>
 __global__ void kUpdateBC(KArgs a) {
     const int i = slotId();
     if (!live(a, i)) return;
@@ -1130,6 +1266,8 @@ __global__ void kUpdateBC(KArgs a) {
         bc = (small < P.beta_pnd && nn < P.beta_neigh) ? P.surface : P.inner;
     } else if (P.free_surf_type == MPS_FS_PND_NPCD) {
         if (bc == P.surface && nn > 4 && a.D[DNPCD2][i] < P.delta2) bc = P.inner;
+    } else if (P.free_surf_type == MPS_FS_PND_ARC) {
+        bc = arcRule<LIST>(a, i, bc);
     }
     a.I[IBC][i] = bc;
 }
@@ -1720,10 +1858,10 @@ int launchOp(Ctx* ctx, const char* op) {
         LT(kMeanPnd);
         L1(kMeanPndFinish);
     } else if (o == "mean_neigh_fluid_pnd") LT(kMeanNeighFluidPnd);
-    else if (o == "update_bc") {
-        if (P.free_surf_type == MPS_FS_PND_ARC) { ctx->err = "ARC free-surface rule is not supported"; return MPS_E_UNSUPPORTED; }
-        L1(kUpdateBC);
-    } else if (o == "press_explicit") L1(kPressExplicit);
+    else if (o == "update_bc") LT(kUpdateBC);
+    else if (o == "normal_particles") LT(kNormalParticles);
+    else if (o == "wall_normal_particles") LT(kWallNormalParticles);
+    else if (o == "press_explicit") L1(kPressExplicit);
     else if (o == "vel_divergence") LT(kVelDivergence);
     else if (o == "wall_vel_divergence") LT(kWallVelDivergence);
     else if (o == "extrapolate_wall_dummy") LT(kExtrapolateWallDummy2); else if (o == "press_gradient") LT(kPressGradient);

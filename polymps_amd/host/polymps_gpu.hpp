@@ -346,11 +346,11 @@ public:
         polymps_gpu::check(mps_shifting(P->ctx), P->ctx, "calcConcAndConcGradient");
     }
     void calcWallConcAndConcGradient(MpsParticleSystem*, MpsParticle*, MpsBucket*) {}
-    void calcNormalParticles(MpsParticleSystem*, MpsParticle*, MpsBucket*) {
-        throw MpsGpuError("the ARC free-surface rule (particle normals) is not supported");
+    void calcNormalParticles(MpsParticleSystem*, MpsParticle* P, MpsBucket*) {
+        polymps_gpu::check(mps_normal_particles(P->ctx), P->ctx, "calcNormalParticles");
     }
-    void calcWallNormalParticles(MpsParticleSystem*, MpsParticle*, MpsBucket*) {
-        throw MpsGpuError("the ARC free-surface rule (particle normals) is not supported");
+    void calcWallNormalParticles(MpsParticleSystem*, MpsParticle* P, MpsBucket*) {
+        polymps_gpu::check(mps_wall_normal_particles(P->ctx), P->ctx, "calcWallNormalParticles");
     }
 };
 

@@ -19,14 +19,15 @@ _LIB_PATH = Path(__file__).resolve().parent / "libmps_gpu.so"
 
 _OPS = [
     "bucket_update", "viscosity", "add_gravity", "prediction", "check_out_domain", "build_neighbor_list",
-    "prediction_press_gradient", "prediction_wall_press_gradient", "collision", "mesh_closest_point", "mesh_update_near", "wall_npcd", "correction_matrix", "initial_pnd",
+    "prediction_press_gradient", "prediction_wall_press_gradient", "collision", "mesh_closest_point",
+    "mesh_update_near", "wall_npcd", "correction_matrix", "initial_pnd",
     "pnd_neigh_npcd", "pnd_diffusive", "mean_pnd", "mean_neigh_fluid_pnd", "update_bc", "calc_press",
     "vel_divergence", "ppe_assemble", "ppe_solve", "extrapolate_wall_dummy", "press_gradient", "wall_press_gradient",
-    "wall_viscosity", "correction", "shifting", "wall_dummy_velocity", "init", "step",
+    "normal_particles", "wall_normal_particles", "wall_viscosity", "correction", "shifting", "wall_dummy_velocity", "init", "step",
 ]
 
 VEC_FIELDS = {"pos", "vel", "acc", "acc_star", "npcd", "mirror", "wall_pos", "polygon_normal", "force_wall",
-              "cm1", "cm2", "cm3", "grad_conc"}
+              "cm1", "cm2", "cm3", "grad_conc", "normal"}
 INT_FIELDS = {"type", "bc", "num_neigh", "num_neigh_wall", "near_wall", "near_mesh_type"}
 
 

@@ -129,6 +129,10 @@ def lockstep_step(g: Simulation, o: Oracle, step: int, press_rtol=1e-6, log=None
         both("pnd_diffusive", ops, ["pndi", "div_u", "diffusive"])
     elif p.pnd_type == 1:
         both("mean_pnd", (["mean_wall_pnd"] if poly else []) + ["mean_pnd"], ["pndi"])
+    if p.free_surf_type == 2:
+        both("normal_particles", ["normal_particles"], ["normal"])
+        if poly:
+            both("wall_normal_particles", ["wall_normal_particles"], ["normal"])
     both("update_bc", ["update_bc"], ["bc"])
     if p.mps_type in (0, 1):
         g.call("calc_press")

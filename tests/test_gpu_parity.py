@@ -39,6 +39,11 @@ def lockstep_init(g, o):
     check(g, o, ["pndi"], "init mean_pnd")
     g.call("mean_neigh_fluid_pnd"); o.call("mean_neigh_fluid_pnd")
     check(g, o, ["pndski"], "init mean_neigh_fluid_pnd")
+    if p.free_surf_type == 2:
+        g.call("normal_particles"); o.call("normal_particles")
+        if poly:
+            g.call("wall_normal_particles"); o.call("wall_normal_particles")
+        check(g, o, ["normal"], "init normals")
     g.call("update_bc"); o.call("update_bc")
     check(g, o, ["bc"], "init update_bc")
     if p.mps_type >= 2:
@@ -113,6 +118,22 @@ def test_lockstep_relaxed_pressure_gradient(built, mk):
     part) before the predictor, the relaxed gradient after the solve."""
     w = configs.small_dam3d(l0=0.025) if mk == "dam3d" else configs.small_dam2d(l0=0.02)
     w.params.relax_press = 0.8
+    g, o = new_pair(w)
+    lockstep_init(g, o)
+    for s in range(2):
+        lockstep_step(g, o, s)
+    o.steps(40)
+    sync_state(g, o)
+    for s in range(2):
+        lockstep_step(g, o, 10 + s)
+
+
+@pytest.mark.parametrize("mk", ["dam3d", "dam2d"])
+def test_lockstep_arc_free_surface(built, mk):
+    """ARC free-surface rule (type 2): particle normals (with the polygon-wall
+    part) and the angle test with its early exit, in walk order."""
+    w = configs.small_dam3d(l0=0.025) if mk == "dam3d" else configs.small_dam2d(l0=0.02)
+    w.params.free_surf_type = 2
     g, o = new_pair(w)
     lockstep_init(g, o)
     for s in range(2):

@@ -544,3 +544,28 @@ def test_prediction_gradient_of_linear_pressure_on_lattice(built):
     o.call("press_gradient")
     main = o.get("acc").reshape(-1, 3)
     assert np.allclose(pred[inn] + main[inn], -grad / p.density_fluid, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_arc_rule_on_lattice_block(built, dim):
+    """ARC free-surface rule on a bare lattice block: interior normals vanish
+    (symmetric neighbourhood), particles on the outer faces see no neighbour
+    within thetaArc of their outward normal and become surface, and a
+    particle one layer inside a face (away from edges) has the face particle
+    straight above it at l0 (angle 0) and becomes inner."""
+    p, t, x = lattice_block(10, 10, 10 if dim == 3 else 1, dim=dim, re_l=2.1 if dim == 2 else 3.1)
+    p.free_surf_type = 2
+    o = Oracle(p, t, x)
+    for op in ["update_buckets", "pnd_neigh_npcd", "normal_particles", "update_bc"]:
+        o.call(op)
+    q = o.params
+    inn = interior(x, p.part_dist, 4, dim)
+    assert np.max(np.abs(o.get("normal").reshape(-1, 3)[inn])) < 1e-12 * q.pnd_small_zero
+    face = ~interior(x, p.part_dist, 0.5, dim)
+    bc = o.get("bc")
+    assert np.all(bc[face] == q.surface)
+    P = x.reshape(-1, 3)[:, :dim] / p.part_dist
+    depth = np.minimum(P - P.min(0), P.max(0) - P).round().astype(int)
+    deep = np.all(depth >= 2, axis=1)
+    below_face = (np.sum(depth == 1, axis=1) == 1) & (np.sum(depth >= 2, axis=1) == dim - 1)
+    assert below_face.any() and np.all(bc[deep | below_face] == q.inner)
