@@ -72,7 +72,7 @@ typedef struct mps_params {
     double relax_press;         /* gradient.relax_fact                */
     double sound_speed;         /* explicit_mps.equation_state.speed_sound */
     double gamma;               /* explicit_mps.equation_state.gamma  */
-    int solver_type;            /* semi_implicit_mps.solver_type      */
+    int solver_type;            /* semi_implicit_mps.solver_type (0 CG, 1 BiCGSTAB) */
     int shifting_type;          /* particle_shifting.type             */
     double alpha_compressibility; /* semi_implicit_mps.weak_compressibility.alpha */
     double relax_pnd;           /* semi_implicit_mps.source_term.relax_pnd */

@@ -1683,30 +1683,113 @@ void assemblePPE(Sys& s) {
 // row-major transpose wrapper for UpLo = Lower|Upper on a column-major
 // matrix), else A.  Products accumulate in ascending column index of the
 // iterated matrix, as a compressed sparse traversal does.
+// The iterated matrix in row form with ascending columns: A^T (what
+// Eigen's Lower|Upper CG uses on the row-filled matrix) or A.
+void iterMatrix(const Sys& s, bool transpose, std::vector<int>& rp, std::vector<int>& ci, std::vector<double>& va,
+                std::vector<double>& diag) {
+    const int n = s.N;
+    rp.assign(n + 1, 0);
+    ci.clear();
+    va.clear();
+    diag.assign(n, 0.0);
+    std::vector<std::vector<std::pair<int, double>>> rows(n);
+    for (int r = 0; r < n; r++)
+        for (int k = s.rowPtr[r]; k < s.rowPtr[r + 1]; k++) {
+            int c = s.colIdx[k];
+            double v = s.vals[k];
+            if (c == r) diag[r] = v;
+            if (transpose) rows[c].push_back({r, v});
+            else rows[r].push_back({c, v});
+        }
+    for (int r = 0; r < n; r++) {
+        std::stable_sort(rows[r].begin(), rows[r].end(),
+                         [](const std::pair<int, double>& a, const std::pair<int, double>& b) { return a.first < b.first; });
+        rp[r + 1] = rp[r] + (int)rows[r].size();
+        for (auto& e : rows[r]) { ci.push_back(e.first); va.push_back(e.second); }
+    }
+}
+
+// Eigen BiCGSTAB with its default DiagonalPreconditioner, restated from the
+// published algorithm (Eigen/src/IterativeLinearSolvers/BiCGSTAB.h, not in
+// the reference tree): restart on rho below eps^2 |r0|^2 (the first restart
+// also resets the iteration count), stop on |r|^2 <= tol^2 |b|^2 or 2N
+// iterations; iterates with A itself (MpsPressure::
+// solveBiConjugateGradientStabilized, src/MpsPressure.cpp:448-468).
+void solveBiCGSTAB(Sys& s) {
+    const mps_params& P = s.p;
+    const int n = s.N;
+    std::vector<int> rp, ci;
+    std::vector<double> va, diag;
+    iterMatrix(s, false, rp, ci, va, diag);
+    auto spmv = [&](const std::vector<double>& x, std::vector<double>& y) {
+#pragma omp parallel for
+        for (int r = 0; r < n; r++) {
+            double acc = 0.0;
+            for (int k = rp[r]; k < rp[r + 1]; k++) acc += va[k] * x[ci[k]];
+            y[r] = acc;
+        }
+    };
+    auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {
+        double acc = 0.0;
+        for (int r = 0; r < n; r++) acc += a[r] * b[r];
+        return acc;
+    };
+    std::vector<double> invd(n);
+    for (int r = 0; r < n; r++) invd[r] = (diag[r] != 0.0) ? 1.0 / diag[r] : 1.0;
+    std::vector<double>& x = s.ppeX;
+    if ((int)x.size() < n) x.resize(n, 0.0);
+    const std::vector<double>& b = s.ppeB;
+    const double tol = P.solver_tol > 0 ? P.solver_tol : DBL_EPSILON;
+    const int maxIt = P.solver_max_iter > 0 ? P.solver_max_iter : 2 * n;
+    std::vector<double> r(n), r0(n), tmp(n), v(n, 0.0), p(n, 0.0), y(n), z(n), sv(n), t(n);
+    spmv(x, tmp);
+    for (int k = 0; k < n; k++) r[k] = b[k] - tmp[k];
+    r0 = r;
+    double r0n2 = dot(r0, r0);
+    const double rhsN2 = dot(b, b);
+    if (rhsN2 == 0) {
+        for (int k = 0; k < n; k++) x[k] = 0.0;
+        s.solverIter = 0; s.solverErr = 0.0;
+        return;
+    }
+    double rho = 1.0, alpha = 1.0, w = 1.0;
+    const double tol2 = tol * tol * rhsN2;
+    const double eps2 = DBL_EPSILON * DBL_EPSILON;
+    int i = 0, restarts = 0;
+    while (dot(r, r) > tol2 && i < maxIt) {
+        const double rhoOld = rho;
+        rho = dot(r0, r);
+        if (std::fabs(rho) < eps2 * r0n2) {
+            spmv(x, tmp);
+            for (int k = 0; k < n; k++) r[k] = b[k] - tmp[k];
+            r0 = r;
+            rho = r0n2 = dot(r, r);
+            if (restarts++ == 0) i = 0;
+        }
+        const double beta = (rho / rhoOld) * (alpha / w);
+        for (int k = 0; k < n; k++) p[k] = r[k] + beta * (p[k] - w * v[k]);
+        for (int k = 0; k < n; k++) y[k] = invd[k] * p[k];
+        spmv(y, v);
+        alpha = rho / dot(r0, v);
+        for (int k = 0; k < n; k++) sv[k] = r[k] - alpha * v[k];
+        for (int k = 0; k < n; k++) z[k] = invd[k] * sv[k];
+        spmv(z, t);
+        const double tt = dot(t, t);
+        w = tt > 0.0 ? dot(t, sv) / tt : 0.0;
+        for (int k = 0; k < n; k++) x[k] += alpha * y[k] + w * z[k];
+        for (int k = 0; k < n; k++) r[k] = sv[k] - w * t[k];
+        ++i;
+    }
+    s.solverIter = i;
+    s.solverErr = sqrt(dot(r, r) / rhsN2);
+}
+
 void solveCG(Sys& s) {
     const mps_params& P = s.p;
     const int n = s.N;
-    // build the iterated matrix in row form with ascending columns
-    std::vector<int> rp(n + 1, 0), ci;
-    std::vector<double> va;
-    std::vector<double> diag(n, 0.0);
-    {
-        std::vector<std::vector<std::pair<int, double>>> rows(n);
-        for (int r = 0; r < n; r++)
-            for (int k = s.rowPtr[r]; k < s.rowPtr[r + 1]; k++) {
-                int c = s.colIdx[k];
-                double v = s.vals[k];
-                if (c == r) diag[r] = v;
-                if (P.ppe_transpose) rows[c].push_back({r, v});
-                else rows[r].push_back({c, v});
-            }
-        for (int r = 0; r < n; r++) {
-            std::stable_sort(rows[r].begin(), rows[r].end(),
-                             [](const std::pair<int, double>& a, const std::pair<int, double>& b) { return a.first < b.first; });
-            rp[r + 1] = rp[r] + (int)rows[r].size();
-            for (auto& e : rows[r]) { ci.push_back(e.first); va.push_back(e.second); }
-        }
-    }
+    std::vector<int> rp, ci;
+    std::vector<double> va, diag;
+    iterMatrix(s, P.ppe_transpose != 0, rp, ci, va, diag);
     auto spmv = [&](const std::vector<double>& x, std::vector<double>& y) {
 #pragma omp parallel for
         for (int r = 0; r < n; r++) {
@@ -1773,9 +1856,9 @@ int calcPress(Sys& s) {
         velDivergence(s);
         if (P.wall_type == MPS_WALL_POLYGON) wallVelDivergence(s);
     }
-    if (P.solver_type != MPS_SOLVER_CG) { s.err = "only the CG solver is supported"; return -1; }
     assemblePPE(s);
-    solveCG(s);
+    if (P.solver_type == MPS_SOLVER_BICGSTAB) solveBiCGSTAB(s);
+    else solveCG(s);
     for (int i = 0; i < s.N; i++) s.press[i] = s.ppeX[i];
     for (int i = 0; i < s.N; i++) if (s.press[i] < 0) s.press[i] = 0.0;
     return 0;
@@ -2469,7 +2552,9 @@ int orc_call(void* h, const char* op) {
     else if (o == "wall_normal_particles") wallNormalParticles(*s);
     else if (o == "prediction_wall_press_gradient") predictionWallPressGradient(*s);
     else if (o == "assemble_ppe") assemblePPE(*s);
-    else if (o == "solve_ppe") { solveCG(*s); for (int i = 0; i < s->N; i++) s->press[i] = s->ppeX[i] < 0 ? 0.0 : s->ppeX[i]; }
+    else if (o == "solve_ppe") {
+        if (s->p.solver_type == MPS_SOLVER_BICGSTAB) solveBiCGSTAB(*s); else solveCG(*s);
+        for (int i = 0; i < s->N; i++) s->press[i] = s->ppeX[i] < 0 ? 0.0 : s->ppeX[i]; }
     else if (o == "calc_press") return calcPress(*s);
     else if (o == "extrapolate_wall_dummy") extrapolateWallDummy(*s);
     else if (o == "press_gradient") pressGradient(*s);

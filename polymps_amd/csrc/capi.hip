@@ -244,7 +244,6 @@ static int calcPress(Ctx* ctx) {
     int rc;
     if (P.mps_type == MPS_EXPLICIT || P.mps_type == MPS_WEAKLY) return launchOp(ctx, "press_explicit");
     if (P.mps_type == MPS_IMPLICIT_PND_DIVU && (rc = velDivergence(ctx))) return rc;
-    if (P.solver_type != MPS_SOLVER_CG) { ctx->err = "only the CG solver is supported"; return MPS_E_UNSUPPORTED; }
     if ((rc = ppeAssemble(ctx))) return rc;
     return ppeSolve(ctx);
 }
@@ -409,6 +408,7 @@ int mps_destroy(mps_ctx* ctx) {
                     ctx->aVal, ctx->aCol, ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->aVal32, ctx->aDiagInv32,
                     ctx->aDiag32, ctx->cgR, ctx->cgP,
                     ctx->cgZ, ctx->cgQ, ctx->cgInvD, ctx->cgX, ctx->fR, ctx->fP, ctx->fZ, ctx->fQ, ctx->fX,
+                    ctx->bcgR0, ctx->bcgY, ctx->bcgS, ctx->bcgT,
                     ctx->ds, ctx->is, ctx->red};
     for (void* b : bufs)
         if (b) hipFree(b);

@@ -38,9 +38,10 @@ enum IField { ITYPE, ICANON, IBC, INNEIGH, NPI, INNWALL = NPI, INEARWALL, INEARM
 
 // device scalar slots (double)
 enum DScalar { S_VELMAX, S_ABSNEW, S_ABSOLD, S_RHSN2, S_THR, S_RN2, S_ALPHA, S_BETA, S_PAP, S_BN2, S_TRUE_RN2,
-               S_RELTOL, NDS };
+               S_RELTOL, S_RHO, S_RHOOLD, S_BW, S_R0N2, NDS };
 // device scalar slots (int)
-enum IScalar { I_NLIVE, I_OUTCNT, I_OVERFLOW, I_CG_ITER, I_CG_DONE, I_COMPACT_OVF, I_LIST_MAX, I_NNZ, I_NESC, NIS };
+enum IScalar { I_NLIVE, I_OUTCNT, I_OVERFLOW, I_CG_ITER, I_CG_DONE, I_COMPACT_OVF, I_LIST_MAX, I_NNZ, I_NESC,
+               I_BCG_RESTART, I_BCG_RESTARTS, NIS };
 
 constexpr int kWallTable = 50;
 // escaped (non-delta) columns per PPE row: the first entry and stencil-row
@@ -143,6 +144,10 @@ struct Ctx {
     double* cgQ = nullptr;
     double* cgInvD = nullptr;
     double* cgX = nullptr;      // refinement correction
+    double* bcgR0 = nullptr;    // BiCGSTAB shadow residual, y, s, t (allocated on first use)
+    double* bcgY = nullptr;
+    double* bcgS = nullptr;
+    double* bcgT = nullptr;
     float* fR = nullptr;
     float* fP = nullptr;
     float* fZ = nullptr;

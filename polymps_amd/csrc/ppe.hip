@@ -604,14 +604,16 @@ static int ensureMixed(Ctx* ctx) {
 }
 
 int ppeAssemble(Ctx* ctx) {
-    const bool mixed = ctx->P.precision_mode == MPS_PREC_MIXED;
+    // BiCGSTAB runs in FP64 on A; CG on A^T (ppe_transpose) and in the chosen precision
+    const bool bicg = ctx->P.solver_type == MPS_SOLVER_BICGSTAB;
+    const bool mixed = !bicg && ctx->P.precision_mode == MPS_PREC_MIXED;
     if (mixed) {
         int rc = ensureMixed(ctx);
         if (rc) return rc;
     }
     KArgs a = ctx->args();
     const int g = gridFor(ctx->N, kB);
-    const bool T = ctx->P.ppe_transpose != 0;
+    const bool T = ctx->P.ppe_transpose != 0 && !bicg;
     float* v32 = mixed ? ctx->aVal32 : nullptr;
     float* d32 = mixed ? ctx->aDiag32 : nullptr;
     float* i32 = mixed ? ctx->aDiagInv32 : nullptr;
@@ -660,6 +662,226 @@ static int readStatus(Ctx* ctx, std::vector<float>* times, int batch) {
             hipEventElapsedTime(&ms, ctx->evPool[2 * q], ctx->evPool[2 * q + 1]);
             times->push_back(ms);
         }
+    return MPS_OK;
+}
+
+// ---------------------------------------------------------- BiCGSTAB
+// Eigen's BiCGSTAB with its DiagonalPreconditioner
+// (solveBiConjugateGradientStabilized, src/MpsPressure.cpp:448-468), in FP64
+// on A itself: restart when rho falls below eps^2 |r0|^2 (the first restart
+// also resets the iteration count), stop on |r|^2 <= tol^2 |b|^2 or the
+// iteration limit.  Device scalars carry rho, alpha, w; each reduction is a
+// block-partials kernel and a one-block finish.
+struct BcgPtrs {
+    CgPtrs c;   // x, r, p, z, q (= v), invd and the FP64 matrix
+    double* r0;
+    double* y;
+    double* s;
+    double* t;
+};
+
+// r = b - A x; r0 = r; p = v = 0; invd; partials |r|^2, |b|^2
+__global__ void kBcgStart(KArgs a, BcgPtrs q) {
+    const int i = slotId();
+    double s0 = 0, s1 = 0;
+    if (i < a.N) {
+        const double ax = spmv64(a, q.c, q.c.x, i);
+        const double r = q.c.b[i] - ax;
+        q.c.r[i] = r;
+        q.r0[i] = r;
+        q.c.p[i] = 0.0;
+        q.c.q[i] = 0.0;
+        const double d = q.c.diag[i];
+        q.c.invd[i] = (d != 0.0) ? 1.0 / d : 1.0;
+        s0 = r * r;
+        s1 = q.c.b[i] * q.c.b[i];
+    }
+    writePartials(a, s0, s1);
+}
+__global__ void kBcgStartFinish(KArgs a, int nb, double tol, int maxIt) {
+    double rn2, bn2;
+    finishPartials(a.red, nb, rn2, bn2);
+    if (threadIdx.x != 0) return;
+    a.ds[S_R0N2] = rn2;
+    a.ds[S_RN2] = rn2;
+    a.ds[S_RHSN2] = bn2;
+    a.ds[S_THR] = tol * tol * bn2;
+    a.ds[S_RHO] = 1.0;
+    a.ds[S_ALPHA] = 1.0;
+    a.ds[S_BW] = 1.0;
+    a.is[I_CG_ITER] = 0;
+    a.is[I_BCG_RESTART] = 0;
+    a.is[I_BCG_RESTARTS] = 0;
+    if (bn2 == 0.0) a.is[I_CG_DONE] = 2;   // x = 0
+    else a.is[I_CG_DONE] = (rn2 > a.ds[S_THR] && maxIt > 0) ? 0 : 1;
+}
+// rho = r0.r
+__global__ void kBcgRho(KArgs a, BcgPtrs q) {
+    if (a.is[I_CG_DONE]) return;
+    const int i = slotId();
+    writePartials(a, i < a.N ? q.r0[i] * q.c.r[i] : 0.0, 0.0);
+}
+__global__ void kBcgRhoFinish(KArgs a, int nb) {
+    if (a.is[I_CG_DONE]) return;
+    double rho, unused;
+    finishPartials(a.red, nb, rho, unused);
+    if (threadIdx.x != 0) return;
+    a.ds[S_RHOOLD] = a.ds[S_RHO];
+    a.ds[S_RHO] = rho;
+    a.is[I_BCG_RESTART] = fabs(rho) < DBL_EPSILON * DBL_EPSILON * a.ds[S_R0N2] ? 1 : 0;
+}
+// restart: r = b - A x; r0 = r; rho = |r0|^2 = |r|^2
+__global__ void kBcgRestart(KArgs a, BcgPtrs q) {
+    if (a.is[I_CG_DONE] || !a.is[I_BCG_RESTART]) return;
+    const int i = slotId();
+    double s0 = 0;
+    if (i < a.N) {
+        const double r = q.c.b[i] - spmv64(a, q.c, q.c.x, i);
+        q.c.r[i] = r;
+        q.r0[i] = r;
+        s0 = r * r;
+    }
+    writePartials(a, s0, 0.0);
+}
+__global__ void kBcgRestartFinish(KArgs a, int nb) {
+    if (a.is[I_CG_DONE] || !a.is[I_BCG_RESTART]) return;
+    double rn2, unused;
+    finishPartials(a.red, nb, rn2, unused);
+    if (threadIdx.x != 0) return;
+    a.ds[S_RHO] = rn2;
+    a.ds[S_R0N2] = rn2;
+    if (a.is[I_BCG_RESTARTS]++ == 0) a.is[I_CG_ITER] = 0;
+    a.is[I_BCG_RESTART] = 0;
+}
+// p = r + beta (p - w v); y = invd p
+__global__ void kBcgP(KArgs a, BcgPtrs q) {
+    if (a.is[I_CG_DONE]) return;
+    const int i = slotId();
+    if (i >= a.N) return;
+    const double w = a.ds[S_BW];
+    const double beta = (a.ds[S_RHO] / a.ds[S_RHOOLD]) * (a.ds[S_ALPHA] / w);
+    const double p = q.c.r[i] + beta * (q.c.p[i] - w * q.c.q[i]);
+    q.c.p[i] = p;
+    q.y[i] = q.c.invd[i] * p;
+}
+// v = A y; partial r0.v
+__global__ __launch_bounds__(256) void kBcgV(KArgs a, BcgPtrs q) {
+    if (a.is[I_CG_DONE]) return;
+    const int i = slotId();
+    double s0 = 0;
+    if (i < a.N) {
+        const double v = spmv64(a, q.c, q.y, i);
+        q.c.q[i] = v;
+        s0 = q.r0[i] * v;
+    }
+    writePartials(a, s0, 0.0);
+}
+__global__ void kBcgAlpha(KArgs a, int nb) {
+    if (a.is[I_CG_DONE]) return;
+    double r0v, unused;
+    finishPartials(a.red, nb, r0v, unused);
+    if (threadIdx.x == 0) a.ds[S_ALPHA] = a.ds[S_RHO] / r0v;
+}
+// s = r - alpha v; z = invd s
+__global__ void kBcgS(KArgs a, BcgPtrs q) {
+    if (a.is[I_CG_DONE]) return;
+    const int i = slotId();
+    if (i >= a.N) return;
+    const double sv = q.c.r[i] - a.ds[S_ALPHA] * q.c.q[i];
+    q.s[i] = sv;
+    q.c.z[i] = q.c.invd[i] * sv;
+}
+// t = A z; partials t.t, t.s
+__global__ __launch_bounds__(256) void kBcgT(KArgs a, BcgPtrs q) {
+    if (a.is[I_CG_DONE]) return;
+    const int i = slotId();
+    double s0 = 0, s1 = 0;
+    if (i < a.N) {
+        const double t = spmv64(a, q.c, q.c.z, i);
+        q.t[i] = t;
+        s0 = t * t;
+        s1 = t * q.s[i];
+    }
+    writePartials(a, s0, s1);
+}
+__global__ void kBcgW(KArgs a, int nb) {
+    if (a.is[I_CG_DONE]) return;
+    double tt, ts;
+    finishPartials(a.red, nb, tt, ts);
+    if (threadIdx.x == 0) a.ds[S_BW] = (tt > 0.0) ? ts / tt : 0.0;
+}
+// x += alpha y + w z; r = s - w t; partial |r|^2
+__global__ void kBcgUpdate(KArgs a, BcgPtrs q) {
+    if (a.is[I_CG_DONE]) return;
+    const int i = slotId();
+    double s0 = 0;
+    if (i < a.N) {
+        const double alpha = a.ds[S_ALPHA], w = a.ds[S_BW];
+        q.c.x[i] = q.c.x[i] + (alpha * q.y[i] + w * q.c.z[i]);
+        const double r = q.s[i] - w * q.t[i];
+        q.c.r[i] = r;
+        s0 = r * r;
+    }
+    writePartials(a, s0, 0.0);
+}
+__global__ void kBcgCheck(KArgs a, int nb, int maxIt) {
+    if (a.is[I_CG_DONE]) return;
+    double rn2, unused;
+    finishPartials(a.red, nb, rn2, unused);
+    if (threadIdx.x != 0) return;
+    a.ds[S_RN2] = rn2;
+    const int it = a.is[I_CG_ITER] + 1;
+    a.is[I_CG_ITER] = it;
+    if (!(rn2 > a.ds[S_THR])) a.is[I_CG_DONE] = 1;
+    else if (it >= maxIt) a.is[I_CG_DONE] = 3;
+}
+
+static int ensureBicgstab(Ctx* ctx) {
+    if (ctx->bcgR0) return MPS_OK;
+    const size_t bytes = sizeof(double) * (size_t)ctx->N;
+    MPS_HIP(hipMalloc(&ctx->bcgR0, bytes));
+    MPS_HIP(hipMalloc(&ctx->bcgY, bytes));
+    MPS_HIP(hipMalloc(&ctx->bcgS, bytes));
+    MPS_HIP(hipMalloc(&ctx->bcgT, bytes));
+    return MPS_OK;
+}
+
+static int solveBicgstab(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int batch) {
+    int rc = ensureBicgstab(ctx);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    const BcgPtrs q{c, ctx->bcgR0, ctx->bcgY, ctx->bcgS, ctx->bcgT};
+    const double tol = ctx->P.solver_tol > 0 ? ctx->P.solver_tol : DBL_EPSILON;
+    const int maxIt = ctx->P.solver_max_iter > 0 ? ctx->P.solver_max_iter : 2 * ctx->nLiveHost;
+    hipLaunchKernelGGL(kBcgStart, dim3(nb), dim3(kB), 0, s, a, q);
+    hipLaunchKernelGGL(kBcgStartFinish, dim3(1), dim3(1024), 0, s, a, nb, tol, maxIt);
+    hipLaunchKernelGGL(kCgZeroX, dim3(nb), dim3(kB), 0, s, a, c.x);
+    int done = 0, iters = 0;
+    // a restart resets the count once, so the host bound allows two passes
+    for (int launched = 0; !done; launched += batch) {
+        for (int k = 0; k < batch; k++) {
+            hipLaunchKernelGGL(kBcgRho, dim3(nb), dim3(kB), 0, s, a, q);
+            hipLaunchKernelGGL(kBcgRhoFinish, dim3(1), dim3(1024), 0, s, a, nb);
+            hipLaunchKernelGGL(kBcgRestart, dim3(nb), dim3(kB), 0, s, a, q);
+            hipLaunchKernelGGL(kBcgRestartFinish, dim3(1), dim3(1024), 0, s, a, nb);
+            hipLaunchKernelGGL(kBcgP, dim3(nb), dim3(kB), 0, s, a, q);
+            hipLaunchKernelGGL(kBcgV, dim3(nb), dim3(kB), 0, s, a, q);
+            hipLaunchKernelGGL(kBcgAlpha, dim3(1), dim3(1024), 0, s, a, nb);
+            hipLaunchKernelGGL(kBcgS, dim3(nb), dim3(kB), 0, s, a, q);
+            hipLaunchKernelGGL(kBcgT, dim3(nb), dim3(kB), 0, s, a, q);
+            hipLaunchKernelGGL(kBcgW, dim3(1), dim3(1024), 0, s, a, nb);
+            hipLaunchKernelGGL(kBcgUpdate, dim3(nb), dim3(kB), 0, s, a, q);
+            hipLaunchKernelGGL(kBcgCheck, dim3(1), dim3(1024), 0, s, a, nb, maxIt);
+        }
+        MPS_HIP(hipGetLastError());
+        if ((rc = readStatus(ctx, nullptr, 0))) return rc;
+        done = ctx->hostInts[I_CG_DONE];
+        iters = ctx->hostInts[I_CG_ITER];
+        if (launched > 8 * maxIt + 64) break;   // the device flag ends every batch; this only bounds the host loop
+    }
+    ctx->solverIter = iters;
+    ctx->outerIter = 1;
+    ctx->solverErr = ctx->hostScalars[S_RHSN2] > 0 ? sqrt(ctx->hostScalars[S_RN2] / ctx->hostScalars[S_RHSN2]) : 0.0;
     return MPS_OK;
 }
 
@@ -775,7 +997,8 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
 }
 
 int ppeSolve(Ctx* ctx) {
-    const bool mixed = ctx->P.precision_mode == MPS_PREC_MIXED;
+    const bool bicg = ctx->P.solver_type == MPS_SOLVER_BICGSTAB;
+    const bool mixed = !bicg && ctx->P.precision_mode == MPS_PREC_MIXED;
     if (mixed && !ctx->aVal32) {
         ctx->err = "mixed-precision solve needs a mixed-precision assembly first";
         return MPS_E_ARG;
@@ -798,7 +1021,8 @@ int ppeSolve(Ctx* ctx) {
         if ((rc = gnnForward(ctx))) return rc;
         if ((rc = gnnWarmStart(ctx))) return rc;
     }
-    rc = mixed ? solveMixed(ctx, a, c, nb, batch) : solveFp64(ctx, a, c, nb, batch);
+    if (bicg) rc = solveBicgstab(ctx, a, c, nb, batch);
+    else rc = mixed ? solveMixed(ctx, a, c, nb, batch) : solveFp64(ctx, a, c, nb, batch);
     if (rc) return rc;
     if (!mixed) {   // the mixed path ends on a fresh true residual already
         hipLaunchKernelGGL(kTrueResidual, dim3(nb), dim3(kB), 0, s, a, c);

@@ -66,11 +66,13 @@ def ppe_check(g, o, where, press_rtol):
     rpg, cg, vg = g.ppe_matrix()
     rpo, co, vo = o.ppe_matrix()
     n = len(rpo) - 1
-    # oracle rows are A; the device iterates A^T when ppe_transpose is set
+    # oracle rows are A; the device's CG iterates A^T when ppe_transpose is
+    # set, its BiCGSTAB A
+    trans = bool(p.ppe_transpose) and p.solver_type == 0
     ent_o = {}
     for r in range(n):
         for k in range(rpo[r], rpo[r + 1]):
-            key = (int(co[k]), r) if p.ppe_transpose else (r, int(co[k]))
+            key = (int(co[k]), r) if trans else (r, int(co[k]))
             ent_o[key] = vo[k]
     ent_g = {}
     for r in range(len(rpg) - 1):
@@ -83,7 +85,7 @@ def ppe_check(g, o, where, press_rtol):
     for key, v in ent_o.items():
         if ent_g[key] != v:
             raise Mismatch(f"{where}: PPE entry {key} gpu {ent_g[key]!r} oracle {v!r}")
-    if not p.ppe_transpose:
+    if not trans:
         # without the transpose the rows must also come in the same order
         compare_exact("ppe_cols", cg, co, where)
     compare_exact("ppe_b", g.get("ppe_b"), o.get("ppe_b"), where)

@@ -144,6 +144,23 @@ def test_lockstep_arc_free_surface(built, mk):
         lockstep_step(g, o, 10 + s)
 
 
+@pytest.mark.parametrize("mk", ["dam3d", "channel2d_inc"])
+def test_lockstep_bicgstab(built, mk):
+    """BiCGSTAB (solver type 1) on A itself: the matrix bit-exact, the
+    pressure within 1e-6 of the largest pressure of the oracle's solve."""
+    w = configs.small_dam3d(l0=0.025) if mk == "dam3d" else configs.periodic_channel2d(mps_type=2)
+    w.params.solver_type = 1
+    g, o = new_pair(w)
+    lockstep_init(g, o)
+    for s in range(2):
+        lockstep_step(g, o, s)
+    o.steps(30)
+    sync_state(g, o)
+    for s in range(2):
+        lockstep_step(g, o, 10 + s)
+    assert g.scalar("true_residual") < 1e-8
+
+
 def test_lockstep_dam2d_wc(built):
     w = configs.small_dam2d(l0=0.02)
     g, o = new_pair(w)

@@ -569,3 +569,31 @@ def test_arc_rule_on_lattice_block(built, dim):
     deep = np.all(depth >= 2, axis=1)
     below_face = (np.sum(depth == 1, axis=1) == 1) & (np.sum(depth >= 2, axis=1) == dim - 1)
     assert below_face.any() and np.all(bc[deep | below_face] == q.inner)
+
+
+@pytest.mark.parametrize("guess", ["zero", "warm"])
+def test_bicgstab_matches_dense_solve(built, guess):
+    """BiCGSTAB (solver type 1) iterates A itself, not the transpose; from a
+    zero or a perturbed warm guess it reaches the dense solution of A x = b."""
+    w = configs.small_dam3d(l0=0.025, tol=1e-13)
+    w.params.solver_type = 1
+    o = Oracle(w.params, w.types, w.pos, mesh=w.mesh)
+    o.call("init")
+    o.steps(30)
+    o.call("update_buckets")
+    o.call("assemble_ppe")
+    rp, cols, vals = o.ppe_matrix()
+    n = len(rp) - 1
+    A = np.zeros((n, n))
+    for r in range(n):
+        for k in range(rp[r], rp[r + 1]):
+            A[r, cols[k]] += vals[k]
+    b = o.get("ppe_b")[:n]
+    x_dense = np.linalg.solve(A, b)
+    x0 = np.zeros(n) if guess == "zero" else x_dense * (1.0 + 1e-3 * np.sin(np.arange(n)))
+    o.set("ppe_x", x0)
+    o.call("solve_ppe")
+    x = o.get("ppe_x")[:n]
+    assert np.linalg.norm(x - x_dense) <= 1e-10 * np.linalg.norm(x_dense)
+    assert np.linalg.norm(b - A @ x) <= 1e-12 * np.linalg.norm(b)
+    assert 0 < o.scalar("solver_iter") < 2 * n
