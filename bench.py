@@ -150,8 +150,11 @@ def main(argv=None):
     ap.add_argument("--gnn", type=Path, default=ROOT / "polymps_amd" / "data" / "gnn_dam3d.bin",
                     help="learned-operator weights for the learned+FP32 PPE comparison")
     ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--inner-tol", type=float, default=None,
-                    help="relative tolerance of the inner FP32 solve of the mixed PPE")
+    ap.add_argument("--inner-tol", type=float, default=1e-5,
+                    help="relative tolerance of the inner FP32 solve of the mixed PPE (with the extrapolated "
+                         "initial guess one inner solve to 1e-5 usually reaches the FP64 target)")
+    ap.add_argument("--warm-start", type=int, choices=[0, 1, 2], default=2,
+                    help="initial guess of each PPE solve: 0 previous solution, 1 linear, 2 quadratic extrapolation")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration)")
@@ -176,6 +179,7 @@ def main(argv=None):
         wname = (f"channel4m l0={args.l0:g} m, {{n}} particles, {w.info['triangles']}-triangle heightmap bed, "
                  "channel and 3 obstacles")
     w.params.precision_mode = 1 if args.precision == "mixed" else 0
+    w.params.ppe_warm_start = args.warm_start
     if args.inner_tol is not None:
         w.params.mixed_inner_tol = args.inner_tol
     sim = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh, device=0)
@@ -212,6 +216,7 @@ def main(argv=None):
                  "synthetic (seeded jittered lattice, generated fractal heightmap bed, channel and obstacles)"),
         "config": {"workload": wname.format(n=n),
                    "particles": n, "precision": args.precision, "solver_tol": w.params.solver_tol,
+                   "ppe_warm_start": args.warm_start, "mixed_inner_tol": w.params.mixed_inner_tol,
                    "dt": w.params.time_step},
         "ppe": {"last_solve_ms": sim.scalar("ppe_ms"), "last_iterations": sim.scalar("solver_iter"),
                 "last_outer_iterations": sim.scalar("outer_iterations"),

@@ -111,7 +111,9 @@ typedef struct mps_params {
                                    column-major matrix; 0: iterate with A          */
     double mixed_inner_tol;     /* inner FP32 PCG tolerance of the refinement loop */
     int learned_mode;           /* 0 off, 1 warm start, 2 warm start + preconditioner */
-    int pad2;
+    int ppe_warm_start;         /* initial guess of each pressure solve: 0 the previous solution
+                                   (the reference), 1 / 2 linear / quadratic extrapolation of the
+                                   last 2 / 3 solutions */
 
     /* ---- derived by mps_params_derive (setParameters + bucket grid) ---- */
     double eps_zero;            /* 10 * DBL_EPSILON                   */

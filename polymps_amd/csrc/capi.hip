@@ -234,6 +234,7 @@ static int runInit(Ctx* ctx) {
     } else {
         if (P.mps_type == MPS_IMPLICIT_PND_DIVU && (rc = velDivergence(ctx))) return rc;
         if ((rc = ppeAssemble(ctx))) return rc;
+        if ((rc = ppeWarmStart(ctx))) return rc;
         if ((rc = ppeSolve(ctx))) return rc;
     }
     return syncStatus(ctx);
@@ -245,6 +246,7 @@ static int calcPress(Ctx* ctx) {
     if (P.mps_type == MPS_EXPLICIT || P.mps_type == MPS_WEAKLY) return launchOp(ctx, "press_explicit");
     if (P.mps_type == MPS_IMPLICIT_PND_DIVU && (rc = velDivergence(ctx))) return rc;
     if ((rc = ppeAssemble(ctx))) return rc;
+    if ((rc = ppeWarmStart(ctx))) return rc;
     return ppeSolve(ctx);
 }
 
@@ -611,6 +613,13 @@ int mps_set_scalar(mps_ctx* ctx, const char* name, double value) {
             return MPS_E_ARG;
         }
         ctx->P.learned_mode = (int)value;
+    } else if (s == "ppe_warm_start") {
+        // initial guess of each pressure solve: 0 previous solution, 1 linear, 2 quadratic extrapolation
+        if (value != 0.0 && value != 1.0 && value != 2.0) {
+            ctx->err = "ppe_warm_start must be 0, 1 or 2";
+            return MPS_E_ARG;
+        }
+        ctx->P.ppe_warm_start = (int)value;
     } else if (s == "precision_mode") {
         // takes effect at the next assembly; a solve needs an assembly in the same mode
         if (value != MPS_PREC_FP64 && value != MPS_PREC_MIXED) {

@@ -25,7 +25,7 @@ enum DField {
     // (pndki, div_u, diffusive and grad_conc are written for a subset of the
     // particles in some modes, so they are carried with their particle)
     DX, DY, DZ, DU, DV, DW, DMX, DMY, DMZ, DPRESS, DPNDI, DPPEX, DPAVG, DFWX, DFWY, DFWZ, DPNDSKI,
-    DPNDKI, DDIVU, DDIFF, DGCX, DGCY, DGCZ,
+    DPNDKI, DDIVU, DDIFF, DGCX, DGCY, DGCZ, DPPEX1, DPPEX2,   // the two solutions before ppe_x
     NPD,
     // per step
     DAX = NPD, DAY, DAZ, DASX, DASY, DASZ, DNPX, DNPY, DNPZ, DNPCD2, DPNDSMALL, DCONC,
@@ -167,6 +167,7 @@ struct Ctx {
     int outerIter = 0;
     Gnn* gnn = nullptr;          // learned PPE operator (created on first use)
     int redCap = 0;              // partial-sum slots in red (pairs)
+    int ppeSolves = 0;           // pressure solves so far (history of the extrapolated warm start)
     double solverErr = 0.0;
     double trueResidual = 0.0;
     double ppeMs = 0.0;
@@ -203,6 +204,7 @@ int launchBuildList(Ctx* ctx);
 int launchOp(Ctx* ctx, const char* op);
 int ppeAssemble(Ctx* ctx);
 int ppeSolve(Ctx* ctx);
+int ppeWarmStart(Ctx* ctx);
 int launchCompaction(Ctx* ctx);
 // learned PPE operator (gnn.hip)
 int gnnEnsure(Ctx* ctx);

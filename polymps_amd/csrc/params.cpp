@@ -201,6 +201,7 @@ void paramsDefaults(mps_params* p) {
     p->ppe_transpose = 1;
     p->mixed_inner_tol = 1e-4;
     p->learned_mode = 0;
+    p->ppe_warm_start = 0;
 }
 
 int paramsFromJson(const char* path, mps_params* p, std::string& grid, std::string& mesh, std::string& err) {
@@ -283,6 +284,7 @@ int paramsFromJson(const char* path, mps_params* p, std::string& grid, std::stri
     p->mixed_inner_tol = jnum(si, "mixed_inner_tolerance", 1e-4);
     const JVal* ln = si ? si->get("learned") : nullptr;
     p->learned_mode = jint(ln, "mode", 0);
+    p->ppe_warm_start = jint(si, "warm_start", 0);
     const JVal* sh = nu ? nu->get("particle_shifting") : nullptr;
     p->shifting_type = jint(sh, "type", 2);
     p->dri = jnum(sh, "DRI", 0.01);

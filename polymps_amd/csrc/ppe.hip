@@ -996,6 +996,34 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
     return MPS_OK;
 }
 
+// Initial guess of the next solve from the history of solutions (x_n in
+// ppe_x, x_{n-1}, x_{n-2}): x_n (order 0), 2 x_n - x_{n-1} (1) or
+// 3 x_n - 3 x_{n-1} + x_{n-2} (2); the history shifts by one.  The
+// reference starts from x_n (solveWithGuess, src/MpsPressure.cpp:433); the
+// solve still runs to the same residual target.
+__global__ void kPpeExtrapolate(KArgs a, int order) {
+    const int i = slotId();
+    if (i >= a.N) return;
+    const double x = a.D[DPPEX][i], x1 = a.D[DPPEX1][i], x2 = a.D[DPPEX2][i];
+    double g = x;
+    if (order == 1) g = 2.0 * x - x1;
+    else if (order == 2) g = 3.0 * x - 3.0 * x1 + x2;
+    a.D[DPPEX2][i] = x1;
+    a.D[DPPEX1][i] = x;
+    a.D[DPPEX][i] = g;
+}
+
+int ppeWarmStart(Ctx* ctx) {
+    const int want = ctx->P.ppe_warm_start;
+    if (want <= 0) return MPS_OK;
+    // solutions available before this solve: ppeSolves (the first guess is not one)
+    const int order = std::min(want, std::max(ctx->ppeSolves - 1, 0));
+    KArgs a = ctx->args();
+    hipLaunchKernelGGL(kPpeExtrapolate, dim3(gridFor(ctx->N, kB)), dim3(kB), 0, ctx->stream, a, std::min(order, 2));
+    MPS_HIP(hipGetLastError());
+    return MPS_OK;
+}
+
 int ppeSolve(Ctx* ctx) {
     const bool bicg = ctx->P.solver_type == MPS_SOLVER_BICGSTAB;
     const bool mixed = !bicg && ctx->P.precision_mode == MPS_PREC_MIXED;
@@ -1037,6 +1065,7 @@ int ppeSolve(Ctx* ctx) {
     ctx->ppeMs = ms;
     const double* hs = ctx->hostScalars;
     ctx->trueResidual = hs[S_BN2] > 0 ? sqrt(hs[S_TRUE_RN2] / hs[S_BN2]) : 0.0;
+    ctx->ppeSolves++;
     return MPS_OK;
 }
 
