@@ -316,7 +316,10 @@ __global__ void kPrediction(KArgs a) {
 // MpsParticleVelPos::correctionVelocityPosition (src/MpsParticleVelPos.cpp:53-97);
 // velMax = sqrt(max |u|^2) over fluid particles, reduced with an integer max
 // on the (non-negative) bit pattern so the result does not depend on order.
-__global__ void kCorrection(KArgs a, unsigned long long* vmax2bits) {
+// The largest |u|^2 of each block goes to red[block]; kVelMaxFinish takes
+// the maximum of the partials (one atomic per wavefront on a single address
+// serialises device-wide and tripled the kernel's time).
+__global__ __launch_bounds__(kBlock) void kCorrection(KArgs a) {
     const int i = slotId();
     double v2 = 0.0;
     if (live(a, i) && a.I[ITYPE][i] == a.P.fluid) {
@@ -332,12 +335,27 @@ __global__ void kCorrection(KArgs a, unsigned long long* vmax2bits) {
         a.D[DAX][i] = 0.0; a.D[DAY][i] = 0.0; a.D[DAZ][i] = 0.0;
         v2 = u * u + v * v + w * w;
     }
+    __shared__ double wm[kBlock / 64];
     v2 = waveMax(v2);
-    if ((threadIdx.x & 63) == 0 && v2 > 0.0) atomicMax(vmax2bits, (unsigned long long)__double_as_longlong(v2));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wm[0];
+        for (int q = 1; q < kBlock / 64; q++) m = fmax(m, wm[q]);
+        a.red[blockIdx.x] = m;
+    }
 }
 
-__global__ void kVelMaxFinish(KArgs a, const unsigned long long* vmax2bits) {
-    a.ds[S_VELMAX] = sqrt(__longlong_as_double((long long)*vmax2bits));
+__global__ __launch_bounds__(1024) void kVelMaxFinish(KArgs a, int nb) {
+    __shared__ double wm[16];
+    double m = 0.0;
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) m = fmax(m, a.red[q]);
+    m = waveMax(m);
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int q = 1; q < (int)(blockDim.x >> 6); q++) m = fmax(m, wm[q]);
+    a.ds[S_VELMAX] = sqrt(m);
 }
 
 // --------------------------------------------------------------- domain
@@ -1870,10 +1888,8 @@ int launchOp(Ctx* ctx, const char* op) {
     else if (o == "prediction_wall_press_gradient") LT(kPredWallPressGradient);
     else if (o == "wall_viscosity") LT(kWallViscosity);
     else if (o == "correction") {
-        unsigned long long* bits = (unsigned long long*)(ctx->red + 3 * ctx->redCap + 8);
-        MPS_HIP(hipMemsetAsync(bits, 0, sizeof(unsigned long long), ctx->stream));
-        hipLaunchKernelGGL(kCorrection, dim3(g), dim3(kBlock), 0, ctx->stream, a, bits);
-        hipLaunchKernelGGL(kVelMaxFinish, dim3(1), dim3(1), 0, ctx->stream, a, bits);
+        hipLaunchKernelGGL(kCorrection, dim3(g), dim3(kBlock), 0, ctx->stream, a);   // g <= redCap partials
+        hipLaunchKernelGGL(kVelMaxFinish, dim3(1), dim3(1024), 0, ctx->stream, a, g);
         ctx->listValid = false;
     } else if (o == "shifting") {
         if (P.shifting_type == 1) {

@@ -7,14 +7,19 @@
 
 namespace mpsgpu {
 
-// bucket index as updateParticlesID computes it: +eps only in 2D
-__global__ void kBucketKey(KArgs a, int* key, int* rank, int* count) {
+// Bucket key of every slot and its rank among the slots of its bucket (any
+// permutation of 0..count-1; the segment sort orders a bucket by canonical
+// id).  Slots are mostly in bucket order from the previous step, so a
+// wavefront holds a few runs of equal keys: one lane per run takes the run's
+// ranks with a single atomic (per-lane atomics on a few addresses serialise).
+// The bucket index is computed as updateParticlesID does: +eps only in 2D.
+__global__ __launch_bounds__(256) void kBucketKey(KArgs a, int* key, int* rank, int* count) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.N) return;
+    const bool in = k < a.N;
     const mps_params& P = a.P;
     int ib = a.nB;
-    const int t = a.I[ITYPE][k];
-    if (a.I[ICANON][k] < a.is[I_NLIVE] && t != P.ghost) {
+    const int t = in ? a.I[ITYPE][k] : P.ghost;
+    if (in && a.I[ICANON][k] < a.is[I_NLIVE] && t != P.ghost) {
         const double x = a.D[DX][k], y = a.D[DY][k], z = a.D[DZ][k];
         if ((int)P.dim == 2) {
             const int ix = (int)((x - P.domain_min[0]) * P.inv_bucket_side + KEPS);
@@ -28,8 +33,21 @@ __global__ void kBucketKey(KArgs a, int* key, int* rank, int* count) {
                 ib = iz * P.num_buckets_xy + iy * P.num_buckets_x + ix;
         }
     }
-    key[k] = ib;
-    rank[k] = atomicAdd(&count[ib], 1);
+    if (in) key[k] = ib;
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(in);
+    int r = 0;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int kb = __shfl(ib, leader, 64);
+        const unsigned long long run = __ballot(in && ib == kb) & pending;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&count[kb], __popcll(run));
+        base = __shfl(base, leader, 64);
+        if ((run >> lane) & 1ull) r = base + __popcll(run & ((1ull << lane) - 1ull));
+        pending &= ~run;
+    }
+    if (in) rank[k] = r;
 }
 
 // exclusive scan, three passes: 4096 elements per block
