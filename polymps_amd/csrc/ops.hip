@@ -129,11 +129,18 @@ __device__ __forceinline__ int rowStart(const KArgs& a, int ix, int iy, int iz, 
 // Decode the candidate bitmasks into the column-major index list.  Every
 // lane pulls its next neighbour in the same step and the k-th entries of all
 // lanes are stored by one instruction into one list column, so the stores
-// coalesce.  The WMAX words of a row are loaded together when the decoder
-// enters the row (WMAX >= maskW; unused words read as empty).
+// coalesce.  Each lane first packs the non-zero mask words of its stencil
+// rows, in walk order and with the slot of their bit 0, into its own stretch
+// of LDS; the next neighbour is then the lowest set bit of the current word,
+// and a spent word is replaced by the one loaded ahead of it, so a step never
+// waits on LDS or loops over empty words.  One wavefront per workgroup: a
+// lane only reads what it wrote itself, so no barrier is needed.
 template <int WMAX>
-__global__ void kMaskToList(KArgs a, const unsigned* mask, int* nbr) {
-    const int i = slotId();
+__global__ __launch_bounds__(64) void kMaskToList(KArgs a, const unsigned* mask, int* nbr) {
+    constexpr int kCap = 9 * WMAX;       // words of the 3-D stencil rows
+    constexpr int kStride = kCap + 1;    // odd (in 8-byte units): distinct banks for one index
+    __shared__ uint2 lds[64 * kStride];
+    const int i = blockIdx.x * 64 + threadIdx.x;
     if (i >= a.N || !live(a, i)) return;
     const mps_params& P = a.P;
     const int n = a.nbrCnt[i];
@@ -146,37 +153,49 @@ __global__ void kMaskToList(KArgs a, const unsigned* mask, int* nbr) {
     const int W = a.maskW;
     const size_t N = a.N;
     const unsigned* __restrict__ M = mask + i;
-    unsigned wd[WMAX];
-    int r = -1, sr = 0, w = 0;   // current row, its start slot, index of wd[0] in the row
-    auto loadRow = [&]() {
+    uint2* const E = lds + threadIdx.x * kStride;   // (word, slot of bit 0), non-zero words only
+    int m = 0;
+    if constexpr (WMAX <= 8) {
+        // every word of the rows in flight at once (one memory latency per lane)
+        unsigned wd[9][WMAX];
+        int sr[9];
 #pragma unroll
-        for (int k = 0; k < WMAX; k++) wd[k] = (k < W) ? M[(size_t)(r * W + k) * N] : 0u;
-        sr = rowStart(a, ix, iy, iz, r);
-        w = 0;
-    };
-    auto next = [&]() -> int {
-        for (;;) {
-            if (r >= 0 && wd[0] != 0u) break;
-            bool rowDone = true;
-            if (r >= 0) {
+        for (int r = 0; r < 9; r++) {
 #pragma unroll
-                for (int k = 0; k < WMAX - 1; k++) rowDone = rowDone && wd[k + 1] == 0u;
-            }
-            if (rowDone) {
-                if (++r == rows) return -1;   // not reached: n bits are set
-                loadRow();
-            } else {
-#pragma unroll
-                for (int k = 0; k < WMAX - 1; k++) wd[k] = wd[k + 1];
-                wd[WMAX - 1] = 0u;
-                w++;
-            }
+            for (int k = 0; k < WMAX; k++) wd[r][k] = (r < rows && k < W) ? M[(size_t)(r * W + k) * N] : 0u;
+            sr[r] = (r < rows) ? rowStart(a, ix, iy, iz, r) : 0;
         }
-        const int b = __builtin_ctz(wd[0]);
-        wd[0] &= wd[0] - 1;
-        return sr + 32 * w + b;
+#pragma unroll
+        for (int r = 0; r < 9; r++)
+#pragma unroll
+            for (int k = 0; k < WMAX; k++)
+                if (wd[r][k] != 0u) E[m++] = make_uint2(wd[r][k], (unsigned)(sr[r] + 32 * k));
+    } else {
+        for (int r = 0; r < rows; r++) {
+            unsigned wd[WMAX];
+#pragma unroll
+            for (int k = 0; k < WMAX; k++) wd[k] = (k < W) ? M[(size_t)(r * W + k) * N] : 0u;
+            const int sr = rowStart(a, ix, iy, iz, r);
+#pragma unroll
+            for (int k = 0; k < WMAX; k++)
+                if (wd[k] != 0u) E[m++] = make_uint2(wd[k], (unsigned)(sr + 32 * k));
+        }
+    }
+    // m >= 1 words hold the n set bits
+    int q = 0;
+    uint2 cur = E[0];
+    uint2 ahead = E[m > 1 ? 1 : 0];
+    auto next = [&]() -> int {
+        if (cur.x == 0u) {
+            cur = ahead;
+            ++q;
+            ahead = E[q + 1 < m ? q + 1 : q];
+        }
+        const int b = __builtin_ctz(cur.x);
+        cur.x &= cur.x - 1u;
+        return (int)cur.y + b;
     };
-    constexpr int kChunk = 16;
+    constexpr int kChunk = 8;
     for (int c0 = 0; c0 < n; c0 += kChunk) {
         int e[kChunk];
 #pragma unroll
@@ -1782,14 +1801,15 @@ int launchBuildList(Ctx* ctx) {
         ctx->listValid = true;
         return MPS_OK;
     }
+    const int gw = gridFor(ctx->N, 64);
     hipLaunchKernelGGL(kBuildList, dim3(gridFor(ctx->N, kWave)), dim3(kWave), 0, ctx->stream, a, ctx->nbrMask,
                        ctx->nbrCnt, r2);
     if (ctx->maskW <= 4)
-        hipLaunchKernelGGL(kMaskToList<4>, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr);
+        hipLaunchKernelGGL(kMaskToList<4>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr);
     else if (ctx->maskW <= 8)
-        hipLaunchKernelGGL(kMaskToList<8>, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr);
+        hipLaunchKernelGGL(kMaskToList<8>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr);
     else if (ctx->maskW <= 16)
-        hipLaunchKernelGGL(kMaskToList<16>, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr);
+        hipLaunchKernelGGL(kMaskToList<16>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr);
     else {
         ctx->err = "neighbour bitmask rows wider than 512 candidates are not supported";
         return MPS_E_UNSUPPORTED;
