@@ -35,6 +35,7 @@ KArgs Ctx::args() const {
     a.nbr = nbr;
     a.nbrCnt = nbrCnt;
     a.P4 = P4;
+    a.V4 = V4;
     a.K = K;
     a.mV = mV;
     a.mF = mF;
@@ -250,6 +251,22 @@ static int calcPress(Ctx* ctx) {
     return ppeSolve(ctx);
 }
 
+// Skin (m) of a neighbour list kept over the step, 0 to build the list after
+// the prediction as usual.  Automatic: three times the largest displacement
+// the last velocity maximum and gravity give in one step, plus 0.002 l0; a
+// faster flow (skin above 0.25 l0) builds after the prediction, where the
+// larger list would cost its consumers more than the second build saves.
+// The refresh checks every particle's actual displacement, so the choice only
+// affects speed.
+static double keptListSkin(const Ctx* ctx) {
+    const mps_params& P = ctx->P;
+    if (ctx->listSkin == 0.0 || P.domain_type_bc == 1 || P.collision_type != MPS_COLLISION_PC) return 0.0;
+    if (ctx->listSkin > 0.0) return ctx->listSkin * P.part_dist;
+    const double g = std::sqrt(P.gravity[0] * P.gravity[0] + P.gravity[1] * P.gravity[1] + P.gravity[2] * P.gravity[2]);
+    const double skin = 3.0 * P.time_step * (ctx->velMaxHost + P.time_step * g) + 0.002 * P.part_dist;
+    return skin > 0.25 * P.part_dist ? 0.0 : skin;
+}
+
 static int runStep(Ctx* ctx) {
     const mps_params& P = ctx->P;
     int rc;
@@ -258,6 +275,9 @@ static int runStep(Ctx* ctx) {
         if ((rc = launchOp(ctx, name))) return rc; \
     } while (0)
     if ((rc = launchBucketSort(ctx))) return rc;
+    const double skin = keptListSkin(ctx);
+    ctx->lastSkin = skin;
+    if (skin > 0.0 && (rc = launchListKeepStart(ctx, skin))) return rc;
     OP("viscosity");
     OP("add_gravity");
     if (P.relax_press < 1.0) {
@@ -266,7 +286,10 @@ static int runStep(Ctx* ctx) {
     }
     OP("prediction");
     if ((rc = launchCompaction(ctx))) return rc;
-    if (P.collision_type == MPS_COLLISION_PC) {
+    if (skin > 0.0) {
+        if ((rc = launchListKeepRefresh(ctx, skin))) return rc;
+        OP("collision");
+    } else if (P.collision_type == MPS_COLLISION_PC) {
         if ((rc = launchBuildList(ctx))) return rc;
         OP("collision");
     } else {
@@ -410,7 +433,7 @@ int mps_destroy(mps_ctx* ctx) {
                     ctx->aVal, ctx->aCol, ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->aVal32, ctx->aDiagInv32,
                     ctx->aDiag32, ctx->cgR, ctx->cgP,
                     ctx->cgZ, ctx->cgQ, ctx->cgInvD, ctx->cgX, ctx->fR, ctx->fP, ctx->fZ, ctx->fQ, ctx->fX,
-                    ctx->bcgR0, ctx->bcgY, ctx->bcgS, ctx->bcgT,
+                    ctx->bcgR0, ctx->bcgY, ctx->bcgS, ctx->bcgT, ctx->patchList, ctx->V4,
                     ctx->ds, ctx->is, ctx->red};
     for (void* b : bufs)
         if (b) hipFree(b);
@@ -574,6 +597,9 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "true_residual") *out = ctx->trueResidual;
     else if (s == "fallback") *out = ctx->fallbackUsed;
     else if (s == "list_max") *out = ctx->hostInts[I_LIST_MAX];
+    else if (s == "list_skin") *out = ctx->lastSkin / ctx->P.part_dist;
+    else if (s == "list_rebuilt") *out = ctx->lastSkin > 0.0 ? ctx->hostInts[I_REBUILD] : 1;
+    else if (s == "list_patched") *out = ctx->lastSkin > 0.0 ? ctx->hostInts[I_PATCHCNT] : 0;
     else if (s == "list_capacity") *out = 32 * ctx->maskW;
     else if (s == "capacity") *out = ctx->N;
     else if (s == "nnz") *out = ctx->hostInts[I_NNZ];
@@ -613,6 +639,9 @@ int mps_set_scalar(mps_ctx* ctx, const char* name, double value) {
             return MPS_E_ARG;
         }
         ctx->P.learned_mode = (int)value;
+    } else if (s == "list_skin") {
+        // neighbour list kept over the step: skin in units of l0, < 0 automatic, 0 off
+        ctx->listSkin = value;
     } else if (s == "ppe_warm_start") {
         // initial guess of each pressure solve: 0 previous solution, 1 linear, 2 quadratic extrapolation
         if (value != 0.0 && value != 1.0 && value != 2.0) {

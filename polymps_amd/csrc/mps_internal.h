@@ -41,7 +41,7 @@ enum DScalar { S_VELMAX, S_ABSNEW, S_ABSOLD, S_RHSN2, S_THR, S_RN2, S_ALPHA, S_B
                S_RELTOL, S_RHO, S_RHOOLD, S_BW, S_R0N2, NDS };
 // device scalar slots (int)
 enum IScalar { I_NLIVE, I_OUTCNT, I_OVERFLOW, I_CG_ITER, I_CG_DONE, I_COMPACT_OVF, I_LIST_MAX, I_NNZ, I_NESC,
-               I_BCG_RESTART, I_BCG_RESTARTS, NIS };
+               I_BCG_RESTART, I_BCG_RESTARTS, I_REBUILD, I_PATCHCNT, NIS };
 
 constexpr int kWallTable = 50;
 // escaped (non-delta) columns per PPE row: the first entry and stencil-row
@@ -69,6 +69,7 @@ struct KArgs {
     const int* nbr;           // neighbour list, column-major [K][N] (decoded from the bitmasks)
     const int* nbrCnt;
     double4* P4;              // (x, y, z, type) per slot, written by the list build; valid with the list
+    const double4* V4;        // (u, v, w, 0) per slot, packed for the list-based viscosity
     int K;                    // PPE row capacity (ELL width)
     const double* mV;      // mesh
     const int* mF;
@@ -108,14 +109,20 @@ struct Ctx {
     int* slotOf = nullptr;      // N (canonical -> slot)
     int* scanTmp = nullptr;
     int* outList = nullptr;     // compaction candidates
+    int* patchList = nullptr;   // N: particles whose kept list is rebuilt alone (allocated on first use)
     // neighbour list
     unsigned* nbrMask = nullptr;
     int maskW = 0;
     int* nbr = nullptr;
     int* nbrCnt = nullptr;
     double4* P4 = nullptr;
+    double4* V4 = nullptr;      // packed velocities (viscosity with the list; allocated on first use)
     int K = 0;
     bool listValid = false;
+    // list kept over the time step: skin in units of l0 (< 0 automatic from
+    // the last velocity maximum, 0 off) and the skin of the last step (m)
+    double listSkin = -1.0;
+    double lastSkin = 0.0;
     // mesh
     double* mV = nullptr;
     int* mF = nullptr;
@@ -200,7 +207,9 @@ inline int gridFor(int n, int block) { return (n + block - 1) / block; }
 
 // launchers (defined in the .hip translation units)
 int launchBucketSort(Ctx* ctx);
-int launchBuildList(Ctx* ctx);
+int launchBuildList(Ctx* ctx, double skin = 0.0, const int* gate = nullptr);
+int launchListKeepStart(Ctx* ctx, double skin);
+int launchListKeepRefresh(Ctx* ctx, double skin);
 int launchOp(Ctx* ctx, const char* op);
 int ppeAssemble(Ctx* ctx);
 int ppeSolve(Ctx* ctx);

@@ -24,7 +24,8 @@ constexpr int kBlock = 256;
 // workgroups (coopRows).
 constexpr int kWave = 64;
 
-__global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int* cnt, double r2) {
+__global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int* cnt, double r2, const int* gate) {
+    if (gate != nullptr && *gate == 0) return;   // conditional rebuild not needed
     __shared__ double4 stage[kWave];
     const int i = blockIdx.x * kWave + threadIdx.x;
     const bool inRange = i < a.N;
@@ -136,7 +137,8 @@ __device__ __forceinline__ int rowStart(const KArgs& a, int ix, int iy, int iz, 
 // waits on LDS or loops over empty words.  One wavefront per workgroup: a
 // lane only reads what it wrote itself, so no barrier is needed.
 template <int WMAX>
-__global__ __launch_bounds__(64) void kMaskToList(KArgs a, const unsigned* mask, int* nbr) {
+__global__ __launch_bounds__(64) void kMaskToList(KArgs a, const unsigned* mask, int* nbr, const int* gate) {
+    if (gate != nullptr && *gate == 0) return;
     constexpr int kCap = 9 * WMAX;       // words of the 3-D stencil rows
     constexpr int kStride = kCap + 1;    // odd (in 8-byte units): distinct banks for one index
     __shared__ uint2 lds[64 * kStride];
@@ -224,22 +226,37 @@ __global__ void kViscosity(KArgs a) {
     const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double ax = 0, ay = 0, az = 0;
-    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+    // with the list, the velocities come packed from V4 (one gather per neighbour)
+    const double4* __restrict__ V4 = a.V4;
+    forNbrsPosType<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj, int tj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
         const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
         if (d2 < P.re_l2 && (d2 < dm2 || pw)) {
             const double wL = wgt(sqrt(d2), P.re_l, P.weight_type);
-            const double nu = (a.I[ITYPE][j] == P.wall) ? nuW : nuF;
-            ax += (a.D[DU][j] - ux) * wL * nu;
-            ay += (a.D[DV][j] - uy) * wL * nu;
-            az += (a.D[DW][j] - uz) * wL * nu;
+            const double nu = (tj == P.wall) ? nuW : nuF;
+            double uj, vj, wj;
+            if (LIST) {
+                const double4 q = V4[j];
+                uj = q.x; vj = q.y; wj = q.z;
+            } else {
+                uj = a.D[DU][j]; vj = a.D[DV][j]; wj = a.D[DW][j];
+            }
+            ax += (uj - ux) * wL * nu;
+            ay += (vj - uy) * wL * nu;
+            az += (wj - uz) * wL * nu;
         }
     });
     a.D[DAX][i] = P.coeff_visc_multiphase * ax;
     a.D[DAY][i] = P.coeff_visc_multiphase * ay;
     a.D[DAZ][i] = P.coeff_visc_multiphase * az;
+}
+
+__global__ void kPackVel(KArgs a, double4* v4) {
+    const int i = slotId();
+    if (i >= a.N) return;
+    v4[i] = make_double4(a.D[DU][i], a.D[DV][i], a.D[DW][i], 0.0);
 }
 
 // The same from a cooperative bucket walk (one-wave workgroups): the
@@ -418,6 +435,7 @@ __global__ void kOutFlag(KArgs a, int* outList, int cap) {
 __global__ void kCompact(KArgs a, int* outList, int cap, const int* slotOf) {
     int n = a.is[I_OUTCNT];
     if (n == 0) return;
+    a.is[I_REBUILD] = 1;   // particles change slots: a list kept over the step is void
     if (n > cap) n = cap;
     for (int p = 1; p < n; p++) {   // insertion sort (few entries)
         const int v = outList[p];
@@ -1788,13 +1806,18 @@ __global__ void kWallDummyVelocity(KArgs a) {
     } while (0)
 #define LW(K) hipLaunchKernelGGL(K<false>, dim3(g), dim3(kBlock), 0, ctx->stream, a)
 
-int launchBuildList(Ctx* ctx) {
+static double listRadius2(const mps_params& P) {
+    double r2 = std::max(P.re_l2, P.re_s2);
+    r2 = std::max(r2, P.dist_collision_limit2);
+    return std::max(r2, P.part_dist * P.part_dist);
+}
+
+int launchBuildList(Ctx* ctx, double skin, const int* gate) {
     KArgs a = ctx->args();
     const int g = gridFor(ctx->N, kBlock);
     const mps_params& P = ctx->P;
-    double r2 = std::max(P.re_l2, P.re_s2);
-    r2 = std::max(r2, P.dist_collision_limit2);
-    r2 = std::max(r2, P.part_dist * P.part_dist);
+    double r2 = listRadius2(P);
+    if (skin > 0.0) r2 = (std::sqrt(r2) + skin) * (std::sqrt(r2) + skin);
     if (P.domain_type_bc == 1) {
         hipLaunchKernelGGL(kBuildListWalk, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbr, ctx->nbrCnt, r2);
         MPS_HIP(hipGetLastError());
@@ -1803,17 +1826,106 @@ int launchBuildList(Ctx* ctx) {
     }
     const int gw = gridFor(ctx->N, 64);
     hipLaunchKernelGGL(kBuildList, dim3(gridFor(ctx->N, kWave)), dim3(kWave), 0, ctx->stream, a, ctx->nbrMask,
-                       ctx->nbrCnt, r2);
+                       ctx->nbrCnt, r2, gate);
     if (ctx->maskW <= 4)
-        hipLaunchKernelGGL(kMaskToList<4>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr);
+        hipLaunchKernelGGL(kMaskToList<4>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr, gate);
     else if (ctx->maskW <= 8)
-        hipLaunchKernelGGL(kMaskToList<8>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr);
+        hipLaunchKernelGGL(kMaskToList<8>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr, gate);
     else if (ctx->maskW <= 16)
-        hipLaunchKernelGGL(kMaskToList<16>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr);
+        hipLaunchKernelGGL(kMaskToList<16>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr, gate);
     else {
         ctx->err = "neighbour bitmask rows wider than 512 candidates are not supported";
         return MPS_E_UNSUPPORTED;
     }
+    MPS_HIP(hipGetLastError());
+    ctx->listValid = true;
+    return MPS_OK;
+}
+
+// ------------------------------------------------- list kept over the step
+// The list of a time step can be built once, right after the bucket sort, at
+// the radius plus a skin, and serve the viscosity at the old positions as well
+// as every operator after the prediction:
+//  * every consumer tests its own radius on the positions it is given, so a
+//    superset of the neighbours, in the same slot order, gives bit-identical
+//    sums;
+//  * candidates come from the stencil of the particle's current bucket over
+//    the buckets of the sort (src/MpsBucket.cpp:299-353 runs once per step),
+//    so a list stays a superset while neither end of a pair moved more than
+//    half the skin and the particle kept its stencil centre.
+// After the prediction kListRefresh copies the new positions into P4, raises
+// the rebuild flag when a particle moved farther (or when the compaction moved
+// particles between slots) and queues the particles whose stencil centre
+// changed; the full rebuild then runs only under the flag, and kPatchList
+// rebuilds the queued particles' lists alone.
+__global__ void kListFlagsReset(int* is) {
+    is[I_REBUILD] = 0;
+    is[I_PATCHCNT] = 0;
+}
+
+__device__ __forceinline__ int3 stencilCentre(const mps_params& P, double x, double y, double z) {
+    return make_int3((int)((x - P.domain_min[0]) * P.inv_bucket_side + KEPS),
+                     (int)((y - P.domain_min[1]) * P.inv_bucket_side + KEPS),
+                     ((int)P.dim == 3) ? (int)((z - P.domain_min[2]) * P.inv_bucket_side + KEPS) : 0);
+}
+
+__global__ __launch_bounds__(256) void kListRefresh(KArgs a, int* patch, double half2) {
+    const int i = slotId();
+    if (i >= a.N) return;
+    const double x = PX(i), y = PY(i), z = PZ(i);
+    const double4 q = a.P4[i];
+    a.P4[i] = make_double4(x, y, z, (double)a.I[ITYPE][i]);
+    if (!live(a, i)) return;
+    const double dx = x - q.x, dy = y - q.y, dz = z - q.z;
+    if (dx * dx + dy * dy + dz * dz > half2) a.is[I_REBUILD] = 1;
+    const int3 c1 = stencilCentre(a.P, x, y, z), c0 = stencilCentre(a.P, q.x, q.y, q.z);
+    if (c1.x != c0.x || c1.y != c0.y || c1.z != c0.z) {
+        const int s = atomicAdd(&a.is[I_PATCHCNT], 1);
+        patch[s] = i;
+    }
+}
+
+// The list of each queued particle from its own stencil walk, in walk order
+// (the order of kBuildList + kMaskToList).
+__global__ __launch_bounds__(256) void kPatchList(KArgs a, int* nbr, int* cnt, const int* patch, double r2) {
+    if (a.is[I_REBUILD]) return;   // the whole list was rebuilt
+    const int t = slotId();
+    if (t >= a.is[I_PATCHCNT]) return;
+    const int i = patch[t];
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const size_t N = a.N;
+    int c = 0;
+    walkCellsPos(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        if (!(v0 * v0 + v1 * v1 + v2 * v2 < r2)) return;
+        if (c < a.K) nbr[(size_t)c * N + i] = j;
+        c++;
+    });
+    if (c > a.K) {
+        atomicMax(&a.is[I_OVERFLOW], c);
+        c = a.K;
+    }
+    atomicMax(&a.is[I_LIST_MAX], c);
+    cnt[i] = c;
+}
+
+int launchListKeepStart(Ctx* ctx, double skin) {
+    hipLaunchKernelGGL(kListFlagsReset, dim3(1), dim3(1), 0, ctx->stream, ctx->is);
+    MPS_HIP(hipGetLastError());
+    return launchBuildList(ctx, skin, nullptr);
+}
+
+int launchListKeepRefresh(Ctx* ctx, double skin) {
+    if (!ctx->patchList) MPS_HIP(hipMalloc(&ctx->patchList, sizeof(int) * (size_t)ctx->N));
+    KArgs a = ctx->args();
+    const int g = gridFor(ctx->N, kBlock);
+    const double h = 0.5 * skin;
+    hipLaunchKernelGGL(kListRefresh, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->patchList, h * h);
+    MPS_HIP(hipGetLastError());
+    int rc = launchBuildList(ctx, 0.0, ctx->is + I_REBUILD);
+    if (rc) return rc;
+    hipLaunchKernelGGL(kPatchList, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbr, ctx->nbrCnt, ctx->patchList,
+                       listRadius2(ctx->P));
     MPS_HIP(hipGetLastError());
     ctx->listValid = true;
     return MPS_OK;
@@ -1862,7 +1974,12 @@ int launchOp(Ctx* ctx, const char* op) {
     const std::string o(op);
     const mps_params& P = ctx->P;
     if (o == "viscosity") {
-        if (ctx->listValid) L1(kViscosity<true>);
+        if (ctx->listValid) {
+            if (!ctx->V4) MPS_HIP(hipMalloc(&ctx->V4, sizeof(double4) * (size_t)ctx->N));
+            a.V4 = ctx->V4;
+            hipLaunchKernelGGL(kPackVel, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->V4);
+            L1(kViscosity<true>);
+        }
         else if (ctx->P.domain_type_bc == 1) LW(kViscosity);   // walk through the periodic ghosts
         else hipLaunchKernelGGL(kViscosityWalk, dim3(gridFor(ctx->N, kWave)), dim3(kWave), 0, ctx->stream, a);
     }

@@ -1,0 +1,94 @@
+"""Neighbour list kept over the time step (needs a gfx950 GPU).
+
+A step can build its list once, after the bucket sort, at the list radius
+plus a skin, and use it for the viscosity at the old positions and for every
+operator after the prediction.  Each consumer tests its own radius and the
+list keeps slot order, so the kept list must give bit-identical states to the
+list built after the prediction (list_skin 0), whether the step keeps it,
+patches the particles whose stencil centre moved, or rebuilds it because a
+particle moved farther than half the skin or the compaction moved slots.
+"""
+import numpy as np
+import pytest
+
+from polymps_amd import configs
+from polymps_amd.lib import Simulation
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["pos", "vel", "press", "pndi", "pnd_small", "bc", "num_neigh", "npcd", "ppe_x", "force_wall"]
+
+
+def _state(g):
+    return {f: g.get(f).copy() for f in FIELDS}
+
+
+def _assert_same(a, b, where):
+    for f in FIELDS:
+        if not np.array_equal(a[f], b[f]):
+            d = np.nonzero(a[f] != b[f])[0]
+            raise AssertionError(f"{where}: field {f} differs at {len(d)} entries, first {d[:5]}")
+
+
+def _pair(w, skin):
+    plain = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh)
+    kept = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh)
+    plain.set_scalar("list_skin", 0)
+    kept.set_scalar("list_skin", skin)
+    plain.call("init")
+    kept.call("init")
+    return plain, kept
+
+
+@pytest.mark.parametrize("skin", [-1.0, 1e-4, 0.2])
+def test_kept_list_matches_plain(built, skin):
+    """Automatic skin (kept, a few particles patched per step), a skin so thin
+    that most steps rebuild, and a wide fixed skin."""
+    w = configs.small_dam3d(l0=0.0125, dt=5e-4)
+    plain, kept = _pair(w, skin)
+    print(f"kept list skin {skin}", flush=True)
+    patched = rebuilt = 0
+    for s in range(120):
+        plain.call("step")
+        kept.call("step")
+        assert kept.scalar("list_skin") > 0
+        patched += int(kept.scalar("list_patched"))
+        rebuilt += int(kept.scalar("list_rebuilt"))
+        if s % 20 == 19:
+            _assert_same(_state(plain), _state(kept), f"step {s + 1}")
+    print(f"patched {patched} rebuilt {rebuilt} steps of 120", flush=True)
+    assert float(np.abs(plain.get("vel")).max()) > 0.1   # the flow has developed
+    if skin == -1.0 or skin == 0.2:
+        assert patched > 0
+        assert rebuilt < 10
+    else:
+        assert rebuilt > 60
+    plain.close()
+    kept.close()
+
+
+def test_kept_list_with_compaction(built):
+    """Particles outside the tank moving out of the domain: the compaction
+    moves slots and the kept list must be rebuilt in those steps."""
+    w = configs.small_dam3d(l0=0.0125)
+    extra = np.array([[-0.002 - 0.0007 * k, 0.05 + 0.01 * k, 0.1] for k in range(6)])
+    pos = np.concatenate([w.pos.reshape(-1, 3), extra]).reshape(-1)
+    vel = np.concatenate([w.vel.reshape(-1, 3), np.tile([-0.5, 0.0, 0.0], (6, 1))]).reshape(-1)
+    types = np.concatenate([w.types, np.zeros(6, dtype=w.types.dtype) + w.types[0]])
+    w.pos, w.vel, w.types = pos, vel, types
+    plain, kept = _pair(w, 0.2)
+    n0 = int(kept.scalar("num_particles"))
+    compactions = 0
+    for s in range(40):
+        plain.call("step")
+        kept.call("step")
+        n = int(kept.scalar("num_particles"))
+        assert n == int(plain.scalar("num_particles"))
+        if n < n0:
+            assert kept.scalar("list_rebuilt") == 1, f"step {s + 1}"
+            compactions += 1
+            n0 = n
+        _assert_same(_state(plain), _state(kept), f"step {s + 1}")
+    assert compactions >= 1
+    plain.close()
+    kept.close()
