@@ -34,6 +34,11 @@ KArgs Ctx::args() const {
     a.maskW = maskW;
     a.nbr = nbr;
     a.nbrCnt = nbrCnt;
+    // periodic domains build the full list only, which serves as the small one
+    const bool oneList = P.domain_type_bc == 1 || nbrS == nullptr;
+    a.nbrS = oneList ? nbr : nbrS;
+    a.nbrCntS = oneList ? nbrCnt : nbrCntS;
+    a.KS = oneList ? K : KS;
     a.P4 = P4;
     a.V4 = V4;
     a.K = K;
@@ -376,6 +381,10 @@ int mps_create(mps_ctx** out, const mps_params* p, int n, int device) {
     const double rr = std::max(p->re_l, p->re_s) / p->part_dist;
     const double ball = ((int)p->dim == 3) ? 4.18879 * rr * rr * rr : 3.14159 * rr * rr;
     ctx->K = std::max(32, ((int)(ball * 2.0) + 31) / 32 * 32);
+    // the small-radius list (re_s, collision distance)
+    const double rs = std::max(p->re_s / p->part_dist, p->dist_limit_ratio);
+    const double ballS = ((int)p->dim == 3) ? 4.18879 * rs * rs * rs : 3.14159 * rs * rs;
+    ctx->KS = std::min(ctx->K, std::max(32, ((int)(ballS * 2.0) + 31) / 32 * 32));
     // neighbour bitmasks: a stencil row spans 3 buckets of side bucket_side,
     // (bucket_side / l0)^dim particles each at rest; 1.6x that leaves room for
     // compression (the build reports an overflow as MPS_E_CAPACITY)
@@ -386,6 +395,10 @@ int mps_create(mps_ctx** out, const mps_params* p, int n, int device) {
     MPS_HIP(hipMalloc(&ctx->nbr, sizeof(int) * (size_t)ctx->K * n));
     MPS_HIP(hipMalloc(&ctx->nbrCnt, ni));
     MPS_HIP(hipMemsetAsync(ctx->nbrCnt, 0, ni, ctx->stream));
+    MPS_HIP(hipMalloc(&ctx->nbrMaskS, sizeof(unsigned) * (size_t)rows * ctx->maskW * n));
+    MPS_HIP(hipMalloc(&ctx->nbrS, sizeof(int) * (size_t)ctx->KS * n));
+    MPS_HIP(hipMalloc(&ctx->nbrCntS, ni));
+    MPS_HIP(hipMemsetAsync(ctx->nbrCntS, 0, ni, ctx->stream));
     MPS_HIP(hipMalloc(&ctx->P4, sizeof(double4) * (size_t)n));
     MPS_HIP(hipMalloc(&ctx->aVal, sizeof(double) * (size_t)ctx->K * n));
     MPS_HIP(hipMalloc(&ctx->aCol, sizeof(unsigned short) * (size_t)ctx->K * n));
@@ -433,7 +446,7 @@ int mps_destroy(mps_ctx* ctx) {
                     ctx->aVal, ctx->aCol, ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->aVal32, ctx->aDiagInv32,
                     ctx->aDiag32, ctx->cgR, ctx->cgP,
                     ctx->cgZ, ctx->cgQ, ctx->cgInvD, ctx->cgX, ctx->fR, ctx->fP, ctx->fZ, ctx->fQ, ctx->fX,
-                    ctx->bcgR0, ctx->bcgY, ctx->bcgS, ctx->bcgT, ctx->patchList, ctx->V4,
+                    ctx->bcgR0, ctx->bcgY, ctx->bcgS, ctx->bcgT, ctx->patchList, ctx->V4, ctx->nbrMaskS, ctx->nbrS, ctx->nbrCntS,
                     ctx->ds, ctx->is, ctx->red};
     for (void* b : bufs)
         if (b) hipFree(b);

@@ -172,12 +172,13 @@ __device__ __forceinline__ void walkCellsPos(const KArgs& a, int i, double x, do
 }
 
 // The per-step neighbour list of slot i: column-major [K][N] slot indices in
-// bucket-walk order, nbrCnt[i] entries.  Four entries and their packed
+// bucket-walk order, cnt[i] entries (the full list nbr / nbrCnt, or the
+// small-radius list nbrS / nbrCntS).  Four entries and their packed
 // (x, y, z, type) copies are loaded per batch before g(j, q) runs for each.
 template <class G>
-__device__ __forceinline__ void listNbrsQ(const KArgs& a, int i, G&& g) {
-    const int n = a.nbrCnt[i];
-    const int* __restrict__ col = a.nbr + i;
+__device__ __forceinline__ void listNbrsQ(const KArgs& a, const int* nbr, const int* cnt, int i, G&& g) {
+    const int n = cnt[i];
+    const int* __restrict__ col = nbr + i;
     const double4* __restrict__ P4 = a.P4;
     const size_t N = a.N;
     int c = 0;
@@ -216,7 +217,15 @@ __device__ __forceinline__ void listNbrsQ(const KArgs& a, int i, G&& g) {
 // (LIST) or from a bucket walk.
 template <bool LIST, class F>
 __device__ __forceinline__ void forNbrsPos(const KArgs& a, int i, double x, double y, double z, F&& f) {
-    if (LIST) listNbrsQ(a, i, [&](int j, const double4& q) { f(j, q.x, q.y, q.z); });
+    if (LIST) listNbrsQ(a, a.nbr, a.nbrCnt, i, [&](int j, const double4& q) { f(j, q.x, q.y, q.z); });
+    else walkCellsPos(a, i, x, y, z, f);
+}
+
+// The same for operators whose radii are at most the small one (re_s and
+// the collision distance): with the list, the small-radius list.
+template <bool LIST, class F>
+__device__ __forceinline__ void forNbrsPosS(const KArgs& a, int i, double x, double y, double z, F&& f) {
+    if (LIST) listNbrsQ(a, a.nbrS, a.nbrCntS, i, [&](int j, const double4& q) { f(j, q.x, q.y, q.z); });
     else walkCellsPos(a, i, x, y, z, f);
 }
 
@@ -224,7 +233,7 @@ __device__ __forceinline__ void forNbrsPos(const KArgs& a, int i, double x, doub
 template <bool LIST, class F>
 __device__ __forceinline__ void forNbrsPosType(const KArgs& a, int i, double x, double y, double z, F&& f) {
     if (LIST) {
-        listNbrsQ(a, i, [&](int j, const double4& q) { f(j, q.x, q.y, q.z, (int)q.w); });
+        listNbrsQ(a, a.nbr, a.nbrCnt, i, [&](int j, const double4& q) { f(j, q.x, q.y, q.z, (int)q.w); });
     } else {
         const int* __restrict__ T = a.I[ITYPE];
         walkCellsPos(a, i, x, y, z, [&](int j, double xj, double yj, double zj) { f(j, xj, yj, zj, T[j]); });

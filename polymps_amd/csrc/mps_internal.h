@@ -68,9 +68,12 @@ struct KArgs {
     int maskW;                // 32-bit words per stencil row
     const int* nbr;           // neighbour list, column-major [K][N] (decoded from the bitmasks)
     const int* nbrCnt;
+    const int* nbrS;          // entries within the small radius (re_s, collision), column-major [KS][N]
+    const int* nbrCntS;
     double4* P4;              // (x, y, z, type) per slot, written by the list build; valid with the list
     const double4* V4;        // (u, v, w, 0) per slot, packed for the list-based viscosity
     int K;                    // PPE row capacity (ELL width)
+    int KS;                   // small-radius list capacity
     const double* mV;      // mesh
     const int* mF;
     const double* mN;
@@ -115,6 +118,10 @@ struct Ctx {
     int maskW = 0;
     int* nbr = nullptr;
     int* nbrCnt = nullptr;
+    unsigned* nbrMaskS = nullptr;   // small-radius bitmasks, list and counts
+    int* nbrS = nullptr;
+    int* nbrCntS = nullptr;
+    int KS = 0;
     double4* P4 = nullptr;
     double4* V4 = nullptr;      // packed velocities (viscosity with the list; allocated on first use)
     int K = 0;

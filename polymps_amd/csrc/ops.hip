@@ -24,7 +24,10 @@ constexpr int kBlock = 256;
 // workgroups (coopRows).
 constexpr int kWave = 64;
 
-__global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int* cnt, double r2, const int* gate) {
+// A second mask, maskS, marks the candidates inside the small radius r2s
+// (<= r2) for the small-radius list.
+__global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int* cnt, unsigned* maskS, int* cntS,
+                                                    double r2, double r2s, const int* gate) {
     if (gate != nullptr && *gate == 0) return;   // conditional rebuild not needed
     __shared__ double4 stage[kWave];
     const int i = blockIdx.x * kWave + threadIdx.x;
@@ -32,15 +35,19 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
     const double xi = inRange ? PX(i) : 0.0, yi = inRange ? PY(i) : 0.0, zi = inRange ? PZ(i) : 0.0;
     if (inRange) a.P4[i] = make_double4(xi, yi, zi, (double)a.I[ITYPE][i]);
     const bool act = inRange && live(a, i);
-    if (inRange && !act) cnt[i] = 0;
+    if (inRange && !act) {
+        cnt[i] = 0;
+        cntS[i] = 0;
+    }
     const double* __restrict__ X = a.D[DX];
     const double* __restrict__ Y = a.D[DY];
     const double* __restrict__ Z = a.D[DZ];
     const size_t N = a.N;
     const int W = a.maskW;
     unsigned* out = mask + (act ? i : 0);
-    int total = 0, over = 0;
-    unsigned cur = 0;
+    unsigned* outS = maskS + (act ? i : 0);
+    int total = 0, totalS = 0, over = 0;
+    unsigned cur = 0, curS = 0;
     int w = 0, row = 0, s0 = 0;
     coopRows(
         a, act, xi, yi, zi, stage, 2 * 32 * W,
@@ -51,6 +58,7 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
                 e = s + 32 * W;
             }
             cur = 0;
+            curS = 0;
             w = 0;
             row = r;
             s0 = s;
@@ -60,20 +68,29 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
             const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
             const int k = j - s0;
             cur |= ((j != i && d2 < r2) ? 1u : 0u) << (k & 31);
+            curS |= ((j != i && d2 < r2s) ? 1u : 0u) << (k & 31);
             if ((k & 31) == 31) {
                 out[(size_t)(row * W + w) * N] = cur;
+                outS[(size_t)(row * W + w) * N] = curS;
                 total += __builtin_popcount(cur);
+                totalS += __builtin_popcount(curS);
                 w++;
                 cur = 0;
+                curS = 0;
             }
         },
         [&](int r, int s, int e) {
             if ((e - s) & 31) {
                 out[(size_t)(r * W + w) * N] = cur;
+                outS[(size_t)(r * W + w) * N] = curS;
                 total += __builtin_popcount(cur);
+                totalS += __builtin_popcount(curS);
                 w++;
             }
-            for (; w < W; w++) out[(size_t)(r * W + w) * N] = 0u;
+            for (; w < W; w++) {
+                out[(size_t)(r * W + w) * N] = 0u;
+                outS[(size_t)(r * W + w) * N] = 0u;
+            }
         });
     if (!act) return;
     if (over > 0) atomicMax(&a.is[I_OVERFLOW], over);
@@ -81,8 +98,13 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
         atomicMax(&a.is[I_OVERFLOW], total);
         total = a.K;
     }
+    if (totalS > a.KS) {
+        atomicMax(&a.is[I_OVERFLOW], totalS);
+        totalS = a.KS;
+    }
     atomicMax(&a.is[I_LIST_MAX], total);
     cnt[i] = total;
+    cntS[i] = totalS;
 }
 
 // Periodic domains: the list is built by a per-particle walk through the
@@ -137,7 +159,8 @@ __device__ __forceinline__ int rowStart(const KArgs& a, int ix, int iy, int iz, 
 // waits on LDS or loops over empty words.  One wavefront per workgroup: a
 // lane only reads what it wrote itself, so no barrier is needed.
 template <int WMAX>
-__global__ __launch_bounds__(64) void kMaskToList(KArgs a, const unsigned* mask, int* nbr, const int* gate) {
+__global__ __launch_bounds__(64) void kMaskToList(KArgs a, const unsigned* mask, const int* cnt, int* nbr,
+                                                  const int* gate) {
     if (gate != nullptr && *gate == 0) return;
     constexpr int kCap = 9 * WMAX;       // words of the 3-D stencil rows
     constexpr int kStride = kCap + 1;    // odd (in 8-byte units): distinct banks for one index
@@ -145,7 +168,7 @@ __global__ __launch_bounds__(64) void kMaskToList(KArgs a, const unsigned* mask,
     const int i = blockIdx.x * 64 + threadIdx.x;
     if (i >= a.N || !live(a, i)) return;
     const mps_params& P = a.P;
-    const int n = a.nbrCnt[i];
+    const int n = cnt[i];
     if (n == 0) return;
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const int ix = (int)((xi - P.domain_min[0]) * P.inv_bucket_side + KEPS);
@@ -485,7 +508,7 @@ __global__ void kCollisionPC(KArgs a) {
     const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double dux = 0, duy = 0, duz = 0;
-    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+    forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -746,7 +769,7 @@ __global__ void kWallNPCD(KArgs a) {
     double n[3], R[9], ns;
     reflectSqrt(xi, xm, n, R, &ns);
     double nx = 0, ny = 0, nz = 0;
-    forNbrsPos<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
+    forNbrsPosS<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
@@ -803,7 +826,7 @@ __global__ void kCorrectionMatrix(KArgs a) {
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+    forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -1055,7 +1078,7 @@ __global__ void kMeanPndWDS(KArgs a) {
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double up = 0, dn = 0;
-    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+    forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -1093,7 +1116,7 @@ __global__ void kMeanWallPnd(KArgs a) {
     const double xi[3] = {PX(i), PY(i), PZ(i)};
     const double xm[3] = {a.D[DMX][i], a.D[DMY][i], a.D[DMZ][i]};
     double up = 0, dn = 0;
-    forNbrsPos<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
+    forNbrsPosS<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
@@ -1125,7 +1148,7 @@ __global__ void kMeanPnd(KArgs a) {
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double up = 0, dn = 0;
-    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+    forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -1158,7 +1181,7 @@ __global__ void kMeanNeighFluidPnd(KArgs a) {
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double up = 0, dn = 0;
-    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+    forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -1182,7 +1205,7 @@ __global__ void kNormalParticles(KArgs a) {
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double dx = 0.0, dy = 0.0, dz = 0.0;
-    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+    forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -1213,7 +1236,7 @@ __global__ void kWallNormalParticles(KArgs a) {
     double n[3], R[9], ns;
     reflectSqrt(xi, xm, n, R, &ns);
     double dx = 0.0, dy = 0.0, dz = 0.0;
-    forNbrsPos<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
+    forNbrsPosS<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
@@ -1359,7 +1382,7 @@ __global__ void kVelDivergence(KArgs a) {
     double MC[9];
     for (int q = 0; q < 9; q++) MC[q] = a.D[DCM0 + q][i];
     double DivV = 0.0;
-    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+    forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -1399,7 +1422,7 @@ __device__ __forceinline__ void pressGradientBody(const KArgs& a) {
     double MC[9];
     for (int q = 0; q < 9; q++) MC[q] = PRED ? 0.0 : a.D[DCM0 + q][i];
     if (gt == 0 || gt == 2)
-        forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+        forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
             const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
             const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
             const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -1410,7 +1433,7 @@ __device__ __forceinline__ void pressGradientBody(const KArgs& a) {
             }
         });
     double ax = 0, ay = 0, az = 0;
-    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+    forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -1630,7 +1653,7 @@ __global__ void kShift1(KArgs a) {
         const double ux = nu, uy = nv, uz = nw;
         const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
         double dx = 0, dy = 0, dz = 0;
-        forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+        forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
             const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
             const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
             const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -1670,7 +1693,7 @@ __global__ void kShift2Conc(KArgs a) {
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double c = 0.0;
-    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+    forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -1693,7 +1716,7 @@ __global__ void kShift2Grad(KArgs a) {
         const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
         const double ci = a.D[DCONC][i];
         double g0 = 0, g1 = 0, g2 = 0;
-        forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+        forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
             const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
             const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
             const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -1730,7 +1753,7 @@ __global__ void kShift2Wall(KArgs a) {
         double n[3], R[9], ns;
         reflectSqrt(xi, xm, n, R, &ns);
         double d0 = 0, d1 = 0, dd2 = 0;
-        forNbrsPos<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
+        forNbrsPosS<LIST>(a, i, xi[0], xi[1], xi[2], [&](int j, double xj, double yj, double zj) {
             const double v0 = xj - xi[0], v1 = yj - xi[1], v2 = zj - xi[2];
             const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
             const double w0 = xj - xm[0], w1 = yj - xm[1], w2 = zj - xm[2];
@@ -1811,13 +1834,17 @@ static double listRadius2(const mps_params& P) {
     r2 = std::max(r2, P.dist_collision_limit2);
     return std::max(r2, P.part_dist * P.part_dist);
 }
+static double listRadiusS2(const mps_params& P) { return std::max(P.re_s2, P.dist_collision_limit2); }
 
 int launchBuildList(Ctx* ctx, double skin, const int* gate) {
     KArgs a = ctx->args();
     const int g = gridFor(ctx->N, kBlock);
     const mps_params& P = ctx->P;
-    double r2 = listRadius2(P);
-    if (skin > 0.0) r2 = (std::sqrt(r2) + skin) * (std::sqrt(r2) + skin);
+    double r2 = listRadius2(P), r2s = listRadiusS2(P);
+    if (skin > 0.0) {
+        r2 = (std::sqrt(r2) + skin) * (std::sqrt(r2) + skin);
+        r2s = (std::sqrt(r2s) + skin) * (std::sqrt(r2s) + skin);
+    }
     if (P.domain_type_bc == 1) {
         hipLaunchKernelGGL(kBuildListWalk, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbr, ctx->nbrCnt, r2);
         MPS_HIP(hipGetLastError());
@@ -1826,17 +1853,22 @@ int launchBuildList(Ctx* ctx, double skin, const int* gate) {
     }
     const int gw = gridFor(ctx->N, 64);
     hipLaunchKernelGGL(kBuildList, dim3(gridFor(ctx->N, kWave)), dim3(kWave), 0, ctx->stream, a, ctx->nbrMask,
-                       ctx->nbrCnt, r2, gate);
-    if (ctx->maskW <= 4)
-        hipLaunchKernelGGL(kMaskToList<4>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr, gate);
-    else if (ctx->maskW <= 8)
-        hipLaunchKernelGGL(kMaskToList<8>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr, gate);
-    else if (ctx->maskW <= 16)
-        hipLaunchKernelGGL(kMaskToList<16>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMask, ctx->nbr, gate);
+                       ctx->nbrCnt, ctx->nbrMaskS, ctx->nbrCntS, r2, r2s, gate);
+#define M2L(WM)                                                                                              \
+    do {                                                                                                     \
+        hipLaunchKernelGGL(kMaskToList<WM>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMask, ctx->nbrCnt,  \
+                           ctx->nbr, gate);                                                                  \
+        hipLaunchKernelGGL(kMaskToList<WM>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMaskS,             \
+                           ctx->nbrCntS, ctx->nbrS, gate);                                                   \
+    } while (0)
+    if (ctx->maskW <= 4) M2L(4);
+    else if (ctx->maskW <= 8) M2L(8);
+    else if (ctx->maskW <= 16) M2L(16);
     else {
         ctx->err = "neighbour bitmask rows wider than 512 candidates are not supported";
         return MPS_E_UNSUPPORTED;
     }
+#undef M2L
     MPS_HIP(hipGetLastError());
     ctx->listValid = true;
     return MPS_OK;
@@ -1887,26 +1919,37 @@ __global__ __launch_bounds__(256) void kListRefresh(KArgs a, int* patch, double 
 
 // The list of each queued particle from its own stencil walk, in walk order
 // (the order of kBuildList + kMaskToList).
-__global__ __launch_bounds__(256) void kPatchList(KArgs a, int* nbr, int* cnt, const int* patch, double r2) {
+__global__ __launch_bounds__(256) void kPatchList(KArgs a, int* nbr, int* cnt, int* nbrS, int* cntS, const int* patch,
+                                                  double r2, double r2s) {
     if (a.is[I_REBUILD]) return;   // the whole list was rebuilt
     const int t = slotId();
     if (t >= a.is[I_PATCHCNT]) return;
     const int i = patch[t];
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const size_t N = a.N;
-    int c = 0;
+    int c = 0, cs = 0;
     walkCellsPos(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
-        if (!(v0 * v0 + v1 * v1 + v2 * v2 < r2)) return;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        if (!(d2 < r2)) return;
         if (c < a.K) nbr[(size_t)c * N + i] = j;
         c++;
+        if (d2 < r2s) {
+            if (cs < a.KS) nbrS[(size_t)cs * N + i] = j;
+            cs++;
+        }
     });
     if (c > a.K) {
         atomicMax(&a.is[I_OVERFLOW], c);
         c = a.K;
     }
+    if (cs > a.KS) {
+        atomicMax(&a.is[I_OVERFLOW], cs);
+        cs = a.KS;
+    }
     atomicMax(&a.is[I_LIST_MAX], c);
     cnt[i] = c;
+    cntS[i] = cs;
 }
 
 int launchListKeepStart(Ctx* ctx, double skin) {
@@ -1924,8 +1967,8 @@ int launchListKeepRefresh(Ctx* ctx, double skin) {
     MPS_HIP(hipGetLastError());
     int rc = launchBuildList(ctx, 0.0, ctx->is + I_REBUILD);
     if (rc) return rc;
-    hipLaunchKernelGGL(kPatchList, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbr, ctx->nbrCnt, ctx->patchList,
-                       listRadius2(ctx->P));
+    hipLaunchKernelGGL(kPatchList, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbr, ctx->nbrCnt, ctx->nbrS,
+                       ctx->nbrCntS, ctx->patchList, listRadius2(ctx->P), listRadiusS2(ctx->P));
     MPS_HIP(hipGetLastError());
     ctx->listValid = true;
     return MPS_OK;
@@ -1955,7 +1998,7 @@ __global__ void kExtrapolateWallDummy2(KArgs a) {
     const double rho = P.dns_fl1;
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     double ni = 0.0, pr = 0.0;
-    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+    forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         if (d2 < P.re_s2 && a.I[ITYPE][j] == P.fluid) {
