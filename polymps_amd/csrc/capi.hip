@@ -40,6 +40,7 @@ KArgs Ctx::args() const {
     a.nbrCntS = oneList ? nbrCnt : nbrCntS;
     a.KS = oneList ? K : KS;
     a.P4 = P4;
+    a.p4Flags = p4Flags ? 1 : 0;
     a.V4 = V4;
     a.K = K;
     a.mV = mV;
@@ -588,6 +589,7 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
     }
     // positions or types changed: the neighbour list and its packed copy are stale
     if ((fd->kind != 2 && fd->f0 == DX) || (fd->kind == 2 && fd->f0 == ITYPE)) ctx->listValid = false;
+    ctx->p4Flags = false;   // the boundary flags packed by update_bc may be stale
     MPS_HIP(hipGetLastError());
     return MPS_OK;
 }

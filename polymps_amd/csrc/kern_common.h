@@ -213,6 +213,12 @@ __device__ __forceinline__ void listNbrsQ(const KArgs& a, const int* nbr, const 
     }
 }
 
+// Particle type from the w component of P4: its low 10 bits, sign-extended
+// (types in [-512, 511]); bits 10 and up hold the flags of update_bc.
+__device__ __forceinline__ int p4Type(double w) { return ((((int)w) & 1023) ^ 512) - 512; }
+__device__ __forceinline__ int p4FlagBits(double w) { return ((int)w) >> 10; }
+__device__ __forceinline__ int p4TypeOf(const KArgs& a, double w) { return a.p4Flags ? p4Type(w) : (int)w; }
+
 // Neighbours of slot i with their positions, f(j, xj, yj, zj): from the list
 // (LIST) or from a bucket walk.
 template <bool LIST, class F>
@@ -233,10 +239,24 @@ __device__ __forceinline__ void forNbrsPosS(const KArgs& a, int i, double x, dou
 template <bool LIST, class F>
 __device__ __forceinline__ void forNbrsPosType(const KArgs& a, int i, double x, double y, double z, F&& f) {
     if (LIST) {
-        listNbrsQ(a, a.nbr, a.nbrCnt, i, [&](int j, const double4& q) { f(j, q.x, q.y, q.z, (int)q.w); });
+        listNbrsQ(a, a.nbr, a.nbrCnt, i, [&](int j, const double4& q) { f(j, q.x, q.y, q.z, p4TypeOf(a, q.w)); });
     } else {
         const int* __restrict__ T = a.I[ITYPE];
         walkCellsPos(a, i, x, y, z, [&](int j, double xj, double yj, double zj) { f(j, xj, yj, zj, T[j]); });
+    }
+}
+
+// Neighbours with position, type and the update_bc flags of P4.w,
+// f(j, xj, yj, zj, tj, flags); flags is 0 unless the list carries them.
+template <bool LIST, class F>
+__device__ __forceinline__ void forNbrsPosTypeFlags(const KArgs& a, int i, double x, double y, double z, F&& f) {
+    if (LIST) {
+        listNbrsQ(a, a.nbr, a.nbrCnt, i, [&](int j, const double4& q) {
+            f(j, q.x, q.y, q.z, p4TypeOf(a, q.w), a.p4Flags ? p4FlagBits(q.w) : 0);
+        });
+    } else {
+        const int* __restrict__ T = a.I[ITYPE];
+        walkCellsPos(a, i, x, y, z, [&](int j, double xj, double yj, double zj) { f(j, xj, yj, zj, T[j], 0); });
     }
 }
 

@@ -70,7 +70,9 @@ struct KArgs {
     const int* nbrCnt;
     const int* nbrS;          // entries within the small radius (re_s, collision), column-major [KS][N]
     const int* nbrCntS;
-    double4* P4;              // (x, y, z, type) per slot, written by the list build; valid with the list
+    double4* P4;              // (x, y, z, w) per slot, written by the list build; valid with the list.
+                              // w = type, or after update_bc (p4Flags) type & 1023 | inner << 10 | far-from-wall << 11
+    int p4Flags;
     const double4* V4;        // (u, v, w, 0) per slot, packed for the list-based viscosity
     int K;                    // PPE row capacity (ELL width)
     int KS;                   // small-radius list capacity
@@ -123,6 +125,7 @@ struct Ctx {
     int* nbrCntS = nullptr;
     int KS = 0;
     double4* P4 = nullptr;
+    bool p4Flags = false;       // P4.w carries the inner / far-from-wall flags of update_bc
     double4* V4 = nullptr;      // packed velocities (viscosity with the list; allocated on first use)
     int K = 0;
     bool listValid = false;

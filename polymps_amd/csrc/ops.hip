@@ -1836,7 +1836,20 @@ static double listRadius2(const mps_params& P) {
 }
 static double listRadiusS2(const mps_params& P) { return std::max(P.re_s2, P.dist_collision_limit2); }
 
+// P4.w = type & 1023 | (bc == inner) << 10 | (riw2 >= 4 reL^2) << 11: the two
+// neighbour properties the PPE assembly tests, gathered with the position.
+__global__ void kPackP4Flags(KArgs a) {
+    const int i = slotId();
+    if (i >= a.N) return;
+    const mps_params& P = a.P;
+    const int t = a.I[ITYPE][i];
+    const int inner = a.I[IBC][i] == P.inner ? 1 : 0;
+    const int far = a.D[DRIW2][i] >= 4.0 * P.re_l2 ? 1 : 0;
+    a.P4[i].w = (double)((t & 1023) | (inner << 10) | (far << 11));
+}
+
 int launchBuildList(Ctx* ctx, double skin, const int* gate) {
+    ctx->p4Flags = false;
     KArgs a = ctx->args();
     const int g = gridFor(ctx->N, kBlock);
     const mps_params& P = ctx->P;
@@ -1959,6 +1972,7 @@ int launchListKeepStart(Ctx* ctx, double skin) {
 }
 
 int launchListKeepRefresh(Ctx* ctx, double skin) {
+    ctx->p4Flags = false;
     if (!ctx->patchList) MPS_HIP(hipMalloc(&ctx->patchList, sizeof(int) * (size_t)ctx->N));
     KArgs a = ctx->args();
     const int g = gridFor(ctx->N, kBlock);
@@ -1983,6 +1997,7 @@ int launchCompaction(Ctx* ctx) {
     hipLaunchKernelGGL(kCompact, dim3(1), dim3(1), 0, ctx->stream, a, ctx->outList, cap, ctx->slotOf);
     MPS_HIP(hipGetLastError());
     ctx->listValid = false;   // a compaction may move particles between slots
+    ctx->p4Flags = false;
     return MPS_OK;
 }
 
@@ -2012,9 +2027,12 @@ __global__ void kExtrapolateWallDummy2(KArgs a) {
 }
 
 int launchOp(Ctx* ctx, const char* op) {
+    const std::string o(op);
+    // operators that rewrite the wall distance or the boundary flags that
+    // update_bc packed into P4.w
+    if (o == "prediction" || o == "closest_point") ctx->p4Flags = false;
     KArgs a = ctx->args();
     const int g = gridFor(ctx->N, kBlock);
-    const std::string o(op);
     const mps_params& P = ctx->P;
     if (o == "viscosity") {
         if (ctx->listValid) {
@@ -2056,7 +2074,16 @@ int launchOp(Ctx* ctx, const char* op) {
         LT(kMeanPnd);
         L1(kMeanPndFinish);
     } else if (o == "mean_neigh_fluid_pnd") LT(kMeanNeighFluidPnd);
-    else if (o == "update_bc") LT(kUpdateBC);
+    else if (o == "update_bc") {
+        LT(kUpdateBC);
+        // the assembly reads the neighbours' inner / far-from-wall flags with their positions
+        const bool typesFit = std::max({P.fluid, P.wall, P.dummy_wall, P.ghost}) < 512 &&
+                              std::min({P.fluid, P.wall, P.dummy_wall, P.ghost}) >= -512;
+        if (ctx->listValid && typesFit) {
+            hipLaunchKernelGGL(kPackP4Flags, dim3(g), dim3(kBlock), 0, ctx->stream, a);
+            ctx->p4Flags = true;
+        }
+    }
     else if (o == "normal_particles") LT(kNormalParticles);
     else if (o == "wall_normal_particles") LT(kWallNormalParticles);
     else if (o == "press_explicit") L1(kPressExplicit);

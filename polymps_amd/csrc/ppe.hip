@@ -53,7 +53,10 @@ __global__ void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc
     int c = 0, prev = -1, ne = 0;
     bool full = false;
     const bool periodic = periodicDomain(P);
-    forNbrsPosType<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj, int tj) {
+    // with the list, the inner and far-from-wall tests of j come with its
+    // position (P4.w flags of update_bc) instead of two more gathers
+    const bool flags = LIST && a.p4Flags;
+    forNbrsPosTypeFlags<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj, int tj, int fj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         // Entry (j, i) of A is formed in row j of the reference, where i is
@@ -89,11 +92,12 @@ __global__ void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc
         sum -= subSum;
         rhs -= subRhs;
         bool put = false;
+        const bool innerJ = flags ? (fj & 1) != 0 : BC[j] == P.inner;
         if (!TRANS) {
-            put = okRow && !dummy && BC[j] == P.inner;
-        } else if (BC[j] == P.inner) {
+            put = okRow && !dummy && innerJ;
+        } else if (innerJ) {
             // entry (j, i) of A: i seen from row j, mirror test with j's mirror
-            put = pw || RIW2[j] >= farWall2;
+            put = pw || (flags ? (fj & 2) != 0 : RIW2[j] >= farWall2);
             if (!put) {
                 const double u0 = xi - a.D[DMX][j], u1 = yi - a.D[DMY][j], u2 = zi - a.D[DMZ][j];
                 const double dmj = u0 * u0 + u1 * u1 + u2 * u2;
