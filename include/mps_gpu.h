@@ -78,7 +78,12 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
 /* Scalars: "vel_max" "cfl" "time" "solver_iter" "solver_err" "num_particles"
  * "iterations" "ppe_ms" "true_residual" "fallback" "outer_iterations"
  * "list_max" "list_capacity" "capacity" "nnz" "spmv_ms_total"
- * "spmv_launches" "spmv_bytes_total".  Settable: "vel_max" "time"
+ * "spmv_launches" "spmv_bytes_total" "list_skin" (skin of the last step's
+ * kept neighbour list, in l0; 0: built after the prediction) "list_rebuilt"
+ * (1 when the last step rebuilt its list after the prediction)
+ * "list_patched" (particles whose list the last step rebuilt alone).
+ * Settable: "vel_max" "time" "list_skin" (< 0 automatic, 0 off, > 0 a fixed
+ * skin in l0; every choice gives the same results)
  * "profile" (nonzero: time every SpMV launch with HIP events, counters reset)
  * "precision_mode" (MPS_PREC_*, used from the next PPE assembly)
  * "learned_mode" (0 off, 1 learned warm start, 2 warm start + learned
