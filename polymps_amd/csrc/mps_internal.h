@@ -185,6 +185,8 @@ struct Ctx {
     Gnn* gnn = nullptr;          // learned PPE operator (created on first use)
     int redCap = 0;              // partial-sum slots in red (pairs)
     int ppeSolves = 0;           // pressure solves so far (history of the extrapolated warm start)
+    int lastCgIters = 0;         // iterations of the last FP64 CG solve / first mixed inner pass
+    int lastInnerIters = 0;
     double solverErr = 0.0;
     double trueResidual = 0.0;
     double ppeMs = 0.0;

@@ -655,6 +655,12 @@ static void addSpmvStats(Ctx* ctx, const std::vector<float>& t, int real, double
     ctx->spmvLaunches += real;
 }
 
+// Iterations queued before the first convergence read: the last solve's
+// count plus one (a typical solve then needs one host read), at least the
+// batch; kernels queued past convergence exit at once on the device flag.
+constexpr int kMaxBatch = 64;
+static int firstBatch(int last, int batch) { return std::min(std::max(last + 1, batch), kMaxBatch); }
+
 static int readStatus(Ctx* ctx, std::vector<float>* times, int batch) {
     hipStream_t s = ctx->stream;
     MPS_HIP(hipMemcpyAsync(ctx->hostInts, ctx->is, sizeof(int) * NIS, hipMemcpyDeviceToHost, s));
@@ -901,20 +907,21 @@ static int solveFp64(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int batc
     hipLaunchKernelGGL(kCgInitPFinish, dim3(1), dim3(1024), 0, s, a, nb);
     std::vector<float> times;
     int done = 0, iters = 0;
-    for (int launched = 0; !done; launched += batch) {
-        for (int q = 0; q < batch; q++) {
+    for (int launched = 0, b = firstBatch(ctx->lastCgIters, batch); !done; launched += b, b = batch) {
+        for (int q = 0; q < b; q++) {
             timedSpmv(ctx, kCgSpmv, a, c, nb, q);
             hipLaunchKernelGGL(kCgAlpha, dim3(1), dim3(1024), 0, s, a, nb);
             hipLaunchKernelGGL(kCgUpdate, dim3(nb), dim3(kB), 0, s, a, c);
             hipLaunchKernelGGL(kCgBeta, dim3(1), dim3(1024), 0, s, a, nb, maxIt);
             hipLaunchKernelGGL(kCgP, dim3(nb), dim3(kB), 0, s, a, c);
         }
-        int rc = readStatus(ctx, &times, batch);
+        int rc = readStatus(ctx, &times, b);
         if (rc) return rc;
         done = ctx->hostInts[I_CG_DONE];
         iters = ctx->hostInts[I_CG_ITER];
         if (launched > 4 * maxIt + 64) break;   // the device flag ends every batch; this only bounds the host loop
     }
+    ctx->lastCgIters = iters;
     const int D = ctx->hostInts[I_CG_DONE];
     addSpmvStats(ctx, times, (D == 1 || D == 3) ? iters + 1 : 0,
                  10.0 * ctx->hostInts[I_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 28.0 * ctx->nLiveHost);
@@ -965,8 +972,9 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
         hipLaunchKernelGGL(kMxInitFinish, dim3(1), dim3(1024), 0, s, a, nb, tolIn, tol);
         int done = 0, it = 0;
         times.clear();
-        for (int launched = 0; !done; launched += batch) {
-            for (int q = 0; q < batch; q++) {
+        for (int launched = 0, bq = firstBatch(outer == 1 ? ctx->lastInnerIters : 0, batch); !done;
+             launched += bq, bq = batch) {
+            for (int q = 0; q < bq; q++) {
                 timedSpmv(ctx, kMxSpmv, a, c, nb, q);
                 hipLaunchKernelGGL(kCgAlpha, dim3(1), dim3(1024), 0, s, a, nb);
                 if (useM) {
@@ -980,11 +988,12 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
                 }
                 hipLaunchKernelGGL(kMxP, dim3(nb), dim3(kB), 0, s, a, c);
             }
-            if ((rc = readStatus(ctx, &times, batch))) return rc;
+            if ((rc = readStatus(ctx, &times, bq))) return rc;
             done = ctx->hostInts[I_CG_DONE];
             it = ctx->hostInts[I_CG_ITER];
             if (launched > 4 * maxIt + 64) break;
         }
+        if (outer == 1) ctx->lastInnerIters = it;
         const int D = ctx->hostInts[I_CG_DONE];
         addSpmvStats(ctx, times, (D == 1 || D == 3 || D == 4) ? it + 1 : 0, bytes32);
         total += it + 1;
@@ -1041,8 +1050,8 @@ int ppeSolve(Ctx* ctx) {
     const int nb = gridFor(N, kB);
     const int batch = 8;
     hipStream_t s = ctx->stream;
-    if (ctx->profile && ctx->evPool.size() < 2 * batch) {
-        ctx->evPool.resize(2 * batch);
+    if (ctx->profile && ctx->evPool.size() < 2 * kMaxBatch) {
+        ctx->evPool.resize(2 * kMaxBatch);
         for (auto& e : ctx->evPool) MPS_HIP(hipEventCreate(&e));
     }
     MPS_HIP(hipEventRecord(ctx->ev0, s));
