@@ -103,6 +103,26 @@ int mps_mesh_read_stl(const char* path, int* nv, double* V, int* nf, int* F, dou
 /* Install the rigid wall mesh (V[3nv], F[3nf], unit normals N[3nf]) and
  * build the wall tables of PolygonMesh::initWijnNeigh (src/PolygonMesh.cpp:377-527). */
 int mps_set_mesh(mps_ctx* ctx, int nv, const double* V, int nf, const int* F, const double* N);
+/* Mesh by type, as main.cpp's mesh[meshType] (src/main.cpp:471-515):
+ * MPS_MESH_FIXED (= mps_set_mesh), MPS_MESH_DEFORMABLE (fem_MESH; static here,
+ * the reference's FEM coupling is not part of it), MPS_MESH_FORCED (forced_MESH,
+ * moved by mps_forced_mesh_update).  The closest-point search visits the
+ * installed meshes in type order and keeps a strictly closer point
+ * (src/PolygonMesh.cpp:615-650), recording the type in nearMeshType. */
+#define MPS_MESH_FIXED 0
+#define MPS_MESH_DEFORMABLE 1
+#define MPS_MESH_FORCED 2
+int mps_set_mesh_type(mps_ctx* ctx, int type, int nv, const double* V, int nf, const int* F, const double* N);
+/* Current geometry of an installed mesh (the forced one moves); arrays may be
+ * NULL to query the counts. */
+int mps_get_mesh(mps_ctx* ctx, int type, int* nv, double* V, int* nf, int* F, double* N);
+/* PolygonMesh::updateForcedPolygonMesh (src/PolygonMesh.cpp:173-374): move the
+ * forced mesh's nodes by one step of the forced motion at time `time` (the
+ * parameters forced_*), recompute its face normals, and set uniform_vel_wall. */
+int mps_forced_mesh_update(mps_ctx* ctx, double dt, double time);
+/* PolygonMesh::writePolygonMeshFile (src/PolygonMesh.cpp:110-139): ASCII STL
+ * of a mesh (facet normals from the faces). */
+int mps_mesh_write_stl(const char* path, int nv, const double* V, int nf, const int* F);
 /* Wall tables xPND, xNeigh, Zij, nNeighWall (50 entries each). */
 int mps_get_wall_tables(mps_ctx* ctx, double* xpnd, double* xneigh, double* zij, int* nneigh);
 
@@ -124,8 +144,11 @@ int mps_build_neighbor_list(mps_ctx* ctx);
 /* MpsParticleCollision::checkParticleCollisions / checkDynamicParticleCollisions
  * (src/MpsParticleCollision.cpp:22-119, 124-292) */
 int mps_collision(mps_ctx* ctx);
-/* PolygonMesh::closestPointPNDBoundaryAABB (src/PolygonMesh.cpp:585-715) */
+/* PolygonMesh::closestPointPNDBoundaryAABB (src/PolygonMesh.cpp:585-715), for
+ * every installed mesh in type order (main.cpp's loop, src/main.cpp:296-300) */
 int mps_mesh_closest_point(mps_ctx* ctx);
+/* The same for one mesh type (one call of that loop). */
+int mps_mesh_closest_point_type(mps_ctx* ctx, int type);
 /* PolygonMesh::updateParticlesNearPolygonMesh (src/PolygonMesh.cpp:718-795) */
 int mps_mesh_update_near(mps_ctx* ctx);
 /* MpsPndNeigh::calcWallNPCD (src/MpsPndNeigh.cpp:105-195) */
@@ -216,6 +239,9 @@ int mps_write_grid(const char* path, int n, const int* type, const double* pos, 
 /* The output part of MpsInputOutput::readInputFile (src/MpsInputOutput.cpp:140-151,
  * 173-179): pathNames.vtu_output_folder and the MPS_VTU_* flags. */
 int mps_output_from_json(const char* json_path, char* vtu_folder, size_t folder_cap, int* vtu_flags);
+/* pathNames.mesh_deformable_file and mesh_forced_file (src/MpsInputOutput.cpp:177-178). */
+int mps_mesh_paths_from_json(const char* json_path, char* deformable_path, size_t deformable_cap, char* forced_path,
+                             size_t forced_cap);
 /* MpsInputOutput::writeVtuAscii (src/MpsInputOutput.cpp:1313-1579) for the
  * context's current state; flags: MPS_VTU_PND .. MPS_VTU_AUXILIAR;
  * MPS_VTU_ONLY_FREE_SURFACE for writeVtuAsciiFreeSurface (:1582-1875: the

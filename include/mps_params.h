@@ -36,8 +36,8 @@ enum { MPS_SHIFT_JACOBI = 0, MPS_SHIFT_SEQUENTIAL = 1 };
 typedef struct mps_params {
     /* ---- JSON "flags" ---- */
     int wall_type;              /* wall_type                          */
-    int fem_on;                 /* fem_MESH (only false supported)    */
-    int forced_on;              /* forced_MESH (only false supported) */
+    int fem_on;                 /* fem_MESH: a second, static mesh (meshType 1) */
+    int forced_on;              /* forced_MESH: a mesh moved by the forced motion (meshType 2) */
     /* ---- JSON "domain" ---- */
     double domain_min[3];       /* min.x/y/z (extended by 3 l0 in derive) */
     double domain_max[3];
@@ -114,6 +114,17 @@ typedef struct mps_params {
     int ppe_warm_start;         /* initial guess of each pressure solve: 0 the previous solution
                                    (the reference), 1 / 2 linear / quadratic extrapolation of the
                                    last 2 / 3 solutions */
+    /* forced mesh motion (the reference hard-codes it in updateForcedPolygonMesh,
+       src/PolygonMesh.cpp:173-374; the defaults are its active law): 1 rotation
+       about the y axis through (center_x, center_z), angle theta_max sin(2 pi t / period)
+       degrees; 0 translation along z at forced_velocity */
+    int forced_motion;
+    int pad2;
+    double forced_center[2];    /* Xcg, Zcg                           */
+    double forced_theta_max;    /* degrees                            */
+    double forced_period;       /* s                                  */
+    double forced_velocity;     /* m/s (translation)                  */
+    double uniform_vel_wall[3]; /* uniformVelWall: velocity of the forced mesh's wall points */
 
     /* ---- derived by mps_params_derive (setParameters + bucket grid) ---- */
     double eps_zero;            /* 10 * DBL_EPSILON                   */

@@ -76,6 +76,8 @@ struct Sys {
     std::vector<int> bucketPBC;   // bucketPeriodicBC: -1 / +1 on the ghost layers, 0 elsewhere
     Mesh mesh;
     bool hasMesh = false;
+    Mesh extra[2];            // meshType 1 (deformable, static) and 2 (forced)
+    bool extraOn[2] = {false, false};
     double velMax = 0.0, cflCurrent = 0.0, timeCurrent = 0.0;
     int numIter = 0;
     int nThreads = 0;
@@ -452,7 +454,8 @@ void calcWallViscosity(Sys& s) {
             for (int k = 0; k < 9; k++) Rm[k] = R[k];
         } else {
             double Ri[9] = {-1.0, 0.0, 0.0, 0.0, -1.0, 0.0, 0.0, 0.0, -1.0};
-            double vw[3] = {0.0, 0.0, 0.0};
+            double vw[3] = {0.0, 0.0, 0.0};   // wall velocity: uniformVelWall near the forced mesh
+            if (s.nearMeshType[i] == 2) for (int k = 0; k < 3; k++) vw[k] = P.uniform_vel_wall[k];
             double dotnv = n[0] * vw[0] + n[1] * vw[1] + n[2] * vw[2];
             double vt[3];
             vt[0] = ux - 2.0 * (vw[0] - dotnv * n[0]);
@@ -883,10 +886,10 @@ inline void closestPtTriangle(const double* p, const double* a, const double* b,
     q[2] = a[2] + ab[2] * v + ac[2] * w;
 }
 
-// PolygonMesh::closestPointPNDBoundaryAABB (src/PolygonMesh.cpp:585-715), one
-// static mesh, brute force over faces.
-void closestPoint(Sys& s) {
-    const Mesh& m = s.mesh;
+// PolygonMesh::closestPointPNDBoundaryAABB (src/PolygonMesh.cpp:585-715) for
+// one mesh, brute force over faces: a strictly closer point replaces the
+// particle's wall point, mirror, normal and mesh type (:615-650).
+void closestPointMesh(Sys& s, const Mesh& m, int meshType) {
 #pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < s.N; i++) {
         const double* p = &s.pos[i * 3];
@@ -907,9 +910,73 @@ void closestPoint(Sys& s) {
                 s.mirror[i * 3 + k] = p[k] + 2.0 * (bq[k] - p[k]);
                 s.polyN[i * 3 + k] = m.Nrm[bf * 3 + k];
             }
-            s.nearMeshType[i] = 0;
+            s.nearMeshType[i] = meshType;
         }
     }
+}
+
+// main.cpp's loop over the meshes in type order (src/main.cpp:296-300)
+void closestPoint(Sys& s) {
+    closestPointMesh(s, s.mesh, 0);
+    for (int t = 1; t <= 2; t++)
+        if (s.extraOn[t - 1]) closestPointMesh(s, s.extra[t - 1], t);
+}
+
+// unit face normals (v1 - v0) x (v2 - v0), zero for a degenerate face
+// (igl::per_face_normals as updateForcedPolygonMesh calls it, :362-369)
+void faceNormals(Mesh& m) {
+    m.Nrm.assign(3 * (size_t)m.nf, 0.0);
+    for (int t = 0; t < m.nf; t++) {
+        const double* a = &m.V[(size_t)m.F[t * 3] * 3];
+        const double* b = &m.V[(size_t)m.F[t * 3 + 1] * 3];
+        const double* c = &m.V[(size_t)m.F[t * 3 + 2] * 3];
+        const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const double r = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (int k = 0; k < 3; k++) m.Nrm[(size_t)t * 3 + k] = (r == 0) ? 0.0 : n[k] / r;
+    }
+}
+
+// PolygonMesh::updateForcedPolygonMesh (src/PolygonMesh.cpp:173-374): the
+// reference's active law is the rotation about the y axis through
+// (Xcg, Zcg) with steady amplitude (its ramp is off, :289-313); each node
+// moves by omega x r dt with r taken before the move (:332-356).  The
+// translation branch sets uniformVelWall = (0, 0, vel) (:322-330).
+int forcedMeshUpdate(Sys& s, double dt, double time) {
+    if (!s.extraOn[1]) { s.err = "no forced mesh"; return -1; }
+    mps_params& P = s.p;
+    Mesh& m = s.extra[1];
+    const bool rotation = P.forced_motion == 1;
+    double omegaAmp = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
+    if (rotation) {
+        const double kPi = 3.14159265358979323846;   // M_PI
+        const double Tmotion = P.forced_period;
+        const double R_ksi = P.forced_theta_max, R_ksi_dot = 0.0;
+        omegaAmp = R_ksi_dot * sin(2.0 * kPi * time / Tmotion) + R_ksi * 2.0 * kPi / Tmotion * cos(2.0 * kPi * time / Tmotion);
+        omegaAmp = omegaAmp * kPi / 180.0;
+    } else {
+        const double vel = P.forced_velocity;
+        P.uniform_vel_wall[0] = 0.0;
+        P.uniform_vel_wall[1] = 0.0;
+        P.uniform_vel_wall[2] = vel;
+        dz = vel * dt;
+    }
+    const double Xcg = P.forced_center[0], Zcg = P.forced_center[1];
+    for (int q = 0; q < m.nv; q++) {
+        double* v = &m.V[(size_t)q * 3];
+        if (rotation) {
+            const double xi = v[0] - Xcg, zi = v[2] - Zcg;
+            const double vx = omegaAmp * zi, vz = -omegaAmp * xi;
+            dx = vx * dt;
+            dz = vz * dt;
+        }
+        v[0] += dx;
+        v[1] += dy;
+        v[2] += dz;
+    }
+    faceNormals(m);
+    return 0;
 }
 
 // PolygonMesh::updateParticlesNearPolygonMesh (src/PolygonMesh.cpp:718-795)
@@ -1120,7 +1187,8 @@ void wallPndDiffusive(Sys& s) {
             umy = R[3] * ux + R[4] * uy + R[5] * uz;
             umz = R[6] * ux + R[7] * uy + R[8] * uz;
         } else {
-            double vw[3] = {0.0, 0.0, 0.0};
+            double vw[3] = {0.0, 0.0, 0.0};   // wall velocity: uniformVelWall near the forced mesh
+            if (s.nearMeshType[i] == 2) for (int k = 0; k < 3; k++) vw[k] = P.uniform_vel_wall[k];
             double dotnv = n[0] * vw[0] + n[1] * vw[1] + n[2] * vw[2];
             double vt0 = ux - 2.0 * (vw[0] - dotnv * n[0]);
             double vt1 = uy - 2.0 * (vw[1] - dotnv * n[1]);
@@ -1591,7 +1659,8 @@ void wallVelDivergence(Sys& s) {
             vmz = (R[6] * velXi + R[7] * velYi + R[8] * velZi);
         } else {
             const double Rinv[9] = {-1.0, 0.0, 0.0, 0.0, -1.0, 0.0, 0.0, 0.0, -1.0};
-            const double vw[3] = {0.0, 0.0, 0.0};   // fixed wall (forced meshes are not supported)
+            double vw[3] = {0.0, 0.0, 0.0};   // wall velocity: uniformVelWall near the forced mesh
+            if (s.nearMeshType[i] == 2) for (int k = 0; k < 3; k++) vw[k] = P.uniform_vel_wall[k];
             const double dotnv = n[0] * vw[0] + n[1] * vw[1] + n[2] * vw[2];
             const double vt[3] = {velXi - 2.0 * (vw[0] - dotnv * n[0]), velYi - 2.0 * (vw[1] - dotnv * n[1]),
                                   velZi - 2.0 * (vw[2] - dotnv * n[2])};
@@ -2279,6 +2348,10 @@ int initSystem(Sys& s) {
     checkOutDomain(s);
     if (P.wall_type == MPS_WALL_POLYGON) {
         if (!s.hasMesh) { s.err = "polygon walls need a mesh"; return -1; }
+        if ((P.fem_on && !s.extraOn[0]) || (P.forced_on && !s.extraOn[1])) {
+            s.err = "fem_MESH / forced_MESH need their meshes";
+            return -1;
+        }
         initWijnNeigh(s);
         wallContributions(s);
     }
@@ -2332,6 +2405,8 @@ int stepSystem(Sys& s) {
     pressGradient(s);
     if (P.wall_type == MPS_WALL_POLYGON) wallPressGradient(s);
     if (P.wall_type == MPS_WALL_POLYGON) calcWallViscosity(s);
+    // src/main.cpp:410-414
+    if (P.forced_on && forcedMeshUpdate(s, P.time_step, s.timeCurrent)) return -1;
     correction(s);
     checkOutDomain(s);
     if (P.shifting_type == 1) shifting1(s);
@@ -2440,6 +2515,33 @@ int orc_set_mesh(void* h, int nv, const double* V, int nf, const int* F, const d
     initWijnNeigh(*s);
     return 0;
 }
+
+int orc_set_mesh_type(void* h, int type, int nv, const double* V, int nf, const int* F, const double* Nrm) {
+    if (type == 0) return orc_set_mesh(h, nv, V, nf, F, Nrm);
+    if (type != 1 && type != 2) return -1;
+    Sys* s = (Sys*)h;
+    Mesh& m = s->extra[type - 1];
+    m.nv = nv; m.nf = nf;
+    m.V.assign(V, V + 3 * nv);
+    m.F.assign(F, F + 3 * nf);
+    m.Nrm.assign(Nrm, Nrm + 3 * nf);
+    s->extraOn[type - 1] = true;
+    return 0;
+}
+
+int orc_get_mesh(void* h, int type, int* nv, double* V, int* nf, int* F, double* Nrm) {
+    Sys* s = (Sys*)h;
+    const Mesh* m = type == 0 ? &s->mesh : ((type == 1 || type == 2) && s->extraOn[type - 1] ? &s->extra[type - 1] : nullptr);
+    if (!m) return -1;
+    if (nv) *nv = m->nv;
+    if (nf) *nf = m->nf;
+    if (V) std::copy(m->V.begin(), m->V.end(), V);
+    if (F) std::copy(m->F.begin(), m->F.end(), F);
+    if (Nrm) std::copy(m->Nrm.begin(), m->Nrm.end(), Nrm);
+    return 0;
+}
+
+int orc_forced_mesh_update(void* h, double dt, double time) { return forcedMeshUpdate(*(Sys*)h, dt, time); }
 
 int orc_get_wall_tables(void* h, double* xpnd, double* xneigh, double* zij, int* nneigh) {
     Sys* s = (Sys*)h;

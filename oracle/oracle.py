@@ -36,6 +36,9 @@ def lib():
         L.orc_last_error.restype = cp
         L.orc_set_threads.argtypes = [vp, ip]
         L.orc_set_mesh.argtypes = [vp, ip, vp, ip, vp, vp]
+        L.orc_set_mesh_type.argtypes = [vp, ip, ip, vp, ip, vp, vp]
+        L.orc_get_mesh.argtypes = [vp, ip, ctypes.POINTER(ip), vp, ctypes.POINTER(ip), vp, vp]
+        L.orc_forced_mesh_update.argtypes = [vp, ctypes.c_double, ctypes.c_double]
         L.orc_get_wall_tables.argtypes = [vp, vp, vp, vp, vp]
         L.orc_field.argtypes = [vp, cp, vp, ip]
         L.orc_scalar.argtypes = [vp, cp, ctypes.POINTER(ctypes.c_double)]
@@ -54,7 +57,8 @@ def _p(a):
 
 
 class Oracle:
-    def __init__(self, params: MpsParams, types, pos, vel=None, press=None, press_avg=None, mesh=None, threads=0):
+    def __init__(self, params: MpsParams, types, pos, vel=None, press=None, press_avg=None, mesh=None, threads=0,
+                 meshes=None):
         L = lib()
         types = np.ascontiguousarray(types, dtype=np.int32)
         n = len(types)
@@ -75,6 +79,25 @@ class Oracle:
             F = np.ascontiguousarray(mesh[1], dtype=np.int32)
             N = np.ascontiguousarray(mesh[2], dtype=np.float64)
             L.orc_set_mesh(self.h, len(V) // 3, _p(V), len(F) // 3, _p(F), _p(N))
+        for t, m in (meshes or {}).items():
+            V = np.ascontiguousarray(m[0], dtype=np.float64)
+            F = np.ascontiguousarray(m[1], dtype=np.int32)
+            N = np.ascontiguousarray(m[2], dtype=np.float64)
+            if L.orc_set_mesh_type(self.h, t, len(V) // 3, _p(V), len(F) // 3, _p(F), _p(N)):
+                raise RuntimeError("oracle mesh type")
+
+    def get_mesh(self, mesh_type: int):
+        L = lib()
+        nv, nf = ctypes.c_int(), ctypes.c_int()
+        if L.orc_get_mesh(self.h, mesh_type, ctypes.byref(nv), None, ctypes.byref(nf), None, None):
+            raise RuntimeError("oracle: mesh type not installed")
+        V, F, N = np.zeros(3 * nv.value), np.zeros(3 * nf.value, dtype=np.int32), np.zeros(3 * nf.value)
+        L.orc_get_mesh(self.h, mesh_type, ctypes.byref(nv), _p(V), ctypes.byref(nf), _p(F), _p(N))
+        return V, F, N
+
+    def forced_mesh_update(self, dt: float, time: float):
+        if lib().orc_forced_mesh_update(self.h, dt, time):
+            raise RuntimeError(f"oracle forced_mesh_update: {lib().orc_last_error(self.h).decode()}")
 
     def close(self):
         if self.h:

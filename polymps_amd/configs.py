@@ -46,6 +46,7 @@ class Workload:
     vel: np.ndarray
     mesh: tuple | None = None
     steps: int = 1
+    extra_meshes: dict | None = None   # further meshes by type: {1: deformable (static), 2: forced}
     info: dict = field(default_factory=dict)
 
     @property
@@ -483,6 +484,31 @@ def write_case(w: Workload, folder, name: str = "case", vtu: dict | None = None)
     path = folder / f"{name}.json"
     path.write_text(json.dumps(case_json(w.params, grid, mesh, f"{name}_out", vtu), indent=1))
     return path
+
+
+def forced_paddle3d(l0: float = 0.025, dt: float = 1e-3, steps: int = 40, theta_max: float = 30.0,
+                    period: float = 0.4, rotation: int = 1, velocity: float = 0.0) -> Workload:
+    """Dam break with all three mesh types: the closed tank 0.8 x 0.2 x 0.4 m
+    (rigid), a static slab 0.20 x 0.2 x 0.03 m 1 cm above the fluid block
+    (the deformable slot, meshType 1) and a paddle plate x in [0.33, 0.35],
+    z in [0.02, 0.30] (forced, meshType 2) 3 cm from the block's face,
+    rotating about (0.34, 0) with amplitude theta_max degrees and the given
+    period (or translating along z at `velocity` with rotation=0)."""
+    w = dam3d_inc(l0=l0, tank=(0.8, 0.2, 0.4), block=(0.3, 0.2, 0.2), steps=steps, dt=dt)
+    p = w.params
+    p.fem_on = 1
+    p.forced_on = 1
+    p.forced_motion = rotation
+    p.forced_center[0], p.forced_center[1] = 0.34, 0.0
+    p.forced_theta_max = theta_max
+    p.forced_period = period
+    p.forced_velocity = velocity
+    SV, SF = box_mesh((0.05, 0.0, 0.21), (0.25, 0.2, 0.24))
+    PV, PF = box_mesh((0.33, 0.0, 0.02), (0.35, 0.2, 0.30))
+    # reversed winding: normals into the solids, away from the fluid
+    w.extra_meshes = {1: mesh_arrays(SV, SF[:, ::-1]), 2: mesh_arrays(PV, PF[:, ::-1])}
+    w.name = f"forced_paddle3d_l0_{l0:g}"
+    return w
 
 
 def small_dam3d(l0: float = 0.025, **kw) -> Workload:

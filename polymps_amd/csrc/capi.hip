@@ -224,6 +224,14 @@ static int runInit(Ctx* ctx) {
     if ((rc = launchCompaction(ctx))) return rc;
     if (P.wall_type == MPS_WALL_POLYGON) {
         if (!ctx->hasMesh) { ctx->err = "polygon walls need a mesh (mps_set_mesh)"; return MPS_E_ARG; }
+        if (P.fem_on && !ctx->extraMesh[0].on) {
+            ctx->err = "fem_MESH needs the deformable mesh (mps_set_mesh_type)";
+            return MPS_E_ARG;
+        }
+        if (P.forced_on && !ctx->extraMesh[1].on) {
+            ctx->err = "forced_MESH needs the forced mesh (mps_set_mesh_type)";
+            return MPS_E_ARG;
+        }
         if ((rc = launchOp(ctx, "closest_point"))) return rc;
         if ((rc = launchOp(ctx, "update_near_mesh"))) return rc;
     }
@@ -256,6 +264,9 @@ static int calcPress(Ctx* ctx) {
     if ((rc = ppeWarmStart(ctx))) return rc;
     return ppeSolve(ctx);
 }
+
+static int forcedMeshUpdate(Ctx* ctx, double dt, double time);
+static void freeSlot(MeshSlot& m);
 
 // Skin (m) of a neighbour list kept over the step, 0 to build the list after
 // the prediction as usual.  Automatic: three times the largest displacement
@@ -326,6 +337,8 @@ static int runStep(Ctx* ctx) {
         OP("wall_press_gradient");
         OP("wall_viscosity");
     }
+    // src/main.cpp:410-414: the forced mesh moves before the corrector
+    if (P.forced_on && (rc = forcedMeshUpdate(ctx, P.time_step, ctx->timeCurrent))) return rc;
     OP("correction");
     if ((rc = launchCompaction(ctx))) return rc;
     OP("shifting");
@@ -451,6 +464,8 @@ int mps_destroy(mps_ctx* ctx) {
                     ctx->ds, ctx->is, ctx->red};
     for (void* b : bufs)
         if (b) hipFree(b);
+    freeSlot(ctx->extraMesh[0]);
+    freeSlot(ctx->extraMesh[1]);
     if (ctx->hostScalars) hipHostFree(ctx->hostScalars);
     if (ctx->hostInts) hipHostFree(ctx->hostInts);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
@@ -688,7 +703,8 @@ int mps_sync(mps_ctx* ctx) {
 namespace mpsgpu {
 // Median-split BVH over the triangles (longest axis of the centroid bounds,
 // at most 4 triangles per leaf), built once on the host for a static mesh.
-static int buildBvh(Ctx* ctx, const double* V, const int* F, int nf) {
+static int buildBvhInto(Ctx* ctx, const double* V, const int* F, int nf, BvhNode** outBvh, int** outTri,
+                        int* outNodes) {
     struct Box { double lo[3], hi[3]; };
     std::vector<Box> tb(nf);
     std::vector<double> cen(3 * (size_t)nf);
@@ -747,14 +763,109 @@ static int buildBvh(Ctx* ctx, const double* V, const int* F, int nf) {
         work.push_back({nd.a, it.begin, mid});
         work.push_back({nd.b, mid, it.end});
     }
-    if (ctx->bvh) (void)hipFree(ctx->bvh);
-    if (ctx->bvhTri) (void)hipFree(ctx->bvhTri);
-    MPS_HIP(hipMalloc(&ctx->bvh, sizeof(BvhNode) * nodes.size()));
-    MPS_HIP(hipMalloc(&ctx->bvhTri, sizeof(int) * (size_t)nf));
-    MPS_HIP(hipMemcpy(ctx->bvh, nodes.data(), sizeof(BvhNode) * nodes.size(), hipMemcpyHostToDevice));
-    MPS_HIP(hipMemcpy(ctx->bvhTri, idx.data(), sizeof(int) * (size_t)nf, hipMemcpyHostToDevice));
-    ctx->bvhNodes = (int)nodes.size();
+    // the tree's shape depends on nf only, so a rebuild of a moved mesh reuses the buffers
+    if (*outBvh == nullptr || *outNodes != (int)nodes.size()) {
+        if (*outBvh) (void)hipFree(*outBvh);
+        if (*outTri) (void)hipFree(*outTri);
+        *outBvh = nullptr;
+        *outTri = nullptr;
+        MPS_HIP(hipMalloc(outBvh, sizeof(BvhNode) * nodes.size()));
+        MPS_HIP(hipMalloc(outTri, sizeof(int) * (size_t)nf));
+    }
+    MPS_HIP(hipMemcpyAsync(*outBvh, nodes.data(), sizeof(BvhNode) * nodes.size(), hipMemcpyHostToDevice, ctx->stream));
+    MPS_HIP(hipMemcpyAsync(*outTri, idx.data(), sizeof(int) * (size_t)nf, hipMemcpyHostToDevice, ctx->stream));
+    MPS_HIP(hipStreamSynchronize(ctx->stream));
+    *outNodes = (int)nodes.size();
     return MPS_OK;
+}
+static int buildBvh(Ctx* ctx, const double* V, const int* F, int nf) {
+    return buildBvhInto(ctx, V, F, nf, &ctx->bvh, &ctx->bvhTri, &ctx->bvhNodes);
+}
+
+// unit face normals (v1 - v0) x (v2 - v0), zero for a degenerate face (the
+// normals of mps_mesh_read_stl and of igl::per_face_normals)
+static void faceNormals(const std::vector<double>& V, const std::vector<int>& F, std::vector<double>& N) {
+    const int nf = (int)(F.size() / 3);
+    N.resize(3 * (size_t)nf);
+    for (int t = 0; t < nf; t++) {
+        const double* a = &V[(size_t)F[t * 3] * 3];
+        const double* b = &V[(size_t)F[t * 3 + 1] * 3];
+        const double* c = &V[(size_t)F[t * 3 + 2] * 3];
+        const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const double r = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (int k = 0; k < 3; k++) N[(size_t)t * 3 + k] = (r == 0) ? 0.0 : n[k] / r;
+    }
+}
+
+static int uploadSlot(Ctx* ctx, MeshSlot& m, bool rebuildBvh) {
+    if (!m.V) {
+        MPS_HIP(hipMalloc(&m.V, sizeof(double) * 3 * (size_t)m.nv));
+        MPS_HIP(hipMalloc(&m.F, sizeof(int) * 3 * (size_t)m.nf));
+        MPS_HIP(hipMalloc(&m.N, sizeof(double) * 3 * (size_t)m.nf));
+        MPS_HIP(hipMemcpyAsync(m.F, m.hF.data(), sizeof(int) * 3 * (size_t)m.nf, hipMemcpyHostToDevice, ctx->stream));
+    }
+    MPS_HIP(hipMemcpyAsync(m.V, m.hV.data(), sizeof(double) * 3 * (size_t)m.nv, hipMemcpyHostToDevice, ctx->stream));
+    MPS_HIP(hipMemcpyAsync(m.N, m.hN.data(), sizeof(double) * 3 * (size_t)m.nf, hipMemcpyHostToDevice, ctx->stream));
+    // the BVH when the search will use it (launchClosestPoint's rule)
+    if (rebuildBvh && (ctx->bvhMode == 1 || (ctx->bvhMode < 0 && m.nf > 64)))
+        return buildBvhInto(ctx, m.hV.data(), m.hF.data(), m.nf, &m.bvh, &m.bvhTri, &m.bvhNodes);
+    MPS_HIP(hipStreamSynchronize(ctx->stream));   // the host copies may change after the call
+    return MPS_OK;
+}
+
+static void freeSlot(MeshSlot& m) {
+    void* bufs[] = {m.V, m.F, m.N, m.bvh, m.bvhTri};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    m = MeshSlot();
+}
+
+// PolygonMesh::updateForcedPolygonMesh (src/PolygonMesh.cpp:173-374) with the
+// law the reference has switched on (rotation about the y axis through
+// (Xcg, Zcg), steady amplitude, its ramp off) or its translation branch.
+// Every node moves by its velocity at the start of the step times dt (the
+// reference's explicit update); the angle itself is not integrated.
+static int forcedMeshUpdate(Ctx* ctx, double dt, double time) {
+    MeshSlot& m = ctx->extraMesh[1];
+    if (!m.on) {
+        ctx->err = "no forced mesh installed";
+        return MPS_E_ARG;
+    }
+    mps_params& P = ctx->P;
+    const bool rotation = P.forced_motion == 1;
+    double omegaAmp = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
+    if (rotation) {
+        const double kPi = 3.14159265358979323846;
+        const double Tmotion = P.forced_period;
+        const double R_ksi = P.forced_theta_max, R_ksi_dot = 0.0;
+        omegaAmp = R_ksi_dot * sin(2.0 * kPi * time / Tmotion) + R_ksi * 2.0 * kPi / Tmotion * cos(2.0 * kPi * time / Tmotion);
+        omegaAmp = omegaAmp * kPi / 180.0;   // degrees/s -> rad/s
+    } else {
+        const double vel = P.forced_velocity;
+        P.uniform_vel_wall[0] = 0.0;
+        P.uniform_vel_wall[1] = 0.0;
+        P.uniform_vel_wall[2] = vel;
+        dz = vel * dt;
+    }
+    const double Xcg = P.forced_center[0], Zcg = P.forced_center[1];
+    for (int q = 0; q < m.nv; q++) {
+        double* v = &m.hV[(size_t)q * 3];
+        if (rotation) {
+            const double xi = v[0] - Xcg;
+            const double zi = v[2] - Zcg;
+            const double vx = omegaAmp * zi;
+            const double vz = -omegaAmp * xi;
+            dx = vx * dt;
+            dz = vz * dt;
+        }
+        v[0] += dx;
+        v[1] += dy;
+        v[2] += dz;
+    }
+    faceNormals(m.hV, m.hF, m.hN);
+    return uploadSlot(ctx, m, ctx->bvhMode != 0);
 }
 }  // namespace mpsgpu
 
@@ -778,7 +889,67 @@ int mps_set_mesh(mps_ctx* ctx, int nv, const double* V, int nf, const int* F, co
     ctx->nv = nv;
     ctx->nf = nf;
     ctx->hasMesh = true;
+    ctx->rigidV.assign(V, V + 3 * (size_t)nv);
+    ctx->rigidF.assign(F, F + 3 * (size_t)nf);
+    ctx->rigidN.assign(N, N + 3 * (size_t)nf);
     return buildBvh(ctx, V, F, nf);
+}
+
+int mps_set_mesh_type(mps_ctx* ctx, int type, int nv, const double* V, int nf, const int* F, const double* N) {
+    CHECK_CTX(ctx);
+    if (type == MPS_MESH_FIXED) return mps_set_mesh(ctx, nv, V, nf, F, N);
+    if (type != MPS_MESH_DEFORMABLE && type != MPS_MESH_FORCED) { ctx->err = "mesh type must be 0, 1 or 2"; return MPS_E_ARG; }
+    if (nv <= 0 || nf <= 0 || !V || !F || !N) return MPS_E_ARG;
+    for (int q = 0; q < 3 * nf; q++)
+        if (F[q] < 0 || F[q] >= nv) { ctx->err = "face index out of range"; return MPS_E_ARG; }
+    MPS_HIP(hipSetDevice(ctx->dev));
+    MeshSlot& m = ctx->extraMesh[type - 1];
+    freeSlot(m);
+    m.nv = nv;
+    m.nf = nf;
+    m.hV.assign(V, V + 3 * (size_t)nv);
+    m.hF.assign(F, F + 3 * (size_t)nf);
+    m.hN.assign(N, N + 3 * (size_t)nf);
+    m.on = true;
+    return uploadSlot(ctx, m, ctx->bvhMode != 0);
+}
+
+int mps_get_mesh(mps_ctx* ctx, int type, int* nv, double* V, int* nf, int* F, double* N) {
+    CHECK_CTX(ctx);
+    const std::vector<double>* hv;
+    const std::vector<int>* hf;
+    const std::vector<double>* hn;
+    if (type == MPS_MESH_FIXED) {
+        if (!ctx->hasMesh) { ctx->err = "no rigid mesh installed"; return MPS_E_ARG; }
+        hv = &ctx->rigidV; hf = &ctx->rigidF; hn = &ctx->rigidN;
+    } else if (type == MPS_MESH_DEFORMABLE || type == MPS_MESH_FORCED) {
+        const MeshSlot& m = ctx->extraMesh[type - 1];
+        if (!m.on) { ctx->err = "mesh type not installed"; return MPS_E_ARG; }
+        hv = &m.hV; hf = &m.hF; hn = &m.hN;
+    } else {
+        ctx->err = "mesh type must be 0, 1 or 2";
+        return MPS_E_ARG;
+    }
+    if (nv) *nv = (int)(hv->size() / 3);
+    if (nf) *nf = (int)(hf->size() / 3);
+    if (V) memcpy(V, hv->data(), sizeof(double) * hv->size());
+    if (F) memcpy(F, hf->data(), sizeof(int) * hf->size());
+    if (N) memcpy(N, hn->data(), sizeof(double) * hn->size());
+    return MPS_OK;
+}
+
+int mps_forced_mesh_update(mps_ctx* ctx, double dt, double time) {
+    CHECK_CTX(ctx);
+    MPS_HIP(hipSetDevice(ctx->dev));
+    return forcedMeshUpdate(ctx, dt, time);
+}
+
+int mps_mesh_closest_point_type(mps_ctx* ctx, int type) {
+    CHECK_CTX(ctx);
+    MPS_HIP(hipSetDevice(ctx->dev));
+    int rc = launchClosestPoint(ctx, type);
+    if (rc) return rc;
+    return syncStatus(ctx);
 }
 
 int mps_get_wall_tables(mps_ctx* ctx, double* xpnd, double* xneigh, double* zij, int* nneigh) {

@@ -64,6 +64,36 @@ bool readStlAscii(FILE* f, std::vector<Tri>& tris) {
 
 extern "C" {
 
+// PolygonMesh::writePolygonMeshFile (src/PolygonMesh.cpp:110-139) writes the
+// mesh with igl::writeSTL; here an ASCII STL with each facet's normal from
+// its vertices.
+int mps_mesh_write_stl(const char* path, int nv, const double* V, int nf, const int* F) {
+    if (!path || nv <= 0 || nf <= 0 || !V || !F) return MPS_E_ARG;
+    FILE* fp = fopen(path, "w");
+    if (!fp) {
+        mpsgpu::setGlobalError(std::string("cannot write ") + path);
+        return MPS_E_IO;
+    }
+    fprintf(fp, "solid mesh\n");
+    for (int t = 0; t < nf; t++) {
+        const double* a = &V[(size_t)F[t * 3] * 3];
+        const double* b = &V[(size_t)F[t * 3 + 1] * 3];
+        const double* c = &V[(size_t)F[t * 3 + 2] * 3];
+        const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const double r = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (int k = 0; k < 3; k++) n[k] = (r == 0) ? 0.0 : n[k] / r;
+        fprintf(fp, "facet normal %e %e %e\nouter loop\n", n[0], n[1], n[2]);
+        for (const double* v : {a, b, c}) fprintf(fp, "vertex %e %e %e\n", v[0], v[1], v[2]);
+        fprintf(fp, "endloop\nendfacet\n");
+    }
+    fprintf(fp, "endsolid mesh\n");
+    const bool ok = fclose(fp) == 0;
+    (void)nv;
+    return ok ? MPS_OK : MPS_E_IO;
+}
+
 int mps_mesh_read_stl(const char* path, int* nv_out, double* V, int* nf_out, int* F, double* N) {
     if (!path || !nv_out || !nf_out) return MPS_E_ARG;
     FILE* f = fopen(path, "rb");

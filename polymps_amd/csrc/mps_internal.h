@@ -92,6 +92,21 @@ struct KArgs {
 struct Gnn;
 struct VtuQueue;
 
+// A wall mesh besides the rigid one (deformable or forced): device arrays,
+// its BVH and the host copy the forced motion updates.
+struct MeshSlot {
+    bool on = false;
+    int nv = 0, nf = 0;
+    double* V = nullptr;
+    int* F = nullptr;
+    double* N = nullptr;
+    BvhNode* bvh = nullptr;
+    int* bvhTri = nullptr;
+    int bvhNodes = 0;
+    std::vector<double> hV, hN;
+    std::vector<int> hF;
+};
+
 
 struct Ctx {
     mps_params P;
@@ -146,6 +161,9 @@ struct Ctx {
     std::vector<double> hxPND, hxNeigh, hZij;
     std::vector<int> hnNeigh;
     bool hasMesh = false;
+    std::vector<double> rigidV, rigidN;   // host copy of the rigid mesh
+    std::vector<int> rigidF;
+    MeshSlot extraMesh[2];                // meshType 1 (deformable) and 2 (forced)
     // PPE
     double* aVal = nullptr;     // [K][N]
     unsigned short* aCol = nullptr;   // [K][N] column codes: slot delta from the previous entry, 0 = escape
@@ -227,6 +245,7 @@ int ppeAssemble(Ctx* ctx);
 int ppeSolve(Ctx* ctx);
 int ppeWarmStart(Ctx* ctx);
 int launchCompaction(Ctx* ctx);
+int launchClosestPoint(Ctx* ctx, int type);   // one mesh type (0 rigid, 1 deformable, 2 forced)
 // learned PPE operator (gnn.hip)
 int gnnEnsure(Ctx* ctx);
 int gnnSetParams(Ctx* ctx, const double* theta, unsigned long long seed);

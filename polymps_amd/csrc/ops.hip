@@ -649,7 +649,7 @@ __device__ __forceinline__ void closestPtTri(const double* p, const double* A, c
 
 // PolygonMesh::closestPointPNDBoundaryAABB (src/PolygonMesh.cpp:585-715),
 // brute force over the faces; ties keep the lowest face index.
-__global__ void kClosestPoint(KArgs a) {
+__global__ void kClosestPoint(KArgs a, int meshType) {
     const int i = slotId();
     if (!live(a, i)) return;
     const double p[3] = {PX(i), PY(i), PZ(i)};
@@ -670,7 +670,7 @@ __global__ void kClosestPoint(KArgs a) {
         a.D[DMY][i] = p[1] + 2.0 * (bq[1] - p[1]);
         a.D[DMZ][i] = p[2] + 2.0 * (bq[2] - p[2]);
         a.D[DPNX][i] = a.mN[bf * 3]; a.D[DPNY][i] = a.mN[bf * 3 + 1]; a.D[DPNZ][i] = a.mN[bf * 3 + 2];
-        a.I[INEARMESH][i] = 0;
+        a.I[INEARMESH][i] = meshType;
     }
 }
 
@@ -689,7 +689,7 @@ __device__ __forceinline__ double boxDist2(const BvhNode& n, const double* p) {
     return s;
 }
 constexpr int kBvhStack = 64;
-__global__ void kClosestPointBvh(KArgs a) {
+__global__ void kClosestPointBvh(KArgs a, int meshType) {
     const int i = slotId();
     if (!live(a, i)) return;
     const double p[3] = {PX(i), PY(i), PZ(i)};
@@ -724,7 +724,7 @@ __global__ void kClosestPointBvh(KArgs a) {
         a.D[DMY][i] = p[1] + 2.0 * (bq[1] - p[1]);
         a.D[DMZ][i] = p[2] + 2.0 * (bq[2] - p[2]);
         a.D[DPNX][i] = a.mN[bf * 3]; a.D[DPNY][i] = a.mN[bf * 3 + 1]; a.D[DPNZ][i] = a.mN[bf * 3 + 2];
-        a.I[INEARMESH][i] = 0;
+        a.I[INEARMESH][i] = meshType;
     }
 }
 
@@ -997,10 +997,15 @@ __device__ double wallDivV(const KArgs& a, int i, double ni) {
         umy = R[3] * ux + R[4] * uy + R[5] * uz;
         umz = R[6] * ux + R[7] * uy + R[8] * uz;
     } else {
-        const double dotnv = n[0] * 0.0 + n[1] * 0.0 + n[2] * 0.0;
-        const double t0 = ux - 2.0 * (0.0 - dotnv * n[0]);
-        const double t1 = uy - 2.0 * (0.0 - dotnv * n[1]);
-        const double t2 = uz - 2.0 * (0.0 - dotnv * n[2]);
+        // wall velocity: uniformVelWall near the forced mesh, 0 otherwise
+        const bool forced = a.I[INEARMESH][i] == 2;
+        const double vw0 = forced ? P.uniform_vel_wall[0] : 0.0;
+        const double vw1 = forced ? P.uniform_vel_wall[1] : 0.0;
+        const double vw2 = forced ? P.uniform_vel_wall[2] : 0.0;
+        const double dotnv = n[0] * vw0 + n[1] * vw1 + n[2] * vw2;
+        const double t0 = ux - 2.0 * (vw0 - dotnv * n[0]);
+        const double t1 = uy - 2.0 * (vw1 - dotnv * n[1]);
+        const double t2 = uz - 2.0 * (vw2 - dotnv * n[2]);
         umx = -1.0 * t0 + 0.0 * t1 + 0.0 * t2;
         umy = 0.0 * t0 + -1.0 * t1 + 0.0 * t2;
         umz = 0.0 * t0 + 0.0 * t1 + -1.0 * t2;
@@ -1587,10 +1592,15 @@ __global__ void kWallViscosity(KArgs a) {
         umz = R[6] * ux + R[7] * uy + R[8] * uz;
         for (int q = 0; q < 9; q++) Rm[q] = R[q];
     } else {
-        const double dotnv = n[0] * 0.0 + n[1] * 0.0 + n[2] * 0.0;
-        const double t0 = ux - 2.0 * (0.0 - dotnv * n[0]);
-        const double t1 = uy - 2.0 * (0.0 - dotnv * n[1]);
-        const double t2 = uz - 2.0 * (0.0 - dotnv * n[2]);
+        // wall velocity: uniformVelWall near the forced mesh, 0 otherwise
+        const bool forced = a.I[INEARMESH][i] == 2;
+        const double vw0 = forced ? P.uniform_vel_wall[0] : 0.0;
+        const double vw1 = forced ? P.uniform_vel_wall[1] : 0.0;
+        const double vw2 = forced ? P.uniform_vel_wall[2] : 0.0;
+        const double dotnv = n[0] * vw0 + n[1] * vw1 + n[2] * vw2;
+        const double t0 = ux - 2.0 * (vw0 - dotnv * n[0]);
+        const double t1 = uy - 2.0 * (vw1 - dotnv * n[1]);
+        const double t2 = uz - 2.0 * (vw2 - dotnv * n[2]);
         umx = -1.0 * t0 + 0.0 * t1 + 0.0 * t2;
         umy = 0.0 * t0 + -1.0 * t1 + 0.0 * t2;
         umz = 0.0 * t0 + 0.0 * t1 + -1.0 * t2;
@@ -2026,6 +2036,30 @@ __global__ void kExtrapolateWallDummy2(KArgs a) {
     a.D[DPRESS][i] = (ni > 0) ? pr / ni : pr;
 }
 
+// PolygonMesh::closestPointPNDBoundaryAABB for one mesh: the kernels keep a
+// strictly closer point than the particle's current one and record the type.
+int launchClosestPoint(Ctx* ctx, int type) {
+    KArgs a = ctx->args();
+    const int g = gridFor(ctx->N, kBlock);
+    if (type == 0) {
+        if (!ctx->hasMesh) { ctx->err = "no wall mesh installed"; return MPS_E_ARG; }
+    } else if (type == 1 || type == 2) {
+        const MeshSlot& m = ctx->extraMesh[type - 1];
+        if (!m.on) { ctx->err = "mesh type not installed"; return MPS_E_ARG; }
+        a.mV = m.V; a.mF = m.F; a.mN = m.N; a.nf = m.nf;
+        a.bvh = (ctx->bvhMode == 1 || (ctx->bvhMode < 0 && m.nf > 64)) ? m.bvh : nullptr;
+        a.bvhTri = m.bvhTri;
+    } else {
+        ctx->err = "mesh type must be 0, 1 or 2";
+        return MPS_E_ARG;
+    }
+    ctx->p4Flags = false;   // the wall distances change
+    if (a.bvh) hipLaunchKernelGGL(kClosestPointBvh, dim3(g), dim3(kBlock), 0, ctx->stream, a, type);
+    else hipLaunchKernelGGL(kClosestPoint, dim3(g), dim3(kBlock), 0, ctx->stream, a, type);
+    MPS_HIP(hipGetLastError());
+    return MPS_OK;
+}
+
 int launchOp(Ctx* ctx, const char* op) {
     const std::string o(op);
     // operators that rewrite the wall distance or the boundary flags that
@@ -2052,9 +2086,12 @@ int launchOp(Ctx* ctx, const char* op) {
         hipLaunchKernelGGL(kApplyDvel, dim3(g), dim3(kBlock), 0, ctx->stream, a, P.collision_type == MPS_COLLISION_DPC ? 1 : 0);
         if (P.collision_type == MPS_COLLISION_DPC) ctx->listValid = false;
     } else if (o == "closest_point") {
-        if (!ctx->hasMesh) { ctx->err = "no wall mesh installed"; return MPS_E_ARG; }
-        if (a.bvh) L1(kClosestPointBvh);
-        else L1(kClosestPoint);
+        // every installed mesh in type order (src/main.cpp:296-300)
+        for (int t = 0; t < 3; t++) {
+            if (t > 0 && !ctx->extraMesh[t - 1].on) continue;
+            int rc = launchClosestPoint(ctx, t);
+            if (rc) return rc;
+        }
     } else if (o == "update_near_mesh") L1(kUpdateNear);
     else if (o == "wall_npcd") LT(kWallNPCD);
     else if (o == "correction_matrix") LT(kCorrectionMatrix);

@@ -202,6 +202,13 @@ void paramsDefaults(mps_params* p) {
     p->mixed_inner_tol = 1e-4;
     p->learned_mode = 0;
     p->ppe_warm_start = 0;
+    p->forced_motion = 1;
+    p->forced_center[0] = 0.45;
+    p->forced_center[1] = 0.0;
+    p->forced_theta_max = 4.0;
+    p->forced_period = 1.9171;
+    p->forced_velocity = 0.0;
+    p->uniform_vel_wall[0] = p->uniform_vel_wall[1] = p->uniform_vel_wall[2] = 0.0;
 }
 
 int paramsFromJson(const char* path, mps_params* p, std::string& grid, std::string& mesh, std::string& err) {
@@ -220,6 +227,15 @@ int paramsFromJson(const char* path, mps_params* p, std::string& grid, std::stri
     const JVal* pn = root.get("pathNames");
     grid = jstr(pn, "particle_grid_file", "oops");
     mesh = jstr(pn, "mesh_rigid_file", "oops");
+    // forced mesh motion: no reference key (the reference hard-codes the law)
+    if (const JVal* fm = root.get("forced_motion")) {
+        p->forced_motion = jint(fm, "rotation", p->forced_motion);
+        p->forced_center[0] = jnum(fm, "center_x", p->forced_center[0]);
+        p->forced_center[1] = jnum(fm, "center_z", p->forced_center[1]);
+        p->forced_theta_max = jnum(fm, "theta_max_deg", p->forced_theta_max);
+        p->forced_period = jnum(fm, "period", p->forced_period);
+        p->forced_velocity = jnum(fm, "velocity", p->forced_velocity);
+    }
     const JVal* dom = root.get("domain");
     const JVal* dmin = dom ? dom->get("min") : nullptr;
     const JVal* dmax = dom ? dom->get("max") : nullptr;
@@ -364,11 +380,19 @@ int paramsDerive(mps_params* p, int n, const double* pos, std::string& err) {
         if (p->wall_type != MPS_WALL_PARTICLE) { err = "periodic domains are supported with particle walls"; return MPS_E_UNSUPPORTED; }
     }
     if (p->fluid_type != 0) { err = "only Newtonian fluids are supported"; return MPS_E_UNSUPPORTED; }
-    if (p->fem_on || p->forced_on) { err = "deformable / forced meshes are not supported"; return MPS_E_UNSUPPORTED; }
+    if ((p->fem_on || p->forced_on) && p->wall_type != MPS_WALL_POLYGON) {
+        err = "deformable / forced meshes need polygon walls";
+        return MPS_E_UNSUPPORTED;
+    }
+    if (p->forced_on && p->forced_motion == 1 && !(p->forced_period > 0.0)) {
+        err = "forced rotation needs a positive period";
+        return MPS_E_ARG;
+    }
     // domain extension (setParameters :139-148)
     for (int k = 0; k < 3; k++) { p->domain_min[k] -= l0 * 3.0; p->domain_max[k] += l0 * 3.0; }
     if ((int)p->dim == 2) { p->domain_min[2] = 0.0; p->domain_max[2] = 0.0; }
-    p->num_of_meshs = (p->wall_type == 1 ? 1 : 0);
+    // MpsParticle::setParameters (src/MpsParticle.cpp:151-155)
+    p->num_of_meshs = (p->wall_type == 1 ? 1 : 0) + (p->fem_on ? 1 : 0) + (p->forced_on ? 1 : 0);
     p->re_s = l0 * p->re_s;
     p->re_l = l0 * p->re_l;
     p->re_repulsive_force = l0 * p->re_repulsive_force;
@@ -520,6 +544,30 @@ int mps_params_from_json(const char* json_path, mps_params* p, char* grid_path, 
     }
     if (grid_path && grid_cap) snprintf(grid_path, grid_cap, "%s", g.c_str());
     if (mesh_path && mesh_cap) snprintf(mesh_path, mesh_cap, "%s", m.c_str());
+    return MPS_OK;
+}
+
+int mps_mesh_paths_from_json(const char* json_path, char* deformable_path, size_t deformable_cap, char* forced_path,
+                             size_t forced_cap) {
+    if (!json_path) return MPS_E_ARG;
+    std::ifstream f(json_path);
+    if (!f) {
+        mpsgpu::setGlobalError(std::string("cannot open ") + json_path);
+        return MPS_E_IO;
+    }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    std::string text = ss.str(), err;
+    mpsgpu::JVal root;
+    if (!mpsgpu::JParser(text).parse(root, err)) {
+        mpsgpu::setGlobalError("JSON parse error: " + err);
+        return MPS_E_IO;
+    }
+    const mpsgpu::JVal* pn = root.get("pathNames");
+    if (deformable_path && deformable_cap)
+        snprintf(deformable_path, deformable_cap, "%s", mpsgpu::jstr(pn, "mesh_deformable_file", "oops").c_str());
+    if (forced_path && forced_cap)
+        snprintf(forced_path, forced_cap, "%s", mpsgpu::jstr(pn, "mesh_forced_file", "oops").c_str());
     return MPS_OK;
 }
 

@@ -14,22 +14,58 @@
 #include "polymps_gpu.hpp"
 
 static std::vector<int> partNearMesh;
+static std::vector<double> nodeFRWX, nodeFRWY, nodeFRWZ;   // forced mesh nodes (src/main.cpp:60-62)
+
+// The meshes present, in type order: the reference keeps them in an array
+// indexed by meshType (mesh[FORCED] = mesh[2]) that it sizes by their count,
+// so a forced mesh without a deformable one falls outside it; here the array
+// always has the three entries and only the present ones are used.
+static std::vector<int> presentMeshes(const MpsParticleSystem* partSyst) {
+    std::vector<int> t{FIXED};
+    if (partSyst->femOn) t.push_back(DEFORMABLE);
+    if (partSyst->forcedOn) t.push_back(FORCED);
+    return t;
+}
+
+static void closestPoints(MpsParticleSystem* partSyst, MpsParticle* part, PolygonMesh* mesh) {
+    for (int me : presentMeshes(partSyst))
+        mesh[me].closestPointPNDBoundaryAABB(partSyst->reS2, partSyst->reL2, part->numParticles, partSyst->weightType,
+                                             part->particleType, partSyst->fluid, me, FIXED, DEFORMABLE, FORCED,
+                                             part->pos, part->particleAtWallPos, part->mirrorParticlePos,
+                                             part->distParticleWall2, part->elementID, part->nearMeshType,
+                                             part->polygonNormal);
+}
 
 static void initMesh(MpsParticleSystem* partSyst, MpsParticle* part, PolygonMesh* mesh, MpsPndNeigh* pndNeigh,
                      MpsInputOutput* io, MpsBucket* buck) {
     printf("Reading STL File... \n");
-    if (partSyst->femOn || partSyst->forcedOn) throw MpsGpuError("deformable and forced meshes are not supported");
-    mesh[0].initPolygonMesh(part->numParticles);
-    mesh[0].readPolygonMeshFile(io->meshRigidFilename);
+    for (int me : presentMeshes(partSyst)) {
+        mesh[me].setMeshType(me);
+        mesh[me].initPolygonMesh(part->numParticles);
+        mesh[me].readPolygonMeshFile(me == FIXED ? io->meshRigidFilename
+                                                 : (me == DEFORMABLE ? io->meshDeformableFilename : io->meshForcedFilename));
+    }
     printf(" Number of meshes: %d\n", partSyst->numOfMeshs);
-    printf(" after removing duplicates, mesh containts %d vertices and %d faces\n", mesh[0].NV.rows(),
-           mesh[0].NF.rows());
+    for (int me : presentMeshes(partSyst))
+        printf(" after removing duplicates, mesh containts %d vertices and %d faces\n", mesh[me].NV.rows(),
+               mesh[me].NF.rows());
     printf("...OK\n");
-    mesh[0].initWijnNeigh((int)partSyst->dim, partSyst->weightType, partSyst->partDist, partSyst->reL, partSyst->reS);
-    mesh[0].closestPointPNDBoundaryAABB(partSyst->reS2, partSyst->reL2, part->numParticles, partSyst->weightType,
-                                        part->particleType, partSyst->fluid, 0, FIXED, DEFORMABLE, FORCED, part->pos,
-                                        part->particleAtWallPos, part->mirrorParticlePos, part->distParticleWall2,
-                                        part->elementID, part->nearMeshType, part->polygonNormal);
+    for (int me : presentMeshes(partSyst))
+        mesh[me].initWijnNeigh((int)partSyst->dim, partSyst->weightType, partSyst->partDist, partSyst->reL,
+                               partSyst->reS);
+    if (partSyst->forcedOn) {
+        const int nn = mesh[FORCED].NV.rows();
+        nodeFRWX.resize(nn);
+        nodeFRWY.resize(nn);
+        nodeFRWZ.resize(nn);
+        for (int q = 0; q < nn; q++) {
+            nodeFRWX[q] = mesh[FORCED].NV(q, 0);
+            nodeFRWY[q] = mesh[FORCED].NV(q, 1);
+            nodeFRWZ[q] = mesh[FORCED].NV(q, 2);
+        }
+        partSyst->uniformVelWall[0] = partSyst->uniformVelWall[1] = partSyst->uniformVelWall[2] = 0.0;
+    }
+    closestPoints(partSyst, part, mesh);
     partNearMesh.clear();
     mesh[0].updateParticlesNearPolygonMesh(partSyst->reS2, partSyst->reL2, part->numParticles, partSyst->weightType,
                                            part->particleType, partSyst->fluid, part->distParticleWall2,
@@ -47,6 +83,12 @@ static void mainLoop(MpsParticleSystem* partSyst, MpsParticle* part, PolygonMesh
         io->displayInfo(partSyst, part, partPress, 100);
         if (partSyst->numOfIterations % partSyst->iterOutput == 0) {
             io->writeOutputFiles(partSyst, part);
+            if (poly) {
+                if (partSyst->femOn) mesh[DEFORMABLE].writePolygonMeshFile(DEFORMABLE, io->vtuOutputFoldername,
+                                                                         partSyst->fileNumber);
+                if (partSyst->forcedOn) mesh[FORCED].writePolygonMeshFile(FORCED, io->vtuOutputFoldername,
+                                                                        partSyst->fileNumber);
+            }
             partSyst->fileNumber++;
         }
         buck->updateParticlesID(partSyst, part);
@@ -65,11 +107,7 @@ static void mainLoop(MpsParticleSystem* partSyst, MpsParticle* part, PolygonMesh
         if (partSyst->collisionType == PC) partColl->checkParticleCollisions(partSyst, part, buck);
         else partColl->checkDynamicParticleCollisions(partSyst, part, buck);
         if (poly) {
-            mesh[0].closestPointPNDBoundaryAABB(partSyst->reS2, partSyst->reL2, part->numParticles, partSyst->weightType,
-                                                part->particleType, partSyst->fluid, 0, FIXED, DEFORMABLE, FORCED,
-                                                part->pos, part->particleAtWallPos, part->mirrorParticlePos,
-                                                part->distParticleWall2, part->elementID, part->nearMeshType,
-                                                part->polygonNormal);
+            closestPoints(partSyst, part, mesh);
             partNearMesh.clear();
             mesh[0].updateParticlesNearPolygonMesh(partSyst->reS2, partSyst->reL2, part->numParticles,
                                                    partSyst->weightType, part->particleType, partSyst->fluid,
@@ -106,6 +144,9 @@ static void mainLoop(MpsParticleSystem* partSyst, MpsParticle* part, PolygonMesh
             if (partSyst->slipCondition == FREE_SLIP) partVisc->calcWallSlipViscosity(partSyst, part, buck);
             else partVisc->calcWallNoSlipViscosity(partSyst, part, buck);
         }
+        if (partSyst->forcedOn)
+            mesh[FORCED].updateForcedPolygonMesh(nodeFRWX.data(), nodeFRWY.data(), nodeFRWZ.data(),
+                                                 partSyst->uniformVelWall, partSyst->timeStep, partSyst->timeCurrent);
         partVelPos->correctionVelocityPosition(partSyst, part);
         part->checkParticleOutDomain(partSyst);
         if (partSyst->shiftingType == 1) {
@@ -142,7 +183,7 @@ int main(int argc, char** argv) {
         MpsPndNeigh particlePndNeigh;
         std::vector<PolygonMesh> solidMesh;
         if (particleSystem.wallType == POLYGON) {
-            solidMesh.resize(particleSystem.numOfMeshs);
+            solidMesh.resize(3);   // indexed by meshType
             initMesh(&particleSystem, &particles, solidMesh.data(), &particlePndNeigh, &inputOutput, &buckets);
             printf("Mesh class initialized.\n");
         }

@@ -391,6 +391,10 @@ public:
     Rows NV, NF;
 
     void initPolygonMesh(const int nP) { (void)nP; }
+    // which of the context's meshes this object is (meshType: FIXED,
+    // DEFORMABLE, FORCED); main.cpp indexes its mesh array by that type
+    void setMeshType(int t) { type_ = t; }
+    int meshTypeOf() const { return type_; }
     // PolygonMesh::readPolygonMeshFile (src/PolygonMesh.cpp:42-108)
     void readPolygonMeshFile(const std::string& path) {
         int nv = 0, nf = 0;
@@ -410,7 +414,8 @@ public:
     void initWijnNeigh(int dim, int wijType, double lo, double reL, double reS) {
         (void)dim; (void)wijType; (void)lo; (void)reL; (void)reS;
         mps_ctx* ctx = polymps_gpu::currentContext();
-        polymps_gpu::check(mps_set_mesh(ctx, NV.n, V_.data(), NF.n, F_.data(), N_.data()), ctx, "initWijnNeigh");
+        polymps_gpu::check(mps_set_mesh_type(ctx, type_, NV.n, V_.data(), NF.n, F_.data(), N_.data()), ctx,
+                           "initWijnNeigh");
     }
     // PolygonMesh::closestPointPNDBoundaryAABB (src/PolygonMesh.cpp:585-715)
     void closestPointPNDBoundaryAABB(double reS2, double reL2, int nP, int wijType, int* Typ, int fld, int msh_id,
@@ -418,10 +423,9 @@ public:
                                      double* mirrorPos, double* riw2, int* elementID, int* meshID, double* NormalWall) {
         (void)reS2; (void)reL2; (void)nP; (void)wijType; (void)Typ; (void)fld; (void)sta_id; (void)fem_id;
         (void)frw_id; (void)Pos; (void)wallPos; (void)mirrorPos; (void)riw2; (void)elementID; (void)meshID;
-        (void)NormalWall;
-        if (msh_id != 0) throw MpsGpuError("only the rigid mesh (mesh 0) is supported");
+        (void)NormalWall; (void)msh_id;
         mps_ctx* ctx = polymps_gpu::currentContext();
-        polymps_gpu::check(mps_mesh_closest_point(ctx), ctx, "closestPointPNDBoundaryAABB");
+        polymps_gpu::check(mps_mesh_closest_point_type(ctx, type_), ctx, "closestPointPNDBoundaryAABB");
     }
     // PolygonMesh::updateParticlesNearPolygonMesh (src/PolygonMesh.cpp:718-795)
     void updateParticlesNearPolygonMesh(double reS2, double reL2, int nP, int wijType, int* Typ, int fld,
@@ -432,14 +436,45 @@ public:
         mps_ctx* ctx = polymps_gpu::currentContext();
         polymps_gpu::check(mps_mesh_update_near(ctx), ctx, "updateParticlesNearPolygonMesh");
     }
-    void writePolygonMeshFile(const int, const std::string&, const int) {}
-    void updateForcedPolygonMesh(double*, double*, double*, double*, const double, const double) {
-        throw MpsGpuError("forced meshes are not supported");
+    // PolygonMesh::writePolygonMeshFile (src/PolygonMesh.cpp:110-139): <path>/meshNN_XXXXX.stl
+    void writePolygonMeshFile(const int mesh_ID, const std::string& path, const int iF) {
+        mps_ctx* ctx = polymps_gpu::currentContext();
+        int nv = 0, nf = 0;
+        polymps_gpu::check(mps_get_mesh(ctx, type_, &nv, nullptr, &nf, nullptr, nullptr), ctx, "writePolygonMeshFile");
+        std::vector<double> V(3 * (size_t)nv);
+        std::vector<int> F(3 * (size_t)nf);
+        polymps_gpu::check(mps_get_mesh(ctx, type_, &nv, V.data(), &nf, F.data(), nullptr), ctx, "writePolygonMeshFile");
+        char name[64];
+        snprintf(name, sizeof name, "/mesh%02d_%05d.stl", mesh_ID, iF);
+        polymps_gpu::check(mps_mesh_write_stl((path + name).c_str(), nv, V.data(), nf, F.data()), ctx,
+                           "writePolygonMeshFile");
+    }
+    // PolygonMesh::updateForcedPolygonMesh (src/PolygonMesh.cpp:173-374): the
+    // nodes move in the context; the caller's node arrays and wall velocity
+    // receive the new values.
+    void updateForcedPolygonMesh(double* nodeX, double* nodeY, double* nodeZ, double* velVWall, const double dt,
+                                 const double time) {
+        mps_ctx* ctx = polymps_gpu::currentContext();
+        polymps_gpu::check(mps_forced_mesh_update(ctx, dt, time), ctx, "updateForcedPolygonMesh");
+        std::vector<double> V(3 * (size_t)NV.n);
+        polymps_gpu::check(mps_get_mesh(ctx, type_, nullptr, V.data(), nullptr, nullptr, nullptr), ctx,
+                           "updateForcedPolygonMesh");
+        for (int q = 0; q < NV.n; q++) {
+            if (nodeX) nodeX[q] = V[3 * (size_t)q];
+            if (nodeY) nodeY[q] = V[3 * (size_t)q + 1];
+            if (nodeZ) nodeZ[q] = V[3 * (size_t)q + 2];
+        }
+        V_ = V;
+        mps_params prm;
+        polymps_gpu::check(mps_get_params(ctx, &prm), ctx, "updateForcedPolygonMesh");
+        if (velVWall)
+            for (int k = 0; k < 3; k++) velVWall[k] = prm.uniform_vel_wall[k];
     }
 
 private:
     std::vector<double> V_, N_;
     std::vector<int> F_;
+    int type_ = FIXED;
 };
 
 // include/MpsInputOutput.h: input (JSON, .grid) and output (VTU, PVD).
@@ -477,6 +512,11 @@ public:
         gridFilename = grid;
         meshRigidFilename = mesh;
         vtuOutputFoldername = folder;
+        char deformable[4096], forced[4096];
+        polymps_gpu::check(mps_mesh_paths_from_json(path.c_str(), deformable, sizeof deformable, forced, sizeof forced),
+                           nullptr, "readInputFile (mesh paths)");
+        meshDeformableFilename = deformable;
+        meshForcedFilename = forced;
         PSystem->syncFromParams();
         printf("OK\n");
         printf("Reading GRID File... ");

@@ -28,6 +28,7 @@ _OPS = [
 
 VEC_FIELDS = {"pos", "vel", "acc", "acc_star", "npcd", "mirror", "wall_pos", "polygon_normal", "force_wall",
               "cm1", "cm2", "cm3", "grad_conc", "normal"}
+MESH_FIXED, MESH_DEFORMABLE, MESH_FORCED = 0, 1, 2
 INT_FIELDS = {"type", "bc", "num_neigh", "num_neigh_wall", "near_wall", "near_mesh_type"}
 
 
@@ -59,6 +60,12 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     lib.mps_sync.argtypes = [vp]
     lib.mps_mesh_read_stl.argtypes = [cp, ctypes.POINTER(ip), vp, ctypes.POINTER(ip), vp, vp]
     lib.mps_set_mesh.argtypes = [vp, ip, vp, ip, vp, vp]
+    lib.mps_set_mesh_type.argtypes = [vp, ip, ip, vp, ip, vp, vp]
+    lib.mps_get_mesh.argtypes = [vp, ip, ctypes.POINTER(ip), vp, ctypes.POINTER(ip), vp, vp]
+    lib.mps_forced_mesh_update.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+    lib.mps_mesh_closest_point_type.argtypes = [vp, ip]
+    lib.mps_mesh_write_stl.argtypes = [cp, ip, vp, ip, vp]
+    lib.mps_mesh_paths_from_json.argtypes = [cp, cp, ctypes.c_size_t, cp, ctypes.c_size_t]
     lib.mps_get_wall_tables.argtypes = [vp, vp, vp, vp, vp]
     lib.mps_ppe_matrix.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ip)]
     lib.mps_steps.argtypes = [vp, ip]
@@ -143,7 +150,10 @@ def read_stl(path: str | os.PathLike):
 class Simulation:
     """A device-resident particle system driven through the C ABI."""
 
-    def __init__(self, params: MpsParams, types, pos, vel=None, press=None, press_avg=None, mesh=None, device=0):
+    def __init__(self, params: MpsParams, types, pos, vel=None, press=None, press_avg=None, mesh=None, device=0,
+                 meshes=None):
+        """mesh: the rigid mesh (V, F, N); meshes: further meshes by type
+        ({MESH_DEFORMABLE: (V, F, N), MESH_FORCED: (V, F, N)})."""
         self._lib = lib()
         types = np.ascontiguousarray(types, dtype=np.int32)
         n = len(types)
@@ -166,6 +176,28 @@ class Simulation:
             V, F, N = (np.ascontiguousarray(mesh[0], dtype=np.float64), np.ascontiguousarray(mesh[1], dtype=np.int32),
                        np.ascontiguousarray(mesh[2], dtype=np.float64))
             self._check(self._lib.mps_set_mesh(h, len(V) // 3, _ptr(V), len(F) // 3, _ptr(F), _ptr(N)))
+        for t, m in (meshes or {}).items():
+            self.set_mesh_type(t, *m)
+
+    def set_mesh_type(self, mesh_type: int, V, F, N):
+        V, F, N = (np.ascontiguousarray(V, dtype=np.float64), np.ascontiguousarray(F, dtype=np.int32),
+                   np.ascontiguousarray(N, dtype=np.float64))
+        self._check(self._lib.mps_set_mesh_type(self._h, mesh_type, len(V) // 3, _ptr(V), len(F) // 3, _ptr(F),
+                                                _ptr(N)))
+
+    def get_mesh(self, mesh_type: int):
+        nv, nf = ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.mps_get_mesh(self._h, mesh_type, ctypes.byref(nv), None, ctypes.byref(nf), None, None))
+        V, F, N = np.zeros(3 * nv.value), np.zeros(3 * nf.value, dtype=np.int32), np.zeros(3 * nf.value)
+        self._check(self._lib.mps_get_mesh(self._h, mesh_type, ctypes.byref(nv), _ptr(V), ctypes.byref(nf), _ptr(F),
+                                           _ptr(N)))
+        return V, F, N
+
+    def forced_mesh_update(self, dt: float, time: float):
+        self._check(self._lib.mps_forced_mesh_update(self._h, dt, time))
+
+    def closest_point_type(self, mesh_type: int):
+        self._check(self._lib.mps_mesh_closest_point_type(self._h, mesh_type))
 
     def _check(self, rc):
         if rc:

@@ -49,8 +49,9 @@ def sync_state(g: Simulation, o: Oracle, fields=PERSISTENT):
 
 
 def new_pair(w, threads=0):
-    g = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh)
-    o = Oracle(w.params, w.types, w.pos, w.vel, mesh=w.mesh, threads=threads)
+    extra = getattr(w, "extra_meshes", None)
+    g = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh, meshes=extra)
+    o = Oracle(w.params, w.types, w.pos, w.vel, mesh=w.mesh, threads=threads, meshes=extra)
     return g, o
 
 
@@ -91,7 +92,7 @@ def ppe_check(g, o, where, press_rtol):
     compare_exact("ppe_b", g.get("ppe_b"), o.get("ppe_b"), where)
 
 
-def lockstep_step(g: Simulation, o: Oracle, step: int, press_rtol=1e-6, log=None):
+def lockstep_step(g: Simulation, o: Oracle, step: int, press_rtol=1e-6, log=None, time=None):
     p = g.params
     W = f"step {step}"
     poly = p.wall_type == 1
@@ -120,7 +121,7 @@ def lockstep_step(g: Simulation, o: Oracle, step: int, press_rtol=1e-6, log=None
         both("collision", ["collision"], ["vel", "pos"])
         g.call("build_neighbor_list")
     if poly:
-        both("mesh_closest_point", ["closest_point"], ["riw2", "mirror", "wall_pos", "polygon_normal"])
+        both("mesh_closest_point", ["closest_point"], ["riw2", "mirror", "wall_pos", "polygon_normal", "near_mesh_type"])
         both("mesh_update_near", ["update_near_mesh"], ["pnd_wall", "num_neigh_wall", "near_wall"])
         both("wall_npcd", ["wall_npcd"], ["npcd"])
     if p.gradient_correction:
@@ -162,6 +163,13 @@ def lockstep_step(g: Simulation, o: Oracle, step: int, press_rtol=1e-6, log=None
     if poly:
         both("wall_press_gradient", ["wall_press_gradient"], ["acc", "force_wall"])
         both("wall_viscosity", ["wall_viscosity"], ["acc", "force_wall"])
+    if p.forced_on:
+        t = step * p.time_step if time is None else time
+        g.forced_mesh_update(p.time_step, t)
+        o.forced_mesh_update(p.time_step, t)
+        for a, b, name in zip(g.get_mesh(2), o.get_mesh(2), ("nodes", "faces", "normals")):
+            if not np.array_equal(a, b):
+                raise Mismatch(f"{W}: forced mesh {name} differ")
     both("correction", ["correction"], ["pos", "vel", "acc"])
     vg, vo = g.scalar("vel_max"), o.scalar("vel_max")
     if vg != vo:

@@ -597,3 +597,71 @@ def test_bicgstab_matches_dense_solve(built, guess):
     assert np.linalg.norm(x - x_dense) <= 1e-10 * np.linalg.norm(x_dense)
     assert np.linalg.norm(b - A @ x) <= 1e-12 * np.linalg.norm(b)
     assert 0 < o.scalar("solver_iter") < 2 * n
+
+
+# ---------------------------------------------------------- extra meshes
+def _box_dist(P, lo, hi, inside):
+    """Distance from points to the surface of an axis-aligned box: to the
+    nearest face for points inside, to the clamped point for points outside."""
+    if inside:
+        return np.stack([P - lo, hi - P], 1).reshape(len(P), 6).min(1)
+    d = np.maximum(np.maximum(lo - P, P - hi), 0.0)
+    return np.sqrt((d * d).sum(1))
+
+
+def test_closest_point_over_three_meshes(built):
+    """Tank (rigid, particles inside), static slab (meshType 1) and paddle
+    (meshType 2): the wall distance is the smallest of the three closed-form
+    box distances and nearMeshType names the mesh that attains it (the
+    meshes are visited in type order and only a strictly closer point
+    replaces the current one)."""
+    w = configs.forced_paddle3d(l0=0.025)
+    o = Oracle(w.params, w.types, w.pos, mesh=w.mesh, meshes=w.extra_meshes)
+    o.call("closest_point")
+    P = w.pos.reshape(-1, 3)
+    boxes = [(w.mesh[0], True), (w.extra_meshes[1][0], False), (w.extra_meshes[2][0], False)]
+    D = np.stack([_box_dist(P, V.reshape(-1, 3).min(0), V.reshape(-1, 3).max(0), inside) for V, inside in boxes], 1)
+    dmin = D.min(1)
+    assert np.allclose(np.sqrt(o.get("riw2")), dmin, rtol=1e-12, atol=1e-15)
+    kind = o.get("near_mesh_type")
+    srt = np.sort(D, 1)
+    clear = srt[:, 1] - srt[:, 0] > 1e-9          # no near-tie between two meshes
+    assert np.array_equal(kind[clear], D.argmin(1)[clear])
+    assert set(np.unique(kind)) == {0, 1, 2}        # every mesh is the nearest for some particle
+    M = o.get("mirror").reshape(-1, 3)
+    Wp = o.get("wall_pos").reshape(-1, 3)
+    assert np.allclose(M, 2 * Wp - P, atol=1e-15)
+
+
+@pytest.mark.parametrize("rotation", [1, 0])
+def test_forced_motion_closed_form(built, rotation):
+    """updateForcedPolygonMesh moves each node by its velocity at the start of
+    the step: for the rotation about (Xcg, Zcg) with rate w the node's
+    distance from the axis grows by sqrt(1 + (w dt)^2) and its angle
+    atan2(z - Zcg, x - Xcg) falls by atan(w dt), y unchanged, with
+    w = theta_max (2 pi / T) cos(2 pi t / T) in rad/s; the translation moves
+    every node by (0, 0, v dt).  The normals are those of the moved faces."""
+    w = configs.forced_paddle3d(l0=0.025, theta_max=12.0, period=0.8, rotation=rotation, velocity=0.7)
+    o = Oracle(w.params, w.types, w.pos, mesh=w.mesh, meshes=w.extra_meshes)
+    p = w.params
+    dt = 1e-3
+    V0 = o.get_mesh(2)[0].reshape(-1, 3).copy()
+    for t in (0.0, 0.13, 0.37):
+        o.forced_mesh_update(dt, t)
+        V1, F1, N1 = o.get_mesh(2)
+        V1 = V1.reshape(-1, 3)
+        if rotation:
+            om = p.forced_theta_max * 2 * math.pi / p.forced_period * math.cos(2 * math.pi * t / p.forced_period)
+            om *= math.pi / 180.0
+            c = np.array([p.forced_center[0], p.forced_center[1]])
+            r0 = np.hypot(V0[:, 0] - c[0], V0[:, 2] - c[1])
+            r1 = np.hypot(V1[:, 0] - c[0], V1[:, 2] - c[1])
+            assert np.allclose(r1, r0 * math.sqrt(1 + (om * dt) ** 2), rtol=1e-13)
+            a0 = np.arctan2(V0[:, 2] - c[1], V0[:, 0] - c[0])
+            a1 = np.arctan2(V1[:, 2] - c[1], V1[:, 0] - c[0])
+            assert np.allclose(a0 - a1, math.atan(om * dt), atol=1e-12)
+        else:
+            assert np.allclose(V1 - V0, [0.0, 0.0, 0.7 * dt], atol=1e-15)
+        assert np.array_equal(V1[:, 1], V0[:, 1])
+        assert np.allclose(N1, configs.face_normals(V1.reshape(-1), F1), atol=1e-15)
+        V0 = V1.copy()
