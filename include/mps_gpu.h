@@ -181,6 +181,11 @@ int mps_vel_divergence(mps_ctx* ctx);
 int mps_ppe_assemble(mps_ctx* ctx);
 /* PPE solve only, from the current ppe_x guess (second half of mps_calc_press) */
 int mps_ppe_solve(mps_ctx* ctx);
+/* The initial guess mps_calc_press gives the solve (no reference
+ * counterpart): ppe_x becomes the extrapolation of order ppe_warm_start from
+ * ppe_x and the two solutions before it, which are shifted back by one;
+ * nothing happens for ppe_warm_start 0. */
+int mps_ppe_warm_start(mps_ctx* ctx);
 /* Assembled iteration matrix in reference order: row_ptr[N+1], cols/vals[nnz]
  * (column = reference particle index).  Arrays may be NULL to query nnz. */
 int mps_ppe_matrix(mps_ctx* ctx, int* row_ptr, int* cols, double* vals, int* nnz);

@@ -1048,6 +1048,14 @@ int mps_ppe_solve(mps_ctx* ctx) {
     return syncStatus(ctx);
 }
 
+int mps_ppe_warm_start(mps_ctx* ctx) {
+    CHECK_CTX(ctx);
+    hipSetDevice(ctx->dev);
+    int rc = ppeWarmStart(ctx);
+    if (rc) return rc;
+    return syncStatus(ctx);
+}
+
 int mps_ppe_matrix(mps_ctx* ctx, int* row_ptr, int* cols, double* vals, int* nnz_out) {
     CHECK_CTX(ctx);
     hipSetDevice(ctx->dev);

@@ -22,7 +22,7 @@ _OPS = [
     "prediction_press_gradient", "prediction_wall_press_gradient", "collision", "mesh_closest_point",
     "mesh_update_near", "wall_npcd", "correction_matrix", "initial_pnd",
     "pnd_neigh_npcd", "pnd_diffusive", "mean_pnd", "mean_neigh_fluid_pnd", "update_bc", "calc_press",
-    "vel_divergence", "ppe_assemble", "ppe_solve", "extrapolate_wall_dummy", "press_gradient", "wall_press_gradient",
+    "vel_divergence", "ppe_assemble", "ppe_solve", "ppe_warm_start", "extrapolate_wall_dummy", "press_gradient", "wall_press_gradient",
     "normal_particles", "wall_normal_particles", "wall_viscosity", "correction", "shifting", "wall_dummy_velocity", "init", "step",
 ]
 
