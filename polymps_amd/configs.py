@@ -406,17 +406,24 @@ def channel4m(l0: float = 0.005, seed: int = 5, jitter: float = 0.01, steps: int
 
 
 # -------------------------------------------------------------- case files
-def case_json(p: MpsParams, grid: str, mesh: str, vtu_folder: str, vtu: dict | None = None) -> dict:
+def case_json(p: MpsParams, grid: str, mesh: str, vtu_folder: str, vtu: dict | None = None,
+              extra_mesh: dict | None = None) -> dict:
     """The parameter block as an input JSON file of the reference's schema
     (the keys read by src/MpsInputOutput.cpp:136-500, plus the solver
     extension keys under numerical.semi_implicit_mps)."""
     ax = ("x", "y", "z")
+    names = {"particle_grid_file": grid, "mesh_rigid_file": mesh, "vtu_output_folder": vtu_folder}
+    if extra_mesh:
+        names.update(extra_mesh)
     return {
+        "forced_motion": {"rotation": int(p.forced_motion), "center_x": p.forced_center[0],
+                          "center_z": p.forced_center[1], "theta_max_deg": p.forced_theta_max,
+                          "period": p.forced_period, "velocity": p.forced_velocity},
         "flags": {"wall_type": int(p.wall_type), "fem_MESH": bool(p.fem_on), "forced_MESH": bool(p.forced_on),
                   "output_VTU": dict({"type": 0, "only_freeSurface": False, "pnd": False, "neigh": False,
                                       "deviation": False, "concentration": False, "auxiliar": False},
                                      **(vtu or {}))},
-        "pathNames": {"particle_grid_file": grid, "mesh_rigid_file": mesh, "vtu_output_folder": vtu_folder},
+        "pathNames": names,
         "domain": {"min": {a: float(p.domain_min[k]) for k, a in enumerate(ax)},
                    "max": {a: float(p.domain_max[k]) for k, a in enumerate(ax)},
                    "boundary": {"type": int(p.domain_type_bc), "limit": int(p.limit_type_bc),
@@ -481,8 +488,13 @@ def write_case(w: Workload, folder, name: str = "case", vtu: dict | None = None)
     if w.mesh is not None:
         mesh = f"{name}.stl"
         write_stl(folder / mesh, w.mesh[0], w.mesh[1])
+    extra = {}
+    for t, key in ((1, "mesh_deformable_file"), (2, "mesh_forced_file")):
+        if w.extra_meshes and t in w.extra_meshes:
+            extra[key] = f"{name}_mesh{t}.stl"
+            write_stl(folder / extra[key], w.extra_meshes[t][0], w.extra_meshes[t][1])
     path = folder / f"{name}.json"
-    path.write_text(json.dumps(case_json(w.params, grid, mesh, f"{name}_out", vtu), indent=1))
+    path.write_text(json.dumps(case_json(w.params, grid, mesh, f"{name}_out", vtu, extra), indent=1))
     return path
 
 

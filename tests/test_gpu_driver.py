@@ -24,6 +24,7 @@ pytestmark = pytest.mark.gpu
     (lambda: configs.small_dam3d(l0=0.025, steps=10), 1 | 2 | 16),
     (lambda: configs.small_dam2d(l0=0.02, steps=10), 1 | 2 | 8 | 16),
     (lambda: configs.periodic_channel2d(steps=10), 1 | 2 | 16),
+    (lambda: configs.forced_paddle3d(l0=0.025, steps=10), 1 | 2 | 16),
 ])
 def test_driver_matches_library_steps(built, tmp_path, make, flags):
     w = make()
@@ -38,7 +39,7 @@ def test_driver_matches_library_steps(built, tmp_path, make, flags):
     out = tmp_path / "case_out"
     assert (tmp_path / "case_out.pvd").exists()
     assert (out / "output00000.vtu").exists() and (out / "output00001.vtu").exists()
-    g = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh, device=0)
+    g = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh, device=0, meshes=w.extra_meshes)
     g.call("init")
     g.write_vtu(tmp_path / "lib0.vtu", flags)
     for _ in range(5):
@@ -46,6 +47,12 @@ def test_driver_matches_library_steps(built, tmp_path, make, flags):
     g.write_vtu(tmp_path / "lib5.vtu", flags)
     assert (out / "output00000.vtu").read_bytes() == (tmp_path / "lib0.vtu").read_bytes()
     assert (out / "output00001.vtu").read_bytes() == (tmp_path / "lib5.vtu").read_bytes()
+    if w.params.forced_on:
+        # the moved paddle as an ASCII STL at each output (mesh02_XXXXX.stl)
+        text = (out / "mesh02_00001.stl").read_text().split()
+        verts = np.array([float(v) for k, tok in enumerate(text) if tok == "vertex" for v in text[k + 1:k + 4]])
+        V, F, _ = g.get_mesh(2)
+        assert np.allclose(verts, V.reshape(-1, 3)[F.reshape(-1)].reshape(-1), rtol=1e-5, atol=1e-9)
 
 
 def test_binary_vtu_matches_ascii(built, tmp_path):
