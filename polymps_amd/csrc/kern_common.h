@@ -45,6 +45,17 @@ __device__ __forceinline__ double delWgt(double dst, double re, int t) {
 // measured slower than this plain mapping on the 1.73M dam break.)
 __device__ __forceinline__ int slotId() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
+// Slot of a thread when the workgroups, dealt round-robin to the 8 XCDs, are
+// renumbered so that XCD x takes one contiguous range of slot blocks: the
+// neighbour data a list consumer gathers for adjacent particles is then
+// fetched into one XCD's L2 instead of eight.
+__device__ __forceinline__ int xcdSlotId() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, k = b >> 3;
+    return (x * q + min(x, r) + k) * blockDim.x + threadIdx.x;
+}
+
 // Particle slot k takes part in per-particle loops when its reference index
 // is below the live particle count (the reference loops i < numParticles).
 __device__ __forceinline__ bool live(const KArgs& a, int k) {
