@@ -41,8 +41,9 @@ __device__ __forceinline__ double delWgt(double dst, double re, int t) {
 }
 
 // Slot handled by this thread in a one-thread-per-slot launch.  (A
-// renumbering that gives each of the 8 XCDs one contiguous slot range was
-// measured slower than this plain mapping on the 1.73M dam break.)
+// renumbering that gives each of the 8 XCDs one contiguous slot range,
+// xcdSlotId below, was measured slower than this plain mapping on the 1.73M
+// dam break for every kernel but the list-based viscosity.)
 __device__ __forceinline__ int slotId() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
 // Slot of a thread when the workgroups, dealt round-robin to the 8 XCDs, are
