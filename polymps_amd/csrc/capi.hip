@@ -248,8 +248,8 @@ static int runInit(Ctx* ctx) {
         if ((rc = launchOp(ctx, "press_explicit"))) return rc;
     } else {
         if (P.mps_type == MPS_IMPLICIT_PND_DIVU && (rc = velDivergence(ctx))) return rc;
-        if ((rc = ppeAssemble(ctx))) return rc;
         if ((rc = ppeWarmStart(ctx))) return rc;
+        if ((rc = ppeAssemble(ctx))) return rc;
         if ((rc = ppeSolve(ctx))) return rc;
     }
     return syncStatus(ctx);
@@ -260,8 +260,9 @@ static int calcPress(Ctx* ctx) {
     int rc;
     if (P.mps_type == MPS_EXPLICIT || P.mps_type == MPS_WEAKLY) return launchOp(ctx, "press_explicit");
     if (P.mps_type == MPS_IMPLICIT_PND_DIVU && (rc = velDivergence(ctx))) return rc;
-    if ((rc = ppeAssemble(ctx))) return rc;
+    // the guess first: the mixed assembly computes the first residual from it
     if ((rc = ppeWarmStart(ctx))) return rc;
+    if ((rc = ppeAssemble(ctx))) return rc;
     return ppeSolve(ctx);
 }
 
@@ -605,6 +606,7 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
     // positions or types changed: the neighbour list and its packed copy are stale
     if ((fd->kind != 2 && fd->f0 == DX) || (fd->kind == 2 && fd->f0 == ITYPE)) ctx->listValid = false;
     ctx->p4Flags = false;   // the boundary flags packed by update_bc may be stale
+    ctx->r0Fresh = false;   // and so may the residual the assembly left
     MPS_HIP(hipGetLastError());
     return MPS_OK;
 }
@@ -643,6 +645,7 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
 
 int mps_set_scalar(mps_ctx* ctx, const char* name, double value) {
     CHECK_CTX(ctx);
+    ctx->r0Fresh = false;   // a solve setting may change: the next solve recomputes its first residual
     const std::string s(name ? name : "");
     if (s == "vel_max") {
         MPS_HIP(hipMemcpyAsync(ctx->ds + S_VELMAX, &value, sizeof(double), hipMemcpyHostToDevice, ctx->stream));

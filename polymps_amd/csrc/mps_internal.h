@@ -205,6 +205,7 @@ struct Ctx {
     int ppeSolves = 0;           // pressure solves so far (history of the extrapolated warm start)
     int lastCgIters = 0;         // iterations of the last FP64 CG solve / first mixed inner pass
     int lastInnerIters = 0;
+    bool r0Fresh = false;        // cgR and the reduction partials hold b - A x of the current guess (assembly)
     double solverErr = 0.0;
     double trueResidual = 0.0;
     double ppeMs = 0.0;

@@ -22,14 +22,19 @@ namespace mpsgpu {
 
 constexpr int kB = 256;
 
-template <bool LIST, bool TRANS>
-__global__ void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc, int* cnt, double* diag, double* b,
-                             float* val32, float* diag32, float* invd32) {
-    const int i = slotId();
-    if (i >= a.N) return;
+// RES: the row also gives the FP64 true residual r_i = b_i - (A x)_i of the
+// initial guess x (the refinement loop's first residual, which kTrueResidual
+// would otherwise compute from the stored row), and |r_i|^2, |b_i|^2 for the
+// block partials.  kTrueResidual sums diag x_i first; here the off-diagonal
+// products are summed first: the same FP64 quantity, rounded differently.
+template <bool LIST, bool TRANS, bool RES>
+__device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, unsigned short* col, int* esc,
+                                            int* cnt, double* diag, double* b, float* val32, float* diag32,
+                                            float* invd32, double* x, double* r, double& rn2, double& bn2) {
     if (!live(a, i)) {
         cnt[i] = 0; diag[i] = 1.0; b[i] = 0.0;
         if (invd32) { invd32[i] = 1.0f; diag32[i] = 1.0f; }
+        if (RES) { x[i] = 0.0; r[i] = 0.0; }
         return;
     }
     const mps_params& P = a.P;
@@ -37,8 +42,14 @@ __global__ void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc
     if (bci == P.other || bci == P.surface) {
         cnt[i] = 0; diag[i] = 1.0; b[i] = 0.0;
         if (invd32) { invd32[i] = 1.0f; diag32[i] = 1.0f; }
+        if (RES) {
+            const double ri = 0.0 - 1.0 * x[i];
+            r[i] = ri;
+            rn2 = ri * ri;
+        }
         return;
     }
+    double ax = 0.0;   // sum of the row's off-diagonal products with x (RES)
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
     const double xi = a.D[DX][i], yi = a.D[DY][i], zi = a.D[DZ][i];
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
@@ -116,6 +127,7 @@ __global__ void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc
                 const double mv = (TRANS && d2T != d2) ? wgt(sqrt(d2T), P.re_l, P.weight_type) * P.coeff_ppe : mij;
                 val[o] = mv;
                 if (val32) val32[o] = (float)mv;
+                if (RES) ax += mv * x[j];
                 if (delta) {
                     col[o] = (unsigned short)dj;
                 } else {
@@ -145,8 +157,19 @@ __global__ void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc
         invd32[i] = (sum != 0.0) ? (float)(1.0 / sum) : 1.0f;
         diag32[i] = (float)sum;
     }
+    if (RES) {
+        const double ri = rhs - (sum * x[i] + ax);
+        r[i] = ri;
+        rn2 = ri * ri;
+        bn2 = rhs * rhs;
+    }
     atomicAdd(&a.is[I_NNZ], c);
 }
+
+template <bool LIST, bool TRANS, bool RES>
+__global__ __launch_bounds__(256) void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc, int* cnt,
+                                                    double* diag, double* b, float* val32, float* diag32,
+                                                    float* invd32, double* x, double* r);
 
 // ------------------------------------------------------------ reductions
 // Every vector kernel runs one thread per row; each block leaves a pair of
@@ -168,6 +191,17 @@ __device__ __forceinline__ void blockReduce2(double& s0, double& s1) {
 __device__ __forceinline__ void writePartials(const KArgs& a, double s0, double s1) {
     blockReduce2(s0, s1);
     if (threadIdx.x == 0) { a.red[blockIdx.x * 2] = s0; a.red[blockIdx.x * 2 + 1] = s1; }
+}
+
+template <bool LIST, bool TRANS, bool RES>
+__global__ __launch_bounds__(256) void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc, int* cnt,
+                                                    double* diag, double* b, float* val32, float* diag32,
+                                                    float* invd32, double* x, double* r) {
+    const int i = slotId();
+    double rn2 = 0.0, bn2 = 0.0;
+    if (i < a.N)
+        assembleRow<LIST, TRANS, RES>(a, i, val, col, esc, cnt, diag, b, val32, diag32, invd32, x, r, rn2, bn2);
+    if (RES) writePartials(a, rn2, bn2);   // the partials kTrueResidualFinish sums
 }
 // three sums, partials at red[3 * block]
 __device__ __forceinline__ void writePartials3(const KArgs& a, double s0, double s1, double s2) {
@@ -622,16 +656,27 @@ int ppeAssemble(Ctx* ctx) {
     float* d32 = mixed ? ctx->aDiag32 : nullptr;
     float* i32 = mixed ? ctx->aDiagInv32 : nullptr;
     MPS_HIP(hipMemsetAsync(ctx->is + I_NNZ, 0, 2 * sizeof(int), ctx->stream));   // I_NNZ, I_NESC
-#define ASM(L, TT)                                                                                         \
-    hipLaunchKernelGGL((kPpeAssemble<L, TT>), dim3(g), dim3(kB), 0, ctx->stream, a, ctx->aVal, ctx->aCol,   \
-                       ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->D[DPPEB], v32, d32, i32)
-    if (ctx->listValid) {
-        if (T) ASM(true, true); else ASM(true, false);
+    // the mixed solve's first true residual comes out of the assembly (the
+    // learned warm start changes the guess after the assembly, so not then)
+    const bool res = mixed && ctx->P.learned_mode == 0;
+    double* x = ctx->D[DPPEX];
+#define ASM(L, TT, R)                                                                                          \
+    hipLaunchKernelGGL((kPpeAssemble<L, TT, R>), dim3(g), dim3(kB), 0, ctx->stream, a, ctx->aVal, ctx->aCol,   \
+                       ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->D[DPPEB], v32, d32, i32, x, ctx->cgR)
+    if (res) {
+        if (ctx->listValid) {
+            if (T) ASM(true, true, true); else ASM(true, false, true);
+        } else {
+            if (T) ASM(false, true, true); else ASM(false, false, true);
+        }
+    } else if (ctx->listValid) {
+        if (T) ASM(true, true, false); else ASM(true, false, false);
     } else {
-        if (T) ASM(false, true); else ASM(false, false);
+        if (T) ASM(false, true, false); else ASM(false, false, false);
     }
 #undef ASM
     MPS_HIP(hipGetLastError());
+    ctx->r0Fresh = res;
     return MPS_OK;
 }
 
@@ -945,7 +990,9 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
     double prevRn2 = 0.0;
     const double bytes32 = 6.0 * ctx->hostInts[I_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 16.0 * ctx->nLiveHost;
     for (;;) {
-        hipLaunchKernelGGL(kTrueResidual, dim3(nb), dim3(kB), 0, s, a, c);
+        // the first residual may come from the assembly (r and its partials)
+        if (!(outer == 0 && ctx->r0Fresh)) hipLaunchKernelGGL(kTrueResidual, dim3(nb), dim3(kB), 0, s, a, c);
+        ctx->r0Fresh = false;
         hipLaunchKernelGGL(kTrueResidualFinish, dim3(1), dim3(1024), 0, s, a, nb);
         if ((rc = readStatus(ctx, nullptr, 0))) return rc;
         const double rn2 = ctx->hostScalars[S_TRUE_RN2], bn2 = ctx->hostScalars[S_BN2];
@@ -1029,6 +1076,7 @@ __global__ void kPpeExtrapolate(KArgs a, int order) {
 int ppeWarmStart(Ctx* ctx) {
     const int want = ctx->P.ppe_warm_start;
     if (want <= 0) return MPS_OK;
+    ctx->r0Fresh = false;   // the guess changes after any assembly
     // solutions available before this solve: ppeSolves (the first guess is not one)
     const int order = std::min(want, std::max(ctx->ppeSolves - 1, 0));
     KArgs a = ctx->args();
@@ -1079,6 +1127,7 @@ int ppeSolve(Ctx* ctx) {
     const double* hs = ctx->hostScalars;
     ctx->trueResidual = hs[S_BN2] > 0 ? sqrt(hs[S_TRUE_RN2] / hs[S_BN2]) : 0.0;
     ctx->ppeSolves++;
+    ctx->r0Fresh = false;
     return MPS_OK;
 }
 
