@@ -206,6 +206,7 @@ struct Ctx {
     int lastCgIters = 0;         // iterations of the last FP64 CG solve / first mixed inner pass
     int lastInnerIters = 0;
     bool r0Fresh = false;        // cgR and the reduction partials hold b - A x of the current guess (assembly)
+    bool r0FreshMixed = false;   // ... in the mixed solve's partial order (else the FP64 CG's, with 1/diag)
     double solverErr = 0.0;
     double trueResidual = 0.0;
     double ppeMs = 0.0;

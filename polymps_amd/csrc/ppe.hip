@@ -30,11 +30,13 @@ constexpr int kB = 256;
 template <bool LIST, bool TRANS, bool RES>
 __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, unsigned short* col, int* esc,
                                             int* cnt, double* diag, double* b, float* val32, float* diag32,
-                                            float* invd32, double* x, double* r, double& rn2, double& bn2) {
+                                            float* invd32, double* invd64, double* x, double* r, double& rn2,
+                                            double& bn2) {
     if (!live(a, i)) {
         cnt[i] = 0; diag[i] = 1.0; b[i] = 0.0;
         if (invd32) { invd32[i] = 1.0f; diag32[i] = 1.0f; }
         if (RES) { x[i] = 0.0; r[i] = 0.0; }
+        if (invd64) invd64[i] = 1.0;
         return;
     }
     const mps_params& P = a.P;
@@ -42,6 +44,7 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
     if (bci == P.other || bci == P.surface) {
         cnt[i] = 0; diag[i] = 1.0; b[i] = 0.0;
         if (invd32) { invd32[i] = 1.0f; diag32[i] = 1.0f; }
+        if (invd64) invd64[i] = 1.0;
         if (RES) {
             const double ri = 0.0 - 1.0 * x[i];
             r[i] = ri;
@@ -157,6 +160,7 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
         invd32[i] = (sum != 0.0) ? (float)(1.0 / sum) : 1.0f;
         diag32[i] = (float)sum;
     }
+    if (invd64) invd64[i] = (sum != 0.0) ? 1.0 / sum : 1.0;
     if (RES) {
         const double ri = rhs - (sum * x[i] + ax);
         r[i] = ri;
@@ -169,7 +173,8 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
 template <bool LIST, bool TRANS, bool RES>
 __global__ __launch_bounds__(256) void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc, int* cnt,
                                                     double* diag, double* b, float* val32, float* diag32,
-                                                    float* invd32, double* x, double* r);
+                                                    float* invd32, double* invd64, double* x, double* r,
+                                                    int bFirst);
 
 // ------------------------------------------------------------ reductions
 // Every vector kernel runs one thread per row; each block leaves a pair of
@@ -196,12 +201,18 @@ __device__ __forceinline__ void writePartials(const KArgs& a, double s0, double 
 template <bool LIST, bool TRANS, bool RES>
 __global__ __launch_bounds__(256) void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc, int* cnt,
                                                     double* diag, double* b, float* val32, float* diag32,
-                                                    float* invd32, double* x, double* r) {
+                                                    float* invd32, double* invd64, double* x, double* r,
+                                                    int bFirst) {
     const int i = slotId();
     double rn2 = 0.0, bn2 = 0.0;
     if (i < a.N)
-        assembleRow<LIST, TRANS, RES>(a, i, val, col, esc, cnt, diag, b, val32, diag32, invd32, x, r, rn2, bn2);
-    if (RES) writePartials(a, rn2, bn2);   // the partials kTrueResidualFinish sums
+        assembleRow<LIST, TRANS, RES>(a, i, val, col, esc, cnt, diag, b, val32, diag32, invd32, invd64, x, r, rn2,
+                                      bn2);
+    // the partials kTrueResidualFinish (|r|^2, |b|^2) or kCgStartFinish (|b|^2, |r|^2) sums
+    if (RES) {
+        if (bFirst) writePartials(a, bn2, rn2);
+        else writePartials(a, rn2, bn2);
+    }
 }
 // three sums, partials at red[3 * block]
 __device__ __forceinline__ void writePartials3(const KArgs& a, double s0, double s1, double s2) {
@@ -656,13 +667,15 @@ int ppeAssemble(Ctx* ctx) {
     float* d32 = mixed ? ctx->aDiag32 : nullptr;
     float* i32 = mixed ? ctx->aDiagInv32 : nullptr;
     MPS_HIP(hipMemsetAsync(ctx->is + I_NNZ, 0, 2 * sizeof(int), ctx->stream));   // I_NNZ, I_NESC
-    // the mixed solve's first true residual comes out of the assembly (the
+    // the CG solve's first true residual comes out of the assembly (the
     // learned warm start changes the guess after the assembly, so not then)
-    const bool res = mixed && ctx->P.learned_mode == 0;
+    const bool res = !bicg && ctx->P.learned_mode == 0;
     double* x = ctx->D[DPPEX];
+    double* invd64 = (res && !mixed) ? ctx->cgInvD : nullptr;
+    const int bFirst = mixed ? 0 : 1;
 #define ASM(L, TT, R)                                                                                          \
     hipLaunchKernelGGL((kPpeAssemble<L, TT, R>), dim3(g), dim3(kB), 0, ctx->stream, a, ctx->aVal, ctx->aCol,   \
-                       ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->D[DPPEB], v32, d32, i32, x, ctx->cgR)
+                       ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->D[DPPEB], v32, d32, i32, invd64, x, ctx->cgR, bFirst)
     if (res) {
         if (ctx->listValid) {
             if (T) ASM(true, true, true); else ASM(true, false, true);
@@ -677,6 +690,7 @@ int ppeAssemble(Ctx* ctx) {
 #undef ASM
     MPS_HIP(hipGetLastError());
     ctx->r0Fresh = res;
+    ctx->r0FreshMixed = mixed;
     return MPS_OK;
 }
 
@@ -945,7 +959,9 @@ static int solveFp64(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int batc
     hipStream_t s = ctx->stream;
     const double tol = ctx->P.solver_tol > 0 ? ctx->P.solver_tol : DBL_EPSILON;
     const int maxIt = ctx->P.solver_max_iter > 0 ? ctx->P.solver_max_iter : 2 * ctx->nLiveHost;
-    hipLaunchKernelGGL(kCgStart, dim3(nb), dim3(kB), 0, s, a, c);
+    // r = b - A x, 1 / diag and the partials may come from the assembly
+    if (!(ctx->r0Fresh && !ctx->r0FreshMixed)) hipLaunchKernelGGL(kCgStart, dim3(nb), dim3(kB), 0, s, a, c);
+    ctx->r0Fresh = false;
     hipLaunchKernelGGL(kCgStartFinish, dim3(1), dim3(1024), 0, s, a, nb, tol);
     hipLaunchKernelGGL(kCgZeroX, dim3(nb), dim3(kB), 0, s, a, c.x);
     hipLaunchKernelGGL(kCgInitP, dim3(nb), dim3(kB), 0, s, a, c);
@@ -991,7 +1007,8 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
     const double bytes32 = 6.0 * ctx->hostInts[I_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 16.0 * ctx->nLiveHost;
     for (;;) {
         // the first residual may come from the assembly (r and its partials)
-        if (!(outer == 0 && ctx->r0Fresh)) hipLaunchKernelGGL(kTrueResidual, dim3(nb), dim3(kB), 0, s, a, c);
+        if (!(outer == 0 && ctx->r0Fresh && ctx->r0FreshMixed))
+            hipLaunchKernelGGL(kTrueResidual, dim3(nb), dim3(kB), 0, s, a, c);
         ctx->r0Fresh = false;
         hipLaunchKernelGGL(kTrueResidualFinish, dim3(1), dim3(1024), 0, s, a, nb);
         if ((rc = readStatus(ctx, nullptr, 0))) return rc;
