@@ -167,7 +167,7 @@ def main(argv=None):
     ap.add_argument("--inner-tol", type=float, default=1e-5,
                     help="relative tolerance of the inner FP32 solve of the mixed PPE (with the extrapolated "
                          "initial guess one inner solve to 1e-5 usually reaches the FP64 target)")
-    ap.add_argument("--warm-start", type=int, choices=[0, 1, 2], default=2,
+    ap.add_argument("--warm-start", type=int, choices=[0, 1, 2, 3], default=2,
                     help="initial guess of each PPE solve: 0 previous solution, 1 linear, 2 quadratic extrapolation")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",

@@ -83,7 +83,9 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
  * (1 when the last step rebuilt its list after the prediction)
  * "list_patched" (particles whose list the last step rebuilt alone).
  * Settable: "vel_max" "time" "list_skin" (< 0 automatic, 0 off, > 0 a fixed
- * skin in l0; every choice gives the same results)
+ * skin in l0; every choice gives the same results) "mask_words" (candidate
+ * window per stencil row in 32-slot words, at most the allocated width;
+ * particles with a longer row are listed by a direct walk, same results)
  * "profile" (nonzero: time every SpMV launch with HIP events, counters reset)
  * "precision_mode" (MPS_PREC_*, used from the next PPE assembly)
  * "learned_mode" (0 off, 1 learned warm start, 2 warm start + learned

@@ -112,8 +112,8 @@ typedef struct mps_params {
     double mixed_inner_tol;     /* inner FP32 PCG tolerance of the refinement loop */
     int learned_mode;           /* 0 off, 1 warm start, 2 warm start + preconditioner */
     int ppe_warm_start;         /* initial guess of each pressure solve: 0 the previous solution
-                                   (the reference), 1 / 2 linear / quadratic extrapolation of the
-                                   last 2 / 3 solutions */
+                                   (the reference), 1 / 2 / 3 linear / quadratic / cubic
+                                   extrapolation of the last 2 / 3 / 4 solutions */
     /* forced mesh motion (the reference hard-codes it in updateForcedPolygonMesh,
        src/PolygonMesh.cpp:173-374; the defaults are its active law): 1 rotation
        about the y axis through (center_x, center_z), angle theta_max sin(2 pi t / period)

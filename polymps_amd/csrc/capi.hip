@@ -405,6 +405,7 @@ int mps_create(mps_ctx** out, const mps_params* p, int n, int device) {
     // compression (the build reports an overflow as MPS_E_CAPACITY)
     const double perRow = 3.0 * std::pow(p->bucket_side / p->part_dist, p->dim);
     ctx->maskW = std::max(1, (int)std::ceil(1.6 * perRow / 32.0));
+    ctx->maskWAlloc = ctx->maskW;
     const int rows = ((int)p->dim == 3) ? 9 : 3;
     MPS_HIP(hipMalloc(&ctx->nbrMask, sizeof(unsigned) * (size_t)rows * ctx->maskW * n));
     MPS_HIP(hipMalloc(&ctx->nbr, sizeof(int) * (size_t)ctx->K * n));
@@ -461,7 +462,7 @@ int mps_destroy(mps_ctx* ctx) {
                     ctx->aVal, ctx->aCol, ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->aVal32, ctx->aDiagInv32,
                     ctx->aDiag32, ctx->cgR, ctx->cgP,
                     ctx->cgZ, ctx->cgQ, ctx->cgInvD, ctx->cgX, ctx->fR, ctx->fP, ctx->fZ, ctx->fQ, ctx->fX,
-                    ctx->bcgR0, ctx->bcgY, ctx->bcgS, ctx->bcgT, ctx->patchList, ctx->V4, ctx->nbrMaskS, ctx->nbrS, ctx->nbrCntS,
+                    ctx->bcgR0, ctx->bcgY, ctx->bcgS, ctx->bcgT, ctx->patchList, ctx->fixList, ctx->V4, ctx->nbrMaskS, ctx->nbrS, ctx->nbrCntS,
                     ctx->ds, ctx->is, ctx->red};
     for (void* b : bufs)
         if (b) hipFree(b);
@@ -672,13 +673,23 @@ int mps_set_scalar(mps_ctx* ctx, const char* name, double value) {
             return MPS_E_ARG;
         }
         ctx->P.learned_mode = (int)value;
+    } else if (s == "mask_words") {
+        // candidate window of a stencil row in 32-slot words (at most the
+        // allocated width); rows beyond it are listed by a direct walk
+        const int w = (int)value;
+        if (w < 1 || w > ctx->maskWAlloc || (double)w != value) {
+            ctx->err = "mask_words must be an integer in [1, allocated width]";
+            return MPS_E_ARG;
+        }
+        ctx->maskW = w;
+        ctx->listValid = false;
     } else if (s == "list_skin") {
         // neighbour list kept over the step: skin in units of l0, < 0 automatic, 0 off
         ctx->listSkin = value;
     } else if (s == "ppe_warm_start") {
         // initial guess of each pressure solve: 0 previous solution, 1 linear, 2 quadratic extrapolation
-        if (value != 0.0 && value != 1.0 && value != 2.0) {
-            ctx->err = "ppe_warm_start must be 0, 1 or 2";
+        if (value != 0.0 && value != 1.0 && value != 2.0 && value != 3.0) {
+            ctx->err = "ppe_warm_start must be 0, 1, 2 or 3";
             return MPS_E_ARG;
         }
         ctx->P.ppe_warm_start = (int)value;

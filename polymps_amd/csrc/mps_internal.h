@@ -25,7 +25,7 @@ enum DField {
     // (pndki, div_u, diffusive and grad_conc are written for a subset of the
     // particles in some modes, so they are carried with their particle)
     DX, DY, DZ, DU, DV, DW, DMX, DMY, DMZ, DPRESS, DPNDI, DPPEX, DPAVG, DFWX, DFWY, DFWZ, DPNDSKI,
-    DPNDKI, DDIVU, DDIFF, DGCX, DGCY, DGCZ, DPPEX1, DPPEX2,   // the two solutions before ppe_x
+    DPNDKI, DDIVU, DDIFF, DGCX, DGCY, DGCZ, DPPEX1, DPPEX2, DPPEX3,   // the three solutions before ppe_x
     NPD,
     // per step
     DAX = NPD, DAY, DAZ, DASX, DASY, DASZ, DNPX, DNPY, DNPZ, DNPCD2, DPNDSMALL, DCONC,
@@ -41,7 +41,7 @@ enum DScalar { S_VELMAX, S_ABSNEW, S_ABSOLD, S_RHSN2, S_THR, S_RN2, S_ALPHA, S_B
                S_RELTOL, S_RHO, S_RHOOLD, S_BW, S_R0N2, NDS };
 // device scalar slots (int)
 enum IScalar { I_NLIVE, I_OUTCNT, I_OVERFLOW, I_CG_ITER, I_CG_DONE, I_COMPACT_OVF, I_LIST_MAX, I_NNZ, I_NESC,
-               I_BCG_RESTART, I_BCG_RESTARTS, I_REBUILD, I_PATCHCNT, NIS };
+               I_BCG_RESTART, I_BCG_RESTARTS, I_REBUILD, I_PATCHCNT, I_FIXCNT, NIS };
 
 constexpr int kWallTable = 50;
 // escaped (non-delta) columns per PPE row: the first entry and stencil-row
@@ -130,9 +130,11 @@ struct Ctx {
     int* scanTmp = nullptr;
     int* outList = nullptr;     // compaction candidates
     int* patchList = nullptr;   // N: particles whose kept list is rebuilt alone (allocated on first use)
+    int* fixList = nullptr;     // N: particles whose candidate rows overflow the bitmask windows
     // neighbour list
     unsigned* nbrMask = nullptr;
     int maskW = 0;
+    int maskWAlloc = 0;         // words per stencil row the masks were allocated with
     int* nbr = nullptr;
     int* nbrCnt = nullptr;
     unsigned* nbrMaskS = nullptr;   // small-radius bitmasks, list and counts

@@ -27,7 +27,7 @@ constexpr int kWave = 64;
 // A second mask, maskS, marks the candidates inside the small radius r2s
 // (<= r2) for the small-radius list.
 __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int* cnt, unsigned* maskS, int* cntS,
-                                                    double r2, double r2s, const int* gate) {
+                                                    double r2, double r2s, const int* gate, int* fixList) {
     if (gate != nullptr && *gate == 0) return;   // conditional rebuild not needed
     __shared__ double4 stage[kWave];
     const int i = blockIdx.x * kWave + threadIdx.x;
@@ -93,7 +93,9 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
             }
         });
     if (!act) return;
-    if (over > 0) atomicMax(&a.is[I_OVERFLOW], over);
+    // a stencil row with more candidates than the mask words hold (dense
+    // clusters): the particle's lists are rebuilt by kFixList's direct walk
+    if (over > 0) fixList[atomicAdd(&a.is[I_FIXCNT], 1)] = i;
     if (total > a.K) {
         atomicMax(&a.is[I_OVERFLOW], total);
         total = a.K;
@@ -105,6 +107,35 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
     atomicMax(&a.is[I_LIST_MAX], total);
     cnt[i] = total;
     cntS[i] = totalS;
+}
+
+// Lists of the particles whose candidate rows overflowed the mask windows,
+// from a per-particle walk in walk order (no window limit), both radii.
+__global__ __launch_bounds__(256) void kFixList(KArgs a, int* nbr, int* cnt, int* nbrS, int* cntS,
+                                                const int* fixList, double r2, double r2s, const int* gate) {
+    if (gate != nullptr && *gate == 0) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.is[I_FIXCNT]) return;
+    const int i = fixList[t];
+    const double xi = PX(i), yi = PY(i), zi = PZ(i);
+    const size_t N = a.N;
+    int c = 0, cs = 0;
+    walkCellsPos(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+        const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        if (!(d2 < r2)) return;
+        if (c < a.K) nbr[(size_t)c * N + i] = j;
+        c++;
+        if (d2 < r2s) {
+            if (cs < a.KS) nbrS[(size_t)cs * N + i] = j;
+            cs++;
+        }
+    });
+    if (c > a.K) { atomicMax(&a.is[I_OVERFLOW], c); c = a.K; }
+    if (cs > a.KS) { atomicMax(&a.is[I_OVERFLOW], cs); cs = a.KS; }
+    atomicMax(&a.is[I_LIST_MAX], c);
+    cnt[i] = c;
+    cntS[i] = cs;
 }
 
 // Periodic domains: the list is built by a per-particle walk through the
@@ -1877,8 +1908,10 @@ int launchBuildList(Ctx* ctx, double skin, const int* gate) {
         return MPS_OK;
     }
     const int gw = gridFor(ctx->N, 64);
+    if (!ctx->fixList) MPS_HIP(hipMalloc(&ctx->fixList, sizeof(int) * (size_t)ctx->N));
+    MPS_HIP(hipMemsetAsync(ctx->is + I_FIXCNT, 0, sizeof(int), ctx->stream));
     hipLaunchKernelGGL(kBuildList, dim3(gridFor(ctx->N, kWave)), dim3(kWave), 0, ctx->stream, a, ctx->nbrMask,
-                       ctx->nbrCnt, ctx->nbrMaskS, ctx->nbrCntS, r2, r2s, gate);
+                       ctx->nbrCnt, ctx->nbrMaskS, ctx->nbrCntS, r2, r2s, gate, ctx->fixList);
 #define M2L(WM)                                                                                              \
     do {                                                                                                     \
         hipLaunchKernelGGL(kMaskToList<WM>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMask, ctx->nbrCnt,  \
@@ -1894,6 +1927,8 @@ int launchBuildList(Ctx* ctx, double skin, const int* gate) {
         return MPS_E_UNSUPPORTED;
     }
 #undef M2L
+    hipLaunchKernelGGL(kFixList, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbr, ctx->nbrCnt, ctx->nbrS,
+                       ctx->nbrCntS, ctx->fixList, r2, r2s, gate);
     MPS_HIP(hipGetLastError());
     ctx->listValid = true;
     return MPS_OK;

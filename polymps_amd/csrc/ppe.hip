@@ -1081,10 +1081,12 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
 __global__ void kPpeExtrapolate(KArgs a, int order) {
     const int i = slotId();
     if (i >= a.N) return;
-    const double x = a.D[DPPEX][i], x1 = a.D[DPPEX1][i], x2 = a.D[DPPEX2][i];
+    const double x = a.D[DPPEX][i], x1 = a.D[DPPEX1][i], x2 = a.D[DPPEX2][i], x3 = a.D[DPPEX3][i];
     double g = x;
     if (order == 1) g = 2.0 * x - x1;
     else if (order == 2) g = 3.0 * x - 3.0 * x1 + x2;
+    else if (order == 3) g = 4.0 * x - 6.0 * x1 + 4.0 * x2 - x3;
+    a.D[DPPEX3][i] = x2;
     a.D[DPPEX2][i] = x1;
     a.D[DPPEX1][i] = x;
     a.D[DPPEX][i] = g;
@@ -1097,7 +1099,7 @@ int ppeWarmStart(Ctx* ctx) {
     // solutions available before this solve: ppeSolves (the first guess is not one)
     const int order = std::min(want, std::max(ctx->ppeSolves - 1, 0));
     KArgs a = ctx->args();
-    hipLaunchKernelGGL(kPpeExtrapolate, dim3(gridFor(ctx->N, kB)), dim3(kB), 0, ctx->stream, a, std::min(order, 2));
+    hipLaunchKernelGGL(kPpeExtrapolate, dim3(gridFor(ctx->N, kB)), dim3(kB), 0, ctx->stream, a, std::min(order, 3));
     MPS_HIP(hipGetLastError());
     return MPS_OK;
 }
