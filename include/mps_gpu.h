@@ -90,8 +90,8 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
  * "precision_mode" (MPS_PREC_*, used from the next PPE assembly)
  * "learned_mode" (0 off, 1 learned warm start, 2 warm start + learned
  * flexible-CG preconditioner) "ppe_warm_start" (initial guess of each
- * pressure solve: 0 previous solution, 1 / 2 linear / quadratic extrapolation
- * of the last solutions) "mesh_bvh" (closest-point search: -1 BVH for
+ * pressure solve: 0 previous solution, 1 / 2 / 3 linear / quadratic / cubic
+ * extrapolation of the last solutions) "mesh_bvh" (closest-point search: -1 BVH for
  * meshes above 64 faces, 0 brute force, 1 BVH; both give the same result). */
 int mps_get_scalar(mps_ctx* ctx, const char* name, double* out);
 int mps_set_scalar(mps_ctx* ctx, const char* name, double value);
