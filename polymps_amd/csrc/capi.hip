@@ -281,8 +281,9 @@ static double keptListSkin(const Ctx* ctx) {
     if (ctx->listSkin == 0.0 || P.domain_type_bc == 1 || P.collision_type != MPS_COLLISION_PC) return 0.0;
     if (ctx->listSkin > 0.0) return ctx->listSkin * P.part_dist;
     const double g = std::sqrt(P.gravity[0] * P.gravity[0] + P.gravity[1] * P.gravity[1] + P.gravity[2] * P.gravity[2]);
-    const double skin = 3.0 * P.time_step * (ctx->velMaxHost + P.time_step * g) + 0.002 * P.part_dist;
-    return skin > 0.25 * P.part_dist ? 0.0 : skin;
+    const double skin =
+        ctx->listSkinFactor * P.time_step * (ctx->velMaxHost + P.time_step * g) + 0.002 * P.part_dist;
+    return skin > ctx->listSkinCap * P.part_dist ? 0.0 : skin;
 }
 
 static int runStep(Ctx* ctx) {
@@ -631,6 +632,8 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "fallback") *out = ctx->fallbackUsed;
     else if (s == "list_max") *out = ctx->hostInts[I_LIST_MAX];
     else if (s == "list_skin") *out = ctx->lastSkin / ctx->P.part_dist;
+    else if (s == "list_skin_cap") *out = ctx->listSkinCap;
+    else if (s == "list_skin_factor") *out = ctx->listSkinFactor;
     else if (s == "list_rebuilt") *out = ctx->lastSkin > 0.0 ? ctx->hostInts[I_REBUILD] : 1;
     else if (s == "list_patched") *out = ctx->lastSkin > 0.0 ? ctx->hostInts[I_PATCHCNT] : 0;
     else if (s == "list_capacity") *out = 32 * ctx->maskW;
@@ -686,6 +689,21 @@ int mps_set_scalar(mps_ctx* ctx, const char* name, double value) {
     } else if (s == "list_skin") {
         // neighbour list kept over the step: skin in units of l0, < 0 automatic, 0 off
         ctx->listSkin = value;
+    } else if (s == "list_skin_cap") {
+        // largest automatic skin, in units of l0
+        if (!(value >= 0.0 && value <= 1.0)) {
+            ctx->err = "list_skin_cap must be in [0, 1]";
+            return MPS_E_ARG;
+        }
+        ctx->listSkinCap = value;
+    } else if (s == "list_skin_factor") {
+        // automatic skin: multiple of the largest displacement of one step (at least 2: a pair
+        // closes in by up to twice one particle's displacement)
+        if (!(value >= 2.0 && value <= 10.0)) {
+            ctx->err = "list_skin_factor must be in [2, 10]";
+            return MPS_E_ARG;
+        }
+        ctx->listSkinFactor = value;
     } else if (s == "ppe_warm_start") {
         // initial guess of each pressure solve: 0 previous solution, 1 linear, 2 quadratic extrapolation
         if (value != 0.0 && value != 1.0 && value != 2.0 && value != 3.0) {

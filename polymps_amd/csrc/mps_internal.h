@@ -150,6 +150,11 @@ struct Ctx {
     // the last velocity maximum, 0 off) and the skin of the last step (m)
     double listSkin = -1.0;
     double lastSkin = 0.0;
+    // largest automatic skin (l0); beyond it the step builds its list after
+    // the prediction and the viscosity walks the buckets
+    double listSkinCap = 0.25;
+    // automatic skin: this many times the largest displacement of one step
+    double listSkinFactor = 3.0;
     // mesh
     double* mV = nullptr;
     int* mF = nullptr;

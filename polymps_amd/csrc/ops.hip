@@ -2012,6 +2012,56 @@ __global__ __launch_bounds__(256) void kPatchList(KArgs a, int* nbr, int* cnt, i
     cntS[i] = cs;
 }
 
+// Drops from a kept row the entries that lie outside the list radius at the
+// new positions, in place and in order (entry k is written only after entries
+// 0..k were read).  The result is the list kBuildList would build now: the
+// kept candidates of a particle that kept its stencil centre are the
+// candidates of that stencil, and kPatchList rows are already tight.  Without
+// it the consumers after the prediction walk the skin's extra entries, whose
+// radius test leaves the lanes of a wave at different entry counts, so the
+// PPE assembly's per-entry matrix stores no longer coalesce.
+__device__ __forceinline__ void tightenRow(const double4* P4, int* col, int* cnt, size_t N, int i, double xi,
+                                           double yi, double zi, double r2) {
+    const int n = cnt[i];
+    int k = 0, c = 0;
+    for (; c + 4 <= n; c += 4) {
+        int j[4];
+        double4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) j[u] = col[(size_t)(c + u) * N];
+#pragma unroll
+        for (int u = 0; u < 4; u++) q[u] = P4[j[u]];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double v0 = q[u].x - xi, v1 = q[u].y - yi, v2 = q[u].z - zi;
+            if (v0 * v0 + v1 * v1 + v2 * v2 < r2) {
+                col[(size_t)k * N] = j[u];
+                k++;
+            }
+        }
+    }
+    for (; c < n; c++) {
+        const int j = col[(size_t)c * N];
+        const double4 q = P4[j];
+        const double v0 = q.x - xi, v1 = q.y - yi, v2 = q.z - zi;
+        if (v0 * v0 + v1 * v1 + v2 * v2 < r2) {
+            col[(size_t)k * N] = j;
+            k++;
+        }
+    }
+    cnt[i] = k;
+}
+
+__global__ __launch_bounds__(256) void kListTighten(KArgs a, int* nbr, int* cnt, int* nbrS, int* cntS, double r2,
+                                                    double r2s) {
+    if (a.is[I_REBUILD]) return;   // the whole list was rebuilt at the radius
+    const int i = slotId();
+    if (i >= a.N || !live(a, i)) return;
+    const double4 qi = a.P4[i];
+    tightenRow(a.P4, nbr + i, cnt, a.N, i, qi.x, qi.y, qi.z, r2);
+    tightenRow(a.P4, nbrS + i, cntS, a.N, i, qi.x, qi.y, qi.z, r2s);
+}
+
 int launchListKeepStart(Ctx* ctx, double skin) {
     hipLaunchKernelGGL(kListFlagsReset, dim3(1), dim3(1), 0, ctx->stream, ctx->is);
     MPS_HIP(hipGetLastError());
@@ -2030,6 +2080,8 @@ int launchListKeepRefresh(Ctx* ctx, double skin) {
     if (rc) return rc;
     hipLaunchKernelGGL(kPatchList, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbr, ctx->nbrCnt, ctx->nbrS,
                        ctx->nbrCntS, ctx->patchList, listRadius2(ctx->P), listRadiusS2(ctx->P));
+    hipLaunchKernelGGL(kListTighten, dim3(g), dim3(kBlock), 0, ctx->stream, a, ctx->nbr, ctx->nbrCnt, ctx->nbrS,
+                       ctx->nbrCntS, listRadius2(ctx->P), listRadiusS2(ctx->P));
     MPS_HIP(hipGetLastError());
     ctx->listValid = true;
     return MPS_OK;

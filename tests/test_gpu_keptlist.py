@@ -111,3 +111,41 @@ def test_overflowing_candidate_rows_match(built):
             _assert_same(_state(wide), _state(narrow), f"step {s + 1}")
     wide.close()
     narrow.close()
+
+
+def test_skin_rule_settings_match_plain(built):
+    """The automatic skin's cap and factor only move work between the kept
+    list, its tightening after the prediction and the second build: a low cap
+    (most steps build after the prediction) and the smallest factor (the
+    refresh rebuilds whenever a particle outran half the skin) both give the
+    plain list's states."""
+    from polymps_amd.lib import MpsError
+    w = configs.small_dam3d(l0=0.0125, dt=5e-4)
+    plain = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh)
+    plain.set_scalar("list_skin", 0)
+    plain.call("init")
+    sims = []
+    for cap, factor in ((0.05, 3.0), (0.5, 2.0)):
+        s = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh)
+        s.set_scalar("list_skin_cap", cap)
+        s.set_scalar("list_skin_factor", factor)
+        assert s.scalar("list_skin_cap") == cap and s.scalar("list_skin_factor") == factor
+        s.call("init")
+        sims.append(s)
+    for bad in (("list_skin_cap", -0.1), ("list_skin_cap", 1.5), ("list_skin_factor", 1.0)):
+        with pytest.raises(MpsError):
+            sims[0].set_scalar(*bad)
+    kept = [0, 0]
+    for step in range(80):
+        plain.call("step")
+        for k, s in enumerate(sims):
+            s.call("step")
+            kept[k] += s.scalar("list_skin") > 0
+        if step % 20 == 19:
+            ref = _state(plain)
+            for s in sims:
+                _assert_same(ref, _state(s), f"step {step + 1}")
+    print(f"steps with a kept list: {kept}", flush=True)
+    assert kept[1] > kept[0]
+    for s in sims + [plain]:
+        s.close()
