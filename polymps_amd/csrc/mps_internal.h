@@ -152,7 +152,7 @@ struct Ctx {
     double lastSkin = 0.0;
     // largest automatic skin (l0); beyond it the step builds its list after
     // the prediction and the viscosity walks the buckets
-    double listSkinCap = 0.25;
+    double listSkinCap = 0.08;
     // automatic skin: this many times the largest displacement of one step
     double listSkinFactor = 3.0;
     // mesh

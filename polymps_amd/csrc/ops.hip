@@ -2035,7 +2035,7 @@ __device__ __forceinline__ void tightenRow(const double4* P4, int* col, int* cnt
         for (int u = 0; u < 4; u++) {
             const double v0 = q[u].x - xi, v1 = q[u].y - yi, v2 = q[u].z - zi;
             if (v0 * v0 + v1 * v1 + v2 * v2 < r2) {
-                col[(size_t)k * N] = j[u];
+                if (k != c + u) col[(size_t)k * N] = j[u];   // entries before the first drop stay put
                 k++;
             }
         }
@@ -2045,11 +2045,11 @@ __device__ __forceinline__ void tightenRow(const double4* P4, int* col, int* cnt
         const double4 q = P4[j];
         const double v0 = q.x - xi, v1 = q.y - yi, v2 = q.z - zi;
         if (v0 * v0 + v1 * v1 + v2 * v2 < r2) {
-            col[(size_t)k * N] = j;
+            if (k != c) col[(size_t)k * N] = j;
             k++;
         }
     }
-    cnt[i] = k;
+    if (k != n) cnt[i] = k;
 }
 
 __global__ __launch_bounds__(256) void kListTighten(KArgs a, int* nbr, int* cnt, int* nbrS, int* cntS, double r2,

@@ -35,6 +35,8 @@ def _pair(w, skin):
     kept = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh)
     plain.set_scalar("list_skin", 0)
     kept.set_scalar("list_skin", skin)
+    # an automatic skin up to 0.25 l0 keeps the list in every step of this flow
+    kept.set_scalar("list_skin_cap", 0.25)
     plain.call("init")
     kept.call("init")
     return plain, kept
