@@ -155,8 +155,8 @@ def ppe_compare(sim, solves: int, gnn_weights):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300,
-                    help="timed steps (300: the length of the 1.73M mixed-precision run)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the run length of the workload, 300 for dam3d, 200 for channel4m)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["dam3d", "channel4m"], default="dam3d",
                     help="dam3d: the 1.73M dam break (the metric's workload); channel4m: 4M particles over a "
@@ -186,6 +186,9 @@ def main(argv=None):
 
     from polymps_amd import configs
     from polymps_amd.lib import Simulation
+
+    if args.steps is None:
+        args.steps = 300 if args.workload == "dam3d" else 200
 
     if args.workload == "dam3d":
         args.l0 = args.l0 or 0.0025
