@@ -1123,7 +1123,7 @@ int mps_ppe_matrix(mps_ctx* ctx, int* row_ptr, int* cols, double* vals, int* nnz
         if (row_ptr) row_ptr[r] = p;
         for (int c = 0; c < cnt[k]; c++) {
             if (cols) cols[p] = canon[col[(size_t)c * n + k]];
-            if (vals) vals[p] = val[(size_t)c * n + k];
+            if (vals) vals[p] = val[ellIdx64(c, k, n)];
             p++;
         }
         if (cols) cols[p] = r;

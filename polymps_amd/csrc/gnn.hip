@@ -626,7 +626,7 @@ __global__ void kGnnResid(KArgs a, const double* val, const unsigned short* col,
             const size_t o = (size_t)c * N + i;
             const unsigned d = col[o];
             j = d ? j + (int)d : esc[(size_t)(e++) * N + i];
-            s += val[o] * (double)z[j];
+            s += val[ellIdx64(c, i, N)] * (double)z[j];
         }
         const double yi = s - (double)v[i];
         y[i] = yi;
@@ -653,7 +653,7 @@ __global__ void kGnnLossGrad(KArgs a, const double* val, const unsigned short* c
         const size_t o = (size_t)c * N + i;
         const unsigned d = col[o];
         j = d ? j + (int)d : esc[(size_t)(e++) * N + i];
-        s += val[o] * y[j];
+        s += val[ellIdx64(c, i, N)] * y[j];
     }
     gz[i] = (float)(2.0 * s / acc[1]);
 }
@@ -670,7 +670,7 @@ __global__ void kGnnResidualVec(KArgs a, const double* val, const unsigned short
         const size_t o = (size_t)c * N + i;
         const unsigned d = col[o];
         j = d ? j + (int)d : esc[(size_t)(e++) * N + i];
-        s += val[o] * x[j];
+        s += val[ellIdx64(c, i, N)] * x[j];
     }
     v[i] = (float)(b[i] - s);
 }

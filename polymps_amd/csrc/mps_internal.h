@@ -172,7 +172,7 @@ struct Ctx {
     std::vector<int> rigidF;
     MeshSlot extraMesh[2];                // meshType 1 (deformable) and 2 (forced)
     // PPE
-    double* aVal = nullptr;     // [K][N]
+    double* aVal = nullptr;     // [K / 2][N][2] (ellIdx64)
     unsigned short* aCol = nullptr;   // [K][N] column codes: slot delta from the previous entry, 0 = escape
     int* aEsc = nullptr;              // [kPpeEsc][N] absolute slots of the escaped entries
     int* aCnt = nullptr;        // N

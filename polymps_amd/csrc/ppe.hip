@@ -128,7 +128,7 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
             if (c < a.K && (delta || ne < kPpeEsc)) {
                 const size_t o = (size_t)c * a.N + i;
                 const double mv = (TRANS && d2T != d2) ? wgt(sqrt(d2T), P.re_l, P.weight_type) * P.coeff_ppe : mij;
-                val[o] = mv;
+                val[ellIdx64(c, i, a.N)] = mv;
                 if (val32) val32[o] = (float)mv;
                 if (RES) ax += mv * x[j];
                 if (delta) {
@@ -255,18 +255,24 @@ __device__ __forceinline__ void finishPartials(const double* red, int nb, double
     for (int q = 0; q < (int)(blockDim.x >> 6); q++) { s0 += sh[0][q]; s1 += sh[1][q]; }
 }
 
-// y_i = diag_i x_i + sum_c val[c][i] x[j_c]; the column of entry c is the
+// y_i = diag_i x_i + sum_c val(c, i) x[j_c]; the column of entry c is the
 // previous one plus code[c][i], or the next slot of the escape table when the
-// code is 0.  Eight entries are read per step so that eight value / code
-// loads and then eight gathers are in flight per thread; the value and code
-// streams are read once per launch and are loaded non-temporally so that they
-// do not displace the gathered x from the caches.
-template <typename T>
+// code is 0.  Eight entries are read per step so that their value and code
+// loads (FP64 values as four pair loads, ellIdx64) and then eight gathers are
+// in flight per thread; the value and code streams are read once per launch
+// and are loaded non-temporally so that they do not displace the gathered x
+// from the caches.
+template <class T>
+__device__ __forceinline__ size_t ellValIdx(int c, int i, size_t N) {
+    return sizeof(T) == 8 ? ellIdx64(c, i, N) : (size_t)c * N + i;
+}
+
+template <class T>
 __device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsigned short* __restrict__ code,
                                          const int* __restrict__ esc, int n, size_t N, const T* __restrict__ x, int i,
                                          double acc) {
+    typedef double D2 __attribute__((ext_vector_type(2)));
     const unsigned short* cd = code + i;
-    const T* vl = val + i;
     const int* es = esc + i;
     int j = 0, e = 0;
     auto next = [&](unsigned d) {
@@ -279,11 +285,21 @@ __device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsign
         const size_t o = (size_t)c * N;
         T v[8];
         unsigned d[8];
+        if (sizeof(T) == 8) {
+            // pair m of row i is element m * N + i of the pair view
+            const D2* vp = reinterpret_cast<const D2*>(val) + i + (size_t)(c >> 1) * N;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            v[k] = __builtin_nontemporal_load(vl + o + k * N);
-            d[k] = __builtin_nontemporal_load(cd + o + k * N);
+            for (int k = 0; k < 4; k++) {
+                const D2 w = __builtin_nontemporal_load(vp + k * N);
+                v[2 * k] = (T)w.x;
+                v[2 * k + 1] = (T)w.y;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = __builtin_nontemporal_load(val + i + o + k * N);
         }
+#pragma unroll
+        for (int k = 0; k < 8; k++) d[k] = __builtin_nontemporal_load(cd + o + k * N);
         int jj[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) jj[k] = next(d[k]);
@@ -302,9 +318,8 @@ __device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsign
         }
     }
     for (; c < n; c++) {
-        const size_t o = (size_t)c * N;
-        const int jc = next(cd[o]);
-        acc += (double)(vl[o] * x[jc]);
+        const int jc = next(cd[(size_t)c * N]);
+        acc += (double)(val[ellValIdx<T>(c, i, N)] * x[jc]);
     }
     return acc;
 }
