@@ -14,9 +14,10 @@ namespace mpsgpu {
 // FP64 value of PPE matrix (ELL) entry c of row i, K x N entries: entries 2m
 // and 2m + 1 of a row are adjacent, so one 16-byte load per lane fetches two
 // entries of its row and a wavefront of 64 rows reads one contiguous 1 KiB
-// block per entry pair.  (The same pairing of the FP32 values and the 16-bit
-// codes made the FP32 SpMV slower: 0.299 against 0.291 ms; they stay at
-// c * N + i.)
+// block per entry pair.  The FP32 values and the 16-bit codes stay at
+// c * N + i: paired they made the FP32 SpMV slower (0.299 against 0.291 ms),
+// in groups of four faster (0.288 ms) but the assembly's stores, scattered
+// over four times the span, slower by more.
 __host__ __device__ __forceinline__ size_t ellIdx64(int c, int i, size_t N) {
     return (((size_t)(c >> 1) * N + (size_t)i) << 1) | (size_t)(c & 1);
 }
