@@ -66,6 +66,8 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
     double sum = 0.0, rhs = 0.0;
     int c = 0, prev = -1, ne = 0;
     bool full = false;
+    typedef double D2 __attribute__((ext_vector_type(2)));
+    double pend = 0.0;   // FP64 value of the last even entry, stored with the next one
     const bool periodic = periodicDomain(P);
     // with the list, the inner and far-from-wall tests of j come with its
     // position (P4.w flags of update_bc) instead of two more gathers
@@ -128,7 +130,15 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
             if (c < a.K && (delta || ne < kPpeEsc)) {
                 const size_t o = (size_t)c * a.N + i;
                 const double mv = (TRANS && d2T != d2) ? wgt(sqrt(d2T), P.re_l, P.weight_type) * P.coeff_ppe : mij;
-                val[ellIdx64(c, i, a.N)] = mv;
+                // FP64 values leave in pairs (one 16-byte store per two entries)
+                if (c & 1) {
+                    D2 w;
+                    w.x = pend;
+                    w.y = mv;
+                    reinterpret_cast<D2*>(val)[(size_t)(c >> 1) * a.N + i] = w;
+                } else {
+                    pend = mv;
+                }
                 if (val32) val32[o] = (float)mv;
                 if (RES) ax += mv * x[j];
                 if (delta) {
@@ -145,6 +155,7 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
             }
         }
     });
+    if (c & 1) val[ellIdx64(c - 1, i, a.N)] = pend;
     if (full) atomicMax(&a.is[I_OVERFLOW], c + 1);   // reported as a capacity error after the step
     if (ne) atomicAdd(&a.is[I_NESC], ne);
     sum -= P.alpha_compressibility * rho / (P.time_step * P.time_step);
