@@ -1102,7 +1102,7 @@ int mps_ppe_matrix(mps_ctx* ctx, int* row_ptr, int* cols, double* vals, int* nnz
     for (int k = 0; k < n; k++) {   // decode the column codes
         int j = 0, e = 0;
         for (int c = 0; c < cnt[k]; c++) {
-            const unsigned d = code[(size_t)c * n + k];
+            const unsigned d = code[ellIdx4(c, k, n)];
             j = d ? j + (int)d : esc[(size_t)(e++) * n + k];
             col[(size_t)c * n + k] = j;
         }

@@ -623,8 +623,7 @@ __global__ void kGnnResid(KArgs a, const double* val, const unsigned short* col,
         double s = diag[i] * (double)z[i];
         int j = 0, e = 0;
         for (int c = 0; c < cnt[i]; c++) {
-            const size_t o = (size_t)c * N + i;
-            const unsigned d = col[o];
+            const unsigned d = col[ellIdx4(c, i, N)];
             j = d ? j + (int)d : esc[(size_t)(e++) * N + i];
             s += val[ellIdx64(c, i, N)] * (double)z[j];
         }
@@ -650,8 +649,7 @@ __global__ void kGnnLossGrad(KArgs a, const double* val, const unsigned short* c
     double s = diag[i] * y[i];
     int j = 0, e = 0;
     for (int c = 0; c < cnt[i]; c++) {
-        const size_t o = (size_t)c * N + i;
-        const unsigned d = col[o];
+        const unsigned d = col[ellIdx4(c, i, N)];
         j = d ? j + (int)d : esc[(size_t)(e++) * N + i];
         s += val[ellIdx64(c, i, N)] * y[j];
     }
@@ -667,8 +665,7 @@ __global__ void kGnnResidualVec(KArgs a, const double* val, const unsigned short
     double s = diag[i] * x[i];
     int j = 0, e = 0;
     for (int c = 0; c < cnt[i]; c++) {
-        const size_t o = (size_t)c * N + i;
-        const unsigned d = col[o];
+        const unsigned d = col[ellIdx4(c, i, N)];
         j = d ? j + (int)d : esc[(size_t)(e++) * N + i];
         s += val[ellIdx64(c, i, N)] * x[j];
     }

@@ -14,12 +14,15 @@ namespace mpsgpu {
 // FP64 value of PPE matrix (ELL) entry c of row i, K x N entries: entries 2m
 // and 2m + 1 of a row are adjacent, so one 16-byte load per lane fetches two
 // entries of its row and a wavefront of 64 rows reads one contiguous 1 KiB
-// block per entry pair.  The FP32 values and the 16-bit codes stay at
-// c * N + i: paired they made the FP32 SpMV slower (0.299 against 0.291 ms),
-// in groups of four faster (0.288 ms) but the assembly's stores, scattered
-// over four times the span, slower by more.
+// block per entry pair.
 __host__ __device__ __forceinline__ size_t ellIdx64(int c, int i, size_t N) {
     return (((size_t)(c >> 1) * N + (size_t)i) << 1) | (size_t)(c & 1);
+}
+// FP32 value and 16-bit column code of entry c of row i: groups of four
+// entries of a row are adjacent (one 16-byte / 8-byte load per lane per four
+// entries; the assembly stores whole groups).
+__host__ __device__ __forceinline__ size_t ellIdx4(int c, int i, size_t N) {
+    return (((size_t)(c >> 2) * N + (size_t)i) << 2) | (size_t)(c & 3);
 }
 
 #define KEPS (10.0 * 2.220446049250313e-16)

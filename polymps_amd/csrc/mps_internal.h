@@ -173,11 +173,11 @@ struct Ctx {
     MeshSlot extraMesh[2];                // meshType 1 (deformable) and 2 (forced)
     // PPE
     double* aVal = nullptr;     // [K / 2][N][2] (ellIdx64)
-    unsigned short* aCol = nullptr;   // [K][N] column codes: slot delta from the previous entry, 0 = escape
+    unsigned short* aCol = nullptr;   // [K / 4][N][4] (ellIdx4) column codes: slot delta from the previous entry, 0 = escape
     int* aEsc = nullptr;              // [kPpeEsc][N] absolute slots of the escaped entries
     int* aCnt = nullptr;        // N
     double* aDiag = nullptr;    // N
-    float* aVal32 = nullptr;    // [K][N] (mixed precision)
+    float* aVal32 = nullptr;    // [K / 4][N][4] (ellIdx4, mixed precision)
     float* aDiagInv32 = nullptr;
     float* aDiag32 = nullptr;
     double* cgR = nullptr;

@@ -68,6 +68,10 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
     bool full = false;
     typedef double D2 __attribute__((ext_vector_type(2)));
     double pend = 0.0;   // FP64 value of the last even entry, stored with the next one
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    typedef unsigned short U4 __attribute__((ext_vector_type(4)));
+    F4 g32 = {0.0f, 0.0f, 0.0f, 0.0f};   // the current group of four FP32 values / codes
+    U4 gcd = {0, 0, 0, 0};
     const bool periodic = periodicDomain(P);
     // with the list, the inner and far-from-wall tests of j come with its
     // position (P4.w flags of update_bc) instead of two more gathers
@@ -128,7 +132,6 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
             const int dj = j - prev;
             const bool delta = prev >= 0 && dj > 0 && dj < 65536;
             if (c < a.K && (delta || ne < kPpeEsc)) {
-                const size_t o = (size_t)c * a.N + i;
                 const double mv = (TRANS && d2T != d2) ? wgt(sqrt(d2T), P.re_l, P.weight_type) * P.coeff_ppe : mij;
                 // FP64 values leave in pairs (one 16-byte store per two entries)
                 if (c & 1) {
@@ -139,12 +142,22 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
                 } else {
                     pend = mv;
                 }
-                if (val32) val32[o] = (float)mv;
+                // FP32 values and codes leave in groups of four (ellIdx4)
+                const float f = (float)mv;
+                const unsigned short cd = delta ? (unsigned short)dj : (unsigned short)0;
+                switch (c & 3) {
+                    case 0: g32.x = f; gcd.x = cd; break;
+                    case 1: g32.y = f; gcd.y = cd; break;
+                    case 2: g32.z = f; gcd.z = cd; break;
+                    default: g32.w = f; gcd.w = cd; break;
+                }
+                if ((c & 3) == 3) {
+                    const size_t gi = (size_t)(c >> 2) * a.N + i;
+                    if (val32) reinterpret_cast<F4*>(val32)[gi] = g32;
+                    reinterpret_cast<U4*>(col)[gi] = gcd;
+                }
                 if (RES) ax += mv * x[j];
-                if (delta) {
-                    col[o] = (unsigned short)dj;
-                } else {
-                    col[o] = 0;
+                if (!delta) {
                     esc[(size_t)ne * a.N + i] = j;
                     ne++;
                 }
@@ -156,6 +169,11 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
         }
     });
     if (c & 1) val[ellIdx64(c - 1, i, a.N)] = pend;
+    if (c & 3) {   // the last, partial group (entries past the count are never read)
+        const size_t gi = (size_t)(c >> 2) * a.N + i;
+        if (val32) reinterpret_cast<F4*>(val32)[gi] = g32;
+        reinterpret_cast<U4*>(col)[gi] = gcd;
+    }
     if (full) atomicMax(&a.is[I_OVERFLOW], c + 1);   // reported as a capacity error after the step
     if (ne) atomicAdd(&a.is[I_NESC], ne);
     sum -= P.alpha_compressibility * rho / (P.time_step * P.time_step);
@@ -267,15 +285,15 @@ __device__ __forceinline__ void finishPartials(const double* red, int nb, double
 }
 
 // y_i = diag_i x_i + sum_c val(c, i) x[j_c]; the column of entry c is the
-// previous one plus code[c][i], or the next slot of the escape table when the
+// previous one plus code(c, i), or the next slot of the escape table when the
 // code is 0.  Eight entries are read per step so that their value and code
-// loads (FP64 values as four pair loads, ellIdx64) and then eight gathers are
-// in flight per thread; the value and code streams are read once per launch
+// loads (FP64 values as four pair loads, ellIdx64; codes and FP32 values as
+// two loads of four, ellIdx4) and then eight gathers are in flight per thread; the value and code streams are read once per launch
 // and are loaded non-temporally so that they do not displace the gathered x
 // from the caches.
 template <class T>
 __device__ __forceinline__ size_t ellValIdx(int c, int i, size_t N) {
-    return sizeof(T) == 8 ? ellIdx64(c, i, N) : (size_t)c * N + i;
+    return sizeof(T) == 8 ? ellIdx64(c, i, N) : ellIdx4(c, i, N);
 }
 
 template <class T>
@@ -283,7 +301,10 @@ __device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsign
                                          const int* __restrict__ esc, int n, size_t N, const T* __restrict__ x, int i,
                                          double acc) {
     typedef double D2 __attribute__((ext_vector_type(2)));
-    const unsigned short* cd = code + i;
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    typedef unsigned short U4 __attribute__((ext_vector_type(4)));
+    // groups of four codes / FP32 values: group g of row i is element g * N + i
+    const U4* cg = reinterpret_cast<const U4*>(code) + i;
     const int* es = esc + i;
     int j = 0, e = 0;
     auto next = [&](unsigned d) {
@@ -293,7 +314,6 @@ __device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsign
     };
     int c = 0;
     for (; c + 8 <= n; c += 8) {
-        const size_t o = (size_t)c * N;
         T v[8];
         unsigned d[8];
         if (sizeof(T) == 8) {
@@ -306,11 +326,24 @@ __device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsign
                 v[2 * k + 1] = (T)w.y;
             }
         } else {
+            const F4* vg = reinterpret_cast<const F4*>(val) + i + (size_t)(c >> 2) * N;
 #pragma unroll
-            for (int k = 0; k < 8; k++) v[k] = __builtin_nontemporal_load(val + i + o + k * N);
+            for (int k = 0; k < 2; k++) {
+                const F4 w = __builtin_nontemporal_load(vg + k * N);
+                v[4 * k] = (T)w.x;
+                v[4 * k + 1] = (T)w.y;
+                v[4 * k + 2] = (T)w.z;
+                v[4 * k + 3] = (T)w.w;
+            }
         }
 #pragma unroll
-        for (int k = 0; k < 8; k++) d[k] = __builtin_nontemporal_load(cd + o + k * N);
+        for (int k = 0; k < 2; k++) {
+            const U4 w = __builtin_nontemporal_load(cg + (size_t)(c >> 2) * N + k * N);
+            d[4 * k] = w.x;
+            d[4 * k + 1] = w.y;
+            d[4 * k + 2] = w.z;
+            d[4 * k + 3] = w.w;
+        }
         int jj[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) jj[k] = next(d[k]);
@@ -329,7 +362,7 @@ __device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsign
         }
     }
     for (; c < n; c++) {
-        const int jc = next(cd[(size_t)c * N]);
+        const int jc = next(code[ellIdx4(c, i, N)]);
         acc += (double)(val[ellValIdx<T>(c, i, N)] * x[jc]);
     }
     return acc;
