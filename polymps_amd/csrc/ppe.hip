@@ -286,9 +286,9 @@ __device__ __forceinline__ void finishPartials(const double* red, int nb, double
 
 // y_i = diag_i x_i + sum_c val(c, i) x[j_c]; the column of entry c is the
 // previous one plus code(c, i), or the next slot of the escape table when the
-// code is 0.  Eight entries are read per step so that their value and code
-// loads (FP64 values as four pair loads, ellIdx64; codes and FP32 values as
-// two loads of four, ellIdx4) and then eight gathers are in flight per thread; the value and code streams are read once per launch
+// code is 0.  Sixteen entries are read per step so that their value and code
+// loads (FP64 values as pair loads, ellIdx64; codes and FP32 values as loads
+// of four, ellIdx4) and then sixteen gathers are in flight per thread; the value and code streams are read once per launch
 // and are loaded non-temporally so that they do not displace the gathered x
 // from the caches.
 template <class T>
@@ -296,15 +296,72 @@ __device__ __forceinline__ size_t ellValIdx(int c, int i, size_t N) {
     return sizeof(T) == 8 ? ellIdx64(c, i, N) : ellIdx4(c, i, N);
 }
 
+// B consecutive entries c0 .. c0 + B - 1 of row i (c0 and B multiples of 4):
+// all value and code loads, then all gathers, then the products in entry order.
+template <class T, int B, class Next>
+__device__ __forceinline__ void ellBatch(const T* __restrict__ val, const unsigned short* __restrict__ code, size_t N,
+                                         const T* __restrict__ x, int i, int c0, Next&& next, double& acc) {
+    typedef double D2 __attribute__((ext_vector_type(2)));
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    typedef unsigned short U4 __attribute__((ext_vector_type(4)));
+    T v[B];
+    unsigned d[B];
+    if (sizeof(T) == 8) {
+        // pair m of row i is element m * N + i of the pair view
+        const D2* vp = reinterpret_cast<const D2*>(val) + i + (size_t)(c0 >> 1) * N;
+#pragma unroll
+        for (int k = 0; k < B / 2; k++) {
+            const D2 w = __builtin_nontemporal_load(vp + k * N);
+            v[2 * k] = (T)w.x;
+            v[2 * k + 1] = (T)w.y;
+        }
+    } else {
+        // group g of row i is element g * N + i of the group view
+        const F4* vg = reinterpret_cast<const F4*>(val) + i + (size_t)(c0 >> 2) * N;
+#pragma unroll
+        for (int k = 0; k < B / 4; k++) {
+            const F4 w = __builtin_nontemporal_load(vg + k * N);
+            v[4 * k] = (T)w.x;
+            v[4 * k + 1] = (T)w.y;
+            v[4 * k + 2] = (T)w.z;
+            v[4 * k + 3] = (T)w.w;
+        }
+    }
+    const U4* cg = reinterpret_cast<const U4*>(code) + i + (size_t)(c0 >> 2) * N;
+#pragma unroll
+    for (int k = 0; k < B / 4; k++) {
+        const U4 w = __builtin_nontemporal_load(cg + k * N);
+        d[4 * k] = w.x;
+        d[4 * k + 1] = w.y;
+        d[4 * k + 2] = w.z;
+        d[4 * k + 3] = w.w;
+    }
+    int jj[B];
+#pragma unroll
+    for (int k = 0; k < B; k++) jj[k] = next(d[k]);
+    T xv[B];
+#pragma unroll
+    for (int k = 0; k < B; k++) xv[k] = x[jj[k]];
+    if (sizeof(T) == 4) {
+        // FP32 (inner solve): each run of up to eight entries is summed in
+        // FP32, then added in FP64
+#pragma unroll
+        for (int h = 0; h < B; h += 8) {
+            T b8 = v[h] * xv[h];
+#pragma unroll
+            for (int k = h + 1; k < h + 8 && k < B; k++) b8 += v[k] * xv[k];
+            acc += (double)b8;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < B; k++) acc += (double)(v[k] * xv[k]);
+    }
+}
+
 template <class T>
 __device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsigned short* __restrict__ code,
                                          const int* __restrict__ esc, int n, size_t N, const T* __restrict__ x, int i,
                                          double acc) {
-    typedef double D2 __attribute__((ext_vector_type(2)));
-    typedef float F4 __attribute__((ext_vector_type(4)));
-    typedef unsigned short U4 __attribute__((ext_vector_type(4)));
-    // groups of four codes / FP32 values: group g of row i is element g * N + i
-    const U4* cg = reinterpret_cast<const U4*>(code) + i;
     const int* es = esc + i;
     int j = 0, e = 0;
     auto next = [&](unsigned d) {
@@ -313,53 +370,14 @@ __device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsign
         return j;
     };
     int c = 0;
-    for (; c + 8 <= n; c += 8) {
-        T v[8];
-        unsigned d[8];
-        if (sizeof(T) == 8) {
-            // pair m of row i is element m * N + i of the pair view
-            const D2* vp = reinterpret_cast<const D2*>(val) + i + (size_t)(c >> 1) * N;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const D2 w = __builtin_nontemporal_load(vp + k * N);
-                v[2 * k] = (T)w.x;
-                v[2 * k + 1] = (T)w.y;
-            }
-        } else {
-            const F4* vg = reinterpret_cast<const F4*>(val) + i + (size_t)(c >> 2) * N;
-#pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const F4 w = __builtin_nontemporal_load(vg + k * N);
-                v[4 * k] = (T)w.x;
-                v[4 * k + 1] = (T)w.y;
-                v[4 * k + 2] = (T)w.z;
-                v[4 * k + 3] = (T)w.w;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-            const U4 w = __builtin_nontemporal_load(cg + (size_t)(c >> 2) * N + k * N);
-            d[4 * k] = w.x;
-            d[4 * k + 1] = w.y;
-            d[4 * k + 2] = w.z;
-            d[4 * k + 3] = w.w;
-        }
-        int jj[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) jj[k] = next(d[k]);
-        T xv[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) xv[k] = x[jj[k]];
-        if (sizeof(T) == 4) {
-            // FP32 (inner solve): the batch is summed in FP32, then added in FP64
-            T b8 = v[0] * xv[0];
-#pragma unroll
-            for (int k = 1; k < 8; k++) b8 += v[k] * xv[k];
-            acc += (double)b8;
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; k++) acc += (double)(v[k] * xv[k]);
-        }
+    for (; c + 16 <= n; c += 16) ellBatch<T, 16>(val, code, N, x, i, c, next, acc);
+    if (c + 8 <= n) {
+        ellBatch<T, 8>(val, code, N, x, i, c, next, acc);
+        c += 8;
+    }
+    if (c + 4 <= n) {
+        ellBatch<T, 4>(val, code, N, x, i, c, next, acc);
+        c += 4;
     }
     for (; c < n; c++) {
         const int jc = next(code[ellIdx4(c, i, N)]);
