@@ -84,7 +84,7 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
  * "list_patched" (particles whose list the last step rebuilt alone).
  * Settable: "vel_max" "time" "list_skin" (< 0 automatic, 0 off, > 0 a fixed
  * skin in l0; every choice gives the same results) "list_skin_cap" (largest
- * automatic skin in l0, default 0.25; a step that needs more builds its list
+ * automatic skin in l0, default 0.08; a step that needs more builds its list
  * after the prediction) "list_skin_factor" (automatic skin as a multiple of
  * the largest one-step displacement, in [2, 10], default 3) "mask_words" (candidate
  * window per stencil row in 32-slot words, at most the allocated width;

@@ -270,10 +270,11 @@ static int forcedMeshUpdate(Ctx* ctx, double dt, double time);
 static void freeSlot(MeshSlot& m);
 
 // Skin (m) of a neighbour list kept over the step, 0 to build the list after
-// the prediction as usual.  Automatic: three times the largest displacement
-// the last velocity maximum and gravity give in one step, plus 0.002 l0; a
-// faster flow (skin above 0.25 l0) builds after the prediction, where the
-// larger list would cost its consumers more than the second build saves.
+// the prediction as usual.  Automatic: list_skin_factor (3) times the largest
+// displacement the last velocity maximum and gravity give in one step, plus
+// 0.002 l0; a faster flow (skin above list_skin_cap, 0.08 l0) builds after
+// the prediction, where the larger list and its tightening would cost more
+// than the second build saves.
 // The refresh checks every particle's actual displacement, so the choice only
 // affects speed.
 static double keptListSkin(const Ctx* ctx) {
