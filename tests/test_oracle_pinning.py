@@ -15,45 +15,10 @@ import pytest
 from polymps_amd import configs
 from polymps_amd import lib as L
 from oracle.oracle import Oracle
+from pin_util import interior, lattice_block
 
 
 # ----------------------------------------------------------------- helpers
-def lattice_block(nx, ny, nz, l0=0.01, re_s=2.1, re_l=3.1, dim=3, mps_type=2, jitter=0.0, seed=7):
-    """A bare block of fluid particles (particle-wall mode, no walls)."""
-    if dim == 3:
-        ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
-        pos = np.stack([ix.ravel() * l0, iy.ravel() * l0, iz.ravel() * l0], 1).astype(float)
-    else:
-        ix, iy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
-        pos = np.stack([ix.ravel() * l0, iy.ravel() * l0, np.zeros(ix.size)], 1).astype(float)
-    n = len(pos)
-    if jitter:
-        u = configs.splitmix64_uniform(seed, 3 * n).reshape(n, 3)
-        pos[:, :dim] += (2 * u[:, :dim] - 1) * jitter * l0
-    p = L.defaults()
-    p.wall_type = 0
-    p.dim = float(dim)
-    p.part_dist = l0
-    p.re_s, p.re_l = re_s, re_l
-    p.mps_type = mps_type
-    p.pnd_type = 0 if mps_type >= 2 else 2
-    p.time_step = 1e-4
-    p.gradient_correction = 0
-    p.shifting_type = 0
-    p.free_surf_type = 0
-    p.solver_tol = 1e-12
-    types = np.zeros(n, dtype=np.int32)
-    return p, types, pos.reshape(-1)
-
-
-def interior(pos, l0, margin, dim=3):
-    P = pos.reshape(-1, 3)
-    m = np.ones(len(P), bool)
-    for k in range(dim):
-        m &= (P[:, k] > P[:, k].min() + margin * l0 - 1e-12) & (P[:, k] < P[:, k].max() - margin * l0 + 1e-12)
-    return m
-
-
 def lattice_sums(re_s, re_l, dim):
     """n0S, n0L, lambda/l0^2, neighbour count on the unit lattice (exact integers)."""
     R = int(math.ceil(re_l)) + 1
