@@ -31,7 +31,14 @@ extern "C" {
 #define MPS_E_UNSUPPORTED (-3) /* configuration not supported   */
 #define MPS_E_IO (-4)       /* file could not be read / written */
 #define MPS_E_CAPACITY (-5) /* a fixed-size device table overflowed */
-#define MPS_E_SOLVER (-6)   /* the PPE solve did not converge   */
+#define MPS_E_SOLVER (-6)   /* the PPE solve did not converge: it stopped above its
+                             * tolerance (FP64 CG / BiCGSTAB: Eigen's rule, the
+                             * recursive relative residual; mixed: the FP64 true
+                             * residual) or met a non-finite residual.  The
+                             * pressure is written either way, as the reference
+                             * does after printing "Failed solving"
+                             * (src/MpsPressure.cpp:440-442); scalar
+                             * "solver_failed" tells the two cases apart. */
 
 typedef struct mps_ctx mps_ctx;
 
@@ -77,6 +84,9 @@ int mps_download(mps_ctx* ctx, const char* field, void* out, size_t count);
 int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t count);
 /* Scalars: "vel_max" "cfl" "time" "solver_iter" "solver_err" "num_particles"
  * "iterations" "ppe_ms" "true_residual" "fallback" "outer_iterations"
+ * "solver_failed" (last pressure solve: 0 converged, 1 stopped above the
+ * tolerance, 2 non-finite residual) "solver_failures" (failed solves since
+ * the count was last set; settable)
  * "list_max" "list_capacity" "capacity" "nnz" "spmv_ms_total"
  * "spmv_launches" "spmv_bytes_total" "list_skin" (skin of the last step's
  * kept neighbour list, in l0; 0: built after the prediction) "list_rebuilt"
@@ -135,6 +145,14 @@ int mps_get_wall_tables(mps_ctx* ctx, double* xpnd, double* xneigh, double* zij,
 /* MpsBucket::updateParticlesID (src/MpsBucket.cpp:299-353): counting sort of
  * particles into buckets, particles reordered on the device. */
 int mps_bucket_update(mps_ctx* ctx);
+/* The bucket lists of the last mps_bucket_update as the reference keeps them
+ * (firstParticleInBucket, lastParticleInBucket, nextParticleInSameBucket,
+ * include/MpsParticle.h:139-141, filled by src/MpsBucket.cpp:299-353 and, for
+ * a periodic domain, copyDataBetweenBuckets :356-417): reference particle
+ * indices, -1 for none; first / last have num_buckets_xyz entries, next the
+ * context's capacity.  Arrays may be NULL.  Returns num_buckets_xyz (>= 0) or
+ * an error code. */
+int mps_get_buckets(mps_ctx* ctx, int* first, int* last, int* next);
 /* MpsViscosity::calcViscosity (src/MpsViscosity.cpp:20-102) */
 int mps_viscosity(mps_ctx* ctx);
 /* MpsParticle::addGravity (src/MpsParticle.cpp:111-130) */
@@ -184,7 +202,10 @@ int mps_calc_press(mps_ctx* ctx);
 int mps_vel_divergence(mps_ctx* ctx);
 /* PPE assembly only (the first half of mps_calc_press) */
 int mps_ppe_assemble(mps_ctx* ctx);
-/* PPE solve only, from the current ppe_x guess (second half of mps_calc_press) */
+/* PPE solve only, from the current ppe_x guess (second half of mps_calc_press).
+ * MPS_E_ARG when no assembly matches the current slot order (a bucket update,
+ * compaction or state upload came after it) or the assembly was made for
+ * another solver_type / precision_mode; MPS_E_SOLVER as above. */
 int mps_ppe_solve(mps_ctx* ctx);
 /* The initial guess mps_calc_press gives the solve (no reference
  * counterpart): ppe_x becomes the extrapolation of order ppe_warm_start from
@@ -226,9 +247,15 @@ int mps_wall_dummy_velocity(mps_ctx* ctx);
  * :471-556): buckets, out-of-domain check, wall distance, initial PND, mean
  * PND, BC, first pressure. */
 int mps_init(mps_ctx* ctx);
-/* One time step of mainLoopOfSimulation (src/main.cpp:252-467). */
+/* One time step of mainLoopOfSimulation (src/main.cpp:252-467).  A pressure
+ * solve that stops above its tolerance lets the step finish, as the
+ * reference's loop goes on after "Failed solving", and the step then returns
+ * MPS_E_SOLVER; a non-finite residual returns at once. */
 int mps_step(mps_ctx* ctx);
-/* k steps without host synchronisation in between. */
+/* k steps without host synchronisation in between; solves that stop above
+ * their tolerance do not stop the k steps, MPS_E_SOLVER is returned after
+ * the last one (message: the first failing step); a non-finite residual
+ * returns at once. */
 int mps_steps(mps_ctx* ctx, int k);
 
 /* ------------------------------------------------------------------ I/O */

@@ -202,6 +202,12 @@ static int syncStatus(Ctx* ctx) {
     return MPS_OK;
 }
 
+// A pressure solve that stops above its tolerance still leaves a pressure
+// field, and the reference's step goes on after printing "Failed solving"
+// (src/MpsPressure.cpp:440-442): such a failure lets the step finish and is
+// returned at its end.  A non-finite residual ends the call at once.
+static bool softSolverFailure(const Ctx* ctx, int rc) { return rc == MPS_E_SOLVER && ctx->solverFailed == 1; }
+
 // calcNormalParticles, then the polygon-wall part (src/main.cpp order before
 // updateParticleBC when the ARC rule is used)
 static int arcNormals(Ctx* ctx) {
@@ -250,9 +256,11 @@ static int runInit(Ctx* ctx) {
         if (P.mps_type == MPS_IMPLICIT_PND_DIVU && (rc = velDivergence(ctx))) return rc;
         if ((rc = ppeWarmStart(ctx))) return rc;
         if ((rc = ppeAssemble(ctx))) return rc;
-        if ((rc = ppeSolve(ctx))) return rc;
+        rc = ppeSolve(ctx);
+        if (rc && !softSolverFailure(ctx, rc)) return rc;
     }
-    return syncStatus(ctx);
+    const int rs = syncStatus(ctx);
+    return rs ? rs : rc;
 }
 
 static int calcPress(Ctx* ctx) {
@@ -332,7 +340,8 @@ static int runStep(Ctx* ctx) {
     else if (P.pnd_type == MPS_PND_MEAN_SUM_WIJ) OP("mean_pnd");
     if (P.free_surf_type == MPS_FS_PND_ARC && (rc = arcNormals(ctx))) return rc;
     OP("update_bc");
-    if ((rc = calcPress(ctx))) return rc;
+    const int solveRc = calcPress(ctx);
+    if (solveRc && !softSolverFailure(ctx, solveRc)) return solveRc;
     if (P.wall_type == MPS_WALL_PARTICLE) OP("extrapolate_wall_dummy");
     if (P.gradient_correction) OP("correction_matrix");
     OP("press_gradient");
@@ -350,7 +359,8 @@ static int runStep(Ctx* ctx) {
 #undef OP
     ctx->numIter++;
     ctx->timeCurrent += P.time_step;
-    return syncStatus(ctx);
+    rc = syncStatus(ctx);
+    return rc ? rc : solveRc;
 }
 
 }  // namespace mpsgpu
@@ -432,6 +442,7 @@ int mps_create(mps_ctx** out, const mps_params* p, int n, int device) {
     MPS_HIP(hipMalloc(&ctx->is, sizeof(int) * NIS));
     ctx->redCap = std::max(kRedBlocks, gridFor(n, 256));
     MPS_HIP(hipMalloc(&ctx->red, sizeof(double) * (3 * ctx->redCap + 16)));
+    MPS_HIP(hipMalloc(&ctx->asmRed, sizeof(double) * (3 * ctx->redCap + 16)));
     MPS_HIP(hipMalloc(&ctx->wtab, sizeof(double) * 4 * kWallTable));
     MPS_HIP(hipHostMalloc(&ctx->hostScalars, sizeof(double) * NDS));
     MPS_HIP(hipHostMalloc(&ctx->hostInts, sizeof(int) * NIS));
@@ -465,7 +476,7 @@ int mps_destroy(mps_ctx* ctx) {
                     ctx->aDiag32, ctx->cgR, ctx->cgP,
                     ctx->cgZ, ctx->cgQ, ctx->cgInvD, ctx->cgX, ctx->fR, ctx->fP, ctx->fZ, ctx->fQ, ctx->fX,
                     ctx->bcgR0, ctx->bcgY, ctx->bcgS, ctx->bcgT, ctx->patchList, ctx->fixList, ctx->V4, ctx->nbrMaskS, ctx->nbrS, ctx->nbrCntS,
-                    ctx->ds, ctx->is, ctx->red};
+                    ctx->ds, ctx->is, ctx->red, ctx->asmRed};
     for (void* b : bufs)
         if (b) hipFree(b);
     freeSlot(ctx->extraMesh[0]);
@@ -523,6 +534,8 @@ int mps_upload_state(mps_ctx* ctx, const int* type, const double* pos, const dou
     ctx->listValid = false;
     ctx->numIter = 0;
     ctx->timeCurrent = 0.0;
+    ctx->asmMode = -1;
+    ctx->r0Fresh = false;
     return MPS_OK;
 }
 
@@ -631,6 +644,8 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "ppe_ms") *out = ctx->ppeMs;
     else if (s == "true_residual") *out = ctx->trueResidual;
     else if (s == "fallback") *out = ctx->fallbackUsed;
+    else if (s == "solver_failed") *out = ctx->solverFailed;
+    else if (s == "solver_failures") *out = (double)ctx->solverFailures;
     else if (s == "list_max") *out = ctx->hostInts[I_LIST_MAX];
     else if (s == "list_skin") *out = ctx->lastSkin / ctx->P.part_dist;
     else if (s == "list_skin_cap") *out = ctx->listSkinCap;
@@ -658,6 +673,8 @@ int mps_set_scalar(mps_ctx* ctx, const char* name, double value) {
         ctx->velMaxHost = value;
     } else if (s == "time") {
         ctx->timeCurrent = value;
+    } else if (s == "solver_failures") {
+        ctx->solverFailures = (long)value;
     } else if (s == "profile") {
         ctx->profile = value != 0.0;
         ctx->spmvMsTotal = 0.0;
@@ -1089,6 +1106,42 @@ int mps_ppe_warm_start(mps_ctx* ctx) {
     return syncStatus(ctx);
 }
 
+int mps_get_buckets(mps_ctx* ctx, int* first, int* last, int* next) {
+    CHECK_CTX(ctx);
+    MPS_HIP(hipSetDevice(ctx->dev));
+    const int n = ctx->N, nB = ctx->nB;
+    const mps_params& P = ctx->P;
+    std::vector<int> start(nB + 2), canon(n);
+    MPS_HIP(hipMemcpyAsync(start.data(), ctx->cellStart, sizeof(int) * (nB + 2), hipMemcpyDeviceToHost, ctx->stream));
+    MPS_HIP(hipMemcpyAsync(canon.data(), ctx->I[ICANON], sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+    MPS_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<int> f(nB, -1), l(nB, -1), nx(n, -1);
+    // bucket b holds slots start[b] .. start[b + 1] - 1 in ascending reference index
+    for (int b = 0; b < nB; b++) {
+        if (start[b + 1] <= start[b]) continue;
+        f[b] = canon[start[b]];
+        l[b] = canon[start[b + 1] - 1];
+        for (int k = start[b]; k + 1 < start[b + 1]; k++) nx[canon[k]] = canon[k + 1];
+    }
+    // MpsBucket::copyDataBetweenBuckets (src/MpsBucket.cpp:356-417): the ghost
+    // layers of the periodic direction list the opposite interior layer
+    if (P.domain_type_bc == 1) {
+        const int d = P.periodic_dir[0] ? 0 : P.periodic_dir[1] ? 1 : 2;
+        const int nd[3] = {P.num_buckets_x, P.num_buckets_y, (int)P.dim == 3 ? P.num_buckets_z : 1};
+        const int stride = d == 0 ? 1 : d == 1 ? P.num_buckets_x : P.num_buckets_xy;
+        const int off = stride * (nd[d] - 2);
+        for (int b = 0; b < nB; b++) {
+            const int c = (b / stride) % nd[d];
+            if (c == 0) { f[b] = f[b + off]; l[b] = l[b + off]; }
+            else if (c == nd[d] - 1) { f[b] = f[b - off]; l[b] = l[b - off]; }
+        }
+    }
+    if (first) std::copy(f.begin(), f.end(), first);
+    if (last) std::copy(l.begin(), l.end(), last);
+    if (next) std::copy(nx.begin(), nx.end(), next);
+    return nB;
+}
+
 int mps_ppe_matrix(mps_ctx* ctx, int* row_ptr, int* cols, double* vals, int* nnz_out) {
     CHECK_CTX(ctx);
     hipSetDevice(ctx->dev);
@@ -1248,11 +1301,19 @@ int mps_step(mps_ctx* ctx) {
 int mps_steps(mps_ctx* ctx, int k) {
     CHECK_CTX(ctx);
     hipSetDevice(ctx->dev);
+    int soft = MPS_OK;
+    std::string msg;
     for (int t = 0; t < k; t++) {
         int rc = runStep(ctx);
-        if (rc) return rc;
+        if (softSolverFailure(ctx, rc)) {
+            if (soft == MPS_OK) msg = "step " + std::to_string(ctx->numIter) + ": " + ctx->err;
+            soft = rc;
+        } else if (rc) {
+            return rc;
+        }
     }
-    return MPS_OK;
+    if (soft != MPS_OK) ctx->err = msg;
+    return soft;
 }
 
 }  // extern "C"

@@ -212,8 +212,17 @@ struct Ctx {
     int ppeSolves = 0;           // pressure solves so far (history of the extrapolated warm start)
     int lastCgIters = 0;         // iterations of the last FP64 CG solve / first mixed inner pass
     int lastInnerIters = 0;
-    bool r0Fresh = false;        // cgR and the reduction partials hold b - A x of the current guess (assembly)
+    bool r0Fresh = false;        // cgR and asmRed hold b - A x of the current guess and its partials (assembly)
     bool r0FreshMixed = false;   // ... in the mixed solve's partial order (else the FP64 CG's, with 1/diag)
+    double* asmRed = nullptr;    // block partials of the assembly's first residual (no other kernel writes them)
+    // solver kind of the last PPE assembly (MPS_SOLVER_* / precision), -1 when
+    // there is none for the current slot order; a solve must match it
+    int asmMode = -1;
+    // outcome of the last pressure solve: 0 converged, 1 stopped at an
+    // iteration limit above the tolerance, 2 non-finite residual
+    int solverFailed = 0;
+    long solverFailures = 0;     // solves that did not converge since the count was last reset
+    int lastDone = 0;            // the solver's final device status (5: a non-finite residual or product)
     double solverErr = 0.0;
     double trueResidual = 0.0;
     double ppeMs = 0.0;

@@ -2094,6 +2094,7 @@ int launchListKeepRefresh(Ctx* ctx, double skin) {
 }
 
 int launchCompaction(Ctx* ctx) {
+    ctx->asmMode = -1;   // slots move: an assembled matrix no longer matches them
     KArgs a = ctx->args();
     const int g = gridFor(ctx->N, kBlock);
     const int cap = 4096;

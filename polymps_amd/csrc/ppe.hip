@@ -15,6 +15,8 @@
 // iteration count convention.  Dot products are FP64 reductions in a fixed
 // order, so a solve is deterministic from run to run.
 #include <cfloat>
+#include <cmath>
+#include <cstdio>
 
 #include "kern_common.h"
 
@@ -439,6 +441,10 @@ __global__ void kCgStartFinish(KArgs a, int nb, double tol) {
     a.ds[S_RN2] = rn2;
     a.is[I_CG_ITER] = 0;
     a.is[I_CG_DONE] = 0;
+    if (!isfinite(bn2) || !isfinite(rn2)) {
+        a.is[I_CG_DONE] = 5;   // non-finite system or guess: nothing to iterate
+        return;
+    }
     if (bn2 == 0.0) {
         a.is[I_CG_DONE] = 2;   // x = 0
         return;
@@ -521,6 +527,10 @@ __global__ void kCgBeta(KArgs a, int nb, int maxIt) {
         a.is[I_CG_DONE] = 1;
         return;
     }
+    if (!isfinite(rn2)) {
+        a.is[I_CG_DONE] = 5;
+        return;
+    }
     const double absOld = a.ds[S_ABSNEW];
     a.ds[S_ABSOLD] = absOld;
     a.ds[S_ABSNEW] = rz;
@@ -587,7 +597,7 @@ __global__ void kMxInitFinish(KArgs a, int nb, double tolIn, double tol) {
     const double target = 0.25 * tol * tol * a.ds[S_BN2];
     a.ds[S_THR] = fmax(fmax(tolIn * tolIn * rn2, target), DBL_MIN);
     a.is[I_CG_ITER] = 0;
-    a.is[I_CG_DONE] = (rn2 == 0.0) ? 2 : 0;
+    a.is[I_CG_DONE] = (rn2 == 0.0) ? 2 : (isfinite(rn2) && isfinite(rz)) ? 0 : 5;
 }
 __global__ __launch_bounds__(256) void kMxSpmv(KArgs a, CgPtrs c) {
     if (a.is[I_CG_DONE]) return;
@@ -687,6 +697,10 @@ __global__ void kFcgBeta(KArgs a, int nb, int maxIt) {
         a.is[I_CG_DONE] = 1;
         return;
     }
+    if (!isfinite(rn2)) {
+        a.is[I_CG_DONE] = 5;
+        return;
+    }
     if (!(rz > 0.0)) {
         a.is[I_CG_DONE] = 4;
         return;
@@ -729,6 +743,13 @@ static int ensureMixed(Ctx* ctx) {
     return MPS_OK;
 }
 
+// The solver kind an assembly prepares: FP64 CG, mixed CG (FP32 values as
+// well) or BiCGSTAB (A instead of A^T).
+static int asmModeOf(const mps_params& P) {
+    if (P.solver_type == MPS_SOLVER_BICGSTAB) return 2;
+    return P.precision_mode == MPS_PREC_MIXED ? 1 : 0;
+}
+
 int ppeAssemble(Ctx* ctx) {
     // BiCGSTAB runs in FP64 on A; CG on A^T (ppe_transpose) and in the chosen precision
     const bool bicg = ctx->P.solver_type == MPS_SOLVER_BICGSTAB;
@@ -738,6 +759,7 @@ int ppeAssemble(Ctx* ctx) {
         if (rc) return rc;
     }
     KArgs a = ctx->args();
+    a.red = ctx->asmRed;   // the first residual's partials stay valid until the solve
     const int g = gridFor(ctx->N, kB);
     const bool T = ctx->P.ppe_transpose != 0 && !bicg;
     float* v32 = mixed ? ctx->aVal32 : nullptr;
@@ -768,6 +790,7 @@ int ppeAssemble(Ctx* ctx) {
     MPS_HIP(hipGetLastError());
     ctx->r0Fresh = res;
     ctx->r0FreshMixed = mixed;
+    ctx->asmMode = asmModeOf(ctx->P);
     return MPS_OK;
 }
 
@@ -858,7 +881,8 @@ __global__ void kBcgStartFinish(KArgs a, int nb, double tol, int maxIt) {
     a.is[I_CG_ITER] = 0;
     a.is[I_BCG_RESTART] = 0;
     a.is[I_BCG_RESTARTS] = 0;
-    if (bn2 == 0.0) a.is[I_CG_DONE] = 2;   // x = 0
+    if (!isfinite(bn2) || !isfinite(rn2)) a.is[I_CG_DONE] = 5;
+    else if (bn2 == 0.0) a.is[I_CG_DONE] = 2;   // x = 0
     else a.is[I_CG_DONE] = (rn2 > a.ds[S_THR] && maxIt > 0) ? 0 : 1;
 }
 // rho = r0.r
@@ -978,7 +1002,8 @@ __global__ void kBcgCheck(KArgs a, int nb, int maxIt) {
     a.ds[S_RN2] = rn2;
     const int it = a.is[I_CG_ITER] + 1;
     a.is[I_CG_ITER] = it;
-    if (!(rn2 > a.ds[S_THR])) a.is[I_CG_DONE] = 1;
+    if (!isfinite(rn2)) a.is[I_CG_DONE] = 5;
+    else if (!(rn2 > a.ds[S_THR])) a.is[I_CG_DONE] = 1;
     else if (it >= maxIt) a.is[I_CG_DONE] = 3;
 }
 
@@ -1027,6 +1052,7 @@ static int solveBicgstab(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int 
     }
     ctx->solverIter = iters;
     ctx->outerIter = 1;
+    ctx->lastDone = ctx->hostInts[I_CG_DONE];
     ctx->solverErr = ctx->hostScalars[S_RHSN2] > 0 ? sqrt(ctx->hostScalars[S_RN2] / ctx->hostScalars[S_RHSN2]) : 0.0;
     return MPS_OK;
 }
@@ -1037,9 +1063,12 @@ static int solveFp64(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int batc
     const double tol = ctx->P.solver_tol > 0 ? ctx->P.solver_tol : DBL_EPSILON;
     const int maxIt = ctx->P.solver_max_iter > 0 ? ctx->P.solver_max_iter : 2 * ctx->nLiveHost;
     // r = b - A x, 1 / diag and the partials may come from the assembly
-    if (!(ctx->r0Fresh && !ctx->r0FreshMixed)) hipLaunchKernelGGL(kCgStart, dim3(nb), dim3(kB), 0, s, a, c);
+    const bool fresh = ctx->r0Fresh && !ctx->r0FreshMixed;
+    if (!fresh) hipLaunchKernelGGL(kCgStart, dim3(nb), dim3(kB), 0, s, a, c);
     ctx->r0Fresh = false;
-    hipLaunchKernelGGL(kCgStartFinish, dim3(1), dim3(1024), 0, s, a, nb, tol);
+    KArgs a0 = a;
+    if (fresh) a0.red = ctx->asmRed;
+    hipLaunchKernelGGL(kCgStartFinish, dim3(1), dim3(1024), 0, s, a0, nb, tol);
     hipLaunchKernelGGL(kCgZeroX, dim3(nb), dim3(kB), 0, s, a, c.x);
     hipLaunchKernelGGL(kCgInitP, dim3(nb), dim3(kB), 0, s, a, c);
     hipLaunchKernelGGL(kCgInitPFinish, dim3(1), dim3(1024), 0, s, a, nb);
@@ -1061,6 +1090,7 @@ static int solveFp64(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int batc
     }
     ctx->lastCgIters = iters;
     const int D = ctx->hostInts[I_CG_DONE];
+    ctx->lastDone = D;
     addSpmvStats(ctx, times, (D == 1 || D == 3) ? iters + 1 : 0,
                  10.0 * ctx->hostInts[I_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 28.0 * ctx->nLiveHost);
     ctx->solverIter = iters;
@@ -1081,15 +1111,19 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
     int total = 0, outer = 0, rc;
     bool useM = ctx->P.learned_mode == 2 && ctx->gnn != nullptr;
     double prevRn2 = 0.0;
+    ctx->lastDone = 0;
     const double bytes32 = 6.0 * ctx->hostInts[I_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 16.0 * ctx->nLiveHost;
     for (;;) {
         // the first residual may come from the assembly (r and its partials)
-        if (!(outer == 0 && ctx->r0Fresh && ctx->r0FreshMixed))
-            hipLaunchKernelGGL(kTrueResidual, dim3(nb), dim3(kB), 0, s, a, c);
+        const bool fresh = outer == 0 && ctx->r0Fresh && ctx->r0FreshMixed;
+        if (!fresh) hipLaunchKernelGGL(kTrueResidual, dim3(nb), dim3(kB), 0, s, a, c);
         ctx->r0Fresh = false;
-        hipLaunchKernelGGL(kTrueResidualFinish, dim3(1), dim3(1024), 0, s, a, nb);
+        KArgs a0 = a;
+        if (fresh) a0.red = ctx->asmRed;
+        hipLaunchKernelGGL(kTrueResidualFinish, dim3(1), dim3(1024), 0, s, a0, nb);
         if ((rc = readStatus(ctx, nullptr, 0))) return rc;
         const double rn2 = ctx->hostScalars[S_TRUE_RN2], bn2 = ctx->hostScalars[S_BN2];
+        if (!std::isfinite(rn2) || !std::isfinite(bn2)) break;   // reported by ppeSolve
         if (bn2 == 0.0) {   // zero right-hand side: x = 0, as the FP64 path
             hipLaunchKernelGGL(kZero, dim3(nb), dim3(kB), 0, s, a, c.x);
             break;
@@ -1143,9 +1177,14 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
             ctx->fallbackUsed = 1;
         }
         hipLaunchKernelGGL(kMxApply, dim3(nb), dim3(kB), 0, s, a, c);
+        if (D == 5) {   // non-finite inner residual or product
+            ctx->lastDone = 5;
+            break;
+        }
     }
     ctx->solverIter = total;
     ctx->outerIter = outer;
+    if (ctx->lastDone != 5) ctx->lastDone = ctx->hostInts[I_CG_DONE];
     ctx->solverErr = ctx->hostScalars[S_BN2] > 0 ? sqrt(ctx->hostScalars[S_TRUE_RN2] / ctx->hostScalars[S_BN2]) : 0.0;
     return MPS_OK;
 }
@@ -1184,8 +1223,14 @@ int ppeWarmStart(Ctx* ctx) {
 int ppeSolve(Ctx* ctx) {
     const bool bicg = ctx->P.solver_type == MPS_SOLVER_BICGSTAB;
     const bool mixed = !bicg && ctx->P.precision_mode == MPS_PREC_MIXED;
-    if (mixed && !ctx->aVal32) {
-        ctx->err = "mixed-precision solve needs a mixed-precision assembly first";
+    if (ctx->asmMode < 0) {
+        ctx->err = "the pressure solve needs a PPE assembly after the last bucket update, compaction or state upload";
+        return MPS_E_ARG;
+    }
+    if (ctx->asmMode != asmModeOf(ctx->P)) {
+        static const char* kinds[] = {"FP64 CG", "mixed-precision CG", "BiCGSTAB"};
+        ctx->err = std::string("the PPE was assembled for ") + kinds[ctx->asmMode] + " but the solve is " +
+                   kinds[asmModeOf(ctx->P)] + " (solver_type / precision_mode changed in between)";
         return MPS_E_ARG;
     }
     KArgs a = ctx->args();
@@ -1222,8 +1267,28 @@ int ppeSolve(Ctx* ctx) {
     ctx->ppeMs = ms;
     const double* hs = ctx->hostScalars;
     ctx->trueResidual = hs[S_BN2] > 0 ? sqrt(hs[S_TRUE_RN2] / hs[S_BN2]) : 0.0;
+    if (!std::isfinite(hs[S_BN2]) || !std::isfinite(hs[S_TRUE_RN2])) ctx->trueResidual = NAN;
     ctx->ppeSolves++;
     ctx->r0Fresh = false;
+    // Outcome.  FP64 CG and BiCGSTAB follow Eigen's rule (success when the
+    // recursive relative residual ends at or below the tolerance; the
+    // reference prints "Failed solving" otherwise, src/MpsPressure.cpp:440-442);
+    // the mixed solve stops on its FP64 true residual.  The pressure is
+    // written either way, as the reference does.
+    const double tol = ctx->P.solver_tol > 0 ? ctx->P.solver_tol : (mixed ? 1e-8 : DBL_EPSILON);
+    const bool nonFinite = ctx->lastDone == 5 || !std::isfinite(ctx->trueResidual) || !std::isfinite(ctx->solverErr);
+    const bool converged = mixed ? ctx->trueResidual < tol : ctx->solverErr <= tol;
+    ctx->solverFailed = nonFinite ? 2 : converged ? 0 : 1;
+    if (ctx->solverFailed) {
+        ctx->solverFailures++;
+        char msg[256];
+        snprintf(msg, sizeof msg,
+                 "%s: the pressure solve %s after %d iterations (relative residual %.3e, true %.3e, tolerance %.1e)",
+                 "Failed solving", nonFinite ? "produced a non-finite residual" : "stopped above the tolerance",
+                 ctx->solverIter, ctx->solverErr, ctx->trueResidual, tol);
+        ctx->err = msg;
+        return MPS_E_SOLVER;
+    }
     return MPS_OK;
 }
 

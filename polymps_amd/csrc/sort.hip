@@ -197,6 +197,7 @@ int launchScan(Ctx* ctx, const int* in, int* out, int n, int tailValue) {
 }
 
 int launchBucketSort(Ctx* ctx) {
+    ctx->asmMode = -1;   // slots move: an assembled matrix no longer matches them
     KArgs a = ctx->args();
     const int N = ctx->N, nB = ctx->nB;
     MPS_HIP(hipMemsetAsync(ctx->cellCount, 0, sizeof(int) * (nB + 1), ctx->stream));

@@ -268,8 +268,16 @@ public:
     void allocateMemory(MpsParticle*) {}
     // MpsPressure::calcPress (src/MpsPressure.cpp:32-58): EOS pressure or PPE
     // assembly + Jacobi-PCG + setZeroOnNegativePressure.
+    // A solve that stops above its tolerance prints the reference's message
+    // and the run goes on, as there (src/MpsPressure.cpp:440-442); a
+    // non-finite residual throws.
     void calcPress(MpsParticleSystem*, MpsParticle* P, MpsBucket*) {
-        polymps_gpu::check(mps_calc_press(P->ctx), P->ctx, "calcPress");
+        const int rc = mps_calc_press(P->ctx);
+        double failed = 0;
+        if (rc == MPS_E_SOLVER && mps_get_scalar(P->ctx, "solver_failed", &failed) == MPS_OK && failed == 1.0)
+            fprintf(stderr, "Error: Failed solving.\n");
+        else
+            polymps_gpu::check(rc, P->ctx, "calcPress");
         double v = 0;
         mps_get_scalar(P->ctx, "solver_iter", &v);
         solverIter_ = (int)v;

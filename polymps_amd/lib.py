@@ -36,6 +36,14 @@ class MpsError(RuntimeError):
     pass
 
 
+MPS_E_SOLVER = -6
+
+
+class MpsSolverError(MpsError):
+    """A pressure solve did not reach its tolerance (the state is written
+    and a step finishes first) or produced a non-finite residual."""
+
+
 def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     p = Path(path) if path else _LIB_PATH
     if not p.exists():
@@ -68,6 +76,7 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     lib.mps_mesh_paths_from_json.argtypes = [cp, cp, ctypes.c_size_t, cp, ctypes.c_size_t]
     lib.mps_get_wall_tables.argtypes = [vp, vp, vp, vp, vp]
     lib.mps_ppe_matrix.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ip)]
+    lib.mps_get_buckets.argtypes = [vp, vp, vp, vp]
     lib.mps_steps.argtypes = [vp, ip]
     lib.mps_read_grid.argtypes = [cp, ctypes.POINTER(ip), vp, vp, vp, vp, vp]
     lib.mps_write_grid.argtypes = [cp, ip, vp, vp, vp, vp, vp]
@@ -200,6 +209,8 @@ class Simulation:
         self._check(self._lib.mps_mesh_closest_point_type(self._h, mesh_type))
 
     def _check(self, rc):
+        if rc == MPS_E_SOLVER:
+            raise MpsSolverError(f"MPS call failed ({rc}): {self._lib.mps_last_error(self._h).decode()}")
         if rc:
             raise MpsError(f"MPS call failed ({rc}): {self._lib.mps_last_error(self._h).decode()}")
 
@@ -258,6 +269,21 @@ class Simulation:
         vals = np.zeros(nnz.value)
         self._check(self._lib.mps_ppe_matrix(self._h, _ptr(rp), _ptr(cols), _ptr(vals), ctypes.byref(nnz)))
         return rp, cols, vals
+
+    def buckets(self):
+        """(first, last, next) bucket lists of the last bucket update in
+        reference indices (firstParticleInBucket, lastParticleInBucket,
+        nextParticleInSameBucket of the reference)."""
+        nb = self._lib.mps_get_buckets(self._h, None, None, None)
+        if nb < 0:
+            self._check(nb)
+        first = np.zeros(nb, dtype=np.int32)
+        last = np.zeros(nb, dtype=np.int32)
+        nxt = np.zeros(self.n, dtype=np.int32)
+        rc = self._lib.mps_get_buckets(self._h, _ptr(first), _ptr(last), _ptr(nxt))
+        if rc < 0:
+            self._check(rc)
+        return first, last, nxt
 
     # ------------------------------------------------ learned PPE operator
     def gnn_init(self, seed: int = 0):

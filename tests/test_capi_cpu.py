@@ -143,3 +143,23 @@ def test_workload_sizes():
     assert configs.dam3d_inc(l0=0.005, steps=1).n == 216000
     w = configs.dam2d_wc()
     assert w.info["fluid"] == 5000 and 1200 < w.info["walls"] < 1400
+
+
+def test_solver_failure_status_mapping(built):
+    """MPS_E_SOLVER of include/mps_gpu.h reaches Python as MpsSolverError,
+    every other non-zero code as MpsError (a subclass relation the callers
+    that carry on after a failed solve rely on)."""
+    text = (ROOT / "include" / "mps_gpu.h").read_text()
+    code = int(re.search(r"#define MPS_E_SOLVER \((-?\d+)\)", text).group(1))
+    assert code == L.MPS_E_SOLVER
+
+    class Stub:
+        _h = None
+        _lib = L.lib()
+
+    with pytest.raises(L.MpsSolverError):
+        L.Simulation._check(Stub(), code)
+    with pytest.raises(L.MpsError) as e:
+        L.Simulation._check(Stub(), -1)
+    assert not isinstance(e.value, L.MpsSolverError)
+    L.Simulation._check(Stub(), 0)

@@ -14,7 +14,7 @@ import pytest
 
 from polymps_amd import configs
 from tests.parity_util import check, lockstep_step, new_pair
-from tests.test_gpu_parity import lockstep_init
+from tests.parity_util import lockstep_init
 
 pytestmark = pytest.mark.gpu
 

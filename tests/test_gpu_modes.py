@@ -6,8 +6,7 @@ advanced, bit-exact except the PPE solution (1e-6 of the largest pressure)."""
 import pytest
 
 from polymps_amd import configs
-from tests import test_gpu_parity as tp
-from tests.parity_util import lockstep_step, new_pair, sync_state
+from tests.parity_util import lockstep_init, lockstep_step, new_pair, sync_state
 
 pytestmark = pytest.mark.gpu
 
@@ -52,7 +51,7 @@ def test_option_lockstep(built, case):
     w = make(case)
     print(f"option case {case}", flush=True)
     g, o = new_pair(w)
-    tp.lockstep_init(g, o)
+    lockstep_init(g, o)
     lockstep_step(g, o, 0)
     o.steps(30)
     sync_state(g, o)

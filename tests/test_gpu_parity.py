@@ -11,58 +11,10 @@ import numpy as np
 import pytest
 
 from polymps_amd import configs
-from tests.parity_util import (Mismatch, check, compare_close, lockstep_step, new_pair, ppe_check, sync_state)
+from tests.parity_util import (Mismatch, check, compare_close, lockstep_init, lockstep_step, new_pair, ppe_check,
+                               sync_state)
 
 pytestmark = pytest.mark.gpu
-
-
-def lockstep_init(g, o):
-    p = g.params
-    poly = p.wall_type == 1
-    g.call("bucket_update"); o.call("update_buckets")
-    g.call("check_out_domain"); o.call("check_out_domain")
-    if poly:
-        g.call("mesh_closest_point"); o.call("closest_point")
-        check(g, o, ["riw2", "mirror", "wall_pos", "polygon_normal"], "init closest_point")
-        g.call("mesh_update_near"); o.call("update_near_mesh")
-        check(g, o, ["pnd_wall", "num_neigh_wall", "near_wall"], "init update_near")
-    g.call("build_neighbor_list")
-    if poly:
-        g.call("wall_npcd"); o.call("wall_npcd")
-        check(g, o, ["npcd"], "init wall_npcd")
-    g.call("initial_pnd"); o.call("initial_pnd")
-    check(g, o, ["pndi", "pnd_small", "pndki", "num_neigh", "npcd", "npcd2"], "init initial_pnd")
-    g.call("mean_pnd")
-    if poly:
-        o.call("mean_wall_pnd")
-    o.call("mean_pnd")
-    check(g, o, ["pndi"], "init mean_pnd")
-    g.call("mean_neigh_fluid_pnd"); o.call("mean_neigh_fluid_pnd")
-    check(g, o, ["pndski"], "init mean_neigh_fluid_pnd")
-    if p.free_surf_type == 2:
-        g.call("normal_particles"); o.call("normal_particles")
-        if poly:
-            g.call("wall_normal_particles"); o.call("wall_normal_particles")
-        check(g, o, ["normal"], "init normals")
-    g.call("update_bc"); o.call("update_bc")
-    check(g, o, ["bc"], "init update_bc")
-    if p.mps_type >= 2:
-        if p.mps_type == 3:
-            g.call("vel_divergence"); o.call("vel_divergence")
-            if poly:
-                o.call("wall_vel_divergence")
-            check(g, o, ["div_u"], "init vel_divergence")
-        g.call("ppe_assemble"); o.call("assemble_ppe")
-        ppe_check(g, o, "init ppe", 1e-6)
-        g.call("ppe_solve"); o.call("solve_ppe")
-        po = o.get("press")
-        compare_close("press", g.get("press"), po, 0.0, 1e-6 * max(np.abs(po).max(), 1e-30), "init ppe_solve")
-        g.set("ppe_x", o.get("ppe_x"))
-        g.set("press", po)
-    else:
-        g.call("calc_press"); o.call("calc_press")
-        compare_close("press", g.get("press"), o.get("press"), 1e-13, 1e-9, "init calc_press")
-        g.set("press", o.get("press"))
 
 
 def test_lockstep_dam3d_from_rest(built):
