@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <climits>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -70,6 +71,17 @@ struct Sys {
     std::vector<int> rowPtr, colIdx;
     std::vector<double> vals;
     int solverIter = 0;
+    // CG with OpenMP dot products and vector updates (the CPU baseline's
+    // parallel CSR Jacobi-PCG); off: serial, in index order (the reference's
+    // Eigen CG is serial, src/MpsPressure.cpp:424-445)
+    bool parallelCg = false;
+    // wall time of the last CG solve: building the CSR iteration matrix, then the iterations
+    double solveSetupSeconds = 0.0, solveIterSeconds = 0.0;
+    // CG iteration matrix (CSR) of assembly number itAsm, transposed or not
+    long asmCount = 0, itAsm = -1;
+    int itTrans = -1;
+    std::vector<int> itRp, itCi;
+    std::vector<double> itVa, itDiag;
     double solverErr = 0.0;
     // buckets
     std::vector<int> first, last, next;
@@ -1702,6 +1714,7 @@ void wallVelDivergence(Sys& s) {
 // diagonal); identity rows for surface / other particles.
 void assemblePPE(Sys& s) {
     const mps_params& P = s.p;
+    s.asmCount++;
     const bool pw = partWall(s);
     const bool divu = P.mps_type == MPS_IMPLICIT_PND_DIVU;
     s.ppeB.assign(s.N, 0.0);
@@ -1754,6 +1767,10 @@ void assemblePPE(Sys& s) {
 // iterated matrix, as a compressed sparse traversal does.
 // The iterated matrix in row form with ascending columns: A^T (what
 // Eigen's Lower|Upper CG uses on the row-filled matrix) or A.
+static double wallSeconds() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 void iterMatrix(const Sys& s, bool transpose, std::vector<int>& rp, std::vector<int>& ci, std::vector<double>& va,
                 std::vector<double>& diag) {
     const int n = s.N;
@@ -1856,9 +1873,20 @@ void solveBiCGSTAB(Sys& s) {
 void solveCG(Sys& s) {
     const mps_params& P = s.p;
     const int n = s.N;
-    std::vector<int> rp, ci;
-    std::vector<double> va, diag;
-    iterMatrix(s, P.ppe_transpose != 0, rp, ci, va, diag);
+    // the iteration matrix of the last assembly is built once and kept
+    const double tSetup = wallSeconds();
+    const int want = P.ppe_transpose != 0 ? 1 : 0;
+    if (s.itAsm != s.asmCount || s.itTrans != want) {
+        iterMatrix(s, want != 0, s.itRp, s.itCi, s.itVa, s.itDiag);
+        s.itAsm = s.asmCount;
+        s.itTrans = want;
+    }
+    const std::vector<int>& rp = s.itRp;
+    const std::vector<int>& ci = s.itCi;
+    const std::vector<double>& va = s.itVa;
+    const std::vector<double>& diag = s.itDiag;
+    const double tIter = wallSeconds();
+    s.solveSetupSeconds = tIter - tSetup;
     auto spmv = [&](const std::vector<double>& x, std::vector<double>& y) {
 #pragma omp parallel for
         for (int r = 0; r < n; r++) {
@@ -1867,9 +1895,15 @@ void solveCG(Sys& s) {
             y[r] = acc;
         }
     };
+    const bool par = s.parallelCg;
     auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {
         double acc = 0.0;
-        for (int r = 0; r < n; r++) acc += a[r] * b[r];
+        if (par) {
+#pragma omp parallel for reduction(+ : acc)
+            for (int r = 0; r < n; r++) acc += a[r] * b[r];
+        } else {
+            for (int r = 0; r < n; r++) acc += a[r] * b[r];
+        }
         return acc;
     };
     std::vector<double> invd(n);
@@ -1900,19 +1934,25 @@ void solveCG(Sys& s) {
     while (it < maxIt) {
         spmv(p, tmp);
         double alpha = absNew / dot(p, tmp);
-        for (int k = 0; k < n; k++) x[k] += alpha * p[k];
-        for (int k = 0; k < n; k++) r[k] -= alpha * tmp[k];
+#pragma omp parallel for if (par)
+        for (int k = 0; k < n; k++) {
+            x[k] += alpha * p[k];
+            r[k] -= alpha * tmp[k];
+        }
         rN2 = dot(r, r);
         if (rN2 < thr) break;
+#pragma omp parallel for if (par)
         for (int k = 0; k < n; k++) z[k] = invd[k] * r[k];
         double absOld = absNew;
         absNew = dot(r, z);
         double beta = absNew / absOld;
+#pragma omp parallel for if (par)
         for (int k = 0; k < n; k++) p[k] = z[k] + beta * p[k];
         it++;
     }
     s.solverErr = sqrt(rN2 / rhsN2);
     s.solverIter = it;
+    s.solveIterSeconds = wallSeconds() - tIter;
 }
 
 int calcPress(Sys& s) {
@@ -2573,6 +2613,8 @@ int orc_scalar(void* h, const char* name, double* out) {
     else if (f == "solver_iter") *out = s->solverIter;
     else if (f == "solver_err") *out = s->solverErr;
     else if (f == "num_particles") *out = s->N;
+    else if (f == "solve_setup_ms") *out = s->solveSetupSeconds * 1e3;
+    else if (f == "solve_iter_ms") *out = s->solveIterSeconds * 1e3;
     else return -1;
     return 0;
 }
@@ -2581,6 +2623,9 @@ int orc_set_scalar(void* h, const char* name, double v) {
     Sys* s = (Sys*)h;
     std::string f(name);
     if (f == "vel_max") s->velMax = v;
+    else if (f == "solver_tol") s->p.solver_tol = v;        // 0: machine epsilon, as Eigen's default
+    else if (f == "parallel_cg") s->parallelCg = v != 0.0;
+    else if (f == "threads") orc_set_threads(h, (int)v);
     else return -1;
     return 0;
 }

@@ -47,3 +47,44 @@ def test_channel_lockstep_with_oracle(built):
     sync_state(g, o)
     for s in range(2):
         lockstep_step(g, o, s)
+
+
+def test_reference_terrain_closest_point(built):
+    """The reference's own 46,215-triangle terrain mesh and parameter file
+    (input/stl/BRUMADINHO_space10_model_lucas_top_0p50.stl,
+    input/BRUMADINHO_space10_lo10p00.json, kept in tests/golden) with 4,000
+    seeded query points over its bounding box: the BVH search, the
+    brute-force search and the oracle's brute force agree bit for bit."""
+    from pathlib import Path
+
+    from oracle.oracle import Oracle
+    from polymps_amd import lib as L
+    data = Path(__file__).resolve().parent / "golden" / "reference_input"
+    V, F, N = L.read_stl(data / "stl" / "BRUMADINHO_space10_model_lucas_top_0p50.stl")
+    p, _, _ = L.params_from_json(data / "BRUMADINHO_space10_lo10p00.json")
+    v = V.reshape(-1, 3)
+    lo, hi = v.min(0), v.max(0)
+    rng = np.random.default_rng(7)
+    n = 4000
+    pos = np.column_stack([rng.uniform(lo[0], hi[0], n), rng.uniform(lo[1], hi[1], n),
+                           rng.uniform(lo[2], hi[2] + 50.0, n)]).reshape(-1)
+    types = np.zeros(n, dtype=np.int32)
+    print(f"terrain closest point: {len(F) // 3} triangles, {n} points", flush=True)
+    g = Simulation(p, types, pos, np.zeros(3 * n), mesh=(V, F, N), device=0)
+    o = Oracle(p, types, pos, np.zeros(3 * n), mesh=(V, F, N))
+    far = np.full(n, 10e8 * p.part_dist)
+    fields = ["riw2", "mirror", "wall_pos", "polygon_normal"]
+    out = {}
+    for mode in (0, 1):
+        g.set("riw2", far)
+        g.set_scalar("mesh_bvh", mode)
+        g.call("mesh_closest_point")
+        out[mode] = {f: g.get(f) for f in fields}
+    o.set("riw2", far)
+    o.call("closest_point")
+    for f in fields:
+        assert np.array_equal(out[0][f], out[1][f]), f
+        assert np.array_equal(out[1][f], o.get(f)), f
+    assert np.all(out[1]["riw2"] < 500.0 ** 2)
+    g.close()
+    o.close()

@@ -1,0 +1,160 @@
+"""The reference's own input files through the library's readers (no GPU).
+
+tests/golden/reference_input holds, unchanged, the 16 parameter files of
+the reference's input/ folder and the 8 STL meshes of input/stl (data
+files; the .grid particle files they name are not part of the reference
+tree).  Every JSON file is read by the library's parser
+(MpsInputOutput::readInputFile, src/MpsInputOutput.cpp:58-641) and compared,
+key by key, with Python's json module on the same file; every STL is read
+by the library (PolygonMesh::readPolygonMeshFile, src/PolygonMesh.cpp:42-108)
+and compared with a direct decoding of its binary records.
+"""
+import json
+import struct
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from polymps_amd import lib as L
+
+DATA = Path(__file__).resolve().parent / "golden" / "reference_input"
+JSONS = sorted(DATA.glob("*.json"))
+STLS = sorted((DATA / "stl").glob("*.stl"))
+
+AX = ("x", "y", "z")
+# (JSON key path, parameter field[, index]) for every key readInputFile reads
+KEYS = [
+    (("flags", "wall_type"), "wall_type"), (("flags", "fem_MESH"), "fem_on"),
+    (("flags", "forced_MESH"), "forced_on"),
+    *[(("domain", "min", a), "domain_min", k) for k, a in enumerate(AX)],
+    *[(("domain", "max", a), "domain_max", k) for k, a in enumerate(AX)],
+    (("domain", "boundary", "type"), "domain_type_bc"), (("domain", "boundary", "limit"), "limit_type_bc"),
+    *[(("domain", "boundary", "direction", a), "periodic_dir", k) for k, a in enumerate(AX)],
+    (("physical", "fluid_density"), "density_fluid"), (("physical", "wall_density"), "density_wall"),
+    (("physical", "kinematic_visc"), "knm_vs1"), (("physical", "fluid_type"), "fluid_type"),
+    *[(("physical", "gravity", a), "gravity", k) for k, a in enumerate(AX)],
+    (("physical", "rheological", "kinematic_visc_phase_2"), "knm_vs2"),
+    (("physical", "rheological", "fluid_density_phase_1"), "dns_fl1"),
+    (("physical", "rheological", "fluid_density_phase_2"), "dns_fl2"),
+    (("numerical", "dimension"), "dim"), (("numerical", "particle_dist"), "part_dist"),
+    (("numerical", "time_step"), "time_step"), (("numerical", "final_time"), "time_simulation"),
+    (("numerical", "iter_output"), "iter_output"), (("numerical", "CFL_number"), "cfl_number"),
+    (("numerical", "weight_type"), "weight_type"), (("numerical", "slip_condition"), "slip_condition"),
+    (("numerical", "effective_radius", "small"), "re_s"), (("numerical", "effective_radius", "large"), "re_l"),
+    (("numerical", "gradient", "type"), "gradient_type"),
+    (("numerical", "gradient", "correction"), "gradient_correction"),
+    (("numerical", "gradient", "relax_fact"), "relax_press"), (("numerical", "mps_type"), "mps_type"),
+    (("numerical", "explicit_mps", "equation_state", "speed_sound"), "sound_speed"),
+    (("numerical", "explicit_mps", "equation_state", "gamma"), "gamma"),
+    (("numerical", "semi_implicit_mps", "solver_type"), "solver_type"),
+    (("numerical", "semi_implicit_mps", "weak_compressibility", "alpha"), "alpha_compressibility"),
+    (("numerical", "semi_implicit_mps", "source_term", "relax_pnd"), "relax_pnd"),
+    (("numerical", "particle_shifting", "type"), "shifting_type"),
+    (("numerical", "particle_shifting", "DRI"), "dri"),
+    (("numerical", "particle_shifting", "coef_A"), "coef_a"),
+    (("numerical", "particle_shifting", "mach_number"), "mach_number"),
+    (("numerical", "pnd", "type"), "pnd_type"), (("numerical", "pnd", "diffusive_coeff"), "diffusive_coef"),
+    (("numerical", "wall_repulsive_force", "type"), "repulsive_force_type"),
+    (("numerical", "wall_repulsive_force", "re"), "re_repulsive_force"),
+    (("numerical", "wall_repulsive_force", "maxVel"), "expect_max_velocity"),
+    (("numerical", "wall_repulsive_force", "coefficient", "Mitsume"), "rep_coef_mitsume"),
+    (("numerical", "wall_repulsive_force", "coefficient", "Lennard-Jones"), "rep_coef_lennard_jones"),
+    (("numerical", "wall_repulsive_force", "coefficient", "Monaghan-Kajtar"), "rep_coef_monaghan_kajtar"),
+    (("numerical", "wall_repulsive_force", "eps_re"), "eps_re"),
+    (("numerical", "free_surface_threshold", "type"), "free_surf_type"),
+    (("numerical", "free_surface_threshold", "pnd"), "pnd_threshold"),
+    (("numerical", "free_surface_threshold", "neigh"), "neigh_threshold"),
+    (("numerical", "free_surface_threshold", "NPCD"), "npcd_threshold"),
+    (("numerical", "free_surface_threshold", "ARC"), "theta_threshold"),
+    (("numerical", "free_surface_threshold", "normal"), "norm_threshold"),
+    (("numerical", "particle_collision", "type"), "collision_type"),
+    (("numerical", "particle_collision", "ratio"), "collision_ratio"),
+    (("numerical", "particle_collision", "dist_limit_ratio"), "dist_limit_ratio"),
+    (("numerical", "particle_collision", "lambda"), "lambda_collision"),
+    (("numerical", "particle_type", "ghost"), "ghost"), (("numerical", "particle_type", "fluid"), "fluid"),
+    (("numerical", "particle_type", "wall"), "wall"), (("numerical", "particle_type", "dummyWall"), "dummy_wall"),
+    (("numerical", "boundary_type", "free_surface"), "surface"),
+    (("numerical", "boundary_type", "inner"), "inner"), (("numerical", "boundary_type", "other"), "other"),
+]
+
+
+def _lookup(d, path):
+    for k in path:
+        if not isinstance(d, dict) or k not in d:
+            return None
+        d = d[k]
+    return d
+
+
+def test_fixture_set_is_complete():
+    assert len(JSONS) == 16 and len(STLS) == 8
+
+
+@pytest.mark.parametrize("path", JSONS, ids=[p.stem for p in JSONS])
+def test_reference_json_parses_as_python_json(built, path):
+    ref = json.loads(path.read_text())
+    p, grid, mesh = L.params_from_json(path)
+    checked = 0
+    for entry in KEYS:
+        keys, field = entry[0], entry[1]
+        v = _lookup(ref, keys)
+        if v is None:
+            continue
+        got = getattr(p, field)
+        if len(entry) == 3:
+            got = got[entry[2]]
+        assert float(got) == float(v), (path.name, "/".join(keys), got, v)
+        checked += 1
+    assert checked >= 60, checked
+    assert grid == ref["pathNames"]["particle_grid_file"]
+    assert mesh == ref["pathNames"]["mesh_rigid_file"]
+
+
+def test_reference_json_values_spot_checks(built):
+    """Stated values of a few files (the reference's input/README.md
+    descriptions and the file contents)."""
+    p, _, mesh = L.params_from_json(DATA / "InputDamINC.json")
+    assert (p.mps_type, p.wall_type, p.solver_type, p.gradient_correction, p.shifting_type) == (2, 1, 0, 1, 0)
+    assert p.relax_pnd == 0.01 and p.alpha_compressibility == 1e-6
+    assert mesh == "input/stl/dam1610.stl"
+    b, _, mesh = L.params_from_json(DATA / "BRUMADINHO_space10_lo10p00.json")
+    assert b.part_dist == 10.0 and b.mps_type == 1 and b.wall_type == 1
+    assert mesh.endswith("BRUMADINHO_space10_model_lucas_top_0p50.stl")
+    # the Newtonian-only build reads the non-Newtonian files too
+    e, _, _ = L.params_from_json(DATA / "damErosion3D_WC.json")
+    assert e.fluid_type == 1
+
+
+def _decode_binary_stl(path):
+    b = path.read_bytes()
+    n = struct.unpack("<I", b[80:84])[0]
+    rec = np.frombuffer(b, dtype=np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")]), count=n, offset=84)
+    return rec["v"].astype(np.float64)   # [n][3][3]
+
+
+@pytest.mark.parametrize("path", STLS, ids=[p.stem for p in STLS])
+def test_reference_stl_reads(built, path):
+    V, F, N = L.read_stl(path)
+    tri = _decode_binary_stl(path)
+    V = V.reshape(-1, 3)
+    F = F.reshape(-1, 3)
+    N = N.reshape(-1, 3)
+    assert len(F) == len(tri)
+    # every face, in file order, has its three vertices
+    assert np.array_equal(V[F], tri)
+    # duplicates merged: no two distinct vertex rows are equal
+    assert len(np.unique(V, axis=0)) == len(V)
+    # unit normals of (v1 - v0) x (v2 - v0)
+    c = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    nrm = np.linalg.norm(c, axis=1)
+    ok = nrm > 0
+    assert np.allclose(N[ok], c[ok] / nrm[ok, None], rtol=0, atol=1e-12)
+
+
+def test_reference_stl_face_counts(built):
+    counts = {p.name: len(L.read_stl(p)[1]) // 3 for p in STLS}
+    assert counts["dam1610.stl"] == 10
+    assert counts["hydro_3D_box.stl"] == 12
+    assert counts["BRUMADINHO_space10_model_lucas_top_0p50.stl"] == 46215
+    assert counts["oneTriangle_0.stl"] == 1
