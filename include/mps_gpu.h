@@ -80,6 +80,14 @@ int mps_upload_state(mps_ctx* ctx, const int* type, const double* pos, const dou
  * "bc" "num_neigh" "num_neigh_wall" "near_wall" "near_mesh_type".  `count`
  * is the element capacity of out. */
 int mps_download(mps_ctx* ctx, const char* field, void* out, size_t count);
+/* mps_download without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory (tight serving
+ * loops, code that forbids hipMalloc).  The field is staged in a 64 MiB area
+ * the library holds per GPU until the process ends; a field larger than that
+ * (a vector field beyond 2,796,202 particles, a scalar beyond 8,388,608)
+ * returns MPS_E_CAPACITY.  All contexts on one GPU share the area: calls that
+ * overlap on one GPU (other threads, other contexts) race. */
+int mps_download_noalloc(mps_ctx* ctx, const char* field, void* out, size_t count);
 /* Overwrite one field (reference order), same names as mps_download. */
 int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t count);
 /* Scalars: "vel_max" "cfl" "time" "solver_iter" "solver_err" "num_particles"

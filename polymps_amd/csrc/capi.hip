@@ -369,6 +369,37 @@ using namespace mpsgpu;
 
 struct mps_ctx : public Ctx {};
 
+// Staging area of mps_download_noalloc: a file-scope device array, one copy
+// per GPU, held until the process ends.  In host code `scratch` names a
+// host-side stand-in, not device memory: host code uses only the device
+// address hipGetSymbolAddress returns, looked up once per device with that
+// device current and kept in gScratchPtr.
+constexpr size_t kScratchBytes = size_t(64) << 20;   // 64 MiB: 8M doubles (a vector field of 2.79M particles)
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex gScratchMu;
+static void* gScratchPtr[kMaxDevices] = {};
+
+// The current device's scratch address (ctx->dev must be current).
+static int scratchAddress(Ctx* ctx, void** out) {
+    if (ctx->dev < 0 || ctx->dev >= kMaxDevices) {
+        ctx->err = "device index out of range for the staging area";
+        return MPS_E_ARG;
+    }
+    std::lock_guard<std::mutex> lk(gScratchMu);
+    if (!gScratchPtr[ctx->dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            ctx->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return MPS_E_HIP;
+        }
+        gScratchPtr[ctx->dev] = ptr;
+    }
+    *out = gScratchPtr[ctx->dev];
+    return MPS_OK;
+}
+
 #define CHECK_CTX(c) \
     if (!(c)) return MPS_E_ARG
 
@@ -387,6 +418,11 @@ int mps_create(mps_ctx** out, const mps_params* p, int n, int device) {
     ctx->dev = device;
     MPS_HIP(hipSetDevice(device));
     MPS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    {   // the device's staging area for mps_download_noalloc, looked up once per device
+        void* st = nullptr;
+        const int rc = scratchAddress(ctx, &st);
+        if (rc) return rc;
+    }
     MPS_HIP(hipEventCreate(&ctx->ev0));
     MPS_HIP(hipEventCreate(&ctx->ev1));
     const size_t nd = sizeof(double) * (size_t)n, ni = sizeof(int) * (size_t)n;
@@ -588,6 +624,43 @@ int mps_download(mps_ctx* ctx, const char* field, void* out, size_t count) {
         if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return MPS_E_HIP; }
     }
     MPS_HIP(hipGetLastError());
+    return MPS_OK;
+}
+
+int mps_download_noalloc(mps_ctx* ctx, const char* field, void* out, size_t count) {
+    CHECK_CTX(ctx);
+    if (!field || !out) return MPS_E_ARG;
+    const FieldDesc* fd = findField(field);
+    if (!fd) { ctx->err = std::string("unknown field ") + field; return MPS_E_ARG; }
+    MPS_HIP(hipSetDevice(ctx->dev));
+    const int n = ctx->N;
+    const int comps = fd->kind == 1 ? 3 : 1;
+    if (count < (size_t)n * comps) { ctx->err = "output buffer too small"; return MPS_E_ARG; }
+    const size_t bytes = (fd->kind == 2 ? sizeof(int) : sizeof(double)) * (size_t)n * comps;
+    if (bytes > kScratchBytes) {
+        char msg[200];
+        snprintf(msg, sizeof msg, "mps_download_noalloc: field %s needs %zu bytes of staging, more than the %zu bytes "
+                 "the library holds; use mps_download", field, bytes, kScratchBytes);
+        ctx->err = msg;
+        return MPS_E_CAPACITY;
+    }
+    void* st = nullptr;
+    int rc = scratchAddress(ctx, &st);
+    if (rc) return rc;
+    // canon is a permutation of 0..n-1, so the gather writes every staged entry
+    const int g = gridFor(n, 256);
+    if (fd->kind == 2) {
+        hipLaunchKernelGGL(kToCanonI, dim3(g), dim3(256), 0, ctx->stream, n, ctx->I[ICANON], ctx->I[fd->f0], (int*)st);
+    } else {
+        const double* s0 = ctx->D[fd->f0];
+        const double* s1 = comps == 3 ? ctx->D[fd->f0 + 1] : s0;
+        const double* s2 = comps == 3 ? ctx->D[fd->f0 + 2] : s0;
+        hipLaunchKernelGGL(kToCanonD, dim3(g), dim3(256), 0, ctx->stream, n, ctx->I[ICANON], s0, s1, s2, (double*)st,
+                           comps);
+    }
+    MPS_HIP(hipGetLastError());
+    MPS_HIP(hipMemcpyAsync(out, st, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    MPS_HIP(hipStreamSynchronize(ctx->stream));
     return MPS_OK;
 }
 

@@ -62,6 +62,7 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     lib.mps_get_params.argtypes = [vp, P]
     lib.mps_upload_state.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.mps_download.argtypes = [vp, cp, vp, ctypes.c_size_t]
+    lib.mps_download_noalloc.argtypes = [vp, cp, vp, ctypes.c_size_t]
     lib.mps_upload_field.argtypes = [vp, cp, vp, ctypes.c_size_t]
     lib.mps_get_scalar.argtypes = [vp, cp, dp]
     lib.mps_set_scalar.argtypes = [vp, cp, ctypes.c_double]
