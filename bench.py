@@ -305,6 +305,9 @@ def main(argv=None):
                    "particles": n, "precision": args.precision, "solver_tol": w.params.solver_tol,
                    "ppe_warm_start": args.warm_start, "mixed_inner_tol": w.params.mixed_inner_tol,
                    "dt": w.params.time_step},
+        "sizes": {"particles": n, "nnz": sim.scalar("nnz"), "nnz_escaped": sim.scalar("nnz_escaped"),
+                  "list_entries": sim.scalar("list_entries"), "list_entries_small": sim.scalar("list_entries_small"),
+                  "buckets": sim.params.num_buckets_xyz},
         "ppe": {"last_solve_ms": sim.scalar("ppe_ms"), "last_iterations": sim.scalar("solver_iter"),
                 "last_outer_iterations": sim.scalar("outer_iterations"),
                 "true_residual": sim.scalar("true_residual"), "spmv_launches": launches},

@@ -90,7 +90,7 @@ int mps_download(mps_ctx* ctx, const char* field, void* out, size_t count);
 int mps_download_noalloc(mps_ctx* ctx, const char* field, void* out, size_t count);
 /* Overwrite one field (reference order), same names as mps_download. */
 int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t count);
-/* Scalars: "vel_max" "cfl" "time" "solver_iter" "solver_err" "num_particles"
+/* Scalars: "vel_max" "cfl" "cfl_visc" (granular phase, src/MpsViscosity.cpp:691) "time" "solver_iter" "solver_err" "num_particles"
  * "iterations" "ppe_ms" "true_residual" "fallback" "outer_iterations"
  * "solver_failed" (last pressure solve: 0 converged, 1 stopped above the
  * tolerance, 2 non-finite residual) "solver_failures" (failed solves since
@@ -99,6 +99,8 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
  * "spmv_launches" "spmv_bytes_total" "list_skin" (skin of the last step's
  * kept neighbour list, in l0; 0: built after the prediction) "list_rebuilt"
  * (1 when the last step rebuilt its list after the prediction)
+ * "list_entries" "list_entries_small" (entries of the current neighbour
+ * lists at the large and the small radius, summed over the particles)
  * "list_patched" (particles whose list the last step rebuilt alone).
  * Settable: "vel_max" "time" "list_skin" (< 0 automatic, 0 off, > 0 a fixed
  * skin in l0; every choice gives the same results) "list_skin_cap" (largest
@@ -239,6 +241,19 @@ int mps_prediction_wall_press_gradient(mps_ctx* ctx);
 int mps_press_gradient(mps_ctx* ctx);
 /* MpsPressure::calcWallPressGradient (src/MpsPressure.cpp:1341-1550) */
 int mps_wall_press_gradient(mps_ctx* ctx);
+/* Granular (non-Newtonian) phase, fluid_type 1; the particles of type 1 in
+ * the state uploaded by mps_upload_state are its particles (fluid of phase
+ * 2, src/MpsInputOutput.cpp:839-855).
+ * MpsViscosity::calcVolumeFraction (src/MpsViscosity.cpp:104-191) */
+int mps_volume_fraction(mps_ctx* ctx);
+/* MpsViscosity::calcViscosityInteractionVal (src/MpsViscosity.cpp:194-700):
+ * bed levels, strain rate, mu(I) rheology (MEU, RHO, ...).  The reference
+ * fills its bed-level grid from an unsynchronised parallel loop; the result
+ * here is that of the loop in index order. */
+int mps_viscosity_interaction(mps_ctx* ctx);
+/* MpsViscosity::calcWall{Slip,NoSlip}ViscosityInteractionVal
+ * (src/MpsViscosity.cpp:703-1526), polygon walls in 3-D */
+int mps_wall_viscosity_interaction(mps_ctx* ctx);
 /* MpsViscosity::calcWall{Slip,NoSlip}Viscosity (src/MpsViscosity.cpp:1527-1910) */
 int mps_wall_viscosity(mps_ctx* ctx);
 /* MpsParticleVelPos::correctionVelocityPosition (src/MpsParticleVelPos.cpp:53-97) */

@@ -55,6 +55,20 @@ typedef struct mps_params {
     double knm_vs2;             /* rheological.kinematic_visc_phase_2 */
     double dns_fl1;             /* rheological.fluid_density_phase_1  */
     double dns_fl2;             /* rheological.fluid_density_phase_2  */
+    /* rheological parameters of the non-Newtonian (granular) phase,
+       fluid_type 1 (src/MpsInputOutput.cpp:266-314) */
+    double dns_sdt;             /* rheological.sediment_density (DNS_SDT) */
+    double power_n;             /* rheological.power_law_index (N)        */
+    double meu0;                /* rheological.consistency_index (MEU0)   */
+    double phi_1, phi_wal, phi_bed, phi_2;   /* rheological.phi.lower / wall / bed / second */
+    double cohes;               /* rheological.cohes_coeff                */
+    double dg;                  /* rheological.grain_size                 */
+    double i0;                  /* rheological.I0                         */
+    double mm;                  /* rheological.mm                         */
+    double cd;                  /* rheological.drag_coeff                 */
+    double vf_min, vf_max;      /* rheological.volume_fraction.min / max  */
+    int fraction_method;        /* rheological.fraction_method: 1 neighbour count, 2 kernel weighted */
+    int stress_calc_method;     /* rheological.stress_calc_method (read, not used by the reference) */
     /* ---- JSON "numerical" ---- */
     double dim;                 /* dimension                          */
     double part_dist;           /* particle_dist  (l0)                */

@@ -63,6 +63,11 @@ struct Sys {
     std::vector<char> nearWall;
     std::vector<double> press, pressAvg, pndi, pndki, pndski, pndSmall, npcd2, conc,
         divU, diffTerm, aux1, aux2, pndWall, devDotN, nNeighSurf, riw2, RHO, MEU;
+    // non-Newtonian (granular) phase, fluid_type 1: volume fraction, second
+    // invariant of the strain rate, yield viscosity, inertial number, normal
+    // stress, smoothed pressure, grain volume fraction, strain-rate tensor
+    std::vector<double> Cv, II, MEUY, inertia, pRheo, pSmooth, VF, S11, S12, S13, S22, S23, S33;
+    double cflVisc = 0.0;
     // vectors (AoS, 3 per particle)
     std::vector<double> pos, vel, acc, accStar, npcd, gradC, cm1, cm2, cm3, normal, dvel,
         wallPos, mirror, wpf1, wpf2, polyN, forceWall;
@@ -525,6 +530,393 @@ void calcWallViscosity(Sys& s) {
     }
 }
 
+// ------------------------------------------- non-Newtonian (granular) phase
+// MpsViscosity::calcVolumeFraction (src/MpsViscosity.cpp:104-191): share of
+// phase-2 (PTYPE >= 2) fluid neighbours within re_S, counted (method 1) or
+// kernel weighted (method 2).
+void volumeFraction(Sys& s) {
+    const mps_params& P = s.p;
+    if (P.fraction_method != 1 && P.fraction_method != 2) return;
+    const bool weighted = P.fraction_method == 2;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        if (s.type[i] != P.fluid) continue;
+        double sum1 = 0.0, sum2 = 0.0;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            if (d2 < P.re_s2) {
+                if (j != i && s.type[j] == P.fluid) {
+                    if (weighted) {
+                        double wS = weight(sqrt(d2), P.re_s, P.weight_type);
+                        sum1 += wS;
+                        if (s.ptype[j] >= 2) sum2 += wS;
+                    } else {
+                        sum1 += 1.0;
+                        if (s.ptype[j] >= 2) sum2 += 1.0;
+                    }
+                }
+            }
+        });
+        s.Cv[i] = (sum1 < kEps) ? 0.0 : sum2 / sum1;
+    }
+}
+
+// Bed level BL, water level WL and the pressure PS of the bed-level particle
+// on a horizontal grid of 2 l0 columns (src/MpsViscosity.cpp:236-305).  The
+// reference fills it from a parallel loop without synchronisation; the
+// defined result is that of the loop in index order: BL is the highest
+// phase-2-rich (Cv > 0.5) fluid particle of a column, PS the pressure of the
+// lowest index among the particles at that height, WL the highest phase-1
+// particle; a column nobody raises keeps its initial level and PS = 0.
+// Particles outside the grid (an out-of-bounds write in the reference) are
+// left out.  wallForm: the polygon-wall variants' grid (3-D only, :766-783).
+struct BedGrid {
+    int kxMax = 0, kyMax = 0;
+    double xmin = 0, ymin = 0, aa = 0, level0 = 0;
+    std::vector<double> BL, WL, PS;
+    // column index (1-based as the reference's arrays), -1 outside
+    int cell(int kx, int ky) const {
+        if (kx < 1 || kx > kxMax || ky < 1 || ky > kyMax) return -1;
+        return (kx - 1) * kyMax + (ky - 1);
+    }
+};
+
+static BedGrid bedGrid(const Sys& s, bool wallForm) {
+    const mps_params& P = s.p;
+    BedGrid g;
+    const bool d3 = (int)P.dim == 3;
+    const double Xmin = P.domain_min[0], Xmax = P.domain_max[0], Ymin = P.domain_min[1];
+    const double Ymax = d3 ? P.domain_max[1] : Ymin;
+    const double Zmin = d3 ? P.domain_min[2] : 0.0;
+    g.aa = 2.0 * P.part_dist;
+    g.kxMax = int((Xmax - Xmin) / g.aa) + 1;
+    g.kyMax = int((Ymax - Ymin) / g.aa) + 1;
+    g.xmin = Xmin;
+    g.ymin = Ymin;
+    g.level0 = (d3 || wallForm) ? Zmin : Ymin;
+    const size_t nc = (size_t)g.kxMax * g.kyMax;
+    g.BL.assign(nc, g.level0);
+    g.WL.assign(nc, g.level0);
+    g.PS.assign(nc, 0.0);
+    for (int i = 0; i < s.N; i++) {
+        if (s.type[i] != P.fluid) continue;
+        const double x = s.pos[i * 3], y = s.pos[i * 3 + 1], z = s.pos[i * 3 + 2];
+        const int kx = int((x - Xmin) / g.aa) + 1;
+        const int ky = d3 ? int((y - Ymin) / g.aa) + 1 : 1;
+        const int c = g.cell(kx, ky);
+        if (c < 0) continue;
+        const double h = d3 ? z : y;
+        if (h > g.BL[c] && s.Cv[i] > 0.5) { g.BL[c] = h; g.PS[c] = s.press[i]; }
+        if (h > g.WL[c] && s.ptype[i] == 1) g.WL[c] = h;
+    }
+    return g;
+}
+
+// Strain-rate tensor, its second invariant and the smoothed pressure of the
+// fluid particles (src/MpsViscosity.cpp:307-440).
+static void strainRate(Sys& s) {
+    const mps_params& P = s.p;
+    const bool pw = partWall(s);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        if (s.type[i] != P.fluid) continue;
+        double sum1 = 0.0, sum2 = 0.0, sum3 = 0.0, sum4 = 0.0, sum5 = 0.0, sum6 = 0.0, sum7 = 0.0, sum8 = 0.0,
+               sum9 = 0.0, sum10 = 0.0, sumWs = 0.0;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double ux = s.vel[i * 3], uy = s.vel[i * 3 + 1], uz = s.vel[i * 3 + 2];
+        const double mx = s.mirror[i * 3], my = s.mirror[i * 3 + 1], mz = s.mirror[i * 3 + 2];
+        const double MC[9] = {s.cm1[i * 3], s.cm1[i * 3 + 1], s.cm1[i * 3 + 2], s.cm2[i * 3], s.cm2[i * 3 + 1],
+                              s.cm2[i * 3 + 2], s.cm3[i * 3], s.cm3[i * 3 + 1], s.cm3[i * 3 + 2]};
+        walk(s, xi, yi, zi, [&](int j) {
+            double v0 = PJ(s, j, 0) - xi, v1 = PJ(s, j, 1) - yi, v2 = PJ(s, j, 2) - zi;
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = PJ(s, j, 0) - mx, w1 = PJ(s, j, 1) - my, w2 = PJ(s, j, 2) - mz;
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && (d2 < dm2 || pw)) {
+                if (j != i) {
+                    double wS = weight(sqrt(d2), P.re_s, P.weight_type);
+                    double ex = s.vel[j * 3] - ux, ey = s.vel[j * 3 + 1] - uy, ez = s.vel[j * 3 + 2] - uz;
+                    double inv = 1.0 / d2;
+                    if (!P.gradient_correction) {
+                        sum1 += ex * v0 * wS * inv; sum2 += ex * v1 * wS * inv; sum3 += ex * v2 * wS * inv;
+                        sum4 += ey * v0 * wS * inv; sum5 += ey * v1 * wS * inv; sum6 += ey * v2 * wS * inv;
+                        sum7 += ez * v0 * wS * inv; sum8 += ez * v1 * wS * inv; sum9 += ez * v2 * wS * inv;
+                    } else {
+                        double c0 = (v0 * MC[0] + v1 * MC[1] + v2 * MC[2]);
+                        double c1 = (v0 * MC[3] + v1 * MC[4] + v2 * MC[5]);
+                        double c2 = (v0 * MC[6] + v1 * MC[7] + v2 * MC[8]);
+                        sum1 += ex * c0 * wS * inv; sum2 += ex * c1 * wS * inv; sum3 += ex * c2 * wS * inv;
+                        sum4 += ey * c0 * wS * inv; sum5 += ey * c1 * wS * inv; sum6 += ey * c2 * wS * inv;
+                        sum7 += ez * c0 * wS * inv; sum8 += ez * c1 * wS * inv; sum9 += ez * c2 * wS * inv;
+                    }
+                    sum10 += s.press[j] * wS;
+                    sumWs += wS;
+                }
+            }
+        });
+        const double c = P.coeff_press_grad;
+        const double Uxx = -c * sum1, Uxy = -c * sum2, Uxz = -c * sum3;
+        const double Uyx = -c * sum4, Uyy = -c * sum5, Uyz = -c * sum6;
+        const double Uzx = -c * sum7, Uzy = -c * sum8, Uzz = -c * sum9;
+        s.pSmooth[i] = (sumWs > kEps) ? sum10 / sumWs : s.press[i];
+        if (s.pSmooth[i] < kEps) s.pSmooth[i] = 0.0;
+        s.S11[i] = 0.5 * (Uxx + Uxx);
+        s.S12[i] = 0.5 * (Uxy + Uyx);
+        s.S13[i] = 0.5 * (Uxz + Uzx);
+        s.S22[i] = 0.5 * (Uyy + Uyy);
+        s.S23[i] = 0.5 * (Uyz + Uzy);
+        s.S33[i] = 0.5 * (Uzz + Uzz);
+        double ii = 0.5 * (s.S11[i] * s.S11[i] + 2.0 * s.S12[i] * s.S12[i] + 2.0 * s.S13[i] * s.S13[i] +
+                           s.S22[i] * s.S22[i] + 2.0 * s.S23[i] * s.S23[i] + s.S33[i] * s.S33[i]);
+        if (ii < kEps || ii * 0 != 0) ii = 0.0;
+        s.II[i] = ii;
+    }
+}
+
+// Granular rheology of the fluid particles (src/MpsViscosity.cpp:461-691):
+// mu(I) with a Drucker-Prager yield stress; sets MEU, MEU_Y, inertia, VF, RHO,
+// the normal stress and CFLvisc.  The reference runs this loop serially.
+static void rheologyFluid(Sys& s, const BedGrid& g) {
+    const mps_params& P = s.p;
+    const double gmod = sqrt(P.gravity[0] * P.gravity[0] + P.gravity[1] * P.gravity[1] + P.gravity[2] * P.gravity[2]);
+    const bool d3 = (int)P.dim == 3;
+    const double f21 = P.dns_fl2 - P.dns_fl1;
+    double miMax = 0.0;
+    for (int i = 0; i < s.N; i++) {
+        if (s.type[i] != P.fluid) continue;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double vx = s.vel[i * 3], vy = s.vel[i * 3 + 1], vz = s.vel[i * 3 + 2];
+        const double vel2i = vx * vx + vy * vy + vz * vz;
+        if (s.ptype[i] == 1) {
+            s.MEU[i] = P.knm_vs1 * P.dns_fl1;
+        } else if (s.ptype[i] == 2) {
+            double phi = (s.Cv[i] - 0.25) * P.phi_1 / (1.0 - 0.25);
+            double phi2 = (s.Cv[i] - 0.25) * P.phi_2 / (1.0 - 0.25);
+            if (s.Cv[i] <= 0.25) { phi = kEps; phi2 = kEps; }
+            s.pRheo[i] = s.pSmooth[i];
+            const int kx = int((xi - g.xmin) / g.aa) + 1;
+            const int ky = d3 ? int((yi - g.ymin) / g.aa) + 1 : 1;
+            const int c = g.cell(kx, ky);
+            const double BL = c < 0 ? g.level0 : g.BL[c];
+            const double WL = c < 0 ? g.level0 : g.WL[c];
+            const double PS = c < 0 ? 0.0 : g.PS[c];
+            double ns;
+            if (!d3) {
+                ns = (BL - yi + P.part_dist * 0.5) * f21 * gmod - vel2i * f21 * 0.5;
+                if (WL < BL) {
+                    ns = s.pSmooth[i];
+                } else {
+                    // the reference first forms (p_smooth - PS) f21 VF / RHO and overwrites it
+                    ns = s.pSmooth[i] - PS;
+                }
+            } else {
+                ns = (BL - zi + P.part_dist * 0.5) * f21 * gmod - vel2i * f21 * 0.5;
+                if (s.pSmooth[i] - (WL - zi) * P.dns_fl1 * gmod < kEps) s.pSmooth[i] = (WL - zi) * P.dns_fl1 * gmod;
+                if (s.timeCurrent <= 1.0)
+                    ns = (1.0 - s.timeCurrent) * (s.pSmooth[i] - (WL - zi) * P.dns_fl1 * gmod) + s.timeCurrent * ns;
+            }
+            if (ns < 1.0 || s.Cv[i] < 0.5) ns = 1.0;
+            s.pRheo[i] = ns;
+            const double modE = sqrt(s.II[i]);
+            if (WL < BL) s.inertia[i] = modE * P.dg / sqrt(ns / P.dns_sdt);
+            else s.inertia[i] = modE * P.dg / sqrt(ns / (P.dns_fl1 * P.cd));
+            s.VF[i] = P.vf_max - (P.vf_max - P.vf_min) * s.inertia[i];
+            if (s.VF[i] < P.vf_min) s.VF[i] = P.vf_min;
+            s.RHO[i] = P.dns_sdt * s.VF[i] + (1.0 - s.VF[i]) * P.dns_fl1;
+            phi *= (s.VF[i] / P.vf_max);
+            const double alpha1 = 2.0 * sqrt(3.0) * sin(phi) / (3.0 - sin(phi));
+            const double beta1 = 2.0 * sqrt(3.0) * cos(phi) / (3.0 - sin(phi));
+            double yield = P.cohes * beta1 + ns * alpha1;
+            if (yield < kEps) yield = 0.0;
+            double viscMax = (P.meu0 + yield * P.mm * 0.5);
+            s.MEUY[i] = yield / (2.0 * sqrt(modE * modE + kEps));
+            double meu0;
+            if (WL < BL)
+                meu0 = 0.5 * (tan(phi2) - tan(phi)) * ns * P.dg / (P.i0 * sqrt(ns / P.dns_fl2) + modE * P.dg) * modE /
+                       (sqrt(modE * modE + kEps));
+            else
+                meu0 = 0.5 * (tan(phi2) - tan(phi)) * ns * P.dg / (P.i0 * sqrt(ns / (P.dns_fl1 * P.cd)) + modE * P.dg) *
+                       modE / (sqrt(modE * modE + kEps));
+            if (modE <= kEps || (meu0 * 0) != 0) meu0 = P.meu0;
+            viscMax = (meu0 + yield * P.mm * 0.5);
+            s.MEU[i] = s.MEUY[i] + meu0;
+            if (s.II[i] <= kEps) s.MEU[i] = viscMax;
+            if (s.MEU[i] > miMax) miMax = s.MEU[i];
+        }
+        if (s.ptype[i] >= 2) {
+            if (s.Cv[i] > 0.5) s.RHO[i] = P.dns_fl2;
+            else s.RHO[i] = s.Cv[i] * P.dns_fl2 + (1.0 - s.Cv[i]) * P.dns_fl1;
+        }
+    }
+    s.cflVisc = P.time_step * miMax /
+                ((P.dns_sdt * P.vf_max + (1.0 - P.vf_max) * P.dns_fl1) * P.part_dist * P.part_dist);
+}
+
+// MpsViscosity::calcViscosityInteractionVal (src/MpsViscosity.cpp:194-700)
+void viscosityInteractionVal(Sys& s) {
+    BedGrid g = bedGrid(s, false);
+    strainRate(s);
+    rheologyFluid(s, g);
+}
+
+// MpsViscosity::calcWall{Slip,NoSlip}ViscosityInteractionVal
+// (src/MpsViscosity.cpp:703-1110, 1112-1526), polygon walls in 3-D: the
+// mirror-neighbourhood strain rate of the fluid particles near the wall
+// replaces S (II and the smoothed pressure add to the fluid values), then
+// the rheology of those particles with a Papanastasiou-regularised yield
+// viscosity.
+void wallViscosityInteractionVal(Sys& s) {
+    const mps_params& P = s.p;
+    const bool noSlip = P.slip_condition == 0;
+    BedGrid g = bedGrid(s, true);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s.N; i++) {
+        if (!(s.type[i] == P.fluid && s.nearWall[i])) continue;
+        double sum1 = 0.0, sum2 = 0.0, sum3 = 0.0, sum4 = 0.0, sum5 = 0.0, sum6 = 0.0, sum7 = 0.0, sum8 = 0.0,
+               sum9 = 0.0, sum10 = 0.0;
+        const double xi[3] = {s.pos[i * 3], s.pos[i * 3 + 1], s.pos[i * 3 + 2]};
+        const double ux = s.vel[i * 3], uy = s.vel[i * 3 + 1], uz = s.vel[i * 3 + 2];
+        const double xm[3] = {s.mirror[i * 3], s.mirror[i * 3 + 1], s.mirror[i * 3 + 2]};
+        double n[3], R[9];
+        reflectionMod2(xi, xm, n, R);
+        double vmx, vmy, vmz;
+        if (!noSlip) {   // mirror velocity R v (:826-829)
+            vmx = (R[0] * ux + R[1] * uy + R[2] * uz);
+            vmy = (R[3] * ux + R[4] * uy + R[5] * uz);
+            vmz = (R[6] * ux + R[7] * uy + R[8] * uz);
+        } else {         // -(v - 2 (v_wall - (n.v_wall) n)) (:1239-1259)
+            double vw[3] = {0.0, 0.0, 0.0};
+            if (s.nearMeshType[i] == 2)
+                for (int k = 0; k < 3; k++) vw[k] = P.uniform_vel_wall[k];
+            const double dotnv = n[0] * vw[0] + n[1] * vw[1] + n[2] * vw[2];
+            const double t0 = ux - 2.0 * (vw[0] - dotnv * n[0]);
+            const double t1 = uy - 2.0 * (vw[1] - dotnv * n[1]);
+            const double t2 = uz - 2.0 * (vw[2] - dotnv * n[2]);
+            vmx = (-1.0 * t0 + 0.0 * t1 + 0.0 * t2);
+            vmy = (0.0 * t0 + -1.0 * t1 + 0.0 * t2);
+            vmz = (0.0 * t0 + 0.0 * t1 + -1.0 * t2);
+        }
+        walk(s, xi[0], xi[1], xi[2], [&](int j) {
+            double v0 = PJ(s, j, 0) - xi[0], v1 = PJ(s, j, 1) - xi[1], v2 = PJ(s, j, 2) - xi[2];
+            double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+            double w0 = PJ(s, j, 0) - xm[0], w1 = PJ(s, j, 1) - xm[1], w2 = PJ(s, j, 2) - xm[2];
+            double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
+            if (d2 < P.re_s2 && dm2 < P.re_s2 && d2 < dm2) {
+                if (j != i) {
+                    double wS = weight(sqrt(dm2), P.re_s, P.weight_type);
+                    double ex = s.vel[j * 3] - vmx, ey = s.vel[j * 3 + 1] - vmy, ez = s.vel[j * 3 + 2] - vmz;
+                    sum1 += ex * w0 * wS / dm2; sum2 += ex * w1 * wS / dm2; sum3 += ex * w2 * wS / dm2;
+                    sum4 += ey * w0 * wS / dm2; sum5 += ey * w1 * wS / dm2; sum6 += ey * w2 * wS / dm2;
+                    sum7 += ez * w0 * wS / dm2; sum8 += ez * w1 * wS / dm2; sum9 += ez * w2 * wS / dm2;
+                    sum10 += s.press[j] * wS;
+                }
+            }
+        });
+        const double c = P.coeff_press_grad;
+        double Uxx, Uxy, Uxz, Uyx, Uyy, Uyz, Uzx, Uzy, Uzz;
+        if (!noSlip) {   // R grad(u) R (:878-905)
+            const double aUxx = -c * (R[0] * sum1 + R[1] * sum4 + R[2] * sum7);
+            const double aUxy = -c * (R[0] * sum2 + R[1] * sum5 + R[2] * sum8);
+            const double aUxz = -c * (R[0] * sum3 + R[1] * sum6 + R[2] * sum9);
+            const double aUyx = -c * (R[3] * sum1 + R[4] * sum4 + R[5] * sum7);
+            const double aUyy = -c * (R[3] * sum2 + R[4] * sum5 + R[5] * sum8);
+            const double aUyz = -c * (R[3] * sum3 + R[4] * sum6 + R[5] * sum9);
+            const double aUzx = -c * (R[6] * sum1 + R[7] * sum4 + R[8] * sum7);
+            const double aUzy = -c * (R[6] * sum2 + R[7] * sum5 + R[8] * sum8);
+            const double aUzz = -c * (R[6] * sum3 + R[7] * sum6 + R[8] * sum9);
+            Uxx = aUxx * R[0] + aUxy * R[3] + aUxz * R[6];
+            Uxy = aUxx * R[1] + aUxy * R[4] + aUxz * R[7];
+            Uxz = aUxx * R[2] + aUxy * R[5] + aUxz * R[8];
+            Uyx = aUyx * R[0] + aUyy * R[3] + aUyz * R[6];
+            Uyy = aUyx * R[1] + aUyy * R[4] + aUyz * R[7];
+            Uyz = aUyx * R[2] + aUyy * R[5] + aUyz * R[8];
+            Uzx = aUzx * R[0] + aUzy * R[3] + aUzz * R[6];
+            Uzy = aUzx * R[1] + aUzy * R[4] + aUzz * R[7];
+            Uzz = aUzx * R[2] + aUzy * R[5] + aUzz * R[8];
+        } else {         // -grad(u) R (:1310-1320)
+            Uxx = c * (sum1 * R[0] + sum2 * R[3] + sum3 * R[6]);
+            Uxy = c * (sum1 * R[1] + sum2 * R[4] + sum3 * R[7]);
+            Uxz = c * (sum1 * R[2] + sum2 * R[5] + sum3 * R[8]);
+            Uyx = c * (sum4 * R[0] + sum5 * R[3] + sum6 * R[6]);
+            Uyy = c * (sum4 * R[1] + sum5 * R[4] + sum6 * R[7]);
+            Uyz = c * (sum4 * R[2] + sum5 * R[5] + sum6 * R[8]);
+            Uzx = c * (sum7 * R[0] + sum8 * R[3] + sum9 * R[6]);
+            Uzy = c * (sum7 * R[1] + sum8 * R[4] + sum9 * R[7]);
+            Uzz = c * (sum7 * R[2] + sum8 * R[5] + sum9 * R[8]);
+        }
+        s.pSmooth[i] += sum10 / P.pnd_gradient_zero;
+        if (s.pSmooth[i] < kEps) s.pSmooth[i] = 0.0;
+        s.S11[i] = 0.5 * (Uxx + Uxx);
+        s.S12[i] = 0.5 * (Uxy + Uyx);
+        s.S13[i] = 0.5 * (Uxz + Uzx);
+        s.S22[i] = 0.5 * (Uyy + Uyy);
+        s.S23[i] = 0.5 * (Uyz + Uzy);
+        s.S33[i] = 0.5 * (Uzz + Uzz);
+        s.II[i] += 0.5 * (s.S11[i] * s.S11[i] + 2.0 * s.S12[i] * s.S12[i] + 2.0 * s.S13[i] * s.S13[i] +
+                          s.S22[i] * s.S22[i] + 2.0 * s.S23[i] * s.S23[i] + s.S33[i] * s.S33[i]);
+        if (s.II[i] < kEps || s.II[i] * 0 != 0) s.II[i] = 0.0;
+    }
+    const double gmod = sqrt(P.gravity[0] * P.gravity[0] + P.gravity[1] * P.gravity[1] + P.gravity[2] * P.gravity[2]);
+    const double f21 = P.dns_fl2 - P.dns_fl1;
+    for (int i = 0; i < s.N; i++) {
+        if (!(s.type[i] == P.fluid && s.nearWall[i])) continue;
+        const double xi = s.pos[i * 3], yi = s.pos[i * 3 + 1], zi = s.pos[i * 3 + 2];
+        const double vx = s.vel[i * 3], vy = s.vel[i * 3 + 1], vz = s.vel[i * 3 + 2];
+        const double vel2i = vx * vx + vy * vy + vz * vz;
+        if (s.ptype[i] == 1) {
+            s.MEU[i] = P.knm_vs1 * P.dns_fl1;
+        } else if (s.ptype[i] == 2) {
+            double phi = (s.Cv[i] - 0.25) * P.phi_1 / (1.0 - 0.25);
+            double phi2 = (s.Cv[i] - 0.25) * P.phi_2 / (1.0 - 0.25);
+            if (s.Cv[i] <= 0.25) { phi = kEps; phi2 = kEps; }
+            s.pRheo[i] = s.pSmooth[i];
+            const int kx = int((xi - g.xmin) / g.aa) + 1;
+            const int ky = int((yi - g.ymin) / g.aa) + 1;
+            const int c = g.cell(kx, ky);
+            const double BL = c < 0 ? g.level0 : g.BL[c];
+            const double WL = c < 0 ? g.level0 : g.WL[c];
+            double ns = (BL - zi + P.part_dist * 0.5) * f21 * gmod - vel2i * f21 * 0.5;
+            if (s.pSmooth[i] < (WL - zi) * P.dns_fl1 * gmod) s.pSmooth[i] = (WL - zi) * P.dns_fl1 * gmod;
+            if (s.timeCurrent <= 1.0)
+                ns = (1.0 - s.timeCurrent) * (s.pSmooth[i] - (WL - zi) * P.dns_fl1 * gmod) + s.timeCurrent * ns;
+            if (ns < 1.0 || s.Cv[i] < 0.5) ns = 1.0;
+            s.pRheo[i] = ns;
+            s.inertia[i] = sqrt(s.II[i]) * P.dg / sqrt(ns / (P.dns_fl1 * P.cd));
+            s.VF[i] = P.vf_max - (P.vf_max - P.vf_min) * s.inertia[i];
+            if (s.VF[i] < P.vf_min) s.VF[i] = P.vf_min;
+            s.RHO[i] = P.dns_sdt * s.VF[i] + (1.0 - s.VF[i]) * P.dns_fl1;
+            phi = phi * s.VF[i] / P.vf_max;
+            double yield = P.cohes * cos(phi) + ns * sin(phi);
+            if (noSlip ? yield < 0.0 : yield < kEps) yield = 0.0;
+            double viscMax = (yield * P.mm * 0.5 + P.meu0);
+            if (s.II[i] > kEps)
+                s.MEUY[i] = yield * (1.0 - exp(-P.mm * sqrt(s.II[i]))) * 0.5 / sqrt(s.II[i]);
+            else
+                s.MEUY[i] = viscMax;
+            double meu0 = 0.5 * (tan(phi2) - tan(phi)) * ns * P.dg /
+                          (P.i0 * sqrt(ns / (P.dns_fl1 * P.cd)) + sqrt(s.II[i]) * P.dg);
+            if (s.II[i] <= kEps || (meu0 * 0) != 0) meu0 = P.meu0;
+            viscMax = (yield * P.mm * 0.5 + meu0);
+            s.MEU[i] = s.MEUY[i] + P.meu0 * pow(4.0 * s.II[i], (P.power_n - 1.0) * 0.5);
+            if (s.II[i] == 0 || s.MEU[i] > viscMax) s.MEU[i] = viscMax;
+        }
+        if (s.ptype[i] >= 2) {
+            if (s.Cv[i] > 0.5) s.RHO[i] = P.dns_fl2;
+            else s.RHO[i] = s.Cv[i] * P.dns_fl2 + (1.0 - s.Cv[i]) * P.dns_fl1;
+        }
+    }
+}
+
+// main.cpp's non-Newtonian calls after the bucket update (src/main.cpp:259-262)
+// and before the wall viscosity (:387-398)
+void nonNewtonian(Sys& s, bool withWall) {
+    volumeFraction(s);
+    viscosityInteractionVal(s);
+    if (withWall && s.p.wall_type == MPS_WALL_POLYGON) wallViscosityInteractionVal(s);
+}
+
 // ------------------------------------------------ MpsParticle / VelPos
 // MpsParticle::addGravity (src/MpsParticle.cpp:111-130)
 void addGravity(Sys& s) {
@@ -642,6 +1034,12 @@ void checkOutDomain(Sys& s) {
             s.pndWall[i] = s.pndWall[L]; s.devDotN[i] = s.devDotN[L]; s.nNeighSurf[i] = s.nNeighSurf[L];
             s.riw2[i] = s.riw2[L]; s.ptype[i] = s.ptype[L]; s.RHO[i] = s.RHO[L]; s.MEU[i] = s.MEU[L];
             s.elementID[i] = s.elementID[L];
+            if (P.fluid_type == 1) {   // src/MpsParticle.cpp:322-337
+                s.Cv[i] = s.Cv[L]; s.II[i] = s.II[L]; s.MEUY[i] = s.MEUY[L]; s.inertia[i] = s.inertia[L];
+                s.pRheo[i] = s.pRheo[L]; s.pSmooth[i] = s.pSmooth[L]; s.VF[i] = s.VF[L];
+                s.S12[i] = s.S12[L]; s.S13[i] = s.S13[L]; s.S23[i] = s.S23[L];
+                s.S11[i] = s.S11[L]; s.S22[i] = s.S22[L]; s.S33[i] = s.S33[L];
+            }
             if (P.mps_type == MPS_IMPLICIT_PND || P.mps_type == MPS_IMPLICIT_PND_DIVU) {
                 s.ppeX[i] = s.ppeX[L];
                 s.ppeB[i] = s.ppeB[L];
@@ -2408,10 +2806,11 @@ int initSystem(Sys& s) {
     return 0;
 }
 
-// mainLoopOfSimulation body (src/main.cpp:252-467), Newtonian fluid
+// mainLoopOfSimulation body (src/main.cpp:252-467)
 int stepSystem(Sys& s) {
     const mps_params& P = s.p;
     updateParticlesID(s);
+    if (P.fluid_type == 1) nonNewtonian(s, false);
     calcViscosity(s);
     addGravity(s);
     if (P.relax_press < 1.0) {
@@ -2444,6 +2843,7 @@ int stepSystem(Sys& s) {
     if (P.gradient_correction) correctionMatrix(s);
     pressGradient(s);
     if (P.wall_type == MPS_WALL_POLYGON) wallPressGradient(s);
+    if (P.fluid_type == 1) nonNewtonian(s, true);
     if (P.wall_type == MPS_WALL_POLYGON) calcWallViscosity(s);
     // src/main.cpp:410-414
     if (P.forced_on && forcedMeshUpdate(s, P.time_step, s.timeCurrent)) return -1;
@@ -2480,6 +2880,9 @@ void* fieldPtr(Sys& s, const std::string& f, int& kind, int& comps) {
     D1("ppe_x", s.ppeX) D1("ppe_b", s.ppeB) D1("dev_dot_n", s.devDotN)
     I1("type", s.type) I1("bc", s.bc) I1("num_neigh", s.numNeigh) I1("num_neigh_wall", s.numNeighWall)
     I1("ptype", s.ptype) I1("near_mesh_type", s.nearMeshType)
+    D1("rho", s.RHO) D1("meu", s.MEU) D1("cv", s.Cv) D1("ii", s.II) D1("meu_y", s.MEUY) D1("inertia", s.inertia)
+    D1("p_rheo", s.pRheo) D1("p_smooth", s.pSmooth) D1("vf", s.VF) D1("s11", s.S11) D1("s12", s.S12)
+    D1("s13", s.S13) D1("s22", s.S22) D1("s23", s.S23) D1("s33", s.S33)
     if (f == "near_wall") { kind = 2; comps = 1; return s.nearWall.data(); }
 #undef D1
 #undef D3
@@ -2514,6 +2917,9 @@ void* orc_create(const mps_params* p, int n, const int* type, const double* pos,
     s->ptype.assign(n, 1);
     s->RHO.assign(n, p->dns_fl1);
     s->MEU.assign(n, p->knm_vs1 * p->dns_fl1);
+    for (auto* v : {&s->Cv, &s->II, &s->MEUY, &s->inertia, &s->pRheo, &s->pSmooth, &s->VF, &s->S11, &s->S12, &s->S13,
+                    &s->S22, &s->S23, &s->S33})
+        v->assign(n, 0.0);
     s->ppeX.assign(n, 0.0);
     s->ppeB.assign(n, 0.0);
     if (deriveParams(*s) != 0) {
@@ -2521,9 +2927,19 @@ void* orc_create(const mps_params* p, int n, const int* type, const double* pos,
         delete s;
         return nullptr;
     }
-    // RHO / MEU use DNS_FL1 of the derived parameter block (readMpsParticleFile :833-837)
+    // RHO / MEU use DNS_FL1 of the derived parameter block (readMpsParticleFile :833-837);
+    // with the granular phase, particles of type 1 in the file are its fluid
+    // particles (PTYPE 2, density and viscosity of phase 2, :839-855)
     s->RHO.assign(n, s->p.dns_fl1);
     s->MEU.assign(n, s->p.knm_vs1 * s->p.dns_fl1);
+    if (s->p.fluid_type == 1)
+        for (int i = 0; i < n; i++)
+            if (s->type[i] == 1) {
+                s->type[i] = 0;
+                s->ptype[i] = 2;
+                s->RHO[i] = s->p.dns_fl2;
+                s->MEU[i] = s->p.knm_vs2 * s->p.dns_fl2;
+            }
     s->first.assign(s->p.num_buckets_xyz, -1);
     s->last.assign(s->p.num_buckets_xyz, -1);
     s->next.assign(n, -1);
@@ -2613,6 +3029,7 @@ int orc_scalar(void* h, const char* name, double* out) {
     else if (f == "solver_iter") *out = s->solverIter;
     else if (f == "solver_err") *out = s->solverErr;
     else if (f == "num_particles") *out = s->N;
+    else if (f == "cfl_visc") *out = s->cflVisc;
     else if (f == "solve_setup_ms") *out = s->solveSetupSeconds * 1e3;
     else if (f == "solve_iter_ms") *out = s->solveIterSeconds * 1e3;
     else return -1;
@@ -2673,6 +3090,9 @@ int orc_call(void* h, const char* op) {
     std::string o(op);
     if (o == "update_buckets") updateParticlesID(*s);
     else if (o == "viscosity") calcViscosity(*s);
+    else if (o == "volume_fraction") volumeFraction(*s);
+    else if (o == "viscosity_interaction") viscosityInteractionVal(*s);
+    else if (o == "wall_viscosity_interaction") wallViscosityInteractionVal(*s);
     else if (o == "add_gravity") addGravity(*s);
     else if (o == "prediction") prediction(*s);
     else if (o == "check_out_domain") checkOutDomain(*s);

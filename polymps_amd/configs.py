@@ -535,6 +535,69 @@ def small_dam2d(l0: float = 0.02, **kw) -> Workload:
     return dam2d_wc(l0=l0, **kw)
 
 
+def _granular_params(p, phi=(0.4, 0.611), mm=300.0, vf_max=0.69):
+    """Rheology of the reference's granular dam-break inputs
+    (input/2D_dam_fraccarollo_*.json, input/damErosion3D_WC.json)."""
+    p.fluid_type = 1
+    p.knm_vs2 = 1e-6
+    p.dns_fl1, p.dns_fl2, p.dns_sdt = 1000.0, 1540.0, 1540.0
+    p.power_n, p.meu0 = 1.2, 0.03
+    p.phi_1 = p.phi_wal = p.phi_bed = phi[0]
+    p.phi_2 = phi[1]
+    p.cohes, p.fraction_method = 0.0, 2
+    p.dg, p.i0, p.mm, p.cd = 0.0035, 0.75, mm, 0.47
+    p.vf_min, p.vf_max = 0.25, vf_max
+
+
+def granular_dam2d(l0: float = 0.01, tank=(0.6, 0.3), bed: float = 0.06, column=(0.15, 0.15), mps_type: int = 3,
+                   seed: int = 11, steps: int = 100, dt: float = 2e-4) -> Workload:
+    """2-D dam break over a mobile bed (the reference's Fraccarollo-Capart
+    inputs, input/2D_dam_fraccarollo_lo02p50e-03_{INC,WC}.json, at a coarse
+    spacing): a granular layer of height `bed` over the whole tank (type 1 in
+    the state: fluid of phase 2) and a water column on its left, particle
+    walls.  mps_type 3: incompressible PND + div u with gradient correction
+    (the INC file); 1: weakly compressible with the diffusive PND and
+    gradient type 3 (the WC file)."""
+    w = dam2d_wc(l0=l0, tank=tank, column=(tank[0], bed + column[1]), seed=seed, steps=steps, dt=dt)
+    pos = w.pos.reshape(-1, 3)
+    types = w.types.copy()
+    fl = types == FLUID
+    keep = ~fl | (pos[:, 1] < bed) | (pos[:, 0] < column[0])
+    pos, types = pos[keep], types[keep]
+    fl = types == FLUID
+    types[fl & (pos[:, 1] < bed)] = 1   # the granular phase (readMpsParticleFile's type 1)
+    p = w.params
+    _granular_params(p)
+    p.mps_type = mps_type
+    p.re_s, p.re_l = 2.1, 3.1
+    if mps_type == 3:
+        p.pnd_type, p.gradient_type, p.gradient_correction, p.shifting_type = 0, 0, 1, 0
+        p.relax_pnd, p.alpha_compressibility, p.solver_tol = 0.01, 1e-6, 1e-8
+    else:
+        p.pnd_type, p.gradient_type, p.gradient_correction, p.shifting_type = 2, 3, 0, 2
+        p.time_step = 5e-5
+    n = len(types)
+    return Workload(f"granular_dam2d_l0_{l0:g}", p, types.astype(np.int32), pos.reshape(-1), np.zeros(3 * n),
+                    mesh=None, steps=steps, info={"granular": int((types == 1).sum()), "water": int((types == 0).sum())})
+
+
+def granular_dam3d(l0: float = 0.025, bed: float = 0.1, steps: int = 40, dt: float = 2e-4, slip: int = 0) -> Workload:
+    """3-D dam break over a granular bed in the closed 12-triangle box
+    (polygon walls, as input/damErosion3D_WC.json): the lower `bed` metres
+    of the dam3d fluid block are the granular phase."""
+    w = dam3d_inc(l0=l0, steps=steps, dt=dt)
+    pos = w.pos.reshape(-1, 3)
+    types = w.types.copy()
+    types[(types == FLUID) & (pos[:, 2] < bed)] = 1
+    p = w.params
+    _granular_params(p, phi=(0.541, 0.6), mm=100.0, vf_max=0.65)
+    p.slip_condition = slip
+    w.types = types.astype(np.int32)
+    w.name = f"granular_dam3d_l0_{l0:g}"
+    w.info = dict(w.info or {}, granular=int((types == 1).sum()))
+    return w
+
+
 WORKLOADS = {
     "dam2d_wc": lambda: dam2d_wc(),
     "dam3d_inc_216k": lambda: dam3d_inc(l0=0.005, steps=500),

@@ -73,6 +73,11 @@ static const FieldDesc kFields[] = {
     {"ppe_x", 0, DPPEX}, {"ppe_b", 0, DPPEB}, {"dev_dot_n", 0, DDEVN},
     {"type", 2, ITYPE}, {"bc", 2, IBC}, {"num_neigh", 2, INNEIGH}, {"num_neigh_wall", 2, INNWALL},
     {"near_wall", 2, INEARWALL}, {"near_mesh_type", 2, INEARMESH},
+    // granular phase (fluid_type 1)
+    {"rho", 0, DRHO}, {"meu", 0, DMEU}, {"cv", 0, DCV}, {"ii", 0, DII}, {"meu_y", 0, DMEUY},
+    {"inertia", 0, DINERTIA}, {"p_rheo", 0, DPRHEO}, {"p_smooth", 0, DPSMOOTH}, {"vf", 0, DVF},
+    {"s11", 0, DS11}, {"s12", 0, DS12}, {"s13", 0, DS13}, {"s22", 0, DS22}, {"s23", 0, DS23}, {"s33", 0, DS33},
+    {"ptype", 2, IPTYPE},
 };
 static const FieldDesc* findField(const char* name) {
     for (const auto& f : kFields)
@@ -306,6 +311,10 @@ static int runStep(Ctx* ctx) {
     const double skin = keptListSkin(ctx);
     ctx->lastSkin = skin;
     if (skin > 0.0 && (rc = launchListKeepStart(ctx, skin))) return rc;
+    if (P.fluid_type == 1) {   // src/main.cpp:259-262
+        if ((rc = launchGranular(ctx, "volume_fraction"))) return rc;
+        if ((rc = launchGranular(ctx, "viscosity_interaction"))) return rc;
+    }
     OP("viscosity");
     OP("add_gravity");
     if (P.relax_press < 1.0) {
@@ -347,8 +356,13 @@ static int runStep(Ctx* ctx) {
     OP("press_gradient");
     if (P.wall_type == MPS_WALL_POLYGON) {
         OP("wall_press_gradient");
-        OP("wall_viscosity");
     }
+    if (P.fluid_type == 1) {   // src/main.cpp:387-398
+        if ((rc = launchGranular(ctx, "volume_fraction"))) return rc;
+        if ((rc = launchGranular(ctx, "viscosity_interaction"))) return rc;
+        if (P.wall_type == MPS_WALL_POLYGON && (rc = launchGranular(ctx, "wall_viscosity_interaction"))) return rc;
+    }
+    if (P.wall_type == MPS_WALL_POLYGON) OP("wall_viscosity");
     // src/main.cpp:410-414: the forced mesh moves before the corrector
     if (P.forced_on && (rc = forcedMeshUpdate(ctx, P.time_step, ctx->timeCurrent))) return rc;
     OP("correction");
@@ -479,6 +493,14 @@ int mps_create(mps_ctx** out, const mps_params* p, int n, int device) {
     ctx->redCap = std::max(kRedBlocks, gridFor(n, 256));
     MPS_HIP(hipMalloc(&ctx->red, sizeof(double) * (3 * ctx->redCap + 16)));
     MPS_HIP(hipMalloc(&ctx->asmRed, sizeof(double) * (3 * ctx->redCap + 16)));
+    if (p->fluid_type == 1) {   // granular phase: carried per-step fields, bed-level grid
+        for (int q = 0; q < kNNCarriedCount; q++) MPS_HIP(hipMalloc(&ctx->DaltNN[q], nd));
+        MPS_HIP(hipMalloc(&ctx->IaltPtype, ni));
+        granularGridSize(*p, &ctx->bedKx, &ctx->bedKy);
+        const size_t cells = (size_t)ctx->bedKx * ctx->bedKy;
+        MPS_HIP(hipMalloc(&ctx->bedKeys, sizeof(unsigned long long) * (2 * cells + 1)));
+        MPS_HIP(hipMalloc(&ctx->bedIdx, sizeof(int) * cells));
+    }
     MPS_HIP(hipMalloc(&ctx->wtab, sizeof(double) * 4 * kWallTable));
     MPS_HIP(hipHostMalloc(&ctx->hostScalars, sizeof(double) * NDS));
     MPS_HIP(hipHostMalloc(&ctx->hostInts, sizeof(int) * NIS));
@@ -506,13 +528,15 @@ int mps_destroy(mps_ctx* ctx) {
     for (int f = 0; f < NPD; f++) hipFree(ctx->Dalt[f]);
     for (int f = 0; f < NI; f++) hipFree(ctx->I[f]);
     for (int f = 0; f < NPI; f++) hipFree(ctx->Ialt[f]);
+    for (int q = 0; q < kNNCarriedCount; q++)
+        if (ctx->DaltNN[q]) hipFree(ctx->DaltNN[q]);
     void* bufs[] = {ctx->cellStart, ctx->cellCount, ctx->key, ctx->rank, ctx->order, ctx->order2, ctx->slotOf,
                     ctx->bvh, ctx->bvhTri, ctx->scanTmp, ctx->outList, ctx->nbrMask, ctx->nbr, ctx->nbrCnt, ctx->P4, ctx->mV, ctx->mF, ctx->mN, ctx->wtab,
                     ctx->aVal, ctx->aCol, ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->aVal32, ctx->aDiagInv32,
                     ctx->aDiag32, ctx->cgR, ctx->cgP,
                     ctx->cgZ, ctx->cgQ, ctx->cgInvD, ctx->cgX, ctx->fR, ctx->fP, ctx->fZ, ctx->fQ, ctx->fX,
                     ctx->bcgR0, ctx->bcgY, ctx->bcgS, ctx->bcgT, ctx->patchList, ctx->fixList, ctx->V4, ctx->nbrMaskS, ctx->nbrS, ctx->nbrCntS,
-                    ctx->ds, ctx->is, ctx->red, ctx->asmRed};
+                    ctx->ds, ctx->is, ctx->red, ctx->asmRed, ctx->IaltPtype, ctx->bedKeys, ctx->bedIdx};
     for (void* b : bufs)
         if (b) hipFree(b);
     freeSlot(ctx->extraMesh[0]);
@@ -562,6 +586,10 @@ int mps_upload_state(mps_ctx* ctx, const int* type, const double* pos, const dou
     }
     MPS_HIP(hipMemcpyAsync(ctx->D[DPRESS], press, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
     MPS_HIP(hipMemcpyAsync(ctx->D[DPAVG], press_avg, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    {   // phase, density and viscosity of every particle (granular type 1 becomes fluid of phase 2)
+        const int rc = launchInitPhase(ctx);
+        if (rc) return rc;
+    }
     MPS_HIP(hipMemsetAsync(ctx->is, 0, sizeof(int) * NIS, ctx->stream));
     MPS_HIP(hipMemsetAsync(ctx->ds, 0, sizeof(double) * NDS, ctx->stream));
     MPS_HIP(hipMemcpyAsync(ctx->is + I_NLIVE, &n, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
@@ -718,6 +746,10 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "true_residual") *out = ctx->trueResidual;
     else if (s == "fallback") *out = ctx->fallbackUsed;
     else if (s == "solver_failed") *out = ctx->solverFailed;
+    else if (s == "cfl_visc") {
+        const int rc2 = granularCflVisc(ctx, out);
+        if (rc2) return rc2;
+    }
     else if (s == "solver_failures") *out = (double)ctx->solverFailures;
     else if (s == "list_max") *out = ctx->hostInts[I_LIST_MAX];
     else if (s == "list_skin") *out = ctx->lastSkin / ctx->P.part_dist;
@@ -729,6 +761,16 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "capacity") *out = ctx->N;
     else if (s == "nnz") *out = ctx->hostInts[I_NNZ];
     else if (s == "nnz_escaped") *out = ctx->hostInts[I_NESC];
+    else if (s == "list_entries" || s == "list_entries_small") {
+        // entries of the last neighbour list (large radius / small radius), summed over the slots
+        const int* cnt = s == "list_entries" ? ctx->nbrCnt : ctx->args().nbrCntS;
+        std::vector<int> h(ctx->N);
+        MPS_HIP(hipMemcpyAsync(h.data(), cnt, sizeof(int) * ctx->N, hipMemcpyDeviceToHost, ctx->stream));
+        MPS_HIP(hipStreamSynchronize(ctx->stream));
+        double sum = 0.0;
+        for (int v : h) sum += v;
+        *out = sum;
+    }
     else if (s == "spmv_ms_total") *out = ctx->spmvMsTotal;
     else if (s == "spmv_launches") *out = (double)ctx->spmvLaunches;
     else if (s == "spmv_bytes_total") *out = ctx->spmvBytesTotal;
@@ -1094,6 +1136,19 @@ int mps_get_wall_tables(mps_ctx* ctx, double* xpnd, double* xneigh, double* zij,
         if (rc) return rc;                      \
         return syncStatus(ctx);                 \
     }
+
+#define GRANULAR(fn, name)                      \
+    int fn(mps_ctx* ctx) {                      \
+        CHECK_CTX(ctx);                         \
+        hipSetDevice(ctx->dev);                 \
+        int rc = launchGranular(ctx, name);     \
+        if (rc) return rc;                      \
+        return syncStatus(ctx);                 \
+    }
+GRANULAR(mps_volume_fraction, "volume_fraction")
+GRANULAR(mps_viscosity_interaction, "viscosity_interaction")
+GRANULAR(mps_wall_viscosity_interaction, "wall_viscosity_interaction")
+#undef GRANULAR
 
 int mps_bucket_update(mps_ctx* ctx) {
     CHECK_CTX(ctx);

@@ -77,6 +77,20 @@ __device__ __forceinline__ bool live(const KArgs& a, int k) {
     return k < a.N && a.I[ICANON][k] < a.is[I_NLIVE];
 }
 
+// Density RHO, viscosity MEU and phase density (DNS_FL1 for phase 1,
+// DNS_FL2 otherwise) of particle k: per particle with the granular phase
+// (fluid_type 1), the reference's constant values for a Newtonian fluid
+// (readMpsParticleFile, src/MpsInputOutput.cpp:832-837).
+__device__ __forceinline__ double rhoOf(const KArgs& a, int k) {
+    return a.P.fluid_type == 1 ? a.D[DRHO][k] : a.P.dns_fl1;
+}
+__device__ __forceinline__ double meuOf(const KArgs& a, int k) {
+    return a.P.fluid_type == 1 ? a.D[DMEU][k] : a.P.knm_vs1 * a.P.dns_fl1;
+}
+__device__ __forceinline__ double phaseDensity(const KArgs& a, int k) {
+    return (a.P.fluid_type == 1 && a.I[IPTYPE][k] != 1) ? a.P.dns_fl2 : a.P.dns_fl1;
+}
+
 // Stencil walk over the 3x3(x3) buckets around the bucket of (x,y,z) with the
 // +eps truncation of MpsBucket::bucketCoordinates (src/MpsBucket.cpp:420-425),
 // jz -> jy -> jx ascending; inside a bucket slots are in reference-index

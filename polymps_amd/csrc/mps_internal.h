@@ -31,10 +31,21 @@ enum DField {
     DAX = NPD, DAY, DAZ, DASX, DASY, DASZ, DNPX, DNPY, DNPZ, DNPCD2, DPNDSMALL, DCONC,
     DAUX1, DAUX2, DPNDWALL, DRIW2, DDEVN, DWPX, DWPY, DWPZ, DPNX, DPNY, DPNZ,
     DCM0, DCM1, DCM2, DCM3, DCM4, DCM5, DCM6, DCM7, DCM8, DDVX, DDVY, DDVZ, DPPEB, DNRMX, DNRMY, DNRMZ,
+    // granular phase (fluid_type 1): density and viscosity per particle (a
+    // Newtonian fluid uses DNS_FL1 and KNM_VS1 DNS_FL1 for every particle),
+    // volume fraction, second strain-rate invariant, yield viscosity,
+    // inertial number, normal stress, smoothed pressure, grain volume
+    // fraction, strain-rate tensor
+    DRHO, DMEU, DCV, DII, DMEUY, DINERTIA, DPRHEO, DPSMOOTH, DVF, DS11, DS12, DS13, DS22, DS23, DS33,
     ND
 };
+// per-step fields that a granular-phase run carries with its particles
+// through the sort (the correction matrix is read at the start of the next
+// step by the strain rate)
+constexpr int kNNCarried[] = {DRHO, DMEU, DCM0, DCM1, DCM2, DCM3, DCM4, DCM5, DCM6, DCM7, DCM8};
+constexpr int kNNCarriedCount = sizeof(kNNCarried) / sizeof(kNNCarried[0]);
 // Per-particle int fields; the first NPI are persistent.
-enum IField { ITYPE, ICANON, IBC, INNEIGH, NPI, INNWALL = NPI, INEARWALL, INEARMESH, NI };
+enum IField { ITYPE, ICANON, IBC, INNEIGH, NPI, INNWALL = NPI, INEARWALL, INEARMESH, IPTYPE, NI };
 
 // device scalar slots (double)
 enum DScalar { S_VELMAX, S_ABSNEW, S_ABSOLD, S_RHSN2, S_THR, S_RN2, S_ALPHA, S_BETA, S_PAP, S_BN2, S_TRUE_RN2,
@@ -119,6 +130,14 @@ struct Ctx {
     double* Dalt[NPD] = {};
     int* I[NI] = {};
     int* Ialt[NPI] = {};
+    // granular phase: second buffers of the carried per-step fields and of
+    // PTYPE, and the bed-level grid (allocated with fluid_type 1 only)
+    double* DaltNN[kNNCarriedCount] = {};
+    int* IaltPtype = nullptr;
+    unsigned long long* bedKeys = nullptr;   // [2][cells]: ordered keys of the bed and water levels
+    int* bedIdx = nullptr;                   // [cells]: reference index of the bed-level particle
+    int bedKx = 0, bedKy = 0;
+    double cflVisc = 0.0;
     // buckets
     int* cellStart = nullptr;   // nB + 2
     int* cellCount = nullptr;   // nB + 1
@@ -264,6 +283,12 @@ int ppeSolve(Ctx* ctx);
 int ppeWarmStart(Ctx* ctx);
 int launchCompaction(Ctx* ctx);
 int launchClosestPoint(Ctx* ctx, int type);   // one mesh type (0 rigid, 1 deformable, 2 forced)
+// granular phase (granular.hip)
+void granularGridSize(const mps_params& P, int* kx, int* ky);
+int launchInitPhase(Ctx* ctx);
+int launchViscosityNN(Ctx* ctx);
+int launchGranular(Ctx* ctx, const char* op);
+int granularCflVisc(Ctx* ctx, double* out);
 // learned PPE operator (gnn.hip)
 int gnnEnsure(Ctx* ctx);
 int gnnSetParams(Ctx* ctx, const double* theta, unsigned long long seed);

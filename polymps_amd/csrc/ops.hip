@@ -536,7 +536,7 @@ __global__ void kCollisionPC(KArgs a) {
     const mps_params& P = a.P;
     if (a.I[ITYPE][i] != P.fluid) return;
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
-    const double mi = P.dns_fl1;
+    const double mi = phaseDensity(a, i);
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
@@ -550,7 +550,7 @@ __global__ void kCollisionPC(KArgs a) {
             double fDT = (ux - a.D[DU][j]) * v0 + (uy - a.D[DV][j]) * v1 + (uz - a.D[DW][j]) * v2;
             if (fDT > 0.0) {
                 const bool fj = a.I[ITYPE][j] == P.fluid;
-                const double mj = fj ? P.dns_fl1 : P.dns_wall;
+                const double mj = fj ? phaseDensity(a, j) : P.dns_wall;
                 fDT *= P.restitution_collision * mj / (mi + mj) / d2;
                 if (fj) {
                     dux -= v0 * fDT; duy -= v1 * fDT; duz -= v2 * fDT;
@@ -573,7 +573,7 @@ __global__ void kCollisionDPC(KArgs a) {
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
     const double Wij5 = 0.5 * 0.5 * 0.5 * 0.5 * 3.0;
     const double gmod = sqrt(P.gravity[0] * P.gravity[0] + P.gravity[1] * P.gravity[1] + P.gravity[2] * P.gravity[2]);
-    const double mi = P.dns_fl1, rhoi = P.dns_fl1;
+    const double mi = phaseDensity(a, i), rhoi = rhoOf(a, i);
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
@@ -586,7 +586,7 @@ __global__ void kCollisionDPC(KArgs a) {
         const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
         if (d2 < P.part_dist * P.part_dist && (d2 < dm2 || pw)) {
             const bool fj = a.I[ITYPE][j] == P.fluid;
-            const double mj = fj ? P.dns_fl1 : P.dns_wall;
+            const double mj = fj ? phaseDensity(a, j) : P.dns_wall;
             const double dst = sqrt(d2);
             double Wij = 0.0;
             if (dst > KEPS) {
@@ -967,7 +967,7 @@ __global__ void kPndDiffusive(KArgs a) {
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
     const double ni = a.D[DPNDI][i];
     if (ni < KEPS) return;
-    const double rho = P.dns_fl1;
+    const double rho = rhoOf(a, i);
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
@@ -986,7 +986,8 @@ __global__ void kPndDiffusive(KArgs a) {
             const double dst = sqrt(d2);
             const double wL = wgt(dst, P.re_l, P.weight_type);
             if (fi && a.I[ITYPE][j] == P.fluid) {
-                const double pgh = -2.0 * (rho * rho / (rho + rho)) *
+                const double rhoj = rhoOf(a, j);
+                const double pgh = -2.0 * (rho * rhoj / (rho + rhoj)) *
                                    (P.gravity[0] * v0 + P.gravity[1] * v1 + P.gravity[2] * v2);
                 const double Pj = a.D[DPRESS][j];
                 Di += P.time_step / rho * P.coeff_visc_multiphase * (Pj - Pi - pgh) * wL;
@@ -1394,7 +1395,7 @@ __global__ void kPressExplicit(KArgs a) {
     const int i = slotId();
     if (!live(a, i)) return;
     const mps_params& P = a.P;
-    const double mi = P.dns_fl1;
+    const double mi = phaseDensity(a, i);
     double pr = 0.0;
     if (a.I[IBC][i] == P.inner) {
         if (P.mps_type == MPS_EXPLICIT) pr = (a.D[DPNDI][i] - P.pnd_small_zero) * P.coeff_press_emps * mi;
@@ -1495,7 +1496,7 @@ __device__ __forceinline__ void pressGradientBody(const KArgs& a) {
             }
         }
     });
-    const double rho = P.dns_fl1;
+    const double rho = rhoOf(a, i);
     if (PRED) {
         const double f = 1.0 - P.relax_press;
         a.D[DAX][i] = a.D[DAX][i] + f * ax * P.coeff_press_grad / rho;
@@ -1525,7 +1526,7 @@ __device__ __forceinline__ void wallPressGradientBody(const KArgs& a) {
     const double vol = a.volForce;
     const double vm = fmin(a.ds[S_VELMAX], P.expect_max_velocity);
     const double vmax2 = vm * vm;
-    const double rho = P.dns_fl1;
+    const double rho = rhoOf(a, i);
     const double xi[3] = {PX(i), PY(i), PZ(i)};
     const double xm[3] = {a.D[DMX][i], a.D[DMY][i], a.D[DMZ][i]};
     const double Pi = a.D[DPRESS][i], ni = a.D[DPNDI][i];
@@ -1612,7 +1613,8 @@ __global__ void kWallViscosity(KArgs a) {
     if (!(a.I[ITYPE][i] == P.fluid && a.I[INEARWALL][i])) return;
     const bool slip = P.slip_condition == MPS_FREE_SLIP;
     const double vol = a.volForce;
-    const double meu = P.knm_vs1 * P.dns_fl1, rho = P.dns_fl1, dnsI = P.dns_fl1;
+    const double meu = meuOf(a, i), rho = rhoOf(a, i), dnsI = phaseDensity(a, i);
+    const bool nn = P.fluid_type == 1;   // per-neighbour viscosity of the granular phase
     const double xi[3] = {PX(i), PY(i), PZ(i)};
     const double xm[3] = {a.D[DMX][i], a.D[DMY][i], a.D[DMZ][i]};
     const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
@@ -1655,9 +1657,16 @@ __global__ void kWallViscosity(KArgs a) {
         const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
         if (d2 < P.re_l2 && dm2 < P.re_l2 && d2 < dm2) {
             const double wL = wgt(sqrt(dm2), P.re_l, P.weight_type);
-            ax += (a.D[DU][j] - umx) * wL * nuJ;
-            ay += (a.D[DV][j] - umy) * wL * nuJ;
-            az += (a.D[DW][j] - umz) * wL * nuJ;
+            double nu = nuJ;
+            if (nn) {
+                const double mj = a.D[DMEU][j];
+                if ((meu + mj) > KEPS) nu = 2.0 * meu * mj / (meu + mj);
+                else nu = 0.0;
+                nu = nu / dnsI;
+            }
+            ax += (a.D[DU][j] - umx) * wL * nu;
+            ay += (a.D[DV][j] - umy) * wL * nu;
+            az += (a.D[DW][j] - umz) * wL * nu;
         }
     });
     {
@@ -2116,7 +2125,6 @@ __global__ void kExtrapolateWallDummy2(KArgs a) {
     const mps_params& P = a.P;
     const int t = a.I[ITYPE][i];
     if (!(t == P.dummy_wall || (P.mps_type == MPS_WEAKLY && t == P.wall))) return;
-    const double rho = P.dns_fl1;
     const double xi = PX(i), yi = PY(i), zi = PZ(i);
     double ni = 0.0, pr = 0.0;
     forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
@@ -2125,7 +2133,7 @@ __global__ void kExtrapolateWallDummy2(KArgs a) {
         if (d2 < P.re_s2 && a.I[ITYPE][j] == P.fluid) {
             const double wS = wgt(sqrt(d2), P.re_s, P.weight_type);
             ni += wS;
-            pr += (a.D[DPRESS][j] - rho * (P.gravity[0] * v0 + P.gravity[1] * v1 + P.gravity[2] * v2)) * wS;
+            pr += (a.D[DPRESS][j] - rhoOf(a, j) * (P.gravity[0] * v0 + P.gravity[1] * v1 + P.gravity[2] * v2)) * wS;
         }
     });
     if (pr < 0.0) return;
@@ -2164,7 +2172,9 @@ int launchOp(Ctx* ctx, const char* op) {
     KArgs a = ctx->args();
     const int g = gridFor(ctx->N, kBlock);
     const mps_params& P = ctx->P;
-    if (o == "viscosity") {
+    if (o == "viscosity" && P.fluid_type == 1) {
+        return launchViscosityNN(ctx);
+    } else if (o == "viscosity") {
         if (ctx->listValid) {
             if (!ctx->V4) MPS_HIP(hipMalloc(&ctx->V4, sizeof(double4) * (size_t)ctx->N));
             a.V4 = ctx->V4;

@@ -177,6 +177,10 @@ void paramsDefaults(mps_params* p) {
     p->density_fluid = 1000.0; p->density_wall = 1000.0; p->knm_vs1 = 0.000001; p->fluid_type = 0;
     p->gravity[0] = 0.0; p->gravity[1] = 0.0; p->gravity[2] = -9.81;
     p->knm_vs2 = 0.000001; p->dns_fl1 = 1000.0; p->dns_fl2 = 1540.0;
+    p->dns_sdt = 1540.0; p->power_n = 1.2; p->meu0 = 0.03;
+    p->phi_1 = 0.541; p->phi_wal = 0.541; p->phi_bed = 0.541; p->phi_2 = 0.6;
+    p->cohes = 0.0; p->fraction_method = 2; p->dg = 0.0035; p->i0 = 0.75; p->mm = 100.0; p->stress_calc_method = 1;
+    p->cd = 0.47; p->vf_min = 0.25; p->vf_max = 0.65;
     p->dim = 3.0; p->part_dist = 0.01; p->time_step = 0.0005; p->time_simulation = 1.0;
     p->iter_output = 80; p->cfl_number = 0.2; p->weight_type = 0; p->slip_condition = 0;
     p->re_s = 2.1; p->re_l = 2.1;
@@ -265,6 +269,24 @@ int paramsFromJson(const char* path, mps_params* p, std::string& grid, std::stri
     p->knm_vs2 = jnum(rh, "kinematic_visc_phase_2", 0.000001);
     p->dns_fl1 = jnum(rh, "fluid_density_phase_1", 1000.0);
     p->dns_fl2 = jnum(rh, "fluid_density_phase_2", 1540.0);
+    p->dns_sdt = jnum(rh, "sediment_density", 1540.0);
+    p->power_n = jnum(rh, "power_law_index", 1.2);
+    p->meu0 = jnum(rh, "consistency_index", 0.03);
+    const JVal* phi = rh ? rh->get("phi") : nullptr;
+    p->phi_1 = jnum(phi, "lower", 0.541);
+    p->phi_wal = jnum(phi, "wall", 0.541);
+    p->phi_bed = jnum(phi, "bed", 0.541);
+    p->phi_2 = jnum(phi, "second", 0.6);
+    p->cohes = jnum(rh, "cohes_coeff", 0.0);
+    p->fraction_method = jint(rh, "fraction_method", 2);
+    p->dg = jnum(rh, "grain_size", 0.0035);
+    p->i0 = jnum(rh, "I0", 0.75);
+    p->mm = jnum(rh, "mm", 100.0);
+    p->stress_calc_method = jint(rh, "stress_calc_method", 1);
+    p->cd = jnum(rh, "drag_coeff", 0.47);
+    const JVal* vfr = rh ? rh->get("volume_fraction") : nullptr;
+    p->vf_min = jnum(vfr, "min", 0.25);
+    p->vf_max = jnum(vfr, "max", 0.65);
     const JVal* nu = root.get("numerical");
     p->dim = jnum(nu, "dimension", 3.0);
     p->part_dist = jnum(nu, "particle_dist", 0.01);
@@ -379,7 +401,13 @@ int paramsDerive(mps_params* p, int n, const double* pos, std::string& err) {
         if ((int)p->dim == 2 && pdir == 2) { err = "periodic z in a 2-D case"; return MPS_E_UNSUPPORTED; }
         if (p->wall_type != MPS_WALL_PARTICLE) { err = "periodic domains are supported with particle walls"; return MPS_E_UNSUPPORTED; }
     }
-    if (p->fluid_type != 0) { err = "only Newtonian fluids are supported"; return MPS_E_UNSUPPORTED; }
+    if (p->fluid_type != 0 && p->fluid_type != 1) { err = "unknown fluid type"; return MPS_E_UNSUPPORTED; }
+    if (p->fluid_type == 1 && p->wall_type == MPS_WALL_POLYGON && (int)p->dim == 2) {
+        // the polygon-wall rheology indexes its bed grid with a 3-D column
+        // (src/MpsViscosity.cpp:766-783)
+        err = "the granular phase with polygon walls needs a 3-D case";
+        return MPS_E_UNSUPPORTED;
+    }
     if ((p->fem_on || p->forced_on) && p->wall_type != MPS_WALL_POLYGON) {
         err = "deformable / forced meshes need polygon walls";
         return MPS_E_UNSUPPORTED;

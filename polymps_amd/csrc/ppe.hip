@@ -58,7 +58,7 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
     const bool pw = P.wall_type == MPS_WALL_PARTICLE;
     const double xi = a.D[DX][i], yi = a.D[DY][i], zi = a.D[DZ][i];
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
-    const double rho = P.dns_fl1;
+    const double rho = phaseDensity(a, i);   // DNS_FL1 or DNS_FL2 by phase (src/MpsPressure.cpp:237-239)
     // A neighbour j whose wall distance is at least 2 reL has its mirror point
     // at least 3 reL away from any particle of its reL ball, so the mirror test
     // of row j passes for i without reading j's mirror point.
@@ -107,7 +107,7 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
         // accumulator is selected through memory
         double subSum = 0.0, subRhs = 0.0;
         if (okRow && dummy) {
-            const double pgh = rho * (v0 * P.gravity[0] + v1 * P.gravity[1] + v2 * P.gravity[2]) * wL;
+            const double pgh = rhoOf(a, j) * (v0 * P.gravity[0] + v1 * P.gravity[1] + v2 * P.gravity[2]) * wL;
             subRhs = P.coeff_ppe * pgh;
         }
         if (okRow && !dummy) subSum = mij;

@@ -181,6 +181,22 @@ __global__ void kPermute(PermArgs pa, int n, const int* order, int* slotOf) {
     slotOf[pa.isrc[ICANON][s]] = k;
 }
 
+// the granular phase's carried per-step fields and PTYPE (kNNCarried)
+struct PermArgsNN {
+    const double* src[kNNCarriedCount];
+    double* dst[kNNCarriedCount];
+    const int* isrc;
+    int* idst;
+};
+__global__ void kPermuteNN(PermArgsNN pa, int n, const int* order) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int s = order[k];
+#pragma unroll
+    for (int f = 0; f < kNNCarriedCount; f++) pa.dst[f][k] = pa.src[f][s];
+    pa.idst[k] = pa.isrc[s];
+}
+
 int launchScan(Ctx* ctx, const int* in, int* out, int n, int tailValue) {
     // exclusive scan of n values into out[0..n-1], out[n] = tailValue
     const int per = kScanBlock * kScanPer;
@@ -216,6 +232,15 @@ int launchBucketSort(Ctx* ctx) {
     hipLaunchKernelGGL(kPermute, dim3(gridFor(N, 256)), dim3(256), 0, ctx->stream, pa, N, ctx->order2, ctx->slotOf);
     for (int f = 0; f < NPD; f++) std::swap(ctx->D[f], ctx->Dalt[f]);
     for (int f = 0; f < NPI; f++) std::swap(ctx->I[f], ctx->Ialt[f]);
+    if (ctx->P.fluid_type == 1) {
+        PermArgsNN pn;
+        for (int q = 0; q < kNNCarriedCount; q++) { pn.src[q] = ctx->D[kNNCarried[q]]; pn.dst[q] = ctx->DaltNN[q]; }
+        pn.isrc = ctx->I[IPTYPE];
+        pn.idst = ctx->IaltPtype;
+        hipLaunchKernelGGL(kPermuteNN, dim3(gridFor(N, 256)), dim3(256), 0, ctx->stream, pn, N, ctx->order2);
+        for (int q = 0; q < kNNCarriedCount; q++) std::swap(ctx->D[kNNCarried[q]], ctx->DaltNN[q]);
+        std::swap(ctx->I[IPTYPE], ctx->IaltPtype);
+    }
     ctx->listValid = false;
     MPS_HIP(hipGetLastError());
     return MPS_OK;

@@ -24,12 +24,13 @@ _OPS = [
     "pnd_neigh_npcd", "pnd_diffusive", "mean_pnd", "mean_neigh_fluid_pnd", "update_bc", "calc_press",
     "vel_divergence", "ppe_assemble", "ppe_solve", "ppe_warm_start", "extrapolate_wall_dummy", "press_gradient", "wall_press_gradient",
     "normal_particles", "wall_normal_particles", "wall_viscosity", "correction", "shifting", "wall_dummy_velocity", "init", "step",
+    "volume_fraction", "viscosity_interaction", "wall_viscosity_interaction",
 ]
 
 VEC_FIELDS = {"pos", "vel", "acc", "acc_star", "npcd", "mirror", "wall_pos", "polygon_normal", "force_wall",
               "cm1", "cm2", "cm3", "grad_conc", "normal"}
 MESH_FIXED, MESH_DEFORMABLE, MESH_FORCED = 0, 1, 2
-INT_FIELDS = {"type", "bc", "num_neigh", "num_neigh_wall", "near_wall", "near_mesh_type"}
+INT_FIELDS = {"type", "bc", "num_neigh", "num_neigh_wall", "near_wall", "near_mesh_type", "ptype"}
 
 
 class MpsError(RuntimeError):

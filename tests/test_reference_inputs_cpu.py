@@ -37,6 +37,18 @@ KEYS = [
     (("physical", "rheological", "kinematic_visc_phase_2"), "knm_vs2"),
     (("physical", "rheological", "fluid_density_phase_1"), "dns_fl1"),
     (("physical", "rheological", "fluid_density_phase_2"), "dns_fl2"),
+    (("physical", "rheological", "sediment_density"), "dns_sdt"),
+    (("physical", "rheological", "power_law_index"), "power_n"),
+    (("physical", "rheological", "consistency_index"), "meu0"),
+    (("physical", "rheological", "phi", "lower"), "phi_1"), (("physical", "rheological", "phi", "wall"), "phi_wal"),
+    (("physical", "rheological", "phi", "bed"), "phi_bed"), (("physical", "rheological", "phi", "second"), "phi_2"),
+    (("physical", "rheological", "cohes_coeff"), "cohes"),
+    (("physical", "rheological", "fraction_method"), "fraction_method"),
+    (("physical", "rheological", "grain_size"), "dg"), (("physical", "rheological", "I0"), "i0"),
+    (("physical", "rheological", "mm"), "mm"), (("physical", "rheological", "drag_coeff"), "cd"),
+    (("physical", "rheological", "stress_calc_method"), "stress_calc_method"),
+    (("physical", "rheological", "volume_fraction", "min"), "vf_min"),
+    (("physical", "rheological", "volume_fraction", "max"), "vf_max"),
     (("numerical", "dimension"), "dim"), (("numerical", "particle_dist"), "part_dist"),
     (("numerical", "time_step"), "time_step"), (("numerical", "final_time"), "time_simulation"),
     (("numerical", "iter_output"), "iter_output"), (("numerical", "CFL_number"), "cfl_number"),
@@ -106,7 +118,7 @@ def test_reference_json_parses_as_python_json(built, path):
             got = got[entry[2]]
         assert float(got) == float(v), (path.name, "/".join(keys), got, v)
         checked += 1
-    assert checked >= 60, checked
+    assert checked >= 75, checked
     assert grid == ref["pathNames"]["particle_grid_file"]
     assert mesh == ref["pathNames"]["mesh_rigid_file"]
 
