@@ -296,6 +296,7 @@ int mps_write_grid(const char* path, int n, const int* type, const double* pos, 
 #define MPS_VTU_AUXILIAR 16
 #define MPS_VTU_ONLY_FREE_SURFACE 32  /* free-surface and wall particles only (writeVtuAsciiFreeSurface) */
 #define MPS_VTU_BINARY 64             /* binary VTU (appended raw data) instead of ASCII */
+#define MPS_VTU_NON_NEWTONIAN 128     /* the granular phase's arrays (output_VTU.non_newtonian) */
 /* The output part of MpsInputOutput::readInputFile (src/MpsInputOutput.cpp:140-151,
  * 173-179): pathNames.vtu_output_folder and the MPS_VTU_* flags. */
 int mps_output_from_json(const char* json_path, char* vtu_folder, size_t folder_cap, int* vtu_flags);

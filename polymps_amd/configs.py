@@ -421,7 +421,8 @@ def case_json(p: MpsParams, grid: str, mesh: str, vtu_folder: str, vtu: dict | N
                           "period": p.forced_period, "velocity": p.forced_velocity},
         "flags": {"wall_type": int(p.wall_type), "fem_MESH": bool(p.fem_on), "forced_MESH": bool(p.forced_on),
                   "output_VTU": dict({"type": 0, "only_freeSurface": False, "pnd": False, "neigh": False,
-                                      "deviation": False, "concentration": False, "auxiliar": False},
+                                      "deviation": False, "concentration": False, "auxiliar": False,
+                                      "non_newtonian": False},
                                      **(vtu or {}))},
         "pathNames": names,
         "domain": {"min": {a: float(p.domain_min[k]) for k, a in enumerate(ax)},
@@ -432,7 +433,14 @@ def case_json(p: MpsParams, grid: str, mesh: str, vtu_folder: str, vtu: dict | N
                      "kinematic_visc": p.knm_vs1, "fluid_type": int(p.fluid_type),
                      "gravity": {a: float(p.gravity[k]) for k, a in enumerate(ax)},
                      "rheological": {"kinematic_visc_phase_2": p.knm_vs2, "fluid_density_phase_1": p.dns_fl1,
-                                     "fluid_density_phase_2": p.dns_fl2}},
+                                     "fluid_density_phase_2": p.dns_fl2, "sediment_density": p.dns_sdt,
+                                     "power_law_index": p.power_n, "consistency_index": p.meu0,
+                                     "phi": {"lower": p.phi_1, "wall": p.phi_wal, "bed": p.phi_bed,
+                                             "second": p.phi_2},
+                                     "cohes_coeff": p.cohes, "fraction_method": int(p.fraction_method),
+                                     "grain_size": p.dg, "I0": p.i0, "mm": p.mm,
+                                     "stress_calc_method": int(p.stress_calc_method), "drag_coeff": p.cd,
+                                     "volume_fraction": {"min": p.vf_min, "max": p.vf_max}}},
         "numerical": {
             "dimension": p.dim, "particle_dist": p.part_dist, "time_step": p.time_step,
             "final_time": p.time_simulation, "iter_output": int(p.iter_output), "CFL_number": p.cfl_number,

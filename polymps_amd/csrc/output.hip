@@ -116,6 +116,20 @@ std::vector<ArrSpec> layout(int flags, int wallType, int N, size_t* bytes) {
         a.push_back({"concentration", K_F32, 1, DCONC, 0});
         a.push_back({"GradCi", K_F32, 3, DGCX, 0});
     }
+    if (flags & MPS_VTU_NON_NEWTONIAN) {   // src/MpsInputOutput.cpp:1463-1495, 1732-1775
+        a.push_back({"RHO", K_F32, 1, DRHO, 0});
+        a.push_back({"ConcVol", K_F32, 1, DCV, 0});
+        a.push_back({"MEU", K_F32, 1, DMEU, 0});
+        a.push_back({"MEUy", K_F32, 1, DMEUY, 0});
+        a.push_back({"II", K_F32, 1, DII, 0});
+        a.push_back({"PTYPE", K_I32, 1, IPTYPE, 0});
+        a.push_back({"p_smooth", K_F32, 1, DPSMOOTH, 0});
+        if (!fs) {   // the free-surface file ends the block at p_smooth
+            a.push_back({"Inertia", K_F32, 1, DINERTIA, 0});
+            a.push_back({"Normal_Stress", K_F32, 1, DPRHEO, 0});
+            a.push_back({"VF", K_F32, 1, DVF, 0});
+        }
+    }
     if (flags & MPS_VTU_AUXILIAR) {
         // the free-surface file has no ParticleType array
         if (!fs) a.push_back({"ParticleType", K_I32, 1, ITYPE, 0});

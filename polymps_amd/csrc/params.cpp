@@ -620,6 +620,7 @@ int mps_output_from_json(const char* json_path, char* vtu_folder, size_t folder_
         if (mpsgpu::jint(vtu, "deviation", 0)) v |= MPS_VTU_DEVIATION;
         if (mpsgpu::jint(vtu, "concentration", 0)) v |= MPS_VTU_CONCENTRATION;
         if (mpsgpu::jint(vtu, "auxiliar", 0)) v |= MPS_VTU_AUXILIAR;
+        if (mpsgpu::jint(vtu, "non_newtonian", 0)) v |= MPS_VTU_NON_NEWTONIAN;
         if (mpsgpu::jint(vtu, "only_freeSurface", 0)) v |= MPS_VTU_ONLY_FREE_SURFACE;
         if (mpsgpu::jint(vtu, "type", 0)) v |= MPS_VTU_BINARY;
         *vtu_flags = v;

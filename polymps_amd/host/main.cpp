@@ -139,8 +139,19 @@ static void mainLoop(MpsParticleSystem* partSyst, MpsParticle* part, PolygonMesh
         if (!poly) partPress->extrapolatePressParticlesWallDummy(partSyst, part, buck);
         if (partSyst->gradientCorrection) vectMatr->correctionMatrix(partSyst, part, buck);
         partPress->calcPressGradient(partSyst, part, buck);
+        if (poly) partPress->calcWallPressGradient(partSyst, part, buck);
+        // src/main.cpp:387-398
+        if (partSyst->fluidType == NON_NEWTONIAN) {
+            partVisc->calcVolumeFraction(partSyst, part, buck);
+            partVisc->calcViscosityInteractionVal(partSyst, part, buck);
+            if (poly) {
+                if (partSyst->slipCondition == FREE_SLIP)
+                    partVisc->calcWallSlipViscosityInteractionVal(partSyst, part, buck);
+                else
+                    partVisc->calcWallNoSlipViscosityInteractionVal(partSyst, part, buck);
+            }
+        }
         if (poly) {
-            partPress->calcWallPressGradient(partSyst, part, buck);
             if (partSyst->slipCondition == FREE_SLIP) partVisc->calcWallSlipViscosity(partSyst, part, buck);
             else partVisc->calcWallNoSlipViscosity(partSyst, part, buck);
         }

@@ -91,7 +91,7 @@ public:
     int numOfIterations = 0, fileNumber = 0, gradientCorrection = 0;
     bool forcedOn = false, femOn = false;
     double relaxPress = 1.0, timeCurrent = 0.0, timeStep = 0.0, timeSimulation = 0.0, dim = 3.0, partDist = 0.0;
-    double reS = 0.0, reL = 0.0, reS2 = 0.0, reL2 = 0.0, velMax = 0.0, CFLcurrent = 0.0;
+    double reS = 0.0, reL = 0.0, reS2 = 0.0, reL2 = 0.0, velMax = 0.0, CFLcurrent = 0.0, CFLvisc = 0.0;
     double timerStart = 0.0, timerEnd = 0.0;
     double uniformVelWall[3] = {0.0, 0.0, 0.0};
 
@@ -319,13 +319,23 @@ public:
     void calcWallNoSlipViscosity(MpsParticleSystem*, MpsParticle* P, MpsBucket*) {
         polymps_gpu::check(mps_wall_viscosity(P->ctx), P->ctx, "calcWallNoSlipViscosity");
     }
-    void calcVolumeFraction(MpsParticleSystem*, MpsParticle*, MpsBucket*) { nonNewtonian(); }
-    void calcViscosityInteractionVal(MpsParticleSystem*, MpsParticle*, MpsBucket*) { nonNewtonian(); }
-    void calcWallSlipViscosityInteractionVal(MpsParticleSystem*, MpsParticle*, MpsBucket*) { nonNewtonian(); }
-    void calcWallNoSlipViscosityInteractionVal(MpsParticleSystem*, MpsParticle*, MpsBucket*) { nonNewtonian(); }
-
-private:
-    static void nonNewtonian() { throw MpsGpuError("non-Newtonian viscosity is not supported"); }
+    // the granular phase (src/MpsViscosity.cpp:104-1526)
+    void calcVolumeFraction(MpsParticleSystem*, MpsParticle* P, MpsBucket*) {
+        polymps_gpu::check(mps_volume_fraction(P->ctx), P->ctx, "calcVolumeFraction");
+    }
+    // the 3-D normal stress ramps with PSystem->timeCurrent (src/MpsViscosity.cpp:596-597)
+    void calcViscosityInteractionVal(MpsParticleSystem* S, MpsParticle* P, MpsBucket*) {
+        mps_set_scalar(P->ctx, "time", S->timeCurrent);
+        polymps_gpu::check(mps_viscosity_interaction(P->ctx), P->ctx, "calcViscosityInteractionVal");
+    }
+    void calcWallSlipViscosityInteractionVal(MpsParticleSystem* S, MpsParticle* P, MpsBucket*) {
+        mps_set_scalar(P->ctx, "time", S->timeCurrent);
+        polymps_gpu::check(mps_wall_viscosity_interaction(P->ctx), P->ctx, "calcWallSlipViscosityInteractionVal");
+    }
+    void calcWallNoSlipViscosityInteractionVal(MpsParticleSystem* S, MpsParticle* P, MpsBucket*) {
+        mps_set_scalar(P->ctx, "time", S->timeCurrent);
+        polymps_gpu::check(mps_wall_viscosity_interaction(P->ctx), P->ctx, "calcWallNoSlipViscosityInteractionVal");
+    }
 };
 
 // include/MpsParticleCollision.h.  The per-step neighbour list is built here
@@ -558,6 +568,10 @@ public:
         printf("Iteration: %5dth Time: %lfsec Num. Particles: %d Max Velocity: %lfm/s Courant: %lf",
                PSystem->numOfIterations, PSystem->timeCurrent, Particles->numParticles, PSystem->velMax,
                PSystem->CFLcurrent);
+        if (PSystem->fluidType == NON_NEWTONIAN) {
+            mps_get_scalar(Particles->ctx, "cfl_visc", &PSystem->CFLvisc);
+            printf(" CFLvisc: %lf", PSystem->CFLvisc);
+        }
         if (PSystem->mpsType == IMPLICIT_PND || PSystem->mpsType == IMPLICIT_PND_DIVU)
             printf(" Solver iterations: %3d Estimated error: %.2e", PPress->getSolverIter(), PPress->getSolverError());
         printf(" RunTime: %02dh%02dm%02dsec\n", hours, minutes % 60, seconds % 60);
