@@ -133,7 +133,7 @@ def test_reference_json_values_spot_checks(built):
     b, _, mesh = L.params_from_json(DATA / "BRUMADINHO_space10_lo10p00.json")
     assert b.part_dist == 10.0 and b.mps_type == 1 and b.wall_type == 1
     assert mesh.endswith("BRUMADINHO_space10_model_lucas_top_0p50.stl")
-    # the Newtonian-only build reads the non-Newtonian files too
+    # the granular (non-Newtonian) erosion input
     e, _, _ = L.params_from_json(DATA / "damErosion3D_WC.json")
     assert e.fluid_type == 1
 
