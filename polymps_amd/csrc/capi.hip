@@ -482,6 +482,7 @@ int mps_create(mps_ctx** out, const mps_params* p, int n, int device) {
     MPS_HIP(hipMalloc(&ctx->aCol, sizeof(unsigned short) * (size_t)ctx->K * n));
     MPS_HIP(hipMalloc(&ctx->aEsc, sizeof(int) * (size_t)kPpeEsc * n));
     MPS_HIP(hipMalloc(&ctx->aCnt, ni));
+    MPS_HIP(hipMemsetAsync(ctx->aCnt, 0, ni, ctx->stream));   // nnz reads 0 before the first assembly
     MPS_HIP(hipMalloc(&ctx->aDiag, nd));
     MPS_HIP(hipMalloc(&ctx->cgR, nd));
     MPS_HIP(hipMalloc(&ctx->cgP, nd));
@@ -751,7 +752,12 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
         if (rc2) return rc2;
     }
     else if (s == "solver_failures") *out = (double)ctx->solverFailures;
-    else if (s == "list_max") *out = ctx->hostInts[I_LIST_MAX];
+    else if (s == "list_max") {
+        // the longest row of the current neighbour list
+        const int rc2 = reduceCounts(ctx, ctx->nbrCnt, ctx->N, I_LIST_MAX, true);
+        if (rc2) return rc2;
+        *out = ctx->hostInts[I_LIST_MAX];
+    }
     else if (s == "list_skin") *out = ctx->lastSkin / ctx->P.part_dist;
     else if (s == "list_skin_cap") *out = ctx->listSkinCap;
     else if (s == "list_skin_factor") *out = ctx->listSkinFactor;
@@ -759,7 +765,12 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "list_patched") *out = ctx->lastSkin > 0.0 ? ctx->hostInts[I_PATCHCNT] : 0;
     else if (s == "list_capacity") *out = 32 * ctx->maskW;
     else if (s == "capacity") *out = ctx->N;
-    else if (s == "nnz") *out = ctx->hostInts[I_NNZ];
+    else if (s == "nnz") {
+        // entries of the last PPE assembly
+        const int rc2 = reduceCounts(ctx, ctx->aCnt, ctx->N, I_NNZ, false);
+        if (rc2) return rc2;
+        *out = ctx->hostInts[I_NNZ];
+    }
     else if (s == "nnz_escaped") *out = ctx->hostInts[I_NESC];
     else if (s == "list_entries" || s == "list_entries_small") {
         // entries of the last neighbour list (large radius / small radius), summed over the slots

@@ -213,10 +213,15 @@ __device__ __forceinline__ void walkCellsPos(const KArgs& a, int i, double x, do
 
 // The per-step neighbour list of slot i: column-major [K][N] slot indices in
 // bucket-walk order, cnt[i] entries (the full list nbr / nbrCnt, or the
-// small-radius list nbrS / nbrCntS).  Four entries and their packed
-// (x, y, z, type) copies are loaded per batch before g(j, q) runs for each.
-template <class G>
-__device__ __forceinline__ void listNbrsQ(const KArgs& a, const int* nbr, const int* cnt, int i, G&& g) {
+// small-radius list nbrS / nbrCntS).  Entries go in batches of four: the
+// batch's packed (x, y, z, type) copies P4[j] and its extra per-slot values
+// load(j) are all gathered before g(j, q, e) runs for each entry, and the
+// next batch's slot indices are loaded while this batch is worked on.  (A
+// two-stage pipeline that also keeps the next batch's gathers in flight
+// needs twice the registers and ran up to 4x slower.)
+struct NoExtra {};
+template <class L, class G>
+__device__ __forceinline__ void listNbrsX(const KArgs& a, const int* nbr, const int* cnt, int i, L&& load, G&& g) {
     const int n = cnt[i];
     const int* __restrict__ col = nbr + i;
     const double4* __restrict__ P4 = a.P4;
@@ -234,23 +239,46 @@ __device__ __forceinline__ void listNbrsQ(const KArgs& a, const int* nbr, const 
             q.x = q.x + pbc * P.periodic_length[0];
             q.y = q.y + pbc * P.periodic_length[1];
             q.z = q.z + pbc * P.periodic_length[2];
-            g(j, q);
+            g(j, q, load(j));
         }
         return;
     }
-    for (; c + 4 <= n; c += 4) {
-        const int j0 = col[c * N], j1 = col[(c + 1) * N], j2 = col[(c + 2) * N], j3 = col[(c + 3) * N];
-        const double4 q0 = P4[j0], q1 = P4[j1], q2 = P4[j2], q3 = P4[j3];
-        g(j0, q0);
-        g(j1, q1);
-        g(j2, q2);
-        g(j3, q3);
+    if (n >= 4) {
+        int j0 = col[0], j1 = col[N], j2 = col[2 * N], j3 = col[3 * N];
+        for (; c + 4 <= n; c += 4) {
+            const double4 q0 = P4[j0], q1 = P4[j1], q2 = P4[j2], q3 = P4[j3];
+            const auto e0 = load(j0), e1 = load(j1), e2 = load(j2), e3 = load(j3);
+            // issued after the gathers: loads complete in issue order, so
+            // waiting for the gathers does not wait for these.  Entries
+            // c + 4 .. c + 7 exist when c + 8 <= n (n <= K: in bounds)
+            int k0 = 0, k1 = 0, k2 = 0, k3 = 0;
+            if (c + 8 <= n) {
+                k0 = col[(c + 4) * N]; k1 = col[(c + 5) * N]; k2 = col[(c + 6) * N]; k3 = col[(c + 7) * N];
+            }
+            g(j0, q0, e0);
+            g(j1, q1, e1);
+            g(j2, q2, e2);
+            g(j3, q3, e3);
+            j0 = k0; j1 = k1; j2 = k2; j3 = k3;
+        }
     }
     for (; c < n; c++) {
         const int j = col[c * N];
-        const double4 q = P4[j];
-        g(j, q);
+        g(j, P4[j], load(j));
     }
+}
+
+// Positions only: g(j, q).
+template <class G>
+__device__ __forceinline__ void listNbrsQ(const KArgs& a, const int* nbr, const int* cnt, int i, G&& g) {
+    listNbrsX(a, nbr, cnt, i, [](int) { return NoExtra{}; }, [&](int j, const double4& q, NoExtra) { g(j, q); });
+}
+
+// The large-radius list with a second packed per-slot array: g(j, q, v)
+// receives P4[j] and V[j], both gathered in the batch.
+template <class G>
+__device__ __forceinline__ void listNbrsQV(const KArgs& a, const double4* __restrict__ V, int i, G&& g) {
+    listNbrsX(a, a.nbr, a.nbrCnt, i, [&](int j) { return V[j]; }, g);
 }
 
 // Particle type from the w component of P4: its low 10 bits, sign-extended
@@ -273,6 +301,22 @@ template <bool LIST, class F>
 __device__ __forceinline__ void forNbrsPosS(const KArgs& a, int i, double x, double y, double z, F&& f) {
     if (LIST) listNbrsQ(a, a.nbrS, a.nbrCntS, i, [&](int j, const double4& q) { f(j, q.x, q.y, q.z); });
     else walkCellsPos(a, i, x, y, z, f);
+}
+
+// Small-radius neighbours with one more per-slot FP64 value S[j]:
+// f(j, xj, yj, zj, sOf), sOf() returning S[j].  With the list S[j] is
+// gathered in the batch (every entry is a valid slot); a walk reads it only
+// when sOf is called.
+template <bool LIST, class F>
+__device__ __forceinline__ void forNbrsPosSX(const KArgs& a, int i, double x, double y, double z,
+                                             const double* __restrict__ S, F&& f) {
+    if (LIST) {
+        listNbrsX(a, a.nbrS, a.nbrCntS, i, [&](int j) { return S[j]; },
+                  [&](int j, const double4& q, double s) { f(j, q.x, q.y, q.z, [s]() { return s; }); });
+    } else {
+        walkCellsPos(a, i, x, y, z,
+                     [&](int j, double xj, double yj, double zj) { f(j, xj, yj, zj, [S, j]() { return S[j]; }); });
+    }
 }
 
 // The same with the neighbour's particle type, f(j, xj, yj, zj, tj).

@@ -282,6 +282,9 @@ int ppeAssemble(Ctx* ctx);
 int ppeSolve(Ctx* ctx);
 int ppeWarmStart(Ctx* ctx);
 int launchCompaction(Ctx* ctx);
+// sum (max = false) or maximum of the n counts cnt[] into is[slot], then to
+// hostInts[slot] (synchronises the stream)
+int reduceCounts(Ctx* ctx, const int* cnt, int n, int slot, bool max);
 int launchClosestPoint(Ctx* ctx, int type);   // one mesh type (0 rigid, 1 deformable, 2 forced)
 // granular phase (granular.hip)
 void granularGridSize(const mps_params& P, int* kx, int* ky);

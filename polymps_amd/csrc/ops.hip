@@ -104,7 +104,6 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
         atomicMax(&a.is[I_OVERFLOW], totalS);
         totalS = a.KS;
     }
-    atomicMax(&a.is[I_LIST_MAX], total);
     cnt[i] = total;
     cntS[i] = totalS;
 }
@@ -133,7 +132,6 @@ __global__ __launch_bounds__(256) void kFixList(KArgs a, int* nbr, int* cnt, int
     });
     if (c > a.K) { atomicMax(&a.is[I_OVERFLOW], c); c = a.K; }
     if (cs > a.KS) { atomicMax(&a.is[I_OVERFLOW], cs); cs = a.KS; }
-    atomicMax(&a.is[I_LIST_MAX], c);
     cnt[i] = c;
     cntS[i] = cs;
 }
@@ -163,7 +161,6 @@ __global__ __launch_bounds__(256) void kBuildListWalk(KArgs a, int* nbr, int* cn
         atomicMax(&a.is[I_OVERFLOW], c);
         c = a.K;
     }
-    atomicMax(&a.is[I_LIST_MAX], c);
     cnt[i] = c;
 }
 
@@ -282,9 +279,7 @@ __global__ void kViscosity(KArgs a) {
     const double ux = a.D[DU][i], uy = a.D[DV][i], uz = a.D[DW][i];
     const double mx = a.D[DMX][i], my = a.D[DMY][i], mz = a.D[DMZ][i];
     double ax = 0, ay = 0, az = 0;
-    // with the list, the velocities come packed from V4 (one gather per neighbour)
-    const double4* __restrict__ V4 = a.V4;
-    forNbrsPosType<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj, int tj) {
+    auto term = [&](double xj, double yj, double zj, int tj, double uj, double vj, double wj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
@@ -292,18 +287,21 @@ __global__ void kViscosity(KArgs a) {
         if (d2 < P.re_l2 && (d2 < dm2 || pw)) {
             const double wL = wgt(sqrt(d2), P.re_l, P.weight_type);
             const double nu = (tj == P.wall) ? nuW : nuF;
-            double uj, vj, wj;
-            if (LIST) {
-                const double4 q = V4[j];
-                uj = q.x; vj = q.y; wj = q.z;
-            } else {
-                uj = a.D[DU][j]; vj = a.D[DV][j]; wj = a.D[DW][j];
-            }
             ax += (uj - ux) * wL * nu;
             ay += (vj - uy) * wL * nu;
             az += (wj - uz) * wL * nu;
         }
-    });
+    };
+    if (LIST) {
+        // positions and packed velocities (V4) of a batch gathered together
+        listNbrsQV(a, a.V4, i, [&](int, const double4& q, const double4& v) {
+            term(q.x, q.y, q.z, p4TypeOf(a, q.w), v.x, v.y, v.z);
+        });
+    } else {
+        forNbrsPosType<false>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj, int tj) {
+            term(xj, yj, zj, tj, a.D[DU][j], a.D[DV][j], a.D[DW][j]);
+        });
+    }
     a.D[DAX][i] = P.coeff_visc_multiphase * ax;
     a.D[DAY][i] = P.coeff_visc_multiphase * ay;
     a.D[DAZ][i] = P.coeff_visc_multiphase * az;
@@ -1460,26 +1458,27 @@ __device__ __forceinline__ void pressGradientBody(const KArgs& a) {
     double pmin = Pi;
     double MC[9];
     for (int q = 0; q < 9; q++) MC[q] = PRED ? 0.0 : a.D[DCM0 + q][i];
+    const double* __restrict__ PR = a.D[DPRESS];
     if (gt == 0 || gt == 2)
-        forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+        forNbrsPosSX<LIST>(a, i, xi, yi, zi, PR, [&](int j, double xj, double yj, double zj, auto pOf) {
             const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
             const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
             const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
             const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
             if (d2 < P.re_s2 && (d2 < dm2 || pw)) {
-                const double pj = a.D[DPRESS][j];
+                const double pj = pOf();
                 if (pmin > pj) pmin = pj;
             }
         });
     double ax = 0, ay = 0, az = 0;
-    forNbrsPosS<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+    forNbrsPosSX<LIST>(a, i, xi, yi, zi, PR, [&](int j, double xj, double yj, double zj, auto pOf) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         const double w0 = xj - mx, w1 = yj - my, w2 = zj - mz;
         const double dm2 = w0 * w0 + w1 * w1 + w2 * w2;
         if (d2 < P.re_s2 && (d2 < dm2 || pw)) {
             double wS = wgt(sqrt(d2), P.re_s, P.weight_type);
-            const double pj = a.D[DPRESS][j];
+            const double pj = pOf();
             if (gt == 0) wS *= (pj - pmin) / d2;
             else if (gt == 1) wS *= (pj + Pi) / d2;
             else if (gt == 2) wS *= (pj + Pi - 2.0 * pmin) / d2;
@@ -2016,7 +2015,6 @@ __global__ __launch_bounds__(256) void kPatchList(KArgs a, int* nbr, int* cnt, i
         atomicMax(&a.is[I_OVERFLOW], cs);
         cs = a.KS;
     }
-    atomicMax(&a.is[I_LIST_MAX], c);
     cnt[i] = c;
     cntS[i] = cs;
 }
@@ -2099,6 +2097,38 @@ int launchListKeepRefresh(Ctx* ctx, double skin) {
                            ctx->nbrCntS, listRadius2(ctx->P), listRadiusS2(ctx->P));
     MPS_HIP(hipGetLastError());
     ctx->listValid = true;
+    return MPS_OK;
+}
+
+// Sum or maximum of n counts: grid-stride partial per thread, a block
+// reduction, then one atomic per workgroup.
+__global__ __launch_bounds__(256) void kReduceCounts(const int* __restrict__ cnt, int n, int* out, int isMax) {
+    __shared__ long long sh[4];
+    long long v = 0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        v = isMax ? (long long)max((int)v, cnt[k]) : v + cnt[k];
+    for (int o = 32; o > 0; o >>= 1) {
+        const long long w = __shfl_xor(v, o, 64);
+        v = isMax ? (w > v ? w : v) : v + w;
+    }
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = sh[0];
+        for (int q = 1; q < 4; q++) t = isMax ? (sh[q] > t ? sh[q] : t) : t + sh[q];
+        if (isMax) atomicMax(out, (int)t);
+        else atomicAdd(out, (int)t);
+    }
+}
+
+int reduceCounts(Ctx* ctx, const int* cnt, int n, int slot, bool max) {
+    MPS_HIP(hipMemsetAsync(ctx->is + slot, 0, sizeof(int), ctx->stream));
+    if (cnt != nullptr && n > 0)
+        hipLaunchKernelGGL(kReduceCounts, dim3(std::min(gridFor(n, 256), 1024)), dim3(256), 0, ctx->stream, cnt, n,
+                           ctx->is + slot, max ? 1 : 0);
+    MPS_HIP(hipGetLastError());
+    MPS_HIP(hipMemcpyAsync(ctx->hostInts + slot, ctx->is + slot, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    MPS_HIP(hipStreamSynchronize(ctx->stream));
     return MPS_OK;
 }
 

@@ -78,7 +78,7 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
     // with the list, the inner and far-from-wall tests of j come with its
     // position (P4.w flags of update_bc) instead of two more gathers
     const bool flags = LIST && a.p4Flags;
-    forNbrsPosTypeFlags<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj, int tj, int fj) {
+    auto entry = [&](int j, double xj, double yj, double zj, int tj, int fj, auto xOf) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         // Entry (j, i) of A is formed in row j of the reference, where i is
@@ -158,7 +158,7 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
                     if (val32) reinterpret_cast<F4*>(val32)[gi] = g32;
                     reinterpret_cast<U4*>(col)[gi] = gcd;
                 }
-                if (RES) ax += mv * x[j];
+                if (RES) ax += mv * xOf();
                 if (!delta) {
                     esc[(size_t)ne * a.N + i] = j;
                     ne++;
@@ -169,7 +169,17 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
                 full = true;
             }
         }
-    });
+    };
+    if (LIST && RES) {
+        // the guess x_j of the row's residual is gathered with the batch's positions
+        listNbrsX(a, a.nbr, a.nbrCnt, i, [&](int j) { return x[j]; }, [&](int j, const double4& q, double xv) {
+            entry(j, q.x, q.y, q.z, p4TypeOf(a, q.w), a.p4Flags ? p4FlagBits(q.w) : 0, [xv]() { return xv; });
+        });
+    } else {
+        forNbrsPosTypeFlags<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj, int tj, int fj) {
+            entry(j, xj, yj, zj, tj, fj, [x, j]() { return x[j]; });
+        });
+    }
     if (c & 1) val[ellIdx64(c - 1, i, a.N)] = pend;
     if (c & 3) {   // the last, partial group (entries past the count are never read)
         const size_t gi = (size_t)(c >> 2) * a.N + i;
@@ -198,7 +208,6 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
         rn2 = ri * ri;
         bn2 = rhs * rhs;
     }
-    atomicAdd(&a.is[I_NNZ], c);
 }
 
 template <bool LIST, bool TRANS, bool RES>
@@ -765,7 +774,10 @@ int ppeAssemble(Ctx* ctx) {
     float* v32 = mixed ? ctx->aVal32 : nullptr;
     float* d32 = mixed ? ctx->aDiag32 : nullptr;
     float* i32 = mixed ? ctx->aDiagInv32 : nullptr;
-    MPS_HIP(hipMemsetAsync(ctx->is + I_NNZ, 0, 2 * sizeof(int), ctx->stream));   // I_NNZ, I_NESC
+    // I_NESC counts the escaped columns; the entry count I_NNZ is summed from
+    // the row counts when asked for (one same-address atomic per wavefront
+    // here would serialise in one L2 channel)
+    MPS_HIP(hipMemsetAsync(ctx->is + I_NESC, 0, sizeof(int), ctx->stream));
     // the CG solve's first true residual comes out of the assembly (the
     // learned warm start changes the guess after the assembly, so not then)
     const bool res = !bicg && ctx->P.learned_mode == 0;
@@ -1091,8 +1103,12 @@ static int solveFp64(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int batc
     ctx->lastCgIters = iters;
     const int D = ctx->hostInts[I_CG_DONE];
     ctx->lastDone = D;
-    addSpmvStats(ctx, times, (D == 1 || D == 3) ? iters + 1 : 0,
-                 10.0 * ctx->hostInts[I_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 28.0 * ctx->nLiveHost);
+    if (ctx->profile) {
+        const int rcN = reduceCounts(ctx, ctx->aCnt, ctx->N, I_NNZ, false);
+        if (rcN) return rcN;
+        addSpmvStats(ctx, times, (D == 1 || D == 3) ? iters + 1 : 0,
+                     10.0 * ctx->hostInts[I_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 28.0 * ctx->nLiveHost);
+    }
     ctx->solverIter = iters;
     ctx->outerIter = 1;
     ctx->solverErr = ctx->hostScalars[S_RHSN2] > 0 ? sqrt(ctx->hostScalars[S_RN2] / ctx->hostScalars[S_RHSN2]) : 0.0;
@@ -1112,6 +1128,10 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
     bool useM = ctx->P.learned_mode == 2 && ctx->gnn != nullptr;
     double prevRn2 = 0.0;
     ctx->lastDone = 0;
+    if (ctx->profile) {
+        rc = reduceCounts(ctx, ctx->aCnt, ctx->N, I_NNZ, false);
+        if (rc) return rc;
+    }
     const double bytes32 = 6.0 * ctx->hostInts[I_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 16.0 * ctx->nLiveHost;
     for (;;) {
         // the first residual may come from the assembly (r and its partials)

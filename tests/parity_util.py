@@ -110,6 +110,10 @@ def ppe_check(g, o, where, press_rtol):
         # without the transpose the rows must also come in the same order
         compare_exact("ppe_cols", cg, co, where)
     compare_exact("ppe_b", g.get("ppe_b"), o.get("ppe_b"), where)
+    # the entry count of the assembly (off-diagonal entries; the download adds the diagonal)
+    n_off = int(rpg[-1]) - (len(rpg) - 1)
+    if g.scalar("nnz") != n_off:
+        raise Mismatch(f"{where}: nnz {g.scalar('nnz')} != {n_off} off-diagonal entries")
 
 
 def compare_buckets(g: Simulation, o: Oracle, where=""):
@@ -253,6 +257,24 @@ def lockstep_step(g: Simulation, o: Oracle, step: int, press_rtol=1e-6, log=None
     both("check_out_domain", ["check_out_domain"], ["type", "pos"])
 
 
+def check_list_max(g: Simulation):
+    """The longest row of a list built at the current positions (no skin)
+    is the largest count of other live particles within the large radius
+    (brute force over all pairs; periodic domains excluded)."""
+    p = g.params
+    if p.domain_type_bc == 1:
+        return
+    n = int(g.scalar("num_particles"))
+    x = g.get("pos").reshape(-1, 3)[:n]
+    best = 0
+    for s0 in range(0, n, 2048):
+        d2 = ((x[s0:s0 + 2048, None, :] - x[None, :, :]) ** 2).sum(-1)
+        best = max(best, int(((d2 < p.re_l * p.re_l).sum(1) - 1).max()))
+    got = g.scalar("list_max")
+    if got != best:
+        raise Mismatch(f"list_max {got} != brute force {best}")
+
+
 def lockstep_init(g: Simulation, o: Oracle):
     """The initial-step pipeline of main() (src/main.cpp:119-180) on both
     sides, compared after each operator, then two lockstep steps."""
@@ -266,6 +288,7 @@ def lockstep_init(g: Simulation, o: Oracle):
         g.call("mesh_update_near"); o.call("update_near_mesh")
         check(g, o, ["pnd_wall", "num_neigh_wall", "near_wall"], "init update_near")
     g.call("build_neighbor_list")
+    check_list_max(g)
     if poly:
         g.call("wall_npcd"); o.call("wall_npcd")
         check(g, o, ["npcd"], "init wall_npcd")
