@@ -311,11 +311,20 @@ static int runStep(Ctx* ctx) {
     const double skin = keptListSkin(ctx);
     ctx->lastSkin = skin;
     if (skin > 0.0 && (rc = launchListKeepStart(ctx, skin))) return rc;
+    // A flow too fast to keep the list (automatic skin above its cap): the
+    // operators before the prediction take their neighbours from a list
+    // built at the step's start positions, with no skin, instead of bucket
+    // walks; only the large radius unless an operator before the prediction
+    // reads the small one.  The list after the prediction is built as usual.
+    const bool smallEarly = P.fluid_type == 1 || P.relax_press < 1.0;
+    ctx->lastStartList = skin == 0.0 && ctx->listSkin < 0.0 && P.domain_type_bc != 1;
+    if (ctx->lastStartList && (rc = launchBuildList(ctx, 0.0, nullptr, smallEarly))) return rc;
     if (P.fluid_type == 1) {   // src/main.cpp:259-262
         if ((rc = launchGranular(ctx, "volume_fraction"))) return rc;
         if ((rc = launchGranular(ctx, "viscosity_interaction"))) return rc;
     }
     OP("viscosity");
+    if (ctx->lastStartList && !smallEarly) ctx->listValid = false;   // its small-radius list is stale
     OP("add_gravity");
     if (P.relax_press < 1.0) {
         OP("prediction_press_gradient");
@@ -759,6 +768,7 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
         *out = ctx->hostInts[I_LIST_MAX];
     }
     else if (s == "list_skin") *out = ctx->lastSkin / ctx->P.part_dist;
+    else if (s == "list_start") *out = ctx->lastStartList ? 1.0 : 0.0;
     else if (s == "list_skin_cap") *out = ctx->listSkinCap;
     else if (s == "list_skin_factor") *out = ctx->listSkinFactor;
     else if (s == "list_rebuilt") *out = ctx->lastSkin > 0.0 ? ctx->hostInts[I_REBUILD] : 1;

@@ -169,6 +169,7 @@ struct Ctx {
     // the last velocity maximum, 0 off) and the skin of the last step (m)
     double listSkin = -1.0;
     double lastSkin = 0.0;
+    bool lastStartList = false;   // the last step built a list at its start positions (no kept list)
     // largest automatic skin (l0); beyond it the step builds its list after
     // the prediction and the viscosity walks the buckets
     double listSkinCap = 0.08;
@@ -274,7 +275,7 @@ inline int gridFor(int n, int block) { return (n + block - 1) / block; }
 
 // launchers (defined in the .hip translation units)
 int launchBucketSort(Ctx* ctx);
-int launchBuildList(Ctx* ctx, double skin = 0.0, const int* gate = nullptr);
+int launchBuildList(Ctx* ctx, double skin = 0.0, const int* gate = nullptr, bool small = true);
 int launchListKeepStart(Ctx* ctx, double skin);
 int launchListKeepRefresh(Ctx* ctx, double skin);
 int launchOp(Ctx* ctx, const char* op);

@@ -25,7 +25,9 @@ constexpr int kBlock = 256;
 constexpr int kWave = 64;
 
 // A second mask, maskS, marks the candidates inside the small radius r2s
-// (<= r2) for the small-radius list.
+// (<= r2) for the small-radius list (SMALL; without it only the large list
+// is built and cntS is left as it is).
+template <bool SMALL>
 __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int* cnt, unsigned* maskS, int* cntS,
                                                     double r2, double r2s, const int* gate, int* fixList) {
     if (gate != nullptr && *gate == 0) return;   // conditional rebuild not needed
@@ -37,7 +39,7 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
     const bool act = inRange && live(a, i);
     if (inRange && !act) {
         cnt[i] = 0;
-        cntS[i] = 0;
+        if (SMALL) cntS[i] = 0;
     }
     const double* __restrict__ X = a.D[DX];
     const double* __restrict__ Y = a.D[DY];
@@ -68,12 +70,12 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
             const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
             const int k = j - s0;
             cur |= ((j != i && d2 < r2) ? 1u : 0u) << (k & 31);
-            curS |= ((j != i && d2 < r2s) ? 1u : 0u) << (k & 31);
+            if (SMALL) curS |= ((j != i && d2 < r2s) ? 1u : 0u) << (k & 31);
             if ((k & 31) == 31) {
                 out[(size_t)(row * W + w) * N] = cur;
-                outS[(size_t)(row * W + w) * N] = curS;
+                if (SMALL) outS[(size_t)(row * W + w) * N] = curS;
                 total += __builtin_popcount(cur);
-                totalS += __builtin_popcount(curS);
+                if (SMALL) totalS += __builtin_popcount(curS);
                 w++;
                 cur = 0;
                 curS = 0;
@@ -82,14 +84,14 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
         [&](int r, int s, int e) {
             if ((e - s) & 31) {
                 out[(size_t)(r * W + w) * N] = cur;
-                outS[(size_t)(r * W + w) * N] = curS;
+                if (SMALL) outS[(size_t)(r * W + w) * N] = curS;
                 total += __builtin_popcount(cur);
-                totalS += __builtin_popcount(curS);
+                if (SMALL) totalS += __builtin_popcount(curS);
                 w++;
             }
             for (; w < W; w++) {
                 out[(size_t)(r * W + w) * N] = 0u;
-                outS[(size_t)(r * W + w) * N] = 0u;
+                if (SMALL) outS[(size_t)(r * W + w) * N] = 0u;
             }
         });
     if (!act) return;
@@ -100,12 +102,12 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
         atomicMax(&a.is[I_OVERFLOW], total);
         total = a.K;
     }
-    if (totalS > a.KS) {
+    if (SMALL && totalS > a.KS) {
         atomicMax(&a.is[I_OVERFLOW], totalS);
         totalS = a.KS;
     }
     cnt[i] = total;
-    cntS[i] = totalS;
+    if (SMALL) cntS[i] = totalS;
 }
 
 // Lists of the particles whose candidate rows overflowed the mask windows,
@@ -1899,7 +1901,7 @@ __global__ void kPackP4Flags(KArgs a) {
     a.P4[i].w = (double)((t & 1023) | (inner << 10) | (far << 11));
 }
 
-int launchBuildList(Ctx* ctx, double skin, const int* gate) {
+int launchBuildList(Ctx* ctx, double skin, const int* gate, bool small) {
     ctx->p4Flags = false;
     KArgs a = ctx->args();
     const int g = gridFor(ctx->N, kBlock);
@@ -1918,14 +1920,19 @@ int launchBuildList(Ctx* ctx, double skin, const int* gate) {
     const int gw = gridFor(ctx->N, 64);
     if (!ctx->fixList) MPS_HIP(hipMalloc(&ctx->fixList, sizeof(int) * (size_t)ctx->N));
     MPS_HIP(hipMemsetAsync(ctx->is + I_FIXCNT, 0, sizeof(int), ctx->stream));
-    hipLaunchKernelGGL(kBuildList, dim3(gridFor(ctx->N, kWave)), dim3(kWave), 0, ctx->stream, a, ctx->nbrMask,
-                       ctx->nbrCnt, ctx->nbrMaskS, ctx->nbrCntS, r2, r2s, gate, ctx->fixList);
+    if (small)
+        hipLaunchKernelGGL(kBuildList<true>, dim3(gridFor(ctx->N, kWave)), dim3(kWave), 0, ctx->stream, a,
+                           ctx->nbrMask, ctx->nbrCnt, ctx->nbrMaskS, ctx->nbrCntS, r2, r2s, gate, ctx->fixList);
+    else
+        hipLaunchKernelGGL(kBuildList<false>, dim3(gridFor(ctx->N, kWave)), dim3(kWave), 0, ctx->stream, a,
+                           ctx->nbrMask, ctx->nbrCnt, ctx->nbrMaskS, ctx->nbrCntS, r2, r2s, gate, ctx->fixList);
 #define M2L(WM)                                                                                              \
     do {                                                                                                     \
         hipLaunchKernelGGL(kMaskToList<WM>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMask, ctx->nbrCnt,  \
                            ctx->nbr, gate);                                                                  \
-        hipLaunchKernelGGL(kMaskToList<WM>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMaskS,             \
-                           ctx->nbrCntS, ctx->nbrS, gate);                                                   \
+        if (small)                                                                                           \
+            hipLaunchKernelGGL(kMaskToList<WM>, dim3(gw), dim3(64), 0, ctx->stream, a, ctx->nbrMaskS,         \
+                               ctx->nbrCntS, ctx->nbrS, gate);                                               \
     } while (0)
     if (ctx->maskW <= 4) M2L(4);
     else if (ctx->maskW <= 8) M2L(8);
