@@ -285,9 +285,29 @@ int mps_steps(mps_ctx* ctx, int k);
 /* .grid reader (src/MpsInputOutput.cpp:645-857); arrays may be NULL to query n */
 int mps_read_grid(const char* path, int* n, int* type, double* pos, double* vel, double* press,
                   double* press_avg);
-/* .grid writer in the same format (checkpoint / restart) */
+/* .grid writer in the same format (the state a .grid file can hold) */
 int mps_write_grid(const char* path, int n, const int* type, const double* pos, const double* vel,
                    const double* press, const double* press_avg);
+/* Checkpoint of the whole simulation state (no reference counterpart: the
+ * reference restarts from a .grid file, src/MpsInputOutput.cpp:645-857,
+ * which leaves out the pressure solver's warm-start vector pressurePPE,
+ * include/MpsParticle.h:205, and the other fields carried between steps).
+ * The file holds every per-particle field carried from one step to the
+ * next, in reference order (positions, velocities, pressures, the PPE
+ * solution and the extrapolation history, PND fields, wall forces, the
+ * granular density / viscosity / correction matrix / phase), the live count,
+ * time, step count, velocity maximum, pressure-solve count, the parameters
+ * and the nodes of the deformable and forced meshes.  mps_checkpoint_read
+ * restores it into a context created with the same parameters, capacity
+ * and meshes (the rigid mesh, and the deformable / forced meshes with the
+ * same node and face counts, installed first); the run then continues bit
+ * for bit as the one that wrote it.  A learned PPE operator's weights are
+ * not included (mps_gnn_save / mps_gnn_load).  Errors: MPS_E_IO for an
+ * unreadable, truncated or foreign file, MPS_E_ARG for a context whose
+ * capacity, bucket grid, phases or meshes differ; the context is unchanged
+ * by a failed read. */
+int mps_checkpoint_write(mps_ctx* ctx, const char* path);
+int mps_checkpoint_read(mps_ctx* ctx, const char* path);
 /* Output options (flags.output_VTU of the JSON file) */
 #define MPS_VTU_PND 1
 #define MPS_VTU_NEIGH 2

@@ -1435,6 +1435,206 @@ int mps_gnn_load(mps_ctx* ctx, const char* path) {
     return mps_gnn_set_params(ctx, th.data(), np);
 }
 
+// ------------------------------------------------------------ checkpoint
+// File layout (little-endian): CkptHeader, the parameter block
+// (paramsSize bytes), then every carried per-particle field in reference
+// order (capacity entries each: the double fields of kCkptD, the int fields
+// of kCkptI, and with the granular phase kNNCarried and IPTYPE), then the
+// node and face-normal arrays of the deformable and forced meshes.
+namespace {
+constexpr char kCkptMagic[8] = {'M', 'P', 'S', 'C', 'K', 'P', 'T', '1'};
+constexpr uint32_t kCkptVersion = 1;
+struct CkptHeader {
+    char magic[8];
+    uint32_t version, paramsSize;
+    int32_t capacity, nLive, numIter, ppeSolves;
+    double time, velMax;
+    int64_t solverFailures;
+    int32_t nD, nI;                  // per-particle double / int fields that follow
+    int32_t meshNv[2], meshNf[2];    // deformable, forced mesh (0: none)
+};
+// persistent int fields besides the reference index, which the order implies
+constexpr int kCkptI[] = {ITYPE, IBC, INNEIGH};
+
+std::vector<int> ckptDFields(const mps_params& P) {
+    std::vector<int> f;
+    for (int q = 0; q < NPD; q++) f.push_back(q);
+    if (P.fluid_type == 1)
+        for (int q = 0; q < kNNCarriedCount; q++) f.push_back(kNNCarried[q]);
+    return f;
+}
+std::vector<int> ckptIFields(const mps_params& P) {
+    std::vector<int> f(std::begin(kCkptI), std::end(kCkptI));
+    if (P.fluid_type == 1) f.push_back(IPTYPE);
+    return f;
+}
+// parameters that fix the context's allocations and bucket grid
+bool ckptCompatible(const mps_params& a, const mps_params& b, std::string& why) {
+    char msg[160];
+#define SAME(f)                                                                        \
+    if (a.f != b.f) {                                                                  \
+        snprintf(msg, sizeof msg, "checkpoint parameter " #f " differs from the context's"); \
+        why = msg;                                                                     \
+        return false;                                                                  \
+    }
+    SAME(dim) SAME(num_buckets_x) SAME(num_buckets_y) SAME(num_buckets_z) SAME(fluid_type) SAME(wall_type)
+    SAME(domain_type_bc) SAME(part_dist) SAME(re_s) SAME(re_l) SAME(bucket_side)
+#undef SAME
+    return true;
+}
+}  // namespace
+
+int mps_checkpoint_write(mps_ctx* ctx, const char* path) {
+    CHECK_CTX(ctx);
+    if (!path) return MPS_E_ARG;
+    int rc = syncStatus(ctx);
+    if (rc) return rc;
+    MPS_HIP(hipSetDevice(ctx->dev));
+    const int n = ctx->N;
+    CkptHeader h{};
+    memcpy(h.magic, kCkptMagic, 8);
+    h.version = kCkptVersion;
+    h.paramsSize = sizeof(mps_params);
+    h.capacity = n;
+    h.nLive = ctx->nLiveHost;
+    h.numIter = ctx->numIter;
+    h.ppeSolves = ctx->ppeSolves;
+    h.time = ctx->timeCurrent;
+    MPS_HIP(hipMemcpy(&h.velMax, ctx->ds + S_VELMAX, sizeof(double), hipMemcpyDeviceToHost));
+    h.solverFailures = ctx->solverFailures;
+    const std::vector<int> df = ckptDFields(ctx->P), itf = ckptIFields(ctx->P);
+    h.nD = (int)df.size();
+    h.nI = (int)itf.size();
+    for (int t = 0; t < 2; t++) {
+        const MeshSlot& m = ctx->extraMesh[t];
+        h.meshNv[t] = m.on ? m.nv : 0;
+        h.meshNf[t] = m.on ? m.nf : 0;
+    }
+    FILE* fp = fopen(path, "wb");
+    if (!fp) { ctx->err = std::string("cannot write ") + path; return MPS_E_IO; }
+    bool ok = fwrite(&h, sizeof h, 1, fp) == 1 && fwrite(&ctx->P, sizeof(mps_params), 1, fp) == 1;
+    double* dtmp = nullptr;
+    int* itmp = nullptr;
+    std::vector<double> hd(n);
+    std::vector<int> hi(n);
+    hipError_t e = hipMalloc(&dtmp, sizeof(double) * n);
+    if (e == hipSuccess) e = hipMalloc(&itmp, sizeof(int) * n);
+    const int g = gridFor(n, 256);
+    for (size_t q = 0; ok && e == hipSuccess && q < df.size(); q++) {
+        const double* src = ctx->D[df[q]];
+        hipLaunchKernelGGL(kToCanonD, dim3(g), dim3(256), 0, ctx->stream, n, ctx->I[ICANON], src, src, src, dtmp, 1);
+        e = hipMemcpyAsync(hd.data(), dtmp, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        ok = ok && fwrite(hd.data(), sizeof(double), n, fp) == (size_t)n;
+    }
+    for (size_t q = 0; ok && e == hipSuccess && q < itf.size(); q++) {
+        hipLaunchKernelGGL(kToCanonI, dim3(g), dim3(256), 0, ctx->stream, n, ctx->I[ICANON], ctx->I[itf[q]], itmp);
+        e = hipMemcpyAsync(hi.data(), itmp, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        ok = ok && fwrite(hi.data(), sizeof(int), n, fp) == (size_t)n;
+    }
+    for (int t = 0; ok && t < 2; t++) {
+        const MeshSlot& m = ctx->extraMesh[t];
+        if (!m.on) continue;
+        ok = fwrite(m.hV.data(), sizeof(double), m.hV.size(), fp) == m.hV.size() &&
+             fwrite(m.hN.data(), sizeof(double), m.hN.size(), fp) == m.hN.size();
+    }
+    if (dtmp) hipFree(dtmp);
+    if (itmp) hipFree(itmp);
+    ok = (fclose(fp) == 0) && ok;
+    if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return MPS_E_HIP; }
+    if (!ok) { ctx->err = std::string("write failed: ") + path; return MPS_E_IO; }
+    MPS_HIP(hipGetLastError());
+    return MPS_OK;
+}
+
+int mps_checkpoint_read(mps_ctx* ctx, const char* path) {
+    CHECK_CTX(ctx);
+    if (!path) return MPS_E_ARG;
+    MPS_HIP(hipSetDevice(ctx->dev));
+    FILE* fp = fopen(path, "rb");
+    if (!fp) { ctx->err = std::string("cannot open ") + path; return MPS_E_IO; }
+    CkptHeader h{};
+    mps_params P{};
+    auto fail = [&](int code, const std::string& msg) {
+        fclose(fp);
+        ctx->err = msg;
+        return code;
+    };
+    if (fread(&h, sizeof h, 1, fp) != 1 || memcmp(h.magic, kCkptMagic, 8) != 0)
+        return fail(MPS_E_IO, std::string("not a checkpoint file: ") + path);
+    if (h.version != kCkptVersion || h.paramsSize != sizeof(mps_params))
+        return fail(MPS_E_IO, "checkpoint written by another version of the library");
+    if (fread(&P, sizeof P, 1, fp) != 1) return fail(MPS_E_IO, "truncated checkpoint");
+    const int n = ctx->N;
+    if (h.capacity != n) return fail(MPS_E_ARG, "checkpoint capacity differs from the context's particle count");
+    if (h.nLive < 0 || h.nLive > n) return fail(MPS_E_IO, "corrupt checkpoint (live count)");
+    std::string why;
+    if (!ckptCompatible(P, ctx->P, why)) return fail(MPS_E_ARG, why);
+    const std::vector<int> df = ckptDFields(P), itf = ckptIFields(P);
+    if (h.nD != (int)df.size() || h.nI != (int)itf.size()) return fail(MPS_E_IO, "corrupt checkpoint (field count)");
+    for (int t = 0; t < 2; t++) {
+        const MeshSlot& m = ctx->extraMesh[t];
+        if ((h.meshNv[t] > 0) != m.on || (m.on && (h.meshNv[t] != m.nv || h.meshNf[t] != m.nf)))
+            return fail(MPS_E_ARG, "checkpoint meshes differ from the context's: install the same deformable / forced "
+                                   "meshes before reading");
+    }
+    // read everything on the host first: a short file leaves the context as it was
+    std::vector<std::vector<double>> hd(df.size(), std::vector<double>(n));
+    std::vector<std::vector<int>> hi(itf.size(), std::vector<int>(n));
+    for (auto& v : hd)
+        if (fread(v.data(), sizeof(double), n, fp) != (size_t)n) return fail(MPS_E_IO, "truncated checkpoint");
+    for (auto& v : hi)
+        if (fread(v.data(), sizeof(int), n, fp) != (size_t)n) return fail(MPS_E_IO, "truncated checkpoint");
+    std::vector<double> mv[2], mn[2];
+    for (int t = 0; t < 2; t++) {
+        if (h.meshNv[t] == 0) continue;
+        mv[t].resize(3 * (size_t)h.meshNv[t]);
+        mn[t].resize(3 * (size_t)h.meshNf[t]);
+        if (fread(mv[t].data(), sizeof(double), mv[t].size(), fp) != mv[t].size() ||
+            fread(mn[t].data(), sizeof(double), mn[t].size(), fp) != mn[t].size())
+            return fail(MPS_E_IO, "truncated checkpoint");
+    }
+    fclose(fp);
+    int rc = vtuWait(ctx);   // snapshots in flight read the old state
+    if (rc) return rc;
+    ctx->P = P;
+    // slot k holds the particle of reference index k; the per-step fields
+    // start as in a new context
+    KArgs a = ctx->args();
+    hipLaunchKernelGGL(kInitState, dim3(gridFor(n, 256)), dim3(256), 0, ctx->stream, a);
+    for (size_t q = 0; q < df.size(); q++)
+        MPS_HIP(hipMemcpyAsync(ctx->D[df[q]], hd[q].data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    for (size_t q = 0; q < itf.size(); q++)
+        MPS_HIP(hipMemcpyAsync(ctx->I[itf[q]], hi[q].data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+    MPS_HIP(hipMemsetAsync(ctx->is, 0, sizeof(int) * NIS, ctx->stream));
+    MPS_HIP(hipMemsetAsync(ctx->ds, 0, sizeof(double) * NDS, ctx->stream));
+    MPS_HIP(hipMemcpyAsync(ctx->is + I_NLIVE, &h.nLive, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    MPS_HIP(hipMemcpyAsync(ctx->ds + S_VELMAX, &h.velMax, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    MPS_HIP(hipStreamSynchronize(ctx->stream));
+    for (int t = 0; t < 2; t++) {
+        MeshSlot& m = ctx->extraMesh[t];
+        if (!m.on) continue;
+        m.hV = mv[t];
+        m.hN = mn[t];
+        rc = uploadSlot(ctx, m, ctx->bvhMode != 0);
+        if (rc) return rc;
+    }
+    ctx->nLiveHost = h.nLive;
+    ctx->velMaxHost = h.velMax;
+    ctx->timeCurrent = h.time;
+    ctx->numIter = h.numIter;
+    ctx->ppeSolves = h.ppeSolves;
+    ctx->solverFailures = h.solverFailures;
+    ctx->solverFailed = 0;
+    ctx->listValid = false;
+    ctx->p4Flags = false;
+    ctx->asmMode = -1;
+    ctx->r0Fresh = false;
+    MPS_HIP(hipGetLastError());
+    return MPS_OK;
+}
+
 int mps_init(mps_ctx* ctx) {
     CHECK_CTX(ctx);
     hipSetDevice(ctx->dev);

@@ -82,6 +82,8 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     lib.mps_steps.argtypes = [vp, ip]
     lib.mps_read_grid.argtypes = [cp, ctypes.POINTER(ip), vp, vp, vp, vp, vp]
     lib.mps_write_grid.argtypes = [cp, ip, vp, vp, vp, vp, vp]
+    lib.mps_checkpoint_write.argtypes = [vp, cp]
+    lib.mps_checkpoint_read.argtypes = [vp, cp]
     lib.mps_write_vtu.argtypes = [vp, cp, ip]
     lib.mps_write_vtu_async.argtypes = [vp, cp, ip]
     lib.mps_output_wait.argtypes = [vp]
@@ -271,6 +273,16 @@ class Simulation:
         vals = np.zeros(nnz.value)
         self._check(self._lib.mps_ppe_matrix(self._h, _ptr(rp), _ptr(cols), _ptr(vals), ctypes.byref(nnz)))
         return rp, cols, vals
+
+    def checkpoint_write(self, path) -> None:
+        """Write the whole state (every field carried between steps, time,
+        counters, moving meshes) for a bit-exact restart."""
+        self._check(self._lib.mps_checkpoint_write(self._h, str(path).encode()))
+
+    def checkpoint_read(self, path) -> None:
+        """Restore a checkpoint into this context (same parameters,
+        capacity and meshes); the run continues bit for bit."""
+        self._check(self._lib.mps_checkpoint_read(self._h, str(path).encode()))
 
     def buckets(self):
         """(first, last, next) bucket lists of the last bucket update in
