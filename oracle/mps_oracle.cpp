@@ -285,13 +285,14 @@ int deriveParams(Sys& s) {
     if (periodic) {
         int nper = 0;
         for (int k = 0; k < 3; k++) if (P.periodic_dir[k]) { nper++; pdir = k; }
-        if (nper != 1 || P.wall_type != MPS_WALL_PARTICLE) { s.err = "periodic: one direction, particle walls"; return -1; }
+        if (nper != 1) { s.err = "periodic: one direction"; return -1; }
     } else if (P.domain_type_bc != 0) {
         s.err = "unknown domain boundary type";
         return -1;
     }
-    if (P.wall_type == MPS_WALL_PARTICLE) {
-        // calcDomainLimits (src/MpsBucket.cpp:71-295): limitTypeBC, then the
+    if (P.wall_type == MPS_WALL_PARTICLE || periodic) {
+        // calcDomainLimits (src/MpsBucket.cpp:71-295, called for particle
+        // walls or a periodic domain, :45-48): limitTypeBC, then the
         // non-periodic or the periodic branch
         double lim[3][2];
         if (P.limit_type_bc == 0) {
@@ -339,8 +340,15 @@ int deriveParams(Sys& s) {
             setNb(d, nbd);
             for (int k = 0; k < 3; k++) {
                 if (k == d || (k == 2 && P.dim != 3)) continue;
-                lim[k][0] -= P.bucket_side;
-                lim[k][1] += P.bucket_side;
+                // :164-171: particle walls one bucket beyond the particles,
+                // polygon walls one bucket beyond the JSON domain
+                if (P.wall_type == MPS_WALL_PARTICLE) {
+                    lim[k][0] -= P.bucket_side;
+                    lim[k][1] += P.bucket_side;
+                } else {
+                    lim[k][0] = P.domain_min[k] - P.bucket_side;
+                    lim[k][1] = P.domain_max[k] + P.bucket_side;
+                }
                 P.phys_min[k] = lim[k][0];
                 P.phys_max[k] = lim[k][1];
                 long nb = (long)((lim[k][1] - lim[k][0]) / P.bucket_side + 1);

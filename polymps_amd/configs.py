@@ -318,6 +318,42 @@ def periodic_channel3d(l0: float = 0.02, size=(0.3, 0.4), depth: float = 0.1, u0
                     steps=steps, info={"fluid": nf, "walls": n - nf, "seed": seed, "u0": u0})
 
 
+def periodic_channel3d_polygon(l0: float = 0.02, size=(0.3, 0.4), depth: float = 0.1, u0: float = 2.5,
+                               seed: int = 4, jitter: float = 1e-3, steps: int = 100, dt: float = 2e-4) -> Workload:
+    """3-D incompressible channel, periodic in y, bounded by a polygon floor
+    (z = 0) and two polygon side walls (x = 0, x = size[0]) that extend one
+    periodic length beyond each seam; the fluid moves at u0 along y.  The
+    periodic axis is sized by the particle positions and the others by the
+    domain of the JSON file, as calcDomainLimits sets them for a periodic
+    domain with polygon walls (src/MpsBucket.cpp:45-48, 164-171)."""
+    nx, ny, cz = int(round(size[0] / l0)), int(round(size[1] / l0)), int(round(depth / l0))
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(cz), indexing="ij")
+    fluid = np.stack([(ix.ravel(order="F") + 0.5) * l0, (iy.ravel(order="F") + 0.5) * l0,
+                      (iz.ravel(order="F") + 0.5) * l0], axis=1)
+    n = len(fluid)
+    u = splitmix64_uniform(seed, 3 * n).reshape(n, 3)
+    pos = fluid + (2.0 * u - 1.0) * jitter * l0
+    types = np.full(n, FLUID, dtype=np.int32)
+    vel = np.zeros((n, 3))
+    vel[:, 1] = u0
+    # the box's floor and x faces (outward normals), six triangles
+    V, F = box_mesh((0.0, -size[1], 0.0), (size[0], 2.0 * size[1], 2.0 * depth))
+    F = F[[0, 1, 8, 9, 10, 11]]
+    p = small_dam3d(l0=0.025, steps=steps).params
+    p.part_dist = l0
+    p.time_step = dt
+    p.time_simulation = steps * dt
+    p.domain_type_bc = 1
+    p.limit_type_bc = 0
+    p.num_bc = 1
+    p.periodic_dir[0], p.periodic_dir[1], p.periodic_dir[2] = 0, 1, 0
+    # the domain of the non-periodic axes (the periodic one comes from the particles)
+    p.domain_min[0], p.domain_min[1], p.domain_min[2] = -0.5 * l0, 0.0, -0.5 * l0
+    p.domain_max[0], p.domain_max[1], p.domain_max[2] = size[0] + 0.5 * l0, size[1], 2.0 * depth
+    return Workload(f"periodic_channel3d_polygon_l0_{l0:g}", p, types, pos.reshape(-1), vel.reshape(-1),
+                    mesh=mesh_arrays(V, F), steps=steps, info={"fluid": n, "seed": seed, "u0": u0})
+
+
 def _lattice_values(seed: int, ix, iy, octave: int):
     """Seeded uniform [0, 1) value per integer lattice point (splitmix64 of a hash)."""
     key = (np.asarray(ix, dtype=np.int64) * 73856093) ^ (np.asarray(iy, dtype=np.int64) * 19349663) ^ (octave * 83492791)

@@ -394,12 +394,11 @@ int paramsDerive(mps_params* p, int n, const double* pos, std::string& err) {
     if (p->domain_type_bc != 0 && !periodic) { err = "unknown domain boundary type"; return MPS_E_UNSUPPORTED; }
     if (periodic) {
         // MpsBucket::calcDomainLimits (src/MpsBucket.cpp:141-287) handles one
-        // periodic direction, with particle walls
+        // periodic direction, with particle or polygon walls
         const int nper = (p->periodic_dir[0] != 0) + (p->periodic_dir[1] != 0) + (p->periodic_dir[2] != 0);
         if (nper != 1) { err = "periodic domains need exactly one periodic direction"; return MPS_E_UNSUPPORTED; }
         for (int k = 0; k < 3; k++) if (p->periodic_dir[k]) pdir = k;
         if ((int)p->dim == 2 && pdir == 2) { err = "periodic z in a 2-D case"; return MPS_E_UNSUPPORTED; }
-        if (p->wall_type != MPS_WALL_PARTICLE) { err = "periodic domains are supported with particle walls"; return MPS_E_UNSUPPORTED; }
     }
     if (p->fluid_type != 0 && p->fluid_type != 1) { err = "unknown fluid type"; return MPS_E_UNSUPPORTED; }
     if (p->fluid_type == 1 && p->wall_type == MPS_WALL_POLYGON && (int)p->dim == 2) {
@@ -484,8 +483,9 @@ int paramsDerive(mps_params* p, int n, const double* pos, std::string& err) {
     p->num_buckets_z = (int)((p->domain_max[2] - p->domain_min[2]) * p->inv_bucket_side) + 3;
     if (two) p->num_buckets_z = 1;
     for (int k = 0; k < 3; k++) p->periodic_length[k] = 0.0;
-    if (p->wall_type == MPS_WALL_PARTICLE) {
-        // calcDomainLimits (:71-295)
+    if (p->wall_type == MPS_WALL_PARTICLE || periodic) {
+        // calcDomainLimits (:71-295), called for particle walls or a periodic
+        // domain (src/MpsBucket.cpp:45-48)
         double lim[3][2];
         if (p->limit_type_bc == 0) {
             if (n <= 0 || !pos) { err = "particle walls need particle positions to size the domain"; return MPS_E_ARG; }
@@ -527,11 +527,17 @@ int paramsDerive(mps_params* p, int n, const double* pos, std::string& err) {
             lim[d][1] += p->bucket_side;
             nbd += 2;
             *nbk[d] = (int)nbd;
-            // the other axes, one bucket beyond the wall particles
+            // the other axes: one bucket beyond the wall particles, or beyond
+            // the domain of the JSON file with polygon walls
             for (int k = 0; k < 3; k++) {
                 if (k == d || (k == 2 && two)) continue;
-                lim[k][0] -= p->bucket_side;
-                lim[k][1] += p->bucket_side;
+                if (p->wall_type == MPS_WALL_PARTICLE) {
+                    lim[k][0] -= p->bucket_side;
+                    lim[k][1] += p->bucket_side;
+                } else {
+                    lim[k][0] = p->domain_min[k] - p->bucket_side;
+                    lim[k][1] = p->domain_max[k] + p->bucket_side;
+                }
                 p->phys_min[k] = lim[k][0];
                 p->phys_max[k] = lim[k][1];
                 long nb = (long)((lim[k][1] - lim[k][0]) / p->bucket_side + 1);

@@ -85,10 +85,11 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
         // seen through the opposite ghost layer: (x_i - pl) - x_j, whose
         // rounding differs from -(x_j + pl - x_i) (periodic domains only)
         double d2T = d2;
+        int pbc = 0;   // ghost shift of j as seen from i (periodic domains)
         if (TRANS && periodic) {
             const int pd = P.periodic_dir[0] ? 0 : (P.periodic_dir[1] ? 1 : 2);
             const double raw = a.D[DX + pd][j], sh = (pd == 0) ? xj : ((pd == 1) ? yj : zj);
-            const int pbc = sh > raw ? 1 : (sh < raw ? -1 : 0);
+            pbc = sh > raw ? 1 : (sh < raw ? -1 : 0);
             if (pbc != 0) {
                 const double u0 = (xi + (-pbc) * P.periodic_length[0]) - a.D[DX][j];
                 const double u1 = (yi + (-pbc) * P.periodic_length[1]) - a.D[DY][j];
@@ -118,12 +119,16 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
         if (!TRANS) {
             put = okRow && !dummy && innerJ;
         } else if (innerJ) {
-            // entry (j, i) of A: i seen from row j, mirror test with j's mirror
+            // entry (j, i) of A: i seen from row j (through the opposite
+            // ghost layer when j is across a periodic seam), mirror test
+            // with j's mirror
             put = pw || (flags ? (fj & 2) != 0 : RIW2[j] >= farWall2);
             if (!put) {
-                const double u0 = xi - a.D[DMX][j], u1 = yi - a.D[DMY][j], u2 = zi - a.D[DMZ][j];
+                const double ix = xi + (-pbc) * P.periodic_length[0], iy = yi + (-pbc) * P.periodic_length[1],
+                             iz = zi + (-pbc) * P.periodic_length[2];
+                const double u0 = ix - a.D[DMX][j], u1 = iy - a.D[DMY][j], u2 = iz - a.D[DMZ][j];
                 const double dmj = u0 * u0 + u1 * u1 + u2 * u2;
-                put = d2 < dmj;
+                put = d2T < dmj;
             }
         }
         if (put) {

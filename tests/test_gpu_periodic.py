@@ -20,6 +20,7 @@ CASES = {
     "2d_wc": (lambda: configs.periodic_channel2d(u0=2.0), 0, 80),
     "2d_inc": (lambda: configs.periodic_channel2d(u0=2.0, mps_type=2), 0, 80),
     "3d_inc": (lambda: configs.periodic_channel3d(u0=2.5), 1, 40),
+    "3d_inc_polygon": (lambda: configs.periodic_channel3d_polygon(u0=2.5), 1, 40),
 }
 
 
