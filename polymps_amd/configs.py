@@ -228,6 +228,19 @@ def dam2d_wc(l0: float = 0.005, tank=(1.0, 0.6), column=(0.25, 0.5), seed: int =
                     info={"fluid": nf, "walls": n - nf, "seed": seed})
 
 
+def dam2d_wc_long(l0: float = 0.005, **kw) -> Workload:
+    """The 2-D weakly compressible dam break with the large radius 3.1 l0 and
+    the corrected pressure gradient of the reference's own 2-D WC example
+    (input/dam1610_h300_lo0p0050_WC.json), which keeps the 2,000-step run
+    stable; with re_l = 2.1 l0 and no correction the run diverges near step
+    1,850 (in the oracle as on the device)."""
+    w = dam2d_wc(l0=l0, **kw)
+    w.params.re_l = 3.1
+    w.params.gradient_correction = 1
+    w.name = f"dam2d_wc_long_l0_{l0:g}"
+    return w
+
+
 def periodic_channel2d(l0: float = 0.01, length: float = 0.4, depth: float = 0.06, u0: float = 2.0, seed: int = 3,
                        jitter: float = 1e-4, steps: int = 200, dt: float = 1e-4, mps_type: int = 1) -> Workload:
     """2-D open channel, periodic in x (the reference's domain.boundary.type 1

@@ -35,3 +35,21 @@ def test_oracle_reproduces_fixture(built, name):
     for f in ("pos", "vel", "press", "bc", "num_neigh", "pndi"):
         assert np.array_equal(o.get(f), g[f"step_{f}"]), f
     o.close()
+
+
+def test_oracle_reproduces_wc_trajectory(built):
+    """The stored 2,000-step trajectory of the 6,332-particle WC dam break
+    (the device test's reference): the oracle reproduces its first
+    checkpoint bit for bit, with another thread count than the one that
+    wrote it."""
+    from oracle.oracle import Oracle
+    g = np.load(GOLDEN / "wc_dam2d_6332_trajectory.npz")
+    w = configs.dam2d_wc_long()
+    assert w.n == int(g["n"])
+    o = Oracle(w.params, w.types, w.pos, w.vel, mesh=w.mesh, threads=4)
+    o.call("init")
+    o.steps(1000)
+    assert int(o.scalar("num_particles")) == int(g["live_1000"])
+    assert np.array_equal(o.get("pos"), g["pos_1000"])
+    assert np.array_equal(o.get("vel"), g["vel_1000"])
+    o.close()
