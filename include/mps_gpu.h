@@ -95,7 +95,8 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
  * "solver_failed" (last pressure solve: 0 converged, 1 stopped above the
  * tolerance, 2 non-finite residual) "solver_failures" (failed solves since
  * the count was last set; settable)
- * "list_max" "list_capacity" "capacity" "nnz" "spmv_ms_total"
+ * "list_max" "list_capacity" "capacity" "nnz" "list_start" (1 when the last
+ * step built a list at its start positions) "spmv_ms_total"
  * "spmv_launches" "spmv_bytes_total" "list_skin" (skin of the last step's
  * kept neighbour list, in l0; 0: built after the prediction) "list_rebuilt"
  * (1 when the last step rebuilt its list after the prediction)
@@ -112,7 +113,10 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
  * "profile" (nonzero: time every SpMV launch with HIP events, counters reset)
  * "precision_mode" (MPS_PREC_*, used from the next PPE assembly)
  * "learned_mode" (0 off, 1 learned warm start, 2 warm start + learned
- * flexible-CG preconditioner) "ppe_warm_start" (initial guess of each
+ * flexible-CG preconditioner) "learned_stall_ratio" (in (0, 1), default
+ * 0.5: the learned preconditioner is dropped for Jacobi, "fallback" 1, when a
+ * refinement pass leaves more than this fraction of the true residual)
+ * "ppe_warm_start" (initial guess of each
  * pressure solve: 0 previous solution, 1 / 2 / 3 linear / quadratic / cubic
  * extrapolation of the last solutions) "mesh_bvh" (closest-point search: -1 BVH for
  * meshes above 64 faces, 0 brute force, 1 BVH; both give the same result). */

@@ -755,6 +755,7 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "ppe_ms") *out = ctx->ppeMs;
     else if (s == "true_residual") *out = ctx->trueResidual;
     else if (s == "fallback") *out = ctx->fallbackUsed;
+    else if (s == "learned_stall_ratio") *out = ctx->learnedStallRatio;
     else if (s == "solver_failed") *out = ctx->solverFailed;
     else if (s == "cfl_visc") {
         const int rc2 = granularCflVisc(ctx, out);
@@ -830,6 +831,14 @@ int mps_set_scalar(mps_ctx* ctx, const char* name, double value) {
             return MPS_E_ARG;
         }
         ctx->P.learned_mode = (int)value;
+    } else if (s == "learned_stall_ratio") {
+        // the learned preconditioner stays in use while each refinement pass
+        // cuts the true residual norm to at most this fraction (default 0.5)
+        if (!(value > 0.0 && value < 1.0)) {
+            ctx->err = "learned_stall_ratio must be in (0, 1)";
+            return MPS_E_ARG;
+        }
+        ctx->learnedStallRatio = value;
     } else if (s == "mask_words") {
         // candidate window of a stencil row in 32-slot words (at most the
         // allocated width); rows beyond it are listed by a direct walk

@@ -247,6 +247,7 @@ struct Ctx {
     double trueResidual = 0.0;
     double ppeMs = 0.0;
     int fallbackUsed = 0;
+    double learnedStallRatio = 0.5;   // learned preconditioner: largest residual ratio of a refinement pass
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // optional per-launch timing of the SpMV kernel (HIP events on the
     // context stream); enabled with mps_set_scalar(ctx, "profile", 1)

@@ -700,7 +700,12 @@ __global__ void kFcgDots(KArgs a, CgPtrs c) {
     }
     writePartials3(a, s0, s1, s2);
 }
-// stop test, breakdown test (r.z <= 0: M not positive on r), beta
+// stop test, breakdown test, beta.  The assembled matrix is negative
+// definite on its coupled rows (diagonal -sum w_ij - alpha rho / dt^2,
+// off-diagonals w_ij > 0), and so is a usable preconditioner (Jacobi: 1 /
+// a_ii < 0): the next step length r.z / p.Ap stays positive while r.z has
+// the sign of the last p.Ap, and a preconditioner that gives r.z of the
+// other sign (or 0) has broken down.
 __global__ void kFcgBeta(KArgs a, int nb, int maxIt) {
     if (a.is[I_CG_DONE]) return;
     double rn2, rz, zq;
@@ -715,7 +720,7 @@ __global__ void kFcgBeta(KArgs a, int nb, int maxIt) {
         a.is[I_CG_DONE] = 5;
         return;
     }
-    if (!(rz > 0.0)) {
+    if (!(rz * a.ds[S_PAP] > 0.0)) {
         a.is[I_CG_DONE] = 4;
         return;
     }
@@ -1154,9 +1159,11 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
             break;
         }
         if (rn2 < tol * tol * bn2 || outer >= kMaxOuter || total >= maxIt) break;
-        // the learned operator stays in use while each refinement pass at
-        // least halves the true residual; otherwise Jacobi takes over
-        if (useM && prevRn2 > 0.0 && rn2 > 0.25 * prevRn2) {
+        // the learned operator stays in use while each refinement pass cuts
+        // the true residual to learned_stall_ratio (default: halves it);
+        // otherwise Jacobi takes over
+        const double stall = ctx->learnedStallRatio;
+        if (useM && prevRn2 > 0.0 && rn2 > stall * stall * prevRn2) {
             useM = false;
             ctx->fallbackUsed = 1;
         }

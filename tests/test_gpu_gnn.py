@@ -161,3 +161,80 @@ def test_trained_network_gain(built):
     g.set_scalar("learned_mode", 2)
     g.call("step")
     assert g.scalar("true_residual") < w.params.solver_tol
+
+
+def _fp64_reference(g, x0):
+    g.set("ppe_x", x0)
+    g.set_scalar("learned_mode", 0)
+    g.set_scalar("precision_mode", 0)
+    g.call("ppe_assemble")
+    g.call("ppe_solve")
+    return g.get("press")
+
+
+def _learned_solve(g, x0):
+    g.set("ppe_x", x0)
+    g.set_scalar("learned_mode", 2)
+    g.set_scalar("precision_mode", 1)
+    g.call("ppe_assemble")
+    g.call("ppe_solve")
+    return g.get("press")
+
+
+def test_stall_hands_the_solve_to_jacobi(built):
+    """A refinement pass of the learned flexible CG that leaves more than
+    learned_stall_ratio of the true residual drops the learned
+    preconditioner for Jacobi.  Forced from the host: a coarse inner
+    tolerance (1e-3) from a zero initial guess makes two passes necessary
+    and a ratio of 1e-12 makes the first one a stall; the solve still
+    reaches the tolerance, reports the fallback and agrees with the FP64
+    solve of the same system."""
+    w = configs.small_dam3d(l0=0.0125)
+    w.params.precision_mode = 1
+    w.params.mixed_inner_tol = 1e-3
+    g = Simulation(w.params, w.types, w.pos, w.vel, mesh=w.mesh, device=0)
+    g.call("init")
+    g.steps(10)
+    print("gnn forced stall", flush=True)
+    g.gnn_init(5)
+    x0 = np.zeros_like(g.get("ppe_x"))
+    g.set_scalar("learned_stall_ratio", 1e-12)
+    assert g.scalar("learned_stall_ratio") == 1e-12
+    pl = _learned_solve(g, x0)
+    assert g.scalar("fallback") == 1
+    assert g.scalar("outer_iterations") >= 2
+    assert g.scalar("true_residual") < w.params.solver_tol
+    pf = _fp64_reference(g, x0)
+    assert np.max(np.abs(pl - pf)) <= 1e-6 * np.abs(pf).max()
+    # the default ratio keeps the (Jacobi-initialised) learned operator
+    g.set_scalar("learned_stall_ratio", 0.5)
+    _learned_solve(g, x0)
+    assert g.scalar("fallback") == 0
+    assert g.scalar("true_residual") < w.params.solver_tol
+    with pytest.raises(Exception):
+        g.set_scalar("learned_stall_ratio", 1.0)
+    g.close()
+
+
+@pytest.mark.parametrize("scale", [1.0, 20.0])
+def test_random_learned_operator_still_converges(built, scale):
+    """Untrained random weights (output weights scaled up) make a poor, in
+    general not positive, preconditioner: the breakdown test or the stall
+    test hand the solve to Jacobi and it still reaches the tolerance with the
+    FP64 solution's pressure."""
+    w, g = developed(steps=10)
+    th = random_theta(21)
+    o = G.offsets()
+    for k in ("v", "vd"):
+        a, s = o[k]
+        n = int(np.prod(s))
+        th[a:a + n] *= scale
+    g.gnn_set_params(th)
+    x0 = g.get("ppe_x")
+    pl = _learned_solve(g, x0)
+    print(f"random operator x{scale}: fallback {g.scalar('fallback'):.0f}, "
+          f"passes {g.scalar('outer_iterations'):.0f}", flush=True)
+    assert g.scalar("true_residual") < w.params.solver_tol
+    pf = _fp64_reference(g, x0)
+    assert np.max(np.abs(pl - pf)) <= 1e-6 * np.abs(pf).max()
+    g.close()
