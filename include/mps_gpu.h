@@ -116,6 +116,9 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
  * flexible-CG preconditioner) "learned_stall_ratio" (in (0, 1), default
  * 0.5: the learned preconditioner is dropped for Jacobi, "fallback" 1, when a
  * refinement pass leaves more than this fraction of the true residual)
+ * "learned_tol_scale" (in (0, 1], default 0.1: with the learned
+ * preconditioner the true residual is driven to this multiple of the
+ * tolerance, since the flexible CG's error at a given residual is larger)
  * "ppe_warm_start" (initial guess of each
  * pressure solve: 0 previous solution, 1 / 2 / 3 linear / quadratic / cubic
  * extrapolation of the last solutions) "mesh_bvh" (closest-point search: -1 BVH for

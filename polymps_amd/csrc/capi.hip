@@ -756,6 +756,7 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "true_residual") *out = ctx->trueResidual;
     else if (s == "fallback") *out = ctx->fallbackUsed;
     else if (s == "learned_stall_ratio") *out = ctx->learnedStallRatio;
+    else if (s == "learned_tol_scale") *out = ctx->learnedTolScale;
     else if (s == "solver_failed") *out = ctx->solverFailed;
     else if (s == "cfl_visc") {
         const int rc2 = granularCflVisc(ctx, out);
@@ -839,6 +840,14 @@ int mps_set_scalar(mps_ctx* ctx, const char* name, double value) {
             return MPS_E_ARG;
         }
         ctx->learnedStallRatio = value;
+    } else if (s == "learned_tol_scale") {
+        // true-residual target of a solve with the learned preconditioner,
+        // as a multiple of the solver tolerance (default 0.1)
+        if (!(value > 0.0 && value <= 1.0)) {
+            ctx->err = "learned_tol_scale must be in (0, 1]";
+            return MPS_E_ARG;
+        }
+        ctx->learnedTolScale = value;
     } else if (s == "mask_words") {
         // candidate window of a stencil row in 32-slot words (at most the
         // allocated width); rows beyond it are listed by a direct walk

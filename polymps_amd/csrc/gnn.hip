@@ -62,6 +62,19 @@ __device__ __forceinline__ unsigned short toBf16(float x) {
 }
 __device__ __forceinline__ float fromBf16(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 __device__ __forceinline__ float relu(float x) { return x > 0.f ? x : 0.f; }
+// the 32 channels of one node, rounded to bfloat16, as four 16-byte stores
+__device__ __forceinline__ void storeBf16x32(unsigned short* dst, const float* v) {
+    uint4* d = reinterpret_cast<uint4*>(dst);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        uint4 w;
+        w.x = (unsigned)toBf16(v[8 * q + 0]) | ((unsigned)toBf16(v[8 * q + 1]) << 16);
+        w.y = (unsigned)toBf16(v[8 * q + 2]) | ((unsigned)toBf16(v[8 * q + 3]) << 16);
+        w.z = (unsigned)toBf16(v[8 * q + 4]) | ((unsigned)toBf16(v[8 * q + 5]) << 16);
+        w.w = (unsigned)toBf16(v[8 * q + 6]) | ((unsigned)toBf16(v[8 * q + 7]) << 16);
+        d[q] = w;
+    }
+}
 
 struct Gnn {
     int N = 0;
@@ -132,7 +145,7 @@ __global__ void kGnnGraph(KArgs a, const double* diag, int* gcol, int* gcnt, flo
     const size_t N = a.N;
     int c = 0;
     double ws = 0.0, div = 0.0;
-    forNbrsPos<LIST>(a, i, xi, yi, zi, [&](int j, double xj, double yj, double zj) {
+    auto edge = [&](int j, double xj, double yj, double zj, double uj, double vj, double wj) {
         const double v0 = xj - xi, v1 = yj - yi, v2 = zj - zi;
         const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
         if (!(d2 < P.re_s2) || c >= KE) return;
@@ -142,9 +155,21 @@ __global__ void kGnnGraph(KArgs a, const double* diag, int* gcol, int* gcnt, flo
         ge[o] = make_float4((float)(v0 * il0), (float)(v1 * il0), (float)(v2 * il0), (float)(w / w0));
         gw[o] = (float)w;
         ws += w;
-        div += ((a.D[DU][j] - ux) * v0 + (a.D[DV][j] - uy) * v1 + (a.D[DW][j] - uz) * v2) * w / d2;
+        div += ((uj - ux) * v0 + (vj - uy) * v1 + (wj - uz) * v2) * w / d2;
         c++;
-    });
+    };
+    const double* __restrict__ U = a.D[DU];
+    const double* __restrict__ V = a.D[DV];
+    const double* __restrict__ W = a.D[DW];
+    if (LIST) {
+        // the small-radius list (entries within reS, or a superset in walk
+        // order), velocities gathered with each batch's positions
+        listNbrsX(a, a.nbrS, a.nbrCntS, i, [&](int j) { return make_double4(U[j], V[j], W[j], 0.0); },
+                  [&](int j, const double4& q, const double4& u) { edge(j, q.x, q.y, q.z, u.x, u.y, u.z); });
+    } else {
+        forNbrsPos<false>(a, i, xi, yi, zi,
+                          [&](int j, double xj, double yj, double zj) { edge(j, xj, yj, zj, U[j], V[j], W[j]); });
+    }
     gcnt[i] = c;
     wsum[i] = (float)ws;
     const double n0 = P.pnd_small_zero;
@@ -160,13 +185,16 @@ __global__ void kGnnEncode(int n, const float4* feat, const float* th, unsigned 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 f = feat[i];
+    float pv[H], hv[H];
 #pragma unroll
     for (int ch = 0; ch < H; ch++) {
         const float* w = th + gnn::oW0 + ch * 4;
         const float p = w[0] * f.x + w[1] * f.y + w[2] * f.z + w[3] * f.w + th[gnn::ob0 + ch];
-        pre0[(size_t)i * H + ch] = toBf16(p);
-        h[(size_t)i * H + ch] = toBf16(relu(p));
+        pv[ch] = p;
+        hv[ch] = relu(p);
     }
+    storeBf16x32(pre0 + (size_t)i * H, pv);
+    storeBf16x32(h + (size_t)i * H, hv);
 }
 
 // ------------------------------------------------------------------ MFMA GEMM
@@ -294,8 +322,7 @@ __global__ __launch_bounds__(256) void kGnnMessage(int n, const int* gcol, const
             acc[ch] += om * relu(q);
         }
     }
-#pragma unroll
-    for (int ch = 0; ch < H; ch++) M[(size_t)i * H + ch] = toBf16(acc[ch]);
+    storeBf16x32(M + (size_t)i * H, acc);
 }
 
 // -------------------------------------------------------------------- head
@@ -656,6 +683,12 @@ __global__ void kGnnLossGrad(KArgs a, const double* val, const unsigned short* c
     gz[i] = (float)(2.0 * s / acc[1]);
 }
 // training vectors: the current residual b - A x, or seeded Gaussian noise
+// FP32 copy of an FP64 vector on live slots, 0 elsewhere
+__global__ void kGnnFromDouble(KArgs a, const double* r, float* v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    v[i] = live(a, i) ? (float)r[i] : 0.f;
+}
 __global__ void kGnnResidualVec(KArgs a, const double* val, const unsigned short* col, const int* esc, const int* cnt,
                                 const double* diag, const double* b, const double* x, float* v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -970,10 +1003,14 @@ int gnnWarmStart(Ctx* ctx) {
     KArgs a = ctx->args();
     hipStream_t s = ctx->stream;
     const int n = ctx->N, gb = g256(n);
-    hipLaunchKernelGGL(kGnnResidualVec, dim3(gb), dim3(256), 0, s, a, ctx->aVal, ctx->aCol, ctx->aEsc, ctx->aCnt,
-                       ctx->aDiag, ctx->D[DPPEB], ctx->D[DPPEX], g->rv);
+    if (ctx->r0Fresh)   // b - A x of the guess, left by the assembly
+        hipLaunchKernelGGL(kGnnFromDouble, dim3(gb), dim3(256), 0, s, a, ctx->cgR, g->rv);
+    else
+        hipLaunchKernelGGL(kGnnResidualVec, dim3(gb), dim3(256), 0, s, a, ctx->aVal, ctx->aCol, ctx->aEsc, ctx->aCnt,
+                           ctx->aDiag, ctx->D[DPPEB], ctx->D[DPPEX], g->rv);
     hipLaunchKernelGGL(kGnnApply, dim3(gb), dim3(256), 0, s, n, g->gcol, g->gcnt, g->cs, g->dsi, g->rv, g->zt);
     hipLaunchKernelGGL(kGnnAddFloat, dim3(gb), dim3(256), 0, s, a, g->zt, ctx->D[DPPEX]);
+    ctx->r0Fresh = false;   // the guess has moved
     MPS_HIP(hipGetLastError());
     return MPS_OK;
 }

@@ -248,6 +248,7 @@ struct Ctx {
     double ppeMs = 0.0;
     int fallbackUsed = 0;
     double learnedStallRatio = 0.5;   // learned preconditioner: largest residual ratio of a refinement pass
+    double learnedTolScale = 0.1;     // learned preconditioner: true-residual target as a multiple of the tolerance
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // optional per-launch timing of the SpMV kernel (HIP events on the
     // context stream); enabled with mps_set_scalar(ctx, "profile", 1)

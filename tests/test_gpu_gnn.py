@@ -135,10 +135,12 @@ def test_learned_solve_reaches_tolerance(built, mode):
 
 def test_trained_network_gain(built):
     """The committed network (polymps_amd/data/gnn_dam3d.bin, trained by
-    polymps_amd/train_gnn.py on columns of seeds 0-5) against the Jacobi
-    operator it starts from, on a held-out column (seed 950): its objective
-    |A M v - v|^2 / |v|^2 is lower on the same states and vectors, and the
-    learned solves still reach the tolerance."""
+    polymps_amd/train_gnn.py on 5,000 systems of columns of seeds 0-49)
+    against the Jacobi operator it starts from, on a held-out column (seed
+    950): its objective |A M v - v|^2 / |v|^2 is lower on the same states and
+    vectors, the flexible CG it preconditions needs fewer iterations than
+    the Jacobi PCG on the same system (to a 10x smaller residual) without
+    falling back, and the learned steps reach the tolerance."""
     from pathlib import Path
     from polymps_amd.train_gnn import column_workload
     weights = Path(__file__).resolve().parent.parent / "polymps_amd" / "data" / "gnn_dam3d.bin"
@@ -158,7 +160,17 @@ def test_trained_network_gain(built):
     print(f"objective trained {np.mean(trained):.4g} jacobi {np.mean(jacobi):.4g}", flush=True)
     assert np.mean(trained) < 0.5 * np.mean(jacobi)
     g.gnn_load(weights)
-    g.set_scalar("learned_mode", 2)
+    x0 = g.get("ppe_x")
+    g.set_scalar("learned_mode", 0)
+    g.call("ppe_assemble")
+    g.call("ppe_solve")
+    jacobi_it = g.scalar("solver_iter")
+    _learned_solve(g, x0)
+    learned_it = g.scalar("solver_iter")
+    print(f"iterations: Jacobi PCG {jacobi_it:.0f}, learned flexible CG {learned_it:.0f}", flush=True)
+    assert g.scalar("fallback") == 0
+    assert learned_it < jacobi_it
+    g.set("ppe_x", x0)
     g.call("step")
     assert g.scalar("true_residual") < w.params.solver_tol
 
