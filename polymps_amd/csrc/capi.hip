@@ -546,7 +546,7 @@ int mps_destroy(mps_ctx* ctx) {
                     ctx->aDiag32, ctx->cgR, ctx->cgP,
                     ctx->cgZ, ctx->cgQ, ctx->cgInvD, ctx->cgX, ctx->fR, ctx->fP, ctx->fZ, ctx->fQ, ctx->fX,
                     ctx->bcgR0, ctx->bcgY, ctx->bcgS, ctx->bcgT, ctx->patchList, ctx->fixList, ctx->V4, ctx->nbrMaskS, ctx->nbrS, ctx->nbrCntS,
-                    ctx->ds, ctx->is, ctx->red, ctx->asmRed, ctx->IaltPtype, ctx->bedKeys, ctx->bedIdx};
+                    ctx->ds, ctx->is, ctx->red, ctx->asmRed, ctx->red64, ctx->IaltPtype, ctx->bedKeys, ctx->bedIdx};
     for (void* b : bufs)
         if (b) hipFree(b);
     freeSlot(ctx->extraMesh[0]);
@@ -765,9 +765,8 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "solver_failures") *out = (double)ctx->solverFailures;
     else if (s == "list_max") {
         // the longest row of the current neighbour list
-        const int rc2 = reduceCounts(ctx, ctx->nbrCnt, ctx->N, I_LIST_MAX, true);
+        const int rc2 = reduceCounts(ctx, ctx->nbrCnt, ctx->N, true, out);
         if (rc2) return rc2;
-        *out = ctx->hostInts[I_LIST_MAX];
     }
     else if (s == "list_skin") *out = ctx->lastSkin / ctx->P.part_dist;
     else if (s == "list_start") *out = ctx->lastStartList ? 1.0 : 0.0;
@@ -779,9 +778,8 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "capacity") *out = ctx->N;
     else if (s == "nnz") {
         // entries of the last PPE assembly
-        const int rc2 = reduceCounts(ctx, ctx->aCnt, ctx->N, I_NNZ, false);
+        const int rc2 = reduceCounts(ctx, ctx->aCnt, ctx->N, false, out);
         if (rc2) return rc2;
-        *out = ctx->hostInts[I_NNZ];
     }
     else if (s == "nnz_escaped") *out = ctx->hostInts[I_NESC];
     else if (s == "list_entries" || s == "list_entries_small") {

@@ -51,7 +51,7 @@ enum IField { ITYPE, ICANON, IBC, INNEIGH, NPI, INNWALL = NPI, INEARWALL, INEARM
 enum DScalar { S_VELMAX, S_ABSNEW, S_ABSOLD, S_RHSN2, S_THR, S_RN2, S_ALPHA, S_BETA, S_PAP, S_BN2, S_TRUE_RN2,
                S_RELTOL, S_RHO, S_RHOOLD, S_BW, S_R0N2, NDS };
 // device scalar slots (int)
-enum IScalar { I_NLIVE, I_OUTCNT, I_OVERFLOW, I_CG_ITER, I_CG_DONE, I_COMPACT_OVF, I_LIST_MAX, I_NNZ, I_NESC,
+enum IScalar { I_NLIVE, I_OUTCNT, I_OVERFLOW, I_CG_ITER, I_CG_DONE, I_COMPACT_OVF, I_NESC,
                I_BCG_RESTART, I_BCG_RESTARTS, I_REBUILD, I_PATCHCNT, I_FIXCNT, NIS };
 
 constexpr int kWallTable = 50;
@@ -235,6 +235,8 @@ struct Ctx {
     bool r0Fresh = false;        // cgR and asmRed hold b - A x of the current guess and its partials (assembly)
     bool r0FreshMixed = false;   // ... in the mixed solve's partial order (else the FP64 CG's, with 1/diag)
     double* asmRed = nullptr;    // block partials of the assembly's first residual (no other kernel writes them)
+    unsigned long long* red64 = nullptr;   // result of reduceCounts (allocated on first use)
+    double nnzHost = 0.0;        // PPE entries of the last assembly (when profiling or asked for)
     // solver kind of the last PPE assembly (MPS_SOLVER_* / precision), -1 when
     // there is none for the current slot order; a solve must match it
     int asmMode = -1;
@@ -285,9 +287,8 @@ int ppeAssemble(Ctx* ctx);
 int ppeSolve(Ctx* ctx);
 int ppeWarmStart(Ctx* ctx);
 int launchCompaction(Ctx* ctx);
-// sum (max = false) or maximum of the n counts cnt[] into is[slot], then to
-// hostInts[slot] (synchronises the stream)
-int reduceCounts(Ctx* ctx, const int* cnt, int n, int slot, bool max);
+// sum (max = false) or maximum of the n counts cnt[] (synchronises the stream)
+int reduceCounts(Ctx* ctx, const int* cnt, int n, bool max, double* out);
 int launchClosestPoint(Ctx* ctx, int type);   // one mesh type (0 rigid, 1 deformable, 2 forced)
 // granular phase (granular.hip)
 void granularGridSize(const mps_params& P, int* kx, int* ky);

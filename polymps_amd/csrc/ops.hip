@@ -2108,34 +2108,40 @@ int launchListKeepRefresh(Ctx* ctx, double skin) {
 }
 
 // Sum or maximum of n counts: grid-stride partial per thread, a block
-// reduction, then one atomic per workgroup.
-__global__ __launch_bounds__(256) void kReduceCounts(const int* __restrict__ cnt, int n, int* out, int isMax) {
-    __shared__ long long sh[4];
-    long long v = 0;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-        v = isMax ? (long long)max((int)v, cnt[k]) : v + cnt[k];
+// reduction, then one 64-bit atomic per workgroup.
+__global__ __launch_bounds__(256) void kReduceCounts(const int* __restrict__ cnt, int n, unsigned long long* out,
+                                                     int isMax) {
+    __shared__ unsigned long long sh[4];
+    unsigned long long v = 0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const unsigned long long c = (unsigned long long)max(cnt[k], 0);
+        v = isMax ? (c > v ? c : v) : v + c;
+    }
     for (int o = 32; o > 0; o >>= 1) {
-        const long long w = __shfl_xor(v, o, 64);
+        const unsigned long long w = __shfl_xor(v, o, 64);
         v = isMax ? (w > v ? w : v) : v + w;
     }
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
-        long long t = sh[0];
+        unsigned long long t = sh[0];
         for (int q = 1; q < 4; q++) t = isMax ? (sh[q] > t ? sh[q] : t) : t + sh[q];
-        if (isMax) atomicMax(out, (int)t);
-        else atomicAdd(out, (int)t);
+        if (isMax) atomicMax(out, t);
+        else atomicAdd(out, t);
     }
 }
 
-int reduceCounts(Ctx* ctx, const int* cnt, int n, int slot, bool max) {
-    MPS_HIP(hipMemsetAsync(ctx->is + slot, 0, sizeof(int), ctx->stream));
+int reduceCounts(Ctx* ctx, const int* cnt, int n, bool max, double* out) {
+    if (!ctx->red64) MPS_HIP(hipMalloc(&ctx->red64, sizeof(unsigned long long)));
+    MPS_HIP(hipMemsetAsync(ctx->red64, 0, sizeof(unsigned long long), ctx->stream));
     if (cnt != nullptr && n > 0)
         hipLaunchKernelGGL(kReduceCounts, dim3(std::min(gridFor(n, 256), 1024)), dim3(256), 0, ctx->stream, cnt, n,
-                           ctx->is + slot, max ? 1 : 0);
+                           ctx->red64, max ? 1 : 0);
     MPS_HIP(hipGetLastError());
-    MPS_HIP(hipMemcpyAsync(ctx->hostInts + slot, ctx->is + slot, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    unsigned long long h = 0;
+    MPS_HIP(hipMemcpyAsync(&h, ctx->red64, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
     MPS_HIP(hipStreamSynchronize(ctx->stream));
+    *out = (double)h;
     return MPS_OK;
 }
 

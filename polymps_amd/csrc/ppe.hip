@@ -784,9 +784,9 @@ int ppeAssemble(Ctx* ctx) {
     float* v32 = mixed ? ctx->aVal32 : nullptr;
     float* d32 = mixed ? ctx->aDiag32 : nullptr;
     float* i32 = mixed ? ctx->aDiagInv32 : nullptr;
-    // I_NESC counts the escaped columns; the entry count I_NNZ is summed from
-    // the row counts when asked for (one same-address atomic per wavefront
-    // here would serialise in one L2 channel)
+    // I_NESC counts the escaped columns; the entry count is summed from the
+    // row counts when asked for (reduceCounts), not by one same-address
+    // atomic per wavefront here
     MPS_HIP(hipMemsetAsync(ctx->is + I_NESC, 0, sizeof(int), ctx->stream));
     // the CG solve's first true residual comes out of the assembly (with the
     // learned warm start it is the residual the warm start corrects)
@@ -1114,10 +1114,10 @@ static int solveFp64(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int batc
     const int D = ctx->hostInts[I_CG_DONE];
     ctx->lastDone = D;
     if (ctx->profile) {
-        const int rcN = reduceCounts(ctx, ctx->aCnt, ctx->N, I_NNZ, false);
+        const int rcN = reduceCounts(ctx, ctx->aCnt, ctx->N, false, &ctx->nnzHost);
         if (rcN) return rcN;
         addSpmvStats(ctx, times, (D == 1 || D == 3) ? iters + 1 : 0,
-                     10.0 * ctx->hostInts[I_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 28.0 * ctx->nLiveHost);
+                     10.0 * ctx->nnzHost + 4.0 * ctx->hostInts[I_NESC] + 28.0 * ctx->nLiveHost);
     }
     ctx->solverIter = iters;
     ctx->outerIter = 1;
@@ -1144,10 +1144,10 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
     double prevRn2 = 0.0;
     ctx->lastDone = 0;
     if (ctx->profile) {
-        rc = reduceCounts(ctx, ctx->aCnt, ctx->N, I_NNZ, false);
+        rc = reduceCounts(ctx, ctx->aCnt, ctx->N, false, &ctx->nnzHost);
         if (rc) return rc;
     }
-    const double bytes32 = 6.0 * ctx->hostInts[I_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 16.0 * ctx->nLiveHost;
+    const double bytes32 = 6.0 * ctx->nnzHost + 4.0 * ctx->hostInts[I_NESC] + 16.0 * ctx->nLiveHost;
     for (;;) {
         // the first residual may come from the assembly (r and its partials)
         const bool fresh = outer == 0 && ctx->r0Fresh && ctx->r0FreshMixed;
