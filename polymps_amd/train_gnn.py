@@ -128,9 +128,9 @@ def evaluate(weights: Path, seeds, l0: float, warm_steps: int, solves: int, log=
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--out", type=Path, default=ROOT / "polymps_amd" / "data" / "gnn_dam3d.bin")
-    # 50 runs x 100 steps: 5,000 training systems
-    ap.add_argument("--runs", type=int, default=50)
-    ap.add_argument("--steps", type=int, default=100)
+    # 17 runs x 300 steps (the whole collapse and impact): 5,100 training systems
+    ap.add_argument("--runs", type=int, default=17)
+    ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--l0", type=float, default=0.005)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--eval-seeds", type=int, nargs="*", default=[901, 902])

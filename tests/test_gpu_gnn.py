@@ -135,7 +135,7 @@ def test_learned_solve_reaches_tolerance(built, mode):
 
 def test_trained_network_gain(built):
     """The committed network (polymps_amd/data/gnn_dam3d.bin, trained by
-    polymps_amd/train_gnn.py on 5,000 systems of columns of seeds 0-49)
+    polymps_amd/train_gnn.py on 5,100 systems of columns of seeds 0-16)
     against the Jacobi operator it starts from, on a held-out column (seed
     950): its objective |A M v - v|^2 / |v|^2 is lower on the same states and
     vectors, the flexible CG it preconditions needs fewer iterations than
