@@ -20,9 +20,34 @@ constexpr int kBlock = 256;
 // (x, y, z, type) copy that list consumers gather.  Word (r, w) of slot i is
 // at mask[(r * maskW + w) * N + i]; bit b marks candidate slot
 // rowStart(r) + 32 w + b of stencil row r as inside the radius.  Every lane
-// writes its words in row order, so the stores coalesce.  One-wave
-// workgroups (coopRows).
+// writes its words in row order, so the stores coalesce.
+//
+// One wavefront per workgroup.  Per stencil row the wave stages the union of
+// its lanes' candidate slots in LDS, 64 at a time, and every lane tests every
+// staged candidate in a wave-uniform loop: the candidate's coordinates are
+// broadcast LDS reads and each test shifts one bit into a 32-bit word of the
+// union (u = 2u + bit, bit-reversed once per word).  A lane's own words are
+// then cut out of two consecutive union words by one funnel shift, which
+// drops the union's candidates before the lane's row start; the candidates
+// after its row end and the particle itself are masked off per word.  Slots
+// past the union's end are staged at infinity, so they never set a bit.
 constexpr int kWave = 64;
+
+// u = 2u + c: the lane's bit of the wave's compare mask shifted into u by
+// one carry-in add (the compiler's select + shift + or takes two).
+__device__ __forceinline__ unsigned shiftIn(unsigned u, bool c) {
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(c);
+    unsigned r;
+    unsigned long long carry;
+    asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry) : "v"(u), "s"(m));
+    return r;
+}
+
+// Word w of a lane's row from union words `hi` (later) and `lo` (earlier),
+// the lane's row starting `sh` bits into `lo`.
+__device__ __forceinline__ unsigned laneWord(unsigned hi, unsigned lo, int sh) {
+    return __builtin_amdgcn_alignbit(hi, lo, (unsigned)sh);
+}
 
 // A second mask, maskS, marks the candidates inside the small radius r2s
 // (<= r2) for the small-radius list (SMALL; without it only the large list
@@ -30,9 +55,10 @@ constexpr int kWave = 64;
 template <bool SMALL>
 __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int* cnt, unsigned* maskS, int* cntS,
                                                     double r2, double r2s, const int* gate, int* fixList) {
-    if (gate != nullptr && *gate == 0) return;   // conditional rebuild not needed
+    if (gate != nullptr && *gate == 0) return;
     __shared__ double4 stage[kWave];
-    const int i = blockIdx.x * kWave + threadIdx.x;
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * kWave + t;
     const bool inRange = i < a.N;
     const double xi = inRange ? PX(i) : 0.0, yi = inRange ? PY(i) : 0.0, zi = inRange ? PZ(i) : 0.0;
     if (inRange) a.P4[i] = make_double4(xi, yi, zi, (double)a.I[ITYPE][i]);
@@ -46,54 +72,130 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
     const double* __restrict__ Z = a.D[DZ];
     const size_t N = a.N;
     const int W = a.maskW;
-    unsigned* out = mask + (act ? i : 0);
-    unsigned* outS = maskS + (act ? i : 0);
+    const mps_params& P = a.P;
+    const int ix = (int)((xi - P.domain_min[0]) * P.inv_bucket_side + KEPS);
+    const int iy = (int)((yi - P.domain_min[1]) * P.inv_bucket_side + KEPS);
+    const int iz = (int)((zi - P.domain_min[2]) * P.inv_bucket_side + KEPS);
+    const int nx = P.num_buckets_x, ny = P.num_buckets_y, nz = P.num_buckets_z;
+    const int rows = ((int)P.dim == 3) ? 9 : 3;
+    const double inf = __builtin_inf();
     int total = 0, totalS = 0, over = 0;
-    unsigned cur = 0, curS = 0;
-    int w = 0, row = 0, s0 = 0;
-    coopRows(
-        a, act, xi, yi, zi, stage, 2 * 32 * W,
-        [&](int j) { return make_double4(X[j], Y[j], Z[j], 0.0); },
-        [&](int r, int& s, int& e) {
+    for (int r = 0; r < rows; r++) {
+        // the lane's candidate slots [s, e) of row r (the walk's row order)
+        int s = 0, e = 0;
+        if (act) {
+            const int jz = ((int)P.dim == 3) ? iz - 1 + r / 3 : 0;
+            const int jy = iy - 1 + r % 3;
+            if (!(jz < 0 || jz >= nz || jy < 0 || jy >= ny)) {
+                const int row = jz * P.num_buckets_xy + jy * nx;
+                int jx0 = ix - 1, jx1 = ix + 1;
+                if (jx0 < 0) jx0 = 0;
+                if (jx1 >= nx) jx1 = nx - 1;
+                if (jx0 <= jx1) {
+                    s = a.cellStart[row + jx0];
+                    e = a.cellStart[row + jx1 + 1];
+                }
+            }
             if (e - s > 32 * W) {
                 over = max(over, e - s);
                 e = s + 32 * W;
             }
-            cur = 0;
-            curS = 0;
-            w = 0;
-            row = r;
-            s0 = s;
-        },
-        [&](int j, const double4& q) {
-            const double v0 = q.x - xi, v1 = q.y - yi, v2 = q.z - zi;
-            const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-            const int k = j - s0;
-            cur |= ((j != i && d2 < r2) ? 1u : 0u) << (k & 31);
-            if (SMALL) curS |= ((j != i && d2 < r2s) ? 1u : 0u) << (k & 31);
-            if ((k & 31) == 31) {
-                out[(size_t)(row * W + w) * N] = cur;
-                if (SMALL) outS[(size_t)(row * W + w) * N] = curS;
-                total += __builtin_popcount(cur);
-                if (SMALL) totalS += __builtin_popcount(curS);
-                w++;
-                cur = 0;
-                curS = 0;
+        }
+        const int len = e - s;
+        unsigned* const o1 = mask + (size_t)r * W * N + (act ? i : 0);
+        unsigned* const o2 = maskS + (size_t)r * W * N + (act ? i : 0);
+        int w = 0;   // words of the row stored
+        // store word w: the lane's candidates only (before its row end, not itself)
+        auto put = [&](unsigned v, unsigned vs) {
+            const int rem = len - 32 * w;
+            const int sk = i - s - 32 * w;
+            const unsigned drop = (rem < 32 ? ~0u << rem : 0u) | ((unsigned)sk < 32u ? 1u << sk : 0u);
+            v &= ~drop;
+            o1[(size_t)w * N] = v;
+            total += __builtin_popcount(v);
+            if (SMALL) {
+                vs &= ~drop;
+                o2[(size_t)w * N] = vs;
+                totalS += __builtin_popcount(vs);
             }
-        },
-        [&](int r, int s, int e) {
-            if ((e - s) & 31) {
-                out[(size_t)(r * W + w) * N] = cur;
-                if (SMALL) outS[(size_t)(r * W + w) * N] = curS;
-                total += __builtin_popcount(cur);
-                if (SMALL) totalS += __builtin_popcount(curS);
-                w++;
+            w++;
+        };
+        const int us = __builtin_amdgcn_readfirstlane(waveMinInt(len > 0 ? s : INT_MAX));
+        const int ue = __builtin_amdgcn_readfirstlane(waveMaxInt(len > 0 ? e : INT_MIN));
+        if (us < ue && ue - us <= 2 * 32 * W) {
+            const int off = s - us;
+            const int sh = off & 31, q0 = off >> 5;
+            unsigned prev = 0, prevS = 0;
+            int c = 0;   // union words tested
+            // the next 64 candidates are loaded while the current ones are tested
+            double nx0 = inf, ny0 = inf, nz0 = inf;
+            if (us + t < ue) {
+                nx0 = X[us + t];
+                ny0 = Y[us + t];
+                nz0 = Z[us + t];
             }
+            for (int cb = us; cb < ue; cb += kWave) {
+                __syncthreads();
+                stage[t] = make_double4(nx0, ny0, nz0, 0.0);
+                __syncthreads();
+                const int jn = cb + kWave + t;
+                if (jn < ue) {
+                    nx0 = X[jn];
+                    ny0 = Y[jn];
+                    nz0 = Z[jn];
+                } else {
+                    nx0 = inf;
+                    ny0 = inf;
+                    nz0 = inf;
+                }
+                const int m = min(kWave, ue - cb);
+                for (int h = 0; h < m; h += 32) {
+                    const int mm = min(32, m - h);
+                    unsigned U = 0, US = 0;
+                    int k = 0;
+                    for (; k < mm; k += 8) {
+#pragma unroll
+                        for (int u = 0; u < 8; u++) {
+                            const double4 q = stage[h + k + u];
+                            const double v0 = q.x - xi, v1 = q.y - yi, v2 = q.z - zi;
+                            const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+                            U = shiftIn(U, d2 < r2);
+                            if (SMALL) US = shiftIn(US, d2 < r2s);
+                        }
+                    }
+                    // candidate h + q of the chunk at bit q
+                    U = __builtin_bitreverse32(U << (32 - k));
+                    if (SMALL) US = __builtin_bitreverse32(US << (32 - k));
+                    if (c > q0 && 32 * w < len) put(laneWord(U, prev, sh), laneWord(US, prevS, sh));
+                    prev = U;
+                    prevS = US;
+                    c++;
+                }
+            }
+            if (c > q0 && 32 * w < len) put(laneWord(0u, prev, sh), laneWord(0u, prevS, sh));
+        } else if (len > 0) {
+            // a union too wide for the shared loop (lanes in distant rows): per lane
+            unsigned v = 0, vs = 0;
+            for (int j = s; j < e; j++) {
+                const double v0 = X[j] - xi, v1 = Y[j] - yi, v2 = Z[j] - zi;
+                const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+                const int k = (j - s) & 31;
+                v |= (d2 < r2 ? 1u : 0u) << k;
+                if (SMALL) vs |= (d2 < r2s ? 1u : 0u) << k;
+                if (k == 31 || j == e - 1) {
+                    put(v, vs);
+                    v = 0;
+                    vs = 0;
+                }
+            }
+        }
+        if (act) {
             for (; w < W; w++) {
-                out[(size_t)(r * W + w) * N] = 0u;
-                if (SMALL) outS[(size_t)(r * W + w) * N] = 0u;
+                o1[(size_t)w * N] = 0u;
+                if (SMALL) o2[(size_t)w * N] = 0u;
             }
-        });
+        }
+    }
     if (!act) return;
     // a stencil row with more candidates than the mask words hold (dense
     // clusters): the particle's lists are rebuilt by kFixList's direct walk
