@@ -488,7 +488,9 @@ int mps_create(mps_ctx** out, const mps_params* p, int n, int device) {
     MPS_HIP(hipMemsetAsync(ctx->nbrCntS, 0, ni, ctx->stream));
     MPS_HIP(hipMalloc(&ctx->P4, sizeof(double4) * (size_t)n));
     MPS_HIP(hipMalloc(&ctx->aVal, sizeof(double) * (size_t)ctx->K * n));
-    MPS_HIP(hipMalloc(&ctx->aCol, sizeof(unsigned short) * (size_t)ctx->K * n));
+    // K (a multiple of 32) entries per row, and the codes one 16-entry batch
+    // more: the row kernels read one batch of codes past a row's end
+    MPS_HIP(hipMalloc(&ctx->aCol, sizeof(unsigned short) * (size_t)(ctx->K + kPpeCodePad) * n));
     MPS_HIP(hipMalloc(&ctx->aEsc, sizeof(int) * (size_t)kPpeEsc * n));
     MPS_HIP(hipMalloc(&ctx->aCnt, ni));
     MPS_HIP(hipMemsetAsync(ctx->aCnt, 0, ni, ctx->stream));   // nnz reads 0 before the first assembly
@@ -542,7 +544,8 @@ int mps_destroy(mps_ctx* ctx) {
         if (ctx->DaltNN[q]) hipFree(ctx->DaltNN[q]);
     void* bufs[] = {ctx->cellStart, ctx->cellCount, ctx->key, ctx->rank, ctx->order, ctx->order2, ctx->slotOf,
                     ctx->bvh, ctx->bvhTri, ctx->scanTmp, ctx->outList, ctx->nbrMask, ctx->nbr, ctx->nbrCnt, ctx->P4, ctx->mV, ctx->mF, ctx->mN, ctx->wtab,
-                    ctx->aVal, ctx->aCol, ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->aVal32, ctx->aDiagInv32,
+                    ctx->aVal, ctx->aCol, ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->aVal32, ctx->aLcol, ctx->aWin,
+                    ctx->aDiagInv32,
                     ctx->aDiag32, ctx->cgR, ctx->cgP,
                     ctx->cgZ, ctx->cgQ, ctx->cgInvD, ctx->cgX, ctx->fR, ctx->fP, ctx->fZ, ctx->fQ, ctx->fX,
                     ctx->bcgR0, ctx->bcgY, ctx->bcgS, ctx->bcgT, ctx->patchList, ctx->fixList, ctx->V4, ctx->nbrMaskS, ctx->nbrS, ctx->nbrCntS,
@@ -757,6 +760,11 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "fallback") *out = ctx->fallbackUsed;
     else if (s == "learned_stall_ratio") *out = ctx->learnedStallRatio;
     else if (s == "learned_tol_scale") *out = ctx->learnedTolScale;
+    else if (s == "ppe_window") *out = ctx->ppeWindow ? 1.0 : 0.0;
+    else if (s == "ppe_window_blocks") {
+        const int rc2 = ppeWindowBlocks(ctx, out);
+        if (rc2) return rc2;
+    }
     else if (s == "solver_failed") *out = ctx->solverFailed;
     else if (s == "cfl_visc") {
         const int rc2 = granularCflVisc(ctx, out);
@@ -846,6 +854,15 @@ int mps_set_scalar(mps_ctx* ctx, const char* name, double value) {
             return MPS_E_ARG;
         }
         ctx->learnedTolScale = value;
+    } else if (s == "ppe_window") {
+        // the mixed inner solve's SpMV gathers from per-block LDS windows
+        // (1, default) or from global memory (0); same results, from the
+        // next assembly on
+        if (!(value == 0.0 || value == 1.0)) {
+            ctx->err = "ppe_window must be 0 or 1";
+            return MPS_E_ARG;
+        }
+        ctx->ppeWindow = value != 0.0;
     } else if (s == "mask_words") {
         // candidate window of a stencil row in 32-slot words (at most the
         // allocated width); rows beyond it are listed by a direct walk

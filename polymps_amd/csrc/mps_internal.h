@@ -58,6 +58,7 @@ constexpr int kWallTable = 50;
 // escaped (non-delta) columns per PPE row: the first entry and stencil-row
 // jumps of 65536 slots or more, at most 1 + 8
 constexpr int kPpeEsc = 12;
+constexpr int kPpeCodePad = 16;   // code entries per row past K (one read-ahead batch)
 constexpr int kRedBlocks = 1024;   // partial-sum slots of a reduction
 
 // Bounding-volume hierarchy over the wall triangles: a leaf holds the
@@ -198,6 +199,10 @@ struct Ctx {
     int* aCnt = nullptr;        // N
     double* aDiag = nullptr;    // N
     float* aVal32 = nullptr;    // [K / 4][N][4] (ellIdx4, mixed precision)
+    unsigned short* aLcol = nullptr;  // [K / 4][N][4] (ellIdx4) entry's index in its block's LDS window (mixed)
+    int* aWin = nullptr;              // per 256-row block: the LDS window table of the mixed SpMV
+    bool winValid = false;            // the last assembly wrote windows
+    bool ppeWindow = true;            // scalar ppe_window: build and use the windows
     float* aDiagInv32 = nullptr;
     float* aDiag32 = nullptr;
     double* cgR = nullptr;
@@ -288,6 +293,7 @@ int ppeSolve(Ctx* ctx);
 int ppeWarmStart(Ctx* ctx);
 int launchCompaction(Ctx* ctx);
 // sum (max = false) or maximum of the n counts cnt[] (synchronises the stream)
+int ppeWindowBlocks(Ctx* ctx, double* out);
 int reduceCounts(Ctx* ctx, const int* cnt, int n, bool max, double* out);
 int launchClosestPoint(Ctx* ctx, int type);   // one mesh type (0 rigid, 1 deformable, 2 forced)
 // granular phase (granular.hip)

@@ -17,6 +17,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <vector>
 
 #include "kern_common.h"
 
@@ -24,16 +25,107 @@ namespace mpsgpu {
 
 constexpr int kB = 256;
 
+// LDS window of a 256-row block for the FP32 SpMV of the inner solve.  The
+// columns of a block's rows lie in the union, over its rows, of their
+// stencil rows' slot ranges (three in 2-D, nine in 3-D); merged into
+// disjoint ascending segments these hold a few thousand slots, which the
+// SpMV copies from the vector into LDS with coalesced loads before its rows
+// gather from LDS instead of from the caches (the gathers of a wave scatter
+// over a stencil row's slots, so from global memory each touches many cache
+// lines).  The assembly writes, next to the column code, each entry's index
+// in its block's window, and a table per block:
+//   [0] 1 = no window (wider than kWinMax, a periodic domain, or a column
+//       outside it): the block's rows gather from global memory;
+//   [1] segment count m, [2 + k] first slot of segment k,
+//   [11 + k] its LDS offset (k = 0 .. m; offset m = the window's size).
+constexpr int kWinMax = 5000;    // slots: 20 KB of FP32 values per block
+constexpr int kWinStride = 24;   // ints per block in the table
+
+// The block's window from its rows' current stencil rows (rows with
+// entries only), merged; every thread returns the segments in sS/sE/sB.
+__device__ __forceinline__ void ppeWindowBegin(const KArgs& a, int i, int* sS, int* sE, int* sB, int& sM,
+                                               int& sOut) {
+    __shared__ int ps[kB / 64][9], pe[kB / 64][9];
+    const mps_params& P = a.P;
+    const int rows = ((int)P.dim == 3) ? 9 : 3;
+    bool has = i < a.N && live(a, i);
+    if (has) {
+        const int bci = a.I[IBC][i];
+        has = !(bci == P.other || bci == P.surface);
+    }
+    const double xi = has ? a.D[DX][i] : 0.0, yi = has ? a.D[DY][i] : 0.0, zi = has ? a.D[DZ][i] : 0.0;
+    const int ix = (int)((xi - P.domain_min[0]) * P.inv_bucket_side + KEPS);
+    const int iy = (int)((yi - P.domain_min[1]) * P.inv_bucket_side + KEPS);
+    const int iz = (int)((zi - P.domain_min[2]) * P.inv_bucket_side + KEPS);
+    const int w = threadIdx.x >> 6;
+    for (int r = 0; r < rows; r++) {
+        int s = 0, e = 0;
+        if (has) {
+            const int jz = ((int)P.dim == 3) ? iz - 1 + r / 3 : 0;
+            const int jy = iy - 1 + r % 3;
+            if (!(jz < 0 || jz >= P.num_buckets_z || jy < 0 || jy >= P.num_buckets_y)) {
+                const int row = jz * P.num_buckets_xy + jy * P.num_buckets_x;
+                int jx0 = ix - 1, jx1 = ix + 1;
+                if (jx0 < 0) jx0 = 0;
+                if (jx1 >= P.num_buckets_x) jx1 = P.num_buckets_x - 1;
+                if (jx0 <= jx1) {
+                    s = a.cellStart[row + jx0];
+                    e = a.cellStart[row + jx1 + 1];
+                }
+            }
+        }
+        const int mn = waveMinInt(e > s ? s : INT_MAX);
+        const int mx = waveMaxInt(e > s ? e : INT_MIN);
+        if ((threadIdx.x & 63) == 0) {
+            ps[w][r] = mn;
+            pe[w][r] = mx;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // per stencil row the union over the block; starts and ends ascend
+        // with the row, so overlapping neighbours merge in one pass
+        int m = 0, W = 0;
+        for (int r = 0; r < rows; r++) {
+            int s = INT_MAX, e = INT_MIN;
+            for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
+                s = min(s, ps[q][r]);
+                e = max(e, pe[q][r]);
+            }
+            if (s >= e) continue;
+            if (m > 0 && s <= sE[m - 1]) {
+                if (e > sE[m - 1]) {
+                    W += e - sE[m - 1];
+                    sE[m - 1] = e;
+                }
+            } else {
+                sS[m] = s;
+                sE[m] = e;
+                sB[m] = W;
+                W += e - s;
+                m++;
+            }
+        }
+        sB[m] = W;
+        sM = m;
+        sOut = (W > kWinMax || periodicDomain(P)) ? 1 : 0;
+    }
+    __syncthreads();
+}
+
 // RES: the row also gives the FP64 true residual r_i = b_i - (A x)_i of the
 // initial guess x (the refinement loop's first residual, which kTrueResidual
 // would otherwise compute from the stored row), and |r_i|^2, |b_i|^2 for the
 // block partials.  kTrueResidual sums diag x_i first; here the off-diagonal
 // products are summed first: the same FP64 quantity, rounded differently.
+// lcol (with a window): each entry's index in the block's LDS window
+// (segments wS/wE/wB, wM of them); outside = some column is not in it.
 template <bool LIST, bool TRANS, bool RES>
 __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, unsigned short* col, int* esc,
                                             int* cnt, double* diag, double* b, float* val32, float* diag32,
                                             float* invd32, double* invd64, double* x, double* r, double& rn2,
-                                            double& bn2) {
+                                            double& bn2, unsigned short* lcol, const int* wS, const int* wE,
+                                            const int* wB, int wM, bool& outside) {
     if (!live(a, i)) {
         cnt[i] = 0; diag[i] = 1.0; b[i] = 0.0;
         if (invd32) { invd32[i] = 1.0f; diag32[i] = 1.0f; }
@@ -74,6 +166,14 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
     typedef unsigned short U4 __attribute__((ext_vector_type(4)));
     F4 g32 = {0.0f, 0.0f, 0.0f, 0.0f};   // the current group of four FP32 values / codes
     U4 gcd = {0, 0, 0, 0};
+    U4 glc = {0, 0, 0, 0};                // and window indices
+    // the window segment of the current column (columns ascend)
+    int wk = 0, curS = 0, curE = 0, curB = 0;
+    if (lcol && wM > 0) {
+        curS = wS[0];
+        curE = wE[0];
+        curB = wB[0];
+    }
     const bool periodic = periodicDomain(P);
     // with the list, the inner and far-from-wall tests of j come with its
     // position (P4.w flags of update_bc) instead of two more gathers
@@ -152,16 +252,30 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
                 // FP32 values and codes leave in groups of four (ellIdx4)
                 const float f = (float)mv;
                 const unsigned short cd = delta ? (unsigned short)dj : (unsigned short)0;
+                unsigned short lc = 0;
+                if (lcol) {
+                    while (wk < wM && j >= curE) {
+                        wk++;
+                        if (wk < wM) {
+                            curS = wS[wk];
+                            curE = wE[wk];
+                            curB = wB[wk];
+                        }
+                    }
+                    if (wk < wM && j >= curS) lc = (unsigned short)(curB + (j - curS));
+                    else outside = true;
+                }
                 switch (c & 3) {
-                    case 0: g32.x = f; gcd.x = cd; break;
-                    case 1: g32.y = f; gcd.y = cd; break;
-                    case 2: g32.z = f; gcd.z = cd; break;
-                    default: g32.w = f; gcd.w = cd; break;
+                    case 0: g32.x = f; gcd.x = cd; glc.x = lc; break;
+                    case 1: g32.y = f; gcd.y = cd; glc.y = lc; break;
+                    case 2: g32.z = f; gcd.z = cd; glc.z = lc; break;
+                    default: g32.w = f; gcd.w = cd; glc.w = lc; break;
                 }
                 if ((c & 3) == 3) {
                     const size_t gi = (size_t)(c >> 2) * a.N + i;
                     if (val32) reinterpret_cast<F4*>(val32)[gi] = g32;
                     reinterpret_cast<U4*>(col)[gi] = gcd;
+                    if (lcol) reinterpret_cast<U4*>(lcol)[gi] = glc;
                 }
                 if (RES) ax += mv * xOf();
                 if (!delta) {
@@ -190,6 +304,7 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
         const size_t gi = (size_t)(c >> 2) * a.N + i;
         if (val32) reinterpret_cast<F4*>(val32)[gi] = g32;
         reinterpret_cast<U4*>(col)[gi] = gcd;
+        if (lcol) reinterpret_cast<U4*>(lcol)[gi] = glc;
     }
     if (full) atomicMax(&a.is[I_OVERFLOW], c + 1);   // reported as a capacity error after the step
     if (ne) atomicAdd(&a.is[I_NESC], ne);
@@ -219,7 +334,7 @@ template <bool LIST, bool TRANS, bool RES>
 __global__ __launch_bounds__(256) void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc, int* cnt,
                                                     double* diag, double* b, float* val32, float* diag32,
                                                     float* invd32, double* invd64, double* x, double* r,
-                                                    int bFirst);
+                                                    int bFirst, unsigned short* lcol, int* win);
 
 // ------------------------------------------------------------ reductions
 // Every vector kernel runs one thread per row; each block leaves a pair of
@@ -243,16 +358,31 @@ __device__ __forceinline__ void writePartials(const KArgs& a, double s0, double 
     if (threadIdx.x == 0) { a.red[blockIdx.x * 2] = s0; a.red[blockIdx.x * 2 + 1] = s1; }
 }
 
+// lcol / win (mixed precision): window indices and the block's window table
 template <bool LIST, bool TRANS, bool RES>
 __global__ __launch_bounds__(256) void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc, int* cnt,
                                                     double* diag, double* b, float* val32, float* diag32,
                                                     float* invd32, double* invd64, double* x, double* r,
-                                                    int bFirst) {
+                                                    int bFirst, unsigned short* lcol, int* win) {
+    __shared__ int wS[9], wE[9], wB[10], wM, wOut;
     const int i = slotId();
+    if (lcol) ppeWindowBegin(a, i, wS, wE, wB, wM, wOut);
     double rn2 = 0.0, bn2 = 0.0;
+    bool outside = false;
     if (i < a.N)
         assembleRow<LIST, TRANS, RES>(a, i, val, col, esc, cnt, diag, b, val32, diag32, invd32, invd64, x, r, rn2,
-                                      bn2);
+                                      bn2, lcol, wS, wE, wB, lcol ? wM : 0, outside);
+    if (lcol) {
+        if (__builtin_amdgcn_ballot_w64(outside) != 0 && (threadIdx.x & 63) == 0) atomicOr(&wOut, 1);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int* t = win + (size_t)blockIdx.x * kWinStride;
+            t[0] = wOut;
+            t[1] = wM;
+            for (int k = 0; k < wM; k++) t[2 + k] = wS[k];
+            for (int k = 0; k <= wM; k++) t[11 + k] = wB[k];
+        }
+    }
     // the partials kTrueResidualFinish (|r|^2, |b|^2) or kCgStartFinish (|b|^2, |r|^2) sums
     if (RES) {
         if (bFirst) writePartials(a, bn2, rn2);
@@ -302,75 +432,132 @@ __device__ __forceinline__ void finishPartials(const double* red, int nb, double
 
 // y_i = diag_i x_i + sum_c val(c, i) x[j_c]; the column of entry c is the
 // previous one plus code(c, i), or the next slot of the escape table when the
-// code is 0.  Sixteen entries are read per step so that their value and code
-// loads (FP64 values as pair loads, ellIdx64; codes and FP32 values as loads
-// of four, ellIdx4) and then sixteen gathers are in flight per thread; the value and code streams are read once per launch
-// and are loaded non-temporally so that they do not displace the gathered x
-// from the caches.
+// code is 0 (the first entry of every row is escaped: escape 0 is the row's
+// first column).
+//
+// A row is read in batches of B entries (16 FP32, 8 FP64), software-
+// pipelined so that a lane's streamed loads and its gathers overlap: the
+// codes of batch b + 1 are loaded right behind the gathers of batch b and
+// decoded after batch b's products, and the values of batch b + 1 (FP64 as
+// pair loads, ellIdx64; FP32 values and the codes as loads of four, ellIdx4)
+// are loaded together with its gathers.  The memory counter retires loads
+// in issue order, so every wait is for the oldest loads in flight and none
+// waits for a load issued after the one it needs.  The row's first escape is
+// loaded with the row, so a batch decodes its columns without a dependent
+// load unless some lane of the wave meets a second escaped entry in it.  The
+// last batch is padded: the arrays hold K entries per row (K a multiple of
+// 16) and the codes one batch more, so the loads past a row stay inside
+// them, and the entries past the count get value 0 and the previous column.
+// All lanes of a wave are at the same batch, so a batch's stream addresses
+// are a wave-uniform base plus the lane's 32-bit offset, and a gather is the
+// vector's base plus a 32-bit offset.  Value and code streams are read once
+// per launch and are loaded non-temporally so that they do not displace the
+// gathered x from the caches.
+typedef unsigned short EllU4 __attribute__((ext_vector_type(4)));
+
+// element g * N + i of an array of 16- or 8-byte elements
+template <class E>
+__device__ __forceinline__ E ellStream(const void* base, int g, size_t N, int i) {
+    const char* b = reinterpret_cast<const char*>(base) + (size_t)g * N * sizeof(E);
+    return __builtin_nontemporal_load(reinterpret_cast<const E*>(b + (unsigned)i * (unsigned)sizeof(E)));
+}
 template <class T>
-__device__ __forceinline__ size_t ellValIdx(int c, int i, size_t N) {
-    return sizeof(T) == 8 ? ellIdx64(c, i, N) : ellIdx4(c, i, N);
+__device__ __forceinline__ T ellGather(const T* __restrict__ x, int j) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(x) + (unsigned)j * (unsigned)sizeof(T));
 }
 
-// B consecutive entries c0 .. c0 + B - 1 of row i (c0 and B multiples of 4):
-// all value and code loads, then all gathers, then the products in entry order.
-template <class T, int B, class Next>
-__device__ __forceinline__ void ellBatch(const T* __restrict__ val, const unsigned short* __restrict__ code, size_t N,
-                                         const T* __restrict__ x, int i, int c0, Next&& next, double& acc) {
+// values of entries c0 .. c0 + B - 1 (c0 a multiple of B)
+template <class T, int B>
+__device__ __forceinline__ void ellLoadValues(const T* __restrict__ val, size_t N, int i, int c0, T* v) {
     typedef double D2 __attribute__((ext_vector_type(2)));
     typedef float F4 __attribute__((ext_vector_type(4)));
-    typedef unsigned short U4 __attribute__((ext_vector_type(4)));
-    T v[B];
-    unsigned d[B];
     if (sizeof(T) == 8) {
         // pair m of row i is element m * N + i of the pair view
-        const D2* vp = reinterpret_cast<const D2*>(val) + i + (size_t)(c0 >> 1) * N;
 #pragma unroll
         for (int k = 0; k < B / 2; k++) {
-            const D2 w = __builtin_nontemporal_load(vp + k * N);
+            const D2 w = ellStream<D2>(val, (c0 >> 1) + k, N, i);
             v[2 * k] = (T)w.x;
             v[2 * k + 1] = (T)w.y;
         }
     } else {
         // group g of row i is element g * N + i of the group view
-        const F4* vg = reinterpret_cast<const F4*>(val) + i + (size_t)(c0 >> 2) * N;
 #pragma unroll
         for (int k = 0; k < B / 4; k++) {
-            const F4 w = __builtin_nontemporal_load(vg + k * N);
+            const F4 w = ellStream<F4>(val, (c0 >> 2) + k, N, i);
             v[4 * k] = (T)w.x;
             v[4 * k + 1] = (T)w.y;
             v[4 * k + 2] = (T)w.z;
             v[4 * k + 3] = (T)w.w;
         }
     }
-    const U4* cg = reinterpret_cast<const U4*>(code) + i + (size_t)(c0 >> 2) * N;
+}
+
+template <int B>
+__device__ __forceinline__ void ellLoadCodes(const unsigned short* __restrict__ code, size_t N, int i, int c0,
+                                             EllU4* d) {
+#pragma unroll
+    for (int k = 0; k < B / 4; k++) d[k] = ellStream<EllU4>(code, (c0 >> 2) + k, N, i);
+}
+
+// columns jj of a batch with rem valid entries (past them: the previous
+// column); j = the last column, e = escapes used, es0 = the first escape
+template <int B>
+__device__ __forceinline__ void ellDecode(const EllU4* dv, int rem, const int* __restrict__ es, size_t N, int es0,
+                                          int& j, int& e, int* jj) {
+    unsigned d[B];
 #pragma unroll
     for (int k = 0; k < B / 4; k++) {
-        const U4 w = __builtin_nontemporal_load(cg + k * N);
-        d[4 * k] = w.x;
-        d[4 * k + 1] = w.y;
-        d[4 * k + 2] = w.z;
-        d[4 * k + 3] = w.w;
+        d[4 * k] = dv[k].x;
+        d[4 * k + 1] = dv[k].y;
+        d[4 * k + 2] = dv[k].z;
+        d[4 * k + 3] = dv[k].w;
     }
-    int jj[B];
+    int zc = 0;
 #pragma unroll
-    for (int k = 0; k < B; k++) jj[k] = next(d[k]);
-    T xv[B];
+    for (int k = 0; k < B; k++) zc += (k < rem && d[k] == 0u) ? 1 : 0;
+    int jl = j;
+    if (__builtin_amdgcn_ballot_w64(e + zc > 1) != 0) {
+        // a second escape in some lane's row: read from the table
+        int el = e;
 #pragma unroll
-    for (int k = 0; k < B; k++) xv[k] = x[jj[k]];
-    if (sizeof(T) == 4) {
-        // FP32 (inner solve): each run of up to eight entries is summed in
-        // FP32, then added in FP64
-#pragma unroll
-        for (int h = 0; h < B; h += 8) {
-            T b8 = v[h] * xv[h];
-#pragma unroll
-            for (int k = h + 1; k < h + 8 && k < B; k++) b8 += v[k] * xv[k];
-            acc += (double)b8;
+        for (int k = 0; k < B; k++) {
+            if (k < rem) {
+                int t = jl + (int)d[k];
+                if (d[k] == 0u) {
+                    t = (el == 0) ? es0 : es[(size_t)el * N];
+                    el++;
+                }
+                jl = t;
+            }
+            jj[k] = jl;
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < B; k++) acc += (double)(v[k] * xv[k]);
+        for (int k = 0; k < B; k++) {
+            if (k < rem) jl = d[k] ? jl + (int)d[k] : es0;
+            jj[k] = jl;
+        }
+    }
+    j = jl;
+    e += zc;
+}
+
+// acc += the batch's products in entry order (entries k >= rem add nothing);
+// FP32 (inner solve): each run of eight is summed in FP32, then added in FP64
+template <class T, int B>
+__device__ __forceinline__ void ellAccum(const T* v, const T* xv, int rem, double& acc) {
+    if (sizeof(T) == 4) {
+#pragma unroll
+        for (int h = 0; h < B; h += 8) {
+            T b8 = (h < rem) ? v[h] * xv[h] : (T)0;
+#pragma unroll
+            for (int k = h + 1; k < h + 8; k++) b8 += (k < rem) ? v[k] * xv[k] : (T)0;
+            if (h < rem) acc += (double)b8;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < B; k++)
+            if (k < rem) acc += (double)(v[k] * xv[k]);
     }
 }
 
@@ -378,26 +565,27 @@ template <class T>
 __device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsigned short* __restrict__ code,
                                          const int* __restrict__ esc, int n, size_t N, const T* __restrict__ x, int i,
                                          double acc) {
+    constexpr int B = 8;
+    if (n <= 0) return acc;
     const int* es = esc + i;
+    const int es0 = es[0];
     int j = 0, e = 0;
-    auto next = [&](unsigned d) {
-        if (d) j += (int)d;
-        else j = es[(size_t)(e++) * N];
-        return j;
-    };
-    int c = 0;
-    for (; c + 16 <= n; c += 16) ellBatch<T, 16>(val, code, N, x, i, c, next, acc);
-    if (c + 8 <= n) {
-        ellBatch<T, 8>(val, code, N, x, i, c, next, acc);
-        c += 8;
-    }
-    if (c + 4 <= n) {
-        ellBatch<T, 4>(val, code, N, x, i, c, next, acc);
-        c += 4;
-    }
-    for (; c < n; c++) {
-        const int jc = next(code[ellIdx4(c, i, N)]);
-        acc += (double)(val[ellValIdx<T>(c, i, N)] * x[jc]);
+    T v[B];
+    EllU4 d[B / 4];
+    int jj[B];
+    ellLoadValues<T, B>(val, N, i, 0, v);
+    ellLoadCodes<B>(code, N, i, 0, d);
+    ellDecode<B>(d, n, es, N, es0, j, e, jj);
+    for (int c = 0;;) {
+        T xv[B];
+#pragma unroll
+        for (int k = 0; k < B; k++) xv[k] = ellGather(x, jj[k]);
+        ellLoadCodes<B>(code, N, i, c + B, d);
+        ellAccum<T, B>(v, xv, n - c, acc);
+        ellDecode<B>(d, n - c - B, es, N, es0, j, e, jj);
+        c += B;
+        if (c >= n) break;
+        ellLoadValues<T, B>(val, N, i, c, v);
     }
     return acc;
 }
@@ -424,7 +612,30 @@ struct CgPtrs {
     float* fz;
     float* fq;
     float* fd;
+    // window indices and the per-block window table (null: no windows)
+    const unsigned short* lcol;
+    const int* win;
 };
+
+// FP32 row from the block's LDS window: entries in batches of eight, each
+// summed in FP32 and added in FP64, the order of ellRow<float>, so both give
+// the same bits.
+__device__ __forceinline__ double ellRowWin(const float* __restrict__ val, const unsigned short* __restrict__ lcol,
+                                            int n, size_t N, const float* win, int i, double acc) {
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    for (int c = 0; c < n; c += 8) {
+        const F4 v0 = ellStream<F4>(val, c >> 2, N, i), v1 = ellStream<F4>(val, (c >> 2) + 1, N, i);
+        const EllU4 k0 = ellStream<EllU4>(lcol, c >> 2, N, i), k1 = ellStream<EllU4>(lcol, (c >> 2) + 1, N, i);
+        const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+        const unsigned k[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+        const int rem = n - c;
+        float b8 = v[0] * win[k[0]];
+#pragma unroll
+        for (int q = 1; q < 8; q++) b8 += (q < rem) ? v[q] * win[(q < rem) ? k[q] : 0u] : 0.0f;
+        acc += (double)b8;
+    }
+    return acc;
+}
 
 __device__ __forceinline__ double spmv64(const KArgs& a, const CgPtrs& c, const double* x, int i) {
     return ellRow<double>(c.val, c.col, c.esc, c.cnt[i], a.N, x, i, c.diag[i] * x[i]);
@@ -613,13 +824,28 @@ __global__ void kMxInitFinish(KArgs a, int nb, double tolIn, double tol) {
     a.is[I_CG_ITER] = 0;
     a.is[I_CG_DONE] = (rn2 == 0.0) ? 2 : (isfinite(rn2) && isfinite(rz)) ? 0 : 5;
 }
+// q = M p: a block with a window first copies its segments of p into LDS
 __global__ __launch_bounds__(256) void kMxSpmv(KArgs a, CgPtrs c) {
     if (a.is[I_CG_DONE]) return;
+    __shared__ float win[kWinMax];
     const int i = slotId();
+    const int* t = c.win ? c.win + (size_t)blockIdx.x * kWinStride : nullptr;
+    const bool lds = t != nullptr && t[0] == 0;
     double s0 = 0;
+    if (lds) {
+        const int m = t[1];
+        for (int k = 0; k < m; k++) {
+            const int S = t[2 + k], B0 = t[11 + k], len = t[12 + k] - B0;
+            for (int q = threadIdx.x; q < len; q += kB) win[B0 + q] = c.fp[S + q];
+        }
+        __syncthreads();
+    }
     if (i < a.N) {
         const float p = c.fp[i];
-        const float q = (float)ellRow<float>(c.val32, c.col, c.esc, c.cnt[i], a.N, c.fp, i, (double)(c.diag32[i] * p));
+        const double d = (double)(c.diag32[i] * p);
+        const int n = c.cnt[i];
+        const float q = lds ? (float)ellRowWin(c.val32, c.lcol, n, a.N, win, i, d)
+                            : (float)ellRow<float>(c.val32, c.col, c.esc, n, a.N, c.fp, i, d);
         c.fq[i] = q;
         s0 = (double)p * (double)q;
     }
@@ -745,6 +971,8 @@ static CgPtrs cgPtrs(Ctx* ctx) {
     c.x = ctx->D[DPPEX]; c.r = ctx->cgR; c.p = ctx->cgP; c.z = ctx->cgZ; c.q = ctx->cgQ; c.invd = ctx->cgInvD;
     c.val32 = ctx->aVal32; c.diag32 = ctx->aDiag32; c.invd32 = ctx->aDiagInv32;
     c.fr = ctx->fR; c.fp = ctx->fP; c.fz = ctx->fZ; c.fq = ctx->fQ; c.fd = ctx->fX;
+    c.lcol = ctx->aLcol;
+    c.win = ctx->winValid ? ctx->aWin : nullptr;
     return c;
 }
 
@@ -759,6 +987,8 @@ static int ensureMixed(Ctx* ctx) {
     MPS_HIP(hipMalloc(&ctx->fZ, sizeof(float) * n));
     MPS_HIP(hipMalloc(&ctx->fQ, sizeof(float) * n));
     MPS_HIP(hipMalloc(&ctx->fX, sizeof(float) * n));
+    MPS_HIP(hipMalloc(&ctx->aLcol, sizeof(unsigned short) * (size_t)ctx->K * n));
+    MPS_HIP(hipMalloc(&ctx->aWin, sizeof(int) * (size_t)kWinStride * gridFor(ctx->N, kB)));
     return MPS_OK;
 }
 
@@ -794,9 +1024,14 @@ int ppeAssemble(Ctx* ctx) {
     double* x = ctx->D[DPPEX];
     double* invd64 = (res && !mixed) ? ctx->cgInvD : nullptr;
     const int bFirst = mixed ? 0 : 1;
+    // the inner solve's LDS windows (mixed precision; ppe_window scalar)
+    const bool wins = mixed && ctx->ppeWindow;
+    unsigned short* lcol = wins ? ctx->aLcol : nullptr;
+    int* win = wins ? ctx->aWin : nullptr;
 #define ASM(L, TT, R)                                                                                          \
     hipLaunchKernelGGL((kPpeAssemble<L, TT, R>), dim3(g), dim3(kB), 0, ctx->stream, a, ctx->aVal, ctx->aCol,   \
-                       ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->D[DPPEB], v32, d32, i32, invd64, x, ctx->cgR, bFirst)
+                       ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->D[DPPEB], v32, d32, i32, invd64, x, ctx->cgR, bFirst, \
+                       lcol, win)
     if (res) {
         if (ctx->listValid) {
             if (T) ASM(true, true, true); else ASM(true, false, true);
@@ -812,7 +1047,23 @@ int ppeAssemble(Ctx* ctx) {
     MPS_HIP(hipGetLastError());
     ctx->r0Fresh = res;
     ctx->r0FreshMixed = mixed;
+    ctx->winValid = wins;
     ctx->asmMode = asmModeOf(ctx->P);
+    return MPS_OK;
+}
+
+// Blocks of the last assembly whose rows the mixed SpMV serves from an LDS
+// window (0 without windows).
+int ppeWindowBlocks(Ctx* ctx, double* out) {
+    *out = 0.0;
+    if (!ctx->winValid) return MPS_OK;
+    const int nb = gridFor(ctx->N, kB);
+    std::vector<int> t((size_t)kWinStride * nb);
+    MPS_HIP(hipStreamSynchronize(ctx->stream));
+    MPS_HIP(hipMemcpy(t.data(), ctx->aWin, sizeof(int) * t.size(), hipMemcpyDeviceToHost));
+    int c = 0;
+    for (int b = 0; b < nb; b++) c += (t[(size_t)b * kWinStride] == 0) ? 1 : 0;
+    *out = c;
     return MPS_OK;
 }
 
