@@ -119,8 +119,8 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
  * "learned_tol_scale" (in (0, 1], default 0.1: with the learned
  * preconditioner the true residual is driven to this multiple of the
  * tolerance, since the flexible CG's error at a given residual is larger)
- * "ppe_window" (1 default: the mixed inner solve's SpMV reads each 256-row
- * block's vector window from LDS; 0: from global memory; same results,
+ * "ppe_window" (1 default: the CG solves' SpMVs read each 256-row block's
+ * vector window from LDS; 0: from global memory; same results,
  * used from the next assembly; read-only "ppe_window_blocks": blocks of the
  * last assembly served from a window) "ppe_warm_start" (initial guess of each
  * pressure solve: 0 previous solution, 1 / 2 / 3 linear / quadratic / cubic

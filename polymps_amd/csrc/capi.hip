@@ -855,9 +855,9 @@ int mps_set_scalar(mps_ctx* ctx, const char* name, double value) {
         }
         ctx->learnedTolScale = value;
     } else if (s == "ppe_window") {
-        // the mixed inner solve's SpMV gathers from per-block LDS windows
-        // (1, default) or from global memory (0); same results, from the
-        // next assembly on
+        // the CG solves' SpMVs gather from per-block LDS windows (1,
+        // default) or from global memory (0); same results, from the next
+        // assembly on
         if (!(value == 0.0 || value == 1.0)) {
             ctx->err = "ppe_window must be 0 or 1";
             return MPS_E_ARG;

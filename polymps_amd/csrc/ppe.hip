@@ -25,7 +25,7 @@ namespace mpsgpu {
 
 constexpr int kB = 256;
 
-// LDS window of a 256-row block for the FP32 SpMV of the inner solve.  The
+// LDS window of a 256-row block for the SpMVs of the CG solves.  The
 // columns of a block's rows lie in the union, over its rows, of their
 // stencil rows' slot ranges (three in 2-D, nine in 3-D); merged into
 // disjoint ascending segments these hold a few thousand slots, which the
@@ -38,7 +38,7 @@ constexpr int kB = 256;
 //       outside it): the block's rows gather from global memory;
 //   [1] segment count m, [2 + k] first slot of segment k,
 //   [11 + k] its LDS offset (k = 0 .. m; offset m = the window's size).
-constexpr int kWinMax = 5000;    // slots: 20 KB of FP32 values per block
+constexpr int kWinMax = 5000;    // slots: 20 KB of FP32 / 40 KB of FP64 values per block
 constexpr int kWinStride = 24;   // ints per block in the table
 
 // The block's window from its rows' current stencil rows (rows with
@@ -617,24 +617,58 @@ struct CgPtrs {
     const int* win;
 };
 
-// FP32 row from the block's LDS window: entries in batches of eight, each
-// summed in FP32 and added in FP64, the order of ellRow<float>, so both give
-// the same bits.
-__device__ __forceinline__ double ellRowWin(const float* __restrict__ val, const unsigned short* __restrict__ lcol,
-                                            int n, size_t N, const float* win, int i, double acc) {
+// Row from the block's LDS window, in the order of ellRow (FP32: batches of
+// eight summed in FP32, then added in FP64; FP64: entry by entry), so both
+// give the same bits.
+template <class T>
+__device__ __forceinline__ double ellRowWin(const T* __restrict__ val, const unsigned short* __restrict__ lcol,
+                                            int n, size_t N, const T* win, int i, double acc) {
     typedef float F4 __attribute__((ext_vector_type(4)));
+    typedef double D2 __attribute__((ext_vector_type(2)));
     for (int c = 0; c < n; c += 8) {
-        const F4 v0 = ellStream<F4>(val, c >> 2, N, i), v1 = ellStream<F4>(val, (c >> 2) + 1, N, i);
+        T v[8];
+        if (sizeof(T) == 4) {
+            const F4 v0 = ellStream<F4>(val, c >> 2, N, i), v1 = ellStream<F4>(val, (c >> 2) + 1, N, i);
+            v[0] = (T)v0.x; v[1] = (T)v0.y; v[2] = (T)v0.z; v[3] = (T)v0.w;
+            v[4] = (T)v1.x; v[5] = (T)v1.y; v[6] = (T)v1.z; v[7] = (T)v1.w;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const D2 w = ellStream<D2>(val, (c >> 1) + q, N, i);
+                v[2 * q] = (T)w.x;
+                v[2 * q + 1] = (T)w.y;
+            }
+        }
         const EllU4 k0 = ellStream<EllU4>(lcol, c >> 2, N, i), k1 = ellStream<EllU4>(lcol, (c >> 2) + 1, N, i);
-        const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
         const unsigned k[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
         const int rem = n - c;
-        float b8 = v[0] * win[k[0]];
+        if (sizeof(T) == 4) {
+            T b8 = v[0] * win[k[0]];
 #pragma unroll
-        for (int q = 1; q < 8; q++) b8 += (q < rem) ? v[q] * win[(q < rem) ? k[q] : 0u] : 0.0f;
-        acc += (double)b8;
+            for (int q = 1; q < 8; q++) b8 += (q < rem) ? v[q] * win[(q < rem) ? k[q] : 0u] : (T)0;
+            acc += (double)b8;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (q < rem) acc += (double)(v[q] * win[k[q]]);
+        }
     }
     return acc;
+}
+
+// Copies the block's window of vector x into LDS when the block has one
+// (block-wide; false: the block's rows gather from global memory).
+template <class T>
+__device__ __forceinline__ bool stageWindow(const int* table, const T* __restrict__ x, T* win) {
+    const int* t = table ? table + (size_t)blockIdx.x * kWinStride : nullptr;
+    if (t == nullptr || t[0] != 0) return false;
+    const int m = t[1];
+    for (int k = 0; k < m; k++) {
+        const int S = t[2 + k], B0 = t[11 + k], len = t[12 + k] - B0;
+        for (int q = threadIdx.x; q < len; q += kB) win[B0 + q] = x[S + q];
+    }
+    __syncthreads();
+    return true;
 }
 
 __device__ __forceinline__ double spmv64(const KArgs& a, const CgPtrs& c, const double* x, int i) {
@@ -707,10 +741,13 @@ __global__ void kCgInitPFinish(KArgs a, int nb) {
 // q = M p; partial p.q
 __global__ __launch_bounds__(256) void kCgSpmv(KArgs a, CgPtrs c) {
     if (a.is[I_CG_DONE]) return;
+    __shared__ double win[kWinMax];
+    const bool lds = stageWindow<double>(c.win, c.p, win);
     const int i = slotId();
     double s0 = 0;
     if (i < a.N) {
-        const double q = spmv64(a, c, c.p, i);
+        const double q = lds ? ellRowWin<double>(c.val, c.lcol, c.cnt[i], a.N, win, i, c.diag[i] * c.p[i])
+                             : spmv64(a, c, c.p, i);
         c.q[i] = q;
         s0 = c.p[i] * q;
     }
@@ -773,11 +810,15 @@ __global__ void kCgP(KArgs a, CgPtrs c) {
 
 // true residual r = b - M x in FP64 (kept in c.r); partials |r|^2, |b|^2
 __global__ __launch_bounds__(256) void kTrueResidual(KArgs a, CgPtrs c) {
+    __shared__ double win[kWinMax];
+    // (slots past the live count, zeroed below, are in no window)
+    const bool lds = stageWindow<double>(c.win, c.x, win);
     const int i = slotId();
     double s0 = 0, s1 = 0;
     if (i < a.N) {
         if (!live(a, i)) c.x[i] = 0.0;
-        const double r = c.b[i] - spmv64(a, c, c.x, i);
+        const double r = c.b[i] - (lds ? ellRowWin<double>(c.val, c.lcol, c.cnt[i], a.N, win, i, c.diag[i] * c.x[i])
+                                       : spmv64(a, c, c.x, i));
         c.r[i] = r;
         s0 = r * r;
         s1 = c.b[i] * c.b[i];
@@ -829,22 +870,13 @@ __global__ __launch_bounds__(256) void kMxSpmv(KArgs a, CgPtrs c) {
     if (a.is[I_CG_DONE]) return;
     __shared__ float win[kWinMax];
     const int i = slotId();
-    const int* t = c.win ? c.win + (size_t)blockIdx.x * kWinStride : nullptr;
-    const bool lds = t != nullptr && t[0] == 0;
+    const bool lds = stageWindow<float>(c.win, c.fp, win);
     double s0 = 0;
-    if (lds) {
-        const int m = t[1];
-        for (int k = 0; k < m; k++) {
-            const int S = t[2 + k], B0 = t[11 + k], len = t[12 + k] - B0;
-            for (int q = threadIdx.x; q < len; q += kB) win[B0 + q] = c.fp[S + q];
-        }
-        __syncthreads();
-    }
     if (i < a.N) {
         const float p = c.fp[i];
         const double d = (double)(c.diag32[i] * p);
         const int n = c.cnt[i];
-        const float q = lds ? (float)ellRowWin(c.val32, c.lcol, n, a.N, win, i, d)
+        const float q = lds ? (float)ellRowWin<float>(c.val32, c.lcol, n, a.N, win, i, d)
                             : (float)ellRow<float>(c.val32, c.col, c.esc, n, a.N, c.fp, i, d);
         c.fq[i] = q;
         s0 = (double)p * (double)q;
@@ -987,7 +1019,12 @@ static int ensureMixed(Ctx* ctx) {
     MPS_HIP(hipMalloc(&ctx->fZ, sizeof(float) * n));
     MPS_HIP(hipMalloc(&ctx->fQ, sizeof(float) * n));
     MPS_HIP(hipMalloc(&ctx->fX, sizeof(float) * n));
-    MPS_HIP(hipMalloc(&ctx->aLcol, sizeof(unsigned short) * (size_t)ctx->K * n));
+    return MPS_OK;
+}
+
+static int ensureWindows(Ctx* ctx) {
+    if (ctx->aLcol) return MPS_OK;
+    MPS_HIP(hipMalloc(&ctx->aLcol, sizeof(unsigned short) * (size_t)ctx->K * ctx->N));
     MPS_HIP(hipMalloc(&ctx->aWin, sizeof(int) * (size_t)kWinStride * gridFor(ctx->N, kB)));
     return MPS_OK;
 }
@@ -1024,8 +1061,12 @@ int ppeAssemble(Ctx* ctx) {
     double* x = ctx->D[DPPEX];
     double* invd64 = (res && !mixed) ? ctx->cgInvD : nullptr;
     const int bFirst = mixed ? 0 : 1;
-    // the inner solve's LDS windows (mixed precision; ppe_window scalar)
-    const bool wins = mixed && ctx->ppeWindow;
+    // the CG solves' LDS windows (ppe_window scalar)
+    const bool wins = !bicg && ctx->ppeWindow;
+    if (wins) {
+        int rc = ensureWindows(ctx);
+        if (rc) return rc;
+    }
     unsigned short* lcol = wins ? ctx->aLcol : nullptr;
     int* win = wins ? ctx->aWin : nullptr;
 #define ASM(L, TT, R)                                                                                          \

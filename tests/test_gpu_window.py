@@ -1,12 +1,13 @@
-"""LDS windows of the mixed inner solve's SpMV (needs a gfx950 GPU).
+"""LDS windows of the CG solves' SpMVs (needs a gfx950 GPU).
 
 With the windows on (scalar ppe_window 1, the default) every 256-row block
 whose columns fit its window (the union of its rows' stencil-row slot
-ranges) copies that part of the search vector into LDS and gathers from it;
-the other blocks gather from global memory, as every block does with the
-windows off.  Both sum the same products in the same order, so whole runs
-must agree bit for bit, and at the benchmark's density almost every block
-must take its window.
+ranges) copies that part of the vector into LDS and gathers from it (the
+mixed solve's FP32 SpMV, its FP64 true residual and the FP64 CG's SpMV); the
+other blocks gather from global memory, as every block does with the windows
+off.  Both sum the same products in the same order, so whole runs must agree
+bit for bit, and at the benchmark's density almost every block must take
+its window.
 """
 import numpy as np
 import pytest
@@ -34,12 +35,12 @@ def _same(on, off, tag):
             raise AssertionError(f"{tag}: {f} differs at {len(d)} entries, first {d[:5]}")
 
 
-@pytest.mark.parametrize("l0,steps", [(0.0125, 40), (0.005, 12)])
-def test_window_runs_are_bit_identical(built, l0, steps):
+@pytest.mark.parametrize("l0,steps,precision", [(0.0125, 40, 1), (0.005, 12, 1), (0.0125, 20, 0)])
+def test_window_runs_are_bit_identical(built, l0, steps, precision):
     w = configs.dam3d_inc(l0=l0) if l0 == 0.005 else configs.small_dam3d(l0=l0)
-    w.params.precision_mode = 1
+    w.params.precision_mode = precision
     w.params.ppe_warm_start = 2
-    print(f"window on/off: {w.n} particles, {steps} steps", flush=True)
+    print(f"window on/off: {w.n} particles, precision {precision}, {steps} steps", flush=True)
     on, off = _pair(w)
     on.call("init")
     off.call("init")
