@@ -79,6 +79,13 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
     const int nx = P.num_buckets_x, ny = P.num_buckets_y, nz = P.num_buckets_z;
     const int rows = ((int)P.dim == 3) ? 9 : 3;
     const double inf = __builtin_inf();
+    // the bounding box of the wave's particles: a candidate farther from it
+    // than the radius (with a margin far above the rounding of either
+    // distance) is in no lane's radius and is not tested
+    const double bxl = waveMin(act ? xi : inf), bxh = waveMax(act ? xi : -inf);
+    const double byl = waveMin(act ? yi : inf), byh = waveMax(act ? yi : -inf);
+    const double bzl = waveMin(act ? zi : inf), bzh = waveMax(act ? zi : -inf);
+    const double rCull2 = r2 * (1.0 + 1e-9);
     int total = 0, totalS = 0, over = 0;
     for (int r = 0; r < rows; r++) {
         // the lane's candidate slots [s, e) of row r (the walk's row order)
@@ -137,6 +144,12 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
             for (int cb = us; cb < ue; cb += kWave) {
                 __syncthreads();
                 stage[t] = make_double4(nx0, ny0, nz0, 0.0);
+                // candidates within the radius of the wave's bounding box
+                // (the only ones a lane can reach; staged infinities never are)
+                const double ex = fmax(fmax(bxl - nx0, nx0 - bxh), 0.0);
+                const double ey = fmax(fmax(byl - ny0, ny0 - byh), 0.0);
+                const double ez = fmax(fmax(bzl - nz0, nz0 - bzh), 0.0);
+                const unsigned long long rel = __builtin_amdgcn_ballot_w64(ex * ex + ey * ey + ez * ez < rCull2);
                 __syncthreads();
                 const int jn = cb + kWave + t;
                 if (jn < ue) {
@@ -150,22 +163,41 @@ __global__ __launch_bounds__(kWave) void kBuildList(KArgs a, unsigned* mask, int
                 }
                 const int m = min(kWave, ue - cb);
                 for (int h = 0; h < m; h += 32) {
-                    const int mm = min(32, m - h);
+                    // the word's reachable candidates in ascending order; each
+                    // one's bit enters at its place (u shifted past the others)
+                    unsigned rb = (unsigned)(rel >> h);
                     unsigned U = 0, US = 0;
-                    int k = 0;
-                    for (; k < mm; k += 8) {
+                    int pos = 0;
+                    while (__builtin_popcount(rb) >= 4) {
+                        int kk[4];
 #pragma unroll
-                        for (int u = 0; u < 8; u++) {
-                            const double4 q = stage[h + k + u];
+                        for (int u = 0; u < 4; u++) {
+                            kk[u] = __builtin_ctz(rb);
+                            rb &= rb - 1;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const double4 q = stage[h + kk[u]];
                             const double v0 = q.x - xi, v1 = q.y - yi, v2 = q.z - zi;
                             const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-                            U = shiftIn(U, d2 < r2);
-                            if (SMALL) US = shiftIn(US, d2 < r2s);
+                            U = shiftIn(U << (kk[u] - pos), d2 < r2);
+                            if (SMALL) US = shiftIn(US << (kk[u] - pos), d2 < r2s);
+                            pos = kk[u] + 1;
                         }
                     }
+                    while (rb) {
+                        const int k1 = __builtin_ctz(rb);
+                        rb &= rb - 1;
+                        const double4 q = stage[h + k1];
+                        const double v0 = q.x - xi, v1 = q.y - yi, v2 = q.z - zi;
+                        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+                        U = shiftIn(U << (k1 - pos), d2 < r2);
+                        if (SMALL) US = shiftIn(US << (k1 - pos), d2 < r2s);
+                        pos = k1 + 1;
+                    }
                     // candidate h + q of the chunk at bit q
-                    U = __builtin_bitreverse32(U << (32 - k));
-                    if (SMALL) US = __builtin_bitreverse32(US << (32 - k));
+                    U = pos ? __builtin_bitreverse32(U << (32 - pos)) : 0u;
+                    if (SMALL) US = pos ? __builtin_bitreverse32(US << (32 - pos)) : 0u;
                     if (c > q0 && 32 * w < len) put(laneWord(U, prev, sh), laneWord(US, prevS, sh));
                     prev = U;
                     prevS = US;
