@@ -16,6 +16,8 @@ from.  The sizes are those of the workload the kernels ran on:
 Per-particle layouts (bytes): P4 packed (x, y, z, type) 32, FP64 vector 24,
 mirror point 24, correction matrix 72, FP64 scalar 8, int 4, FP32 scalar 4.
 
+Per-kernel totals of a kernel trace alone: `--trace T.db --summary`.
+
 Usage (prints a Markdown table):
 
     python -m polymps_amd.roofline --trace T.db --fetch F.db --write W.db \
@@ -206,6 +208,26 @@ def table(args):
     return "\n".join(out)
 
 
+def summary(db, top=30):
+    """Per-kernel totals of a kernel trace: calls, total and mean time, share,
+    VGPRs and scratch bytes per work-item (as the code object declares)."""
+    c = sqlite3.connect(db)
+    per = defaultdict(lambda: [0, 0.0, 0, 0])
+    for name, d, vg, sc in c.execute("select name, duration, vgpr_count, scratch_size from kernels"):
+        k = per[_key(name)]
+        k[0] += 1
+        k[1] += d
+        k[2] = max(k[2], vg or 0)
+        k[3] = max(k[3], sc or 0)
+    total = sum(v[1] for v in per.values())
+    out = ["| kernel | calls | total ms | mean us | share | VGPRs | scratch B |", "|---|---:|---:|---:|---:|---:|---:|"]
+    for k, (n, t, vg, sc) in sorted(per.items(), key=lambda kv: -kv[1][1])[:top]:
+        out.append(f"| {k} | {n} | {t / 1e6:.3f} | {t / n / 1e3:.1f} | {100 * t / total:.1f}% | {vg} | {sc} |")
+    out.append("")
+    out.append(f"Total GPU kernel time: {total / 1e6:.1f} ms")
+    return "\n".join(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--trace", required=True)
@@ -215,9 +237,16 @@ def main(argv=None):
     ap.add_argument("--copy-fetch")
     ap.add_argument("--copy-write")
     ap.add_argument("--copy-bytes", type=float, default=float(1 << 30))
-    ap.add_argument("--sizes", required=True, help="file with bench.py's JSON line")
+    ap.add_argument("--sizes", help="file with bench.py's JSON line")
     ap.add_argument("--top", type=int, default=25)
-    print(table(ap.parse_args(argv)))
+    ap.add_argument("--summary", action="store_true", help="only the per-kernel totals of --trace")
+    args = ap.parse_args(argv)
+    if args.summary:
+        print(summary(args.trace, args.top))
+        return
+    if not args.sizes:
+        ap.error("--sizes is required for the roofline table")
+    print(table(args))
 
 
 if __name__ == "__main__":
