@@ -755,7 +755,10 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
     else if (s == "outer_iterations") *out = ctx->outerIter;
     else if (s == "num_particles") *out = ctx->nLiveHost;
     else if (s == "iterations") *out = ctx->numIter;
-    else if (s == "ppe_ms") *out = ctx->ppeMs;
+    else if (s == "ppe_ms") {
+        const int rc2 = ppeSolveMs(ctx, out);
+        if (rc2) return rc2;
+    }
     else if (s == "true_residual") *out = ctx->trueResidual;
     else if (s == "fallback") *out = ctx->fallbackUsed;
     else if (s == "learned_stall_ratio") *out = ctx->learnedStallRatio;

@@ -49,7 +49,9 @@ enum IField { ITYPE, ICANON, IBC, INNEIGH, NPI, INNWALL = NPI, INEARWALL, INEARM
 
 // device scalar slots (double)
 enum DScalar { S_VELMAX, S_ABSNEW, S_ABSOLD, S_RHSN2, S_THR, S_RN2, S_ALPHA, S_BETA, S_PAP, S_BN2, S_TRUE_RN2,
-               S_RELTOL, S_RHO, S_RHOOLD, S_BW, S_R0N2, NDS };
+               S_RELTOL, S_RHO, S_RHOOLD, S_BW, S_R0N2,
+               S_NNZ,   // PPE entries of the last assembly, counted on the device when profiling
+               NDS };
 // device scalar slots (int)
 enum IScalar { I_NLIVE, I_OUTCNT, I_OVERFLOW, I_CG_ITER, I_CG_DONE, I_COMPACT_OVF, I_NESC,
                I_BCG_RESTART, I_BCG_RESTARTS, I_REBUILD, I_PATCHCNT, I_FIXCNT, NIS };
@@ -241,7 +243,6 @@ struct Ctx {
     bool r0FreshMixed = false;   // ... in the mixed solve's partial order (else the FP64 CG's, with 1/diag)
     double* asmRed = nullptr;    // block partials of the assembly's first residual (no other kernel writes them)
     unsigned long long* red64 = nullptr;   // result of reduceCounts (allocated on first use)
-    double nnzHost = 0.0;        // PPE entries of the last assembly (when profiling or asked for)
     // solver kind of the last PPE assembly (MPS_SOLVER_* / precision), -1 when
     // there is none for the current slot order; a solve must match it
     int asmMode = -1;
@@ -253,6 +254,7 @@ struct Ctx {
     double solverErr = 0.0;
     double trueResidual = 0.0;
     double ppeMs = 0.0;
+    bool ppeMsPending = false;   // ppeMs still to be read from the solve's events (ev0, ev1)
     int fallbackUsed = 0;
     double learnedStallRatio = 0.5;   // learned preconditioner: largest residual ratio of a refinement pass
     double learnedTolScale = 0.1;     // learned preconditioner: true-residual target as a multiple of the tolerance
@@ -295,6 +297,8 @@ int launchCompaction(Ctx* ctx);
 // sum (max = false) or maximum of the n counts cnt[] (synchronises the stream)
 int ppeWindowBlocks(Ctx* ctx, double* out);
 int reduceCounts(Ctx* ctx, const int* cnt, int n, bool max, double* out);
+int launchCountToScalar(Ctx* ctx, const int* cnt, int n, int slot);
+int ppeSolveMs(Ctx* ctx, double* out);
 int launchClosestPoint(Ctx* ctx, int type);   // one mesh type (0 rigid, 1 deformable, 2 forced)
 // granular phase (granular.hip)
 void granularGridSize(const mps_params& P, int* kx, int* ky);

@@ -2265,6 +2265,21 @@ __global__ __launch_bounds__(256) void kReduceCounts(const int* __restrict__ cnt
     }
 }
 
+__global__ void kCountToScalar(const unsigned long long* v, double* out) { *out = (double)*v; }
+
+// The sum of cnt[0 .. n) into the device scalar ds[slot], without waiting:
+// the host reads it with the next status copy.
+int launchCountToScalar(Ctx* ctx, const int* cnt, int n, int slot) {
+    if (!ctx->red64) MPS_HIP(hipMalloc(&ctx->red64, sizeof(unsigned long long)));
+    MPS_HIP(hipMemsetAsync(ctx->red64, 0, sizeof(unsigned long long), ctx->stream));
+    if (cnt != nullptr && n > 0)
+        hipLaunchKernelGGL(kReduceCounts, dim3(std::min(gridFor(n, 256), 1024)), dim3(256), 0, ctx->stream, cnt, n,
+                           ctx->red64, 0);
+    hipLaunchKernelGGL(kCountToScalar, dim3(1), dim3(1), 0, ctx->stream, ctx->red64, ctx->ds + slot);
+    MPS_HIP(hipGetLastError());
+    return MPS_OK;
+}
+
 int reduceCounts(Ctx* ctx, const int* cnt, int n, bool max, double* out) {
     if (!ctx->red64) MPS_HIP(hipMalloc(&ctx->red64, sizeof(unsigned long long)));
     MPS_HIP(hipMemsetAsync(ctx->red64, 0, sizeof(unsigned long long), ctx->stream));

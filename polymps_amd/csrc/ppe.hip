@@ -1386,6 +1386,12 @@ static int solveFp64(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int batc
     hipLaunchKernelGGL(kCgZeroX, dim3(nb), dim3(kB), 0, s, a, c.x);
     hipLaunchKernelGGL(kCgInitP, dim3(nb), dim3(kB), 0, s, a, c);
     hipLaunchKernelGGL(kCgInitPFinish, dim3(1), dim3(1024), 0, s, a, nb);
+    // when profiling, the entry count (for the SpMV bytes) is summed on the
+    // device and comes with the loop's status reads
+    if (ctx->profile) {
+        const int rcN = launchCountToScalar(ctx, ctx->aCnt, ctx->N, S_NNZ);
+        if (rcN) return rcN;
+    }
     std::vector<float> times;
     int done = 0, iters = 0;
     for (int launched = 0, b = firstBatch(ctx->lastCgIters, batch); !done; launched += b, b = batch) {
@@ -1405,12 +1411,9 @@ static int solveFp64(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int batc
     ctx->lastCgIters = iters;
     const int D = ctx->hostInts[I_CG_DONE];
     ctx->lastDone = D;
-    if (ctx->profile) {
-        const int rcN = reduceCounts(ctx, ctx->aCnt, ctx->N, false, &ctx->nnzHost);
-        if (rcN) return rcN;
+    if (ctx->profile)   // the entry count came with the loop's status reads
         addSpmvStats(ctx, times, (D == 1 || D == 3) ? iters + 1 : 0,
-                     10.0 * ctx->nnzHost + 4.0 * ctx->hostInts[I_NESC] + 28.0 * ctx->nLiveHost);
-    }
+                     10.0 * ctx->hostScalars[S_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 28.0 * ctx->nLiveHost);
     ctx->solverIter = iters;
     ctx->outerIter = 1;
     ctx->solverErr = ctx->hostScalars[S_RHSN2] > 0 ? sqrt(ctx->hostScalars[S_RN2] / ctx->hostScalars[S_RHSN2]) : 0.0;
@@ -1435,11 +1438,12 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
     int total = 0, outer = 0, rc;
     double prevRn2 = 0.0;
     ctx->lastDone = 0;
-    if (ctx->profile) {
-        rc = reduceCounts(ctx, ctx->aCnt, ctx->N, false, &ctx->nnzHost);
-        if (rc) return rc;
-    }
-    const double bytes32 = 6.0 * ctx->nnzHost + 4.0 * ctx->hostInts[I_NESC] + 16.0 * ctx->nLiveHost;
+    // when profiling, the entry count (for the SpMV bytes) is summed on the
+    // device and comes with the next status read
+    if (ctx->profile && (rc = launchCountToScalar(ctx, ctx->aCnt, ctx->N, S_NNZ))) return rc;
+    auto bytes32 = [ctx]() {
+        return 6.0 * ctx->hostScalars[S_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 16.0 * ctx->nLiveHost;
+    };
     for (;;) {
         // the first residual may come from the assembly (r and its partials)
         const bool fresh = outer == 0 && ctx->r0Fresh && ctx->r0FreshMixed;
@@ -1499,7 +1503,7 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
         }
         if (outer == 1) ctx->lastInnerIters = it;
         const int D = ctx->hostInts[I_CG_DONE];
-        addSpmvStats(ctx, times, (D == 1 || D == 3 || D == 4) ? it + 1 : 0, bytes32);
+        addSpmvStats(ctx, times, (D == 1 || D == 3 || D == 4) ? it + 1 : 0, bytes32());
         total += it + 1;
         if (D == 4) {   // breakdown of the flexible CG: keep its progress, continue with Jacobi
             useM = false;
@@ -1549,6 +1553,19 @@ int ppeWarmStart(Ctx* ctx) {
     return MPS_OK;
 }
 
+// Time of the last pressure solve from its events (waits for them).
+int ppeSolveMs(Ctx* ctx, double* out) {
+    if (ctx->ppeMsPending) {
+        MPS_HIP(hipEventSynchronize(ctx->ev1));
+        float ms = 0.f;
+        MPS_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->ppeMs = ms;
+        ctx->ppeMsPending = false;
+    }
+    *out = ctx->ppeMs;
+    return MPS_OK;
+}
+
 int ppeSolve(Ctx* ctx) {
     const bool bicg = ctx->P.solver_type == MPS_SOLVER_BICGSTAB;
     const bool mixed = !bicg && ctx->P.precision_mode == MPS_PREC_MIXED;
@@ -1568,9 +1585,14 @@ int ppeSolve(Ctx* ctx) {
     const int nb = gridFor(N, kB);
     const int batch = 8;
     hipStream_t s = ctx->stream;
+    int rc0;
     if (ctx->profile && ctx->evPool.size() < 2 * kMaxBatch) {
         ctx->evPool.resize(2 * kMaxBatch);
         for (auto& e : ctx->evPool) MPS_HIP(hipEventCreate(&e));
+    }
+    if (ctx->ppeMsPending) {   // keep the previous solve's time before its events are reused
+        double prev;
+        if ((rc0 = ppeSolveMs(ctx, &prev))) return rc0;
     }
     MPS_HIP(hipEventRecord(ctx->ev0, s));
     ctx->fallbackUsed = 0;
@@ -1589,11 +1611,12 @@ int ppeSolve(Ctx* ctx) {
     }
     hipLaunchKernelGGL(kPressFromX, dim3(nb), dim3(kB), 0, s, a, c.x);
     MPS_HIP(hipEventRecord(ctx->ev1, s));
-    if ((rc = readStatus(ctx, nullptr, 0))) return rc;
+    // the mixed path's last status read already holds the final true
+    // residual; the other paths read the one computed above.  The solve's
+    // time is read from its events when asked for (ppeSolveMs).
+    if (!mixed && (rc = readStatus(ctx, nullptr, 0))) return rc;
     MPS_HIP(hipGetLastError());
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-    ctx->ppeMs = ms;
+    ctx->ppeMsPending = true;
     const double* hs = ctx->hostScalars;
     ctx->trueResidual = hs[S_BN2] > 0 ? sqrt(hs[S_TRUE_RN2] / hs[S_BN2]) : 0.0;
     if (!std::isfinite(hs[S_BN2]) || !std::isfinite(hs[S_TRUE_RN2])) ctx->trueResidual = NAN;
