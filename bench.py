@@ -21,7 +21,8 @@ iterations as it develops.
 
 The JSON line also carries the HBM roofline of the dominant kernel (the PPE
 SpMV: algorithmic bytes per launch / mean launch time from HIP events on the
-library's stream), the PPE comparison (ms/solve to 1e-8 of plain FP64
+library's stream, around every fourth SpMV launch of the timed steps), the
+PPE comparison (ms/solve to 1e-8 of plain FP64
 Jacobi-PCG, the mixed solve and the learned+FP32 solve on the same in-step
 systems, with the pressure difference to FP64) and the CPU oracle timed on
 this host on one step of the same workload.
@@ -310,13 +311,14 @@ def main(argv=None):
                   "buckets": sim.params.num_buckets_xyz},
         "ppe": {"last_solve_ms": sim.scalar("ppe_ms"), "last_iterations": sim.scalar("solver_iter"),
                 "last_outer_iterations": sim.scalar("outer_iterations"),
-                "true_residual": sim.scalar("true_residual"), "spmv_launches": launches},
+                "true_residual": sim.scalar("true_residual"), "spmv_launches_timed": launches},
         "roofline": {"kernel": "kCgSpmv" if args.precision == "fp64" else "kMxSpmv", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (SPMV_TRAFFIC_RATIO[args.precision] * spmv_bytes
                                  if args.workload == "dam3d" and args.l0 == 0.0025 else None),
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE ratio measured at l0=0.0025",
-                     "bytes_per_launch": spmv_bytes, "mean_launch_ms": spmv_ms},
+                     "bytes_per_launch": spmv_bytes, "mean_launch_ms": spmv_ms,
+                     "timing": f"HIP events around every fourth SpMV launch of the timed steps ({int(launches)} launches)"},
         "init_s": t_init,
     }
     if args.ppe_solves > 0:

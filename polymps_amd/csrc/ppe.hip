@@ -1108,24 +1108,30 @@ int ppeWindowBlocks(Ctx* ctx, double* out) {
     return MPS_OK;
 }
 
-// Launch one SpMV kernel, bracketed by HIP events when profiling.
+// Launch one SpMV kernel; when profiling, every kEventEvery-th launch of a
+// batch is bracketed by HIP events (a sample: each marker pair adds a few
+// microseconds to the stream, and the mean launch time needs no more).
+constexpr int kEventEvery = 4;
 template <typename K>
 static int timedSpmv(Ctx* ctx, K kern, const KArgs& a, const CgPtrs& c, int nb, int slot) {
     hipStream_t s = ctx->stream;
-    if (ctx->profile) MPS_HIP(hipEventRecord(ctx->evPool[2 * slot], s));
+    const bool ev = ctx->profile && slot % kEventEvery == 0;
+    if (ev) MPS_HIP(hipEventRecord(ctx->evPool[2 * slot], s));
     hipLaunchKernelGGL(kern, dim3(nb), dim3(kB), 0, s, a, c);
-    if (ctx->profile) MPS_HIP(hipEventRecord(ctx->evPool[2 * slot + 1], s));
+    if (ev) MPS_HIP(hipEventRecord(ctx->evPool[2 * slot + 1], s));
     return MPS_OK;
 }
 
+// the timed launches among the first `real` of a solve pass (t < 0: not timed)
 static void addSpmvStats(Ctx* ctx, const std::vector<float>& t, int real, double bytes) {
     if (!ctx->profile) return;
     real = std::min(real, (int)t.size());
     for (int q = 0; q < real; q++) {
+        if (t[q] < 0.f) continue;
         ctx->spmvMsTotal += t[q];
         ctx->spmvBytesTotal += bytes;
+        ctx->spmvLaunches++;
     }
-    ctx->spmvLaunches += real;
 }
 
 // Iterations queued before the first convergence read: the last solve's
@@ -1141,8 +1147,8 @@ static int readStatus(Ctx* ctx, std::vector<float>* times, int batch) {
     MPS_HIP(hipStreamSynchronize(s));
     if (ctx->profile && times)
         for (int q = 0; q < batch; q++) {
-            float ms = 0.f;
-            hipEventElapsedTime(&ms, ctx->evPool[2 * q], ctx->evPool[2 * q + 1]);
+            float ms = -1.f;
+            if (q % kEventEvery == 0) hipEventElapsedTime(&ms, ctx->evPool[2 * q], ctx->evPool[2 * q + 1]);
             times->push_back(ms);
         }
     return MPS_OK;
