@@ -487,10 +487,10 @@ int mps_create(mps_ctx** out, const mps_params* p, int n, int device) {
     MPS_HIP(hipMalloc(&ctx->nbrCntS, ni));
     MPS_HIP(hipMemsetAsync(ctx->nbrCntS, 0, ni, ctx->stream));
     MPS_HIP(hipMalloc(&ctx->P4, sizeof(double4) * (size_t)n));
-    MPS_HIP(hipMalloc(&ctx->aVal, sizeof(double) * (size_t)ctx->K * n));
-    // K (a multiple of 32) entries per row, and the codes one 16-entry batch
-    // more: the row kernels read one batch of codes past a row's end
-    MPS_HIP(hipMalloc(&ctx->aCol, sizeof(unsigned short) * (size_t)(ctx->K + kPpeCodePad) * n));
+    // ELL arrays: ellWidth(K) entries per row (K, a multiple of 32, and one
+    // 16-entry batch the row kernels read past a row's end), 64-row tiles
+    MPS_HIP(hipMalloc(&ctx->aVal, sizeof(double) * (size_t)ellWidth(ctx->K) * ellRows(n)));
+    MPS_HIP(hipMalloc(&ctx->aCol, sizeof(unsigned short) * (size_t)ellWidth(ctx->K) * ellRows(n)));
     MPS_HIP(hipMalloc(&ctx->aEsc, sizeof(int) * (size_t)kPpeEsc * n));
     MPS_HIP(hipMalloc(&ctx->aCnt, ni));
     MPS_HIP(hipMemsetAsync(ctx->aCnt, 0, ni, ctx->stream));   // nnz reads 0 before the first assembly
@@ -1330,23 +1330,24 @@ int mps_get_buckets(mps_ctx* ctx, int* first, int* last, int* next) {
 int mps_ppe_matrix(mps_ctx* ctx, int* row_ptr, int* cols, double* vals, int* nnz_out) {
     CHECK_CTX(ctx);
     hipSetDevice(ctx->dev);
-    const int n = ctx->N, K = ctx->K;
+    const int n = ctx->N, K = ctx->K, KE = ellWidth(K);
+    const size_t ne = (size_t)KE * ellRows(n);   // entries of the ELL arrays
     std::vector<int> cnt(n), canon(n), esc((size_t)kPpeEsc * n), col((size_t)K * n);
-    std::vector<unsigned short> code((size_t)K * n);
-    std::vector<double> val((size_t)K * n), diag(n);
+    std::vector<unsigned short> code(ne);
+    std::vector<double> val(ne), diag(n);
     MPS_HIP(hipMemcpy(cnt.data(), ctx->aCnt, sizeof(int) * n, hipMemcpyDeviceToHost));
     MPS_HIP(hipMemcpy(canon.data(), ctx->I[ICANON], sizeof(int) * n, hipMemcpyDeviceToHost));
-    MPS_HIP(hipMemcpy(code.data(), ctx->aCol, sizeof(unsigned short) * (size_t)K * n, hipMemcpyDeviceToHost));
+    MPS_HIP(hipMemcpy(code.data(), ctx->aCol, sizeof(unsigned short) * ne, hipMemcpyDeviceToHost));
     MPS_HIP(hipMemcpy(esc.data(), ctx->aEsc, sizeof(int) * (size_t)kPpeEsc * n, hipMemcpyDeviceToHost));
     for (int k = 0; k < n; k++) {   // decode the column codes
         int j = 0, e = 0;
         for (int c = 0; c < cnt[k]; c++) {
-            const unsigned d = code[ellIdx4(c, k, n)];
+            const unsigned d = code[ellIdx4(c, k, KE)];
             j = d ? j + (int)d : esc[(size_t)(e++) * n + k];
             col[(size_t)c * n + k] = j;
         }
     }
-    MPS_HIP(hipMemcpy(val.data(), ctx->aVal, sizeof(double) * (size_t)K * n, hipMemcpyDeviceToHost));
+    MPS_HIP(hipMemcpy(val.data(), ctx->aVal, sizeof(double) * ne, hipMemcpyDeviceToHost));
     MPS_HIP(hipMemcpy(diag.data(), ctx->aDiag, sizeof(double) * n, hipMemcpyDeviceToHost));
     const int nl = ctx->nLiveHost;
     std::vector<int> slot(n, -1);
@@ -1362,7 +1363,7 @@ int mps_ppe_matrix(mps_ctx* ctx, int* row_ptr, int* cols, double* vals, int* nnz
         if (row_ptr) row_ptr[r] = p;
         for (int c = 0; c < cnt[k]; c++) {
             if (cols) cols[p] = canon[col[(size_t)c * n + k]];
-            if (vals) vals[p] = val[ellIdx64(c, k, n)];
+            if (vals) vals[p] = val[ellIdx64(c, k, KE)];
             p++;
         }
         if (cols) cols[p] = r;

@@ -647,12 +647,13 @@ __global__ void kGnnResid(KArgs a, const double* val, const unsigned short* col,
     double yy = 0.0, vv = 0.0;
     if (i < a.N) {
         const size_t N = a.N;
+        const int ellW = ellWidth(a.K);
         double s = diag[i] * (double)z[i];
         int j = 0, e = 0;
         for (int c = 0; c < cnt[i]; c++) {
-            const unsigned d = col[ellIdx4(c, i, N)];
+            const unsigned d = col[ellIdx4(c, i, ellW)];
             j = d ? j + (int)d : esc[(size_t)(e++) * N + i];
-            s += val[ellIdx64(c, i, N)] * (double)z[j];
+            s += val[ellIdx64(c, i, ellW)] * (double)z[j];
         }
         const double yi = s - (double)v[i];
         y[i] = yi;
@@ -673,12 +674,13 @@ __global__ void kGnnLossGrad(KArgs a, const double* val, const unsigned short* c
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.N) return;
     const size_t N = a.N;
+    const int ellW = ellWidth(a.K);
     double s = diag[i] * y[i];
     int j = 0, e = 0;
     for (int c = 0; c < cnt[i]; c++) {
-        const unsigned d = col[ellIdx4(c, i, N)];
+        const unsigned d = col[ellIdx4(c, i, ellW)];
         j = d ? j + (int)d : esc[(size_t)(e++) * N + i];
-        s += val[ellIdx64(c, i, N)] * y[j];
+        s += val[ellIdx64(c, i, ellW)] * y[j];
     }
     gz[i] = (float)(2.0 * s / acc[1]);
 }
@@ -695,12 +697,13 @@ __global__ void kGnnResidualVec(KArgs a, const double* val, const unsigned short
     if (i >= a.N) return;
     if (!live(a, i)) { v[i] = 0.f; return; }
     const size_t N = a.N;
+    const int ellW = ellWidth(a.K);
     double s = diag[i] * x[i];
     int j = 0, e = 0;
     for (int c = 0; c < cnt[i]; c++) {
-        const unsigned d = col[ellIdx4(c, i, N)];
+        const unsigned d = col[ellIdx4(c, i, ellW)];
         j = d ? j + (int)d : esc[(size_t)(e++) * N + i];
-        s += val[ellIdx64(c, i, N)] * x[j];
+        s += val[ellIdx64(c, i, ellW)] * x[j];
     }
     v[i] = (float)(b[i] - s);
 }

@@ -11,18 +11,29 @@
 
 namespace mpsgpu {
 
-// FP64 value of PPE matrix (ELL) entry c of row i, K x N entries: entries 2m
-// and 2m + 1 of a row are adjacent, so one 16-byte load per lane fetches two
-// entries of its row and a wavefront of 64 rows reads one contiguous 1 KiB
-// block per entry pair.
-__host__ __device__ __forceinline__ size_t ellIdx64(int c, int i, size_t N) {
-    return (((size_t)(c >> 1) * N + (size_t)i) << 1) | (size_t)(c & 1);
+// PPE matrix (ELL) layout: KE entries per row (ellWidth: the row capacity K
+// plus one 16-entry batch the row kernels read past a row's end), rows in
+// tiles of 64 (a wavefront's rows).  Within a tile, element g of row i (a
+// pair or a group of four entries of the row) is at (tile * G + g) * 64 +
+// i % 64, G elements per row: one 16- or 8-byte load per lane reads a
+// contiguous 1 KiB or 512 B of the tile, and a wavefront's whole matrix is
+// one contiguous stretch (a column-major [G][N] layout streams ~8 % slower:
+// a wavefront's elements are N apart).  Arrays hold ellRows(N) rows.
+__host__ __device__ __forceinline__ int ellWidth(int K) { return K + kPpeCodePad; }
+__host__ __device__ __forceinline__ size_t ellRows(size_t N) { return (N + 63) / 64 * 64; }
+__host__ __device__ __forceinline__ size_t ellElem(int g, int i, int G) {
+    return ((size_t)(i >> 6) * (size_t)G + (size_t)g) * 64 + (size_t)(i & 63);
 }
-// FP32 value and 16-bit column code of entry c of row i: groups of four
+// FP64 value of entry c of row i: entries 2m and 2m + 1 of a row are adjacent
+// (one 16-byte load per lane per pair).
+__host__ __device__ __forceinline__ size_t ellIdx64(int c, int i, int KE) {
+    return (ellElem(c >> 1, i, KE >> 1) << 1) | (size_t)(c & 1);
+}
+// FP32 value and 16-bit column codes of entry c of row i: groups of four
 // entries of a row are adjacent (one 16-byte / 8-byte load per lane per four
 // entries; the assembly stores whole groups).
-__host__ __device__ __forceinline__ size_t ellIdx4(int c, int i, size_t N) {
-    return (((size_t)(c >> 2) * N + (size_t)i) << 2) | (size_t)(c & 3);
+__host__ __device__ __forceinline__ size_t ellIdx4(int c, int i, int KE) {
+    return (ellElem(c >> 2, i, KE >> 2) << 2) | (size_t)(c & 3);
 }
 
 #define KEPS (10.0 * 2.220446049250313e-16)

@@ -60,7 +60,7 @@ constexpr int kWallTable = 50;
 // escaped (non-delta) columns per PPE row: the first entry and stencil-row
 // jumps of 65536 slots or more, at most 1 + 8
 constexpr int kPpeEsc = 12;
-constexpr int kPpeCodePad = 16;   // code entries per row past K (one read-ahead batch)
+constexpr int kPpeCodePad = 16;   // ELL entries per row past K (one batch the row kernels read ahead)
 constexpr int kRedBlocks = 1024;   // partial-sum slots of a reduction
 
 // Bounding-volume hierarchy over the wall triangles: a leaf holds the
@@ -195,13 +195,13 @@ struct Ctx {
     std::vector<int> rigidF;
     MeshSlot extraMesh[2];                // meshType 1 (deformable) and 2 (forced)
     // PPE
-    double* aVal = nullptr;     // [K / 2][N][2] (ellIdx64)
-    unsigned short* aCol = nullptr;   // [K / 4][N][4] (ellIdx4) column codes: slot delta from the previous entry, 0 = escape
+    double* aVal = nullptr;     // ellIdx64 (64-row tiles of ellWidth(K) entries per row)
+    unsigned short* aCol = nullptr;   // ellIdx4: column codes, slot delta from the previous entry, 0 = escape
     int* aEsc = nullptr;              // [kPpeEsc][N] absolute slots of the escaped entries
     int* aCnt = nullptr;        // N
     double* aDiag = nullptr;    // N
-    float* aVal32 = nullptr;    // [K / 4][N][4] (ellIdx4, mixed precision)
-    unsigned short* aLcol = nullptr;  // [K / 4][N][4] (ellIdx4) entry's index in its block's LDS window (mixed)
+    float* aVal32 = nullptr;    // ellIdx4 (mixed precision)
+    unsigned short* aLcol = nullptr;  // ellIdx4: entry's index in its block's LDS window
     int* aWin = nullptr;              // per 256-row block: the LDS window table of the mixed SpMV
     bool winValid = false;            // the last assembly wrote windows
     bool ppeWindow = true;            // scalar ppe_window: build and use the windows

@@ -159,6 +159,7 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
     const double* __restrict__ RIW2 = a.D[DRIW2];
     double sum = 0.0, rhs = 0.0;
     int c = 0, prev = -1, ne = 0;
+    const int KE = ellWidth(a.K);
     bool full = false;
     typedef double D2 __attribute__((ext_vector_type(2)));
     double pend = 0.0;   // FP64 value of the last even entry, stored with the next one
@@ -245,7 +246,7 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
                     D2 w;
                     w.x = pend;
                     w.y = mv;
-                    reinterpret_cast<D2*>(val)[(size_t)(c >> 1) * a.N + i] = w;
+                    reinterpret_cast<D2*>(val)[ellElem(c >> 1, i, KE >> 1)] = w;
                 } else {
                     pend = mv;
                 }
@@ -272,7 +273,7 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
                     default: g32.w = f; gcd.w = cd; glc.w = lc; break;
                 }
                 if ((c & 3) == 3) {
-                    const size_t gi = (size_t)(c >> 2) * a.N + i;
+                    const size_t gi = ellElem(c >> 2, i, KE >> 2);
                     if (val32) reinterpret_cast<F4*>(val32)[gi] = g32;
                     reinterpret_cast<U4*>(col)[gi] = gcd;
                     if (lcol) reinterpret_cast<U4*>(lcol)[gi] = glc;
@@ -299,9 +300,9 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
             entry(j, xj, yj, zj, tj, fj, [x, j]() { return x[j]; });
         });
     }
-    if (c & 1) val[ellIdx64(c - 1, i, a.N)] = pend;
+    if (c & 1) val[ellIdx64(c - 1, i, KE)] = pend;
     if (c & 3) {   // the last, partial group (entries past the count are never read)
-        const size_t gi = (size_t)(c >> 2) * a.N + i;
+        const size_t gi = ellElem(c >> 2, i, KE >> 2);
         if (val32) reinterpret_cast<F4*>(val32)[gi] = g32;
         reinterpret_cast<U4*>(col)[gi] = gcd;
         if (lcol) reinterpret_cast<U4*>(lcol)[gi] = glc;
@@ -445,8 +446,8 @@ __device__ __forceinline__ void finishPartials(const double* red, int nb, double
 // waits for a load issued after the one it needs.  The row's first escape is
 // loaded with the row, so a batch decodes its columns without a dependent
 // load unless some lane of the wave meets a second escaped entry in it.  The
-// last batch is padded: the arrays hold K entries per row (K a multiple of
-// 16) and the codes one batch more, so the loads past a row stay inside
+// last batch is padded: the arrays hold ellWidth(K) entries per row (K, a
+// multiple of 16, and one batch more), so the loads past a row stay inside
 // them, and the entries past the count get value 0 and the previous column.
 // All lanes of a wave are at the same batch, so a batch's stream addresses
 // are a wave-uniform base plus the lane's 32-bit offset, and a gather is the
@@ -455,11 +456,12 @@ __device__ __forceinline__ void finishPartials(const double* red, int nb, double
 // gathered x from the caches.
 typedef unsigned short EllU4 __attribute__((ext_vector_type(4)));
 
-// element g * N + i of an array of 16- or 8-byte elements
+// element g of row i of an ELL array of 16- or 8-byte elements, G per row
+// (ellElem): a wave-uniform tile base plus the lane's 32-bit offset
 template <class E>
-__device__ __forceinline__ E ellStream(const void* base, int g, size_t N, int i) {
-    const char* b = reinterpret_cast<const char*>(base) + (size_t)g * N * sizeof(E);
-    return __builtin_nontemporal_load(reinterpret_cast<const E*>(b + (unsigned)i * (unsigned)sizeof(E)));
+__device__ __forceinline__ E ellStream(const void* base, int g, int G, int i) {
+    const char* b = reinterpret_cast<const char*>(base) + (((size_t)(i >> 6) * (size_t)G + (size_t)g) << 6) * sizeof(E);
+    return __builtin_nontemporal_load(reinterpret_cast<const E*>(b + (unsigned)(i & 63) * (unsigned)sizeof(E)));
 }
 template <class T>
 __device__ __forceinline__ T ellGather(const T* __restrict__ x, int j) {
@@ -468,14 +470,14 @@ __device__ __forceinline__ T ellGather(const T* __restrict__ x, int j) {
 
 // values of entries c0 .. c0 + B - 1 (c0 a multiple of B)
 template <class T, int B>
-__device__ __forceinline__ void ellLoadValues(const T* __restrict__ val, size_t N, int i, int c0, T* v) {
+__device__ __forceinline__ void ellLoadValues(const T* __restrict__ val, int KE, int i, int c0, T* v) {
     typedef double D2 __attribute__((ext_vector_type(2)));
     typedef float F4 __attribute__((ext_vector_type(4)));
     if (sizeof(T) == 8) {
         // pair m of row i is element m * N + i of the pair view
 #pragma unroll
         for (int k = 0; k < B / 2; k++) {
-            const D2 w = ellStream<D2>(val, (c0 >> 1) + k, N, i);
+            const D2 w = ellStream<D2>(val, (c0 >> 1) + k, KE >> 1, i);
             v[2 * k] = (T)w.x;
             v[2 * k + 1] = (T)w.y;
         }
@@ -483,7 +485,7 @@ __device__ __forceinline__ void ellLoadValues(const T* __restrict__ val, size_t 
         // group g of row i is element g * N + i of the group view
 #pragma unroll
         for (int k = 0; k < B / 4; k++) {
-            const F4 w = ellStream<F4>(val, (c0 >> 2) + k, N, i);
+            const F4 w = ellStream<F4>(val, (c0 >> 2) + k, KE >> 2, i);
             v[4 * k] = (T)w.x;
             v[4 * k + 1] = (T)w.y;
             v[4 * k + 2] = (T)w.z;
@@ -493,10 +495,10 @@ __device__ __forceinline__ void ellLoadValues(const T* __restrict__ val, size_t 
 }
 
 template <int B>
-__device__ __forceinline__ void ellLoadCodes(const unsigned short* __restrict__ code, size_t N, int i, int c0,
+__device__ __forceinline__ void ellLoadCodes(const unsigned short* __restrict__ code, int KE, int i, int c0,
                                              EllU4* d) {
 #pragma unroll
-    for (int k = 0; k < B / 4; k++) d[k] = ellStream<EllU4>(code, (c0 >> 2) + k, N, i);
+    for (int k = 0; k < B / 4; k++) d[k] = ellStream<EllU4>(code, (c0 >> 2) + k, KE >> 2, i);
 }
 
 // columns jj of a batch with rem valid entries (past them: the previous
@@ -563,8 +565,8 @@ __device__ __forceinline__ void ellAccum(const T* v, const T* xv, int rem, doubl
 
 template <class T>
 __device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsigned short* __restrict__ code,
-                                         const int* __restrict__ esc, int n, size_t N, const T* __restrict__ x, int i,
-                                         double acc) {
+                                         const int* __restrict__ esc, int n, size_t N, int KE,
+                                         const T* __restrict__ x, int i, double acc) {
     constexpr int B = 8;
     if (n <= 0) return acc;
     const int* es = esc + i;
@@ -573,19 +575,19 @@ __device__ __forceinline__ double ellRow(const T* __restrict__ val, const unsign
     T v[B];
     EllU4 d[B / 4];
     int jj[B];
-    ellLoadValues<T, B>(val, N, i, 0, v);
-    ellLoadCodes<B>(code, N, i, 0, d);
+    ellLoadValues<T, B>(val, KE, i, 0, v);
+    ellLoadCodes<B>(code, KE, i, 0, d);
     ellDecode<B>(d, n, es, N, es0, j, e, jj);
     for (int c = 0;;) {
         T xv[B];
 #pragma unroll
         for (int k = 0; k < B; k++) xv[k] = ellGather(x, jj[k]);
-        ellLoadCodes<B>(code, N, i, c + B, d);
+        ellLoadCodes<B>(code, KE, i, c + B, d);
         ellAccum<T, B>(v, xv, n - c, acc);
         ellDecode<B>(d, n - c - B, es, N, es0, j, e, jj);
         c += B;
         if (c >= n) break;
-        ellLoadValues<T, B>(val, N, i, c, v);
+        ellLoadValues<T, B>(val, KE, i, c, v);
     }
     return acc;
 }
@@ -622,24 +624,24 @@ struct CgPtrs {
 // give the same bits.
 template <class T>
 __device__ __forceinline__ double ellRowWin(const T* __restrict__ val, const unsigned short* __restrict__ lcol,
-                                            int n, size_t N, const T* win, int i, double acc) {
+                                            int n, int KE, const T* win, int i, double acc) {
     typedef float F4 __attribute__((ext_vector_type(4)));
     typedef double D2 __attribute__((ext_vector_type(2)));
     for (int c = 0; c < n; c += 8) {
         T v[8];
         if (sizeof(T) == 4) {
-            const F4 v0 = ellStream<F4>(val, c >> 2, N, i), v1 = ellStream<F4>(val, (c >> 2) + 1, N, i);
+            const F4 v0 = ellStream<F4>(val, c >> 2, KE >> 2, i), v1 = ellStream<F4>(val, (c >> 2) + 1, KE >> 2, i);
             v[0] = (T)v0.x; v[1] = (T)v0.y; v[2] = (T)v0.z; v[3] = (T)v0.w;
             v[4] = (T)v1.x; v[5] = (T)v1.y; v[6] = (T)v1.z; v[7] = (T)v1.w;
         } else {
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const D2 w = ellStream<D2>(val, (c >> 1) + q, N, i);
+                const D2 w = ellStream<D2>(val, (c >> 1) + q, KE >> 1, i);
                 v[2 * q] = (T)w.x;
                 v[2 * q + 1] = (T)w.y;
             }
         }
-        const EllU4 k0 = ellStream<EllU4>(lcol, c >> 2, N, i), k1 = ellStream<EllU4>(lcol, (c >> 2) + 1, N, i);
+        const EllU4 k0 = ellStream<EllU4>(lcol, c >> 2, KE >> 2, i), k1 = ellStream<EllU4>(lcol, (c >> 2) + 1, KE >> 2, i);
         const unsigned k[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
         const int rem = n - c;
         if (sizeof(T) == 4) {
@@ -672,7 +674,7 @@ __device__ __forceinline__ bool stageWindow(const int* table, const T* __restric
 }
 
 __device__ __forceinline__ double spmv64(const KArgs& a, const CgPtrs& c, const double* x, int i) {
-    return ellRow<double>(c.val, c.col, c.esc, c.cnt[i], a.N, x, i, c.diag[i] * x[i]);
+    return ellRow<double>(c.val, c.col, c.esc, c.cnt[i], a.N, ellWidth(a.K), x, i, c.diag[i] * x[i]);
 }
 
 // ---------------------------------------------------------- FP64 PCG
@@ -746,7 +748,7 @@ __global__ __launch_bounds__(256) void kCgSpmv(KArgs a, CgPtrs c) {
     const int i = slotId();
     double s0 = 0;
     if (i < a.N) {
-        const double q = lds ? ellRowWin<double>(c.val, c.lcol, c.cnt[i], a.N, win, i, c.diag[i] * c.p[i])
+        const double q = lds ? ellRowWin<double>(c.val, c.lcol, c.cnt[i], ellWidth(a.K), win, i, c.diag[i] * c.p[i])
                              : spmv64(a, c, c.p, i);
         c.q[i] = q;
         s0 = c.p[i] * q;
@@ -817,7 +819,7 @@ __global__ __launch_bounds__(256) void kTrueResidual(KArgs a, CgPtrs c) {
     double s0 = 0, s1 = 0;
     if (i < a.N) {
         if (!live(a, i)) c.x[i] = 0.0;
-        const double r = c.b[i] - (lds ? ellRowWin<double>(c.val, c.lcol, c.cnt[i], a.N, win, i, c.diag[i] * c.x[i])
+        const double r = c.b[i] - (lds ? ellRowWin<double>(c.val, c.lcol, c.cnt[i], ellWidth(a.K), win, i, c.diag[i] * c.x[i])
                                        : spmv64(a, c, c.x, i));
         c.r[i] = r;
         s0 = r * r;
@@ -876,8 +878,8 @@ __global__ __launch_bounds__(256) void kMxSpmv(KArgs a, CgPtrs c) {
         const float p = c.fp[i];
         const double d = (double)(c.diag32[i] * p);
         const int n = c.cnt[i];
-        const float q = lds ? (float)ellRowWin<float>(c.val32, c.lcol, n, a.N, win, i, d)
-                            : (float)ellRow<float>(c.val32, c.col, c.esc, n, a.N, c.fp, i, d);
+        const float q = lds ? (float)ellRowWin<float>(c.val32, c.lcol, n, ellWidth(a.K), win, i, d)
+                            : (float)ellRow<float>(c.val32, c.col, c.esc, n, a.N, ellWidth(a.K), c.fp, i, d);
         c.fq[i] = q;
         s0 = (double)p * (double)q;
     }
@@ -1011,7 +1013,7 @@ static CgPtrs cgPtrs(Ctx* ctx) {
 static int ensureMixed(Ctx* ctx) {
     if (ctx->aVal32) return MPS_OK;
     const size_t n = ctx->N;
-    MPS_HIP(hipMalloc(&ctx->aVal32, sizeof(float) * (size_t)ctx->K * n));
+    MPS_HIP(hipMalloc(&ctx->aVal32, sizeof(float) * (size_t)ellWidth(ctx->K) * ellRows(n)));
     MPS_HIP(hipMalloc(&ctx->aDiagInv32, sizeof(float) * n));
     MPS_HIP(hipMalloc(&ctx->aDiag32, sizeof(float) * n));
     MPS_HIP(hipMalloc(&ctx->fR, sizeof(float) * n));
@@ -1024,7 +1026,7 @@ static int ensureMixed(Ctx* ctx) {
 
 static int ensureWindows(Ctx* ctx) {
     if (ctx->aLcol) return MPS_OK;
-    MPS_HIP(hipMalloc(&ctx->aLcol, sizeof(unsigned short) * (size_t)ctx->K * ctx->N));
+    MPS_HIP(hipMalloc(&ctx->aLcol, sizeof(unsigned short) * (size_t)ellWidth(ctx->K) * ellRows(ctx->N)));
     MPS_HIP(hipMalloc(&ctx->aWin, sizeof(int) * (size_t)kWinStride * gridFor(ctx->N, kB)));
     return MPS_OK;
 }
