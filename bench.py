@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes / algorithmic bytes of the SpMV per launch on this workload, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs calibrated on a 1 GiB copy
 # (FETCH_SIZE x 2, WRITE_SIZE x 1): profiles/spmv_traffic_dam3d_inc_1728k.md
-SPMV_TRAFFIC_RATIO = {"fp64": 1.108, "mixed": 1.102}
+SPMV_TRAFFIC_RATIO = {"fp64": 1.108, "mixed": 1.101}
 
 
 def host_cpu():
