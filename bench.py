@@ -233,9 +233,9 @@ def main(argv=None):
     ap.add_argument("--gnn", type=Path, default=ROOT / "polymps_amd" / "data" / "gnn_dam3d.bin",
                     help="learned-operator weights for the learned+FP32 PPE comparison")
     ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--inner-tol", type=float, default=1e-5,
-                    help="relative tolerance of the inner FP32 solve of the mixed PPE (with the extrapolated "
-                         "initial guess one inner solve to 1e-5 usually reaches the FP64 target)")
+    ap.add_argument("--inner-tol", type=float, default=1e-7,
+                    help="relative tolerance of the inner FP32 solve of the mixed PPE (at 1e-7 each inner "
+                         "solve runs to half the FP64 target, which one pass reaches in most steps)")
     ap.add_argument("--warm-start", type=int, choices=[0, 1, 2, 3], default=2,
                     help="initial guess of each PPE solve: 0 previous solution, 1 linear, 2 quadratic extrapolation")
     ap.add_argument("--no-cpu-baseline", action="store_true")

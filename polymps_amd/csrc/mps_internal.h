@@ -221,7 +221,7 @@ struct Ctx {
     float* fP = nullptr;
     float* fZ = nullptr;
     float* fQ = nullptr;
-    float* fX = nullptr;
+    double* fX = nullptr;   // the inner solve's correction, summed in FP64
     // scalars
     double* ds = nullptr;
     int* is = nullptr;

@@ -613,7 +613,10 @@ struct CgPtrs {
     float* fp;
     float* fz;
     float* fq;
-    float* fd;
+    // the correction d = sum of alpha p, summed in FP64: stored in FP32 its
+    // rounding alone leaves a residual of order 3e-8 |A| |d|, above 1e-8 |b|
+    // for the larger corrections of a developed flow
+    double* fd;
     // window indices and the per-block window table (null: no windows)
     const unsigned short* lcol;
     const int* win;
@@ -846,7 +849,7 @@ __global__ void kMxInit(KArgs a, CgPtrs c) {
         const float r = (float)c.r[i];
         const float z = c.invd32[i] * r;
         c.fr[i] = r;
-        c.fd[i] = 0.0f;
+        c.fd[i] = 0.0;
         c.fz[i] = z;
         c.fp[i] = z;
         s0 = (double)r * (double)z;
@@ -891,7 +894,7 @@ __global__ void kMxUpdate(KArgs a, CgPtrs c) {
     const float alpha = (float)a.ds[S_ALPHA];
     double s0 = 0, s1 = 0;
     if (i < a.N) {
-        c.fd[i] += alpha * c.fp[i];
+        c.fd[i] += (double)alpha * (double)c.fp[i];
         const float r = c.fr[i] - alpha * c.fq[i];
         c.fr[i] = r;
         const float z = c.invd32[i] * r;
@@ -910,10 +913,10 @@ __global__ void kZero(KArgs a, double* x) {
     const int i = slotId();
     if (i < a.N) x[i] = 0.0;
 }
-// x += d (FP64 accumulation of the correction)
+// x += d
 __global__ void kMxApply(KArgs a, CgPtrs c) {
     const int i = slotId();
-    if (i < a.N && live(a, i)) c.x[i] += (double)c.fd[i];
+    if (i < a.N && live(a, i)) c.x[i] += c.fd[i];
 }
 
 // ------------------------------------- flexible CG with the learned operator
@@ -923,7 +926,7 @@ __global__ void kFcgInit(KArgs a, CgPtrs c) {
     const int i = slotId();
     if (i < a.N) {
         c.fr[i] = (float)c.r[i];
-        c.fd[i] = 0.0f;
+        c.fd[i] = 0.0;
     }
 }
 // after z = M r: p = z; partials r.z, |r|^2 (same finish as kMxInit)
@@ -943,7 +946,7 @@ __global__ void kFcgUpdate(KArgs a, CgPtrs c) {
     const int i = slotId();
     const float alpha = (float)a.ds[S_ALPHA];
     if (i < a.N) {
-        c.fd[i] += alpha * c.fp[i];
+        c.fd[i] += (double)alpha * (double)c.fp[i];
         c.fr[i] = c.fr[i] - alpha * c.fq[i];
     }
 }
@@ -1020,7 +1023,7 @@ static int ensureMixed(Ctx* ctx) {
     MPS_HIP(hipMalloc(&ctx->fP, sizeof(float) * n));
     MPS_HIP(hipMalloc(&ctx->fZ, sizeof(float) * n));
     MPS_HIP(hipMalloc(&ctx->fQ, sizeof(float) * n));
-    MPS_HIP(hipMalloc(&ctx->fX, sizeof(float) * n));
+    MPS_HIP(hipMalloc(&ctx->fX, sizeof(double) * n));
     return MPS_OK;
 }
 
@@ -1432,6 +1435,12 @@ static int solveFp64(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int batc
 // FP32 Jacobi-PCG (FP64 dot products).  Stops when |b - A x| < tol |b| in FP64.
 static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int batch) {
     constexpr int kMaxOuter = 30;
+    // A refinement pass with the learned preconditioner ends after this many
+    // iterations: an operator that neither converges nor breaks down (a
+    // poorly trained or random one can drift for thousands of iterations)
+    // then faces the stall test below instead of using up the whole
+    // iteration budget.  A working operator needs tens of iterations per pass.
+    constexpr int kLearnedPassMax = 500;
     hipStream_t s = ctx->stream;
     bool useM = ctx->P.learned_mode == 2 && ctx->gnn != nullptr;
     // The flexible CG's iterate is not the A-norm minimiser the Jacobi PCG's
@@ -1497,7 +1506,7 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
                     hipLaunchKernelGGL(kFcgUpdate, dim3(nb), dim3(kB), 0, s, a, c);
                     gnnApply32(ctx, c.fr, c.fz);
                     hipLaunchKernelGGL(kFcgDots, dim3(nb), dim3(kB), 0, s, a, c);
-                    hipLaunchKernelGGL(kFcgBeta, dim3(1), dim3(1024), 0, s, a, nb, maxIt);
+                    hipLaunchKernelGGL(kFcgBeta, dim3(1), dim3(1024), 0, s, a, nb, std::min(maxIt, kLearnedPassMax));
                 } else {
                     hipLaunchKernelGGL(kMxUpdate, dim3(nb), dim3(kB), 0, s, a, c);
                     hipLaunchKernelGGL(kCgBeta, dim3(1), dim3(1024), 0, s, a, nb, maxIt);
