@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes / algorithmic bytes of the SpMV per launch on this workload, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs calibrated on a 1 GiB copy
 # (FETCH_SIZE x 2, WRITE_SIZE x 1): profiles/spmv_traffic_dam3d_inc_1728k.md
-SPMV_TRAFFIC_RATIO = {"fp64": 1.108, "mixed": 1.101}
+SPMV_TRAFFIC_RATIO = {"fp64": 1.122, "mixed": 1.124}
 
 
 def host_cpu():
@@ -307,6 +307,8 @@ def main(argv=None):
                    "ppe_warm_start": args.warm_start, "mixed_inner_tol": w.params.mixed_inner_tol,
                    "dt": w.params.time_step},
         "sizes": {"particles": n, "nnz": sim.scalar("nnz"), "nnz_escaped": sim.scalar("nnz_escaped"),
+                  "window_escapes": sim.scalar("window_escapes"), "window_blocks": sim.scalar("ppe_window_blocks"),
+                  "blocks": (n + 255) // 256,
                   "list_entries": sim.scalar("list_entries"), "list_entries_small": sim.scalar("list_entries_small"),
                   "buckets": sim.params.num_buckets_xyz},
         "ppe": {"last_solve_ms": sim.scalar("ppe_ms"), "last_iterations": sim.scalar("solver_iter"),

@@ -124,7 +124,9 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
  * "ppe_window" (1 default: the CG solves' SpMVs read each 256-row block's
  * vector window from LDS; 0: from global memory; same results,
  * used from the next assembly; read-only "ppe_window_blocks": blocks of the
- * last assembly served from a window) "ppe_warm_start" (initial guess of each
+ * last assembly served from a window, "window_escapes": window index
+ * escapes of the last assembly, the rows' absolute indices next to their
+ * one-byte steps) "ppe_warm_start" (initial guess of each
  * pressure solve: 0 previous solution, 1 / 2 / 3 linear / quadratic / cubic
  * extrapolation of the last solutions) "mesh_bvh" (closest-point search: -1 BVH for
  * meshes above 64 faces, 0 brute force, 1 BVH; both give the same result). */

@@ -544,7 +544,7 @@ int mps_destroy(mps_ctx* ctx) {
         if (ctx->DaltNN[q]) hipFree(ctx->DaltNN[q]);
     void* bufs[] = {ctx->cellStart, ctx->cellCount, ctx->key, ctx->rank, ctx->order, ctx->order2, ctx->slotOf,
                     ctx->bvh, ctx->bvhTri, ctx->scanTmp, ctx->outList, ctx->nbrMask, ctx->nbr, ctx->nbrCnt, ctx->P4, ctx->mV, ctx->mF, ctx->mN, ctx->wtab,
-                    ctx->aVal, ctx->aCol, ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->aVal32, ctx->aLcol, ctx->aWin,
+                    ctx->aVal, ctx->aCol, ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->aVal32, ctx->aLc8, ctx->aLcEsc, ctx->aWin,
                     ctx->aDiagInv32,
                     ctx->aDiag32, ctx->cgR, ctx->cgP,
                     ctx->cgZ, ctx->cgQ, ctx->cgInvD, ctx->cgX, ctx->fR, ctx->fP, ctx->fZ, ctx->fQ, ctx->fX,
@@ -793,6 +793,7 @@ int mps_get_scalar(mps_ctx* ctx, const char* name, double* out) {
         if (rc2) return rc2;
     }
     else if (s == "nnz_escaped") *out = ctx->hostInts[I_NESC];
+    else if (s == "window_escapes") *out = ctx->hostInts[I_NLCESC];
     else if (s == "list_entries" || s == "list_entries_small") {
         // entries of the last neighbour list (large radius / small radius), summed over the slots
         const int* cnt = s == "list_entries" ? ctx->nbrCnt : ctx->args().nbrCntS;

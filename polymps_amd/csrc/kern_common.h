@@ -403,6 +403,10 @@ __device__ __forceinline__ double waveMax(double v) {
     return v;
 }
 
+__device__ __forceinline__ int waveSumInt(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
 __device__ __forceinline__ int waveMinInt(int v) {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
     return v;

@@ -54,12 +54,16 @@ enum DScalar { S_VELMAX, S_ABSNEW, S_ABSOLD, S_RHSN2, S_THR, S_RN2, S_ALPHA, S_B
                NDS };
 // device scalar slots (int)
 enum IScalar { I_NLIVE, I_OUTCNT, I_OVERFLOW, I_CG_ITER, I_CG_DONE, I_COMPACT_OVF, I_NESC,
-               I_BCG_RESTART, I_BCG_RESTARTS, I_REBUILD, I_PATCHCNT, I_FIXCNT, NIS };
+               I_BCG_RESTART, I_BCG_RESTARTS, I_REBUILD, I_PATCHCNT, I_FIXCNT, I_NLCESC, NIS };
 
 constexpr int kWallTable = 50;
 // escaped (non-delta) columns per PPE row: the first entry and stencil-row
 // jumps of 65536 slots or more, at most 1 + 8
 constexpr int kPpeEsc = 12;
+// Window index escapes per PPE row (one per stencil row the row's entries
+// enter, usually nine, and per step of 256 or more within one); a block with
+// a row that needs more gathers from global memory
+constexpr int kLcEsc = 16;
 constexpr int kPpeCodePad = 16;   // ELL entries per row past K (one batch the row kernels read ahead)
 constexpr int kRedBlocks = 1024;   // partial-sum slots of a reduction
 
@@ -201,7 +205,8 @@ struct Ctx {
     int* aCnt = nullptr;        // N
     double* aDiag = nullptr;    // N
     float* aVal32 = nullptr;    // ellIdx4 (mixed precision)
-    unsigned short* aLcol = nullptr;  // ellIdx4: entry's index in its block's LDS window
+    unsigned char* aLc8 = nullptr;    // ELL, groups of 8: one-byte window index codes (step or 0 = escape)
+    unsigned short* aLcEsc = nullptr; // ELL, kLcEsc per row: the escapes' absolute window indices
     int* aWin = nullptr;              // per 256-row block: the LDS window table of the mixed SpMV
     bool winValid = false;            // the last assembly wrote windows
     bool ppeWindow = true;            // scalar ppe_window: build and use the windows

@@ -118,14 +118,17 @@ __device__ __forceinline__ void ppeWindowBegin(const KArgs& a, int i, int* sS, i
 // would otherwise compute from the stored row), and |r_i|^2, |b_i|^2 for the
 // block partials.  kTrueResidual sums diag x_i first; here the off-diagonal
 // products are summed first: the same FP64 quantity, rounded differently.
-// lcol (with a window): each entry's index in the block's LDS window
-// (segments wS/wE/wB, wM of them); outside = some column is not in it.
+// lc8 / lcEsc (with a window): each entry's index in the block's LDS window
+// (segments wS/wE/wB, wM of them) as a one-byte step from the previous
+// entry's, or an escape to the next of the row's absolute indices; outside =
+// some column is not in the window or the row's indices do not fit the code.
 template <bool LIST, bool TRANS, bool RES>
 __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, unsigned short* col, int* esc,
                                             int* cnt, double* diag, double* b, float* val32, float* diag32,
                                             float* invd32, double* invd64, double* x, double* r, double& rn2,
-                                            double& bn2, unsigned short* lcol, const int* wS, const int* wE,
-                                            const int* wB, int wM, bool& outside) {
+                                            double& bn2, unsigned char* lc8, unsigned short* lcEsc, const int* wS,
+                                            const int* wE, const int* wB, int wM, bool& outside, int& nEsc,
+                                            int& nLc) {
     if (!live(a, i)) {
         cnt[i] = 0; diag[i] = 1.0; b[i] = 0.0;
         if (invd32) { invd32[i] = 1.0f; diag32[i] = 1.0f; }
@@ -167,10 +170,13 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
     typedef unsigned short U4 __attribute__((ext_vector_type(4)));
     F4 g32 = {0.0f, 0.0f, 0.0f, 0.0f};   // the current group of four FP32 values / codes
     U4 gcd = {0, 0, 0, 0};
-    U4 glc = {0, 0, 0, 0};                // and window indices
+    // window index codes: eight one-byte codes per group (lc8), the escapes'
+    // absolute indices in lcEsc
+    unsigned g8lo = 0, g8hi = 0;
+    int prevLc = -1, nle = 0;
     // the window segment of the current column (columns ascend)
     int wk = 0, curS = 0, curE = 0, curB = 0;
-    if (lcol && wM > 0) {
+    if (lc8 && wM > 0) {
         curS = wS[0];
         curE = wE[0];
         curB = wB[0];
@@ -253,8 +259,7 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
                 // FP32 values and codes leave in groups of four (ellIdx4)
                 const float f = (float)mv;
                 const unsigned short cd = delta ? (unsigned short)dj : (unsigned short)0;
-                unsigned short lc = 0;
-                if (lcol) {
+                if (lc8) {
                     while (wk < wM && j >= curE) {
                         wk++;
                         if (wk < wM) {
@@ -263,20 +268,41 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
                             curB = wB[wk];
                         }
                     }
-                    if (wk < wM && j >= curS) lc = (unsigned short)(curB + (j - curS));
+                    int lc = -1;
+                    if (wk < wM && j >= curS) lc = curB + (j - curS);
                     else outside = true;
+                    // code: the step from the previous entry's index (1-255)
+                    // or 0, an escape to the next absolute index of the row
+                    // (at most kLcEsc per row, or the block gathers from
+                    // global memory)
+                    if ((c & 7) == 0) {
+                        g8lo = 0;
+                        g8hi = 0;
+                    }
+                    const int dl = lc - prevLc;
+                    unsigned code = 0;
+                    if (prevLc >= 0 && lc >= 0 && dl > 0 && dl < 256) {
+                        code = (unsigned)dl;
+                    } else {
+                        if (lc >= 0 && nle < kLcEsc) lcEsc[ellElem(nle, i, kLcEsc)] = (unsigned short)lc;
+                        else outside = true;
+                        nle++;
+                    }
+                    prevLc = lc;
+                    if (c & 4) g8hi |= code << (8 * (c & 3));
+                    else g8lo |= code << (8 * (c & 3));
+                    if ((c & 7) == 7) reinterpret_cast<uint2*>(lc8)[ellElem(c >> 3, i, KE >> 3)] = make_uint2(g8lo, g8hi);
                 }
                 switch (c & 3) {
-                    case 0: g32.x = f; gcd.x = cd; glc.x = lc; break;
-                    case 1: g32.y = f; gcd.y = cd; glc.y = lc; break;
-                    case 2: g32.z = f; gcd.z = cd; glc.z = lc; break;
-                    default: g32.w = f; gcd.w = cd; glc.w = lc; break;
+                    case 0: g32.x = f; gcd.x = cd; break;
+                    case 1: g32.y = f; gcd.y = cd; break;
+                    case 2: g32.z = f; gcd.z = cd; break;
+                    default: g32.w = f; gcd.w = cd; break;
                 }
                 if ((c & 3) == 3) {
                     const size_t gi = ellElem(c >> 2, i, KE >> 2);
                     if (val32) reinterpret_cast<F4*>(val32)[gi] = g32;
                     reinterpret_cast<U4*>(col)[gi] = gcd;
-                    if (lcol) reinterpret_cast<U4*>(lcol)[gi] = glc;
                 }
                 if (RES) ax += mv * xOf();
                 if (!delta) {
@@ -305,10 +331,11 @@ __device__ __forceinline__ void assembleRow(const KArgs& a, int i, double* val, 
         const size_t gi = ellElem(c >> 2, i, KE >> 2);
         if (val32) reinterpret_cast<F4*>(val32)[gi] = g32;
         reinterpret_cast<U4*>(col)[gi] = gcd;
-        if (lcol) reinterpret_cast<U4*>(lcol)[gi] = glc;
     }
+    if (lc8 && (c & 7)) reinterpret_cast<uint2*>(lc8)[ellElem(c >> 3, i, KE >> 3)] = make_uint2(g8lo, g8hi);
     if (full) atomicMax(&a.is[I_OVERFLOW], c + 1);   // reported as a capacity error after the step
-    if (ne) atomicAdd(&a.is[I_NESC], ne);
+    nEsc = ne;
+    nLc = lc8 ? nle : 0;
     sum -= P.alpha_compressibility * rho / (P.time_step * P.time_step);
     if (P.mps_type == MPS_IMPLICIT_PND_DIVU)
         rhs += -P.coeff_ppe_source * rho * (a.D[DPNDKI][i] - P.pnd_small_zero) +
@@ -335,7 +362,7 @@ template <bool LIST, bool TRANS, bool RES>
 __global__ __launch_bounds__(256) void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc, int* cnt,
                                                     double* diag, double* b, float* val32, float* diag32,
                                                     float* invd32, double* invd64, double* x, double* r,
-                                                    int bFirst, unsigned short* lcol, int* win);
+                                                    int bFirst, unsigned char* lc8, unsigned short* lcEsc, int* win);
 
 // ------------------------------------------------------------ reductions
 // Every vector kernel runs one thread per row; each block leaves a pair of
@@ -359,21 +386,29 @@ __device__ __forceinline__ void writePartials(const KArgs& a, double s0, double 
     if (threadIdx.x == 0) { a.red[blockIdx.x * 2] = s0; a.red[blockIdx.x * 2 + 1] = s1; }
 }
 
-// lcol / win (mixed precision): window indices and the block's window table
+// lc8 / lcEsc / win: window index codes and escapes, the block's window table
 template <bool LIST, bool TRANS, bool RES>
 __global__ __launch_bounds__(256) void kPpeAssemble(KArgs a, double* val, unsigned short* col, int* esc, int* cnt,
                                                     double* diag, double* b, float* val32, float* diag32,
                                                     float* invd32, double* invd64, double* x, double* r,
-                                                    int bFirst, unsigned short* lcol, int* win) {
+                                                    int bFirst, unsigned char* lc8, unsigned short* lcEsc, int* win) {
     __shared__ int wS[9], wE[9], wB[10], wM, wOut;
     const int i = slotId();
-    if (lcol) ppeWindowBegin(a, i, wS, wE, wB, wM, wOut);
+    if (lc8) ppeWindowBegin(a, i, wS, wE, wB, wM, wOut);
     double rn2 = 0.0, bn2 = 0.0;
     bool outside = false;
+    int nEsc = 0, nLc = 0;
     if (i < a.N)
         assembleRow<LIST, TRANS, RES>(a, i, val, col, esc, cnt, diag, b, val32, diag32, invd32, invd64, x, r, rn2,
-                                      bn2, lcol, wS, wE, wB, lcol ? wM : 0, outside);
-    if (lcol) {
+                                      bn2, lc8, lcEsc, wS, wE, wB, lc8 ? wM : 0, outside, nEsc, nLc);
+    // escaped columns and window index escapes: one atomic per wavefront
+    nEsc = waveSumInt(nEsc);
+    nLc = waveSumInt(nLc);
+    if ((threadIdx.x & 63) == 0) {
+        if (nEsc) atomicAdd(&a.is[I_NESC], nEsc);
+        if (nLc) atomicAdd(&a.is[I_NLCESC], nLc);
+    }
+    if (lc8) {
         if (__builtin_amdgcn_ballot_w64(outside) != 0 && (threadIdx.x & 63) == 0) atomicOr(&wOut, 1);
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -455,6 +490,7 @@ __device__ __forceinline__ void finishPartials(const double* red, int nb, double
 // per launch and are loaded non-temporally so that they do not displace the
 // gathered x from the caches.
 typedef unsigned short EllU4 __attribute__((ext_vector_type(4)));
+typedef unsigned EllW2 __attribute__((ext_vector_type(2)));
 
 // element g of row i of an ELL array of 16- or 8-byte elements, G per row
 // (ellElem): a wave-uniform tile base plus the lane's 32-bit offset
@@ -618,7 +654,8 @@ struct CgPtrs {
     // for the larger corrections of a developed flow
     double* fd;
     // window indices and the per-block window table (null: no windows)
-    const unsigned short* lcol;
+    const unsigned char* lc8;
+    const unsigned short* lcEsc;
     const int* win;
 };
 
@@ -626,10 +663,12 @@ struct CgPtrs {
 // eight summed in FP32, then added in FP64; FP64: entry by entry), so both
 // give the same bits.
 template <class T>
-__device__ __forceinline__ double ellRowWin(const T* __restrict__ val, const unsigned short* __restrict__ lcol,
-                                            int n, int KE, const T* win, int i, double acc) {
+__device__ __forceinline__ double ellRowWin(const T* __restrict__ val, const unsigned char* __restrict__ lc8,
+                                            const unsigned short* __restrict__ lcEsc, int n, int KE, const T* win,
+                                            int i, double acc) {
     typedef float F4 __attribute__((ext_vector_type(4)));
     typedef double D2 __attribute__((ext_vector_type(2)));
+    int lc = 0, k = 0;   // running window index, escapes taken
     for (int c = 0; c < n; c += 8) {
         T v[8];
         if (sizeof(T) == 4) {
@@ -644,18 +683,36 @@ __device__ __forceinline__ double ellRowWin(const T* __restrict__ val, const uns
                 v[2 * q + 1] = (T)w.y;
             }
         }
-        const EllU4 k0 = ellStream<EllU4>(lcol, c >> 2, KE >> 2, i), k1 = ellStream<EllU4>(lcol, (c >> 2) + 1, KE >> 2, i);
-        const unsigned k[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+        // one byte per entry; the lane's next two escape indices are loaded
+        // with the group (a third escape in one group loads its own)
+        const EllW2 cw = ellStream<EllW2>(lc8, c >> 3, KE >> 3, i);
+        const unsigned e0 = lcEsc[ellElem(min(k, kLcEsc - 1), i, kLcEsc)];
+        const unsigned e1 = lcEsc[ellElem(min(k + 1, kLcEsc - 1), i, kLcEsc)];
         const int rem = n - c;
-        if (sizeof(T) == 4) {
-            T b8 = v[0] * win[k[0]];
+        const int k0 = k;
+        unsigned kk[8];
 #pragma unroll
-            for (int q = 1; q < 8; q++) b8 += (q < rem) ? v[q] * win[(q < rem) ? k[q] : 0u] : (T)0;
+        for (int q = 0; q < 8; q++) {
+            const unsigned b = ((q < 4 ? cw.x : cw.y) >> (8 * (q & 3))) & 0xffu;
+            if (q < rem) {
+                if (b != 0u) {
+                    lc += (int)b;
+                } else {
+                    lc = (int)(k == k0 ? e0 : (k == k0 + 1 ? e1 : lcEsc[ellElem(min(k, kLcEsc - 1), i, kLcEsc)]));
+                    k++;
+                }
+            }
+            kk[q] = (unsigned)lc;
+        }
+        if (sizeof(T) == 4) {
+            T b8 = v[0] * win[kk[0]];
+#pragma unroll
+            for (int q = 1; q < 8; q++) b8 += (q < rem) ? v[q] * win[kk[q]] : (T)0;
             acc += (double)b8;
         } else {
 #pragma unroll
             for (int q = 0; q < 8; q++)
-                if (q < rem) acc += (double)(v[q] * win[k[q]]);
+                if (q < rem) acc += (double)(v[q] * win[kk[q]]);
         }
     }
     return acc;
@@ -751,7 +808,7 @@ __global__ __launch_bounds__(256) void kCgSpmv(KArgs a, CgPtrs c) {
     const int i = slotId();
     double s0 = 0;
     if (i < a.N) {
-        const double q = lds ? ellRowWin<double>(c.val, c.lcol, c.cnt[i], ellWidth(a.K), win, i, c.diag[i] * c.p[i])
+        const double q = lds ? ellRowWin<double>(c.val, c.lc8, c.lcEsc, c.cnt[i], ellWidth(a.K), win, i, c.diag[i] * c.p[i])
                              : spmv64(a, c, c.p, i);
         c.q[i] = q;
         s0 = c.p[i] * q;
@@ -822,7 +879,7 @@ __global__ __launch_bounds__(256) void kTrueResidual(KArgs a, CgPtrs c) {
     double s0 = 0, s1 = 0;
     if (i < a.N) {
         if (!live(a, i)) c.x[i] = 0.0;
-        const double r = c.b[i] - (lds ? ellRowWin<double>(c.val, c.lcol, c.cnt[i], ellWidth(a.K), win, i, c.diag[i] * c.x[i])
+        const double r = c.b[i] - (lds ? ellRowWin<double>(c.val, c.lc8, c.lcEsc, c.cnt[i], ellWidth(a.K), win, i, c.diag[i] * c.x[i])
                                        : spmv64(a, c, c.x, i));
         c.r[i] = r;
         s0 = r * r;
@@ -881,7 +938,7 @@ __global__ __launch_bounds__(256) void kMxSpmv(KArgs a, CgPtrs c) {
         const float p = c.fp[i];
         const double d = (double)(c.diag32[i] * p);
         const int n = c.cnt[i];
-        const float q = lds ? (float)ellRowWin<float>(c.val32, c.lcol, n, ellWidth(a.K), win, i, d)
+        const float q = lds ? (float)ellRowWin<float>(c.val32, c.lc8, c.lcEsc, n, ellWidth(a.K), win, i, d)
                             : (float)ellRow<float>(c.val32, c.col, c.esc, n, a.N, ellWidth(a.K), c.fp, i, d);
         c.fq[i] = q;
         s0 = (double)p * (double)q;
@@ -1008,7 +1065,8 @@ static CgPtrs cgPtrs(Ctx* ctx) {
     c.x = ctx->D[DPPEX]; c.r = ctx->cgR; c.p = ctx->cgP; c.z = ctx->cgZ; c.q = ctx->cgQ; c.invd = ctx->cgInvD;
     c.val32 = ctx->aVal32; c.diag32 = ctx->aDiag32; c.invd32 = ctx->aDiagInv32;
     c.fr = ctx->fR; c.fp = ctx->fP; c.fz = ctx->fZ; c.fq = ctx->fQ; c.fd = ctx->fX;
-    c.lcol = ctx->aLcol;
+    c.lc8 = ctx->aLc8;
+    c.lcEsc = ctx->aLcEsc;
     c.win = ctx->winValid ? ctx->aWin : nullptr;
     return c;
 }
@@ -1028,8 +1086,9 @@ static int ensureMixed(Ctx* ctx) {
 }
 
 static int ensureWindows(Ctx* ctx) {
-    if (ctx->aLcol) return MPS_OK;
-    MPS_HIP(hipMalloc(&ctx->aLcol, sizeof(unsigned short) * (size_t)ellWidth(ctx->K) * ellRows(ctx->N)));
+    if (ctx->aLc8) return MPS_OK;
+    MPS_HIP(hipMalloc(&ctx->aLc8, (size_t)ellWidth(ctx->K) * ellRows(ctx->N)));
+    MPS_HIP(hipMalloc(&ctx->aLcEsc, sizeof(unsigned short) * (size_t)kLcEsc * ellRows(ctx->N)));
     MPS_HIP(hipMalloc(&ctx->aWin, sizeof(int) * (size_t)kWinStride * gridFor(ctx->N, kB)));
     return MPS_OK;
 }
@@ -1060,6 +1119,7 @@ int ppeAssemble(Ctx* ctx) {
     // row counts when asked for (reduceCounts), not by one same-address
     // atomic per wavefront here
     MPS_HIP(hipMemsetAsync(ctx->is + I_NESC, 0, sizeof(int), ctx->stream));
+    MPS_HIP(hipMemsetAsync(ctx->is + I_NLCESC, 0, sizeof(int), ctx->stream));
     // the CG solve's first true residual comes out of the assembly (with the
     // learned warm start it is the residual the warm start corrects)
     const bool res = !bicg;
@@ -1072,12 +1132,13 @@ int ppeAssemble(Ctx* ctx) {
         int rc = ensureWindows(ctx);
         if (rc) return rc;
     }
-    unsigned short* lcol = wins ? ctx->aLcol : nullptr;
+    unsigned char* lc8 = wins ? ctx->aLc8 : nullptr;
+    unsigned short* lcEsc = wins ? ctx->aLcEsc : nullptr;
     int* win = wins ? ctx->aWin : nullptr;
 #define ASM(L, TT, R)                                                                                          \
     hipLaunchKernelGGL((kPpeAssemble<L, TT, R>), dim3(g), dim3(kB), 0, ctx->stream, a, ctx->aVal, ctx->aCol,   \
                        ctx->aEsc, ctx->aCnt, ctx->aDiag, ctx->D[DPPEB], v32, d32, i32, invd64, x, ctx->cgR, bFirst, \
-                       lcol, win)
+                       lc8, lcEsc, win)
     if (res) {
         if (ctx->listValid) {
             if (T) ASM(true, true, true); else ASM(true, false, true);
@@ -1144,6 +1205,16 @@ static void addSpmvStats(Ctx* ctx, const std::vector<float>& t, int real, double
 // batch; kernels queued past convergence exit at once on the device flag.
 constexpr int kMaxBatch = 64;
 static int firstBatch(int last, int batch) { return std::min(std::max(last + 1, batch), kMaxBatch); }
+
+// Algorithmic bytes of one SpMV launch: the value (vb bytes) and column code
+// of each entry, the per-row vectors (rb bytes per live row) and the column
+// escapes.  With the LDS windows a code is one byte and an escape two (the
+// window index); without, a code is two bytes and an escape four (the slot).
+static double spmvBytes(const Ctx* ctx, double vb, double rb) {
+    const double nnz = ctx->hostScalars[S_NNZ];
+    if (ctx->winValid) return (vb + 1.0) * nnz + 2.0 * ctx->hostInts[I_NLCESC] + rb * ctx->nLiveHost;
+    return (vb + 2.0) * nnz + 4.0 * ctx->hostInts[I_NESC] + rb * ctx->nLiveHost;
+}
 
 static int readStatus(Ctx* ctx, std::vector<float>* times, int batch) {
     hipStream_t s = ctx->stream;
@@ -1424,7 +1495,7 @@ static int solveFp64(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int batc
     ctx->lastDone = D;
     if (ctx->profile)   // the entry count came with the loop's status reads
         addSpmvStats(ctx, times, (D == 1 || D == 3) ? iters + 1 : 0,
-                     10.0 * ctx->hostScalars[S_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 28.0 * ctx->nLiveHost);
+                     spmvBytes(ctx, 8.0, 28.0));
     ctx->solverIter = iters;
     ctx->outerIter = 1;
     ctx->solverErr = ctx->hostScalars[S_RHSN2] > 0 ? sqrt(ctx->hostScalars[S_RN2] / ctx->hostScalars[S_RHSN2]) : 0.0;
@@ -1458,9 +1529,7 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
     // when profiling, the entry count (for the SpMV bytes) is summed on the
     // device and comes with the next status read
     if (ctx->profile && (rc = launchCountToScalar(ctx, ctx->aCnt, ctx->N, S_NNZ))) return rc;
-    auto bytes32 = [ctx]() {
-        return 6.0 * ctx->hostScalars[S_NNZ] + 4.0 * ctx->hostInts[I_NESC] + 16.0 * ctx->nLiveHost;
-    };
+    auto bytes32 = [ctx]() { return spmvBytes(ctx, 4.0, 16.0); };
     for (;;) {
         // the first residual may come from the assembly (r and its partials)
         const bool fresh = outer == 0 && ctx->r0Fresh && ctx->r0FreshMixed;

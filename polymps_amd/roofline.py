@@ -57,16 +57,19 @@ def _f(s):
 # kernel -> (bytes of one launch, what is counted)
 def model(sizes):
     n, L, LS, nnz, nesc, mask = _f(sizes)
+    nlc = sizes.get("nlc", 0)   # window index escapes
     return {
-        "kMxSpmv": (6 * nnz + 4 * nesc + 16 * n,
-                    "FP32 value 4 + code 2 per entry, 4 per escape; per row count 4, diag 4, p 4, q 4"),
-        "kCgSpmv": (10 * nnz + 4 * nesc + 28 * n,
-                    "FP64 value 8 + code 2 per entry, 4 per escape; per row count 4, diag 8, p 8, q 8"),
-        "kTrueResidual": (10 * nnz + 4 * nesc + 36 * n,
-                          "FP64 values and codes; per row count, diag, x, b in, r out"),
+        "kMxSpmv": (5 * nnz + 2 * nlc + 16 * n,
+                    "FP32 value 4 + window code 1 per entry, 2 per window escape; per row count 4, diag 4, "
+                    "p 4, q 4"),
+        "kCgSpmv": (9 * nnz + 2 * nlc + 28 * n,
+                    "FP64 value 8 + window code 1 per entry, 2 per window escape; per row count 4, diag 8, "
+                    "p 8, q 8"),
+        "kTrueResidual": (9 * nnz + 2 * nlc + 36 * n,
+                          "FP64 values and window codes; per row count, diag, x, b in, r out"),
         "kPpeAssemble<true, true, true>": (
-            14 * nnz + 4 * nesc + 4 * L + 116 * n,
-            "writes FP64 + FP32 values and codes (14 per entry), escapes; reads the large list; "
+            15 * nnz + 4 * nesc + 2 * nlc + 4 * L + 116 * n,
+            "writes FP64 + FP32 values, codes and window codes (15 per entry), escapes; reads the large list; "
             "per particle P4, x, mirror, bc, source in (80), count, diag, b, FP32 diag / 1/diag, r out (36)"),
         "kViscosity<true>": (4 * L + 140 * n,
                              "large list; P4 32, packed velocity 32, own velocity 24, mirror 24, count 4 in; "
@@ -142,7 +145,7 @@ def sizes_from_log(path):
         if line.startswith("{") and '"sizes"' in line:
             s = json.loads(line)["sizes"]
             return {"n": s["particles"], "L": s["list_entries"], "LS": s["list_entries_small"],
-                    "nnz": s["nnz"], "nesc": s["nnz_escaped"]}
+                    "nnz": s["nnz"], "nesc": s["nnz_escaped"], "nlc": s.get("window_escapes", 0)}
     raise ValueError(f"no bench JSON line with sizes in {path}")
 
 

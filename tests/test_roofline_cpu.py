@@ -14,11 +14,13 @@ def test_kernel_names():
 
 def test_spmv_bytes_match_the_library_formula():
     """The SpMV figures are the ones the library's profiling counts per
-    launch: FP32 6 nnz + 4 nesc + 16 n, FP64 10 nnz + 4 nesc + 28 n."""
-    s = {"n": 1000, "L": 120000, "LS": 30000, "nnz": 117000, "nesc": 900}
+    launch with the LDS windows (spmvBytes in ppe.hip): FP32 5 nnz + 2 nlc +
+    16 n, FP64 9 nnz + 2 nlc + 28 n (one-byte window codes, two-byte window
+    escapes)."""
+    s = {"n": 1000, "L": 120000, "LS": 30000, "nnz": 117000, "nesc": 900, "nlc": 9500}
     m = R.model(s)
-    assert m["kMxSpmv"][0] == 6 * 117000 + 4 * 900 + 16 * 1000
-    assert m["kCgSpmv"][0] == 10 * 117000 + 4 * 900 + 28 * 1000
+    assert m["kMxSpmv"][0] == 5 * 117000 + 2 * 9500 + 16 * 1000
+    assert m["kCgSpmv"][0] == 9 * 117000 + 2 * 9500 + 28 * 1000
     # every list consumer counts its list at 4 B per entry
     assert m["kViscosity<true>"][0] == 4 * 120000 + 140 * 1000
     assert m["kCorrectionMatrix<true>"][0] == 4 * 30000 + 136 * 1000
