@@ -244,6 +244,7 @@ struct Ctx {
     int ppeSolves = 0;           // pressure solves so far (history of the extrapolated warm start)
     int lastCgIters = 0;         // iterations of the last FP64 CG solve / first mixed inner pass
     int lastInnerIters = 0;
+    int lastPass2Iters = 8;          // iterations of the last solve's second refinement pass
     bool r0Fresh = false;        // cgR and asmRed hold b - A x of the current guess and its partials (assembly)
     bool r0FreshMixed = false;   // ... in the mixed solve's partial order (else the FP64 CG's, with 1/diag)
     double* asmRed = nullptr;    // block partials of the assembly's first residual (no other kernel writes them)

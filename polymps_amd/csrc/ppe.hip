@@ -1566,8 +1566,11 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
         hipLaunchKernelGGL(kMxInitFinish, dim3(1), dim3(1024), 0, s, a, nb, tolIn, tol);
         int done = 0, it = 0;
         times.clear();
-        for (int launched = 0, bq = firstBatch(outer == 1 ? ctx->lastInnerIters : 0, batch); !done;
-             launched += bq, bq = batch) {
+        // first batch: the last solve's count for this pass plus one (a
+        // second pass, which starts just above the target, is short)
+        const int bq0 = outer == 1 ? firstBatch(ctx->lastInnerIters, batch)
+                                   : (outer == 2 ? std::min(std::max(ctx->lastPass2Iters + 1, 2), batch) : batch);
+        for (int launched = 0, bq = bq0; !done; launched += bq, bq = batch) {
             for (int q = 0; q < bq; q++) {
                 timedSpmv(ctx, kMxSpmv, a, c, nb, q);
                 hipLaunchKernelGGL(kCgAlpha, dim3(1), dim3(1024), 0, s, a, nb);
@@ -1588,6 +1591,7 @@ static int solveMixed(Ctx* ctx, const KArgs& a, const CgPtrs& c, int nb, int bat
             if (launched > 4 * maxIt + 64) break;
         }
         if (outer == 1) ctx->lastInnerIters = it;
+        if (outer == 2) ctx->lastPass2Iters = it;
         const int D = ctx->hostInts[I_CG_DONE];
         addSpmvStats(ctx, times, (D == 1 || D == 3 || D == 4) ? it + 1 : 0, bytes32());
         total += it + 1;
