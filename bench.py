@@ -21,7 +21,7 @@ iterations as it develops.
 
 The JSON line also carries the HBM roofline of the dominant kernel (the PPE
 SpMV: algorithmic bytes per launch / mean launch time from HIP events on the
-library's stream, around every fourth SpMV launch of the timed steps), the
+library's stream, around every eighth SpMV launch of the timed steps), the
 PPE comparison (ms/solve to 1e-8 of plain FP64
 Jacobi-PCG, the mixed solve and the learned+FP32 solve on the same in-step
 systems, with the pressure difference to FP64) and the CPU oracle timed on
@@ -320,7 +320,7 @@ def main(argv=None):
                                  if args.workload == "dam3d" and args.l0 == 0.0025 else None),
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE ratio measured at l0=0.0025",
                      "bytes_per_launch": spmv_bytes, "mean_launch_ms": spmv_ms,
-                     "timing": f"HIP events around every fourth SpMV launch of the timed steps ({int(launches)} launches)"},
+                     "timing": f"HIP events around every eighth SpMV launch of the timed steps ({int(launches)} launches)"},
         "init_s": t_init,
     }
     if args.ppe_solves > 0:

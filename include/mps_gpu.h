@@ -98,7 +98,7 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
  * "list_max" "list_capacity" "capacity" "nnz" "list_start" (1 when the last
  * step built a list at its start positions) "spmv_ms_total"
  * "spmv_launches" "spmv_bytes_total" (with "profile" on: the time, count and
- * algorithmic bytes of the timed SpMV launches, every fourth launch of a
+ * algorithmic bytes of the timed SpMV launches, every eighth launch of a
  * batch) "list_skin" (skin of the last step's
  * kept neighbour list, in l0; 0: built after the prediction) "list_rebuilt"
  * (1 when the last step rebuilt its list after the prediction)
@@ -112,7 +112,7 @@ int mps_upload_field(mps_ctx* ctx, const char* field, const void* in, size_t cou
  * the largest one-step displacement, in [2, 10], default 3) "mask_words" (candidate
  * window per stencil row in 32-slot words, at most the allocated width;
  * particles with a longer row are listed by a direct walk, same results)
- * "profile" (nonzero: time every fourth SpMV launch with HIP events, counters reset)
+ * "profile" (nonzero: time every eighth SpMV launch with HIP events, counters reset)
  * "precision_mode" (MPS_PREC_*, used from the next PPE assembly)
  * "learned_mode" (0 off, 1 learned warm start, 2 warm start + learned
  * flexible-CG preconditioner) "learned_stall_ratio" (in (0, 1), default

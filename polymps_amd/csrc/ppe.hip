@@ -1177,7 +1177,7 @@ int ppeWindowBlocks(Ctx* ctx, double* out) {
 // Launch one SpMV kernel; when profiling, every kEventEvery-th launch of a
 // batch is bracketed by HIP events (a sample: each marker pair adds a few
 // microseconds to the stream, and the mean launch time needs no more).
-constexpr int kEventEvery = 4;
+constexpr int kEventEvery = 8;
 template <typename K>
 static int timedSpmv(Ctx* ctx, K kern, const KArgs& a, const CgPtrs& c, int nb, int slot) {
     hipStream_t s = ctx->stream;
